@@ -1,0 +1,732 @@
+// energy.cpp -- oracle restatement of EnergyFunctional, NonLocalPotential,
+// StructureFactor, ChargeDensity and SlaterDet (density, rs_mul_add, gram).
+// TEST INFRASTRUCTURE ONLY (see qbo.h).
+#include "qbo.h"
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <algorithm>
+
+namespace qbo {
+
+static const double fpi = 4.0 * M_PI;
+
+// ---------------------------------------------------------------- setup
+void System::setup() {
+  const Cell* ref = has_ref ? &refcell : nullptr;
+  wb.resize(cell, ref, ecut);
+  vb.resize(cell, ref, 4.0 * ecut);          // src/ChargeDensity.cpp:37
+  int n[3];
+  for (int i = 0; i < 3; i++) {              // src/ChargeDensity.cpp:44-49
+    n[i] = vb.np[i] + 2;
+    while (!factorizable(n[i])) n[i] += 2;
+  }
+  vft.init(vb, n[0], n[1], n[2]);
+  ft.init(wb, n[0], n[1], n[2]);             // src/ChargeDensity.cpp:98
+  for (auto& s : sp) s.initialize();
+  eself_total = 0.0;
+  for (size_t is = 0; is < sp.size(); is++)
+    eself_total += (tau[is].size() / 3) * sp[is].eself();
+  const long N = (long)n[0] * n[1] * n[2];
+  rhor.assign(N, 0.0);
+  v_r.assign(N, 0.0);
+  vxc_r.assign(N, 0.0);
+  rhog.assign(vb.size, 0.0);
+  rhogt.assign(vb.size, 0.0);
+  rhoelg.assign(vb.size, 0.0);
+  cell_moved();
+  atoms_moved();
+}
+
+// src/EnergyFunctional.cpp:1200-1245
+void System::cell_moved() {
+  const int ngv = vb.size;
+  const double omega_inv = 1.0 / cell.volume;
+  const int nsp = (int)sp.size();
+  vps.assign(nsp, std::vector<double>(ngv));
+  dvps.assign(nsp, std::vector<double>(ngv));
+  rhops.assign(nsp, std::vector<double>(ngv));
+  for (int is = 0; is < nsp; is++) {
+#pragma omp parallel for schedule(static)
+    for (int ig = 0; ig < ngv; ig++) {
+      double v, dv;
+      rhops[is][ig] = sp[is].rhopsg(vb.g[ig]) * omega_inv;
+      sp[is].dvlocg(vb.g[ig], v, dv);
+      vps[is][ig] = v * omega_inv;
+      dvps[is][ig] = dv * omega_inv;
+    }
+  }
+  update_twnl();
+}
+
+// src/NonLocalPotential.cpp:54-232 (KB, wsg = 1) and :235-520 (l = 0, 1)
+void System::update_twnl() {
+  const int ngw = wb.size;
+  const int nsp = (int)sp.size();
+  const double s14pi = std::sqrt(1.0 / fpi);
+  const double s34pi = std::sqrt(3.0 / fpi);
+  twnl.assign(nsp, {});
+  dtwnl.assign(nsp, {});
+  lproj.assign(nsp, {});
+  const double* kpg = wb.g.data();
+  const double* kpgi = wb.gi.data();
+  const double* kx = &wb.gx[0];
+  const double* ky = &wb.gx[ngw];
+  const double* kz = &wb.gx[2 * ngw];
+  for (int is = 0; is < nsp; is++) {
+    const Species& s = sp[is];
+    const int npr = s.nlm;
+    twnl[is].assign((size_t)npr * ngw, 0.0);
+    dtwnl[is].assign((size_t)npr * 6 * ngw, 0.0);
+    lproj[is].assign(npr, 0);
+    int ilm = 0;
+    for (int iop = 0; iop < s.nop; iop++) {
+      const int l = iop;
+      if (l == 0) {
+        lproj[is][ilm] = 0;
+        double* t0 = &twnl[is][(size_t)ngw * ilm];
+        double* dt = &dtwnl[is][(size_t)ngw * 6 * ilm];
+#pragma omp parallel for schedule(static)
+        for (int ig = 0; ig < ngw; ig++) {
+          double v, dv;
+          s.dvnlg(iop, kpg[ig], v, dv);
+          t0[ig] = s14pi * v;
+          const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
+          const double tmp = kpgi[ig] * s14pi * dv;
+          dt[ig] = tmp * tgx * tgx;
+          dt[ngw + ig] = tmp * tgy * tgy;
+          dt[2 * ngw + ig] = tmp * tgz * tgz;
+          dt[3 * ngw + ig] = tmp * tgx * tgy;
+          dt[4 * ngw + ig] = tmp * tgy * tgz;
+          dt[5 * ngw + ig] = tmp * tgx * tgz;
+        }
+      } else {
+        for (int m = 0; m < 3; m++) lproj[is][ilm + m] = 1;
+        double* t1 = &twnl[is][(size_t)ngw * ilm];
+        double* t2 = &twnl[is][(size_t)ngw * (ilm + 1)];
+        double* t3 = &twnl[is][(size_t)ngw * (ilm + 2)];
+        double* d1 = &dtwnl[is][(size_t)ngw * 6 * ilm];
+        double* d2 = &dtwnl[is][(size_t)ngw * 6 * (ilm + 1)];
+        double* d3 = &dtwnl[is][(size_t)ngw * 6 * (ilm + 2)];
+#pragma omp parallel for schedule(static)
+        for (int ig = 0; ig < ngw; ig++) {
+          double v, dv;
+          const double tg = kpg[ig];
+          s.dvnlg(iop, tg, v, dv);
+          const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
+          const double tgx2 = tgx * tgx, tgy2 = tgy * tgy, tgz2 = tgz * tgz;
+          const double tgi = kpgi[ig], tgi2 = tgi * tgi;
+          const double y1 = s34pi * tgx * tgi;
+          const double y2 = s34pi * tgy * tgi;
+          const double y3 = s34pi * tgz * tgi;
+          t1[ig] = y1 * v;
+          t2[ig] = y2 * v;
+          t3[ig] = y3 * v;
+          const double fac1 = -y1 * (v - tg * dv) * tgi2;
+          d1[ig] = fac1 * tgx2 + v * y1;
+          d1[ngw + ig] = fac1 * tgy2;
+          d1[2 * ngw + ig] = fac1 * tgz2;
+          d1[3 * ngw + ig] = fac1 * tgx * tgy;
+          d1[4 * ngw + ig] = fac1 * tgy * tgz;
+          d1[5 * ngw + ig] = fac1 * tgx * tgz;
+          const double fac2 = -y2 * (v - tg * dv) * tgi2;
+          d2[ig] = fac2 * tgx2;
+          d2[ngw + ig] = fac2 * tgy2 + v * y2;
+          d2[2 * ngw + ig] = fac2 * tgz2;
+          d2[3 * ngw + ig] = fac2 * tgx * tgy + v * y1;
+          d2[4 * ngw + ig] = fac2 * tgy * tgz;
+          d2[5 * ngw + ig] = fac2 * tgx * tgz;
+          const double fac3 = -y3 * (v - tg * dv) * tgi2;
+          d3[ig] = fac3 * tgx2;
+          d3[ngw + ig] = fac3 * tgy2;
+          d3[2 * ngw + ig] = fac3 * tgz2 + v * y3;
+          d3[3 * ngw + ig] = fac3 * tgx * tgy;
+          d3[4 * ngw + ig] = fac3 * tgy * tgz + v * y2;
+          d3[5 * ngw + ig] = fac3 * tgx * tgz + v * y1;
+        }
+      }
+      ilm += 2 * l + 1;
+    }
+  }
+}
+
+// src/StructureFactor.cpp:70-162 and src/EnergyFunctional.cpp:1047-1197
+void System::atoms_moved() {
+  const int ngv = vb.size;
+  const int nsp = (int)sp.size();
+  const int* idx = vb.idx.data();
+  sfac.assign(nsp, std::vector<cplx>(ngv, 0.0));
+  const int k0min = vb.idxmin[0], k1min = vb.idxmin[1], k2min = vb.idxmin[2];
+  const int r0 = vb.idxmax[0] - k0min + 1, r1 = vb.idxmax[1] - k1min + 1,
+            r2 = vb.idxmax[2] - k2min + 1;
+  for (int is = 0; is < nsp; is++) {
+    const int na = (int)tau[is].size() / 3;
+    std::vector<double> c0(r0), c1(r1), c2(r2), s0(r0), s1(r1), s2(r2);
+    for (int ia = 0; ia < na; ia++) {
+      const double* t = &tau[is][3 * ia];
+      const double fac0 = cell.b[0][0] * t[0] + cell.b[0][1] * t[1] + cell.b[0][2] * t[2];
+      const double fac1 = cell.b[1][0] * t[0] + cell.b[1][1] * t[1] + cell.b[1][2] * t[2];
+      const double fac2 = cell.b[2][0] * t[0] + cell.b[2][1] * t[1] + cell.b[2][2] * t[2];
+      for (int i = k0min; i <= vb.idxmax[0]; i++) {
+        c0[i - k0min] = std::cos(i * fac0); s0[i - k0min] = std::sin(i * fac0);
+      }
+      for (int i = k1min; i <= vb.idxmax[1]; i++) {
+        c1[i - k1min] = std::cos(i * fac1); s1[i - k1min] = std::sin(i * fac1);
+      }
+      for (int i = k2min; i <= vb.idxmax[2]; i++) {
+        c2[i - k2min] = std::cos(i * fac2); s2[i - k2min] = std::sin(i * fac2);
+      }
+      cplx* sf = sfac[is].data();
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < ngv; i++) {
+        const int kx = idx[3 * i] - k0min, ky = idx[3 * i + 1] - k1min,
+                  kz = idx[3 * i + 2] - k2min;
+        const double ca = c0[kx], cb = c1[ky], cc = c2[kz];
+        const double sa = s0[kx], sb = s1[ky], sc = s2[kz];
+        sf[i] += cplx(ca * cb * cc - sa * sb * cc - sa * cb * sc - ca * sb * sc,
+                      sa * sb * sc - sa * cb * cc - ca * sb * cc - ca * cb * sc);
+      }
+    }
+  }
+  vion_local_g.assign(ngv, 0.0);
+  dvion_local_g.assign(ngv, 0.0);
+  rhopst.assign(ngv, 0.0);
+  for (int is = 0; is < nsp; is++)
+    for (int ig = 0; ig < ngv; ig++) {
+      const cplx sg = sfac[is][ig];
+      rhopst[ig] += sg * rhops[is][ig];
+      vion_local_g[ig] += sg * vps[is][ig];
+      dvion_local_g[ig] += sg * dvps[is][ig];
+    }
+
+  // esr, fion_esr, sigma_esr: :1097-1192
+  const double omega_inv = 1.0 / cell.volume;
+  auto len = [](const double* x) { return std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]); };
+  const double d0 = 2.0 * M_PI / len(cell.b[0]);
+  const double d1 = 2.0 * M_PI / len(cell.b[1]);
+  const double d2 = 2.0 * M_PI / len(cell.b[2]);
+  int nat = 0;
+  std::vector<int> off(nsp);
+  for (int is = 0; is < nsp; is++) { off[is] = nat; nat += (int)tau[is].size() / 3; }
+  fion_esr.assign(3 * nat, 0.0);
+  esr = 0.0;
+  for (int i = 0; i < 6; i++) sigma_esr[i] = 0.0;
+  for (int is1 = 0; is1 < nsp; is1++)
+    for (int is2 = is1; is2 < nsp; is2++) {
+      const double rc1 = sp[is1].rcps, rc2 = sp[is2].rcps;
+      const double rcps12 = std::sqrt(rc1 * rc1 + rc2 * rc2);
+      const int nc0 = (int)(8.0 * rcps12 / d0);
+      const int nc1 = (int)(8.0 * rcps12 / d1);
+      const int nc2 = (int)(8.0 * rcps12 / d2);
+      const double z1 = sp[is1].zval, z2 = sp[is2].zval;
+      const int na1 = (int)tau[is1].size() / 3, na2 = (int)tau[is2].size() / 3;
+      for (int ia1 = 0; ia1 < na1; ia1++) {
+        const int ia2min = (is1 == is2) ? ia1 : 0;
+        for (int ia2 = ia2min; ia2 < na2; ia2++) {
+          const bool same = (is1 == is2 && ia1 == ia2);
+          double v12[3];
+          for (int j = 0; j < 3; j++) v12[j] = tau[is1][3 * ia1 + j] - tau[is2][3 * ia2 + j];
+          cell.fold_in_ws(v12);
+          double* f1 = &fion_esr[3 * (off[is1] + ia1)];
+          double* f2 = &fion_esr[3 * (off[is2] + ia2)];
+          for (int ic0 = -nc0; ic0 <= nc0; ic0++)
+            for (int ic1 = -nc1; ic1 <= nc1; ic1++)
+              for (int ic2 = -nc2; ic2 <= nc2; ic2++) {
+                if (same && ic0 == 0 && ic1 == 0 && ic2 == 0) continue;
+                const double fac = same ? 0.5 : 1.0;
+                double s[3];
+                for (int j = 0; j < 3; j++)
+                  s[j] = (ic0 * cell.a[0][j] + ic1 * cell.a[1][j]) + ic2 * cell.a[2][j];
+                const double x12 = v12[0] + s[0], y12 = v12[1] + s[1], zz12 = v12[2] + s[2];
+                const double r12 = std::sqrt(x12 * x12 + y12 * y12 + zz12 * zz12);
+                const double arg = r12 / rcps12;
+                const double esr_term = z1 * z2 * std::erfc(arg) / r12;
+                esr += fac * esr_term;
+                const double desr_erfc = 2.0 * z1 * z2 * std::exp(-arg * arg) / (rcps12 * std::sqrt(M_PI));
+                const double desrdr = -fac * (esr_term + desr_erfc) / (r12 * r12);
+                f1[0] -= desrdr * x12; f2[0] += desrdr * x12;
+                f1[1] -= desrdr * y12; f2[1] += desrdr * y12;
+                f1[2] -= desrdr * zz12; f2[2] += desrdr * zz12;
+                sigma_esr[0] += desrdr * x12 * x12;
+                sigma_esr[1] += desrdr * y12 * y12;
+                sigma_esr[2] += desrdr * zz12 * zz12;
+                sigma_esr[3] += desrdr * x12 * y12;
+                sigma_esr[4] += desrdr * y12 * zz12;
+                sigma_esr[5] += desrdr * x12 * zz12;
+              }
+        }
+      }
+    }
+  for (int i = 0; i < 6; i++) sigma_esr[i] *= -omega_inv;
+}
+
+// ------------------------------------------------------------ SlaterDet
+// src/SlaterDet.cpp:225-298 (real basis)
+void compute_density(const FFT3& ft, const cplx* c, int ngw, int nst,
+                     const double* occ, double weight, double omega, double* rho) {
+  const long N = ft.np012();
+  std::vector<cplx> tmp(N);
+  const double omega_inv = 1.0 / omega;
+  for (int n = 0; n < nst - 1; n += 2) {
+    const double fac1 = weight * omega_inv * occ[n];
+    const double fac2 = weight * omega_inv * occ[n + 1];
+    if (fac1 + fac2 > 0.0) {
+      ft.backward(c + (long)n * ngw, c + (long)(n + 1) * ngw, tmp.data());
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; i++) {
+        const double p1 = tmp[i].real(), p2 = tmp[i].imag();
+        rho[i] += fac1 * p1 * p1 + fac2 * p2 * p2;
+      }
+    }
+  }
+  if (nst % 2 != 0) {
+    const int n = nst - 1;
+    const double fac1 = weight * omega_inv * occ[n];
+    if (fac1 > 0.0) {
+      ft.backward(c + (long)n * ngw, tmp.data());
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; i++) {
+        const double p1 = tmp[i].real();
+        rho[i] += fac1 * p1 * p1;
+      }
+    }
+  }
+}
+
+// src/SlaterDet.cpp:410-477 (real basis)
+void rs_mul_add(const FFT3& ft, const double* v, const cplx* c, int ngw,
+                int nst, cplx* hpsi) {
+  const long N = ft.np012();
+  std::vector<cplx> tmp(N), c1(ngw), c2(ngw);
+  for (int n = 0; n < nst - 1; n += 2) {
+    ft.backward(c + (long)n * ngw, c + (long)(n + 1) * ngw, tmp.data());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; i++) tmp[i] *= v[i];
+    ft.forward(tmp.data(), c1.data(), c2.data());
+    cplx* h1 = hpsi + (long)n * ngw;
+    cplx* h2 = hpsi + (long)(n + 1) * ngw;
+    for (int ig = 0; ig < ngw; ig++) { h1[ig] += c1[ig]; h2[ig] += c2[ig]; }
+  }
+  if (nst % 2 != 0) {
+    const int n = nst - 1;
+    ft.backward(c + (long)n * ngw, tmp.data());
+    for (long i = 0; i < N; i++) tmp[i] *= v[i];
+    ft.forward(tmp.data(), c1.data());
+    cplx* h1 = hpsi + (long)n * ngw;
+    for (int ig = 0; ig < ngw; ig++) h1[ig] += c1[ig];
+  }
+}
+
+// src/SlaterDet.cpp:1335-1356
+void cleanup(cplx* c, int ngw, int nst) {
+  for (int n = 0; n < nst; n++) c[(long)n * ngw] = cplx(c[(long)n * ngw].real(), 0.0);
+}
+
+// src/SlaterDet.cpp:480-534 (real basis): S = 2 P^T P - p0 p0^T,
+// S = L L^T, C <- C L^-T
+int gram(cplx* c, int ngw, int nst) {
+  cleanup(c, ngw, nst);
+  const long m = 2L * ngw;
+  const double* p = reinterpret_cast<const double*>(c);
+  std::vector<double> S((size_t)nst * nst, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < nst; i++)
+    for (int j = 0; j <= i; j++) {
+      const double* a = p + (long)i * m;
+      const double* b = p + (long)j * m;
+      double s = 0.0;
+      for (long k = 0; k < m; k++) s += a[k] * b[k];
+      S[(size_t)i * nst + j] = 2.0 * s - a[0] * b[0];
+    }
+  // Cholesky, lower, row-major S[i*nst+j] for j <= i
+  for (int j = 0; j < nst; j++) {
+    double d = S[(size_t)j * nst + j];
+    for (int k = 0; k < j; k++) d -= S[(size_t)j * nst + k] * S[(size_t)j * nst + k];
+    if (!(d > 0.0)) return j + 1;
+    d = std::sqrt(d);
+    S[(size_t)j * nst + j] = d;
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i < nst; i++) {
+      double s = S[(size_t)i * nst + j];
+      for (int k = 0; k < j; k++) s -= S[(size_t)i * nst + k] * S[(size_t)j * nst + k];
+      S[(size_t)i * nst + j] = s / d;
+    }
+  }
+  // X L^T = C, row by row of the real view: x_j = (c_j - sum_k<j L_jk x_k)/L_jj
+  double* q = reinterpret_cast<double*>(c);
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < m; r++) {
+    for (int j = 0; j < nst; j++) {
+      double s = q[r + (long)j * m];
+      for (int k = 0; k < j; k++) s -= S[(size_t)j * nst + k] * q[r + (long)k * m];
+      q[r + (long)j * m] = s / S[(size_t)j * nst + j];
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------- ChargeDensity
+// src/ChargeDensity.cpp:136-198
+void System::update_density(const cplx* c) {
+  std::fill(rhor.begin(), rhor.end(), 0.0);
+  compute_density(ft, c, wb.size, nst, occ.data(), 1.0, cell.volume, rhor.data());
+  const long N = ft.np012();
+  const double omega = cell.volume;
+  std::vector<cplx> rhotmp(N);
+  double sum = 0.0;
+  for (long i = 0; i < N; i++) {
+    sum += rhor[i];
+    rhotmp[i] = cplx(omega * rhor[i], 0.0);
+  }
+  total_charge = sum * omega / (double)N;
+  vft.forward(rhotmp.data(), rhog.data());
+}
+
+// src/EnergyFunctional.cpp:231-394 and XCPotential.cpp:124-230, 467-516
+void System::update_vhxc(bool compute_stress) {
+  const int ngv = vb.size;
+  const double omega = cell.volume, omega_inv = 1.0 / omega;
+  const long N = vft.np012();
+  for (int ig = 0; ig < ngv; ig++) rhoelg[ig] = omega_inv * rhog[ig];
+  // LDA
+  double esum = 0.0, dsum = 0.0;
+  for (long i = 0; i < N; i++) {
+    double e, v;
+    lda_unpolarized(rhor[i], e, v);
+    esum += rhor[i] * e;
+    dsum += rhor[i] * (e - v);
+    vxc_r[i] = v;
+  }
+  exc = esum * omega / (double)N;
+  dxc = dsum * omega / (double)N;
+  for (int i = 0; i < 6; i++) sigma_exc[i] = 0.0;
+  if (compute_stress) {
+    const double t = -dsum / (double)N;
+    sigma_exc[0] = sigma_exc[1] = sigma_exc[2] = t;
+  }
+  // eps
+  double s0 = 0.0;
+  for (int ig = 0; ig < ngv; ig++)
+    s0 += rhoelg[ig].real() * vion_local_g[ig].real() +
+          rhoelg[ig].imag() * vion_local_g[ig].imag();
+  s0 = 2.0 * s0 - std::real(std::conj(rhoelg[0]) * vion_local_g[0]);
+  eps = s0 * omega;
+  double ehsum = 0, ehesum = 0, ehepsum = 0, ehpsum = 0;
+  const double* g2i = vb.g2i.data();
+  for (int ig = 0; ig < ngv; ig++) {
+    const cplx r = rhoelg[ig], rp = rhopst[ig];
+    ehsum += std::norm(r + rp) * g2i[ig];
+    ehesum += std::norm(r) * g2i[ig];
+    ehepsum += 2.0 * std::real(std::conj(r) * rp * g2i[ig]);
+    ehpsum += std::norm(rp) * g2i[ig];
+    rhogt[ig] = r + rp;
+  }
+  ehart = omega * fpi * ehsum;
+  ehart_e = omega * fpi * ehesum;
+  ehart_ep = omega * fpi * ehepsum;
+  ehart_p = omega * fpi * ehpsum;
+  std::vector<cplx> vlocal_g(ngv), tmp_r(N);
+  for (int ig = 0; ig < ngv; ig++) vlocal_g[ig] = vion_local_g[ig] + fpi * rhogt[ig] * g2i[ig];
+  vft.backward(vlocal_g.data(), tmp_r.data());
+  for (long i = 0; i < N; i++) v_r[i] = vxc_r[i] + tmp_r[i].real();
+}
+
+// ---------------------------------------------------- blocked GEMM helpers
+// C[i + j*ldc] (+)= alpha * sum_k A[k + i*lda] * B[k + j*ldb]   (TN, K-contiguous)
+static void gemm_tn(int M, int N, long K, double alpha, const double* A, long lda,
+                    const double* B, long ldb, double beta, double* C, int ldc) {
+  std::vector<double> acc((size_t)M * N, 0.0);
+  const long KB = 4096;
+  for (long k0 = 0; k0 < K; k0 += KB) {
+    const long k1 = std::min(K, k0 + KB);
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int i = 0; i < M; i++)
+      for (int j = 0; j < N; j++) {
+        const double* a = A + i * lda;
+        const double* b = B + j * ldb;
+        double s = 0.0;
+        for (long k = k0; k < k1; k++) s += a[k] * b[k];
+        acc[(size_t)i + (size_t)j * M] += s;
+      }
+  }
+  for (int j = 0; j < N; j++)
+    for (int i = 0; i < M; i++)
+      C[i + (long)j * ldc] = alpha * acc[(size_t)i + (size_t)j * M] +
+                             (beta == 0.0 ? 0.0 : beta * C[i + (long)j * ldc]);
+}
+
+// C[k + j*ldc] += sum_i A[k + i*lda] * B[i + j*ldb]   (NN, C += A B)
+static void gemm_nn_acc(long K, int N, int M, const double* A, long lda,
+                        const double* B, int ldb, double* C, long ldc) {
+  const long KB = 1024;
+  const long nkb = (K + KB - 1) / KB;
+#pragma omp parallel for schedule(static)
+  for (long kb = 0; kb < nkb; kb++) {
+    const long k0 = kb * KB, k1 = std::min(K, k0 + KB);
+    for (int j = 0; j < N; j++) {
+      double* c = C + j * ldc;
+      for (int i = 0; i < M; i++) {
+        const double f = B[i + (long)j * ldb];
+        const double* a = A + i * lda;
+        for (long k = k0; k < k1; k++) c[k] += a[k] * f;
+      }
+    }
+  }
+}
+
+// -------------------------------------------------- NonLocalPotential
+// src/NonLocalPotential.cpp:1258-2069 restated for the real basis, KB
+// projectors with l <= 1.  The reference processes atoms in blocks of 32
+// (:1274); the block size only reorders independent columns and is not
+// restated.  Stress terms (:1786-2037) are restated in nl_stress below.
+double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                         bool compute_forces, std::vector<double>& fion_enl,
+                         bool compute_stress, double* sigma_enl) {
+  const int ngw = wb.size;
+  const long K = 2L * ngw;
+  const double omega_inv = 1.0 / cell.volume;
+  const int nsp = (int)sp.size();
+  double enl = 0.0;
+  int nat = 0;
+  for (int is = 0; is < nsp; is++) nat += (int)tau[is].size() / 3;
+  fion_enl.assign(3 * nat, 0.0);
+  for (int i = 0; i < 6; i++) sigma_enl[i] = 0.0;
+  const double* kx[3] = {&wb.gx[0], &wb.gx[ngw], &wb.gx[2 * ngw]};
+  const double* cp = reinterpret_cast<const double*>(c);
+  int atom_off = 0;
+  for (int is = 0; is < nsp; is++) {
+    const Species& s = sp[is];
+    const int na = (int)tau[is].size() / 3;
+    const int npr = s.nlm;
+    const int M = npr * na;
+    // eigr: gr = -kpgx . tau (:1314-1339)
+    std::vector<double> cgr((size_t)na * ngw), sgr((size_t)na * ngw);
+#pragma omp parallel for schedule(static)
+    for (int ia = 0; ia < na; ia++) {
+      const double* t = &tau[is][3 * ia];
+      for (int ig = 0; ig < ngw; ig++) {
+        const double arg = -(kx[0][ig] * t[0] + kx[1][ig] * t[1] + kx[2][ig] * t[2]);
+        sgr[(size_t)ia * ngw + ig] = std::sin(arg);
+        cgr[(size_t)ia * ngw + ig] = std::cos(arg);
+      }
+    }
+    // anl[2*ig + K*(ia + ipr*na)] (:1341-1395)
+    std::vector<double> anl((size_t)K * M);
+    auto build_anl = [&](int dir) {   // dir < 0: anl; dir = 0,1,2: -i G_dir anl
+#pragma omp parallel for schedule(static) collapse(2)
+      for (int ipr = 0; ipr < npr; ipr++)
+        for (int ia = 0; ia < na; ia++) {
+          const double* t = &twnl[is][(size_t)ngw * ipr];
+          const int l = lproj[is][ipr];
+          double* a = &anl[(size_t)K * (ia + ipr * na)];
+          const double* cc = &cgr[(size_t)ia * ngw];
+          const double* ss = &sgr[(size_t)ia * ngw];
+          for (int ig = 0; ig < ngw; ig++) {
+            if (dir < 0) {
+              if (l == 0) { a[2 * ig] = t[ig] * cc[ig]; a[2 * ig + 1] = t[ig] * ss[ig]; }
+              else { a[2 * ig] = t[ig] * ss[ig]; a[2 * ig + 1] = -t[ig] * cc[ig]; }
+            } else {
+              if (l == 0) {       // :1656-1665
+                const double tt = kx[dir][ig] * t[ig];
+                a[2 * ig] = tt * ss[ig]; a[2 * ig + 1] = -tt * cc[ig];
+              } else {            // :1667-1675
+                const double tt = -kx[dir][ig] * t[ig];
+                a[2 * ig] = tt * cc[ig]; a[2 * ig + 1] = tt * ss[ig];
+              }
+            }
+          }
+        }
+    };
+    build_anl(-1);
+    // fnl = 2 (anl^T c) - anl0 c0 (:1399-1464)
+    std::vector<double> fnl((size_t)M * nst), fbuf((size_t)M * nst);
+    gemm_tn(M, nst, K, 1.0, anl.data(), K, cp, K, 0.0, fnl.data(), M);
+    for (int j = 0; j < nst; j++)
+      for (int i = 0; i < M; i++)
+        fnl[i + (size_t)j * M] += -0.5 * anl[(size_t)K * i] * cp[(size_t)K * j];
+    for (auto& x : fnl) x *= 2.0;
+    // D matrix (:1469-1507): F'_I = sum_J D_IJ F_J, diagonal in atom
+    for (int j = 0; j < nst; j++)
+      for (int ipr = 0; ipr < npr; ipr++)
+        for (int ia = 0; ia < na; ia++) {
+          double sacc = 0.0;
+          for (int jpr = 0; jpr < npr; jpr++)
+            sacc += s.dmatrix(ipr, jpr) * fnl[(ia + jpr * na) + (size_t)j * M];
+          fbuf[(ia + ipr * na) + (size_t)j * M] = sacc;
+        }
+    // enl and fnl <- wt/omega * F' (:1550-1571), wt = 1
+    for (int ipr = 0; ipr < npr; ipr++) {
+      const double fac = 1.0 * omega_inv;
+      for (int n = 0; n < nst; n++) {
+        const double facn = fac * occ[n];
+        for (int ia = 0; ia < na; ia++) {
+          const size_t i = ia + ipr * na + (size_t)n * M;
+          enl += facn * fnl[i] * fbuf[i];
+          fnl[i] = fac * fbuf[i];
+        }
+      }
+    }
+    if (compute_hpsi)   // :1599-1614  cp += anl * fnl
+      gemm_nn_acc(K, nst, M, anl.data(), K, fnl.data(), M,
+                  reinterpret_cast<double*>(hpsi), K);
+    if (compute_forces) {   // :1634-1784
+      std::vector<double> dfnl((size_t)M * nst);
+      for (int j = 0; j < 3; j++) {
+        build_anl(j);
+        gemm_tn(M, nst, K, 2.0, anl.data(), K, cp, K, 0.0, dfnl.data(), M);
+        for (int ipr = 0; ipr < npr; ipr++)
+          for (int n = 0; n < nst; n++) {
+            const double facn = 2.0 * occ[n];
+            for (int ia = 0; ia < na; ia++) {
+              const size_t i = ia + ipr * na + (size_t)n * M;
+              fion_enl[3 * (atom_off + ia) + j] -= facn * fnl[i] * dfnl[i];
+            }
+          }
+      }
+    }
+    if (compute_stress)
+      throw std::runtime_error("oracle: NL stress not restated yet");
+    atom_off += na;
+  }
+  return enl;
+}
+
+// src/EnergyFunctional.cpp:397-1044 (nspin = 1, one k-point, LDA, no
+// confinement, no external potential or fields)
+double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                      bool compute_forces, bool compute_stress, Results& r) {
+  const int ngw = wb.size, ngv = vb.size;
+  const double omega = cell.volume, omega_inv = 1.0 / omega;
+  int nat = 0;
+  for (auto& t : tau) nat += (int)t.size() / 3;
+  std::vector<double> fion(3 * nat, 0.0);
+  if (compute_hpsi) std::fill(hpsi, hpsi + (size_t)ngw * nst, cplx(0.0));
+  // ekin (:441-571), fac = 1 for the real basis
+  double ts[7] = {0, 0, 0, 0, 0, 0, 0};
+  {
+    std::vector<double> psi2sum(ngw);
+#pragma omp parallel for schedule(static)
+    for (int ig = 0; ig < ngw; ig++) {
+      double t = 0.0;
+      for (int n = 0; n < nst; n++) t += 1.0 * occ[n] * std::norm(c[ig + (size_t)n * ngw]);
+      psi2sum[ig] = t;
+    }
+    for (int ig = 0; ig < ngw; ig++) {
+      const double p = psi2sum[ig];
+      ts[0] += p * wb.g2[ig];
+      if (compute_stress) {
+        const double gx = wb.gx[ig], gy = wb.gx[ngw + ig], gz = wb.gx[2 * ngw + ig];
+        const double f = 2.0 * p;
+        ts[1] += f * gx * gx; ts[2] += f * gy * gy; ts[3] += f * gz * gz;
+        ts[4] += f * gx * gy; ts[5] += f * gy * gz; ts[6] += f * gx * gz;
+      }
+    }
+  }
+  r.ekin = ts[0];
+  for (int i = 0; i < 6; i++) r.sigma_ekin[i] = ts[i + 1] * omega_inv;
+  for (int i = 0; i < 6; i++) { r.sigma_eps[i] = 0.0; r.sigma_ehart[i] = 0.0; }
+  if (compute_stress) {   // :573-683
+    double sm[6] = {0, 0, 0, 0, 0, 0};
+    for (int ig = 0; ig < ngv; ig++) {
+      const double fac = 2.0 * vb.gi[ig] * std::real(std::conj(rhoelg[ig]) * dvion_local_g[ig]);
+      const double gx = vb.gx[ig], gy = vb.gx[ngv + ig], gz = vb.gx[2 * ngv + ig];
+      sm[0] += fac * gx * gx; sm[1] += fac * gy * gy; sm[2] += fac * gz * gz;
+      sm[3] += fac * gx * gy; sm[4] += fac * gy * gz; sm[5] += fac * gx * gz;
+    }
+    for (int i = 0; i < 6; i++) r.sigma_eps[i] = sm[i] + (i < 3 ? eps * omega_inv : 0.0);
+    double sh[6] = {0, 0, 0, 0, 0, 0};
+    for (int ig = 0; ig < ngv; ig++) {
+      const double temp = std::norm(rhogt[ig]) * vb.g2i[ig] * vb.g2i[ig];
+      const double gx = vb.gx[ig], gy = vb.gx[ngv + ig], gz = vb.gx[2 * ngv + ig];
+      sh[0] += temp * gx * gx; sh[1] += temp * gy * gy; sh[2] += temp * gz * gz;
+      sh[3] += temp * gx * gy; sh[4] += temp * gy * gz; sh[5] += temp * gx * gz;
+    }
+    for (size_t is = 0; is < sp.size(); is++) {
+      const double fac2 = 2.0 * 0.25 * sp[is].rcps * sp[is].rcps;
+      for (int ig = 0; ig < ngv; ig++) {
+        const cplx sg = sfac[is][ig], rg = rhogt[ig];
+        const double temp = fac2 * (rg.real() * sg.real() + rg.imag() * sg.imag()) *
+                            rhops[is][ig] * vb.g2i[ig];
+        const double gx = vb.gx[ig], gy = vb.gx[ngv + ig], gz = vb.gx[2 * ngv + ig];
+        sh[0] += temp * gx * gx; sh[1] += temp * gy * gy; sh[2] += temp * gz * gz;
+        sh[3] += temp * gx * gy; sh[4] += temp * gy * gz; sh[5] += temp * gx * gz;
+      }
+    }
+    for (int i = 0; i < 6; i++)
+      r.sigma_ehart[i] = (i < 3 ? ehart * omega_inv : 0.0) - 2.0 * fpi * sh[i];
+  }
+  // NL (:686-742)
+  std::vector<double> fion_enl;
+  r.enl = nl_energy(c, compute_hpsi, hpsi, compute_forces, fion_enl, compute_stress, r.sigma_enl);
+  if (compute_forces)
+    for (int i = 0; i < 3 * nat; i++) fion[i] += fion_enl[i];
+  r.eps = eps; r.ehart = ehart; r.ehart_e = ehart_e; r.ehart_ep = ehart_ep;
+  r.ehart_p = ehart_p; r.exc = exc; r.dxc = dxc; r.esr = esr; r.eself = eself_total;
+  r.ecoul = ehart + esr - eself_total;
+  r.etotal = r.ekin + 0.0 + eps + r.enl + r.ecoul + exc + 0.0 + 0.0;
+  // H psi (:789-838)
+  if (compute_hpsi) {
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nst; n++)
+      for (int ig = 0; ig < ngw; ig++)
+        hpsi[ig + (size_t)n * ngw] += 0.5 * wb.g2[ig] * c[ig + (size_t)n * ngw];
+    rs_mul_add(ft, v_r.data(), c, ngw, nst, hpsi);
+  }
+  // local forces (:841-930), eigr as products of per-axis tables
+  if (compute_forces) {
+    const int* idx = vb.idx.data();
+    const int k0min = vb.idxmin[0], k1min = vb.idxmin[1], k2min = vb.idxmin[2];
+    const int r0 = vb.idxmax[0] - k0min + 1, r1 = vb.idxmax[1] - k1min + 1,
+              r2 = vb.idxmax[2] - k2min + 1;
+    std::vector<cplx> vtemp(ngv);
+    int atom_off = 0;
+    for (size_t is = 0; is < sp.size(); is++) {
+      for (int ig = 0; ig < ngv; ig++) {
+        const double tmp = fpi * rhops[is][ig] * vb.g2i[ig];
+        vtemp[ig] = tmp * std::conj(rhogt[ig]) + vps[is][ig] * std::conj(rhoelg[ig]);
+      }
+      const int na = (int)tau[is].size() / 3;
+#pragma omp parallel for schedule(dynamic)
+      for (int ia = 0; ia < na; ia++) {
+        const double* t = &tau[is][3 * ia];
+        const double fac0 = cell.b[0][0] * t[0] + cell.b[0][1] * t[1] + cell.b[0][2] * t[2];
+        const double fac1 = cell.b[1][0] * t[0] + cell.b[1][1] * t[1] + cell.b[1][2] * t[2];
+        const double fac2 = cell.b[2][0] * t[0] + cell.b[2][1] * t[1] + cell.b[2][2] * t[2];
+        std::vector<double> c0(r0), c1(r1), c2(r2), s0(r0), s1(r1), s2(r2);
+        for (int i = 0; i < r0; i++) { c0[i] = std::cos((i + k0min) * fac0); s0[i] = std::sin((i + k0min) * fac0); }
+        for (int i = 0; i < r1; i++) { c1[i] = std::cos((i + k1min) * fac1); s1[i] = std::sin((i + k1min) * fac1); }
+        for (int i = 0; i < r2; i++) { c2[i] = std::cos((i + k2min) * fac2); s2[i] = std::sin((i + k2min) * fac2); }
+        double sum0 = 0, sum1 = 0, sum2 = 0;
+        for (int ig = 0; ig < ngv; ig++) {
+          const int kx = idx[3 * ig] - k0min, ky = idx[3 * ig + 1] - k1min, kz = idx[3 * ig + 2] - k2min;
+          const double ca = c0[kx], cb = c1[ky], cc = c2[kz];
+          const double sa = s0[kx], sb = s1[ky], sc = s2[kz];
+          const double re = ca * cb * cc - sa * sb * cc - sa * cb * sc - ca * sb * sc;
+          const double im = sa * sb * sc - sa * cb * cc - ca * sb * cc - ca * cb * sc;
+          const double tmp = re * vtemp[ig].imag() + im * vtemp[ig].real();
+          sum0 += tmp * vb.gx[ig];
+          sum1 += tmp * vb.gx[ngv + ig];
+          sum2 += tmp * vb.gx[2 * ngv + ig];
+        }
+        const double fac = -2.0 * omega;
+        double* f = &fion[3 * (atom_off + ia)];
+        const double* fe = &fion_esr[3 * (atom_off + ia)];
+        f[0] += fe[0] + fac * sum0;
+        f[1] += fe[1] + fac * sum1;
+        f[2] += fe[2] + fac * sum2;
+      }
+      atom_off += na;
+    }
+  }
+  r.fion = fion;
+  for (int i = 0; i < 6; i++) {
+    r.sigma_exc[i] = sigma_exc[i];
+    r.sigma_esr[i] = sigma_esr[i];
+    r.sigma[i] = compute_stress ? (r.sigma_ekin[i] + r.sigma_eps[i] + r.sigma_enl[i] +
+                                   r.sigma_ehart[i] + r.sigma_exc[i] + r.sigma_esr[i])
+                                : 0.0;
+  }
+  return r.etotal;
+}
+
+}  // namespace qbo

@@ -1,0 +1,295 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+REFLIB = HERE / "_ref" / "libqbref.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
+                ("deltar", C.c_double), ("rcps", C.c_double),
+                ("vloc", _dp), ("proj", _dp), ("dl", _dp)]
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"oracle library not built: {LIB}")
+        _lib = C.CDLL(str(LIB))
+        _lib.qbo_create.restype = C.c_void_p
+        _lib.qbo_create.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
+                                    _ip, _dp, C.c_int, _dp]
+        _lib.qbo_last_error.restype = C.c_char_p
+        for name in ["qbo_destroy", "qbo_sizes", "qbo_basis", "qbo_rods", "qbo_fft_backward",
+                     "qbo_fft_backward_pair", "qbo_fft_forward", "qbo_fft_forward_pair",
+                     "qbo_compute_density", "qbo_rs_mul_add", "qbo_update_density",
+                     "qbo_update_vhxc", "qbo_get", "qbo_set_vr", "qbo_species_eval"]:
+            getattr(_lib, name).argtypes = None
+        _lib.qbo_gram.argtypes = [_dp, C.c_int, C.c_int]
+        _lib.qbo_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
+                                    _dp, _dp, _dp]
+    return _lib
+
+
+ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p",
+                "exc", "dxc", "esr", "eself", "ecoul", "etotal", "total_charge"]
+
+
+class Oracle:
+    """One Qbox-like system on the CPU (EnergyFunctional + ChargeDensity)."""
+
+    def __init__(self, sysdef):
+        L = lib()
+        self.sysdef = sysdef
+        self._keep = []
+        descs = (SpeciesDesc * len(sysdef.species))()
+        for i, sp in enumerate(sysdef.species):
+            vloc, proj, dl = sp.radial()
+            self._keep += [vloc, proj, dl]
+            descs[i] = SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
+                                   _ptr(vloc), _ptr(proj), _ptr(dl))
+        cell = np.ascontiguousarray(sysdef.cell, dtype=float).ravel()
+        na = np.array(sysdef.na, dtype=np.int32)
+        tau = np.ascontiguousarray(sysdef.tau, dtype=float).ravel()
+        occ = sysdef.occ.astype(float)
+        self._keep += [cell, na, tau, occ, descs]
+        h = L.qbo_create(_ptr(cell), None, sysdef.ecut, len(sysdef.species), descs,
+                         _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ))
+        if not h:
+            raise RuntimeError(L.qbo_last_error().decode())
+        self.h = C.c_void_p(h)
+        s = np.zeros(12, dtype=np.int32)
+        L.qbo_sizes(self.h, _ptr(s, _ip))
+        (self.ngw, self.ngv, self.np0, self.np1, self.np2, self.nvec, self.ntrans0,
+         self.nvec_v, self.ntrans0_v, self.nrods, self.nrods_v, self.ndft) = [int(x) for x in s]
+        self.N = self.np0 * self.np1 * self.np2
+        self.nst = sysdef.nst
+
+    def __del__(self):
+        try:
+            lib().qbo_destroy(self.h)
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- bases
+    def basis(self, which=0):
+        n = self.ngw if which == 0 else self.ngv
+        nvec = self.nvec if which == 0 else self.nvec_v
+        idx = np.zeros(3 * n, dtype=np.int32)
+        g2 = np.zeros(n)
+        gx = np.zeros(3 * n)
+        ip = np.zeros(n, dtype=np.int32)
+        im = np.zeros(n, dtype=np.int32)
+        z2v = np.zeros(nvec, dtype=np.int32)
+        npb = np.zeros(9, dtype=np.int32)
+        lib().qbo_basis(self.h, which, _ptr(idx, _ip), _ptr(g2), _ptr(gx), _ptr(ip, _ip),
+                        _ptr(im, _ip), _ptr(z2v, _ip), _ptr(npb, _ip))
+        return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip, im=im,
+                    zvec_to_val=z2v, np=npb[:3].copy(), idxmin=npb[3:6].copy(),
+                    idxmax=npb[6:9].copy())
+
+    def rods(self, which=0):
+        n = self.nrods if which == 0 else self.nrods_v
+        a = [np.zeros(n, dtype=np.int32) for _ in range(4)]
+        lib().qbo_rods(self.h, which, *[_ptr(x, _ip) for x in a])
+        return a
+
+    # ---------------------------------------------------------- transforms
+    def fft_backward(self, c, which=0):
+        f = np.zeros(self.N, dtype=np.complex128)
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        lib().qbo_fft_backward(self.h, which, _ptr(c), _ptr(f))
+        return f
+
+    def fft_backward_pair(self, c1, c2):
+        f = np.zeros(self.N, dtype=np.complex128)
+        c1 = np.ascontiguousarray(c1, dtype=np.complex128)
+        c2 = np.ascontiguousarray(c2, dtype=np.complex128)
+        lib().qbo_fft_backward_pair(self.h, _ptr(c1), _ptr(c2), _ptr(f))
+        return f
+
+    def fft_forward(self, f, which=0):
+        n = self.ngw if which == 0 else self.ngv
+        f = np.array(f, dtype=np.complex128)
+        c = np.zeros(n, dtype=np.complex128)
+        lib().qbo_fft_forward(self.h, which, _ptr(f), _ptr(c))
+        return c
+
+    def fft_forward_pair(self, f):
+        f = np.array(f, dtype=np.complex128)
+        c1 = np.zeros(self.ngw, dtype=np.complex128)
+        c2 = np.zeros(self.ngw, dtype=np.complex128)
+        lib().qbo_fft_forward_pair(self.h, _ptr(f), _ptr(c1), _ptr(c2))
+        return c1, c2
+
+    # ------------------------------------------------------------ physics
+    def compute_density(self, c):
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        rho = np.zeros(self.N)
+        lib().qbo_compute_density(self.h, _ptr(c), _ptr(rho))
+        return rho
+
+    def rs_mul_add(self, v, c, hpsi=None):
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        v = np.ascontiguousarray(v, dtype=float)
+        h = np.zeros_like(c) if hpsi is None else np.array(hpsi, dtype=np.complex128)
+        lib().qbo_rs_mul_add(self.h, _ptr(v), _ptr(c), _ptr(h))
+        return h
+
+    def update_density(self, c):
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        lib().qbo_update_density(self.h, _ptr(c))
+
+    def update_vhxc(self, stress=False):
+        lib().qbo_update_vhxc(self.h, int(stress))
+
+    def get(self, what, isp=0):
+        npr = 4
+        sizes = {0: (self.N, float), 1: (self.N, float), 2: (self.N, float),
+                 3: (self.ngv, complex), 4: (self.ngv, complex), 5: (self.ngv, complex),
+                 6: (self.ngv, float), 7: (self.ngv, float), 8: (self.ngv, float),
+                 9: (npr * self.ngw, float), 10: (self.ngv, complex),
+                 11: (3 * self.sysdef.nat, float), 12: (npr * 6 * self.ngw, float),
+                 13: (self.ngv, complex)}
+        n, t = sizes[what]
+        out = np.zeros(n, dtype=np.complex128 if t is complex else np.float64)
+        lib().qbo_get(self.h, what, isp, out.ctypes.data_as(_dp))
+        return out
+
+    def set_vr(self, v):
+        v = np.ascontiguousarray(v, dtype=float)
+        lib().qbo_set_vr(self.h, _ptr(v))
+
+    def energy(self, c, hpsi=True, forces=True, stress=False):
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        h = np.zeros_like(c) if hpsi else None
+        e = np.zeros(14)
+        f = np.zeros(3 * self.sysdef.nat)
+        sig = np.zeros(42)
+        rc = lib().qbo_energy(self.h, _ptr(c), int(hpsi), _ptr(h), int(forces), int(stress),
+                              _ptr(e), _ptr(f), _ptr(sig))
+        if rc != 0:
+            raise RuntimeError(lib().qbo_last_error().decode())
+        out = dict(zip(ENERGY_NAMES, e.tolist()))
+        out["fion"] = f.reshape(-1, 3)
+        out["sigma"] = sig.reshape(7, 6)
+        out["hpsi"] = h
+        return out
+
+    def evaluate(self, c, hpsi=True, forces=True, stress=False):
+        """density -> update_vhxc -> energy (src/BOSampleStepper.cpp:1207-1217)."""
+        self.update_density(c)
+        self.update_vhxc(stress)
+        return self.energy(c, hpsi, forces, stress)
+
+    def species_eval(self, isp, g, nop=2):
+        g = np.ascontiguousarray(g, dtype=float)
+        n = g.size
+        v, dv = np.zeros(n), np.zeros(n)
+        vn, dvn = np.zeros(nop * n), np.zeros(nop * n)
+        lib().qbo_species_eval(self.h, isp, n, _ptr(g), _ptr(v), _ptr(dv), _ptr(vn), _ptr(dvn))
+        return v, dv, vn.reshape(nop, n), dvn.reshape(nop, n)
+
+
+def gram(c):
+    """SlaterDet::gram on (nst, ngw) complex coefficients (in place copy)."""
+    c = np.array(c, dtype=np.complex128, order="C")
+    nst, ngw = c.shape
+    rc = lib().qbo_gram(_ptr(c), ngw, nst)
+    if rc != 0:
+        raise RuntimeError(f"overlap not positive definite at pivot {rc}")
+    return c
+
+
+def lda(rho):
+    rho = np.ascontiguousarray(rho, dtype=float)
+    e, v = np.zeros_like(rho), np.zeros_like(rho)
+    lib().qbo_lda(rho.size, _ptr(rho), _ptr(e), _ptr(v))
+    return e, v
+
+
+def sinft(y):
+    y = np.array(y, dtype=float)
+    lib().qbo_sinft(y.size, _ptr(y))
+    return y
+
+
+def cosft1(y):
+    """y has n+1 entries; returns the transformed first n (y[n] unchanged)."""
+    y = np.array(y, dtype=float)
+    lib().qbo_cosft1(y.size - 1, _ptr(y))
+    return y
+
+
+def spline(x, y, yp1, ypn, bl, br):
+    x = np.ascontiguousarray(x, dtype=float)
+    y = np.ascontiguousarray(y, dtype=float)
+    y2 = np.zeros_like(x)
+    lib().qbo_spline(x.size, _ptr(x), _ptr(y), C.c_double(yp1), C.c_double(ypn), bl, br, _ptr(y2))
+    return y2
+
+
+def splintd(xa, ya, y2a, x):
+    y, dy = C.c_double(), C.c_double()
+    xa = np.ascontiguousarray(xa, dtype=float)
+    ya = np.ascontiguousarray(ya, dtype=float)
+    y2a = np.ascontiguousarray(y2a, dtype=float)
+    lib().qbo_splintd(xa.size, _ptr(xa), _ptr(ya), _ptr(y2a), C.c_double(x), C.byref(y), C.byref(dy))
+    return y.value, dy.value
+
+
+def fft1d(x, sign):
+    x = np.array(x, dtype=np.complex128)
+    lib().qbo_fft1d(_ptr(x), x.size, sign)
+    return x
+
+
+def cell(a):
+    a = np.ascontiguousarray(a, dtype=float).ravel()
+    b = np.zeros(9)
+    vol = C.c_double()
+    lib().qbo_cell(_ptr(a), _ptr(b), C.byref(vol))
+    return b.reshape(3, 3), vol.value
+
+
+def fold_in_ws(a, v):
+    a = np.ascontiguousarray(a, dtype=float).ravel()
+    v = np.array(v, dtype=float)
+    lib().qbo_fold_in_ws(_ptr(a), _ptr(v))
+    return v
+
+
+# --------------------------------------------------------------- reference
+_ref = None
+
+
+def reflib():
+    """The reference's own spline/LDA/UnitCell sources compiled in _ref/."""
+    global _ref
+    if _ref is None:
+        if not REFLIB.exists():
+            return None
+        _ref = C.CDLL(str(REFLIB))
+    return _ref

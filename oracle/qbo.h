@@ -1,0 +1,167 @@
+// qbo.h -- CPU oracle for the Qbox H*psi / energy path.
+//
+// TEST INFRASTRUCTURE ONLY.  This directory holds a plain C++ (FP64, OpenMP)
+// restatement of the reference algorithms (Qbox rel1_78_5, src/*.cpp).  It is
+// the checker the HIP library is compared against and the CPU baseline timed
+// beside it.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it; the product library never links or calls it.
+//
+// Every function cites the reference file:line it restates.  Pinning of each
+// part (golden values, closed forms, invariants, the reference's own spline /
+// LDA / UnitCell sources compiled under oracle/_ref) lives in tests/.
+#pragma once
+#include <complex>
+#include <vector>
+#include <cstdint>
+
+namespace qbo {
+
+typedef std::complex<double> cplx;
+
+// ---------------------------------------------------------------- UnitCell
+// src/UnitCell.cpp:25-124 (set), :155-186 (fold_in_ws)
+struct Cell {
+  double a[3][3];      // a[i] = lattice vector i
+  double b[3][3];      // b[i] = 2 pi (A^-T) row i
+  double volume;
+  double an[13][3], an2h[13];
+  void set(const double* a9);
+  void fold_in_ws(double v[3]) const;
+};
+
+// ------------------------------------------------------------------- Basis
+// src/Basis.cpp:255-690 restated for one task and k = 0 (real basis).
+struct Basis {
+  Cell cell;
+  double ecut = 0.0;            // Hartree
+  int idxmin[3], idxmax[3], np[3];
+  int size = 0, nrods = 0;
+  std::vector<int> rod_h, rod_k, rod_lmin, rod_size, rod_first;
+  std::vector<int> idx;         // idx[3*i+j]
+  std::vector<double> g, g2, gi, g2i, gx;   // gx[j*size+i]
+  // defcell: cell that selects the vectors (refcell if given, else cell)
+  void resize(const Cell& c, const Cell* refcell, double ecut_ha);
+  void update_g();
+};
+bool factorizable(int n);       // src/Basis.cpp:105-133
+
+// ------------------------------------------------------------ BasisMapping
+// src/BasisMapping.cpp:38-228 (single task, real basis)
+struct Mapping {
+  int np0, np1, np2, nvec;
+  int ntrans0;                  // src/FourierTransform.cpp:104
+  std::vector<int> ip, im, zvec_to_val;
+  void build(const Basis& b, int n0, int n1, int n2);
+};
+
+// ---------------------------------------------------------- 3D FFT (sparse)
+// FourierTransform semantics, src/FourierTransform.cpp:147-827.
+struct FFT3 {
+  const Basis* basis;
+  Mapping map;
+  int np012() const { return map.np0 * map.np1 * map.np2; }
+  void init(const Basis& b, int n0, int n1, int n2);
+  // backward: f(r) = sum_G c(G) e^{+iGr}, unscaled
+  void backward(const cplx* c, cplx* f) const;                       // :175
+  void backward(const cplx* c1, const cplx* c2, cplx* f) const;      // :187
+  // forward: c(G) = 1/N sum_r f(r) e^{-iGr}; overwrites f
+  void forward(cplx* f, cplx* c) const;                              // :147
+  void forward(cplx* f, cplx* c1, cplx* c2) const;                   // :161
+  void bwd_grid(std::vector<cplx>& zvec, cplx* f) const;
+  void fwd_grid(cplx* f, std::vector<cplx>& zvec) const;
+};
+
+// 1D complex FFT of length n, sign = +1 (synthesis) or -1 (analysis),
+// strided in/out; no scaling.  Own mixed-radix implementation.
+void fft1d(cplx* x, int n, int stride, int sign);
+
+// ---------------------------------------------------------------- Species
+// SLPP species, src/Species.cpp:510-1053, spline.cpp, sinft.cpp.
+struct Species {
+  int zval; double deltar; double rcps; int lmax;
+  std::vector<double> vlocal;                  // input radial local potential
+  std::vector<std::vector<double>> proj;       // proj[l][r]
+  std::vector<double> dl;                      // D_l (one projector per l)
+  // derived
+  int ndft = 0, nop = 0, nlm = 0;
+  std::vector<double> rps, gspl, vlocg, vlocg2;
+  std::vector<std::vector<double>> vnlg, vnlg2;
+  void initialize();
+  void dvlocg(double g, double& v, double& dv) const;     // :1145-1156
+  void dvnlg(int iop, double g, double& v, double& dv) const; // :1172-1185
+  double rhopsg(double g) const;                            // :1187-1191
+  double eself() const;                                     // Species.h:133
+  double dmatrix(int ipr, int jpr) const;                   // :1450-1464
+};
+void sinft(int n, double* y);                  // src/sinft.cpp:50-105 (restated)
+void cosft1(int n, double* y);                 // src/sinft.cpp:108-174 (restated)
+void spline(int n, const double* x, const double* y, double yp1, double ypn,
+            int bcnat_left, int bcnat_right, double* y2);   // spline.cpp:44-108
+void splint(int n, const double* xa, const double* ya, const double* y2a,
+            double x, double* y);                          // spline.cpp:110-136
+void splintd(int n, const double* xa, const double* ya, const double* y2a,
+             double x, double* y, double* dy);             // spline.cpp:138-168
+
+// LDA (Perdew-Zunger/Ceperley-Alder), src/LDAFunctional.cpp:92-151
+void lda_unpolarized(double rh, double& ee, double& vv);
+
+// --------------------------------------------------------- whole evaluation
+struct Results {
+  double ekin, eps, enl, ehart, ehart_e, ehart_ep, ehart_p, exc, dxc, esr,
+         eself, ecoul, etotal;
+  std::vector<double> fion;          // 3*nat, species-major
+  double sigma[6];
+  double sigma_ekin[6], sigma_eps[6], sigma_enl[6], sigma_ehart[6],
+         sigma_exc[6], sigma_esr[6];
+};
+
+struct System {
+  // inputs
+  Cell cell; bool has_ref = false; Cell refcell;
+  double ecut = 0.0;                  // Hartree
+  std::vector<Species> sp;
+  std::vector<std::vector<double>> tau;   // tau[is][3*ia]
+  int nst = 0;
+  std::vector<double> occ;
+  // wavefunction and density bases and grids
+  Basis wb, vb;
+  FFT3 ft;      // wf basis on the density grid (ChargeDensity ft_[0])
+  FFT3 vft;     // density basis on the density grid
+  // setup tables (cell_moved / atoms_moved)
+  std::vector<std::vector<double>> vps, dvps, rhops;     // [is][ngv]
+  std::vector<std::vector<cplx>> sfac;                   // [is][ngv]
+  std::vector<cplx> vion_local_g, dvion_local_g, rhopst;
+  std::vector<std::vector<double>> twnl, dtwnl;          // [is][ipr*ngw+ig]
+  std::vector<std::vector<int>> lproj;
+  std::vector<double> fion_esr; double esr = 0.0; double sigma_esr[6];
+  double eself_total = 0.0;
+  // state
+  std::vector<double> rhor;           // density, real space
+  std::vector<cplx> rhog;             // density G
+  std::vector<double> v_r, vxc_r;
+  std::vector<cplx> rhoelg, rhogt;
+  double exc = 0, dxc = 0, eps = 0, ehart = 0, ehart_e = 0, ehart_ep = 0,
+         ehart_p = 0, total_charge = 0, sigma_exc[6];
+
+  void setup();                         // EnergyFunctional ctor: bases, tables
+  void cell_moved();                    // src/EnergyFunctional.cpp:1200-1245
+  void atoms_moved();                   // :1047-1197
+  void update_twnl();                   // src/NonLocalPotential.cpp:235-1255
+  void update_density(const cplx* c);   // src/ChargeDensity.cpp:136-198
+  void update_vhxc(bool compute_stress);// src/EnergyFunctional.cpp:231-394
+  double energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                bool compute_forces, bool compute_stress, Results& r);  // :397
+  double nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                   bool compute_forces, std::vector<double>& fion_enl,
+                   bool compute_stress, double* sigma_enl);   // NLP:1258-2069
+};
+
+// SlaterDet pieces (src/SlaterDet.cpp)
+void compute_density(const FFT3& ft, const cplx* c, int ngw, int nst,
+                     const double* occ, double weight, double omega, double* rho);
+void rs_mul_add(const FFT3& ft, const double* v, const cplx* c, int ngw,
+                int nst, cplx* hpsi);
+int gram(cplx* c, int ngw, int nst);   // returns 0 or the failing pivot+1
+void cleanup(cplx* c, int ngw, int nst);
+
+}  // namespace qbo

@@ -1,0 +1,45 @@
+// ref_shim.cpp -- C entry points over the reference's own spline, LDA and
+// UnitCell sources (compiled from /root/reference/src by `make ref` into
+// _ref/libqbref.so).  Test infrastructure: used only to pin the oracle.
+#include "spline.h"
+#include "LDAFunctional.h"
+#include "UnitCell.h"
+#include <vector>
+
+extern "C" {
+
+void ref_spline(int n, double* x, double* y, double yp1, double ypn, int bl,
+                int br, double* y2) {
+  spline(n, x, y, yp1, ypn, bl, br, y2);
+}
+
+void ref_splintd(int n, double* xa, double* ya, double* y2a, double x,
+                 double* y, double* dy) {
+  splintd(n, xa, ya, y2a, x, y, dy);
+}
+
+void ref_lda(int n, const double* rho, double* e, double* v) {
+  std::vector<std::vector<double> > rhoe(1, std::vector<double>(rho, rho + n));
+  LDAFunctional f(rhoe);
+  f.setxc();
+  for (int i = 0; i < n; i++) { e[i] = f.exc[i]; v[i] = f.vxc1[i]; }
+}
+
+void ref_cell(const double* a9, double* b9, double* vol) {
+  UnitCell c(D3vector(a9[0], a9[1], a9[2]), D3vector(a9[3], a9[4], a9[5]),
+             D3vector(a9[6], a9[7], a9[8]));
+  for (int i = 0; i < 3; i++) {
+    b9[3 * i] = c.b(i).x; b9[3 * i + 1] = c.b(i).y; b9[3 * i + 2] = c.b(i).z;
+  }
+  *vol = c.volume();
+}
+
+void ref_fold_in_ws(const double* a9, double* v) {
+  UnitCell c(D3vector(a9[0], a9[1], a9[2]), D3vector(a9[3], a9[4], a9[5]),
+             D3vector(a9[6], a9[7], a9[8]));
+  D3vector w(v[0], v[1], v[2]);
+  c.fold_in_ws(w);
+  v[0] = w.x; v[1] = w.y; v[2] = w.z;
+}
+
+}  // extern "C"

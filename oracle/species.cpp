@@ -1,0 +1,279 @@
+// species.cpp -- oracle restatement of the SLPP species tables, splines,
+// sine/cosine transforms and the LDA functional.
+// TEST INFRASTRUCTURE ONLY (see qbo.h).
+#include "qbo.h"
+#include <cmath>
+#include <cassert>
+#include <stdexcept>
+
+namespace qbo {
+
+// src/Species.cpp:34-48
+static double simpsn(int n, const double* t) {
+  const double c0 = 17.0 / 48.0, c1 = 59.0 / 48.0, c2 = 43.0 / 48.0,
+               c3 = 49.0 / 48.0;
+  double sum = c0 * (t[0] + t[n - 1]) + c1 * (t[1] + t[n - 2]) +
+               c2 * (t[2] + t[n - 3]) + c3 * (t[3] + t[n - 4]);
+  for (int i = 4; i < n - 4; i++) sum += t[i];
+  return sum;
+}
+
+// Table of sin/cos(pi m / n), m in [0, 2n), in extended precision.
+static void trig_table(int n, std::vector<double>& s, std::vector<double>& c) {
+  s.resize(2 * n);
+  c.resize(2 * n);
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (int m = 0; m < 2 * n; m++) {
+    long double a = pi * m / n;
+    s[m] = (double)sinl(a);
+    c[m] = (double)cosl(a);
+  }
+}
+
+// src/sinft.cpp:50-105: y[i] <- sum_{j=1}^{n-1} y[j] sin(pi i j / n),
+// evaluated here as the direct sum it equals (odd extension, 2n-point DFT).
+void sinft(int n, double* y) {
+  std::vector<double> s, c, out(n);
+  trig_table(n, s, c);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double acc = 0.0;
+    long m = 0;
+    for (int j = 1; j < n; j++) {
+      m += i; if (m >= 2 * n) m -= 2 * n;
+      acc += y[j] * s[m];
+    }
+    out[i] = acc;
+  }
+  for (int i = 0; i < n; i++) y[i] = out[i];
+}
+
+// src/sinft.cpp:108-174: y[i] <- y0/2 + (-1)^i y[n]/2 + sum_{j=1}^{n-1}
+// y[j] cos(pi i j / n), i < n (even extension, 2n-point DFT; y has n+1
+// entries and y[n] is left as is).
+void cosft1(int n, double* y) {
+  std::vector<double> s, c, out(n);
+  trig_table(n, s, c);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double acc = 0.5 * y[0] + 0.5 * ((i % 2 == 0) ? y[n] : -y[n]);
+    long m = 0;
+    for (int j = 1; j < n; j++) {
+      m += i; if (m >= 2 * n) m -= 2 * n;
+      acc += y[j] * c[m];
+    }
+    out[i] = acc;
+  }
+  for (int i = 0; i < n; i++) y[i] = out[i];
+}
+
+// src/spline.cpp:21-42
+static void tridsolve(int n, double* d, double* e, double* f, double* x) {
+  for (int i = 1; i < n; i++) {
+    f[i] /= d[i - 1];
+    d[i] -= f[i] * e[i - 1];
+  }
+  for (int i = 1; i < n; i++) x[i] -= f[i] * x[i - 1];
+  x[n - 1] /= d[n - 1];
+  for (int i = n - 2; i >= 0; i--) x[i] = (x[i] - e[i] * x[i + 1]) / d[i];
+}
+
+// src/spline.cpp:44-108
+void spline(int n, const double* x, const double* y, double yp_left,
+            double yp_right, int bcnat_left, int bcnat_right, double* y2) {
+  const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+  std::vector<double> d(n), e(n), f(n);
+  if (bcnat_left == 0) {
+    const double h = x[1] - x[0];
+    d[0] = third * h; e[0] = sixth * h; f[0] = 0.0;
+    y2[0] = (y[1] - y[0]) / h - yp_left;
+  } else {
+    d[0] = 1.0; e[0] = 0.0; f[0] = 0.0; y2[0] = 0.0;
+  }
+  if (bcnat_right == 0) {
+    const double h = x[n - 1] - x[n - 2];
+    d[n - 1] = third * h; e[n - 1] = 0.0; f[n - 1] = sixth * h;
+    y2[n - 1] = yp_right - (y[n - 1] - y[n - 2]) / h;
+  } else {
+    d[n - 1] = 1.0; e[n - 1] = 0.0; f[n - 1] = 0.0; y2[n - 1] = 0.0;
+  }
+  for (int i = 1; i < n - 1; i++) {
+    const double hp = x[i + 1] - x[i], hm = x[i] - x[i - 1];
+    d[i] = third * (hp + hm);
+    e[i] = sixth * hp;
+    f[i] = sixth * hm;
+    y2[i] = (y[i + 1] - y[i]) / hp - (y[i] - y[i - 1]) / hm;
+  }
+  tridsolve(n, d.data(), e.data(), f.data(), y2);
+}
+
+static inline void bracket(int n, const double* xa, double x, int& kl, int& kh) {
+  kl = 0; kh = n - 1;
+  while (kh - kl > 1) {
+    const int k = (kh + kl) / 2;
+    if (xa[k] > x) kh = k; else kl = k;
+  }
+}
+
+// src/spline.cpp:110-136
+void splint(int n, const double* xa, const double* ya, const double* y2a,
+            double x, double* y) {
+  int kl, kh;
+  bracket(n, xa, x, kl, kh);
+  const double h = xa[kh] - xa[kl];
+  const double a = (xa[kh] - x) / h, b = (x - xa[kl]) / h;
+  *y = a * ya[kl] + b * ya[kh] +
+       h * h * (1.0 / 6.0) * ((a * a * a - a) * y2a[kl] + (b * b * b - b) * y2a[kh]);
+}
+
+// src/spline.cpp:138-168
+void splintd(int n, const double* xa, const double* ya, const double* y2a,
+             double x, double* y, double* dy) {
+  int kl, kh;
+  bracket(n, xa, x, kl, kh);
+  const double h = xa[kh] - xa[kl];
+  const double a = (xa[kh] - x) / h, b = (x - xa[kl]) / h;
+  *y = a * ya[kl] + b * ya[kh] +
+       h * h * (1.0 / 6.0) * ((a * a * a - a) * y2a[kl] + (b * b * b - b) * y2a[kh]);
+  *dy = (ya[kh] - ya[kl]) / h +
+        h * (((1.0 / 6.0) - 0.5 * a * a) * y2a[kl] + (0.5 * b * b - (1.0 / 6.0)) * y2a[kh]);
+}
+
+// src/Species.cpp:510-1053 (SLPP; one projector per l, l <= 1)
+void Species::initialize() {
+  const double fpi = 4.0 * M_PI;
+  const int np = (int)vlocal.size();
+  if (lmax > 1) throw std::runtime_error("oracle species: l > 1 not restated");
+  nop = lmax + 1;
+  nlm = 0;
+  for (int l = 0; l <= lmax; l++) nlm += 2 * l + 1;
+  ndft = 1;
+  while (ndft * deltar < 40.0) ndft *= 2;
+  rps.resize(ndft);
+  for (int i = 0; i < ndft; i++) rps[i] = i * deltar;
+  std::vector<std::vector<double>> phi(nop, std::vector<double>(ndft, 0.0));
+  for (int iop = 0; iop < nop; iop++)
+    for (int i = 0; i < np; i++) phi[iop][i] = proj[iop][i];
+  std::vector<double> vlocr(ndft), fint(ndft);
+  for (int i = 0; i < np; i++) vlocr[i] = vlocal[i];
+  for (int i = np; i < ndft; i++) vlocr[i] = -zval / rps[i];
+  const double ztot = zval;
+  // :648-655 subtract the smeared-charge long range part
+  vlocr[0] += (ztot / rcps) * M_2_SQRTPI;
+  for (int i = 1; i < ndft; i++) vlocr[i] += (ztot / rps[i]) * std::erf(rps[i] / rcps);
+  for (int i = 0; i < ndft; i++) vlocr[i] *= fpi * rps[i] * deltar;
+  for (int i = 0; i < ndft; i++) fint[i] = vlocr[i] * rps[i];
+  const double v0 = simpsn(ndft, fint.data());
+  vlocg = vlocr;
+  sinft(ndft, vlocg.data());
+  gspl.resize(ndft);
+  gspl[0] = 0.0;
+  vlocg[0] = v0;
+  const double fac = M_PI / (ndft * deltar);
+  for (int i = 1; i < ndft; i++) {
+    gspl[i] = i * fac;
+    vlocg[i] /= gspl[i];
+  }
+  vlocg2.resize(ndft);
+  spline(ndft, gspl.data(), vlocg.data(), 0.0, 0.0, 0, 1, vlocg2.data());
+
+  vnlg.assign(nop, std::vector<double>(ndft + 1, 0.0));
+  vnlg2.assign(nop, std::vector<double>(ndft + 1, 0.0));
+  for (int iop = 0; iop < nop; iop++) {
+    std::vector<double> vnlr(ndft);
+    for (int i = 0; i < ndft; i++) vnlr[i] = fpi * deltar * phi[iop][i];
+    std::vector<double>& v = vnlg[iop];
+    if (iop == 0) {  // :782-811
+      for (int i = 0; i < ndft; i++) fint[i] = vnlr[i] * rps[i] * rps[i];
+      const double w0 = simpsn(ndft, fint.data());
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i] * rps[i];
+      sinft(ndft, v.data());
+      v[0] = w0;
+      for (int i = 1; i < ndft; i++) v[i] /= gspl[i];
+      spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 0, 1, vnlg2[iop].data());
+    } else {  // l == 1, :812-855
+      const double w0 = 0.0;
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i];
+      sinft(ndft, v.data());
+      fint[0] = w0;
+      for (int i = 1; i < ndft; i++) fint[i] = v[i] / (gspl[i] * gspl[i]);
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i] * rps[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      v[0] = w0;
+      for (int i = 1; i < ndft; i++) v[i] = fint[i] - v[i] / gspl[i];
+      spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 1, 1, vnlg2[iop].data());
+    }
+  }
+}
+
+void Species::dvlocg(double g, double& v, double& dv) const {
+  if (g > gspl[ndft - 1]) { v = 0.0; dv = 0.0; return; }
+  splintd(ndft, gspl.data(), vlocg.data(), vlocg2.data(), g, &v, &dv);
+}
+
+void Species::dvnlg(int iop, double g, double& v, double& dv) const {
+  if (g > gspl[ndft - 1]) { v = 0.0; dv = 0.0; return; }
+  splintd(ndft, gspl.data(), vnlg[iop].data(), vnlg2[iop].data(), g, &v, &dv);
+}
+
+double Species::rhopsg(double g) const {
+  const double arg = 0.25 * rcps * rcps * g * g;
+  return -(double)zval * std::exp(-arg);
+}
+
+double Species::eself() const {
+  const double z = zval;
+  return z * z / (std::sqrt(2.0 * M_PI) * rcps);
+}
+
+// src/Species.cpp:1417-1464: nonzero only on the (l,m) diagonal
+double Species::dmatrix(int ipr, int jpr) const {
+  auto lm = [&](int ipr_, int& l, int& m) {
+    int j = 0;
+    for (int ll = 0; ll <= lmax; ll++)
+      for (int mm = -ll; mm <= ll; mm++) {
+        if (j == ipr_) { l = ll; m = mm; return; }
+        j++;
+      }
+    l = -1; m = 0;
+  };
+  int li, mi, lj, mj;
+  lm(ipr, li, mi);
+  lm(jpr, lj, mj);
+  if (li != lj || mi != mj) return 0.0;
+  return dl[li];
+}
+
+// src/LDAFunctional.cpp:92-151
+void lda_unpolarized(double rh, double& ee, double& vv) {
+  const double c1 = 0.6203504908994001;
+  const double c3 = -0.610887057711;
+  const double A = 0.0311, B = -0.048, b1 = 1.0529, b2 = 0.3334, G = -0.1423;
+  const double D = G / (1.0 + b1 + b2) - B;
+  const double C = -A - D - G * ((b1 / 2.0 + b2) / ((1.0 + b1 + b2) * (1.0 + b1 + b2)));
+  ee = 0.0;
+  vv = 0.0;
+  if (rh > 0.0) {
+    const double ro13 = std::cbrt(rh);
+    const double rs = c1 / ro13;
+    double ex, vx, ec, vc;
+    vx = c3 / rs;
+    ex = 0.75 * vx;
+    if (rs < 1.0) {
+      const double logrs = std::log(rs);
+      ec = A * logrs + B + C * rs * logrs + D * rs;
+      vc = A * logrs + (B - A / 3.0) + (2.0 / 3.0) * C * rs * logrs +
+           ((2.0 * D - C) / 3.0) * rs;
+    } else {
+      const double sqrtrs = std::sqrt(rs);
+      const double den = 1.0 + b1 * sqrtrs + b2 * rs;
+      ec = G / den;
+      vc = ec * (1.0 + (7.0 / 6.0) * b1 * sqrtrs + (4.0 / 3.0) * b2 * rs) / den;
+    }
+    ee = ex + ec;
+    vv = vx + vc;
+  }
+}
+
+}  // namespace qbo

@@ -1,0 +1,191 @@
+"""Synthetic Qbox inputs: cells, SLPP species tables, atom positions and
+Gamma-point wavefunction coefficients.
+
+A species is given as Qbox's ``norm_conserving_semilocal_pseudopotential``
+would hold it (src/SpeciesHandler.cpp:200-241): a local potential
+v_loc(r) = -Z erf(r/rc)/r and one projector per l (l = 0, 1) with radial form
+r^l exp(-r^2 / (2 sigma^2)), on a linear mesh, plus D_l.  Qbox reads species
+through a 12-digit scientific print (src/Species.cpp:1240-1241 via
+src/SpeciesCmd.cpp:42-58), so every radial value is rounded the same way here.
+``rcps`` is fixed at 1.5 bohr for every species (src/AtomSet.cpp:43-44).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RCPS = 1.5          # src/AtomSet.cpp:43-44
+DELTAR = 0.01       # mesh spacing (bohr)
+NMESH = 2000        # radial points
+
+
+def _round12(x: np.ndarray) -> np.ndarray:
+    """Round through '%.12e' like Species::print followed by a re-read."""
+    return np.array([float("%.12e" % v) for v in np.asarray(x, dtype=float)])
+
+
+@dataclass
+class SpeciesDef:
+    zval: int
+    rc: float               # erf width of the local potential
+    sigma: float            # Gaussian width of the projectors
+    d: tuple = (2.0, -1.0)  # D_0, D_1 (Ha)
+    deltar: float = DELTAR
+    nmesh: int = NMESH
+    rcps: float = RCPS
+
+    @property
+    def lmax(self) -> int:
+        return len(self.d) - 1
+
+    def radial(self):
+        """Return (vloc[nmesh], proj[(lmax+1)*nmesh], dl[lmax+1])."""
+        r = np.arange(self.nmesh) * self.deltar
+        v = np.empty_like(r)
+        v[0] = -self.zval * 2.0 / (math.sqrt(math.pi) * self.rc)
+        rr = r[1:]
+        v[1:] = -self.zval * np.array([math.erf(x) for x in rr / self.rc]) / rr
+        g = np.exp(-r * r / (2.0 * self.sigma ** 2))
+        proj = [g] + [r ** l * g for l in range(1, self.lmax + 1)]
+        vloc = _round12(v)
+        proj = np.concatenate([_round12(p) for p in proj])
+        return vloc, proj, np.array(self.d, dtype=float)
+
+
+SPECIES_A = SpeciesDef(zval=4, rc=1.0, sigma=0.8)
+SPECIES_B = SpeciesDef(zval=6, rc=0.7, sigma=0.6)
+
+
+@dataclass
+class SystemDef:
+    name: str
+    cell: np.ndarray                 # 3x3, rows are a0, a1, a2 (bohr)
+    ecut_ry: float
+    species: list                    # list[SpeciesDef]
+    na: list                         # atoms per species
+    nst: int
+    seed: int
+    min_sep: float
+    tau: np.ndarray = field(default=None)   # (nat, 3), species-major
+
+    @property
+    def ecut(self) -> float:         # Hartree (src/Ecut.h:38-57)
+        return 0.5 * self.ecut_ry
+
+    @property
+    def nat(self) -> int:
+        return int(sum(self.na))
+
+    @property
+    def occ(self) -> np.ndarray:
+        return np.full(self.nst, 2.0)
+
+    def build_positions(self) -> "SystemDef":
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        self.tau = random_positions(rng, self.cell, self.nat, self.min_sep)
+        return self
+
+
+def random_positions(rng, cell, nat, min_sep, max_tries=200000):
+    """Uniform positions in the cell, rejection-sampled to a minimum-image
+    separation of at least ``min_sep``."""
+    A = np.asarray(cell, dtype=float)
+    Ainv = np.linalg.inv(A)
+    pos = []
+    tries = 0
+    while len(pos) < nat:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place atoms at this separation")
+        f = rng.random(3)
+        x = f @ A
+        ok = True
+        for y in pos:
+            d = x - y
+            s = d @ Ainv
+            s -= np.round(s)
+            if np.linalg.norm(s @ A) < min_sep:
+                ok = False
+                break
+        if ok:
+            pos.append(x)
+    return np.array(pos)
+
+
+def cubic(L):
+    return np.diag([L, L, L]).astype(float)
+
+
+BASE_SEED = 20261202
+
+
+def config(index: int) -> SystemDef:
+    """The five workloads (index 0..4) plus small test systems (index >= 10)."""
+    if index == 0:
+        s = SystemDef("c1: 8 atoms L=16 ecut=30Ry 16 bands", cubic(16.0), 30.0,
+                      [SPECIES_A], [8], 16, BASE_SEED + 0, 2.5)
+    elif index == 1:
+        s = SystemDef("c2: 64 atoms L=24 ecut=60Ry 128 bands", cubic(24.0), 60.0,
+                      [SPECIES_A], [64], 128, BASE_SEED + 1, 2.5)
+    elif index == 2:
+        s = SystemDef("c3: 256 atoms L=36 ecut=70Ry 512 bands", cubic(36.0), 70.0,
+                      [SPECIES_A, SPECIES_B], [128, 128], 512, BASE_SEED + 2, 2.2)
+    elif index == 4:
+        s = SystemDef("c5: 128 atoms 28x30x32 ecut=60Ry 256 bands",
+                      np.diag([28.0, 30.0, 32.0]), 60.0,
+                      [SPECIES_A, SPECIES_B], [64, 64], 256, BASE_SEED + 4, 2.2)
+    elif index == 10:
+        s = SystemDef("t0: 2 atoms L=8 ecut=12Ry 4 bands", cubic(8.0), 12.0,
+                      [SPECIES_A], [2], 4, BASE_SEED + 10, 2.0)
+    elif index == 11:
+        s = SystemDef("t1: 2+1 atoms 7x8x9 ecut=16Ry 5 bands", np.diag([7.0, 8.0, 9.0]),
+                      16.0, [SPECIES_A, SPECIES_B], [2, 1], 5, BASE_SEED + 11, 2.0)
+    elif index == 12:
+        # non-orthogonal (monoclinic) cell
+        a = np.array([[9.0, 0.0, 0.0], [1.5, 8.5, 0.0], [0.0, 0.7, 8.0]])
+        s = SystemDef("t2: 3 atoms monoclinic ecut=14Ry 6 bands", a, 14.0,
+                      [SPECIES_A], [3], 6, BASE_SEED + 12, 2.0)
+    else:
+        raise ValueError(index)
+    return s.build_positions()
+
+
+def config4_systems():
+    """Twelve uneven Gamma-point systems (first species), for batching."""
+    rng = np.random.Generator(np.random.PCG64(BASE_SEED + 3))
+    out = []
+    for k in range(12):
+        n = int(rng.choice([8, 16, 32, 64, 96, 128]))
+        L = 16.0 * (n / 8.0) ** (1.0 / 3.0)
+        ecut = float(rng.uniform(40.0, 80.0))
+        s = SystemDef(f"c4[{k}]: {n} atoms L={L:.3f} ecut={ecut:.2f}Ry", cubic(L), ecut,
+                      [SPECIES_A], [n], 2 * n, BASE_SEED + 100 + k, 2.5)
+        out.append(s.build_positions())
+    return out
+
+
+def random_coefficients(seed: int, g2: np.ndarray, nst: int) -> np.ndarray:
+    """(N(0,1) + i N(0,1)) / (1 + |G|^2) on the half sphere, Im c(G=0) = 0.
+
+    Returns complex128 array of shape (nst, ngw) (band-major, i.e. Qbox's
+    column-major c[ig + ngw*n])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ngw = g2.shape[0]
+    z = rng.standard_normal((nst, ngw, 2))
+    c = (z[..., 0] + 1j * z[..., 1]) / (1.0 + g2)[None, :]
+    c[:, 0] = c[:, 0].real
+    return np.ascontiguousarray(c)
+
+
+def gram_numpy(c: np.ndarray) -> np.ndarray:
+    """Gamma-convention Cholesky orthonormalization in FP64 (host, numpy):
+    S = 2 P^T P - p0 p0^T with P the real view, C <- C L^-T."""
+    c = c.copy()
+    c[:, 0] = c[:, 0].real
+    P = c.view(np.float64).reshape(c.shape[0], -1)       # (nst, 2 ngw)
+    S = 2.0 * (P @ P.T) - np.outer(P[:, 0], P[:, 0])
+    L = np.linalg.cholesky(S)
+    X = np.linalg.solve(L, P)                             # L X = P  -> rows
+    return np.ascontiguousarray(X).view(np.complex128).reshape(c.shape)
