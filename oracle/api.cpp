@@ -117,9 +117,14 @@ void qbo_compute_density(void* h, const double* c, double* rho) {
   compute_density(s->ft, (const cplx*)c, s->wb.size, s->nst, s->occ.data(), 1.0,
                   s->cell.volume, rho);
 }
-void qbo_rs_mul_add(void* h, const double* v, const double* c, double* hpsi) {
+void qbo_rs_mul_add(void* h, const double* v, const double* c, int nst, double* hpsi) {
   System* s = (System*)h;
-  rs_mul_add(s->ft, v, (const cplx*)c, s->wb.size, s->nst, (cplx*)hpsi);
+  rs_mul_add(s->ft, v, (const cplx*)c, s->wb.size, nst, (cplx*)hpsi);
+}
+void qbo_compute_density_occ(void* h, const double* c, int nst, const double* occ,
+                             double* rho) {
+  System* s = (System*)h;
+  compute_density(s->ft, (const cplx*)c, s->wb.size, nst, occ, 1.0, s->cell.volume, rho);
 }
 int qbo_gram(double* c, int ngw, int nst) { return gram((cplx*)c, ngw, nst); }
 

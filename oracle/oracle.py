@@ -155,7 +155,7 @@ class Oracle:
         c = np.ascontiguousarray(c, dtype=np.complex128)
         v = np.ascontiguousarray(v, dtype=float)
         h = np.zeros_like(c) if hpsi is None else np.array(hpsi, dtype=np.complex128)
-        lib().qbo_rs_mul_add(self.h, _ptr(v), _ptr(c), _ptr(h))
+        lib().qbo_rs_mul_add(self.h, _ptr(v), _ptr(c), c.shape[0], _ptr(h))
         return h
 
     def update_density(self, c):
@@ -211,6 +211,15 @@ class Oracle:
         vn, dvn = np.zeros(nop * n), np.zeros(nop * n)
         lib().qbo_species_eval(self.h, isp, n, _ptr(g), _ptr(v), _ptr(dv), _ptr(vn), _ptr(dvn))
         return v, dv, vn.reshape(nop, n), dvn.reshape(nop, n)
+
+
+def compute_density_occ(o, c, occ):
+    """SlaterDet::compute_density for arbitrary bands/occupations of o's basis."""
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    occ = np.ascontiguousarray(occ, dtype=float)
+    rho = np.zeros(o.N)
+    lib().qbo_compute_density_occ(o.h, _ptr(c), c.shape[0], _ptr(occ), _ptr(rho))
+    return rho
 
 
 def gram(c):

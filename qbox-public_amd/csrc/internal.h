@@ -1,0 +1,114 @@
+// internal.h -- shared definitions of libqbg (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "qbg.h"
+
+namespace qbg {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+// FFT factorization of one line length: radices applied in order
+struct FftPlan {
+  int n;
+  int nstage;
+  int radix[12];
+};
+
+FftPlan make_plan(int n);
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+
+}  // namespace qbg
+
+struct qbg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  // per-kernel event profiler
+  bool prof = false;
+  struct Rec { int kind; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[QBG_K_COUNT] = {0};
+  int prof_n[QBG_K_COUNT] = {0};
+  hipEvent_t prof_event();
+  void prof_begin(int kind);
+  void prof_end();
+  void prof_collect();
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+// column tile of the z pass: columns [col0, col0+ncol), G range [g0, g1)
+struct qbg_tile {
+  int col0, ncol, g0, g1;
+};
+
+struct qbg_grid {
+  qbg_ctx* ctx = nullptr;
+  int np0 = 0, np1 = 0, np2 = 0, ng = 0, nvec = 0;
+  int ntrans0 = 0;     // active y rows: [0, ntrans0) and [np1-ntrans0, np1)
+  int nact = 0;        // number of active rows (2*ntrans0, capped at np1)
+  int ntiles = 0;
+  int zcols = 8;       // columns per z tile
+  int xrows = 8;       // rows per x-pass block
+  int ytile = 8;       // x values per y-pass block
+  qbg::FftPlan plan0, plan1, plan2;
+  // device tables
+  int32_t* d_ip = nullptr;
+  int32_t* d_im = nullptr;
+  qbg_tile* d_tiles = nullptr;
+  int32_t* d_row_start = nullptr;   // nact+1
+  int32_t* d_rc_ivec = nullptr;     // nvec, columns sorted by (row, x)
+  int32_t* d_rc_x = nullptr;        // nvec
+  int32_t* d_act_row = nullptr;     // nact: y index of active row r
+  double* d_tw64[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i k / n)
+  float* d_tw32[3] = {nullptr, nullptr, nullptr};
+  // workspace for batched pairs (grown on demand)
+  void* d_zbuf = nullptr;
+  void* d_xbuf = nullptr;
+  size_t ws_bytes = 0;
+  double* d_fac = nullptr;          // per-band density factors
+  int fac_cap = 0;
+  int batch = 64;                   // max pairs per batch
+};
+
+// launchers (fft.hip)
+namespace qbg {
+enum ZMode { Z_PAIR = 0, Z_SINGLE = 1 };
+enum YMode { Y_VMUL = 0, Y_DENSITY = 1, Y_BWD_FULL = 2, Y_FWD_FULL = 3,
+             Y_BWD_REAL_ADD = 4, Y_FWD_REAL = 5 };
+enum OutMode { OUT_STORE = 0, OUT_ACCUM = 1 };
+
+// batched pass launchers (fft.hip).  Pair b of a batch reads/writes the
+// bands at c + 2*b*pstride (first) and c + 2*(b*pstride + off2) (second,
+// absent for the last pair when last_has2 == 0).
+int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long off2,
+         int last_has2, int nbatch, hipStream_t s);
+int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s);
+int xfwd(qbg_grid* g, int prec, int nbatch, hipStream_t s);
+int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const double* fac,
+          double* rho, double* fgrid, double scale, hipStream_t s);
+int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
+         int last_has2, int nbatch, hipStream_t s);
+}  // namespace qbg
+
+#define QBG_HIP(ctx, call)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return (ctx)->fail(QBG_E_HIP, std::string(#call) + ": " +             \
+                                        hipGetErrorString(e_));              \
+  } while (0)

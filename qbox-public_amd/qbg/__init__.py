@@ -1,0 +1,7 @@
+"""qbg: MI355X-native Hamiltonian application for the Qbox plane-wave path.
+
+The compute lives in libqbg.so (hand-written HIP kernels for gfx950 behind
+the C ABI of include/qbg.h); this package is the Python harness over it.
+"""
+from . import configs  # noqa: F401
+from ._lib import Context, DArray, Grid, QbgError, load  # noqa: F401
