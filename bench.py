@@ -1,0 +1,182 @@
+"""Benchmark: H*psi band-applies/s of one full Gamma-point evaluation on one
+MI355X, with the roofline of the dominant kernel and the CPU oracle timed on
+the host's cores beside it.
+
+One step = ChargeDensity::update_density -> EnergyFunctional::update_vhxc ->
+EnergyFunctional::energy(compute_hpsi=true, compute_forces=true), the
+sequence of src/BOSampleStepper.cpp:1207-1217, on synthetic inputs already
+resident in HBM.  value = bands / (seconds per step).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+                       [--prec fp64] [--cpu-seconds S | --no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "qbox-public_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def kernel_bytes(ef, kind, npairs_per_launch, sc):
+    """Algorithmic HBM bytes of one launch of a pass kernel.
+
+    Per band pair (canonical fused schedule): z-bwd reads two bands (2 ngw
+    complex FP64) and writes the columns (nvec np2 S_c); x-bwd reads the
+    columns and writes the active rows A = 2 ntrans0 np0 np2; the y pass
+    reads and writes A and reads v (N doubles); x-fwd and z-fwd mirror the
+    backward passes, z-fwd read-modify-writes the two H*psi bands.
+    """
+    ngw, nvec, np2 = ef.ngw, ef.nvec, ef.np2
+    cols = nvec * np2 * sc
+    A = ef.nact * ef.np0 * ef.np2 * sc
+    per_pair = {
+        "zbwd": 2 * ngw * 16 + cols,
+        "xbwd": cols + A,
+        "yvmul": 2 * A + ef.N * 8,
+        "ydens": A,
+        "xfwd": A + cols,
+        "zfwd": cols + 2 * 2 * ngw * 16,
+    }[kind]
+    extra = 2 * ef.N * 8 if kind == "ydens" else 0     # rho read-modify-write per launch
+    return per_pair * npairs_per_launch + extra
+
+
+def cpu_baseline(sysdef, c, seconds):
+    """Time the FP64 CPU oracle (OpenMP) on the same workload: repeat the
+    full evaluation until about `seconds` of work is done (at least once)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    o = oracle.Oracle(sysdef)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        o.evaluate(c, hpsi=True, forces=True)
+        reps += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {"value": sysdef.nst / dt, "unit": "band-applies/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} full FP64 evaluation(s) of the same workload on the CPU oracle, "
+                      f"{dt:.2f} s each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--prec", default="fp64", choices=["fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (band sharding + RCCL reduction of rho), so "
+                                   "--gpus must be 1"}))
+        sys.exit(2)
+
+    import qbg
+    from qbg import configs as cf
+
+    sysdef = cf.config(args.config)
+    ctx = qbg.Context(0)
+    ef = qbg.EnergyFunctional(ctx, sysdef)
+    b = ef.basis(0)
+    c = cf.random_coefficients(cf.BASE_SEED + 50 + args.config, b["g2"], sysdef.nst)
+    dc = ctx.to_device(c)
+    qbg.gram(ctx, dc, ef.ngw, ef.ngw, sysdef.nst)        # orthonormalize on the device
+    hp = ctx.array(c.shape, np.complex128)
+    # active x rows of the wavefunction grid (for the byte model)
+    ky = b["idx"][:, 1]
+    ef.nact = 2 * (int(np.abs(ky).max()) + 1)
+
+    for _ in range(args.warmup):
+        ef.evaluate(dc, hp, forces=True, prec=args.prec)
+    ctx.sync()
+    prof = qbg.Profiler(ctx)
+    prof.reset()
+    prof.enable(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    res = None
+    for _ in range(args.steps):
+        res = ef.evaluate(dc, hp, forces=True, prec=args.prec)
+    gpu_ms = ctx.timer_stop()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    prof.enable(False)
+    kt = prof.read()
+
+    ms_per_step = 1e3 * wall / args.steps
+    value = sysdef.nst / (wall / args.steps)
+
+    # dominant kernel and its roofline
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    name, (tot_ms, nlaunch) = dom
+    npairs = (sysdef.nst + 1) // 2
+    sc = 16
+    if name in ("zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd"):
+        # launches of this kind per step that run the band pairs
+        per_step = nlaunch / args.steps
+        pairs_per_launch = npairs if name in ("yvmul", "ydens") else None
+        if pairs_per_launch is None:
+            # density and H*psi each run one band-pair launch; vbasis single
+            # transforms are small and counted with the same model
+            pairs_per_launch = npairs
+        traffic_bytes = kernel_bytes(ef, name, pairs_per_launch, sc)
+        avg_ms = tot_ms / nlaunch
+        achieved = traffic_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "algorithmic_bytes_per_launch": int(traffic_bytes),
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_step}
+    else:
+        roof = {"kernel": name, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": None, "traffic": None}
+
+    out = {
+        "metric": "H*psi band-applies/s (density + update_vhxc + energy with H*psi and forces)",
+        "value": round(value, 2),
+        "unit": "band-applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": sysdef.name, "atoms": sysdef.nat, "bands": sysdef.nst,
+                   "ecut_ry": sysdef.ecut_ry, "grid": [ef.np0, ef.np1, ef.np2], "ngw": ef.ngw,
+                   "precision": args.prec},
+        "gpu_event_ms_per_step": round(gpu_ms / args.steps, 3),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "etotal": res["etotal"],
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sysdef, c_host_orthonormal(dc), args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def c_host_orthonormal(dc):
+    return dc.download()
+
+
+if __name__ == "__main__":
+    main()

@@ -47,6 +47,7 @@ typedef struct qbg_ef qbg_ef;
 /* ---------------------------------------------------------------- context */
 /* Replaces the MPI/BLACS context of a 1x1 run (src/Context.cpp:60-75). */
 int qbg_init(int device, qbg_ctx** ctx);
+/* grids and energy functionals created on ctx must be destroyed first */
 int qbg_finalize(qbg_ctx* ctx);
 const char* qbg_last_error(const qbg_ctx* ctx);
 int qbg_sync(qbg_ctx* ctx);
@@ -114,6 +115,69 @@ int qbg_compute_density(qbg_grid* g, const double* c, int ldc, int nst,
  * hpsi[:, n] += FFT_fwd( v(r) * FFT_bwd(c[:, n]) ) for nst bands. */
 int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc,
                    int nst, double* hpsi, int ldh, int prec);
+
+/* ------------------------------------------------ SlaterDet::gram */
+/* SlaterDet::gram() for the real basis (src/SlaterDet.cpp:480-534):
+ * cleanup (Im c(G=0) = 0), S = 2 Re(C^H C) - c0 c0^T, S = L L^T (lower
+ * Cholesky), C <- C L^-T, in place on the device array c (ngw x nst,
+ * leading dimension ldc).  QBG_E_NOT_SPD if S is not positive definite, in
+ * which case c is left as cleaned up but otherwise unchanged. */
+int qbg_gram(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst, int prec);
+
+/* ---------------------------------------------------- EnergyFunctional */
+/* A norm-conserving semilocal species as Qbox reads it
+ * (norm_conserving_semilocal_pseudopotential, src/SpeciesHandler.cpp:200-241)
+ * with one projector per l = 0..lmax (lmax <= 1): radial local potential
+ * vloc[nmesh] and projectors proj[(lmax+1)*nmesh] on the mesh i*deltar,
+ * D_l in dl[lmax+1]; rcps is the pseudocharge radius (1.5 in Qbox,
+ * src/AtomSet.cpp:43-44).  Host arrays. */
+typedef struct {
+  int zval;
+  int lmax;
+  int nmesh;
+  double deltar;
+  double rcps;
+  const double* vloc;
+  const double* proj;
+  const double* dl;
+} qbg_species_desc;
+
+/* EnergyFunctional(Sample&, ChargeDensity&) for one Gamma-point system
+ * (src/EnergyFunctional.cpp:46-197, src/ChargeDensity.cpp:34-108): builds
+ * the wavefunction basis (ecut, Hartree) and the density basis (4 ecut) on
+ * the density grid, the species tables and projectors (cell_moved), the
+ * structure factor on the device and the esr sums (atoms_moved).  cell9 is
+ * a0,a1,a2 (bohr); tau[3*nat] is species-major; occ[nst].  Host arrays. */
+int qbg_ef_create(qbg_ctx* ctx, const double* cell9, double ecut, int nsp,
+                  const qbg_species_desc* species, const int* na,
+                  const double* tau, int nst, const double* occ, qbg_ef** out);
+int qbg_ef_destroy(qbg_ef* ef);
+/* out[10]: ngw, ngv, np0, np1, np2, nvec_w, nvec_v, nat, nproj, nst */
+int qbg_ef_sizes(const qbg_ef* ef, int* out);
+/* host copies of the wavefunction (which=0) or density (which=1) basis:
+ * idx[3*ng], g2[ng], gx[3*ng] (x block, y block, z block), maps ip[ng],
+ * im[ng], zvec_to_val[nvec]; any pointer may be null */
+int qbg_ef_basis(const qbg_ef* ef, int which, int32_t* idx, double* g2,
+                 double* gx, int32_t* ip, int32_t* im, int32_t* zvec_to_val);
+/* EnergyFunctional::atoms_moved() with new positions (host tau[3*nat]) */
+int qbg_ef_set_positions(qbg_ef* ef, const double* tau);
+/* ChargeDensity::update_density() from device coefficients c */
+int qbg_ef_update_density(qbg_ef* ef, const double* c, int ldc, int prec);
+/* EnergyFunctional::update_vhxc(compute_stress) */
+int qbg_ef_update_vhxc(qbg_ef* ef, int compute_stress);
+/* EnergyFunctional::energy(compute_hpsi, dwf, compute_forces, fion,
+ * compute_stress, sigma): hpsi (device, overwritten with H*psi when
+ * compute_hpsi); energies (host, 14 doubles: ekin eps enl ehart ehart_e
+ * ehart_ep ehart_p exc dxc esr eself ecoul etotal total_charge); fion (host,
+ * 3*nat); sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr). */
+int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
+                  double* hpsi, int ldh, int compute_forces, int compute_stress,
+                  int prec, double* energies, double* fion, double* sigma);
+/* host copy of a device array: 0 rhor, 1 v_r, 2 vxc_r, 3 rhog,
+ * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr */
+int qbg_ef_get(qbg_ef* ef, int what, double* host_out);
+/* replace v_r (host array of np012 doubles) */
+int qbg_ef_set_vr(qbg_ef* ef, const double* host_v);
 
 #ifdef __cplusplus
 }

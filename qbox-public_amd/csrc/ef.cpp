@@ -1,0 +1,632 @@
+// ef.cpp -- EnergyFunctional drop-in: a whole Gamma-point system resident on
+// one GPU (src/EnergyFunctional.cpp, src/ChargeDensity.cpp,
+// src/NonLocalPotential.cpp, src/StructureFactor.cpp), plus SlaterDet::gram.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include "energy.h"
+#include "internal.h"
+#include "setup.h"
+
+using namespace qbg;
+
+namespace {
+
+// scalar slots of the device result buffer
+enum {
+  S_LDA = 0,      // 4: sum rho e, sum rho (e-v), sum rho, -
+  S_VHXC = 4,     // 18: see k_vhxc_g
+  S_G0 = 22,      // 1: Re(conj(rhoelg[0]) vion[0])
+  S_KIN = 24,     // 8: ekin, sigma_ekin[6], -
+  S_ENL = 32,     // 1
+  S_COUNT = 40
+};
+
+template <typename T>
+int dalloc(qbg_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess)
+    return c->fail(QBG_E_NOMEM, "hipMalloc failed (energy functional)");
+  return QBG_OK;
+}
+
+template <typename T>
+int dput(qbg_ctx* c, T** p, const std::vector<T>& h) {
+  int rc = dalloc(c, p, h.size());
+  if (rc) return rc;
+  if (!h.empty() && hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return c->fail(QBG_E_HIP, "hipMemcpy failed (energy functional)");
+  return QBG_OK;
+}
+
+}  // namespace
+
+struct qbg_ef {
+  qbg_ctx* ctx = nullptr;
+  host::Cell cell;
+  double ecut = 0, omega = 0;
+  std::vector<host::Species> sp;
+  std::vector<int> na, sp_first;
+  int nsp = 0, nat = 0, nst = 0;
+  std::vector<double> tau, occ;
+  host::Basis wb, vb;
+  host::Maps wm, vm;
+  int np[3] = {0, 0, 0};
+  long N = 0;
+  qbg_grid* gw = nullptr;
+  qbg_grid* gv = nullptr;
+  int kmin[3] = {0, 0, 0}, krange = 0;
+  // device: bases and tables
+  double *d_g2w = nullptr, *d_gxw = nullptr;
+  int32_t* d_idxv = nullptr;
+  double *d_g2iv = nullptr, *d_giv = nullptr, *d_gxv = nullptr;
+  double *d_vps = nullptr, *d_dvps = nullptr, *d_rhops = nullptr, *d_twnl = nullptr;
+  double *d_tau = nullptr, *d_b = nullptr, *d_sftab = nullptr, *d_sfac = nullptr;
+  double *d_vion = nullptr, *d_dvion = nullptr, *d_rhopst = nullptr;
+  int32_t *d_sp_first = nullptr, *d_atom_sp = nullptr;
+  // device: state
+  double *d_rhor = nullptr, *d_vxc = nullptr, *d_vr = nullptr, *d_rhog = nullptr;
+  double *d_rhoelg = nullptr, *d_rhogt = nullptr, *d_vlocal = nullptr, *d_occ = nullptr;
+  double *d_fesr = nullptr, *d_fion = nullptr, *d_scal = nullptr, *d_partial = nullptr;
+  size_t partial_doubles = 0;
+  // nonlocal
+  int M = 0;
+  int32_t *d_m_atom = nullptr, *d_m_twnl = nullptr, *d_m_l = nullptr, *d_m_ipr = nullptr,
+          *d_m_npr = nullptr, *d_m_stride = nullptr, *d_m_dbase = nullptr;
+  double *d_m_wt = nullptr, *d_dmat = nullptr;
+  int32_t *d_a_m0 = nullptr, *d_a_npr = nullptr, *d_a_stride = nullptr;
+  double *d_anl = nullptr, *d_fnl = nullptr, *d_fout = nullptr, *d_work = nullptr;
+  size_t work_doubles = 0;
+  // host results of atoms_moved
+  double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
+  std::vector<double> fesr;
+  bool have_rho = false, have_vhxc = false, vhxc_stress = false;
+  std::vector<void*> allocs;
+};
+
+static void ef_free(qbg_ef* e) {
+  void* p[] = {e->d_g2w, e->d_gxw, e->d_idxv, e->d_g2iv, e->d_giv, e->d_gxv, e->d_vps, e->d_dvps,
+               e->d_rhops, e->d_twnl, e->d_tau, e->d_b, e->d_sftab, e->d_sfac, e->d_vion,
+               e->d_dvion, e->d_rhopst, e->d_sp_first, e->d_atom_sp, e->d_rhor, e->d_vxc,
+               e->d_vr, e->d_rhog, e->d_rhoelg, e->d_rhogt, e->d_vlocal, e->d_occ, e->d_fesr,
+               e->d_fion, e->d_scal, e->d_partial, e->d_m_atom, e->d_m_twnl, e->d_m_l,
+               e->d_m_ipr, e->d_m_npr, e->d_m_stride, e->d_m_dbase, e->d_m_wt, e->d_dmat,
+               e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  if (e->gw) qbg_grid_destroy(e->gw);
+  if (e->gv) qbg_grid_destroy(e->gv);
+}
+
+// esr, fion_esr, sigma_esr: real-space erfc lattice sums over pairs of ions
+// (src/EnergyFunctional.cpp:1097-1192); O(nat^2) host work per geometry.
+static void ef_esr(qbg_ef* e) {
+  e->esr = 0.0;
+  for (double& x : e->sigma_esr) x = 0.0;
+  e->fesr.assign(3 * e->nat, 0.0);
+  double d[3];
+  for (int i = 0; i < 3; i++) {
+    const double* b = e->cell.b[i];
+    d[i] = 2.0 * M_PI / std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  }
+  const double* a = &e->cell.a[0][0];
+  for (int s1 = 0; s1 < e->nsp; s1++)
+    for (int s2 = s1; s2 < e->nsp; s2++) {
+      const double rc = std::sqrt(e->sp[s1].rcps * e->sp[s1].rcps + e->sp[s2].rcps * e->sp[s2].rcps);
+      int nc[3];
+      for (int i = 0; i < 3; i++) nc[i] = (int)(8.0 * rc / d[i]);
+      const double zz = (double)e->sp[s1].zval * e->sp[s2].zval;
+      for (int a1 = e->sp_first[s1]; a1 < e->sp_first[s1 + 1]; a1++)
+        for (int a2 = (s1 == s2 ? a1 : e->sp_first[s2]); a2 < e->sp_first[s2 + 1]; a2++) {
+          const bool same = a1 == a2;
+          double v[3];
+          for (int j = 0; j < 3; j++) v[j] = e->tau[3 * a1 + j] - e->tau[3 * a2 + j];
+          e->cell.fold_in_ws(v);
+          for (int i0 = -nc[0]; i0 <= nc[0]; i0++)
+            for (int i1 = -nc[1]; i1 <= nc[1]; i1++)
+              for (int i2 = -nc[2]; i2 <= nc[2]; i2++) {
+                if (same && i0 == 0 && i1 == 0 && i2 == 0) continue;
+                const double fac = same ? 0.5 : 1.0;
+                double x[3];
+                for (int j = 0; j < 3; j++)
+                  x[j] = v[j] + ((i0 * a[j] + i1 * a[3 + j]) + i2 * a[6 + j]);
+                const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const double arg = r / rc;
+                const double term = zz * std::erfc(arg) / r;
+                e->esr += fac * term;
+                const double derfc = 2.0 * zz * std::exp(-arg * arg) / (rc * std::sqrt(M_PI));
+                const double dr = -fac * (term + derfc) / (r * r);
+                for (int j = 0; j < 3; j++) {
+                  e->fesr[3 * a1 + j] -= dr * x[j];
+                  e->fesr[3 * a2 + j] += dr * x[j];
+                }
+                e->sigma_esr[0] += dr * x[0] * x[0];
+                e->sigma_esr[1] += dr * x[1] * x[1];
+                e->sigma_esr[2] += dr * x[2] * x[2];
+                e->sigma_esr[3] += dr * x[0] * x[1];
+                e->sigma_esr[4] += dr * x[1] * x[2];
+                e->sigma_esr[5] += dr * x[0] * x[2];
+              }
+        }
+    }
+  for (double& x : e->sigma_esr) x *= -1.0 / e->omega;
+}
+
+static int ef_atoms_moved(qbg_ef* e) {
+  qbg_ctx* c = e->ctx;
+  hipStream_t s = c->stream;
+  QBG_HIP(c, hipMemcpyAsync(e->d_tau, e->tau.data(), sizeof(double) * e->tau.size(),
+                            hipMemcpyHostToDevice, s));
+  c->prof_begin(QBG_K_OTHER);
+  launch_sf_tables(e->d_tau, e->nat, e->d_b, e->kmin, e->krange, e->d_sftab, s);
+  launch_sfac(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->nsp, e->d_sp_first,
+              e->d_vps, e->d_dvps, e->d_rhops, e->d_sfac, e->d_vion, e->d_dvion, e->d_rhopst, s);
+  c->prof_end();
+  ef_esr(e);
+  QBG_HIP(c, hipMemcpyAsync(e->d_fesr, e->fesr.data(), sizeof(double) * e->fesr.size(),
+                            hipMemcpyHostToDevice, s));
+  QBG_HIP(c, hipStreamSynchronize(s));
+  e->have_vhxc = false;
+  return QBG_OK;
+}
+
+extern "C" {
+
+int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
+                  const qbg_species_desc* spd, const int* na, const double* tau, int nst,
+                  const double* occ, qbg_ef** out) {
+  if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
+    return QBG_E_INVAL;
+  *out = nullptr;
+  (void)hipSetDevice(c->device);
+  qbg_ef* e = new (std::nothrow) qbg_ef;
+  if (!e) return QBG_E_NOMEM;
+  e->ctx = c;
+  e->cell.set(cell9);
+  e->omega = e->cell.volume;
+  e->ecut = ecut;
+  e->nsp = nsp;
+  e->nst = nst;
+  e->occ.assign(occ, occ + nst);
+  e->sp_first.assign(1, 0);
+  for (int is = 0; is < nsp; is++) {
+    host::Species q;
+    q.zval = spd[is].zval; q.lmax = spd[is].lmax; q.nmesh = spd[is].nmesh;
+    q.deltar = spd[is].deltar; q.rcps = spd[is].rcps;
+    q.vloc.assign(spd[is].vloc, spd[is].vloc + q.nmesh);
+    q.proj.assign(spd[is].proj, spd[is].proj + (size_t)(q.lmax + 1) * q.nmesh);
+    q.dl.assign(spd[is].dl, spd[is].dl + q.lmax + 1);
+    if (!q.init()) { delete e; return c->fail(QBG_E_INVAL, "species: unsupported definition"); }
+    e->sp.push_back(q);
+    e->na.push_back(na[is]);
+    e->sp_first.push_back(e->sp_first.back() + na[is]);
+  }
+  e->nat = e->sp_first.back();
+  e->tau.assign(tau, tau + 3 * e->nat);
+  for (int is = 0; is < nsp; is++) e->eself += e->na[is] * e->sp[is].eself();
+  // bases and grid (src/ChargeDensity.cpp:34-98)
+  e->wb.build(e->cell, e->cell, ecut);
+  e->vb.build(e->cell, e->cell, 4.0 * ecut);
+  for (int i = 0; i < 3; i++) {
+    int n = e->vb.np[i] + 2;
+    while (!host::factorizable(n)) n += 2;
+    e->np[i] = n;
+  }
+  e->N = (long)e->np[0] * e->np[1] * e->np[2];
+  if (!e->wm.build(e->wb, e->np[0], e->np[1], e->np[2]) ||
+      !e->vm.build(e->vb, e->np[0], e->np[1], e->np[2])) {
+    delete e;
+    return c->fail(QBG_E_INVAL, "basis does not fit in the grid");
+  }
+  int rc;
+  if ((rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->wb.size, e->wm.nvec,
+                            e->wm.ip.data(), e->wm.im.data(), e->wm.zvec_to_val.data(), &e->gw)) ||
+      (rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->vb.size, e->vm.nvec,
+                            e->vm.ip.data(), e->vm.im.data(), e->vm.zvec_to_val.data(), &e->gv))) {
+    ef_free(e);
+    delete e;
+    return rc;
+  }
+  const int ngw = e->wb.size, ngv = e->vb.size;
+  // cell_moved tables (src/EnergyFunctional.cpp:1200-1245)
+  std::vector<double> vps((size_t)nsp * ngv), dvps((size_t)nsp * ngv), rhops((size_t)nsp * ngv);
+  for (int is = 0; is < nsp; is++)
+    for (int ig = 0; ig < ngv; ig++) {
+      double v, dv;
+      const double oi = 1.0 / e->omega, g = e->vb.g[ig];
+      rhops[(size_t)is * ngv + ig] = e->sp[is].rhopsg(g) * oi;
+      e->sp[is].dvlocg(g, v, dv);
+      vps[(size_t)is * ngv + ig] = v * oi;
+      dvps[(size_t)is * ngv + ig] = dv * oi;
+    }
+  // projectors twnl (src/NonLocalPotential.cpp:235-520, KB l = 0, 1)
+  std::vector<double> twnl;
+  std::vector<int> twnl_off(nsp);
+  std::vector<int32_t> m_atom, m_twnl, m_l, m_ipr, m_npr, m_stride, m_dbase, a_m0(e->nat),
+      a_npr(e->nat), a_stride(e->nat);
+  std::vector<double> m_wt, dmat;
+  const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
+  for (int is = 0; is < nsp; is++) {
+    const host::Species& q = e->sp[is];
+    const int npr = q.nlm;
+    twnl_off[is] = (int)twnl.size();
+    twnl.resize(twnl.size() + (size_t)npr * ngw);
+    double* t = &twnl[twnl_off[is]];
+    std::vector<int> lp(npr);
+    for (int ig = 0; ig < ngw; ig++) {
+      double v, dv;
+      q.dvnlg(0, e->wb.g[ig], v, dv);
+      t[ig] = s14pi * v;
+      if (q.lmax >= 1) {
+        q.dvnlg(1, e->wb.g[ig], v, dv);
+        const double gi = e->wb.gi[ig];
+        for (int j = 0; j < 3; j++) t[(size_t)(1 + j) * ngw + ig] = s34pi * e->wb.gx[j * ngw + ig] * gi * v;
+      }
+    }
+    lp[0] = 0;
+    for (int j = 1; j < npr; j++) lp[j] = 1;
+    // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
+    const int dbase = (int)dmat.size();
+    for (int i = 0; i < npr; i++)
+      for (int j = 0; j < npr; j++) dmat.push_back(i == j ? q.dl[lp[i]] : 0.0);
+    const int Ms = (int)m_atom.size();
+    for (int ipr = 0; ipr < npr; ipr++)
+      for (int ia = 0; ia < e->na[is]; ia++) {
+        const int a = e->sp_first[is] + ia;
+        m_atom.push_back(a);
+        m_twnl.push_back(twnl_off[is] + ipr * ngw);
+        m_l.push_back(lp[ipr]);
+        m_ipr.push_back(ipr);
+        m_npr.push_back(npr);
+        m_stride.push_back(e->na[is]);
+        m_dbase.push_back(dbase);
+        m_wt.push_back(1.0);   // wsg = 1 for SLPP (src/Species.cpp:733)
+      }
+    for (int ia = 0; ia < e->na[is]; ia++) {
+      const int a = e->sp_first[is] + ia;
+      a_m0[a] = Ms + ia;
+      a_npr[a] = npr;
+      a_stride[a] = e->na[is];
+    }
+  }
+  e->M = (int)m_atom.size();
+  // structure factor table extents (src/StructureFactor.cpp:27-39)
+  int krange = 0;
+  for (int i = 0; i < 3; i++) {
+    e->kmin[i] = e->vb.idxmin[i];
+    krange = std::max(krange, e->vb.idxmax[i] - e->vb.idxmin[i] + 1);
+  }
+  e->krange = krange;
+  std::vector<double> bm(9);
+  for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
+  std::vector<int32_t> atom_sp(e->nat), spf(e->sp_first.begin(), e->sp_first.end());
+  for (int is = 0; is < nsp; is++)
+    for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a++) atom_sp[a] = is;
+  std::vector<double> g2iv(e->vb.g2i), giv(e->vb.gi);
+  const long N = e->N;
+  const size_t nbmax = (size_t)std::max<long>((N + 255) / 256, (ngv + 255) / 256);
+  e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
+  e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
+  e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)((ngw + 255) / 256) * 8);
+  // split-K partials: at most ~2048 tiles of 64x64 plus one spare split
+  e->work_doubles = (size_t)2048 * 4096 + (size_t)2 * 4 * e->M * nst;
+  if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
+      (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
+      (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
+      (rc = dput(c, &e->d_vps, vps)) || (rc = dput(c, &e->d_dvps, dvps)) ||
+      (rc = dput(c, &e->d_rhops, rhops)) || (rc = dput(c, &e->d_twnl, twnl)) ||
+      (rc = dput(c, &e->d_b, bm)) || (rc = dput(c, &e->d_sp_first, spf)) ||
+      (rc = dput(c, &e->d_atom_sp, atom_sp)) || (rc = dput(c, &e->d_occ, e->occ)) ||
+      (rc = dput(c, &e->d_m_atom, m_atom)) || (rc = dput(c, &e->d_m_twnl, m_twnl)) ||
+      (rc = dput(c, &e->d_m_l, m_l)) || (rc = dput(c, &e->d_m_ipr, m_ipr)) ||
+      (rc = dput(c, &e->d_m_npr, m_npr)) || (rc = dput(c, &e->d_m_stride, m_stride)) ||
+      (rc = dput(c, &e->d_m_dbase, m_dbase)) || (rc = dput(c, &e->d_m_wt, m_wt)) ||
+      (rc = dput(c, &e->d_dmat, dmat)) || (rc = dput(c, &e->d_a_m0, a_m0)) ||
+      (rc = dput(c, &e->d_a_npr, a_npr)) || (rc = dput(c, &e->d_a_stride, a_stride)) ||
+      (rc = dalloc(c, &e->d_tau, 3 * e->nat)) ||
+      (rc = dalloc(c, &e->d_sftab, (size_t)e->nat * 3 * krange * 2)) ||
+      (rc = dalloc(c, &e->d_sfac, (size_t)2 * nsp * ngv)) ||
+      (rc = dalloc(c, &e->d_vion, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_dvion, (size_t)2 * ngv)) ||
+      (rc = dalloc(c, &e->d_rhopst, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhor, N)) ||
+      (rc = dalloc(c, &e->d_vxc, N)) || (rc = dalloc(c, &e->d_vr, N)) ||
+      (rc = dalloc(c, &e->d_rhog, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhoelg, (size_t)2 * ngv)) ||
+      (rc = dalloc(c, &e->d_rhogt, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_vlocal, (size_t)2 * ngv)) ||
+      (rc = dalloc(c, &e->d_fesr, 3 * e->nat)) || (rc = dalloc(c, &e->d_fion, 3 * e->nat)) ||
+      (rc = dalloc(c, &e->d_scal, S_COUNT)) || (rc = dalloc(c, &e->d_partial, e->partial_doubles)) ||
+      (rc = dalloc(c, &e->d_anl, (size_t)4 * e->M * 2 * ngw)) ||
+      (rc = dalloc(c, &e->d_fnl, (size_t)4 * e->M * nst)) ||
+      (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
+      (rc = dalloc(c, &e->d_work, e->work_doubles))) {
+    ef_free(e);
+    delete e;
+    return rc;
+  }
+  if ((rc = ef_atoms_moved(e))) {
+    ef_free(e);
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return QBG_OK;
+}
+
+int qbg_ef_destroy(qbg_ef* e) {
+  if (!e) return QBG_E_INVAL;
+  (void)hipSetDevice(e->ctx->device);
+  (void)hipStreamSynchronize(e->ctx->stream);
+  ef_free(e);
+  delete e;
+  return QBG_OK;
+}
+
+// out: ngw, ngv, np0, np1, np2, nvec_w, nvec_v, nat, M, nst
+int qbg_ef_sizes(const qbg_ef* e, int* out) {
+  if (!e || !out) return QBG_E_INVAL;
+  const int v[10] = {e->wb.size, e->vb.size, e->np[0], e->np[1], e->np[2], e->wm.nvec,
+                     e->vm.nvec, e->nat, e->M, e->nst};
+  std::memcpy(out, v, sizeof(v));
+  return QBG_OK;
+}
+
+// host copies of a basis (which 0: wavefunctions, 1: density):
+// idx[3*ng], g2[ng], gx[3*ng] (each may be null), and the maps
+int qbg_ef_basis(const qbg_ef* e, int which, int32_t* idx, double* g2, double* gx, int32_t* ip,
+                 int32_t* im, int32_t* zvec_to_val) {
+  if (!e) return QBG_E_INVAL;
+  const host::Basis& b = which ? e->vb : e->wb;
+  const host::Maps& m = which ? e->vm : e->wm;
+  if (idx) std::memcpy(idx, b.idx.data(), sizeof(int32_t) * b.idx.size());
+  if (g2) std::memcpy(g2, b.g2.data(), sizeof(double) * b.size);
+  if (gx) std::memcpy(gx, b.gx.data(), sizeof(double) * b.gx.size());
+  if (ip) std::memcpy(ip, m.ip.data(), sizeof(int32_t) * m.ip.size());
+  if (im) std::memcpy(im, m.im.data(), sizeof(int32_t) * m.im.size());
+  if (zvec_to_val) std::memcpy(zvec_to_val, m.zvec_to_val.data(), sizeof(int32_t) * m.nvec);
+  return QBG_OK;
+}
+
+int qbg_ef_set_positions(qbg_ef* e, const double* tau) {
+  if (!e || !tau) return QBG_E_INVAL;
+  (void)hipSetDevice(e->ctx->device);
+  e->tau.assign(tau, tau + 3 * e->nat);
+  return ef_atoms_moved(e);
+}
+
+// ChargeDensity::update_density (src/ChargeDensity.cpp:136-198):
+// rhor = sum_n occ_n/omega |psi_n|^2; rhog = vft.forward(omega * rhor)
+int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
+  if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
+  qbg_ctx* ctx = e->ctx;
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  QBG_HIP(ctx, hipMemsetAsync(e->d_rhor, 0, sizeof(double) * e->N, s));
+  int rc = qbg_compute_density(e->gw, c, ldc, e->nst, e->occ.data(), 1.0, e->omega, e->d_rhor, prec);
+  if (rc) return rc;
+  qbg_grid* g = e->gv;
+  if ((rc = grid_ensure_ws(g, 1))) return rc;
+  int r1, r2, r3;
+  ctx->prof_begin(QBG_K_YOTHER);
+  r1 = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, e->d_rhor, nullptr, nullptr, nullptr, e->omega, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_XFWD);
+  r2 = r1 ? r1 : xfwd(g, QBG_PREC_FP64, 1, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_ZFWD);
+  r3 = r2 ? r2 : zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, e->d_rhog, 0, 0, 0, 1, s);
+  ctx->prof_end();
+  if (r3) return ctx->fail(r3, "update_density: density-grid transform refused");
+  e->have_rho = true;
+  e->have_vhxc = false;
+  hipError_t er = hipGetLastError();
+  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+  return QBG_OK;
+}
+
+// EnergyFunctional::update_vhxc (src/EnergyFunctional.cpp:231-394), LDA
+int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
+  if (!e) return QBG_E_INVAL;
+  qbg_ctx* ctx = e->ctx;
+  if (!e->have_rho) return ctx->fail(QBG_E_STATE, "update_vhxc before update_density");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  ctx->prof_begin(QBG_K_OTHER);
+  launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
+  launch_vhxc_g(e->d_rhog, e->vb.size, 1.0 / e->omega, e->d_vion, e->d_dvion, e->d_rhopst,
+                e->d_g2iv, e->d_giv, e->d_gxv, e->d_rhoelg, e->d_rhogt, e->d_vlocal,
+                compute_stress, e->d_partial, e->d_scal + S_VHXC, s);
+  ctx->prof_end();
+  qbg_grid* g = e->gv;
+  int rc = grid_ensure_ws(g, 1);
+  if (rc) return rc;
+  int r1, r2, r3;
+  ctx->prof_begin(QBG_K_ZBWD);
+  r1 = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_vlocal, 0, 0, 0, 1, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_XBWD);
+  r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_YOTHER);
+  r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr, e->d_vr,
+                       1.0, s);
+  ctx->prof_end();
+  if (r3) return ctx->fail(r3, "update_vhxc: density-grid transform refused");
+  e->have_vhxc = true;
+  e->vhxc_stress = compute_stress != 0;
+  hipError_t er = hipGetLastError();
+  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+  return QBG_OK;
+}
+
+// EnergyFunctional::energy (src/EnergyFunctional.cpp:397-1044): nspin 1,
+// Gamma, LDA.  energies (host, 14): ekin eps enl ehart ehart_e ehart_ep
+// ehart_p exc dxc esr eself ecoul etotal total_charge; fion (host, 3*nat)
+// species-major; sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr).
+int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
+                  int compute_forces, int compute_stress, int prec, double* energies,
+                  double* fion, double* sigma) {
+  if (!e || !c || !energies || ldc < e->wb.size) return QBG_E_INVAL;
+  qbg_ctx* ctx = e->ctx;
+  if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
+  if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
+  if (compute_stress) return ctx->fail(QBG_E_INVAL, "stress: nonlocal stress not available");
+  if (prec != QBG_PREC_FP64) return ctx->fail(QBG_E_INVAL, "energy: only FP64 projectors so far");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  const int ngw = e->wb.size, nst = e->nst, M = e->M;
+  // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf)
+  ctx->prof_begin(QBG_K_OTHER);
+  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
+                 e->d_partial, e->d_scal + S_KIN, s);
+  ctx->prof_end();
+  // nonlocal: anl (+ three -i G_j anl blocks), fnl = 2 anl^T c - anl0 c0^T
+  const int nmodes = compute_forces ? 4 : 1;
+  const long K = 2L * ngw;
+  ctx->prof_begin(QBG_K_NL);
+  launch_anl(ngw, M, nmodes, K, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l,
+             e->d_twnl, e->d_anl, 0, s);
+  if (gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M, e->d_work,
+                     e->work_doubles, s))
+    return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
+  if (compute_forces &&
+      gemm_tn_splitk(e->d_anl + (size_t)M * K, K, c, 2L * ldc, 3 * M, nst, K, 2.0, 0.0,
+                     e->d_fnl + (size_t)M * nst, 3 * M, e->d_work, e->work_doubles, s))
+    return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
+  launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
+                  e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
+                  e->d_scal + S_ENL, s);
+  if (compute_hpsi) gemm_nn(e->d_anl, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, s);
+  if (compute_forces) {
+    QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
+    // dF_j rows of the 3M x nst block are [j*M + m]; pass each j block
+    launch_nl_forces(e->d_fout, e->d_fnl + (size_t)M * nst, M, nst, e->d_occ, e->d_a_m0,
+                     e->d_a_npr, e->d_a_stride, e->nat, e->d_fion, s);
+  }
+  ctx->prof_end();
+  // local potential: hpsi += FFT(v_r * FFT^-1 c)
+  if (compute_hpsi) {
+    int rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, prec);
+    if (rc) return rc;
+  }
+  if (compute_forces) {
+    ctx->prof_begin(QBG_K_OTHER);
+    const int nchunk = 64;
+    launch_local_forces(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->d_atom_sp,
+                        e->nat, e->d_vps, e->d_rhops, e->d_g2iv, e->d_gxv, e->d_rhogt,
+                        e->d_rhoelg, e->omega, e->d_fesr, e->d_partial, nchunk, e->d_fion, s);
+    ctx->prof_end();
+  }
+  // scalars back to the host
+  double sc[S_COUNT];
+  QBG_HIP(ctx, hipMemcpyAsync(sc, e->d_scal, sizeof(sc), hipMemcpyDeviceToHost, s));
+  double g0[4];
+  QBG_HIP(ctx, hipMemcpyAsync(g0, e->d_rhoelg, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  QBG_HIP(ctx, hipMemcpyAsync(g0 + 2, e->d_vion, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (compute_forces && fion)
+    QBG_HIP(ctx, hipMemcpyAsync(fion, e->d_fion, sizeof(double) * 3 * e->nat,
+                                hipMemcpyDeviceToHost, s));
+  QBG_HIP(ctx, hipStreamSynchronize(s));
+  const double omega = e->omega, fpi = 4.0 * M_PI, Nd = (double)e->N;
+  const double exc = sc[S_LDA + 0] * omega / Nd, dxc = sc[S_LDA + 1] * omega / Nd;
+  const double charge = sc[S_LDA + 2] * omega / Nd;
+  const double eps = (2.0 * sc[S_VHXC + 0] - (g0[0] * g0[2] + g0[1] * g0[3])) * omega;
+  const double ehart = omega * fpi * sc[S_VHXC + 1];
+  const double ehart_e = omega * fpi * sc[S_VHXC + 2];
+  const double ehart_ep = omega * fpi * sc[S_VHXC + 3];
+  const double ehart_p = omega * fpi * sc[S_VHXC + 4];
+  const double ekin = sc[S_KIN], enl = sc[S_ENL];
+  const double ecoul = ehart + e->esr - e->eself;
+  const double etotal = ekin + eps + enl + ecoul + exc;
+  const double en[14] = {ekin, eps, enl, ehart, ehart_e, ehart_ep, ehart_p, exc, dxc,
+                         e->esr, e->eself, ecoul, etotal, charge};
+  std::memcpy(energies, en, sizeof(en));
+  if (sigma) std::memset(sigma, 0, sizeof(double) * 42);
+  return QBG_OK;
+}
+
+// device arrays for tests: 0 rhor (N), 1 v_r (N), 2 vxc_r (N), 3 rhog
+// (2*ngv), 4 vion_local_g, 5 rhopst, 6 sfac (2*nsp*ngv), 7 fion_esr (3 nat)
+int qbg_ef_get(qbg_ef* e, int what, double* host) {
+  if (!e || !host) return QBG_E_INVAL;
+  const int ngv = e->vb.size;
+  const double* src = nullptr;
+  size_t n = 0;
+  switch (what) {
+    case 0: src = e->d_rhor; n = e->N; break;
+    case 1: src = e->d_vr; n = e->N; break;
+    case 2: src = e->d_vxc; n = e->N; break;
+    case 3: src = e->d_rhog; n = 2 * (size_t)ngv; break;
+    case 4: src = e->d_vion; n = 2 * (size_t)ngv; break;
+    case 5: src = e->d_rhopst; n = 2 * (size_t)ngv; break;
+    case 6: src = e->d_sfac; n = 2 * (size_t)e->nsp * ngv; break;
+    case 7: std::memcpy(host, e->fesr.data(), sizeof(double) * e->fesr.size()); return QBG_OK;
+    default: return QBG_E_INVAL;
+  }
+  QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                 e->ctx->stream));
+  QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+  return QBG_OK;
+}
+
+// the potential v_r used by H*psi may be replaced (tests of rs_mul_add)
+int qbg_ef_set_vr(qbg_ef* e, const double* host_v) {
+  if (!e || !host_v) return QBG_E_INVAL;
+  QBG_HIP(e->ctx, hipMemcpyAsync(e->d_vr, host_v, sizeof(double) * e->N, hipMemcpyHostToDevice,
+                                 e->ctx->stream));
+  QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+  return QBG_OK;
+}
+
+// SlaterDet::gram (src/SlaterDet.cpp:480-534), real basis:
+// cleanup; S = 2 P^T P - p0 p0^T; S = L L^T; C <- C L^-T
+int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
+  if (!c || !cc || ngw <= 0 || nst <= 0 || ldc < ngw) return QBG_E_INVAL;
+  if (prec != QBG_PREC_FP64) return c->fail(QBG_E_INVAL, "gram: FP64 only");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const long K = 2L * ngw;
+  const int tm = (nst + 63) / 64;
+  const size_t work = (size_t)nst * nst * std::max(1, 2048 / (tm * tm) + 1);
+  double *S = nullptr, *X = nullptr, *T = nullptr, *W = nullptr;
+  int* info = nullptr;
+  int rc = QBG_OK;
+  if ((rc = dalloc(c, &S, (size_t)nst * nst)) || (rc = dalloc(c, &X, (size_t)nst * nst)) ||
+      (rc = dalloc(c, &T, (size_t)K * nst)) || (rc = dalloc(c, &W, work)) ||
+      (rc = dalloc(c, &info, 1))) {
+    (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
+    return rc;
+  }
+  auto release = [&]() {
+    (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
+  };
+  c->prof_begin(QBG_K_GRAM);
+  cleanup(cc, ldc, nst, s);
+  if (gemm_tn_splitk(cc, 2L * ldc, cc, 2L * ldc, nst, nst, K, 2.0, -1.0, S, nst, W, work, s)) {
+    (void)hipStreamSynchronize(s);
+    release();
+    return c->fail(QBG_E_INVAL, "gram workspace");
+  }
+  potrf(S, nst, nst, info, s);
+  c->prof_end();
+  // the coefficients are only touched once the factorization succeeded
+  // (the reference aborts in potrf, src/Matrix.cpp:2324-2327)
+  int hinfo = 0;
+  (void)hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
+  hipError_t er = hipStreamSynchronize(s);
+  if (er != hipSuccess || hinfo) {
+    release();
+    if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+    return c->fail(QBG_E_NOT_SPD, "gram: overlap matrix not positive definite");
+  }
+  c->prof_begin(QBG_K_GRAM);
+  trtri(S, nst, nst, X, s);
+  gemm_nn(cc, 2L * ldc, X, nst, K, nst, nst, T, K, 0, s);
+  (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
+                         nst, hipMemcpyDeviceToDevice, s);
+  c->prof_end();
+  er = hipStreamSynchronize(s);
+  release();
+  if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// energy.h -- launchers of energy.hip and gemm.hip (stream-ordered).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace qbg {
+
+void reduce_partials(const double* partial, int nblocks, int nv, double* out, hipStream_t s);
+
+// returns the number of partial blocks used (partial needs nb*8 doubles)
+int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ngw,
+                   const double* g2, const double* gx, double* hpsi, long ldh, int store,
+                   double* partial, double* out8, hipStream_t s);
+int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
+               hipStream_t s);
+void launch_lda_points(const double* rho, int n, double* e, double* v, hipStream_t s);
+int launch_vhxc_g(const double* rhog, int ngv, double omega_inv, const double* vion,
+                  const double* dvion, const double* rhopst, const double* g2i, const double* gi,
+                  const double* gx, double* rhoelg, double* rhogt, double* vlocal, int stress,
+                  double* partial, double* out18, hipStream_t s);
+void launch_sf_tables(const double* tau, int nat, const double* b, const int* kmin, int krange,
+                      double* tab, hipStream_t s);
+void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, const int* kmin,
+                 int nsp, const int32_t* sp_first, const double* vps, const double* dvps,
+                 const double* rhops, double* sfac, double* vion, double* dvion, double* rhopst,
+                 hipStream_t s);
+void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int krange,
+                         const int* kmin, const int32_t* atom_sp, int nat, const double* vps,
+                         const double* rhops, const double* g2i, const double* gx,
+                         const double* rhogt, const double* rhoelg, double omega,
+                         const double* fesr, double* partial, int nchunk, double* fion,
+                         hipStream_t s);
+void launch_anl(int ngw, int M, int nmodes, long lda, const double* gx, const double* tau,
+                const int32_t* m_atom, const int32_t* m_twnl, const int32_t* m_l,
+                const double* twnl, void* A, int prec, hipStream_t s);
+int launch_dmat_enl(const double* F, int M, int nst, const double* occ, const int32_t* m_ipr,
+                    const int32_t* m_npr, const int32_t* m_stride, const int32_t* m_dbase,
+                    const double* dmat, const double* m_wt, double omega_inv, double* fout,
+                    double* partial, double* out1, hipStream_t s);
+void launch_nl_forces(const double* fout, const double* dF, int M, int nst, const double* occ,
+                      const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                      int nat, double* fion, hipStream_t s);
+
+// gemm.hip
+int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
+                   double alpha, double gamma, double* out, long ldo, double* work,
+                   size_t work_doubles, hipStream_t s);
+void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
+             double* C, long ldc, int accumulate, hipStream_t s);
+void potrf(double* S, int n, int lds, int* info, hipStream_t s);
+void trtri(const double* L, int n, int ldl, double* X, hipStream_t s);
+void cleanup(double* c, long ldc, int nst, hipStream_t s);
+
+}  // namespace qbg

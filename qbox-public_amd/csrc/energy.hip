@@ -1,0 +1,485 @@
+// energy.hip -- pointwise, reduction and structure-factor kernels of the
+// energy evaluation (EnergyFunctional::update_vhxc / energy / atoms_moved,
+// ChargeDensity::update_density, NonLocalPotential projector set-up).
+//
+// Every sum is a fixed-order tree: a block reduces 256 values in LDS and
+// writes one partial per block; a second kernel adds the partials in block
+// order.  Results are therefore bit-reproducible run to run.
+#include <hip/hip_runtime.h>
+#include "energy.h"
+
+namespace qbg {
+
+// --------------------------------------------------------------- reduction
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ out) {
+  __shared__ double red[NV][256];
+#pragma unroll
+  for (int q = 0; q < NV; q++) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int q = 0; q < NV; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NV) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[q] = sum_b partial[b*nv + q] in block order; one block of 256 threads
+// per q-group, each thread summing a strided slice, then a tree.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial,
+                                                         int nblocks, int nv,
+                                                         double* __restrict__ out) {
+  __shared__ double red[256];
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) s += partial[(long)b * nv + q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = red[0];
+}
+
+void reduce_partials(const double* partial, int nblocks, int nv, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(256), 0, s, partial, nblocks, nv, out);
+}
+
+// ------------------------------------------------------- kinetic + ekin
+// src/EnergyFunctional.cpp:441-571 (ekin, sigma_ekin) and :818-827
+// (cp += 0.5 kpg2 c); psi2sum(G) = sum_n occ_n |c_nG|^2 (fac = 1, real basis)
+// partial layout per block: [ekin, s_xx, s_yy, s_zz, s_xy, s_yz, s_xz, 0]
+__global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ c, long ldc, int nst,
+                                                 const double* __restrict__ occ, int ngw,
+                                                 const double* __restrict__ g2,
+                                                 const double* __restrict__ gx,
+                                                 double* __restrict__ hpsi, long ldh, int store,
+                                                 double* __restrict__ partial) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (ig < ngw) {
+    const double gg = g2[ig];
+    double p = 0.0;
+    for (int n = 0; n < nst; n++) {
+      const double re = c[2 * (ig + ldc * n)], im = c[2 * (ig + ldc * n) + 1];
+      p += occ[n] * (re * re + im * im);
+      if (store) {
+        hpsi[2 * (ig + ldh * n)] = 0.5 * gg * re;
+        hpsi[2 * (ig + ldh * n) + 1] = 0.5 * gg * im;
+      }
+    }
+    const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
+    const double f = 2.0 * p;
+    v[0] = p * gg;
+    v[1] = f * x * x; v[2] = f * y * y; v[3] = f * z * z;
+    v[4] = f * x * y; v[5] = f * y * z; v[6] = f * x * z;
+  }
+  block_reduce_store<8>(v, partial + (long)blockIdx.x * 8);
+}
+
+int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ngw,
+                   const double* g2, const double* gx, double* hpsi, long ldh, int store,
+                   double* partial, double* out8, hipStream_t s) {
+  const int nb = (ngw + 255) / 256;
+  hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, s, c, ldc, nst, occ, ngw, g2, gx, hpsi,
+                     ldh, store, partial);
+  reduce_partials(partial, nb, 8, out8, s);
+  return nb;
+}
+
+// ------------------------------------------------------------------ LDA
+// src/LDAFunctional.cpp:92-151 (Perdew-Zunger fit of Ceperley-Alder)
+__device__ __forceinline__ void lda_pz(double rh, double& ee, double& vv) {
+  const double c1 = 0.6203504908994001;
+  const double c3 = -0.610887057711;
+  const double A = 0.0311, B = -0.048, b1 = 1.0529, b2 = 0.3334, G = -0.1423;
+  const double D = G / (1.0 + b1 + b2) - B;
+  const double C = -A - D - G * ((b1 / 2.0 + b2) / ((1.0 + b1 + b2) * (1.0 + b1 + b2)));
+  ee = 0.0;
+  vv = 0.0;
+  if (rh > 0.0) {
+    const double rs = c1 / cbrt(rh);
+    const double vx = c3 / rs;
+    const double ex = 0.75 * vx;
+    double ec, vc;
+    if (rs < 1.0) {
+      const double lr = log(rs);
+      ec = A * lr + B + C * rs * lr + D * rs;
+      vc = A * lr + (B - A / 3.0) + (2.0 / 3.0) * C * rs * lr + ((2.0 * D - C) / 3.0) * rs;
+    } else {
+      const double sr = sqrt(rs);
+      const double den = 1.0 + b1 * sr + b2 * rs;
+      ec = G / den;
+      vc = ec * (1.0 + (7.0 / 6.0) * b1 * sr + (4.0 / 3.0) * b2 * rs) / den;
+    }
+    ee = ex + ec;
+    vv = vx + vc;
+  }
+}
+
+// vxc_r and partial sums [sum rho e, sum rho (e - v), sum rho, 0]
+// (src/XCPotential.cpp:193-229, src/ChargeDensity.cpp:178-187)
+__global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho, long n,
+                                             double* __restrict__ vxc,
+                                             double* __restrict__ partial) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  double v[4] = {0, 0, 0, 0};
+  if (i < n) {
+    double e, vv;
+    const double r = rho[i];
+    lda_pz(r, e, vv);
+    vxc[i] = vv;
+    v[0] = r * e;
+    v[1] = r * (e - vv);
+    v[2] = r;
+  }
+  block_reduce_store<4>(v, partial + (long)blockIdx.x * 4);
+}
+
+int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
+               hipStream_t s) {
+  const int nb = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_lda, dim3(nb), dim3(256), 0, s, rho, n, vxc, partial);
+  reduce_partials(partial, nb, 4, out4, s);
+  return nb;
+}
+
+// LDA pointwise values for a test vector (no reduction)
+__global__ void k_lda_points(const double* __restrict__ rho, int n, double* e, double* v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) lda_pz(rho[i], e[i], v[i]);
+}
+void launch_lda_points(const double* rho, int n, double* e, double* v, hipStream_t s) {
+  hipLaunchKernelGGL(k_lda_points, dim3((n + 255) / 256), dim3(256), 0, s, rho, n, e, v);
+}
+
+// ---------------------------------------------------------- vhxc in G
+// src/EnergyFunctional.cpp:243-383: rhoelg = rhog/omega; eps, Hartree sums;
+// rhogt = rhoelg + rhopst; vlocal_g = vion + 4 pi rhogt g2i.
+// partial per block: [sum Re(conj r vion), ehsum, ehesum, ehepsum, ehpsum,
+//                     6 x hartree-stress gg sums, 6 x eps-stress sums, 0]
+__global__ __launch_bounds__(256) void k_vhxc_g(
+    const double* __restrict__ rhog, int ngv, double omega_inv, const double* __restrict__ vion,
+    const double* __restrict__ dvion, const double* __restrict__ rhopst,
+    const double* __restrict__ g2i, const double* __restrict__ gi, const double* __restrict__ gx,
+    double* __restrict__ rhoelg, double* __restrict__ rhogt, double* __restrict__ vlocal,
+    int stress, double* __restrict__ partial) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  double v[18];
+#pragma unroll
+  for (int q = 0; q < 18; q++) v[q] = 0.0;
+  if (ig < ngv) {
+    const double rr = omega_inv * rhog[2 * ig], ri = omega_inv * rhog[2 * ig + 1];
+    rhoelg[2 * ig] = rr;
+    rhoelg[2 * ig + 1] = ri;
+    const double pr = rhopst[2 * ig], pi = rhopst[2 * ig + 1];
+    const double vr = vion[2 * ig], vi = vion[2 * ig + 1];
+    const double g2i_ = g2i[ig];
+    v[0] = rr * vr + ri * vi;
+    const double tr = rr + pr, ti = ri + pi;
+    v[1] = (tr * tr + ti * ti) * g2i_;
+    v[2] = (rr * rr + ri * ri) * g2i_;
+    v[3] = 2.0 * ((rr * pr + ri * pi) * g2i_);
+    v[4] = (pr * pr + pi * pi) * g2i_;
+    rhogt[2 * ig] = tr;
+    rhogt[2 * ig + 1] = ti;
+    const double fpi = 4.0 * M_PI;
+    vlocal[2 * ig] = vr + fpi * tr * g2i_;
+    vlocal[2 * ig + 1] = vi + fpi * ti * g2i_;
+    if (stress) {
+      const double x = gx[ig], y = gx[ngv + ig], z = gx[2 * ngv + ig];
+      // sigma_ehart gg part (:634-647): |rhogt|^2 g2i^2 G_i G_j
+      const double t = (tr * tr + ti * ti) * g2i_ * g2i_;
+      v[5] = t * x * x; v[6] = t * y * y; v[7] = t * z * z;
+      v[8] = t * x * y; v[9] = t * y * z; v[10] = t * x * z;
+      // sigma_eps (:580-597): 2 gi Re(conj(rhoelg) dvion) G_i G_j
+      const double f = 2.0 * gi[ig] * (rr * dvion[2 * ig] + ri * dvion[2 * ig + 1]);
+      v[11] = f * x * x; v[12] = f * y * y; v[13] = f * z * z;
+      v[14] = f * x * y; v[15] = f * y * z; v[16] = f * x * z;
+    }
+  }
+  block_reduce_store<18>(v, partial + (long)blockIdx.x * 18);
+}
+
+int launch_vhxc_g(const double* rhog, int ngv, double omega_inv, const double* vion,
+                  const double* dvion, const double* rhopst, const double* g2i, const double* gi,
+                  const double* gx, double* rhoelg, double* rhogt, double* vlocal, int stress,
+                  double* partial, double* out18, hipStream_t s) {
+  const int nb = (ngv + 255) / 256;
+  hipLaunchKernelGGL(k_vhxc_g, dim3(nb), dim3(256), 0, s, rhog, ngv, omega_inv, vion, dvion,
+                     rhopst, g2i, gi, gx, rhoelg, rhogt, vlocal, stress, partial);
+  reduce_partials(partial, nb, 18, out18, s);
+  return nb;
+}
+
+// ----------------------------------------------------- structure factor
+// src/StructureFactor.cpp:70-162: per atom and axis cos/sin(k * b_axis.tau)
+// for k in [kmin, kmax]; tables laid out tab[((a*3 + axis)*krange + k-kmin)*2]
+__global__ void k_sf_tables(const double* __restrict__ tau, int nat, const double* __restrict__ b,
+                            int kmin0, int kmin1, int kmin2, int krange,
+                            double* __restrict__ tab) {
+  const int a = blockIdx.x;
+  const int kmins[3] = {kmin0, kmin1, kmin2};
+  const double t0 = tau[3 * a], t1 = tau[3 * a + 1], t2 = tau[3 * a + 2];
+  for (int ax = 0; ax < 3; ax++) {
+    const double fac = b[3 * ax] * t0 + b[3 * ax + 1] * t1 + b[3 * ax + 2] * t2;
+    for (int k = threadIdx.x; k < krange; k += blockDim.x) {
+      const double arg = (double)(k + kmins[ax]) * fac;
+      double sn, cs;
+      sincos(arg, &sn, &cs);
+      double* p = tab + ((long)(a * 3 + ax) * krange + k) * 2;
+      p[0] = cs;
+      p[1] = sn;
+    }
+  }
+}
+
+void launch_sf_tables(const double* tau, int nat, const double* b, const int* kmin, int krange,
+                      double* tab, hipStream_t s) {
+  hipLaunchKernelGGL(k_sf_tables, dim3(nat), dim3(128), 0, s, tau, nat, b, kmin[0], kmin[1],
+                     kmin[2], krange, tab);
+}
+
+// exp(-i G.tau_a) = (ca - i sa)(cb - i sb)(cc - i sc) as in :144-150
+__device__ __forceinline__ void eigr_from_tables(const double* __restrict__ tab, int a, int krange,
+                                                 int kx, int ky, int kz, double& re, double& im) {
+  const double* p0 = tab + ((long)(a * 3 + 0) * krange + kx) * 2;
+  const double* p1 = tab + ((long)(a * 3 + 1) * krange + ky) * 2;
+  const double* p2 = tab + ((long)(a * 3 + 2) * krange + kz) * 2;
+  const double ca = p0[0], sa = p0[1], cb = p1[0], sb = p1[1], cc = p2[0], sc = p2[1];
+  re = ca * cb * cc - sa * sb * cc - sa * cb * sc - ca * sb * sc;
+  im = sa * sb * sc - sa * cb * cc - ca * sb * cc - ca * cb * sc;
+}
+
+// sfac[is][ig] = sum_{a in is} exp(-iG.tau_a); then (src/EnergyFunctional.cpp:1085-1095)
+// rhopst = sum_s sfac rhops, vion = sum_s sfac vps, dvion = sum_s sfac dvps
+__global__ __launch_bounds__(256) void k_sfac(
+    const int32_t* __restrict__ idx, int ngv, const double* __restrict__ tab, int krange,
+    int kmin0, int kmin1, int kmin2, int nsp, const int32_t* __restrict__ sp_first,
+    const double* __restrict__ vps, const double* __restrict__ dvps,
+    const double* __restrict__ rhops, double* __restrict__ sfac, double* __restrict__ vion,
+    double* __restrict__ dvion, double* __restrict__ rhopst) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngv) return;
+  const int kx = idx[3 * ig] - kmin0, ky = idx[3 * ig + 1] - kmin1, kz = idx[3 * ig + 2] - kmin2;
+  double vr = 0, vi = 0, dr = 0, di = 0, pr = 0, pi = 0;
+  for (int is = 0; is < nsp; is++) {
+    double sr = 0.0, si = 0.0;
+    for (int a = sp_first[is]; a < sp_first[is + 1]; a++) {
+      double re, im;
+      eigr_from_tables(tab, a, krange, kx, ky, kz, re, im);
+      sr += re;
+      si += im;
+    }
+    sfac[2 * ((long)is * ngv + ig)] = sr;
+    sfac[2 * ((long)is * ngv + ig) + 1] = si;
+    const long o = (long)is * ngv + ig;
+    pr += sr * rhops[o]; pi += si * rhops[o];
+    vr += sr * vps[o]; vi += si * vps[o];
+    dr += sr * dvps[o]; di += si * dvps[o];
+  }
+  vion[2 * ig] = vr; vion[2 * ig + 1] = vi;
+  dvion[2 * ig] = dr; dvion[2 * ig + 1] = di;
+  rhopst[2 * ig] = pr; rhopst[2 * ig + 1] = pi;
+}
+
+void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, const int* kmin,
+                 int nsp, const int32_t* sp_first, const double* vps, const double* dvps,
+                 const double* rhops, double* sfac, double* vion, double* dvion, double* rhopst,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_sfac, dim3((ngv + 255) / 256), dim3(256), 0, s, idx, ngv, tab, krange,
+                     kmin[0], kmin[1], kmin[2], nsp, sp_first, vps, dvps, rhops, sfac, vion,
+                     dvion, rhopst);
+}
+
+// ---------------------------------------------------------- local forces
+// src/EnergyFunctional.cpp:841-918: per atom a of species s,
+//   vtemp = 4 pi rhops g2i conj(rhogt) + vps conj(rhoelg)
+//   f_a,j = -2 omega sum_G (Re e * Im vtemp + Im e * Re vtemp) G_j
+// grid (chunks, atoms); partial[(a*nchunk + chunk)*3 + j]
+__global__ __launch_bounds__(256) void k_local_forces(
+    const int32_t* __restrict__ idx, int ngv, const double* __restrict__ tab, int krange,
+    int kmin0, int kmin1, int kmin2, const int32_t* __restrict__ atom_sp,
+    const double* __restrict__ vps, const double* __restrict__ rhops,
+    const double* __restrict__ g2i, const double* __restrict__ gx,
+    const double* __restrict__ rhogt, const double* __restrict__ rhoelg, int chunk,
+    double* __restrict__ partial) {
+  const int a = blockIdx.y;
+  const int is = atom_sp[a];
+  const int g0 = blockIdx.x * chunk, g1 = min(ngv, g0 + chunk);
+  const double fpi = 4.0 * M_PI;
+  double v[3] = {0, 0, 0};
+  for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
+    const long o = (long)is * ngv + ig;
+    const double tmp = fpi * rhops[o] * g2i[ig];
+    const double gtr = rhogt[2 * ig], gti = rhogt[2 * ig + 1];
+    const double elr = rhoelg[2 * ig], eli = rhoelg[2 * ig + 1];
+    // conj: (x + iy)* = x - iy
+    const double vtr = tmp * gtr + vps[o] * elr;
+    const double vti = -tmp * gti - vps[o] * eli;
+    double er, ei;
+    eigr_from_tables(tab, a, krange, idx[3 * ig] - kmin0, idx[3 * ig + 1] - kmin1,
+                     idx[3 * ig + 2] - kmin2, er, ei);
+    const double t = er * vti + ei * vtr;
+    v[0] += t * gx[ig];
+    v[1] += t * gx[ngv + ig];
+    v[2] += t * gx[2 * ngv + ig];
+  }
+  block_reduce_store<3>(v, partial + ((long)a * gridDim.x + blockIdx.x) * 3);
+}
+
+// sum chunks per atom: f[3a+j] = fion_esr[3a+j] - 2 omega sum_chunk partial
+__global__ void k_local_forces_final(const double* __restrict__ partial, int nchunk, int nat,
+                                     double omega, const double* __restrict__ fesr,
+                                     double* __restrict__ fion) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * nat) return;
+  const int a = t / 3, j = t % 3;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; c++) s += partial[((long)a * nchunk + c) * 3 + j];
+  fion[t] += fesr[t] + (-2.0 * omega) * s;
+}
+
+void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int krange,
+                         const int* kmin, const int32_t* atom_sp, int nat, const double* vps,
+                         const double* rhops, const double* g2i, const double* gx,
+                         const double* rhogt, const double* rhoelg, double omega,
+                         const double* fesr, double* partial, int nchunk, double* fion,
+                         hipStream_t s) {
+  const int chunk = (ngv + nchunk - 1) / nchunk;
+  hipLaunchKernelGGL(k_local_forces, dim3(nchunk, nat), dim3(256), 0, s, idx, ngv, tab, krange,
+                     kmin[0], kmin[1], kmin[2], atom_sp, vps, rhops, g2i, gx, rhogt, rhoelg,
+                     chunk, partial);
+  hipLaunchKernelGGL(k_local_forces_final, dim3((3 * nat + 255) / 256), dim3(256), 0, s,
+                     partial, nchunk, nat, omega, fesr, fion);
+}
+
+// ------------------------------------------------------- projectors anl
+// src/NonLocalPotential.cpp:1314-1395 (anl = twnl (-i)^l e^{-iG.tau}) and
+// :1642-1698 (anl_j = -i G_j anl) for l = 0, 1.  Column m of block `mode`
+// (0: anl, 1..3: direction j = mode-1) holds atom a = m_atom[m] and
+// projector ipr = m_ipr[m]; A[(mode*M + m)*lda + 2*ig + {0,1}].
+template <typename T>
+__global__ __launch_bounds__(256) void k_anl(int ngw, int M, int nmodes, long lda,
+                                             const double* __restrict__ gx,
+                                             const double* __restrict__ tau,
+                                             const int32_t* __restrict__ m_atom,
+                                             const int32_t* __restrict__ m_twnl,
+                                             const int32_t* __restrict__ m_l,
+                                             const double* __restrict__ twnl,
+                                             T* __restrict__ A) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  const int m = blockIdx.y;
+  if (ig >= ngw) return;
+  const int a = m_atom[m];
+  const int l = m_l[m];
+  const double t = twnl[(long)m_twnl[m] + ig];
+  const double gxx = gx[ig], gyy = gx[ngw + ig], gzz = gx[2 * ngw + ig];
+  const double arg = -(gxx * tau[3 * a] + gyy * tau[3 * a + 1] + gzz * tau[3 * a + 2]);
+  double s, c;
+  sincos(arg, &s, &c);
+  for (int mode = 0; mode < nmodes; mode++) {
+    double re, im;
+    if (mode == 0) {
+      if (l == 0) { re = t * c; im = t * s; }
+      else { re = t * s; im = -t * c; }
+    } else {
+      const double gj = mode == 1 ? gxx : (mode == 2 ? gyy : gzz);
+      if (l == 0) { const double tt = gj * t; re = tt * s; im = -tt * c; }
+      else { const double tt = -gj * t; re = tt * c; im = tt * s; }
+    }
+    T* col = A + ((long)mode * M + m) * lda;
+    col[2 * ig] = (T)re;
+    col[2 * ig + 1] = (T)im;
+  }
+}
+
+void launch_anl(int ngw, int M, int nmodes, long lda, const double* gx, const double* tau,
+                const int32_t* m_atom, const int32_t* m_twnl, const int32_t* m_l,
+                const double* twnl, void* A, int prec, hipStream_t s) {
+  dim3 grid((ngw + 255) / 256, M);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_anl<double>, grid, dim3(256), 0, s, ngw, M, nmodes, lda, gx, tau, m_atom,
+                       m_twnl, m_l, twnl, (double*)A);
+  else
+    hipLaunchKernelGGL(k_anl<float>, grid, dim3(256), 0, s, ngw, M, nmodes, lda, gx, tau, m_atom,
+                       m_twnl, m_l, twnl, (float*)A);
+}
+
+// ------------------------------------------------- D matrix, enl, forces
+// src/NonLocalPotential.cpp:1469-1571.  F (M x nst) holds fnl after the
+// G=0 correction and factor 2.  For column m (atom a, projector ipr of
+// species s): F'_m = sum_jpr D_s[ipr][jpr] F_{m(a,jpr)};
+// enl += occ_n wt/omega F F'; fout = wt/omega F'.
+// m_dbase[m]: offset of the species D block; m_npr[m]; m_stride[m] = na_s
+// (columns of one atom are ipr*na_s apart); m_ipr[m].
+__global__ __launch_bounds__(256) void k_dmat_enl(
+    const double* __restrict__ F, int M, int nst, const double* __restrict__ occ,
+    const int32_t* __restrict__ m_ipr, const int32_t* __restrict__ m_npr,
+    const int32_t* __restrict__ m_stride, const int32_t* __restrict__ m_dbase,
+    const double* __restrict__ dmat, const double* __restrict__ m_wt, double omega_inv,
+    double* __restrict__ fout, double* __restrict__ partial) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  if (t < (long)M * nst) {
+    const int m = (int)(t % M), n = (int)(t / M);
+    const int ipr = m_ipr[m], npr = m_npr[m], st = m_stride[m];
+    const int m0 = m - ipr * st;   // column of projector 0 of this atom
+    const double* D = dmat + m_dbase[m] + (long)ipr * npr;
+    double fp = 0.0;
+    for (int j = 0; j < npr; j++) fp += D[j] * F[(long)n * M + m0 + (long)j * st];
+    const double fac = m_wt[m] * omega_inv;
+    v[0] = fac * occ[n] * F[t] * fp;
+    fout[t] = fac * fp;
+  }
+  block_reduce_store<1>(v, partial + blockIdx.x);
+}
+
+int launch_dmat_enl(const double* F, int M, int nst, const double* occ, const int32_t* m_ipr,
+                    const int32_t* m_npr, const int32_t* m_stride, const int32_t* m_dbase,
+                    const double* dmat, const double* m_wt, double omega_inv, double* fout,
+                    double* partial, double* out1, hipStream_t s) {
+  const long tot = (long)M * nst;
+  const int nb = (int)((tot + 255) / 256);
+  hipLaunchKernelGGL(k_dmat_enl, dim3(nb), dim3(256), 0, s, F, M, nst, occ, m_ipr, m_npr,
+                     m_stride, m_dbase, dmat, m_wt, omega_inv, fout, partial);
+  reduce_partials(partial, nb, 1, out1, s);
+  return nb;
+}
+
+// fion[3a+j] -= sum_{m of atom a} sum_n 2 occ_n fout[m,n] dF_j[m,n]
+// (src/NonLocalPotential.cpp:1734-1751); one block per (atom, j)
+__global__ __launch_bounds__(256) void k_nl_forces(const double* __restrict__ fout,
+                                                   const double* __restrict__ dF, int M, int nst,
+                                                   const double* __restrict__ occ,
+                                                   const int32_t* __restrict__ a_m0,
+                                                   const int32_t* __restrict__ a_npr,
+                                                   const int32_t* __restrict__ a_stride,
+                                                   double* __restrict__ fion) {
+  // dF is the 3M x nst block of the force GEMM: dF[(j*M + m) + n*3M]
+  const int a = blockIdx.x, j = blockIdx.y;
+  const int m0 = a_m0[a], npr = a_npr[a], st = a_stride[a];
+  double v[1] = {0.0};
+  for (int t = threadIdx.x; t < npr * nst; t += 256) {
+    const int ipr = t % npr, n = t / npr;
+    const int m = m0 + ipr * st;
+    v[0] += 2.0 * occ[n] * fout[(long)n * M + m] * dF[(long)n * 3 * M + (long)j * M + m];
+  }
+  __shared__ double res[1];
+  block_reduce_store<1>(v, res);
+  __syncthreads();
+  if (threadIdx.x == 0) fion[3 * a + j] -= res[0];
+}
+
+void launch_nl_forces(const double* fout, const double* dF, int M, int nst, const double* occ,
+                      const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                      int nat, double* fion, hipStream_t s) {
+  hipLaunchKernelGGL(k_nl_forces, dim3(nat, 3), dim3(256), 0, s, fout, dF, M, nst, occ, a_m0,
+                     a_npr, a_stride, fion);
+}
+
+}  // namespace qbg

@@ -494,26 +494,41 @@ static int zfwd_t(qbg_grid* g, int mode, int outmode, double* out, long pstride,
 }
 
 // exported launchers -------------------------------------------------------
+// Each one checks, before launching, that the workspace holds the batch and
+// that the arrays its mode dereferences are present.
+static bool ws_ok(const qbg_grid* g, int nbatch) {
+  return g && g->d_zbuf && g->d_xbuf && nbatch >= 1 && nbatch <= g->ws_pairs;
+}
+
 int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s) {
+  if (!ws_ok(g, nbatch) || !c) return QBG_E_STATE;
   return prec == QBG_PREC_FP64 ? zbwd_t<double>(g, mode, c, pstride, off2, last_has2, nbatch, s)
                                : zbwd_t<float>(g, mode, c, pstride, off2, last_has2, nbatch, s);
 }
 int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
+  if (!ws_ok(g, nbatch)) return QBG_E_STATE;
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, false, nbatch, s)
                                : xpass_t<float>(g, false, nbatch, s);
 }
 int xfwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
+  if (!ws_ok(g, nbatch)) return QBG_E_STATE;
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, true, nbatch, s)
                                : xpass_t<float>(g, true, nbatch, s);
 }
 int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const double* fac,
           double* rho, double* fgrid, double scale, hipStream_t s) {
+  if (!ws_ok(g, nbatch)) return QBG_E_STATE;
+  const bool bad = (mode == Y_VMUL && !v) || (mode == Y_DENSITY && (!fac || !rho)) ||
+                   ((mode == Y_BWD_FULL || mode == Y_FWD_FULL) && !fgrid) ||
+                   (mode == Y_BWD_REAL_ADD && (!v || !fgrid)) || (mode == Y_FWD_REAL && !v);
+  if (bad) return QBG_E_INVAL;
   return prec == QBG_PREC_FP64 ? ypass_t<double>(g, mode, nbatch, v, fac, rho, fgrid, scale, s)
                                : ypass_t<float>(g, mode, nbatch, v, fac, rho, fgrid, scale, s);
 }
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s) {
+  if (!ws_ok(g, nbatch) || !out) return QBG_E_STATE;
   return prec == QBG_PREC_FP64
              ? zfwd_t<double>(g, mode, outmode, out, pstride, off2, last_has2, nbatch, s)
              : zfwd_t<float>(g, mode, outmode, out, pstride, off2, last_has2, nbatch, s);

@@ -368,22 +368,26 @@ int qbg_grid_destroy(qbg_grid* g) {
 
 }  // extern "C"
 
-// workspace for up to npairs pairs at the given precision
-static int ensure_ws(qbg_grid* g, int npairs) {
+// workspace for up to npairs pairs (sized for FP64; FP32 uses half).  The
+// pass launchers refuse to run a batch larger than g->ws_pairs.
+int qbg::grid_ensure_ws(qbg_grid* g, int npairs) {
+  if (npairs <= g->ws_pairs) return QBG_OK;
   const size_t zb = (size_t)npairs * g->nvec * g->np2 * 16;
   const size_t xb = (size_t)npairs * g->nact * g->np0 * g->np2 * 16;
-  if (zb + xb <= g->ws_bytes) return QBG_OK;
   qbg_ctx* c = g->ctx;
   (void)hipStreamSynchronize(c->stream);
   if (g->d_zbuf) (void)hipFree(g->d_zbuf);
   if (g->d_xbuf) (void)hipFree(g->d_xbuf);
   g->d_zbuf = g->d_xbuf = nullptr;
   g->ws_bytes = 0;
+  g->ws_pairs = 0;
   if (hipMalloc(&g->d_zbuf, zb) != hipSuccess || hipMalloc(&g->d_xbuf, xb) != hipSuccess)
     return c->fail(QBG_E_NOMEM, "hipMalloc failed (FFT workspace)");
   g->ws_bytes = zb + xb;
+  g->ws_pairs = npairs;
   return QBG_OK;
 }
+static int ensure_ws(qbg_grid* g, int npairs) { return qbg::grid_ensure_ws(g, npairs); }
 
 static int ensure_fac(qbg_grid* g, int n) {
   if (n <= g->fac_cap) return QBG_OK;
@@ -403,11 +407,12 @@ static int ensure_fac(qbg_grid* g, int n) {
     if (r_ != QBG_OK) return r_; \
   } while (0)
 
-#define PROF(kind, call)     \
-  do {                       \
-    g->ctx->prof_begin(kind); \
-    (void)(call);            \
-    g->ctx->prof_end();      \
+#define PROF(kind, call)                                                   \
+  do {                                                                     \
+    g->ctx->prof_begin(kind);                                              \
+    const int r_ = (call);                                                 \
+    g->ctx->prof_end();                                                    \
+    if (r_ != QBG_OK) return g->ctx->fail(r_, "FFT pass launch refused"); \
   } while (0)
 
 static int checkprec(qbg_grid* g, int prec) {

@@ -80,6 +80,7 @@ struct qbg_grid {
   void* d_zbuf = nullptr;
   void* d_xbuf = nullptr;
   size_t ws_bytes = 0;
+  int ws_pairs = 0;                 // pairs the workspace holds
   double* d_fac = nullptr;          // per-band density factors
   int fac_cap = 0;
   int batch = 64;                   // max pairs per batch
@@ -95,6 +96,10 @@ enum OutMode { OUT_STORE = 0, OUT_ACCUM = 1 };
 // batched pass launchers (fft.hip).  Pair b of a batch reads/writes the
 // bands at c + 2*b*pstride (first) and c + 2*(b*pstride + off2) (second,
 // absent for the last pair when last_has2 == 0).
+// allocate the pass workspace of g for npairs band pairs (no-op if it holds
+// that many already); every launcher below returns QBG_E_STATE without
+// launching when the workspace is missing or smaller than its batch.
+int grid_ensure_ws(qbg_grid* g, int npairs);
 int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s);
 int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s);

@@ -1,0 +1,54 @@
+// setup.h -- host-side setup of one Gamma-point system: the work Qbox does
+// on the host once per geometry (Basis, BasisMapping, Species SLPP tables,
+// NonLocalPotential::update_twnl, the esr lattice sums).  Everything here
+// runs at creation / atoms_moved / cell_moved time, never per evaluation.
+#pragma once
+#include <vector>
+#include <cstdint>
+
+namespace qbg {
+namespace host {
+
+struct Cell {
+  double a[3][3], b[3][3], volume;
+  double an[13][3], an2h[13];
+  void set(const double* a9);                   // src/UnitCell.cpp:25-124
+  void fold_in_ws(double v[3]) const;           // src/UnitCell.cpp:155-186
+};
+
+bool factorizable(int n);                       // src/Basis.cpp:105-133
+
+// Basis at k = 0 on one task (src/Basis.cpp:255-690)
+struct Basis {
+  int size = 0, nrods = 0;
+  int idxmin[3], idxmax[3], np[3];
+  std::vector<int> rod_h, rod_k, rod_lmin, rod_size;
+  std::vector<int32_t> idx;                     // 3*size
+  std::vector<double> g, g2, gi, g2i, gx;       // gx[j*size + i]
+  void build(const Cell& defcell, const Cell& cell, double ecut);
+};
+
+// BasisMapping maps for one task, real basis (src/BasisMapping.cpp:38-228)
+struct Maps {
+  int nvec = 0;
+  std::vector<int32_t> ip, im, zvec_to_val;
+  bool build(const Basis& b, int np0, int np1, int np2);
+};
+
+// SLPP species with one projector per l (src/Species.cpp:510-1053)
+struct Species {
+  int zval = 0, lmax = 0, nmesh = 0;
+  double deltar = 0, rcps = 0;
+  std::vector<double> vloc, proj, dl;           // inputs
+  int ndft = 0, nlm = 0;
+  std::vector<double> gspl, vlocg, vlocg2;
+  std::vector<std::vector<double>> vnlg, vnlg2;
+  bool init();
+  void dvlocg(double g, double& v, double& dv) const;
+  void dvnlg(int l, double g, double& v, double& dv) const;
+  double rhopsg(double g) const;
+  double eself() const;
+};
+
+}  // namespace host
+}  // namespace qbg

@@ -9,6 +9,7 @@ loudly.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -50,7 +51,35 @@ SIGNATURES = {
     "qbg_compute_density": [_vp, _vp, C.c_int, C.c_int, _dp, C.c_double, C.c_double, _vp,
                             C.c_int],
     "qbg_rs_mul_add": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int],
+    "qbg_prof_enable": [_vp, C.c_int],
+    "qbg_prof_reset": [_vp],
+    "qbg_prof_read": [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
+    "qbg_gram": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "qbg_ef_create": [_vp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
+                      C.POINTER(_vp)],
+    "qbg_ef_destroy": [_vp],
+    "qbg_ef_sizes": [_vp, _ip],
+    "qbg_ef_basis": [_vp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip],
+    "qbg_ef_set_positions": [_vp, _dp],
+    "qbg_ef_update_density": [_vp, _vp, C.c_int, C.c_int],
+    "qbg_ef_update_vhxc": [_vp, C.c_int],
+    "qbg_ef_energy": [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                      _dp, _dp, _dp],
+    "qbg_ef_get": [_vp, C.c_int, _dp],
+    "qbg_ef_set_vr": [_vp, _dp],
 }
+
+KERNEL_KINDS = ["zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd", "yother", "nl", "gram",
+                "other"]
+ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p", "exc", "dxc",
+                "esr", "eself", "ecoul", "etotal", "total_charge"]
+
+
+class SpeciesDesc(C.Structure):
+    """qbg_species_desc (include/qbg.h)."""
+    _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
+                ("deltar", C.c_double), ("rcps", C.c_double),
+                ("vloc", _dp), ("proj", _dp), ("dl", _dp)]
 
 _lib = None
 
@@ -89,6 +118,12 @@ class Context:
             raise QbgError(rc, "qbg_init failed (no HIP device?)")
         self.h = h
         self.lib = lib
+        # objects created on this context; released before the context is
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     def check(self, rc):
         if rc != QBG_OK:
@@ -99,6 +134,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.release()
             self.lib.qbg_finalize(self.h)
             self.h = None
 
@@ -137,6 +174,7 @@ class DArray:
         p = _vp()
         ctx.check(ctx.lib.qbg_malloc(ctx.h, self.nbytes, C.byref(p)))
         self.ptr = p
+        ctx.adopt(self)
 
     @property
     def p(self):
@@ -161,14 +199,16 @@ class DArray:
     def offset(self, nbytes):
         return _vp(self.ptr.value + nbytes)
 
-    def free(self):
-        if self.ptr:
+    def release(self):
+        if self.ptr and self.ctx.h:
             self.ctx.lib.qbg_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    free = release
 
     def __del__(self):
         try:
-            self.free()
+            self.release()
         except Exception:
             pass
 
@@ -189,12 +229,16 @@ class Grid:
         self.np = (np0, np1, np2)
         self.ng = ip.size
         self.N = np0 * np1 * np2
+        ctx.adopt(self)
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_grid_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
-                self.ctx.lib.qbg_grid_destroy(self.h)
-                self.h = None
+            self.release()
         except Exception:
             pass
 
@@ -219,3 +263,133 @@ class Grid:
     def rs_mul_add(self, v: DArray, c: DArray, ldc, nst, hpsi: DArray, ldh, prec="fp64"):
         self.ctx.check(self.ctx.lib.qbg_rs_mul_add(self.h, v.p, c.p, ldc, nst, hpsi.p, ldh,
                                                    PREC[prec]))
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Profiler:
+    """Per-kernel HIP-event timing of a context (qbg_prof_*)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def enable(self, on=True):
+        self.ctx.check(self.ctx.lib.qbg_prof_enable(self.ctx.h, int(on)))
+
+    def reset(self):
+        self.ctx.check(self.ctx.lib.qbg_prof_reset(self.ctx.h))
+
+    def read(self):
+        out = {}
+        for k, name in enumerate(KERNEL_KINDS):
+            ms, n = C.c_double(), C.c_int()
+            self.ctx.check(self.ctx.lib.qbg_prof_read(self.ctx.h, k, C.byref(ms), C.byref(n)))
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out
+
+
+def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
+    """SlaterDet::gram on device coefficients (in place)."""
+    ctx.check(ctx.lib.qbg_gram(ctx.h, c.p, ldc, ngw, nst, PREC[prec]))
+
+
+class EnergyFunctional:
+    """qbg_ef: one Gamma-point system resident on the device."""
+
+    def __init__(self, ctx: Context, sysdef):
+        self.ctx = ctx
+        self.sysdef = sysdef
+        descs = (SpeciesDesc * len(sysdef.species))()
+        self._keep = [descs]
+        for i, sp in enumerate(sysdef.species):
+            vloc, proj, dl = sp.radial()
+            self._keep += [vloc, proj, dl]
+            descs[i] = SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
+                                   _pd(vloc), _pd(proj), _pd(dl))
+        cell = np.ascontiguousarray(sysdef.cell, dtype=np.float64).ravel()
+        na = np.ascontiguousarray(sysdef.na, dtype=np.int32)
+        tau = np.ascontiguousarray(sysdef.tau, dtype=np.float64).ravel()
+        occ = np.ascontiguousarray(sysdef.occ, dtype=np.float64)
+        self._keep += [cell, na, tau, occ]
+        h = _vp()
+        ctx.check(ctx.lib.qbg_ef_create(ctx.h, _pd(cell), sysdef.ecut, len(sysdef.species),
+                                        C.cast(descs, _vp), na.ctypes.data_as(_ip), _pd(tau),
+                                        sysdef.nst, _pd(occ), C.byref(h)))
+        self.h = h
+        s = np.zeros(10, dtype=np.int32)
+        ctx.check(ctx.lib.qbg_ef_sizes(h, s.ctypes.data_as(_ip)))
+        (self.ngw, self.ngv, self.np0, self.np1, self.np2, self.nvec, self.nvec_v, self.nat,
+         self.nproj, self.nst) = [int(x) for x in s]
+        self.N = self.np0 * self.np1 * self.np2
+        ctx.adopt(self)
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_ef_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def basis(self, which=0):
+        n = self.ngw if which == 0 else self.ngv
+        nvec = self.nvec if which == 0 else self.nvec_v
+        idx = np.zeros(3 * n, dtype=np.int32)
+        g2 = np.zeros(n)
+        gx = np.zeros(3 * n)
+        ip = np.zeros(n, dtype=np.int32)
+        im = np.zeros(n, dtype=np.int32)
+        z2v = np.zeros(nvec, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.qbg_ef_basis(self.h, which, idx.ctypes.data_as(_ip), _pd(g2),
+                                                 _pd(gx), ip.ctypes.data_as(_ip),
+                                                 im.ctypes.data_as(_ip), z2v.ctypes.data_as(_ip)))
+        return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip, im=im,
+                    zvec_to_val=z2v)
+
+    def set_positions(self, tau):
+        tau = np.ascontiguousarray(tau, dtype=np.float64).ravel()
+        self.ctx.check(self.ctx.lib.qbg_ef_set_positions(self.h, _pd(tau)))
+
+    def update_density(self, c: DArray, prec="fp64"):
+        self.ctx.check(self.ctx.lib.qbg_ef_update_density(self.h, c.p, self.ngw, PREC[prec]))
+
+    def update_vhxc(self, stress=False):
+        self.ctx.check(self.ctx.lib.qbg_ef_update_vhxc(self.h, int(stress)))
+
+    def energy(self, c: DArray, hpsi: DArray = None, forces=True, stress=False, prec="fp64"):
+        e = np.zeros(14)
+        f = np.zeros(3 * self.nat)
+        sig = np.zeros(42)
+        self.ctx.check(self.ctx.lib.qbg_ef_energy(
+            self.h, c.p, self.ngw, int(hpsi is not None), hpsi.p if hpsi is not None else None,
+            self.ngw, int(forces), int(stress), PREC[prec], _pd(e), _pd(f), _pd(sig)))
+        out = dict(zip(ENERGY_NAMES, e.tolist()))
+        out["fion"] = f.reshape(-1, 3)
+        out["sigma"] = sig.reshape(7, 6)
+        return out
+
+    def evaluate(self, c: DArray, hpsi: DArray = None, forces=True, stress=False, prec="fp64"):
+        """density -> update_vhxc -> energy (src/BOSampleStepper.cpp:1207-1217)."""
+        self.update_density(c, prec)
+        self.update_vhxc(stress)
+        return self.energy(c, hpsi, forces, stress, prec)
+
+    def get(self, what):
+        n = {0: self.N, 1: self.N, 2: self.N, 3: 2 * self.ngv, 4: 2 * self.ngv,
+             5: 2 * self.ngv, 6: 2 * len(self.sysdef.species) * self.ngv,
+             7: 3 * self.nat}[what]
+        out = np.zeros(n)
+        self.ctx.check(self.ctx.lib.qbg_ef_get(self.h, what, _pd(out)))
+        if what in (3, 4, 5, 6):
+            return out.view(np.complex128)
+        return out
+
+    def set_vr(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.qbg_ef_set_vr(self.h, _pd(v)))
