@@ -81,7 +81,21 @@ def main():
     ap.add_argument("--prec", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import qbg
+        ctx = qbg.Context(0)
+        nb = args.copy_probe << 20
+        src, dst = ctx.array(nb // 8), ctx.array(nb // 8)
+        src.zero()
+        for _ in range(3):
+            ctx.check(ctx.lib.qbg_stream_copy(ctx.h, dst.p, src.p, nb))
+        ctx.sync()
+        print(json.dumps({"copy_probe_bytes_read": nb, "copy_probe_bytes_written": nb,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (band sharding + RCCL reduction of rho), so "

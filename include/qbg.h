@@ -77,6 +77,10 @@ int qbg_timer_stop(qbg_ctx* ctx, float* ms);
 #define QBG_K_GRAM 8      /* overlap, Cholesky, triangular update */
 #define QBG_K_OTHER 9     /* reductions, pointwise, structure factor */
 #define QBG_K_COUNT 10
+/* A plain device-to-device streaming copy of `bytes` (a multiple of 16):
+ * the reference kernel of known traffic for calibrating HBM byte
+ * counters.  No counterpart in the reference. */
+int qbg_stream_copy(qbg_ctx* ctx, void* dst, const void* src, size_t bytes);
 int qbg_prof_enable(qbg_ctx* ctx, int on);
 int qbg_prof_reset(qbg_ctx* ctx);
 int qbg_prof_read(qbg_ctx* ctx, int kind, double* total_ms, int* count);

@@ -51,6 +51,7 @@ SIGNATURES = {
     "qbg_compute_density": [_vp, _vp, C.c_int, C.c_int, _dp, C.c_double, C.c_double, _vp,
                             C.c_int],
     "qbg_rs_mul_add": [_vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int],
+    "qbg_stream_copy": [_vp, _vp, _vp, C.c_size_t],
     "qbg_prof_enable": [_vp, C.c_int],
     "qbg_prof_reset": [_vp],
     "qbg_prof_read": [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
