@@ -479,27 +479,27 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
                  e->d_partial, e->d_scal + S_KIN, s);
   ctx->prof_end();
   // nonlocal: anl (+ three -i G_j anl blocks), fnl = 2 anl^T c - anl0 c0^T
-  const int nmodes = compute_forces ? 4 : 1;
   const long K = 2L * ngw;
   ctx->prof_begin(QBG_K_NL);
-  launch_anl(ngw, M, nmodes, K, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l,
+  launch_anl(ngw, M, 1, K, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l,
              e->d_twnl, e->d_anl, 0, s);
   if (gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M, e->d_work,
                      e->work_doubles, s))
-    return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
-  if (compute_forces &&
-      gemm_tn_splitk(e->d_anl + (size_t)M * K, K, c, 2L * ldc, 3 * M, nst, K, 2.0, 0.0,
-                     e->d_fnl + (size_t)M * nst, 3 * M, e->d_work, e->work_doubles, s))
     return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
   launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
                   e->d_scal + S_ENL, s);
   if (compute_hpsi) gemm_nn(e->d_anl, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, s);
   if (compute_forces) {
+    // Y = c W with W[n,m] = occ_n fout[m,n] (stored in the second anl block),
+    // then f_a,j = -4 sum_{m in a} sum_K (-i G_j anl)[K,m] Y[K,m]
+    double* W = e->d_fnl + (size_t)M * nst;
+    double* Y = e->d_anl + (size_t)M * K;
     QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
-    // dF_j rows of the 3M x nst block are [j*M + m]; pass each j block
-    launch_nl_forces(e->d_fout, e->d_fnl + (size_t)M * nst, M, nst, e->d_occ, e->d_a_m0,
-                     e->d_a_npr, e->d_a_stride, e->nat, e->d_fion, s);
+    launch_force_w(e->d_fout, e->d_occ, M, nst, W, s);
+    gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, s);
+    launch_nl_forces_y(e->d_anl, Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr, e->d_a_stride,
+                       e->nat, e->d_partial, 64, e->d_fion, s);
   }
   ctx->prof_end();
   // local potential: hpsi += FFT(v_r * FFT^-1 c)

@@ -42,6 +42,12 @@ void launch_nl_forces(const double* fout, const double* dF, int M, int nst, cons
                       const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
                       int nat, double* fion, hipStream_t s);
 
+void launch_force_w(const double* fout, const double* occ, int M, int nst, double* W,
+                    hipStream_t s);
+void launch_nl_forces_y(const double* A, const double* Y, long lda, int ngw, const double* gx,
+                        const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                        int nat, double* partial, int nchunk, double* fion, hipStream_t s);
+
 // gemm.hip
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,

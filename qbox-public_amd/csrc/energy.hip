@@ -482,4 +482,69 @@ void launch_nl_forces(const double* fout, const double* dF, int M, int nst, cons
                      a_npr, a_stride, fion);
 }
 
+// Forces by reassociation of src/NonLocalPotential.cpp:1703-1751:
+//   f_a,j = -sum_{m in a} sum_n 2 occ_n fout[m,n] dF_j[m,n],  dF_j = 2 anl_j^T c,
+//         = -4 sum_{m in a} sum_K anl_j[K,m] Y[K,m],  Y = c W,  W[n,m] = occ_n fout[m,n],
+// with anl_j = -i G_j anl, i.e. in the real view anl_j[2ig] = G_j a_im,
+// anl_j[2ig+1] = -G_j a_re.  One GEMM instead of three.
+__global__ void k_force_w(const double* __restrict__ fout, const double* __restrict__ occ,
+                          int M, int nst, double* __restrict__ W) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)M * nst) return;
+  const int nb = (int)(t % nst), m = (int)(t / nst);
+  W[t] = occ[nb] * fout[m + (long)nb * M];     // W[nb + m*nst]
+}
+
+void launch_force_w(const double* fout, const double* occ, int M, int nst, double* W,
+                    hipStream_t s) {
+  const long tot = (long)M * nst;
+  hipLaunchKernelGGL(k_force_w, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, fout, occ,
+                     M, nst, W);
+}
+
+// partial[(a*nchunk + chunk)*3 + j] = sum over the chunk's G and the atom's
+// projectors of G_j (a_im Y_re - a_re Y_im)
+__global__ __launch_bounds__(256) void k_nl_forces_y(
+    const double* __restrict__ A, const double* __restrict__ Y, long lda, int ngw,
+    const double* __restrict__ gx, const int32_t* __restrict__ a_m0,
+    const int32_t* __restrict__ a_npr, const int32_t* __restrict__ a_stride, int chunk,
+    double* __restrict__ partial) {
+  const int a = blockIdx.y;
+  const int m0 = a_m0[a], npr = a_npr[a], st = a_stride[a];
+  const int g0 = blockIdx.x * chunk, g1 = min(ngw, g0 + chunk);
+  double v[3] = {0, 0, 0};
+  for (int ip = 0; ip < npr; ip++) {
+    const long o = (long)(m0 + ip * st) * lda;
+    for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
+      const double ar = A[o + 2 * ig], ai = A[o + 2 * ig + 1];
+      const double yr = Y[o + 2 * ig], yi = Y[o + 2 * ig + 1];
+      const double t = ai * yr - ar * yi;
+      v[0] += gx[ig] * t;
+      v[1] += gx[ngw + ig] * t;
+      v[2] += gx[2 * ngw + ig] * t;
+    }
+  }
+  block_reduce_store<3>(v, partial + ((long)a * gridDim.x + blockIdx.x) * 3);
+}
+
+__global__ void k_nl_forces_final(const double* __restrict__ partial, int nchunk, int nat,
+                                  double* __restrict__ fion) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * nat) return;
+  const int a = t / 3, j = t % 3;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; c++) s += partial[((long)a * nchunk + c) * 3 + j];
+  fion[t] -= 4.0 * s;
+}
+
+void launch_nl_forces_y(const double* A, const double* Y, long lda, int ngw, const double* gx,
+                        const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                        int nat, double* partial, int nchunk, double* fion, hipStream_t s) {
+  const int chunk = (ngw + nchunk - 1) / nchunk;
+  hipLaunchKernelGGL(k_nl_forces_y, dim3(nchunk, nat), dim3(256), 0, s, A, Y, lda, ngw, gx, a_m0,
+                     a_npr, a_stride, chunk, partial);
+  hipLaunchKernelGGL(k_nl_forces_final, dim3((3 * nat + 255) / 256), dim3(256), 0, s, partial,
+                     nchunk, nat, fion);
+}
+
 }  // namespace qbg

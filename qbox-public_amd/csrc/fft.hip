@@ -500,19 +500,33 @@ static bool ws_ok(const qbg_grid* g, int nbatch) {
   return g && g->d_zbuf && g->d_xbuf && nbatch >= 1 && nbatch <= g->ws_pairs;
 }
 
+static int launched() { return hipGetLastError() == hipSuccess ? QBG_OK : QBG_E_HIP; }
+
 int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch) || !c) return QBG_E_STATE;
+  if (g->f4[2]) {
+    g->f4[2]->zbwd(g, prec, mode == Z_SINGLE, c, pstride, off2, last_has2, nbatch, s);
+    return launched();
+  }
   return prec == QBG_PREC_FP64 ? zbwd_t<double>(g, mode, c, pstride, off2, last_has2, nbatch, s)
                                : zbwd_t<float>(g, mode, c, pstride, off2, last_has2, nbatch, s);
 }
 int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch)) return QBG_E_STATE;
+  if (g->f4[0]) {
+    g->f4[0]->xpass(g, prec, false, nbatch, s);
+    return launched();
+  }
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, false, nbatch, s)
                                : xpass_t<float>(g, false, nbatch, s);
 }
 int xfwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch)) return QBG_E_STATE;
+  if (g->f4[0]) {
+    g->f4[0]->xpass(g, prec, true, nbatch, s);
+    return launched();
+  }
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, true, nbatch, s)
                                : xpass_t<float>(g, true, nbatch, s);
 }
@@ -523,12 +537,22 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
                    ((mode == Y_BWD_FULL || mode == Y_FWD_FULL) && !fgrid) ||
                    (mode == Y_BWD_REAL_ADD && (!v || !fgrid)) || (mode == Y_FWD_REAL && !v);
   if (bad) return QBG_E_INVAL;
+  if (g->f4[1]) {
+    const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? 4 : 1);
+    g->f4[1]->ypass(g, prec, mode, nbatch, ppb, v, fac, rho, fgrid, scale, s);
+    return launched();
+  }
   return prec == QBG_PREC_FP64 ? ypass_t<double>(g, mode, nbatch, v, fac, rho, fgrid, scale, s)
                                : ypass_t<float>(g, mode, nbatch, v, fac, rho, fgrid, scale, s);
 }
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch) || !out) return QBG_E_STATE;
+  if (g->f4[2]) {
+    g->f4[2]->zfwd(g, prec, mode == Z_SINGLE, outmode == OUT_ACCUM, out, pstride, off2,
+                   last_has2, nbatch, s);
+    return launched();
+  }
   return prec == QBG_PREC_FP64
              ? zfwd_t<double>(g, mode, outmode, out, pstride, off2, last_has2, nbatch, s)
              : zfwd_t<float>(g, mode, outmode, out, pstride, off2, last_has2, nbatch, s);

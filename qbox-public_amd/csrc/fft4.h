@@ -1,0 +1,127 @@
+// fft4.h -- four-step register FFT for the pass kernels (device templates).
+//
+// A line of length N = N1*N2 is transformed by N2 threads (phase A: an
+// N1-point DFT over the stride-N2 elements, held in registers, then the
+// twiddle W_N^{n2 k1}) and N1 threads (phase B: an N2-point DFT), with one
+// exchange through LDS in between.  The small DFTs are fully unrolled with
+// compile-time constants (twiddles.h), so a transform touches LDS once per
+// element instead of once per radix stage.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "internal.h"
+#include "twiddles.h"
+
+namespace qbg {
+namespace f4 {
+
+// factor split chosen for a line length: the divisor pair n1 >= n2 with the
+// smallest n1 (both <= 32); n1 = 0 if none
+__host__ __device__ constexpr int split_n1(int n) {
+  for (int a = 1; a <= 32; a++)
+    if (n % a == 0 && n / a <= a && n / a >= 1) return a;
+  return 0;
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int smallest_prime(int n) {
+  for (int p = 2; p <= n; p++)
+    if (n % p == 0) return p;
+  return n;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+// exp(SIGN 2 pi i m / R) from the constant tables
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ cpx<T> wr(int m) {
+  return {(T)Tw<R>::c[m], (T)(SIGN * Tw<R>::s[m])};
+}
+
+// in-register DFT of size R: v[k] <- sum_j v[j] exp(SIGN 2 pi i j k / R)
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void rdft(cpx<T>* v) {
+  if constexpr (R == 1) {
+    return;
+  } else if constexpr (R == 2) {
+    const cpx<T> a = v[0], b = v[1];
+    v[0] = add(a, b);
+    v[1] = sub(a, b);
+  } else if constexpr (R == 4) {
+    const cpx<T> a0 = add(v[0], v[2]), a1 = sub(v[0], v[2]);
+    const cpx<T> b0 = add(v[1], v[3]), b1 = sub(v[1], v[3]);
+    const cpx<T> jb1 = SIGN > 0 ? cpx<T>{-b1.y, b1.x} : cpx<T>{b1.y, -b1.x};
+    v[0] = add(a0, b0);
+    v[2] = sub(a0, b0);
+    v[1] = add(a1, jb1);
+    v[3] = sub(a1, jb1);
+  } else if constexpr (smallest_prime(R) == R) {
+    // odd prime: symmetric form, out[k] and out[R-k] share the cosine sums
+    constexpr int H = (R - 1) / 2;
+    cpx<T> out[R];
+    cpx<T> s0 = v[0];
+#pragma unroll
+    for (int j = 1; j < R; j++) s0 = add(s0, v[j]);
+    out[0] = s0;
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+      T ra = v[0].x, ia = v[0].y, rb = 0, ib = 0;
+#pragma unroll
+      for (int j = 1; j <= H; j++) {
+        const T c = (T)Tw<R>::c[(j * k) % R];
+        const T s = (T)Tw<R>::s[(j * k) % R];
+        const cpx<T> ap = add(v[j], v[R - j]), am = sub(v[j], v[R - j]);
+        ra += c * ap.x;
+        ia += c * ap.y;
+        rb += s * am.x;
+        ib += s * am.y;
+      }
+      // SIGN i sum s am = (-SIGN ib, SIGN rb)
+      out[k] = {ra - SIGN * ib, ia + SIGN * rb};
+      out[R - k] = {ra + SIGN * ib, ia - SIGN * rb};
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = out[k];
+  } else {
+    // R = p q, decimation in time: X[k + q u] = sum_r W_p^{ru} W_R^{rk} Y_r[k],
+    // Y_r = DFT_q of x[r + p m]
+    constexpr int p = (R % 4 == 0) ? 4 : smallest_prime(R);
+    constexpr int q = R / p;
+    cpx<T> t[R];
+#pragma unroll
+    for (int r = 0; r < p; r++) {
+      cpx<T> s[q];
+#pragma unroll
+      for (int m = 0; m < q; m++) s[m] = v[r + p * m];
+      rdft<T, q, SIGN>(s);
+#pragma unroll
+      for (int k = 0; k < q; k++) t[r * q + k] = s[k];
+    }
+#pragma unroll
+    for (int k = 0; k < q; k++) {
+      cpx<T> s[p];
+#pragma unroll
+      for (int r = 0; r < p; r++)
+        s[r] = (r * k == 0) ? t[r * q + k] : mul(t[r * q + k], wr<T, R, SIGN>((r * k) % R));
+      rdft<T, p, SIGN>(s);
+#pragma unroll
+      for (int u = 0; u < p; u++) v[k + q * u] = s[u];
+    }
+  }
+}
+
+// twiddle W_N^{SIGN m} from the LDS table tw[m] = exp(-2 pi i m / N)
+template <typename T, int SIGN>
+__device__ __forceinline__ cpx<T> twl(const cpx<T>* tw, int m) {
+  cpx<T> w = tw[m];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+}  // namespace f4
+}  // namespace qbg

@@ -1,0 +1,23 @@
+// fft4_c.hip -- four-step pass kernels instantiated for line lengths
+// 144, 160, 168 (see fft4_kernels.h).
+#include "fft4_kernels.h"
+
+namespace qbg {
+
+#define QBG_F4(A, B)                                                          \
+  {A * B, A, B, f4::Launch<A, B>::L, &f4::Launch<A, B>::zbwd, &f4::Launch<A, B>::zfwd, \
+   &f4::Launch<A, B>::xpass, &f4::Launch<A, B>::ypass}
+
+static const F4Ops ops_c[] = {
+    QBG_F4(12, 12),   // 144
+    QBG_F4(16, 10),   // 160
+    QBG_F4(14, 12),   // 168
+};
+
+const F4Ops* f4_ops_c(int n) {
+  for (const F4Ops& o : ops_c)
+    if (o.n == n) return &o;
+  return nullptr;
+}
+
+}  // namespace qbg

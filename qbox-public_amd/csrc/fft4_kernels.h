@@ -1,0 +1,499 @@
+// fft4_kernels.h -- the five pass kernels with four-step line transforms of
+// compile-time length (N1 x N2), and their launchers.  Instantiated per line
+// length in fft4_sizes.hip; dispatched from fft.hip through f4_ops().
+//
+// Index conventions (as in fft.hip): zbuf[pair][k][ivec], xbuf[pair][k][r][x],
+// real-space grids i + np0*(j + np1*k).  Within a line of length N,
+// phase A holds x[N2*n1 + n2] (thread n2) and phase B holds X[k1 + N1*k2]
+// (thread k1); a forward transform starting from the phase-B layout
+// (thread k1 holding y = k1 + N1*k2) ends in the phase-A layout.
+#pragma once
+#include "fft4.h"
+
+namespace qbg {
+namespace f4 {
+
+// ------------------------------------------------------------------ z-bwd
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long pstride,
+                                              long off2, int last_has2, int nbatch, int single,
+                                              const int32_t* __restrict__ ip,
+                                              const int32_t* __restrict__ im,
+                                              const qbg_tile* __restrict__ tiles, int nvec, int TC,
+                                              const cpx<T>* __restrict__ tw_g,
+                                              cpx<T>* __restrict__ zbuf) {
+  constexpr int N = N1 * N2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* tw = A + TC * N;
+  const int tid = threadIdx.x;
+  const qbg_tile t = tiles[blockIdx.x];
+  const int pair = blockIdx.y;
+  for (int i = tid; i < TC * N; i += 256) A[i] = {T(0), T(0)};
+  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
+  __syncthreads();
+  // pack two real-basis bands (or one) onto the columns
+  // (src/BasisMapping.cpp:628-700); -G written last so it wins at G=0
+  const int slot0 = t.col0 * N;
+  const double* c1 = c + 2 * (long)pair * pstride;
+  const bool has2 = !single && (pair < nbatch - 1 || last_has2);
+  const double* c2 = c1 + 2 * off2;
+  for (int ig = t.g0 + tid; ig < t.g1; ig += 256) {
+    const double a = c1[2 * ig], b = c1[2 * ig + 1];
+    double cc = 0.0, d = 0.0;
+    if (has2) { cc = c2[2 * ig]; d = c2[2 * ig + 1]; }
+    const int sp = ip[ig] - slot0, sm = im[ig] - slot0;
+    A[(sp % N) * TC + sp / N] = {T(a - d), T(b + cc)};
+    A[(sm % N) * TC + sm / N] = {T(a + d), T(cc - b)};
+  }
+  __syncthreads();
+  const int col = tid % TC, q = tid / TC;
+  if (col < t.ncol && q < N2) {           // phase A, q = n2, in place
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = A[(N2 * n1 + q) * TC + col];
+    rdft<T, N1, +1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; k1++)
+      if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) A[(k1 * N2 + q) * TC + col] = a[k1];
+  }
+  __syncthreads();
+  if (col < t.ncol && q < N1) {           // phase B, q = k1
+    cpx<T> b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) b[n2] = A[(q * N2 + n2) * TC + col];
+    rdft<T, N2, +1>(b);
+    cpx<T>* zp = zbuf + (long)pair * nvec * N + t.col0 + col;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) zp[(long)(q + N1 * k2) * nvec] = b[k2];
+  }
+}
+
+// ------------------------------------------------------------------ z-fwd
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
+                                              double* __restrict__ out, long pstride, long off2,
+                                              int last_has2, int nbatch, int single, int accum,
+                                              const int32_t* __restrict__ ip,
+                                              const int32_t* __restrict__ im,
+                                              const qbg_tile* __restrict__ tiles, int nvec, int TC,
+                                              const cpx<T>* __restrict__ tw_g, double scale) {
+  constexpr int N = N1 * N2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* tw = A + TC * N;
+  const int tid = threadIdx.x;
+  const qbg_tile t = tiles[blockIdx.x];
+  const int pair = blockIdx.y;
+  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
+  __syncthreads();
+  const int col = tid % TC, q = tid / TC;
+  if (col < t.ncol && q < N1) {           // forward phase A', q = k1
+    const cpx<T>* zp = zbuf + (long)pair * nvec * N + t.col0 + col;
+    cpx<T> b[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = zp[(long)(q + N1 * k2) * nvec];
+    rdft<T, N2, -1>(b);
+#pragma unroll
+    for (int m1 = 1; m1 < N2; m1++)
+      if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+#pragma unroll
+    for (int m1 = 0; m1 < N2; m1++) A[(m1 * N1 + q) * TC + col] = b[m1];
+  }
+  __syncthreads();
+  cpx<T> cv[N1];
+  const bool vb = col < t.ncol && q < N2;
+  if (vb) {                               // phase B', q = m1
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) cv[k1] = A[(q * N1 + k1) * TC + col];
+    rdft<T, N1, -1>(cv);
+  }
+  __syncthreads();
+  if (vb) {
+#pragma unroll
+    for (int m2 = 0; m2 < N1; m2++) A[(q + N2 * m2) * TC + col] = cv[m2];
+  }
+  __syncthreads();
+  // scale by 1/np012 (fz, src/FourierTransform.cpp:739-743) and unpack
+  // (zvec_to_doublevector / zvec_to_vector, src/BasisMapping.cpp:703-750)
+  const int slot0 = t.col0 * N;
+  double* o1 = out + 2 * (long)pair * pstride;
+  double* o2 = o1 + 2 * off2;
+  const bool has2 = !single && (pair < nbatch - 1 || last_has2);
+  for (int ig = t.g0 + tid; ig < t.g1; ig += 256) {
+    const int sp = ip[ig] - slot0;
+    const cpx<T> zpv = A[(sp % N) * TC + sp / N];
+    const double a = (double)zpv.x * scale, b = (double)zpv.y * scale;
+    if (has2) {
+      const int sm = im[ig] - slot0;
+      const cpx<T> zmv = A[(sm % N) * TC + sm / N];
+      const double cc = (double)zmv.x * scale, d = (double)zmv.y * scale;
+      const double r1 = 0.5 * (a + cc), i1 = 0.5 * (b - d);
+      const double r2 = 0.5 * (b + d), i2 = 0.5 * (cc - a);
+      if (accum) {
+        o1[2 * ig] += r1; o1[2 * ig + 1] += i1;
+        o2[2 * ig] += r2; o2[2 * ig + 1] += i2;
+      } else {
+        o1[2 * ig] = r1; o1[2 * ig + 1] = i1;
+        o2[2 * ig] = r2; o2[2 * ig + 1] = i2;
+      }
+    } else {
+      if (accum) { o1[2 * ig] += a; o1[2 * ig + 1] += b; }
+      else { o1[2 * ig] = a; o1[2 * ig + 1] = b; }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ x-bwd
+// colmap[r*N + x] = column (ivec) at (row r, x) or -1
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
+                                              cpx<T>* __restrict__ xbuf, int np2, int nvec,
+                                              int nact, int ngroups,
+                                              const int32_t* __restrict__ colmap,
+                                              const cpx<T>* __restrict__ tw_g) {
+  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
+  __shared__ cpx<T> ex[L * NP];
+  __shared__ cpx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x / ngroups;
+  const int r0 = (blockIdx.x - k * ngroups) * L;
+  const int nr = min(L, nact - r0);
+  const int pair = blockIdx.y;
+  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
+  __syncthreads();
+  const cpx<T>* zp = zbuf + ((long)pair * np2 + k) * nvec;
+  {
+    const int n2 = tid % N2, line = tid / N2;
+    if (line < nr) {
+      const int32_t* cm = colmap + (long)(r0 + line) * N;
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; n1++) {
+        const int iv = cm[N2 * n1 + n2];
+        a[n1] = iv >= 0 ? zp[iv] : cpx<T>{T(0), T(0)};
+      }
+      rdft<T, N1, +1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++)
+        if (n2) a[k1] = mul(a[k1], twl<T, +1>(tw, n2 * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) ex[line * NP + k1 * N2 + n2] = a[k1];
+    }
+  }
+  __syncthreads();
+  {
+    const int k1 = tid % N1, line = tid / N1;
+    if (line < nr) {
+      cpx<T> b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[line * NP + k1 * N2 + n2];
+      rdft<T, N2, +1>(b);
+      cpx<T>* xp = xbuf + (((long)pair * np2 + k) * nact + r0 + line) * N;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) xp[k1 + N1 * k2] = b[k2];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ x-fwd
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
+                                              cpx<T>* __restrict__ zbuf, int np2, int nvec,
+                                              int nact, int ngroups,
+                                              const int32_t* __restrict__ colmap,
+                                              const cpx<T>* __restrict__ tw_g) {
+  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
+  __shared__ cpx<T> ex[L * NP];
+  __shared__ cpx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x / ngroups;
+  const int r0 = (blockIdx.x - k * ngroups) * L;
+  const int nr = min(L, nact - r0);
+  const int pair = blockIdx.y;
+  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
+  __syncthreads();
+  {
+    const int k1 = tid % N1, line = tid / N1;
+    if (line < nr) {
+      const cpx<T>* xp = xbuf + (((long)pair * np2 + k) * nact + r0 + line) * N;
+      cpx<T> b[N2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) b[k2] = xp[k1 + N1 * k2];
+      rdft<T, N2, -1>(b);
+#pragma unroll
+      for (int m1 = 1; m1 < N2; m1++)
+        if (k1) b[m1] = mul(b[m1], twl<T, -1>(tw, k1 * m1));
+#pragma unroll
+      for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + k1] = b[m1];
+    }
+  }
+  __syncthreads();
+  {
+    const int m1 = tid % N2, line = tid / N2;
+    if (line < nr) {
+      cpx<T> cv[N1];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + m1 * N1 + k1];
+      rdft<T, N1, -1>(cv);
+      cpx<T>* zp = zbuf + ((long)pair * np2 + k) * nvec;
+      const int32_t* cm = colmap + (long)(r0 + line) * N;
+#pragma unroll
+      for (int m2 = 0; m2 < N1; m2++) {
+        const int iv = cm[m1 + N2 * m2];
+        if (iv >= 0) zp[iv] = cv[m2];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ y pass
+// block: L consecutive x of plane k; threads: line = tid % L (x fastest for
+// coalescing), q = tid / L.  jrow[y] = active row of y or -1.
+// MC: mode class, 0 = Y_VMUL, 1 = Y_DENSITY, 2 = the single full-grid
+// transforms (mode selects which); one instantiation per class keeps each
+// kernel's register allocation to what its mode needs.
+template <typename T, int N1, int N2, int MC>
+__global__ __launch_bounds__(256) void ky(cpx<T>* __restrict__ xbuf, int np0, int np2, int nact,
+                                          const int32_t* __restrict__ jrow, int nbatch, int PPB,
+                                          const cpx<T>* __restrict__ tw_g,
+                                          const double* __restrict__ vin,
+                                          const double* __restrict__ fac, double* __restrict__ rho,
+                                          double* __restrict__ fgrid, double scale, int mode) {
+  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
+  __shared__ cpx<T> ex[L * NP];
+  __shared__ cpx<T> tw[N];
+  __shared__ int jr[N];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * L;
+  const int nx = min(L, np0 - x0);
+  const int k = blockIdx.y;
+  for (int i = tid; i < N; i += 256) { tw[i] = tw_g[i]; jr[i] = jrow[i]; }
+  __syncthreads();
+  const int line = tid % L, q = tid / L;
+  const bool vA = line < nx && q < N2;    // phase A / B' role (N2 per line)
+  const bool vB = line < nx && q < N1;    // phase B / A' role (N1 per line)
+  const int x = x0 + line;
+  const long plane = (long)k * np0 * N;
+
+  if (MC == 0) mode = Y_VMUL;
+  if (MC == 1) mode = Y_DENSITY;
+  if (MC == 2 && (mode == Y_FWD_FULL || mode == Y_FWD_REAL)) {
+    cpx<T> b[N2];
+    if (vB) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) {
+        const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+        b[k2] = mode == Y_FWD_FULL ? cpx<T>{T(fgrid[2 * gi]), T(fgrid[2 * gi + 1])}
+                                   : cpx<T>{T(scale * vin[gi]), T(0)};
+      }
+      rdft<T, N2, -1>(b);
+#pragma unroll
+      for (int m1 = 1; m1 < N2; m1++)
+        if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+#pragma unroll
+      for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + q] = b[m1];
+    }
+    __syncthreads();
+    if (vA) {
+      cpx<T> cv[N1];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + q * N1 + k1];
+      rdft<T, N1, -1>(cv);
+      cpx<T>* xp = xbuf + (long)k * nact * np0;
+#pragma unroll
+      for (int m2 = 0; m2 < N1; m2++) {
+        const int r = jr[q + N2 * m2];
+        if (r >= 0) xp[(long)r * np0 + x] = cv[m2];
+      }
+    }
+    return;
+  }
+
+  constexpr int NACC = MC == 1 ? N2 : 1;
+  double acc[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; i++) acc[i] = 0.0;
+  const int p0 = blockIdx.z * PPB;
+  const int p1 = min(nbatch, p0 + PPB);
+  for (int pair = p0; pair < p1; pair++) {
+    cpx<T>* xp = xbuf + ((long)pair * np2 + k) * (long)nact * np0;
+    __syncthreads();                      // previous pair's readers of ex are done
+    if (vA) {                             // phase A, q = n2 (bwd)
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; n1++) {
+        const int r = jr[N2 * n1 + q];
+        a[n1] = r >= 0 ? xp[(long)r * np0 + x] : cpx<T>{T(0), T(0)};
+      }
+      rdft<T, N1, +1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++)
+        if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) ex[line * NP + k1 * N2 + q] = a[k1];
+    }
+    __syncthreads();
+    cpx<T> b[N2];
+    if (vB) {                             // phase B, q = k1: psi at y = q + N1*k2
+#pragma unroll
+      for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[line * NP + q * N2 + n2];
+      rdft<T, N2, +1>(b);
+    }
+    if (MC == 0) {
+      // tmp *= v (src/SlaterDet.cpp:432-434), then forward along y
+      if (vB) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const T v = (T)vin[plane + (long)(q + N1 * k2) * np0 + x];
+          b[k2] = {b[k2].x * v, b[k2].y * v};
+        }
+        rdft<T, N2, -1>(b);
+#pragma unroll
+        for (int m1 = 1; m1 < N2; m1++)
+          if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+      }
+      __syncthreads();
+      if (vB) {
+#pragma unroll
+        for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + q] = b[m1];
+      }
+      __syncthreads();
+      if (vA) {
+        cpx<T> cv[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + q * N1 + k1];
+        rdft<T, N1, -1>(cv);
+#pragma unroll
+        for (int m2 = 0; m2 < N1; m2++) {
+          const int r = jr[q + N2 * m2];
+          if (r >= 0) xp[(long)r * np0 + x] = cv[m2];
+        }
+      }
+    } else if (MC == 1) {
+      if (vB) {
+        const double f1 = fac[2 * pair], f2 = fac[2 * pair + 1];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const double p1v = (double)b[k2].x, p2v = (double)b[k2].y;
+          acc[k2 % NACC] += f1 * p1v * p1v + f2 * p2v * p2v;
+        }
+      }
+    } else if (mode == Y_BWD_FULL) {
+      if (vB) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+          fgrid[2 * gi] = (double)b[k2].x;
+          fgrid[2 * gi + 1] = (double)b[k2].y;
+        }
+      }
+    } else if (mode == Y_BWD_REAL_ADD) {
+      if (vB) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+          const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+          fgrid[gi] = vin[gi] + (double)b[k2].x;
+        }
+      }
+    }
+  }
+  if (MC == 1 && vB) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) rho[plane + (long)(q + N1 * k2) * np0 + x] += acc[k2 % NACC];
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+inline const cpx<T>* twp(const qbg_grid* g, int axis) {
+  if constexpr (sizeof(T) == 8) return reinterpret_cast<const cpx<T>*>(g->d_tw64[axis]);
+  else return reinterpret_cast<const cpx<T>*>(g->d_tw32[axis]);
+}
+
+template <int N1, int N2>
+struct Launch {
+  static constexpr int N = N1 * N2;
+  static constexpr int L = 256 / cmax(N1, N2);
+
+  template <typename T>
+  static void zbwd_t(qbg_grid* g, int single, const double* c, long pstride, long off2,
+                     int last_has2, int nbatch, hipStream_t s) {
+    const size_t lds = sizeof(cpx<T>) * ((size_t)g->zcols * N + N);
+    hipLaunchKernelGGL((kz_bwd<T, N1, N2>), dim3(g->ntiles, nbatch), dim3(256), lds, s, c,
+                       pstride, off2, last_has2, nbatch, single, g->d_ip, g->d_im, g->d_tiles,
+                       g->nvec, g->zcols, twp<T>(g, 2), (cpx<T>*)g->d_zbuf);
+  }
+  static void zbwd(qbg_grid* g, int prec, int single, const double* c, long pstride, long off2,
+                   int last_has2, int nbatch, hipStream_t s) {
+    if (prec == QBG_PREC_FP64) zbwd_t<double>(g, single, c, pstride, off2, last_has2, nbatch, s);
+    else zbwd_t<float>(g, single, c, pstride, off2, last_has2, nbatch, s);
+  }
+
+  template <typename T>
+  static void zfwd_t(qbg_grid* g, int single, int accum, double* out, long pstride, long off2,
+                     int last_has2, int nbatch, hipStream_t s) {
+    const size_t lds = sizeof(cpx<T>) * ((size_t)g->zcols * N + N);
+    const double scale = 1.0 / ((double)g->np0 * g->np1 * g->np2);
+    hipLaunchKernelGGL((kz_fwd<T, N1, N2>), dim3(g->ntiles, nbatch), dim3(256), lds, s,
+                       (const cpx<T>*)g->d_zbuf, out, pstride, off2, last_has2, nbatch, single,
+                       accum, g->d_ip, g->d_im, g->d_tiles, g->nvec, g->zcols, twp<T>(g, 2), scale);
+  }
+  static void zfwd(qbg_grid* g, int prec, int single, int accum, double* out, long pstride,
+                   long off2, int last_has2, int nbatch, hipStream_t s) {
+    if (prec == QBG_PREC_FP64) zfwd_t<double>(g, single, accum, out, pstride, off2, last_has2, nbatch, s);
+    else zfwd_t<float>(g, single, accum, out, pstride, off2, last_has2, nbatch, s);
+  }
+
+  static void xpass(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s) {
+    const int ngroups = (g->nact + L - 1) / L;
+    dim3 grid(g->np2 * ngroups, nbatch);
+    if (prec == QBG_PREC_FP64) {
+      if (!fwd)
+        hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(256), 0, s,
+                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2, g->nvec,
+                           g->nact, ngroups, g->d_colmap, twp<double>(g, 0));
+      else
+        hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(256), 0, s,
+                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2, g->nvec,
+                           g->nact, ngroups, g->d_colmap, twp<double>(g, 0));
+    } else {
+      if (!fwd)
+        hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(256), 0, s,
+                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2, g->nvec,
+                           g->nact, ngroups, g->d_colmap, twp<float>(g, 0));
+      else
+        hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(256), 0, s,
+                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2, g->nvec,
+                           g->nact, ngroups, g->d_colmap, twp<float>(g, 0));
+    }
+  }
+
+  static void ypass(qbg_grid* g, int prec, int mode, int nbatch, int ppb, const double* v,
+                    const double* fac, double* rho, double* fgrid, double scale, hipStream_t s) {
+    const bool fwd = mode == Y_FWD_FULL || mode == Y_FWD_REAL;
+    if (fwd) { nbatch = 1; ppb = 1; }
+    dim3 grid((g->np0 + L - 1) / L, g->np2, (nbatch + ppb - 1) / ppb);
+    const int mc = mode == Y_VMUL ? 0 : (mode == Y_DENSITY ? 1 : 2);
+    if (prec == QBG_PREC_FP64) ylaunch<double>(mc, grid, g, nbatch, ppb, v, fac, rho, fgrid, scale, mode, s);
+    else ylaunch<float>(mc, grid, g, nbatch, ppb, v, fac, rho, fgrid, scale, mode, s);
+  }
+  template <typename T>
+  static void ylaunch(int mc, dim3 grid, qbg_grid* g, int nbatch, int ppb, const double* v,
+                      const double* fac, double* rho, double* fgrid, double scale, int mode,
+                      hipStream_t s) {
+#define QBG_KY(MC)                                                                            \
+  hipLaunchKernelGGL((ky<T, N1, N2, MC>), grid, dim3(256), 0, s, (cpx<T>*)g->d_xbuf, g->np0,    \
+                     g->np2, g->nact, g->d_jrow, nbatch, ppb, twp<T>(g, 1), v, fac, rho, fgrid, \
+                     scale, mode)
+    if (mc == 0) QBG_KY(0);
+    else if (mc == 1) QBG_KY(1);
+    else QBG_KY(2);
+#undef QBG_KY
+  }
+};
+
+}  // namespace f4
+}  // namespace qbg

@@ -3,6 +3,7 @@
 // rs_mul_add entry points).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include "internal.h"
@@ -186,7 +187,7 @@ int qbg_timer_stop(qbg_ctx* c, float* ms) {
 
 static void grid_free(qbg_grid* g) {
   void* ptrs[] = {g->d_ip, g->d_im, g->d_tiles, g->d_row_start, g->d_rc_ivec, g->d_rc_x,
-                  g->d_act_row, g->d_tw64[0], g->d_tw64[1], g->d_tw64[2], g->d_tw32[0],
+                  g->d_act_row, g->d_colmap, g->d_jrow, g->d_tw64[0], g->d_tw64[1], g->d_tw64[2], g->d_tw32[0],
                   g->d_tw32[1], g->d_tw32[2], g->d_zbuf, g->d_xbuf, g->d_fac};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -253,6 +254,18 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
   // LDS: two buffers of (lines x n) complex doubles within 64 KiB
   auto lines = [](int n) { return std::max(1, std::min(8, 65536 / (2 * 16 * n))); };
   g->zcols = lines(np2);
+  // four-step kernels where compiled for the axis length (fft4_*.hip);
+  // their z tile: zcols*N + N complex doubles of LDS, zcols*max(n1,n2) threads
+  // (QBG_GENERIC_FFT=1 selects the generic Stockham kernels everywhere)
+  const char* gen = std::getenv("QBG_GENERIC_FFT");
+  const bool use_f4 = !(gen && gen[0] == '1');
+  g->f4[0] = use_f4 ? qbg::f4_ops(np0) : nullptr;
+  g->f4[1] = use_f4 ? qbg::f4_ops(np1) : nullptr;
+  g->f4[2] = use_f4 ? qbg::f4_ops(np2) : nullptr;
+  if (g->f4[2]) {
+    const int m = std::max(g->f4[2]->n1, g->f4[2]->n2);
+    g->zcols = std::min(256 / m, 4096 / np2 - 1);
+  }
   g->xrows = lines(np0);
   g->ytile = lines(np1);
   if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);
@@ -323,13 +336,20 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
         return c->fail(QBG_E_INVAL, "grid: two columns on one grid point");
       }
   }
+  // dense (row, x) -> column map for the four-step x passes
+  std::vector<int32_t> colmap((size_t)g->nact * np0, -1);
+  for (int iv = 0; iv < nvec; iv++) {
+    const int x = z2v[iv] % np0, y = z2v[iv] / np0;
+    colmap[(size_t)jrow[y] * np0 + x] = iv;
+  }
   (void)hipSetDevice(c->device);
   int rc = QBG_OK;
   std::vector<int32_t> vip(ip, ip + ng), vim(im, im + ng);
   if ((rc = upload(c, &g->d_ip, vip)) || (rc = upload(c, &g->d_im, vim)) ||
       (rc = upload(c, &g->d_tiles, tiles)) || (rc = upload(c, &g->d_row_start, row_start)) ||
       (rc = upload(c, &g->d_rc_ivec, rc_ivec)) || (rc = upload(c, &g->d_rc_x, rc_x)) ||
-      (rc = upload(c, &g->d_act_row, act_row))) {
+      (rc = upload(c, &g->d_act_row, act_row)) || (rc = upload(c, &g->d_colmap, colmap)) ||
+      (rc = upload(c, &g->d_jrow, jrow))) {
     grid_free(g);
     delete g;
     return rc;

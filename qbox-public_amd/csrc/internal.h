@@ -56,6 +56,22 @@ struct qbg_tile {
   int col0, ncol, g0, g1;
 };
 
+namespace qbg {
+// compiled four-step kernels of one line length n = n1*n2 (fft4_*.hip);
+// L = lines per block of the x and y passes
+struct F4Ops {
+  int n, n1, n2, L;
+  void (*zbwd)(qbg_grid* g, int prec, int single, const double* c, long pstride, long off2,
+               int last_has2, int nbatch, hipStream_t s);
+  void (*zfwd)(qbg_grid* g, int prec, int single, int accum, double* out, long pstride,
+               long off2, int last_has2, int nbatch, hipStream_t s);
+  void (*xpass)(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s);
+  void (*ypass)(qbg_grid* g, int prec, int mode, int nbatch, int ppb, const double* v,
+                const double* fac, double* rho, double* fgrid, double scale, hipStream_t s);
+};
+const F4Ops* f4_ops(int n);   // nullptr if n has no compiled four-step kernels
+}  // namespace qbg
+
 struct qbg_grid {
   qbg_ctx* ctx = nullptr;
   int np0 = 0, np1 = 0, np2 = 0, ng = 0, nvec = 0;
@@ -74,6 +90,9 @@ struct qbg_grid {
   int32_t* d_rc_ivec = nullptr;     // nvec, columns sorted by (row, x)
   int32_t* d_rc_x = nullptr;        // nvec
   int32_t* d_act_row = nullptr;     // nact: y index of active row r
+  int32_t* d_colmap = nullptr;      // nact*np0: column at (row, x) or -1
+  int32_t* d_jrow = nullptr;        // np1: active row of y or -1
+  const qbg::F4Ops* f4[3] = {nullptr, nullptr, nullptr};   // per axis, or generic
   double* d_tw64[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i k / n)
   float* d_tw32[3] = {nullptr, nullptr, nullptr};
   // workspace for batched pairs (grown on demand)
