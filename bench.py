@@ -1,18 +1,22 @@
 """Benchmark: H*psi band-applies/s of one full Gamma-point evaluation on one
-MI355X, with the roofline of the dominant kernel and the CPU oracle timed on
-the host's cores beside it.
+MI355X, mixed precision beside FP64 in the same run, with the rooflines of
+the dominant kernels and the CPU oracle timed on the host's cores beside it.
 
 One step = ChargeDensity::update_density -> EnergyFunctional::update_vhxc ->
 EnergyFunctional::energy(compute_hpsi=true, compute_forces=true), the
 sequence of src/BOSampleStepper.cpp:1207-1217, on synthetic inputs already
-resident in HBM.  value = bands / (seconds per step).
+resident in HBM.  value = bands / (seconds per step) of the mixed path
+(FP32 band FFTs and projector GEMMs, FP64 reductions); the FP64 path runs
+the same steps on the same inputs, and the mixed path's error against it is
+reported per energy term (Ha/atom), for forces (Ha/bohr) and for H*psi.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
-                       [--prec fp64] [--cpu-seconds S | --no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+                       [--prec mixed|fp64] [--cpu-seconds S | --no-cpu]
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -25,22 +29,27 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "qbox-public_amd"))
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+FFT_KINDS = ("zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd")
+# band-pair passes of each kind in one step: density (z, x backward, y
+# density) and H*psi (z, x backward, y v-multiply, x, z forward)
+PASSES_PER_STEP = {"zbwd": 2, "xbwd": 2, "ydens": 1, "yvmul": 1, "xfwd": 1, "zfwd": 1}
+TERMS = ["ekin", "eps", "enl", "ehart", "exc", "esr", "etotal"]
 
 
-def kernel_bytes(ef, kind, npairs_per_launch, sc):
-    """Algorithmic HBM bytes of one launch of a pass kernel.
+def pair_bytes(ef, kind, sc):
+    """Algorithmic HBM bytes of one band pair through a pass kernel.
 
-    Per band pair (canonical fused schedule): z-bwd reads two bands (2 ngw
-    complex FP64) and writes the columns (nvec np2 S_c); x-bwd reads the
-    columns and writes the active rows A = 2 ntrans0 np0 np2; the y pass
-    reads and writes A and reads v (N doubles); x-fwd and z-fwd mirror the
-    backward passes, z-fwd read-modify-writes the two H*psi bands.
+    z-bwd reads two bands (2 ngw complex FP64) and writes the columns
+    (nvec np2 elements of sc bytes); x-bwd reads the columns and writes the
+    active rows A = 2 ntrans0 np0 np2; the y pass reads and writes A and
+    reads v (N doubles); x-fwd and z-fwd mirror the backward passes, z-fwd
+    read-modify-writes the two H*psi bands.
     """
     ngw, nvec, np2 = ef.ngw, ef.nvec, ef.np2
     cols = nvec * np2 * sc
     A = ef.nact * ef.np0 * ef.np2 * sc
-    per_pair = {
+    return {
         "zbwd": 2 * ngw * 16 + cols,
         "xbwd": cols + A,
         "yvmul": 2 * A + ef.N * 8,
@@ -48,28 +57,77 @@ def kernel_bytes(ef, kind, npairs_per_launch, sc):
         "xfwd": A + cols,
         "zfwd": cols + 2 * 2 * ngw * 16,
     }[kind]
-    extra = 2 * ef.N * 8 if kind == "ydens" else 0     # rho read-modify-write per launch
-    return per_pair * npairs_per_launch + extra
+
+
+def fft_roofline(ef, kind, kt, steps, nst, sc):
+    tot_ms, nlaunch = kt[kind]
+    launches_per_step = nlaunch / steps
+    npairs = (nst + 1) // 2
+    pairs_per_launch = npairs * PASSES_PER_STEP[kind] / launches_per_step
+    per_launch = pair_bytes(ef, kind, sc) * pairs_per_launch
+    if kind == "ydens":
+        per_launch += 2 * ef.N * 8                 # rho read-modify-write
+    avg_ms = tot_ms / nlaunch
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes_per_launch": int(per_launch),
+            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches_per_step}
+
+
+def gemm_roofline(ef, kt, nst, prec):
+    tot_ms, nlaunch = kt["gemm"]
+    flops = 2.0 * ef.nproj * 2 * ef.ngw * nst
+    avg_ms = tot_ms / nlaunch
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    peak = MFMA_PEAK_TFLOPS[prec]
+    return {"kernel": "gemm", "bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(avg_ms, 4)}
 
 
 def cpu_baseline(sysdef, c, seconds):
-    """Time the FP64 CPU oracle (OpenMP) on the same workload: repeat the
-    full evaluation until about `seconds` of work is done (at least once)."""
+    """Time the FP64 CPU oracle (OpenMP) on a bounded sample of the same
+    workload: the full evaluation of the same system with the first nb bands
+    (nb doubled from 4 until one evaluation takes about `seconds`)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
-    o = oracle.Oracle(sysdef)
-    t0 = time.perf_counter()
-    reps = 0
+    nb = 4
     while True:
-        o.evaluate(c, hpsi=True, forces=True)
-        reps += 1
-        if time.perf_counter() - t0 > seconds:
+        sub = dataclasses.replace(sysdef, nst=nb)
+        o = oracle.Oracle(sub)
+        t0 = time.perf_counter()
+        o.evaluate(np.ascontiguousarray(c[:nb]), hpsi=True, forces=True)
+        dt = time.perf_counter() - t0
+        if dt > 0.5 * seconds or 2 * nb > sysdef.nst:
             break
-    dt = (time.perf_counter() - t0) / reps
+        nb *= 2
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": sysdef.nst / dt, "unit": "band-applies/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} full FP64 evaluation(s) of the same workload on the CPU oracle, "
-                      f"{dt:.2f} s each"}
+    return {"value": round(nb / dt, 3), "unit": "band-applies/s", "cores": cores, "kind": "port",
+            "sample": f"one full FP64 evaluation (density, vhxc, energy, H*psi, forces) of the "
+                      f"same system restricted to its first {nb} bands on the CPU oracle, "
+                      f"{dt:.2f} s"}
+
+
+def run(ef, dc, hp, prec, steps, warmup, ctx):
+    import qbg
+    for _ in range(warmup):
+        ef.evaluate(dc, hp, forces=True, prec=prec)
+    ctx.sync()
+    prof = qbg.Profiler(ctx)
+    prof.reset()
+    prof.enable(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    res = None
+    for _ in range(steps):
+        res = ef.evaluate(dc, hp, forces=True, prec=prec)
+    gpu_ms = ctx.timer_stop()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    prof.enable(False)
+    return res, wall, gpu_ms, prof.read()
 
 
 def main():
@@ -77,9 +135,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--prec", default="fp64", choices=["fp64"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--prec", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB three times (counter calibration)")
@@ -113,57 +171,58 @@ def main():
     dc = ctx.to_device(c)
     qbg.gram(ctx, dc, ef.ngw, ef.ngw, sysdef.nst)        # orthonormalize on the device
     hp = ctx.array(c.shape, np.complex128)
-    # active x rows of the wavefunction grid (for the byte model)
     ky = b["idx"][:, 1]
-    ef.nact = 2 * (int(np.abs(ky).max()) + 1)
+    ef.nact = 2 * (int(np.abs(ky).max()) + 1)            # active rows (byte model)
+    nst, nat = sysdef.nst, sysdef.nat
+    print(f"[bench] {sysdef.name}: grid {ef.np0}x{ef.np1}x{ef.np2} ngw {ef.ngw} "
+          f"nproj {ef.nproj}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        ef.evaluate(dc, hp, forces=True, prec=args.prec)
-    ctx.sync()
-    prof = qbg.Profiler(ctx)
-    prof.reset()
-    prof.enable(True)
+    # FP64 path
+    r64, wall64, gpu64, kt64 = run(ef, dc, hp, "fp64", args.steps, args.warmup, ctx)
+    h64 = hp.download()
+    print(f"[bench] fp64 {1e3 * wall64 / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+    out_prec = args.prec
+    if args.prec == "mixed":
+        res, wall, gpu_ms, kt = run(ef, dc, hp, "fp32", args.steps, args.warmup, ctx)
+        hmx = hp.download()
+        print(f"[bench] mixed {1e3 * wall / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+        err = {t: abs(res[t] - r64[t]) / nat for t in TERMS}
+        errors = {
+            "dE_per_atom": {t: float("%.3e" % v) for t, v in err.items()},
+            "max_dE_per_atom": float("%.3e" % max(err.values())),
+            "force_max_abs_err": float("%.3e" % np.abs(res["fion"] - r64["fion"]).max()),
+            "hpsi_rel_err": float("%.3e" % (np.abs(hmx - h64).max() / np.abs(h64).max())),
+            "reference": "the FP64 GPU path of the same run (itself matched to the CPU "
+                         "oracle to 1e-10 by the parity tests)",
+        }
+        del hmx
+    else:
+        res, wall, gpu_ms, kt = r64, wall64, gpu64, kt64
+        errors = None
+    del h64
+
+    # H*psi-only rate: energy with forces off, potential fixed
+    hprec = "fp32" if args.prec == "mixed" else "fp64"
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.timer_start()
-    res = None
     for _ in range(args.steps):
-        res = ef.evaluate(dc, hp, forces=True, prec=args.prec)
-    gpu_ms = ctx.timer_stop()
+        ef.energy(dc, hp, forces=False, stress=False, prec=hprec)
     ctx.sync()
-    wall = time.perf_counter() - t0
-    prof.enable(False)
-    kt = prof.read()
+    hpsi_only = nst / ((time.perf_counter() - t0) / args.steps)
 
     ms_per_step = 1e3 * wall / args.steps
-    value = sysdef.nst / (wall / args.steps)
-
-    # dominant kernel and its roofline
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    name, (tot_ms, nlaunch) = dom
-    npairs = (sysdef.nst + 1) // 2
-    sc = 16
-    if name in ("zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd"):
-        # launches of this kind per step that run the band pairs
-        per_step = nlaunch / args.steps
-        pairs_per_launch = npairs if name in ("yvmul", "ydens") else None
-        if pairs_per_launch is None:
-            # density and H*psi each run one band-pair launch; vbasis single
-            # transforms are small and counted with the same model
-            pairs_per_launch = npairs
-        traffic_bytes = kernel_bytes(ef, name, pairs_per_launch, sc)
-        avg_ms = tot_ms / nlaunch
-        achieved = traffic_bytes / (avg_ms * 1e-3) / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "algorithmic_bytes_per_launch": int(traffic_bytes),
-                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_step}
-    else:
-        roof = {"kernel": name, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": None, "traffic": None}
+    value = nst / (wall / args.steps)
+    value64 = nst / (wall64 / args.steps)
+    gprec = "fp32" if args.prec == "mixed" else "fp64"
+    sc = 8 if args.prec == "mixed" else 16
+    fft_kind = max((k for k in FFT_KINDS if k in kt), key=lambda k: kt[k][0])
+    froof = fft_roofline(ef, fft_kind, kt, args.steps, nst, sc)
+    groof = gemm_roofline(ef, kt, nst, gprec)
+    roof = froof if kt[fft_kind][0] >= kt["gemm"][0] else groof
 
     out = {
-        "metric": "H*psi band-applies/s (density + update_vhxc + energy with H*psi and forces)",
+        "metric": "H*psi band-applies/s, mixed vs FP64 on 1 GPU at |dE|<=1e-6 Ha/atom "
+                  "(density + update_vhxc + energy with H*psi and forces)",
         "value": round(value, 2),
         "unit": "band-applies/s",
         "n_gpus": 1,
@@ -173,23 +232,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.prec == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": sysdef.name, "atoms": sysdef.nat, "bands": sysdef.nst,
+        "config": {"workload": sysdef.name, "atoms": nat, "bands": nst,
                    "ecut_ry": sysdef.ecut_ry, "grid": [ef.np0, ef.np1, ef.np2], "ngw": ef.ngw,
-                   "precision": args.prec},
+                   "nproj": ef.nproj, "precision": out_prec},
+        "fp64_value": round(value64, 2),
+        "fp64_ms_per_step": round(1e3 * wall64 / args.steps, 3),
+        "mixed_over_fp64": round(value / value64, 3),
+        "hpsi_only_value": round(hpsi_only, 2),
+        "errors_vs_fp64": errors,
         "gpu_event_ms_per_step": round(gpu_ms / args.steps, 3),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt64.items()},
         "etotal": res["etotal"],
         "roofline": roof,
+        "fft_roofline": froof,
+        "gemm_roofline": groof,
+        "fp64_gemm_roofline": gemm_roofline(ef, kt64, nst, "fp64"),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sysdef, c_host_orthonormal(dc), args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(sysdef, dc.download(), args.cpu_seconds)
     print(json.dumps(out))
-
-
-def c_host_orthonormal(dc):
-    return dc.download()
 
 
 if __name__ == "__main__":

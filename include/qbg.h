@@ -73,10 +73,13 @@ int qbg_timer_stop(qbg_ctx* ctx, float* ms);
 #define QBG_K_XFWD 4      /* x pass forward (+ column gather) */
 #define QBG_K_ZFWD 5      /* z pass forward (+ unpack / accumulate) */
 #define QBG_K_YOTHER 6    /* y pass, full-grid transforms */
-#define QBG_K_NL 7        /* projector GEMMs and their generation */
+#define QBG_K_NL 7        /* projector generation, D contraction, force sums */
 #define QBG_K_GRAM 8      /* overlap, Cholesky, triangular update */
 #define QBG_K_OTHER 9     /* reductions, pointwise, structure factor */
-#define QBG_K_COUNT 10
+#define QBG_K_VGRID 10    /* single transforms on the density basis */
+#define QBG_K_GEMM 11     /* one projector GEMM (anl^T c, anl fout or c W):
+                             2 * nproj * 2ngw * nst flops each */
+#define QBG_K_COUNT 12
 /* A plain device-to-device streaming copy of `bytes` (a multiple of 16):
  * the reference kernel of known traffic for calibrating HBM byte
  * counters.  No counterpart in the reference. */

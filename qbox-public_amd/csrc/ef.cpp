@@ -309,8 +309,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
   e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)((ngw + 255) / 256) * 8);
-  // split-K partials: at most ~2048 tiles of 64x64 plus one spare split
-  e->work_doubles = (size_t)2048 * 4096 + (size_t)2 * 4 * e->M * nst;
+  // split-K partials of the fnl product: up to 40 splits of M x nst
+  e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
   if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
       (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
       (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
@@ -334,8 +334,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
       (rc = dalloc(c, &e->d_rhogt, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_vlocal, (size_t)2 * ngv)) ||
       (rc = dalloc(c, &e->d_fesr, 3 * e->nat)) || (rc = dalloc(c, &e->d_fion, 3 * e->nat)) ||
       (rc = dalloc(c, &e->d_scal, S_COUNT)) || (rc = dalloc(c, &e->d_partial, e->partial_doubles)) ||
-      (rc = dalloc(c, &e->d_anl, (size_t)4 * e->M * 2 * ngw)) ||
-      (rc = dalloc(c, &e->d_fnl, (size_t)4 * e->M * nst)) ||
+      (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngw)) ||
+      (rc = dalloc(c, &e->d_fnl, (size_t)2 * e->M * nst)) ||
       (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
       (rc = dalloc(c, &e->d_work, e->work_doubles))) {
     ef_free(e);
@@ -405,13 +405,13 @@ int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
   qbg_grid* g = e->gv;
   if ((rc = grid_ensure_ws(g, 1))) return rc;
   int r1, r2, r3;
-  ctx->prof_begin(QBG_K_YOTHER);
+  ctx->prof_begin(QBG_K_VGRID);
   r1 = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, e->d_rhor, nullptr, nullptr, nullptr, e->omega, s);
   ctx->prof_end();
-  ctx->prof_begin(QBG_K_XFWD);
+  ctx->prof_begin(QBG_K_VGRID);
   r2 = r1 ? r1 : xfwd(g, QBG_PREC_FP64, 1, s);
   ctx->prof_end();
-  ctx->prof_begin(QBG_K_ZFWD);
+  ctx->prof_begin(QBG_K_VGRID);
   r3 = r2 ? r2 : zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, e->d_rhog, 0, 0, 0, 1, s);
   ctx->prof_end();
   if (r3) return ctx->fail(r3, "update_density: density-grid transform refused");
@@ -439,13 +439,13 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
   int rc = grid_ensure_ws(g, 1);
   if (rc) return rc;
   int r1, r2, r3;
-  ctx->prof_begin(QBG_K_ZBWD);
+  ctx->prof_begin(QBG_K_VGRID);
   r1 = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_vlocal, 0, 0, 0, 1, s);
   ctx->prof_end();
-  ctx->prof_begin(QBG_K_XBWD);
+  ctx->prof_begin(QBG_K_VGRID);
   r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
   ctx->prof_end();
-  ctx->prof_begin(QBG_K_YOTHER);
+  ctx->prof_begin(QBG_K_VGRID);
   r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr, e->d_vr,
                        1.0, s);
   ctx->prof_end();
@@ -469,7 +469,7 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
   if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
   if (compute_stress) return ctx->fail(QBG_E_INVAL, "stress: nonlocal stress not available");
-  if (prec != QBG_PREC_FP64) return ctx->fail(QBG_E_INVAL, "energy: only FP64 projectors so far");
+  if (prec != QBG_PREC_FP64 && prec != QBG_PREC_FP32) return ctx->fail(QBG_E_INVAL, "energy: prec");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   const int ngw = e->wb.size, nst = e->nst, M = e->M;
@@ -478,30 +478,61 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
                  e->d_partial, e->d_scal + S_KIN, s);
   ctx->prof_end();
-  // nonlocal: anl (+ three -i G_j anl blocks), fnl = 2 anl^T c - anl0 c0^T
+  // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
+  // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
+  // the GEMMs run on FP32 MFMA
   const long K = 2L * ngw;
   ctx->prof_begin(QBG_K_NL);
+  const bool f32 = prec == QBG_PREC_FP32;
+  float* anl32 = reinterpret_cast<float*>(e->d_anl);
   launch_anl(ngw, M, 1, K, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l,
-             e->d_twnl, e->d_anl, 0, s);
-  if (gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M, e->d_work,
-                     e->work_doubles, s))
-    return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
+             e->d_twnl, e->d_anl, f32 ? 1 : 0, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_GEMM);
+  const int grc = f32 ? gemm_tn_splitk(anl32, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M,
+                                       e->d_work, e->work_doubles, prec, s)
+                      : gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl,
+                                       M, e->d_work, e->work_doubles, prec, s);
+  ctx->prof_end();
+  if (grc) return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
+  ctx->prof_begin(QBG_K_NL);
   launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
                   e->d_scal + S_ENL, s);
-  if (compute_hpsi) gemm_nn(e->d_anl, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, s);
+  ctx->prof_end();
+  if (compute_hpsi) {
+    ctx->prof_begin(QBG_K_GEMM);
+    if (f32) gemm_nn(anl32, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, prec, s);
+    else gemm_nn(e->d_anl, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, prec, s);
+    ctx->prof_end();
+  }
   if (compute_forces) {
-    // Y = c W with W[n,m] = occ_n fout[m,n] (stored in the second anl block),
+    // Y = c W with W[n,m] = occ_n fout[m,n] (stored in the second fnl block),
     // then f_a,j = -4 sum_{m in a} sum_K (-i G_j anl)[K,m] Y[K,m]
     double* W = e->d_fnl + (size_t)M * nst;
-    double* Y = e->d_anl + (size_t)M * K;
+    ctx->prof_begin(QBG_K_NL);
     QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
     launch_force_w(e->d_fout, e->d_occ, M, nst, W, s);
-    gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, s);
-    launch_nl_forces_y(e->d_anl, Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr, e->d_a_stride,
-                       e->nat, e->d_partial, 64, e->d_fion, s);
+    ctx->prof_end();
+    if (f32) {
+      float* Y = anl32 + (size_t)M * K;
+      ctx->prof_begin(QBG_K_GEMM);
+      gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, prec, s);
+      ctx->prof_end();
+      ctx->prof_begin(QBG_K_NL);
+      launch_nl_forces_y(anl32, (const float*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
+                         e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+    } else {
+      double* Y = e->d_anl + (size_t)M * K;
+      ctx->prof_begin(QBG_K_GEMM);
+      gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, prec, s);
+      ctx->prof_end();
+      ctx->prof_begin(QBG_K_NL);
+      launch_nl_forces_y(e->d_anl, (const double*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
+                         e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+    }
+    ctx->prof_end();
   }
-  ctx->prof_end();
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
   if (compute_hpsi) {
     int rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, prec);
@@ -600,7 +631,8 @@ int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
   };
   c->prof_begin(QBG_K_GRAM);
   cleanup(cc, ldc, nst, s);
-  if (gemm_tn_splitk(cc, 2L * ldc, cc, 2L * ldc, nst, nst, K, 2.0, -1.0, S, nst, W, work, s)) {
+  if (gemm_tn_splitk(cc, 2L * ldc, cc, 2L * ldc, nst, nst, K, 2.0, -1.0, S, nst, W, work,
+                     QBG_PREC_FP64, s)) {
     (void)hipStreamSynchronize(s);
     release();
     return c->fail(QBG_E_INVAL, "gram workspace");
@@ -619,7 +651,7 @@ int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
   }
   c->prof_begin(QBG_K_GRAM);
   trtri(S, nst, nst, X, s);
-  gemm_nn(cc, 2L * ldc, X, nst, K, nst, nst, T, K, 0, s);
+  gemm_nn(cc, 2L * ldc, X, nst, K, nst, nst, T, K, 0, QBG_PREC_FP64, s);
   (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
                          nst, hipMemcpyDeviceToDevice, s);
   c->prof_end();

@@ -44,16 +44,28 @@ void launch_nl_forces(const double* fout, const double* dF, int M, int nst, cons
 
 void launch_force_w(const double* fout, const double* occ, int M, int nst, double* W,
                     hipStream_t s);
-void launch_nl_forces_y(const double* A, const double* Y, long lda, int ngw, const double* gx,
+template <typename T>
+void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double* gx,
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
                         int nat, double* partial, int nchunk, double* fion, hipStream_t s);
 
-// gemm.hip
+// gemm.hip.  prec (QBG_PREC_FP64 / QBG_PREC_FP32) selects the MFMA compute
+// type; operands of either type are converted as they are staged.
+// TN split-K: out[m + n*ldo] = alpha sum_k A[k + m*lda] B[k + n*ldb]
+//                              + gamma A[m*lda] B[n*ldb]   (partials in FP64)
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
-                   size_t work_doubles, hipStream_t s);
+                   size_t work_doubles, int prec, hipStream_t s);
+int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
+                   double alpha, double gamma, double* out, long ldo, double* work,
+                   size_t work_doubles, int prec, hipStream_t s);
+// NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf]  (K rows, N cols, M sum)
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
-             double* C, long ldc, int accumulate, hipStream_t s);
+             double* C, long ldc, int accumulate, int prec, hipStream_t s);
+void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N, int M,
+             double* C, long ldc, int accumulate, int prec, hipStream_t s);
+void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
+             float* C, long ldc, int accumulate, int prec, hipStream_t s);
 void potrf(double* S, int n, int lds, int* info, hipStream_t s);
 void trtri(const double* L, int n, int ldl, double* X, hipStream_t s);
 void cleanup(double* c, long ldc, int nst, hipStream_t s);

@@ -503,9 +503,11 @@ void launch_force_w(const double* fout, const double* occ, int M, int nst, doubl
 }
 
 // partial[(a*nchunk + chunk)*3 + j] = sum over the chunk's G and the atom's
-// projectors of G_j (a_im Y_re - a_re Y_im)
+// projectors of G_j (a_im Y_re - a_re Y_im), accumulated in FP64 for FP32
+// operands too
+template <typename T>
 __global__ __launch_bounds__(256) void k_nl_forces_y(
-    const double* __restrict__ A, const double* __restrict__ Y, long lda, int ngw,
+    const T* __restrict__ A, const T* __restrict__ Y, long lda, int ngw,
     const double* __restrict__ gx, const int32_t* __restrict__ a_m0,
     const int32_t* __restrict__ a_npr, const int32_t* __restrict__ a_stride, int chunk,
     double* __restrict__ partial) {
@@ -516,8 +518,8 @@ __global__ __launch_bounds__(256) void k_nl_forces_y(
   for (int ip = 0; ip < npr; ip++) {
     const long o = (long)(m0 + ip * st) * lda;
     for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
-      const double ar = A[o + 2 * ig], ai = A[o + 2 * ig + 1];
-      const double yr = Y[o + 2 * ig], yi = Y[o + 2 * ig + 1];
+      const double ar = (double)A[o + 2 * ig], ai = (double)A[o + 2 * ig + 1];
+      const double yr = (double)Y[o + 2 * ig], yi = (double)Y[o + 2 * ig + 1];
       const double t = ai * yr - ar * yi;
       v[0] += gx[ig] * t;
       v[1] += gx[ngw + ig] * t;
@@ -537,14 +539,21 @@ __global__ void k_nl_forces_final(const double* __restrict__ partial, int nchunk
   fion[t] -= 4.0 * s;
 }
 
-void launch_nl_forces_y(const double* A, const double* Y, long lda, int ngw, const double* gx,
+template <typename T>
+void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double* gx,
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
                         int nat, double* partial, int nchunk, double* fion, hipStream_t s) {
   const int chunk = (ngw + nchunk - 1) / nchunk;
-  hipLaunchKernelGGL(k_nl_forces_y, dim3(nchunk, nat), dim3(256), 0, s, A, Y, lda, ngw, gx, a_m0,
+  hipLaunchKernelGGL(k_nl_forces_y<T>, dim3(nchunk, nat), dim3(256), 0, s, A, Y, lda, ngw, gx, a_m0,
                      a_npr, a_stride, chunk, partial);
   hipLaunchKernelGGL(k_nl_forces_final, dim3((3 * nat + 255) / 256), dim3(256), 0, s, partial,
                      nchunk, nat, fion);
 }
+template void launch_nl_forces_y<double>(const double*, const double*, long, int, const double*,
+                                         const int32_t*, const int32_t*, const int32_t*, int,
+                                         double*, int, double*, hipStream_t);
+template void launch_nl_forces_y<float>(const float*, const float*, long, int, const double*,
+                                        const int32_t*, const int32_t*, const int32_t*, int,
+                                        double*, int, double*, hipStream_t);
 
 }  // namespace qbg

@@ -538,7 +538,7 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
                    (mode == Y_BWD_REAL_ADD && (!v || !fgrid)) || (mode == Y_FWD_REAL && !v);
   if (bad) return QBG_E_INVAL;
   if (g->f4[1]) {
-    const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? 4 : 1);
+    const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? std::min(g->yppb, nbatch) : 1);
     g->f4[1]->ypass(g, prec, mode, nbatch, ppb, v, fac, rho, fgrid, scale, s);
     return launched();
   }

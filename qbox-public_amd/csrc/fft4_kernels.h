@@ -151,20 +151,25 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
                                               cpx<T>* __restrict__ xbuf, int np2, int nvec,
-                                              int nact, int ngroups,
+                                              int nact, int ngroups, int gpb,
                                               const int32_t* __restrict__ colmap,
                                               const cpx<T>* __restrict__ tw_g) {
   constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
   __shared__ cpx<T> ex[L * NP];
   __shared__ cpx<T> tw[N];
   const int tid = threadIdx.x;
-  const int k = blockIdx.x / ngroups;
-  const int r0 = (blockIdx.x - k * ngroups) * L;
-  const int nr = min(L, nact - r0);
+  // block = (plane k, run of gpb row groups): the runs of one plane share its
+  // column data through the CU's cache
+  const int nruns = (ngroups + gpb - 1) / gpb;
+  const int k = blockIdx.x / nruns;
+  const int gr0 = (blockIdx.x - k * nruns) * gpb, gr1 = min(ngroups, gr0 + gpb);
   const int pair = blockIdx.y;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  __syncthreads();
   const cpx<T>* zp = zbuf + ((long)pair * np2 + k) * nvec;
+  for (int gr = gr0; gr < gr1; gr++) {
+  const int r0 = gr * L;
+  const int nr = min(L, nact - r0);
+  __syncthreads();                       // tw loaded; ex free for this group
   {
     const int n2 = tid % N2, line = tid / N2;
     if (line < nr) {
@@ -196,25 +201,29 @@ __global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
       for (int k2 = 0; k2 < N2; k2++) xp[k1 + N1 * k2] = b[k2];
     }
   }
+  }
 }
 
 // ------------------------------------------------------------------ x-fwd
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
                                               cpx<T>* __restrict__ zbuf, int np2, int nvec,
-                                              int nact, int ngroups,
+                                              int nact, int ngroups, int gpb,
                                               const int32_t* __restrict__ colmap,
                                               const cpx<T>* __restrict__ tw_g) {
   constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
   __shared__ cpx<T> ex[L * NP];
   __shared__ cpx<T> tw[N];
   const int tid = threadIdx.x;
-  const int k = blockIdx.x / ngroups;
-  const int r0 = (blockIdx.x - k * ngroups) * L;
-  const int nr = min(L, nact - r0);
+  const int nruns = (ngroups + gpb - 1) / gpb;
+  const int k = blockIdx.x / nruns;
+  const int gr0 = (blockIdx.x - k * nruns) * gpb, gr1 = min(ngroups, gr0 + gpb);
   const int pair = blockIdx.y;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  __syncthreads();
+  for (int gr = gr0; gr < gr1; gr++) {
+  const int r0 = gr * L;
+  const int nr = min(L, nact - r0);
+  __syncthreads();                       // tw loaded; ex free for this group
   {
     const int k1 = tid % N1, line = tid / N1;
     if (line < nr) {
@@ -246,6 +255,7 @@ __global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
         if (iv >= 0) zp[iv] = cv[m2];
       }
     }
+  }
   }
 }
 
@@ -449,25 +459,27 @@ struct Launch {
 
   static void xpass(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s) {
     const int ngroups = (g->nact + L - 1) / L;
-    dim3 grid(g->np2 * ngroups, nbatch);
+    const int gpb = g->xgpb > 0 ? std::min(g->xgpb, ngroups) : ngroups;
+    const int nruns = (ngroups + gpb - 1) / gpb;
+    dim3 grid(g->np2 * nruns, nbatch);
     if (prec == QBG_PREC_FP64) {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(256), 0, s,
                            (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2, g->nvec,
-                           g->nact, ngroups, g->d_colmap, twp<double>(g, 0));
+                           g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(256), 0, s,
                            (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2, g->nvec,
-                           g->nact, ngroups, g->d_colmap, twp<double>(g, 0));
+                           g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
     } else {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(256), 0, s,
                            (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2, g->nvec,
-                           g->nact, ngroups, g->d_colmap, twp<float>(g, 0));
+                           g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(256), 0, s,
                            (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2, g->nvec,
-                           g->nact, ngroups, g->d_colmap, twp<float>(g, 0));
+                           g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
     }
   }
 

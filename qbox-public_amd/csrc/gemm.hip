@@ -1,188 +1,260 @@
-// gemm.hip -- FP64 MFMA GEMMs and the small dense factorizations of the
+// gemm.hip -- MFMA GEMMs and the small dense factorizations of the
 // projector path (NonLocalPotential::energy dgemm calls,
 // src/NonLocalPotential.cpp:1411-1428, 1611, 1716) and of SlaterDet::gram
 // (syrk/syr/potrf/trsm, src/SlaterDet.cpp:487-520).
 //
-// v_mfma_f64_16x16x4f64 on a 64-wide wave: lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; the 16x16 result is held register-major: accumulator r
-// of lane l is D[4*r + l/16][l%16], r<4.
+// One kernel shape serves every product: D[p][q] = sum_r P(r,p) Q(r,q) on a
+// 128x128 (p x q) tile per block of 256 threads (2x2 waves of 64x64, each a
+// 4x4 grid of 16x16 MFMA blocks), the reduction staged through LDS in
+// chunks of KC with the next chunk prefetched into registers.  P is read
+// along r (P(r,p) = P[r + p*ldp]); Q along r (TN form, Q[r + q*ldq]) or
+// along q (NN form, Q[q + r*ldq]).  q is the contiguous index of the
+// output, so the 16 lanes of an MFMA column group store 16 consecutive
+// elements.  The compute type is FP64 (v_mfma_f64_16x16x4f64) or FP32
+// (v_mfma_f32_16x16x4f32); operands are converted while they are staged.
+//
+// MFMA lane layouts on a 64-wide wave (16x16x4): lane l holds A[l%16][l/16]
+// and B[l/16][l%16]; accumulator e of lane l is D[4e + l/16][l%16] for f64
+// and D[4(l/16) + e][l%16] for f32.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "energy.h"
+#include "qbg.h"
 
 namespace qbg {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+template <typename TC>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef v4d acc;
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int e, int lane) { return 4 * e + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc;
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int e, int lane) { return 4 * (lane >> 4) + e; }
+};
 
-constexpr int BT = 64;    // output tile edge
-constexpr int KC = 16;    // reduction chunk
-constexpr int LP = BT + 2;   // LDS row pitch (doubles)
+constexpr int TP = 128;       // output tile rows (p)
+constexpr int TQ = 128;       // output tile columns (q)
+constexpr int KC = 16;        // reduction chunk
+constexpr int PAD = 16;       // LDS row padding (elements)
 
-// ------------------------------------------------------------- TN split-K
-// partial[s][n][m] = sum_{k in split s} A[k + m*lda] B[k + n*ldb]
-__global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, long lda,
-                                                 const double* __restrict__ B, long ldb, int M,
-                                                 int N, long K, long kper,
-                                                 double* __restrict__ partial) {
-  __shared__ double As[KC][LP];
-  __shared__ double Bs[KC][LP];
-  const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
-  const long kb = (long)blockIdx.z * kper;
-  const long ke = min(K, kb + kper);
+struct GemmArgs {
+  const void* P;
+  long ldp;
+  const void* Q;
+  long ldq;
+  int np, nq;       // output extent
+  long nr;          // reduction extent
+  long rper;        // reduction length of one split (multiple of KC)
+  int tp, tq;       // tiles along p and q
+  int pfast;        // block order: p tiles fastest (1) or q tiles fastest (0)
+  void* out;        // out[split*sstride + q + p*ldo]
+  long ldo, sstride;
+  int accumulate;
+};
+
+template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  typedef Mfma<TC> MF;
+  __shared__ TC Ps[KC][TP + PAD];
+  __shared__ TC Qs[KC][TQ + PAD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-  v4d acc[2][2];
+  // blocks are dealt to the 8 XCDs round robin: renumber so that runs of
+  // consecutive tiles (which share operand panels) stay on one XCD's L2
+  const int nblk = gridDim.x;
+  int b = blockIdx.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int ntile = g.tp * g.tq;
+  const int split = b / ntile, tt = b - split * ntile;
+  int ip, iq;
+  if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
+  else { iq = tt % g.tq; ip = tt / g.tq; }
+  const int p0 = ip * TP, q0 = iq * TQ;
+  const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
+  const TPm* __restrict__ Pp = (const TPm*)g.P;
+  const TQm* __restrict__ Qp = (const TQm*)g.Q;
+  TC pv[8], qv[8];
+  auto load = [&](long r0) {
+    const long r = r0 + (tid & 15);
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = v4d{0, 0, 0, 0};
-  // loader: column c = tid/4 of the tile, 4 consecutive k at kq
-  const int lc = tid >> 2, kq = (tid & 3) * 4;
-  const bool am = (m0 + lc) < M, bn = (n0 + lc) < N;
-  const double* Ap = A + (long)(m0 + lc) * lda;
-  const double* Bp = B + (long)(n0 + lc) * ldb;
-  for (long k0 = kb; k0 < ke; k0 += KC) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const long k = k0 + kq + q;
-      As[kq + q][lc] = (am && k < ke) ? Ap[k] : 0.0;
-      Bs[kq + q][lc] = (bn && k < ke) ? Bp[k] : 0.0;
+    for (int j = 0; j < 8; j++) {
+      const int p = p0 + (tid >> 4) + 16 * j;
+      pv[j] = (p < g.np && r < re) ? (TC)Pp[r + (long)p * g.ldp] : TC(0);
     }
+    if (QR) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int q = q0 + (tid >> 4) + 16 * j;
+        qv[j] = (q < g.nq && r < re) ? (TC)Qp[r + (long)q * g.ldq] : TC(0);
+      }
+    } else {
+      const int q = q0 + (tid & 127);
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const long rr = r0 + (tid >> 7) + 2 * j;
+        qv[j] = (q < g.nq && rr < re) ? (TC)Qp[q + rr * g.ldq] : TC(0);
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < 8; j++) Ps[tid & 15][(tid >> 4) + 16 * j] = pv[j];
+    if (QR) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) Qs[tid & 15][(tid >> 4) + 16 * j] = qv[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) Qs[(tid >> 7) + 2 * j][tid & 127] = qv[j];
+    }
+  };
+  const int wp = (w & 1) * 64, wq = (w >> 1) * 64;
+  typename MF::acc acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = typename MF::acc{0, 0, 0, 0};
+  if (rb < re) load(rb);
+  for (long r0 = rb; r0 < re; r0 += KC) {
+    store();
     __syncthreads();
+    if (r0 + KC < re) load(r0 + KC);
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int kr = kk + (lane >> 4), c = lane & 15;
-      const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
-      const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
-      acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-      acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-      acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-      acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+      TC a[4], bq[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = Ps[kr][wp + 16 * i + c];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bq[j] = Qs[kr][wq + 16 * j + c];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = MF::mma(a[i], bq[j], acc[i][j]);
     }
     __syncthreads();
   }
-  double* P = partial + (long)blockIdx.z * M * N;
+  TO* o = (TO*)g.out + (long)split * g.sstride;
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < 4; j++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = m0 + wm + i * 16 + 4 * r + (lane >> 4);
-        const int n = n0 + wn + j * 16 + (lane & 15);
-        if (m < M && n < N) P[(long)n * M + m] = acc[i][j][r];
-      }
-}
-
-// out[m + n*ldo] = alpha * sum_s partial[s][n][m] + gamma * A[m*lda] * B[n*ldb]
-__global__ __launch_bounds__(256) void k_splitk_reduce(const double* __restrict__ partial,
-                                                       int S, int M, int N, double alpha,
-                                                       double gamma, const double* __restrict__ A,
-                                                       long lda, const double* __restrict__ B,
-                                                       long ldb, double* __restrict__ out,
-                                                       long ldo) {
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)M * N) return;
-  const int m = (int)(t % M), n = (int)(t / M);
-  double s = 0.0;
-  for (int q = 0; q < S; q++) s += partial[(long)q * M * N + t];
-  double v = alpha * s;
-  if (gamma != 0.0) v += gamma * A[(long)m * lda] * B[(long)n * ldb];
-  out[m + (long)n * ldo] = v;
-}
-
-int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
-                   double alpha, double gamma, double* out, long ldo, double* work,
-                   size_t work_doubles, hipStream_t s) {
-  const int tm = (M + BT - 1) / BT, tn = (N + BT - 1) / BT;
-  const long ksteps = (K + KC - 1) / KC;
-  int S = (int)std::max<long>(1, std::min<long>(ksteps / 8, (2048 + tm * tn - 1) / (tm * tn)));
-  while (S > 1 && (size_t)S * M * N > work_doubles) S--;
-  if ((size_t)S * M * N > work_doubles) return -1;
-  const long kper = ((ksteps + S - 1) / S) * KC;
-  S = (int)((K + kper - 1) / kper);
-  hipLaunchKernelGGL(k_gemm_tn, dim3(tm, tn, S), dim3(256), 0, s, A, lda, B, ldb, M, N, K, kper,
-                     work);
-  const long tot = (long)M * N;
-  hipLaunchKernelGGL(k_splitk_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, work, S, M, N,
-                     alpha, gamma, A, lda, B, ldb, out, ldo);
-  return 0;
-}
-
-// ------------------------------------------------------------------- NN
-// C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf]   (K rows, N cols, M sum)
-__global__ __launch_bounds__(256) void k_gemm_nn(const double* __restrict__ A, long lda,
-                                                 const double* __restrict__ F, long ldf, long K,
-                                                 int N, int M, double* __restrict__ C, long ldc,
-                                                 int accumulate) {
-  __shared__ double As[KC][LP];
-  __shared__ double Fs[KC][LP];
-  const long k0 = (long)blockIdx.x * BT;
-  const int n0 = blockIdx.y * BT;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wk = (w & 1) * 32, wn = (w >> 1) * 32;
-  v4d acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = v4d{0, 0, 0, 0};
-  for (int mb = 0; mb < M; mb += KC) {
-    // A tile: KC columns (m) x 64 rows (k), 4 per thread along k
-    {
-      const int mm = tid >> 4, kq = (tid & 15) * 4;
-      const int m = mb + mm;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const long k = k0 + kq + q;
-        As[mm][kq + q] = (m < M && k < K) ? A[k + (long)m * lda] : 0.0;
-      }
-    }
-    // F tile: KC rows (m) x 64 cols (n)
-    {
-      const int nn = tid >> 2, mq = (tid & 3) * 4;
-      const int n = n0 + nn;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int m = mb + mq + q;
-        Fs[mq + q][nn] = (m < M && n < N) ? F[m + (long)n * ldf] : 0.0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int kr = kk + (lane >> 4), c = lane & 15;
-      const double a0 = As[kr][wk + c], a1 = As[kr][wk + 16 + c];
-      const double b0 = Fs[kr][wn + c], b1 = Fs[kr][wn + 16 + c];
-      acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-      acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-      acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-      acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const long k = k0 + wk + i * 16 + 4 * r + (lane >> 4);
-        const int n = n0 + wn + j * 16 + (lane & 15);
-        if (k < K && n < N) {
-          double* p = C + k + (long)n * ldc;
-          *p = accumulate ? *p + acc[i][j][r] : acc[i][j][r];
+      for (int e = 0; e < 4; e++) {
+        const int p = p0 + wp + 16 * i + MF::row(e, lane);
+        const int q = q0 + wq + 16 * j + (lane & 15);
+        if (p < g.np && q < g.nq) {
+          TO* d = o + q + (long)p * g.ldo;
+          const TO v = (TO)acc[i][j][e];
+          *d = g.accumulate ? (TO)(*d + v) : v;
         }
       }
 }
 
+// out[q + p*ldo] = alpha * sum_s partial[s][p][q] + gamma * Q[q*ldq] * P[p*ldp]
+template <typename TQm, typename TPm>
+__global__ __launch_bounds__(256) void k_split_reduce(const double* __restrict__ partial, int S,
+                                                      int nq, int np, double alpha, double gamma,
+                                                      const TQm* __restrict__ Q, long ldq,
+                                                      const TPm* __restrict__ P, long ldp,
+                                                      double* __restrict__ out, long ldo) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)nq * np) return;
+  const int q = (int)(t % nq), p = (int)(t / nq);
+  double s = 0.0;
+  for (int k = 0; k < S; k++) s += partial[(long)k * nq * np + t];
+  double v = alpha * s;
+  if (gamma != 0.0) v += gamma * (double)Q[(long)q * ldq] * (double)P[(long)p * ldp];
+  out[q + (long)p * ldo] = v;
+}
+
+namespace {
+
+template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
+void launch_gemm(const GemmArgs& g, hipStream_t s) {
+  const int nblk = g.tp * g.tq * (int)((g.nr + g.rper - 1) / g.rper);
+  hipLaunchKernelGGL((k_gemm<TPm, TQm, TC, QR, TO>), dim3(nblk), dim3(256), 0, s, g);
+}
+
+// split-K TN product; partials in FP64, reduced in a fixed order
+template <typename TQm>
+int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, long K,
+              double alpha, double gamma, double* out, long ldo, double* work,
+              size_t work_doubles, int prec, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return -1;
+  GemmArgs g{};
+  g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
+  g.np = N; g.nq = M; g.nr = K;
+  g.tp = (N + TP - 1) / TP; g.tq = (M + TQ - 1) / TQ;
+  g.pfast = 0;
+  const long ksteps = (K + KC - 1) / KC;
+  const int ntile = g.tp * g.tq;
+  // about four blocks per CU; partial traffic kept well below operand traffic
+  long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
+  S = std::min<long>(S, std::max<long>(1, ksteps / 4));
+  S = std::min<long>(S, std::max<long>(1, (long)((double)K * (M + N) / (4.0 * M * N))));
+  while (S > 1 && (size_t)S * M * N > work_doubles) S--;
+  if ((size_t)S * M * N > work_doubles) return -1;
+  g.rper = ((ksteps + S - 1) / S) * KC;
+  S = (K + g.rper - 1) / g.rper;
+  g.out = work; g.ldo = M; g.sstride = (long)M * N; g.accumulate = 0;
+  if (prec == QBG_PREC_FP64) launch_gemm<double, TQm, double, true, double>(g, s);
+  else launch_gemm<double, TQm, float, true, double>(g, s);
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL((k_split_reduce<TQm, double>), dim3((unsigned)((tot + 255) / 256)),
+                     dim3(256), 0, s, work, (int)S, M, N, alpha, gamma, A, lda, B, ldb, out, ldo);
+  return 0;
+}
+
+template <typename TQm, typename TO>
+void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M, TO* C, long ldc,
+        int accumulate, int prec, hipStream_t s) {
+  if (K <= 0 || N <= 0 || M <= 0) return;
+  GemmArgs g{};
+  g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
+  g.np = N; g.nq = (int)K; g.nr = M;
+  g.tp = (N + TP - 1) / TP; g.tq = (int)((K + TQ - 1) / TQ);
+  g.pfast = 1;
+  g.rper = ((M + KC - 1) / KC) * KC;
+  g.out = C; g.ldo = ldc; g.sstride = 0; g.accumulate = accumulate;
+  if (prec == QBG_PREC_FP64) launch_gemm<double, TQm, double, false, TO>(g, s);
+  else launch_gemm<double, TQm, float, false, TO>(g, s);
+}
+
+}  // namespace
+
+int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
+                   double alpha, double gamma, double* out, long ldo, double* work,
+                   size_t work_doubles, int prec, hipStream_t s) {
+  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, s);
+}
+int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
+                   double alpha, double gamma, double* out, long ldo, double* work,
+                   size_t work_doubles, int prec, hipStream_t s) {
+  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, s);
+}
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
-             double* C, long ldc, int accumulate, hipStream_t s) {
-  dim3 grid((unsigned)((K + BT - 1) / BT), (N + BT - 1) / BT);
-  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(256), 0, s, A, lda, F, ldf, K, N, M, C, ldc,
-                     accumulate);
+             double* C, long ldc, int accumulate, int prec, hipStream_t s) {
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
+}
+void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N, int M,
+             double* C, long ldc, int accumulate, int prec, hipStream_t s) {
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
+}
+void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
+             float* C, long ldc, int accumulate, int prec, hipStream_t s) {
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
 }
 
 // ------------------------------------------------------- Cholesky, inverse

@@ -266,6 +266,10 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
     const int m = std::max(g->f4[2]->n1, g->f4[2]->n2);
     g->zcols = std::min(256 / m, 4096 / np2 - 1);
   }
+  // launch-shape overrides for tuning sweeps: QBG_XGPB (x-pass row groups
+  // per block, 0 = whole plane), QBG_YPPB (y-pass pairs per block, v multiply)
+  if (const char* e = std::getenv("QBG_XGPB")) g->xgpb = std::max(0, std::atoi(e));
+  if (const char* e = std::getenv("QBG_YPPB")) g->yppb = std::max(1, std::atoi(e));
   g->xrows = lines(np0);
   g->ytile = lines(np1);
   if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);

@@ -71,7 +71,7 @@ SIGNATURES = {
 }
 
 KERNEL_KINDS = ["zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd", "yother", "nl", "gram",
-                "other"]
+                "other", "vgrid", "gemm"]
 ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p", "exc", "dxc",
                 "esr", "eself", "ecoul", "etotal", "total_charge"]
 

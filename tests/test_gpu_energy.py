@@ -75,6 +75,28 @@ def test_evaluate_fp64(system, ctx):
     assert relerr(got["fion"], ref["fion"]) < 1e-10
 
 
+# mixed precision (FP32 band FFTs and projector GEMMs on FP32 MFMA; FP64
+# reductions): every energy term within 1e-6 Ha/atom of the FP64 oracle,
+# forces within 1e-5 Ha/bohr, H*psi within 1e-5 relative
+def test_evaluate_mixed(system, ctx):
+    idx, s, o, ef = system
+    b = o.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(1000 + idx, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, prec="fp32")
+    ref = o.evaluate(c, hpsi=True, forces=True)
+    for t in TERMS:
+        err = abs(got[t] - ref[t]) / s.nat
+        print(f"  {t:13s} {got[t]: .12e} dE/atom {err:.2e}")
+        assert err < 1e-6, (t, got[t], ref[t], err)
+    ferr = np.abs(got["fion"] - ref["fion"]).max()
+    herr = relerr(hp.download(), ref["hpsi"])
+    print(f"  fion max err {ferr:.2e}  hpsi rel err {herr:.2e}")
+    assert ferr < 1e-5
+    assert herr < 1e-5
+
+
 def test_energy_state_errors(ctx):
     import qbg
     s = cf.config(10)
