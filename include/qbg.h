@@ -154,8 +154,11 @@ typedef struct {
  * the wavefunction basis (ecut, Hartree) and the density basis (4 ecut) on
  * the density grid, the species tables and projectors (cell_moved), the
  * structure factor on the device and the esr sums (atoms_moved).  cell9 is
- * a0,a1,a2 (bohr); tau[3*nat] is species-major; occ[nst].  Host arrays. */
-int qbg_ef_create(qbg_ctx* ctx, const double* cell9, double ecut, int nsp,
+ * a0,a1,a2 (bohr); refcell9 (may be NULL) is Wavefunction's reference cell,
+ * which then fixes the basis membership and the grid sizes
+ * (src/Basis.cpp:314-325); tau[3*nat] is species-major; occ[nst].  Host
+ * arrays. */
+int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut, int nsp,
                   const qbg_species_desc* species, const int* na,
                   const double* tau, int nst, const double* occ, qbg_ef** out);
 int qbg_ef_destroy(qbg_ef* ef);
@@ -176,7 +179,8 @@ int qbg_ef_update_vhxc(qbg_ef* ef, int compute_stress);
  * compute_stress, sigma): hpsi (device, overwritten with H*psi when
  * compute_hpsi); energies (host, 14 doubles: ekin eps enl ehart ehart_e
  * ehart_ep ehart_p exc dxc esr eself ecoul etotal total_charge); fion (host,
- * 3*nat); sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr). */
+ * 3*nat); sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr; columns
+ * xx yy zz xy yz xz).  compute_stress needs update_vhxc(compute_stress=1). */
 int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
                   double* hpsi, int ldh, int compute_forces, int compute_stress,
                   int prec, double* energies, double* fion, double* sigma);

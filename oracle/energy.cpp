@@ -478,7 +478,7 @@ static void gemm_nn_acc(long K, int N, int M, const double* A, long lda,
 // src/NonLocalPotential.cpp:1258-2069 restated for the real basis, KB
 // projectors with l <= 1.  The reference processes atoms in blocks of 32
 // (:1274); the block size only reorders independent columns and is not
-// restated.  Stress terms (:1786-2037) are restated in nl_stress below.
+// restated.
 double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                          bool compute_forces, std::vector<double>& fion_enl,
                          bool compute_stress, double* sigma_enl) {
@@ -491,6 +491,7 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
   for (int is = 0; is < nsp; is++) nat += (int)tau[is].size() / 3;
   fion_enl.assign(3 * nat, 0.0);
   for (int i = 0; i < 6; i++) sigma_enl[i] = 0.0;
+  double tsum[6] = {0, 0, 0, 0, 0, 0};
   const double* kx[3] = {&wb.gx[0], &wb.gx[ngw], &wb.gx[2 * ngw]};
   const double* cp = reinterpret_cast<const double*>(c);
   int atom_off = 0;
@@ -584,9 +585,37 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
           }
       }
     }
-    if (compute_stress)
-      throw std::runtime_error("oracle: NL stress not restated yet");
+    if (compute_stress) {   // :1786-2037, KB l = 0, 1
+      std::vector<double> dfnl((size_t)M * nst);
+      for (int ij = 0; ij < 6; ij++) {
+        // danl = dtwnl_ij (-i)^l e^{-iG.tau}, the phases of anl
+#pragma omp parallel for schedule(static) collapse(2)
+        for (int ipr = 0; ipr < npr; ipr++)
+          for (int ia = 0; ia < na; ia++) {
+            const double* d = &dtwnl[is][(size_t)ngw * (ij + 6 * ipr)];
+            const int l = lproj[is][ipr];
+            double* a = &anl[(size_t)K * (ia + ipr * na)];
+            const double* cc = &cgr[(size_t)ia * ngw];
+            const double* ss = &sgr[(size_t)ia * ngw];
+            for (int ig = 0; ig < ngw; ig++) {
+              if (l == 0) { a[2 * ig] = d[ig] * cc[ig]; a[2 * ig + 1] = d[ig] * ss[ig]; }
+              else { a[2 * ig] = d[ig] * ss[ig]; a[2 * ig + 1] = -d[ig] * cc[ig]; }
+            }
+          }
+        // dfnl = 2 danl^T c, no G=0 correction (dtwnl(G=0) = 0)
+        gemm_tn(M, nst, K, 2.0, anl.data(), K, cp, K, 0.0, dfnl.data(), M);
+        for (int n = 0; n < nst; n++) {
+          const double facn = 2.0 * occ[n];
+          for (int i = 0; i < M; i++)
+            tsum[ij] += facn * fnl[i + (size_t)n * M] * dfnl[i + (size_t)n * M];
+        }
+      }
+    }
     atom_off += na;
+  }
+  if (compute_stress) {   // :2057-2066
+    for (int i = 0; i < 3; i++) sigma_enl[i] = (enl + tsum[i]) * omega_inv;
+    for (int i = 3; i < 6; i++) sigma_enl[i] = tsum[i] * omega_inv;
   }
   return enl;
 }

@@ -60,7 +60,10 @@ ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p",
 class Oracle:
     """One Qbox-like system on the CPU (EnergyFunctional + ChargeDensity)."""
 
-    def __init__(self, sysdef):
+    def __init__(self, sysdef, refcell=None):
+        """refcell (3x3, rows a0 a1 a2): the reference cell that fixes the
+        basis membership and grid sizes (src/Basis.cpp:314-325), so that
+        strained copies of a system share one basis."""
         L = lib()
         self.sysdef = sysdef
         self._keep = []
@@ -75,7 +78,12 @@ class Oracle:
         tau = np.ascontiguousarray(sysdef.tau, dtype=float).ravel()
         occ = sysdef.occ.astype(float)
         self._keep += [cell, na, tau, occ, descs]
-        h = L.qbo_create(_ptr(cell), None, sysdef.ecut, len(sysdef.species), descs,
+        ref = None
+        if refcell is not None:
+            ref = np.ascontiguousarray(refcell, dtype=float).ravel()
+            self._keep.append(ref)
+        h = L.qbo_create(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
+                         len(sysdef.species), descs,
                          _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ))
         if not h:
             raise RuntimeError(L.qbo_last_error().decode())

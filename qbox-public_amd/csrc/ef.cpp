@@ -20,7 +20,9 @@ enum {
   S_G0 = 22,      // 1: Re(conj(rhoelg[0]) vion[0])
   S_KIN = 24,     // 8: ekin, sigma_ekin[6], -
   S_ENL = 32,     // 1
-  S_COUNT = 40
+  S_NLS = 33,     // 6: NL stress sums tsum_ij
+  S_HPS = 39,     // 6: species part of the Hartree stress
+  S_COUNT = 48
 };
 
 template <typename T>
@@ -76,6 +78,12 @@ struct qbg_ef {
           *d_m_npr = nullptr, *d_m_stride = nullptr, *d_m_dbase = nullptr;
   double *d_m_wt = nullptr, *d_dmat = nullptr;
   int32_t *d_a_m0 = nullptr, *d_a_npr = nullptr, *d_a_stride = nullptr;
+  // stress: dtwnl[s] at dt_off[s], layout [ipr][ij][ig]; npr per species;
+  // 2 (rcps/2)^2 per species
+  double* d_dtwnl = nullptr;
+  int64_t* d_dt_off = nullptr;
+  int32_t* d_sp_npr = nullptr;
+  double* d_fac2 = nullptr;
   double *d_anl = nullptr, *d_fnl = nullptr, *d_fout = nullptr, *d_work = nullptr;
   size_t work_doubles = 0;
   // host results of atoms_moved
@@ -92,7 +100,8 @@ static void ef_free(qbg_ef* e) {
                e->d_vr, e->d_rhog, e->d_rhoelg, e->d_rhogt, e->d_vlocal, e->d_occ, e->d_fesr,
                e->d_fion, e->d_scal, e->d_partial, e->d_m_atom, e->d_m_twnl, e->d_m_l,
                e->d_m_ipr, e->d_m_npr, e->d_m_stride, e->d_m_dbase, e->d_m_wt, e->d_dmat,
-               e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work};
+               e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work,
+               e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->gw) qbg_grid_destroy(e->gw);
@@ -173,7 +182,7 @@ static int ef_atoms_moved(qbg_ef* e) {
 
 extern "C" {
 
-int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
+int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut, int nsp,
                   const qbg_species_desc* spd, const int* na, const double* tau, int nst,
                   const double* occ, qbg_ef** out) {
   if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
@@ -206,8 +215,10 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
   e->tau.assign(tau, tau + 3 * e->nat);
   for (int is = 0; is < nsp; is++) e->eself += e->na[is] * e->sp[is].eself();
   // bases and grid (src/ChargeDensity.cpp:34-98)
-  e->wb.build(e->cell, e->cell, ecut);
-  e->vb.build(e->cell, e->cell, 4.0 * ecut);
+  host::Cell def = e->cell;
+  if (refcell9) def.set(refcell9);
+  e->wb.build(def, e->cell, ecut);
+  e->vb.build(def, e->cell, 4.0 * ecut);
   for (int i = 0; i < 3; i++) {
     int n = e->vb.np[i] + 2;
     while (!host::factorizable(n)) n += 2;
@@ -245,7 +256,9 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
   std::vector<int> twnl_off(nsp);
   std::vector<int32_t> m_atom, m_twnl, m_l, m_ipr, m_npr, m_stride, m_dbase, a_m0(e->nat),
       a_npr(e->nat), a_stride(e->nat);
-  std::vector<double> m_wt, dmat;
+  std::vector<double> m_wt, dmat, dtwnl, fac2(nsp);
+  std::vector<int64_t> dt_off(nsp);
+  std::vector<int32_t> sp_npr(nsp);
   const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
   for (int is = 0; is < nsp; is++) {
     const host::Species& q = e->sp[is];
@@ -264,6 +277,38 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
         for (int j = 0; j < 3; j++) t[(size_t)(1 + j) * ngw + ig] = s34pi * e->wb.gx[j * ngw + ig] * gi * v;
       }
     }
+    // dtwnl (src/NonLocalPotential.cpp:283-306 for l = 0, :426-512 for l = 1)
+    dt_off[is] = (int64_t)dtwnl.size();
+    dtwnl.resize(dtwnl.size() + (size_t)npr * 6 * ngw);
+    double* d = &dtwnl[dt_off[is]];
+    for (int ig = 0; ig < ngw; ig++) {
+      double v, dv;
+      const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
+      const double x = e->wb.gx[ig], y = e->wb.gx[ngw + ig], z = e->wb.gx[2 * ngw + ig];
+      const double gg[6] = {x * x, y * y, z * z, x * y, y * z, x * z};
+      q.dvnlg(0, tg, v, dv);
+      const double t0 = tgi * s14pi * dv;
+      for (int ij = 0; ij < 6; ij++) d[(size_t)ij * ngw + ig] = t0 * gg[ij];
+      if (q.lmax >= 1) {
+        q.dvnlg(1, tg, v, dv);
+        const double tgi2 = tgi * tgi;
+        const double yv[3] = {s34pi * x * tgi, s34pi * y * tgi, s34pi * z * tgi};
+        for (int m = 0; m < 3; m++) {
+          const double fac = -yv[m] * (v - tg * dv) * tgi2;
+          double* dm = d + (size_t)6 * (1 + m) * ngw;
+          for (int ij = 0; ij < 6; ij++) dm[(size_t)ij * ngw + ig] = fac * gg[ij];
+        }
+        // the v*y terms of :484-511, as the reference writes them
+        d[(size_t)6 * 1 * ngw + 0 * ngw + ig] += v * yv[0];   // m=x: xx
+        d[(size_t)6 * 2 * ngw + 1 * ngw + ig] += v * yv[1];   // m=y: yy
+        d[(size_t)6 * 2 * ngw + 3 * ngw + ig] += v * yv[0];   // m=y: xy
+        d[(size_t)6 * 3 * ngw + 2 * ngw + ig] += v * yv[2];   // m=z: zz
+        d[(size_t)6 * 3 * ngw + 4 * ngw + ig] += v * yv[1];   // m=z: yz
+        d[(size_t)6 * 3 * ngw + 5 * ngw + ig] += v * yv[0];   // m=z: xz
+      }
+    }
+    sp_npr[is] = npr;
+    fac2[is] = 2.0 * 0.25 * q.rcps * q.rcps;
     lp[0] = 0;
     for (int j = 1; j < npr; j++) lp[j] = 1;
     // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
@@ -309,6 +354,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
   e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)((ngw + 255) / 256) * 8);
+  e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
   // split-K partials of the fnl product: up to 40 splits of M x nst
   e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
   if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
@@ -324,6 +370,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, double ecut, int nsp,
       (rc = dput(c, &e->d_m_dbase, m_dbase)) || (rc = dput(c, &e->d_m_wt, m_wt)) ||
       (rc = dput(c, &e->d_dmat, dmat)) || (rc = dput(c, &e->d_a_m0, a_m0)) ||
       (rc = dput(c, &e->d_a_npr, a_npr)) || (rc = dput(c, &e->d_a_stride, a_stride)) ||
+      (rc = dput(c, &e->d_dtwnl, dtwnl)) || (rc = dput(c, &e->d_dt_off, dt_off)) ||
+      (rc = dput(c, &e->d_sp_npr, sp_npr)) || (rc = dput(c, &e->d_fac2, fac2)) ||
       (rc = dalloc(c, &e->d_tau, 3 * e->nat)) ||
       (rc = dalloc(c, &e->d_sftab, (size_t)e->nat * 3 * krange * 2)) ||
       (rc = dalloc(c, &e->d_sfac, (size_t)2 * nsp * ngv)) ||
@@ -468,7 +516,8 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   qbg_ctx* ctx = e->ctx;
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
   if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
-  if (compute_stress) return ctx->fail(QBG_E_INVAL, "stress: nonlocal stress not available");
+  if (compute_stress && !e->vhxc_stress)
+    return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec != QBG_PREC_FP64 && prec != QBG_PREC_FP32) return ctx->fail(QBG_E_INVAL, "energy: prec");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
@@ -506,31 +555,46 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
     else gemm_nn(e->d_anl, K, e->d_fout, M, K, nst, M, hpsi, 2L * ldh, 1, prec, s);
     ctx->prof_end();
   }
-  if (compute_forces) {
-    // Y = c W with W[n,m] = occ_n fout[m,n] (stored in the second fnl block),
-    // then f_a,j = -4 sum_{m in a} sum_K (-i G_j anl)[K,m] Y[K,m]
+  if (compute_forces || compute_stress) {
+    // Y = c W with W[n,m] = occ_n fout[m,n] (stored in the second fnl block);
+    // forces f_a,j = -4 sum_{m in a} sum_K (-i G_j anl)[K,m] Y[K,m] and the
+    // nonlocal stress (src/NonLocalPotential.cpp:1786-2066) both reduce Y
     double* W = e->d_fnl + (size_t)M * nst;
+    void* Y = f32 ? (void*)(anl32 + (size_t)M * K) : (void*)(e->d_anl + (size_t)M * K);
     ctx->prof_begin(QBG_K_NL);
-    QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
+    if (compute_forces)
+      QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
     launch_force_w(e->d_fout, e->d_occ, M, nst, W, s);
     ctx->prof_end();
-    if (f32) {
-      float* Y = anl32 + (size_t)M * K;
-      ctx->prof_begin(QBG_K_GEMM);
-      gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, prec, s);
-      ctx->prof_end();
-      ctx->prof_begin(QBG_K_NL);
-      launch_nl_forces_y(anl32, (const float*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
-                         e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
-    } else {
-      double* Y = e->d_anl + (size_t)M * K;
-      ctx->prof_begin(QBG_K_GEMM);
-      gemm_nn(c, 2L * ldc, W, nst, K, M, nst, Y, K, 0, prec, s);
-      ctx->prof_end();
-      ctx->prof_begin(QBG_K_NL);
-      launch_nl_forces_y(e->d_anl, (const double*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
-                         e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+    ctx->prof_begin(QBG_K_GEMM);
+    if (f32) gemm_nn(c, 2L * ldc, W, nst, K, M, nst, (float*)Y, K, 0, prec, s);
+    else gemm_nn(c, 2L * ldc, W, nst, K, M, nst, (double*)Y, K, 0, prec, s);
+    ctx->prof_end();
+    ctx->prof_begin(QBG_K_NL);
+    if (compute_forces) {
+      if (f32)
+        launch_nl_forces_y(anl32, (const float*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
+                           e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+      else
+        launch_nl_forces_y(e->d_anl, (const double*)Y, K, ngw, e->d_gxw, e->d_a_m0,
+                           e->d_a_npr, e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
     }
+    if (compute_stress) {
+      if (f32)
+        launch_nl_stress_y((const float*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
+                           e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
+                           e->d_partial, 64, e->d_scal + S_NLS, s);
+      else
+        launch_nl_stress_y((const double*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
+                           e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
+                           e->d_partial, 64, e->d_scal + S_NLS, s);
+    }
+    ctx->prof_end();
+  }
+  if (compute_stress) {
+    ctx->prof_begin(QBG_K_OTHER);
+    launch_hart_ps_stress(e->d_rhogt, e->d_sfac, e->d_rhops, e->d_g2iv, e->d_gxv, e->vb.size,
+                          e->nsp, e->d_fac2, e->d_partial, e->d_scal + S_HPS, s);
     ctx->prof_end();
   }
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
@@ -570,7 +634,25 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   const double en[14] = {ekin, eps, enl, ehart, ehart_e, ehart_ep, ehart_p, exc, dxc,
                          e->esr, e->eself, ecoul, etotal, charge};
   std::memcpy(energies, en, sizeof(en));
-  if (sigma) std::memset(sigma, 0, sizeof(double) * 42);
+  if (sigma) {
+    // rows: total, ekin, eps, enl, ehart, exc, esr; columns xx yy zz xy yz xz
+    // (src/EnergyFunctional.cpp:560-683, :932-936; NonLocalPotential.cpp:2057-2066)
+    std::memset(sigma, 0, sizeof(double) * 42);
+    if (compute_stress) {
+      const double oi = 1.0 / omega;
+      for (int i = 0; i < 6; i++) {
+        const bool d = i < 3;
+        double* sg = sigma;
+        sg[6 + i] = sc[S_KIN + 1 + i] * oi;
+        sg[12 + i] = sc[S_VHXC + 11 + i] + (d ? eps * oi : 0.0);
+        sg[18 + i] = (d ? enl : 0.0) * oi + sc[S_NLS + i] * oi;
+        sg[24 + i] = (d ? ehart * oi : 0.0) - 2.0 * fpi * (sc[S_VHXC + 5 + i] + sc[S_HPS + i]);
+        sg[30 + i] = d ? -sc[S_LDA + 1] / Nd : 0.0;
+        sg[36 + i] = e->sigma_esr[i];
+        sg[i] = sg[6 + i] + sg[12 + i] + sg[18 + i] + sg[24 + i] + sg[30 + i] + sg[36 + i];
+      }
+    }
+  }
   return QBG_OK;
 }
 

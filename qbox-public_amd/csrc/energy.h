@@ -49,6 +49,16 @@ void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double*
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
                         int nat, double* partial, int nchunk, double* fion, hipStream_t s);
 
+template <typename T>
+void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const double* tau,
+                        int nsp, const int32_t* sp_first, const int32_t* sp_npr,
+                        const int64_t* dt_off, const double* dtwnl, const int32_t* a_m0,
+                        const int32_t* a_stride, double* partial, int nchunk, double* out6,
+                        hipStream_t s);
+void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double* rhops,
+                           const double* g2i, const double* gx, int ngv, int nsp,
+                           const double* fac2, double* partial, double* out6, hipStream_t s);
+
 // gemm.hip.  prec (QBG_PREC_FP64 / QBG_PREC_FP32) selects the MFMA compute
 // type; operands of either type are converted as they are staged.
 // TN split-K: out[m + n*ldo] = alpha sum_k A[k + m*lda] B[k + n*ldb]

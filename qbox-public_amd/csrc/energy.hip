@@ -557,3 +557,114 @@ template void launch_nl_forces_y<float>(const float*, const float*, long, int, c
                                         double*, int, double*, hipStream_t);
 
 }  // namespace qbg
+
+namespace qbg {
+
+// ------------------------------------------------------------ NL stress
+// Reassociation of src/NonLocalPotential.cpp:1786-2037 (the six dfnl_ij =
+// 2 danl_ij^T c products): with Y = c W, W[n,m] = occ_n fout[m,n] (the
+// matrix of the force GEMM),
+//   tsum_ij = sum_{m,n} 2 occ_n fout[m,n] dfnl_ij[m,n]
+//           = 4 sum_m sum_K danl_ij[K,m] Y[K,m],
+// danl_ij = dtwnl_ij (-i)^l e^{-iG.tau}.  Summing the atoms of a species
+// first, Z_ipr(G) = sum_a Re-part of (phase_a(G) * Y_{m(a,ipr)}(G)), gives
+// tsum_ij = 4 sum_s sum_ipr sum_G dtwnl_s,ipr,ij(G) Z_ipr(G).
+// Block (chunk of G, species); partial[(s*nchunk + chunk)*6 + ij].
+template <typename T>
+__global__ __launch_bounds__(256) void k_nl_stress_y(
+    const T* __restrict__ Y, long lda, int ngw, const double* __restrict__ gx,
+    const double* __restrict__ tau, const int32_t* __restrict__ sp_first,
+    const int32_t* __restrict__ sp_npr, const int64_t* __restrict__ dt_off,
+    const double* __restrict__ dtwnl, const int32_t* __restrict__ a_m0,
+    const int32_t* __restrict__ a_stride, int chunk, double* __restrict__ partial) {
+  const int s = blockIdx.y;
+  const int a0 = sp_first[s], a1 = sp_first[s + 1], npr = sp_npr[s];
+  const double* dt = dtwnl + dt_off[s];
+  const int g0 = blockIdx.x * chunk, g1 = min(ngw, g0 + chunk);
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
+    const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
+    double Z[4] = {0, 0, 0, 0};
+    for (int a = a0; a < a1; a++) {
+      double sn, cs;
+      sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
+      const long mb = a_m0[a], st = a_stride[a];
+      for (int ipr = 0; ipr < npr; ipr++) {
+        const T* yp = Y + (mb + ipr * st) * lda + 2 * ig;
+        const double yr = (double)yp[0], yi = (double)yp[1];
+        // l = 0: phase (c, s); l = 1: (-i)(c + i s) = (s, -c)
+        Z[ipr] += ipr == 0 ? cs * yr + sn * yi : sn * yr - cs * yi;
+      }
+    }
+    for (int ipr = 0; ipr < npr; ipr++)
+#pragma unroll
+      for (int ij = 0; ij < 6; ij++) v[ij] += dt[(long)(ij + 6 * ipr) * ngw + ig] * Z[ipr];
+  }
+  block_reduce_store<6>(v, partial + ((long)s * gridDim.x + blockIdx.x) * 6);
+}
+
+// out6[ij] = 4 * sum of the partials (fixed order)
+__global__ void k_nl_stress_final(const double* __restrict__ partial, int nblk,
+                                  double* __restrict__ out6) {
+  const int ij = threadIdx.x;
+  if (ij >= 6) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += partial[(long)b * 6 + ij];
+  out6[ij] = 4.0 * s;
+}
+
+template <typename T>
+void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const double* tau,
+                        int nsp, const int32_t* sp_first, const int32_t* sp_npr,
+                        const int64_t* dt_off, const double* dtwnl, const int32_t* a_m0,
+                        const int32_t* a_stride, double* partial, int nchunk, double* out6,
+                        hipStream_t s) {
+  const int chunk = (ngw + nchunk - 1) / nchunk;
+  hipLaunchKernelGGL(k_nl_stress_y<T>, dim3(nchunk, nsp), dim3(256), 0, s, Y, lda, ngw, gx, tau,
+                     sp_first, sp_npr, dt_off, dtwnl, a_m0, a_stride, chunk, partial);
+  hipLaunchKernelGGL(k_nl_stress_final, dim3(1), dim3(64), 0, s, partial, nchunk * nsp, out6);
+}
+template void launch_nl_stress_y<double>(const double*, long, int, const double*, const double*,
+                                         int, const int32_t*, const int32_t*, const int64_t*,
+                                         const double*, const int32_t*, const int32_t*, double*,
+                                         int, double*, hipStream_t);
+template void launch_nl_stress_y<float>(const float*, long, int, const double*, const double*,
+                                        int, const int32_t*, const int32_t*, const int64_t*,
+                                        const double*, const int32_t*, const int32_t*, double*,
+                                        int, double*, hipStream_t);
+
+// ------------------------------------------- Hartree pseudocharge stress
+// The species part of sigma_ehart (src/EnergyFunctional.cpp:649-676):
+// sum_s 2 (rcps_s^2 / 4) Re(conj(rhogt) sfac_s) rhops_s g2i G_i G_j.
+__global__ __launch_bounds__(256) void k_hart_ps_stress(
+    const double* __restrict__ rhogt, const double* __restrict__ sfac,
+    const double* __restrict__ rhops, const double* __restrict__ g2i,
+    const double* __restrict__ gx, int ngv, int nsp, const double* __restrict__ fac2,
+    double* __restrict__ partial) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (ig < ngv) {
+    const double tr = rhogt[2 * ig], ti = rhogt[2 * ig + 1];
+    double t = 0.0;
+    for (int is = 0; is < nsp; is++) {
+      const double* sf = sfac + 2 * (long)is * ngv;
+      t += fac2[is] * (tr * sf[2 * ig] + ti * sf[2 * ig + 1]) * rhops[(long)is * ngv + ig];
+    }
+    t *= g2i[ig];
+    const double x = gx[ig], y = gx[ngv + ig], z = gx[2 * ngv + ig];
+    v[0] = t * x * x; v[1] = t * y * y; v[2] = t * z * z;
+    v[3] = t * x * y; v[4] = t * y * z; v[5] = t * x * z;
+  }
+  block_reduce_store<6>(v, partial + (long)blockIdx.x * 6);
+}
+
+void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double* rhops,
+                           const double* g2i, const double* gx, int ngv, int nsp,
+                           const double* fac2, double* partial, double* out6, hipStream_t s) {
+  const int nb = (ngv + 255) / 256;
+  hipLaunchKernelGGL(k_hart_ps_stress, dim3(nb), dim3(256), 0, s, rhogt, sfac, rhops, g2i, gx,
+                     ngv, nsp, fac2, partial);
+  reduce_partials(partial, nb, 6, out6, s);
+}
+
+}  // namespace qbg

@@ -56,7 +56,7 @@ SIGNATURES = {
     "qbg_prof_reset": [_vp],
     "qbg_prof_read": [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "qbg_gram": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
-    "qbg_ef_create": [_vp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
+    "qbg_ef_create": [_vp, _dp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
                       C.POINTER(_vp)],
     "qbg_ef_destroy": [_vp],
     "qbg_ef_sizes": [_vp, _ip],
@@ -300,7 +300,9 @@ def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
 class EnergyFunctional:
     """qbg_ef: one Gamma-point system resident on the device."""
 
-    def __init__(self, ctx: Context, sysdef):
+    def __init__(self, ctx: Context, sysdef, refcell=None):
+        """refcell (3x3, rows a0 a1 a2): Wavefunction's reference cell, which
+        fixes the basis membership and the grids (src/Basis.cpp:314-325)."""
         self.ctx = ctx
         self.sysdef = sysdef
         descs = (SpeciesDesc * len(sysdef.species))()
@@ -314,9 +316,13 @@ class EnergyFunctional:
         na = np.ascontiguousarray(sysdef.na, dtype=np.int32)
         tau = np.ascontiguousarray(sysdef.tau, dtype=np.float64).ravel()
         occ = np.ascontiguousarray(sysdef.occ, dtype=np.float64)
-        self._keep += [cell, na, tau, occ]
+        ref = None
+        if refcell is not None:
+            ref = np.ascontiguousarray(refcell, dtype=np.float64).ravel()
+        self._keep += [cell, na, tau, occ, ref]
         h = _vp()
-        ctx.check(ctx.lib.qbg_ef_create(ctx.h, _pd(cell), sysdef.ecut, len(sysdef.species),
+        ctx.check(ctx.lib.qbg_ef_create(ctx.h, _pd(cell), _pd(ref) if ref is not None else None,
+                                        sysdef.ecut, len(sysdef.species),
                                         C.cast(descs, _vp), na.ctypes.data_as(_ip), _pd(tau),
                                         sysdef.nst, _pd(occ), C.byref(h)))
         self.h = h

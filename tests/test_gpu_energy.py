@@ -97,6 +97,57 @@ def test_evaluate_mixed(system, ctx):
     assert herr < 1e-5
 
 
+# stress: every row (total, ekin, eps, enl, ehart, exc, esr) to 1e-10 of the
+# largest component of the oracle's total
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stress(system, ctx, prec):
+    idx, s, o, ef = system
+    b = o.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(2000 + idx, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    got = ef.evaluate(dc, None, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=False, forces=True, stress=True)
+    scale = np.abs(ref["sigma"][0]).max()
+    err = np.abs(got["sigma"] - ref["sigma"]).max(axis=1) / scale
+    print(f"  sigma rel err per row ({prec}):", " ".join(f"{x:.1e}" for x in err))
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert err.max() < tol
+    # stress without forces takes the same path
+    got2 = ef.energy(dc, None, forces=False, stress=True, prec=prec)
+    assert np.array_equal(got2["sigma"], got["sigma"])
+
+
+def test_stress_strain_fd_device(ctx):
+    """E(+u) - E(-u) = -2 omega tr(sigma u) (test/si2stress/check_stress.sh)
+    on the device path, with a reference cell fixing the basis."""
+    import dataclasses
+    import qbg
+    s = cf.config(11)
+    ref = s.cell * 1.04
+    ef0 = qbg.EnergyFunctional(ctx, s, refcell=ref)
+    b = ef0.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(99, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    sig = ef0.evaluate(dc, None, forces=False, stress=True)["sigma"][0]
+    v = 1e-4 * np.array([0.7, -0.4, 0.9, 0.3, -0.6, 0.2])
+    u = np.zeros((3, 3))
+    for k, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]):
+        u[i, j] = u[j, i] = v[k]
+    e = []
+    for sgn in (1.0, -1.0):
+        F = np.eye(3) + sgn * u
+        st = dataclasses.replace(s, cell=s.cell @ F.T, tau=s.tau @ F.T)
+        ef = qbg.EnergyFunctional(ctx, st, refcell=ref)
+        assert np.array_equal(ef.basis(0)["idx"], b["idx"])
+        e.append(ef.evaluate(dc, None, forces=False, stress=False)["etotal"])
+        ef.release()
+    omega = abs(np.linalg.det(s.cell))
+    pred = -2.0 * omega * (v[:3] @ sig[:3] + 2.0 * v[3:] @ sig[3:])
+    de = e[0] - e[1]
+    print(f"  fd {de:.10e} sigma {pred:.10e}")
+    assert abs(de - pred) <= 2e-6 * abs(de) + 1e-11
+
+
 def test_energy_state_errors(ctx):
     import qbg
     s = cf.config(10)
@@ -105,6 +156,11 @@ def test_energy_state_errors(ctx):
     with pytest.raises(qbg.QbgError) as e:
         ef.energy(c)
     assert e.value.code == -5          # QBG_E_STATE: energy before update_vhxc
+    ef.update_density(c)
+    ef.update_vhxc(False)
+    with pytest.raises(qbg.QbgError) as e:
+        ef.energy(c, None, forces=False, stress=True)
+    assert e.value.code == -5          # stress needs update_vhxc(compute_stress=1)
 
 
 @pytest.mark.parametrize("idx", [10, 0])
