@@ -36,9 +36,16 @@ extern "C" {
 #define QBG_E_NOT_SPD (-4) /* overlap matrix not positive definite */
 #define QBG_E_STATE (-5)   /* call out of order (e.g. energy before vhxc) */
 
-/* precision of the FFT passes and projector GEMMs */
+/* precision of the FFT passes and projector GEMMs.  In every mode the
+ * Cholesky, the orthogonalization update and the energy, force and stress
+ * reductions run in FP64. */
 #define QBG_PREC_FP64 0
-#define QBG_PREC_FP32 1
+#define QBG_PREC_FP32 1    /* FP32 FFTs; projector GEMMs on FP32 MFMA */
+#define QBG_PREC_BF16X3 2  /* FP32 FFTs; projector GEMMs with FP32 operands split
+                              into 3 bfloat16 parts (6 part products, FP32
+                              accumulation: FP32-level accuracy) */
+#define QBG_PREC_BF16X2 3  /* as BF16X3 with 2 parts (3 part products, about
+                              16 significant bits) */
 
 typedef struct qbg_ctx qbg_ctx;
 typedef struct qbg_grid qbg_grid;

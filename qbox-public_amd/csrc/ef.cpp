@@ -86,6 +86,14 @@ struct qbg_ef {
   double* d_fac2 = nullptr;
   double *d_anl = nullptr, *d_fnl = nullptr, *d_fout = nullptr, *d_work = nullptr;
   size_t work_doubles = 0;
+  // BF16-split GEMM workspace (allocated on first use): FP32 arrays padded
+  // to Kp rows (2 ngw up to a multiple of 128), Mp projector and Np band
+  // columns (multiples of 128), zero padding
+  int Mp = 0, Np = 0;
+  long Kp = 0;
+  float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
+        *b_w32 = nullptr, *b_part = nullptr;
+  size_t b_part_floats = 0;
   // host results of atoms_moved
   double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
   std::vector<double> fesr;
@@ -101,7 +109,8 @@ static void ef_free(qbg_ef* e) {
                e->d_fion, e->d_scal, e->d_partial, e->d_m_atom, e->d_m_twnl, e->d_m_l,
                e->d_m_ipr, e->d_m_npr, e->d_m_stride, e->d_m_dbase, e->d_m_wt, e->d_dmat,
                e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work,
-               e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2};
+               e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
+               e->b_y32, e->b_fout, e->b_w32, e->b_part};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->gw) qbg_grid_destroy(e->gw);
@@ -447,8 +456,11 @@ int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
   qbg_ctx* ctx = e->ctx;
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
+  if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
   QBG_HIP(ctx, hipMemsetAsync(e->d_rhor, 0, sizeof(double) * e->N, s));
-  int rc = qbg_compute_density(e->gw, c, ldc, e->nst, e->occ.data(), 1.0, e->omega, e->d_rhor, prec);
+  const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;
+  int rc = qbg_compute_density(e->gw, c, ldc, e->nst, e->occ.data(), 1.0, e->omega, e->d_rhor,
+                               fprec);
   if (rc) return rc;
   qbg_grid* g = e->gv;
   if ((rc = grid_ensure_ws(g, 1))) return rc;
@@ -505,31 +517,13 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
   return QBG_OK;
 }
 
-// EnergyFunctional::energy (src/EnergyFunctional.cpp:397-1044): nspin 1,
-// Gamma, LDA.  energies (host, 14): ekin eps enl ehart ehart_e ehart_ep
-// ehart_p exc dxc esr eself ecoul etotal total_charge; fion (host, 3*nat)
-// species-major; sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr).
-int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
-                  int compute_forces, int compute_stress, int prec, double* energies,
-                  double* fion, double* sigma) {
-  if (!e || !c || !energies || ldc < e->wb.size) return QBG_E_INVAL;
+// EnergyFunctional::energy's nonlocal part (src/NonLocalPotential.cpp:1258-2069)
+// with the projector GEMMs on FP64 or FP32 MFMA (gemm.hip)
+static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
+                   int compute_forces, int compute_stress, int prec) {
   qbg_ctx* ctx = e->ctx;
-  if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
-  if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
-  if (compute_stress && !e->vhxc_stress)
-    return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
-  if (prec != QBG_PREC_FP64 && prec != QBG_PREC_FP32) return ctx->fail(QBG_E_INVAL, "energy: prec");
-  (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   const int ngw = e->wb.size, nst = e->nst, M = e->M;
-  // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf)
-  ctx->prof_begin(QBG_K_OTHER);
-  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
-                 e->d_partial, e->d_scal + S_KIN, s);
-  ctx->prof_end();
-  // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
-  // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
-  // the GEMMs run on FP32 MFMA
   const long K = 2L * ngw;
   ctx->prof_begin(QBG_K_NL);
   const bool f32 = prec == QBG_PREC_FP32;
@@ -591,6 +585,124 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
     }
     ctx->prof_end();
   }
+  return QBG_OK;
+}
+
+// allocate the BF16-split workspace once; padding is zeroed here and never
+// written afterwards
+static int bf16_ensure(qbg_ef* e) {
+  if (e->b_anl) return QBG_OK;
+  qbg_ctx* c = e->ctx;
+  auto up = [](long x, long m) { return (x + m - 1) / m * m; };
+  e->Kp = up(2L * e->wb.size, 128);
+  e->Mp = (int)up(e->M, 128);
+  e->Np = (int)up(e->nst, 128);
+  const int ntile = (e->Mp / 128) * (e->Np / 128);
+  e->b_part_floats = (size_t)(1024 + ntile) * 128 * 128;
+  const size_t n_anl = (size_t)e->Mp * e->Kp, n_c = (size_t)e->Np * e->Kp;
+  const size_t n_f = (size_t)e->Mp * e->Np;
+  float** bufs[5] = {&e->b_anl, &e->b_c32, &e->b_y32, &e->b_fout, &e->b_w32};
+  const size_t sz[5] = {n_anl, n_c, n_anl, n_f, n_f};
+  for (int i = 0; i < 5; i++) {
+    if (hipMalloc((void**)bufs[i], sizeof(float) * sz[i]) != hipSuccess) {
+      *bufs[i] = nullptr;
+      return c->fail(QBG_E_NOMEM, "bf16 workspace");
+    }
+    QBG_HIP(c, hipMemsetAsync(*bufs[i], 0, sizeof(float) * sz[i], c->stream));
+  }
+  if (hipMalloc((void**)&e->b_part, sizeof(float) * e->b_part_floats) != hipSuccess) {
+    e->b_part = nullptr;
+    return c->fail(QBG_E_NOMEM, "bf16 workspace");
+  }
+  return QBG_OK;
+}
+
+// the nonlocal part with the projector GEMMs as BF16-split products
+// (gemm_bf.hip) on FP32 copies of anl, c, fout and W
+static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
+                   int compute_forces, int compute_stress, int parts) {
+  qbg_ctx* ctx = e->ctx;
+  hipStream_t s = ctx->stream;
+  int rc = bf16_ensure(e);
+  if (rc) return rc;
+  const int ngw = e->wb.size, nst = e->nst, M = e->M, Mp = e->Mp, Np = e->Np;
+  const long K = 2L * ngw, Kp = e->Kp;
+  ctx->prof_begin(QBG_K_NL);
+  launch_anl(ngw, M, 1, Kp, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l, e->d_twnl,
+             e->b_anl, 1, s);
+  convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s);
+  ctx->prof_end();
+  ctx->prof_begin(QBG_K_GEMM);
+  rc = gemm_bf_tn(e->b_anl, Kp, e->b_c32, Kp, Mp, Np, Kp, M, nst, c, 2L * ldc, 2.0, -1.0,
+                  e->d_fnl, M, e->b_part, e->b_part_floats, parts, s);
+  ctx->prof_end();
+  if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
+  ctx->prof_begin(QBG_K_NL);
+  launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
+                  e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
+                  e->d_scal + S_ENL, s);
+  const bool needy = compute_forces || compute_stress;
+  fout_f32(e->d_fout, e->d_occ, M, nst, e->b_fout, Mp, needy ? e->b_w32 : nullptr, Np, s);
+  ctx->prof_end();
+  if (compute_hpsi) {
+    ctx->prof_begin(QBG_K_GEMM);
+    rc = gemm_bf_nn(e->b_anl, Kp, e->b_fout, Mp, Kp, Np, Mp, K, nst, hpsi, 2L * ldh, 1, parts, s);
+    ctx->prof_end();
+    if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
+  }
+  if (needy) {
+    ctx->prof_begin(QBG_K_GEMM);
+    rc = gemm_bf_nn(e->b_c32, Kp, e->b_w32, Np, Kp, Mp, Np, Kp, Mp, e->b_y32, Kp, 0, parts, s);
+    ctx->prof_end();
+    if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
+    ctx->prof_begin(QBG_K_NL);
+    if (compute_forces) {
+      QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
+      launch_nl_forces_y(e->b_anl, (const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_a_m0,
+                         e->d_a_npr, e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+    }
+    if (compute_stress)
+      launch_nl_stress_y((const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_tau, e->nsp,
+                         e->d_sp_first, e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0,
+                         e->d_a_stride, e->d_partial, 64, e->d_scal + S_NLS, s);
+    ctx->prof_end();
+  }
+  return QBG_OK;
+}
+
+// EnergyFunctional::energy (src/EnergyFunctional.cpp:397-1044): nspin 1,
+// Gamma, LDA.  energies (host, 14): ekin eps enl ehart ehart_e ehart_ep
+// ehart_p exc dxc esr eself ecoul etotal total_charge; fion (host, 3*nat)
+// species-major; sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr).
+int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
+                  int compute_forces, int compute_stress, int prec, double* energies,
+                  double* fion, double* sigma) {
+  if (!e || !c || !energies || ldc < e->wb.size) return QBG_E_INVAL;
+  qbg_ctx* ctx = e->ctx;
+  if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
+  if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
+  if (compute_stress && !e->vhxc_stress)
+    return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
+  if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
+  const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;   // band FFTs
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  const int ngw = e->wb.size, nst = e->nst, M = e->M;
+  // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf)
+  ctx->prof_begin(QBG_K_OTHER);
+  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
+                 e->d_partial, e->d_scal + S_KIN, s);
+  ctx->prof_end();
+  // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
+  // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
+  // the GEMMs run on FP32 MFMA; in the BF16 modes on the BF16-split path
+  const long K = 2L * ngw;
+  int nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
+                ? nl_bf16(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
+                          prec == QBG_PREC_BF16X3 ? 3 : 2)
+                : nl_mfma(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
+                          prec);
+  if (nrc) return nrc;
   if (compute_stress) {
     ctx->prof_begin(QBG_K_OTHER);
     launch_hart_ps_stress(e->d_rhogt, e->d_sfac, e->d_rhops, e->d_g2iv, e->d_gxv, e->vb.size,
@@ -599,7 +711,7 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   }
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
   if (compute_hpsi) {
-    int rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, prec);
+    int rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, fprec);
     if (rc) return rc;
   }
   if (compute_forces) {

@@ -76,6 +76,26 @@ void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N,
              double* C, long ldc, int accumulate, int prec, hipStream_t s);
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
              float* C, long ldc, int accumulate, int prec, hipStream_t s);
+// gemm_bf.hip: FP32 operands split into `parts` (2 or 3) bfloat16 parts on
+// the BF16 MFMA path; extents padded (Mp, Np, Kp multiples of 128, Kp of 32,
+// leading dimensions of 4) with zero padding.  Returns -1 on a bad shape.
+// TN: out[m + n*ldo] = alpha sum_k A[k + m*lda] B[k + n*ldb] + gamma A[m*lda] B64[n*ldb64]
+int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int Np, long Kp,
+               int M, int N, const double* B64, long ldb64, double alpha, double gamma,
+               double* out, long ldo, float* work, size_t work_floats, int parts,
+               hipStream_t s);
+// NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf], k < K, n < N stored
+int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
+               long K, int N, double* C, long ldc, int accumulate, int parts, hipStream_t s);
+int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
+               long K, int N, float* C, long ldc, int accumulate, int parts, hipStream_t s);
+// c (FP64, real view ld ldc2) -> c32[k + n*ldk]
+void convert_f32(const double* c, long ldc2, long K, int nst, float* c32, long ldk,
+                 hipStream_t s);
+// fout32[m + n*ldf] = fout[m + n*M]; w32[n + m*ldw] = occ_n fout[m + n*M] (if w32)
+void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32, long ldf,
+              float* w32, long ldw, hipStream_t s);
+
 void potrf(double* S, int n, int lds, int* info, hipStream_t s);
 void trtri(const double* L, int n, int ldl, double* X, hipStream_t s);
 void cleanup(double* c, long ldc, int nst, hipStream_t s);

@@ -16,8 +16,9 @@
 //   z-fwd : transform the columns, scale by 1/np012 (fz, :679-761) and
 //           unpack into (or accumulate onto) the two bands
 //           (zvec_to_doublevector, :721-750).
-// Only the active rows (2*ntrans0 per plane) and the nvec columns ever reach
-// HBM.  Each 1D transform is a Stockham mixed-radix FFT in LDS.
+// Only the active rows (2*ntrans0 per plane) and the columns ever reach HBM;
+// the columns are numbered row by row (grid.cpp).  Each 1D transform is a
+// Stockham mixed-radix FFT in LDS.
 #include <hip/hip_runtime.h>
 #include "internal.h"
 
@@ -143,8 +144,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_zbwd(
     const double* __restrict__ c, long pstride, long off2, int last_has2, int nbatch,
     const int32_t* __restrict__ ip, const int32_t* __restrict__ im,
-    const qbg_tile* __restrict__ tiles, int np2, int nvec, int TC, FftPlan plan,
-    const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ zbuf) {
+    const int32_t* __restrict__ glist, const qbg_tile* __restrict__ tiles, int np2, int ncols,
+    int TC, FftPlan plan, const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ zbuf) {
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* B = A + TC * np2;
@@ -156,7 +157,8 @@ __global__ __launch_bounds__(256) void k_zbwd(
   const double* c1 = c + 2 * (long)pair * pstride;
   const bool has2 = (MODE == Z_PAIR) && (pair < nbatch - 1 || last_has2);
   const double* c2 = c1 + 2 * off2;
-  for (int ig = t.g0 + threadIdx.x; ig < t.g1; ig += blockDim.x) {
+  for (int j = t.g0 + threadIdx.x; j < t.g1; j += blockDim.x) {
+    const int ig = glist[j];
     const double a = c1[2 * ig], b = c1[2 * ig + 1];
     double cc = 0.0, d = 0.0;
     if (has2) { cc = c2[2 * ig]; d = c2[2 * ig + 1]; }
@@ -170,18 +172,19 @@ __global__ __launch_bounds__(256) void k_zbwd(
   }
   __syncthreads();
   cpx<T>* res = lds_fft<T, +1>(A, B, plan, t.ncol, 1, TC, true, tw);
-  cpx<T>* zp = zbuf + (long)pair * nvec * np2;
+  cpx<T>* zp = zbuf + (long)pair * ncols * np2;
   const int tot = t.ncol * np2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / t.ncol, cl = i - k * t.ncol;
-    zp[(long)k * nvec + t.col0 + cl] = res[k * TC + cl];
+    const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
+    zp[(long)k * ncols + gcol] = res[k * TC + cl];
   }
 }
 
 // ----------------------------------------------------------------- x-bwd
 template <typename T>
 __global__ __launch_bounds__(256) void k_xbwd(
-    const cpx<T>* __restrict__ zbuf, cpx<T>* __restrict__ xbuf, int np0, int np2, int nvec,
+    const cpx<T>* __restrict__ zbuf, cpx<T>* __restrict__ xbuf, int np0, int np2, int ncols,
     int nact, int RB, int ngroups, const int32_t* __restrict__ row_start,
     const int32_t* __restrict__ rc_ivec, const int32_t* __restrict__ rc_x, FftPlan plan,
     const cpx<T>* __restrict__ tw) {
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(256) void k_xbwd(
   const int pair = blockIdx.y;
   zero_lds(A, RB * np0);
   __syncthreads();
-  const cpx<T>* zp = zbuf + (long)pair * nvec * np2 + (long)k * nvec;
+  const cpx<T>* zp = zbuf + (long)pair * ncols * np2 + (long)k * ncols;
   const int e0 = row_start[r0], e1 = row_start[r0 + nr];
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     // row of entry e: linear search over the few rows of this block
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(256) void k_xbwd(
 // ----------------------------------------------------------------- x-fwd
 template <typename T>
 __global__ __launch_bounds__(256) void k_xfwd(
-    const cpx<T>* __restrict__ xbuf, cpx<T>* __restrict__ zbuf, int np0, int np2, int nvec,
+    const cpx<T>* __restrict__ xbuf, cpx<T>* __restrict__ zbuf, int np0, int np2, int ncols,
     int nact, int RB, int ngroups, const int32_t* __restrict__ row_start,
     const int32_t* __restrict__ rc_ivec, const int32_t* __restrict__ rc_x, FftPlan plan,
     const cpx<T>* __restrict__ tw) {
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256) void k_xfwd(
   for (int i = threadIdx.x; i < nr * np0; i += blockDim.x) A[i] = xp[i];
   __syncthreads();
   cpx<T>* res = lds_fft<T, -1>(A, B, plan, nr, np0, 1, false, tw);
-  cpx<T>* zp = zbuf + (long)pair * nvec * np2 + (long)k * nvec;
+  cpx<T>* zp = zbuf + (long)pair * ncols * np2 + (long)k * ncols;
   const int e0 = row_start[r0], e1 = row_start[r0 + nr];
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     int r = r0;
@@ -240,18 +243,19 @@ template <typename T, int MODE, int OUT>
 __global__ __launch_bounds__(256) void k_zfwd(
     const cpx<T>* __restrict__ zbuf, double* __restrict__ out, long pstride, long off2,
     int last_has2, int nbatch, const int32_t* __restrict__ ip, const int32_t* __restrict__ im,
-    const qbg_tile* __restrict__ tiles, int np2, int nvec, int TC, FftPlan plan,
-    const cpx<T>* __restrict__ tw, double scale) {
+    const int32_t* __restrict__ glist, const qbg_tile* __restrict__ tiles, int np2, int ncols,
+    int TC, FftPlan plan, const cpx<T>* __restrict__ tw, double scale) {
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* B = A + TC * np2;
   const qbg_tile t = tiles[blockIdx.x];
   const int pair = blockIdx.y;
-  const cpx<T>* zp = zbuf + (long)pair * nvec * np2;
+  const cpx<T>* zp = zbuf + (long)pair * ncols * np2;
   const int tot = t.ncol * np2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int k = i / t.ncol, cl = i - k * t.ncol;
-    A[k * TC + cl] = zp[(long)k * nvec + t.col0 + cl];
+    const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
+    A[k * TC + cl] = zp[(long)k * ncols + gcol];
   }
   __syncthreads();
   cpx<T>* res = lds_fft<T, -1>(A, B, plan, t.ncol, 1, TC, true, tw);
@@ -259,7 +263,8 @@ __global__ __launch_bounds__(256) void k_zfwd(
   double* o1 = out + 2 * (long)pair * pstride;
   double* o2 = o1 + 2 * off2;
   const bool has2 = (MODE == Z_PAIR) && (pair < nbatch - 1 || last_has2);
-  for (int ig = t.g0 + threadIdx.x; ig < t.g1; ig += blockDim.x) {
+  for (int j = t.g0 + threadIdx.x; j < t.g1; j += blockDim.x) {
+    const int ig = glist[j];
     const int sp = ip[ig] - slot0;
     const int lp = (sp % np2) * TC + sp / np2;
     const cpx<T> zpv = res[lp];
@@ -418,11 +423,11 @@ static int zbwd_t(qbg_grid* g, int mode, const double* c, long pstride, long off
   dim3 grid(g->ntiles, nbatch);
   if (mode == Z_PAIR)
     hipLaunchKernelGGL((k_zbwd<T, Z_PAIR>), grid, dim3(256), lds, s, c, pstride, off2,
-                       last_has2, nbatch, g->d_ip, g->d_im, g->d_tiles, g->np2, g->nvec,
+                       last_has2, nbatch, g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->np2, g->ncols,
                        g->zcols, g->plan2, tw_of<T>(g, 2), (cpx<T>*)g->d_zbuf);
   else
     hipLaunchKernelGGL((k_zbwd<T, Z_SINGLE>), grid, dim3(256), lds, s, c, pstride, off2,
-                       last_has2, nbatch, g->d_ip, g->d_im, g->d_tiles, g->np2, g->nvec,
+                       last_has2, nbatch, g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->np2, g->ncols,
                        g->zcols, g->plan2, tw_of<T>(g, 2), (cpx<T>*)g->d_zbuf);
   return hipGetLastError() == hipSuccess ? QBG_OK : QBG_E_HIP;
 }
@@ -435,11 +440,11 @@ static int xpass_t(qbg_grid* g, bool fwd, int nbatch, hipStream_t s) {
   dim3 grid(g->np2 * ngroups, nbatch);
   if (!fwd)
     hipLaunchKernelGGL((k_xbwd<T>), grid, dim3(256), lds, s, (const cpx<T>*)g->d_zbuf,
-                       (cpx<T>*)g->d_xbuf, g->np0, g->np2, g->nvec, g->nact, g->xrows, ngroups,
+                       (cpx<T>*)g->d_xbuf, g->np0, g->np2, g->ncols, g->nact, g->xrows, ngroups,
                        g->d_row_start, g->d_rc_ivec, g->d_rc_x, g->plan0, tw_of<T>(g, 0));
   else
     hipLaunchKernelGGL((k_xfwd<T>), grid, dim3(256), lds, s, (const cpx<T>*)g->d_xbuf,
-                       (cpx<T>*)g->d_zbuf, g->np0, g->np2, g->nvec, g->nact, g->xrows, ngroups,
+                       (cpx<T>*)g->d_zbuf, g->np0, g->np2, g->ncols, g->nact, g->xrows, ngroups,
                        g->d_row_start, g->d_rc_ivec, g->d_rc_x, g->plan0, tw_of<T>(g, 0));
   return hipGetLastError() == hipSuccess ? QBG_OK : QBG_E_HIP;
 }
@@ -482,8 +487,8 @@ static int zfwd_t(qbg_grid* g, int mode, int outmode, double* out, long pstride,
   const double scale = 1.0 / ((double)g->np0 * g->np1 * g->np2);
 #define ZF(M, O)                                                                          \
   hipLaunchKernelGGL((k_zfwd<T, M, O>), grid, dim3(256), lds, s, (const cpx<T>*)g->d_zbuf, \
-                     out, pstride, off2, last_has2, nbatch, g->d_ip, g->d_im, g->d_tiles,   \
-                     g->np2, g->nvec, g->zcols, g->plan2, tw_of<T>(g, 2), scale)
+                     out, pstride, off2, last_has2, nbatch, g->d_ip, g->d_im, g->d_glist,   \
+                     g->d_tiles, g->np2, g->ncols, g->zcols, g->plan2, tw_of<T>(g, 2), scale)
   if (mode == Z_PAIR) {
     if (outmode == OUT_ACCUM) ZF(Z_PAIR, OUT_ACCUM); else ZF(Z_PAIR, OUT_STORE);
   } else {

@@ -19,7 +19,8 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
                                               long off2, int last_has2, int nbatch, int single,
                                               const int32_t* __restrict__ ip,
                                               const int32_t* __restrict__ im,
-                                              const qbg_tile* __restrict__ tiles, int nvec, int TC,
+                                              const int32_t* __restrict__ glist,
+                                              const qbg_tile* __restrict__ tiles, int ncols, int TC,
                                               const cpx<T>* __restrict__ tw_g,
                                               cpx<T>* __restrict__ zbuf) {
   constexpr int N = N1 * N2;
@@ -38,7 +39,8 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
   const double* c1 = c + 2 * (long)pair * pstride;
   const bool has2 = !single && (pair < nbatch - 1 || last_has2);
   const double* c2 = c1 + 2 * off2;
-  for (int ig = t.g0 + tid; ig < t.g1; ig += 256) {
+  for (int j = t.g0 + tid; j < t.g1; j += 256) {
+    const int ig = glist[j];
     const double a = c1[2 * ig], b = c1[2 * ig + 1];
     double cc = 0.0, d = 0.0;
     if (has2) { cc = c2[2 * ig]; d = c2[2 * ig + 1]; }
@@ -65,9 +67,10 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
 #pragma unroll
     for (int n2 = 0; n2 < N2; n2++) b[n2] = A[(q * N2 + n2) * TC + col];
     rdft<T, N2, +1>(b);
-    cpx<T>* zp = zbuf + (long)pair * nvec * N + t.col0 + col;
+    const int gcol = col < t.np ? t.col0 + col : t.colm + (col - t.np);
+    cpx<T>* zp = zbuf + (long)pair * ncols * N + gcol;
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) zp[(long)(q + N1 * k2) * nvec] = b[k2];
+    for (int k2 = 0; k2 < N2; k2++) zp[(long)(q + N1 * k2) * ncols] = b[k2];
   }
 }
 
@@ -78,7 +81,8 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
                                               int last_has2, int nbatch, int single, int accum,
                                               const int32_t* __restrict__ ip,
                                               const int32_t* __restrict__ im,
-                                              const qbg_tile* __restrict__ tiles, int nvec, int TC,
+                                              const int32_t* __restrict__ glist,
+                                              const qbg_tile* __restrict__ tiles, int ncols, int TC,
                                               const cpx<T>* __restrict__ tw_g, double scale) {
   constexpr int N = N1 * N2;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -91,10 +95,11 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   __syncthreads();
   const int col = tid % TC, q = tid / TC;
   if (col < t.ncol && q < N1) {           // forward phase A', q = k1
-    const cpx<T>* zp = zbuf + (long)pair * nvec * N + t.col0 + col;
+    const int gcol = col < t.np ? t.col0 + col : t.colm + (col - t.np);
+    const cpx<T>* zp = zbuf + (long)pair * ncols * N + gcol;
     cpx<T> b[N2];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) b[k2] = zp[(long)(q + N1 * k2) * nvec];
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = zp[(long)(q + N1 * k2) * ncols];
     rdft<T, N2, -1>(b);
 #pragma unroll
     for (int m1 = 1; m1 < N2; m1++)
@@ -122,7 +127,8 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   double* o1 = out + 2 * (long)pair * pstride;
   double* o2 = o1 + 2 * off2;
   const bool has2 = !single && (pair < nbatch - 1 || last_has2);
-  for (int ig = t.g0 + tid; ig < t.g1; ig += 256) {
+  for (int j = t.g0 + tid; j < t.g1; j += 256) {
+    const int ig = glist[j];
     const int sp = ip[ig] - slot0;
     const cpx<T> zpv = A[(sp % N) * TC + sp / N];
     const double a = (double)zpv.x * scale, b = (double)zpv.y * scale;
@@ -150,7 +156,7 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
 // colmap[r*N + x] = column (ivec) at (row r, x) or -1
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
-                                              cpx<T>* __restrict__ xbuf, int np2, int nvec,
+                                              cpx<T>* __restrict__ xbuf, int np2, int ncols,
                                               int nact, int ngroups, int gpb,
                                               const int32_t* __restrict__ colmap,
                                               const cpx<T>* __restrict__ tw_g) {
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
   const int gr0 = (blockIdx.x - k * nruns) * gpb, gr1 = min(ngroups, gr0 + gpb);
   const int pair = blockIdx.y;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  const cpx<T>* zp = zbuf + ((long)pair * np2 + k) * nvec;
+  const cpx<T>* zp = zbuf + ((long)pair * np2 + k) * ncols;
   for (int gr = gr0; gr < gr1; gr++) {
   const int r0 = gr * L;
   const int nr = min(L, nact - r0);
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
 // ------------------------------------------------------------------ x-fwd
 template <typename T, int N1, int N2>
 __global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
-                                              cpx<T>* __restrict__ zbuf, int np2, int nvec,
+                                              cpx<T>* __restrict__ zbuf, int np2, int ncols,
                                               int nact, int ngroups, int gpb,
                                               const int32_t* __restrict__ colmap,
                                               const cpx<T>* __restrict__ tw_g) {
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
 #pragma unroll
       for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + m1 * N1 + k1];
       rdft<T, N1, -1>(cv);
-      cpx<T>* zp = zbuf + ((long)pair * np2 + k) * nvec;
+      cpx<T>* zp = zbuf + ((long)pair * np2 + k) * ncols;
       const int32_t* cm = colmap + (long)(r0 + line) * N;
 #pragma unroll
       for (int m2 = 0; m2 < N1; m2++) {
@@ -433,8 +439,8 @@ struct Launch {
                      int last_has2, int nbatch, hipStream_t s) {
     const size_t lds = sizeof(cpx<T>) * ((size_t)g->zcols * N + N);
     hipLaunchKernelGGL((kz_bwd<T, N1, N2>), dim3(g->ntiles, nbatch), dim3(256), lds, s, c,
-                       pstride, off2, last_has2, nbatch, single, g->d_ip, g->d_im, g->d_tiles,
-                       g->nvec, g->zcols, twp<T>(g, 2), (cpx<T>*)g->d_zbuf);
+                       pstride, off2, last_has2, nbatch, single, g->d_ip, g->d_im, g->d_glist,
+                       g->d_tiles, g->ncols, g->zcols, twp<T>(g, 2), (cpx<T>*)g->d_zbuf);
   }
   static void zbwd(qbg_grid* g, int prec, int single, const double* c, long pstride, long off2,
                    int last_has2, int nbatch, hipStream_t s) {
@@ -449,7 +455,8 @@ struct Launch {
     const double scale = 1.0 / ((double)g->np0 * g->np1 * g->np2);
     hipLaunchKernelGGL((kz_fwd<T, N1, N2>), dim3(g->ntiles, nbatch), dim3(256), lds, s,
                        (const cpx<T>*)g->d_zbuf, out, pstride, off2, last_has2, nbatch, single,
-                       accum, g->d_ip, g->d_im, g->d_tiles, g->nvec, g->zcols, twp<T>(g, 2), scale);
+                       accum, g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->ncols, g->zcols, twp<T>(g, 2),
+                       scale);
   }
   static void zfwd(qbg_grid* g, int prec, int single, int accum, double* out, long pstride,
                    long off2, int last_has2, int nbatch, hipStream_t s) {
@@ -465,20 +472,20 @@ struct Launch {
     if (prec == QBG_PREC_FP64) {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2, g->nvec,
+                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2, g->ncols,
                            g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2, g->nvec,
+                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2, g->ncols,
                            g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
     } else {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2, g->nvec,
+                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2, g->ncols,
                            g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2, g->nvec,
+                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2, g->ncols,
                            g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
     }
   }

@@ -1,0 +1,301 @@
+// gemm_bf.hip -- projector GEMMs on the BF16 MFMA path.
+//
+// The operands are FP32 arrays; each value is split while it is staged into
+// LDS into NS bfloat16 parts, x = x0 + x1 (+ x2), each part the
+// round-to-nearest-even bfloat16 of the remainder of the previous ones (the
+// subtractions are exact).  A product is the sum of the part products whose
+// order stays within NS-1 (NS = 3: x0y0 x0y1 x1y0 x0y2 x1y1 x2y0; NS = 2:
+// x0y0 x0y1 x1y0), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  With
+// NS = 3 the dropped terms are below 2^-24 relative, the accuracy of an
+// FP32 product; NS = 2 keeps about 16 bits.
+//
+// Same product shape as gemm.hip: D[p][q] = sum_r P(r,p) Q(r,q) on a
+// 128 x 128 tile per block of 256 threads (2 x 2 waves of 64 x 64), with
+// P(r,p) = P[r + p*ldp] and Q(r,q) = Q[r + q*ldq] (TN) or Q[q + r*ldq] (NN).
+// All operand extents are padded (rows to BK, columns to 128, leading
+// dimensions to multiples of 4) and the padding holds zeros, so loads need
+// no bounds checks; stores are bounded by the valid extents.
+//
+// LDS holds the parts as [part][row][k] with k contiguous: the MFMA lane l
+// reads row l%16 and the 8 consecutive k at 8*(l/16) as one 16-byte load
+// (both operands use the same k order, so the products pair k correctly).
+// Accumulator e of lane l is D[4(l/16) + e][l%16].
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "energy.h"
+
+namespace qbg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int BT = 128;    // tile edge (p and q)
+constexpr int BK = 32;     // reduction chunk (one MFMA k-step)
+constexpr int LR = 40;     // LDS row pitch in bf16 (80 bytes: conflict-free 16-byte reads)
+
+// round-to-nearest-even bfloat16 bits of x; `back` is the value as FP32
+__device__ __forceinline__ unsigned short bf16_rne(float x, float& back) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  back = __uint_as_float(u);
+  return (unsigned short)(u >> 16);
+}
+
+// split 8 values into NS parts of 8 bf16 each
+template <int NS>
+__device__ __forceinline__ void split8(const float* v, u16x8 (&out)[NS]) {
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    float r = v[e], b;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      out[s][e] = bf16_rne(r, b);
+      r -= b;
+    }
+  }
+}
+
+struct BfArgs {
+  const float* P;
+  long ldp;
+  const float* Q;
+  long ldq;
+  int tp, tq;          // tiles along p and q
+  int np, nq;          // valid output extent (stores)
+  long nr;             // padded reduction extent (multiple of BK)
+  long rper;           // reduction length of one split (multiple of BK)
+  int pfast;           // block order: p tiles fastest (1) or q tiles fastest (0)
+  void* out;           // out[split*sstride + q + p*ldo]
+  long ldo, sstride;
+  int accumulate;
+};
+
+template <int NS, bool QR, typename TO>
+__global__ __launch_bounds__(256) void k_gemm_bf(BfArgs g) {
+  __shared__ __align__(16) unsigned short sP[NS][BT][LR];
+  __shared__ __align__(16) unsigned short sQ[NS][BT][LR];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // renumber blocks so runs of consecutive tiles stay on one XCD (shared L2)
+  const int nblk = gridDim.x;
+  int b = blockIdx.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int ntile = g.tp * g.tq;
+  const int split = b / ntile, tt = b - split * ntile;
+  int ip, iq;
+  if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
+  else { iq = tt % g.tq; ip = tt / g.tq; }
+  const long p0 = (long)ip * BT, q0 = (long)iq * BT;
+  const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
+  // staging: r-contiguous operands as (row = tid/2, 16 k at 16*(tid&1));
+  // q-contiguous Q as (q = tid&127, 16 k at 16*(tid>>7))
+  const int srow = tid >> 1, sk = (tid & 1) * 16;
+  const int qrow = QR ? srow : (tid & 127), qk = QR ? sk : (tid >> 7) * 16;
+  const float* Pp = g.P + (p0 + srow) * g.ldp + sk;
+  const float* Qp = QR ? g.Q + (q0 + srow) * g.ldq + sk : g.Q + q0 + qrow + (long)qk * g.ldq;
+  float pv[16], qv[16];
+  auto load = [&](long r0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float4 t = *reinterpret_cast<const float4*>(Pp + r0 + 4 * i);
+      pv[4 * i] = t.x; pv[4 * i + 1] = t.y; pv[4 * i + 2] = t.z; pv[4 * i + 3] = t.w;
+    }
+    if (QR) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float4 t = *reinterpret_cast<const float4*>(Qp + r0 + 4 * i);
+        qv[4 * i] = t.x; qv[4 * i + 1] = t.y; qv[4 * i + 2] = t.z; qv[4 * i + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) qv[i] = Qp[(r0 + i) * g.ldq];
+    }
+  };
+  auto store = [&]() {
+    u16x8 h[NS];
+    split8<NS>(pv, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sP[s][srow][sk]) = h[s];
+    split8<NS>(pv + 8, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sP[s][srow][sk + 8]) = h[s];
+    split8<NS>(qv, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sQ[s][qrow][qk]) = h[s];
+    split8<NS>(qv + 8, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sQ[s][qrow][qk + 8]) = h[s];
+  };
+  const int wp = (w & 1) * 64, wq = (w >> 1) * 64;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  load(rb);
+  for (long r0 = rb; r0 < re; r0 += BK) {
+    store();
+    __syncthreads();
+    if (r0 + BK < re) load(r0 + BK);
+    bf16x8 a[NS][4], bq[NS][4];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[s][i] = *reinterpret_cast<const bf16x8*>(&sP[s][wp + 16 * i + fr][fk]);
+        bq[s][i] = *reinterpret_cast<const bf16x8*>(&sQ[s][wq + 16 * i + fr][fk]);
+      }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v4f c = acc[i][j];
+        // smallest part products first
+        if constexpr (NS == 3) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], bq[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], bq[1][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[2][j], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], bq[0][j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[1][j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[0][j], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    __syncthreads();
+  }
+  TO* o = (TO*)g.out + (long)split * g.sstride;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const long p = p0 + wp + 16 * i + 4 * (lane >> 4) + e;
+        const long q = q0 + wq + 16 * j + fr;
+        if (p < g.np && q < g.nq) {
+          TO* d = o + q + p * g.ldo;
+          const TO v = (TO)acc[i][j][e];
+          *d = g.accumulate ? (TO)(*d + v) : v;
+        }
+      }
+}
+
+// out[q + p*ldo] = alpha sum_s partial[s][q + p*pitch] + gamma Q[q*ldq] P[p*ldp]
+__global__ __launch_bounds__(256) void k_bf_reduce(const float* __restrict__ partial, int S,
+                                                   int nq, int np, long pitch, long sstride,
+                                                   double alpha,
+                                                   double gamma, const float* __restrict__ Q,
+                                                   long ldq, const double* __restrict__ P,
+                                                   long ldp, double* __restrict__ out, long ldo) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)nq * np) return;
+  const int q = (int)(t % nq), p = (int)(t / nq);
+  double s = 0.0;
+  for (int k = 0; k < S; k++) s += (double)partial[(long)k * sstride + q + (long)p * pitch];
+  double v = alpha * s;
+  if (gamma != 0.0) v += gamma * (double)Q[(long)q * ldq] * P[(long)p * ldp];
+  out[q + (long)p * ldo] = v;
+}
+
+template <int NS, bool QR, typename TO>
+void launch(const BfArgs& g, int nsplit, hipStream_t s) {
+  const int nblk = g.tp * g.tq * nsplit;
+  hipLaunchKernelGGL((k_gemm_bf<NS, QR, TO>), dim3(nblk), dim3(256), 0, s, g);
+}
+
+}  // namespace
+
+int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int Np, long Kp,
+               int M, int N, const double* B64, long ldb64, double alpha, double gamma,
+               double* out, long ldo, float* work, size_t work_floats, int parts,
+               hipStream_t s) {
+  if (Mp % BT || Np % BT || Kp % BK || lda % 4 || ldb % 4 || M > Mp || N > Np) return -1;
+  BfArgs g{};
+  g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
+  g.tp = Np / BT; g.tq = Mp / BT;
+  g.np = Np; g.nq = Mp;
+  g.pfast = 0;
+  const long ksteps = Kp / BK;
+  const int ntile = g.tp * g.tq;
+  long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
+  S = std::min<long>(S, std::max<long>(1, ksteps / 8));
+  while (S > 1 && (size_t)S * Mp * Np > work_floats) S--;
+  if ((size_t)S * Mp * Np > work_floats) return -1;
+  g.nr = Kp;
+  g.rper = ((ksteps + S - 1) / S) * BK;
+  S = (Kp + g.rper - 1) / g.rper;
+  g.out = work; g.ldo = Mp; g.sstride = (long)Mp * Np; g.accumulate = 0;
+  if (parts == 3) launch<3, true, float>(g, (int)S, s);
+  else launch<2, true, float>(g, (int)S, s);
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL(k_bf_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, work,
+                     (int)S, M, N, (long)Mp, (long)Mp * Np, alpha, gamma, A, lda, B64, ldb64,
+                     out, ldo);
+  return 0;
+}
+
+template <typename TO>
+static int bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
+                 long K, int N, TO* C, long ldc, int accumulate, int parts, hipStream_t s) {
+  if (Kp % BT || Np % BT || Mp % BK || lda % 4 || ldf % 4) return -1;
+  BfArgs g{};
+  g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
+  g.tp = Np / BT; g.tq = (int)(Kp / BT);
+  g.np = N; g.nq = (int)K;
+  g.pfast = 1;
+  g.nr = Mp;
+  g.rper = Mp;
+  g.out = C; g.ldo = ldc; g.sstride = 0; g.accumulate = accumulate;
+  if (parts == 3) launch<3, false, TO>(g, 1, s);
+  else launch<2, false, TO>(g, 1, s);
+  return 0;
+}
+
+int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
+               long K, int N, double* C, long ldc, int accumulate, int parts, hipStream_t s) {
+  return bf_nn(A, lda, F, ldf, Kp, Np, Mp, K, N, C, ldc, accumulate, parts, s);
+}
+int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
+               long K, int N, float* C, long ldc, int accumulate, int parts, hipStream_t s) {
+  return bf_nn(A, lda, F, ldf, Kp, Np, Mp, K, N, C, ldc, accumulate, parts, s);
+}
+
+// ------------------------------------------------ FP32 operand staging
+// c (FP64 complex, ld ldc complex) -> c32[k + n*ldk], real view, K = 2 ngw
+__global__ void k_to_f32(const double* __restrict__ c, long ldc2, long K, int nst,
+                         float* __restrict__ c32, long ldk) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= K * nst) return;
+  const long k = t % K, n = t / K;
+  c32[k + n * ldk] = (float)c[k + n * ldc2];
+}
+void convert_f32(const double* c, long ldc2, long K, int nst, float* c32, long ldk,
+                 hipStream_t s) {
+  const long tot = K * nst;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c, ldc2, K,
+                     nst, c32, ldk);
+}
+
+// fout (FP64, M x nst, ld M) -> fout32[m + n*ldf] and W32[n + m*ldw] = occ_n fout[m,n]
+__global__ void k_fout_f32(const double* __restrict__ fout, const double* __restrict__ occ, int M,
+                           int nst, float* __restrict__ f32, long ldf, float* __restrict__ w32,
+                           long ldw) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)M * nst) return;
+  const int m = (int)(t % M), n = (int)(t / M);
+  const double v = fout[t];
+  f32[m + n * ldf] = (float)v;
+  if (w32) w32[n + m * ldw] = (float)(occ[n] * v);
+}
+void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32, long ldf,
+              float* w32, long ldw, hipStream_t s) {
+  const long tot = (long)M * nst;
+  hipLaunchKernelGGL(k_fout_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, fout, occ,
+                     M, nst, f32, ldf, w32, ldw);
+}
+
+}  // namespace qbg

@@ -186,7 +186,7 @@ int qbg_timer_stop(qbg_ctx* c, float* ms) {
 }  // extern "C"
 
 static void grid_free(qbg_grid* g) {
-  void* ptrs[] = {g->d_ip, g->d_im, g->d_tiles, g->d_row_start, g->d_rc_ivec, g->d_rc_x,
+  void* ptrs[] = {g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->d_row_start, g->d_rc_ivec, g->d_rc_x,
                   g->d_act_row, g->d_colmap, g->d_jrow, g->d_tw64[0], g->d_tw64[1], g->d_tw64[2], g->d_tw32[0],
                   g->d_tw32[1], g->d_tw32[2], g->d_zbuf, g->d_xbuf, g->d_fac};
   for (void* p : ptrs)
@@ -274,30 +274,7 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
   g->ytile = lines(np1);
   if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);
   if (g->zcols < 3) { delete g; return c->fail(QBG_E_INVAL, "grid: np2 too large"); }
-  // tiles: consecutive rods, at most zcols columns each
-  std::vector<qbg_tile> tiles;
-  {
-    const int nrods = (nvec + 1) / 2;
-    int r = 0;
-    while (r < nrods) {
-      qbg_tile t;
-      t.col0 = r == 0 ? 0 : 2 * r - 1;
-      int ncol = 0, r1 = r;
-      while (r1 < nrods) {
-        const int add = r1 == 0 ? 1 : 2;
-        if (ncol + add > g->zcols) break;
-        ncol += add;
-        r1++;
-      }
-      t.ncol = ncol;
-      t.g0 = gfirst[r];
-      t.g1 = r1 < nrods ? gfirst[r1] : ng;
-      tiles.push_back(t);
-      r = r1;
-    }
-  }
-  g->ntiles = (int)tiles.size();
-  // active rows and columns sorted by (row, x)
+  // active rows: y in [0, ntrans0) and [np1 - ntrans0, np1)
   int nt0 = 0;
   for (int iv = 0; iv < nvec; iv++) {
     const int xy = z2v[iv];
@@ -314,6 +291,61 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
     act_row[r] = r < nt0 ? r : np1 - 2 * nt0 + r;
     jrow[act_row[r]] = r;
   }
+  // Column numbering of the pass workspace.  Qbox numbers columns by rod
+  // (rods sorted by size); here the rods are ranked by the (row, x) of
+  // their +G column, rod of rank i owns column i (+G) and column nrods + i
+  // (-G; unused for rod (0,0)).  The columns of one grid row then form two
+  // runs in x order (ascending +G, descending -G), so the x passes read and
+  // write coalesced; a z tile is a run of ranks [t0, t1) and owns columns
+  // [t0, t1) and [nrods + t0, nrods + t1), its G vectors listed in glist.
+  const int nrods = (nvec + 1) / 2;
+  g->ncols = 2 * nrods;
+  std::vector<int> rank(nrods), byrank(nrods);
+  {
+    auto pcol = [&](int r) { return r == 0 ? 0 : 2 * r - 1; };
+    for (int r = 0; r < nrods; r++) byrank[r] = r;
+    std::stable_sort(byrank.begin(), byrank.end(), [&](int a, int b) {
+      const int ya = jrow[z2v[pcol(a)] / np0], yb = jrow[z2v[pcol(b)] / np0];
+      if (ya != yb) return ya < yb;
+      return z2v[pcol(a)] % np0 < z2v[pcol(b)] % np0;
+    });
+    for (int i = 0; i < nrods; i++) rank[byrank[i]] = i;
+  }
+  auto qcol = [&](int iv) {   // workspace column of Qbox column iv
+    if (iv == 0) return rank[0];
+    const int r = (iv + 1) / 2;
+    return iv % 2 == 1 ? rank[r] : nrods + rank[r];
+  };
+  auto gend = [&](int r) { return r + 1 < nrods ? gfirst[r + 1] : ng; };
+  std::vector<qbg_tile> tiles;
+  std::vector<int32_t> glist;
+  glist.reserve(ng);
+  std::vector<int32_t> vip(ng), vim(ng);
+  {
+    const int rmax = std::max(1, g->zcols / 2);
+    for (int t0 = 0; t0 < nrods; t0 += rmax) {
+      const int t1 = std::min(nrods, t0 + rmax), nr = t1 - t0;
+      qbg_tile t;
+      t.col0 = t0;
+      t.ncol = 2 * nr;
+      t.np = nr;
+      t.colm = nrods + t0;
+      t.g0 = (int)glist.size();
+      for (int i = t0; i < t1; i++) {
+        const int r = byrank[i];
+        for (int ig = gfirst[r]; ig < gend(r); ig++) {
+          glist.push_back(ig);
+          // tile-local slots offset by col0: local column (i - t0) for +G,
+          // nr + (i - t0) for -G (rod (0,0): both in its +G column)
+          vip[ig] = i * np2 + ip[ig] % np2;
+          vim[ig] = (r == 0 ? i : nr + i) * np2 + im[ig] % np2;
+        }
+      }
+      t.g1 = (int)glist.size();
+      tiles.push_back(t);
+    }
+  }
+  g->ntiles = (int)tiles.size();
   std::vector<int32_t> row_start(g->nact + 1, 0), rc_ivec(nvec), rc_x(nvec);
   {
     std::vector<int> rowof(nvec);
@@ -330,7 +362,7 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
       return z2v[a] % np0 < z2v[b] % np0;
     });
     for (int e = 0; e < nvec; e++) {
-      rc_ivec[e] = order[e];
+      rc_ivec[e] = qcol(order[e]);
       rc_x[e] = z2v[order[e]] % np0;
     }
     // columns map to distinct grid points
@@ -340,16 +372,16 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
         return c->fail(QBG_E_INVAL, "grid: two columns on one grid point");
       }
   }
-  // dense (row, x) -> column map for the four-step x passes
+  // dense (row, x) -> workspace column map for the four-step x passes
   std::vector<int32_t> colmap((size_t)g->nact * np0, -1);
   for (int iv = 0; iv < nvec; iv++) {
     const int x = z2v[iv] % np0, y = z2v[iv] / np0;
-    colmap[(size_t)jrow[y] * np0 + x] = iv;
+    colmap[(size_t)jrow[y] * np0 + x] = qcol(iv);
   }
   (void)hipSetDevice(c->device);
   int rc = QBG_OK;
-  std::vector<int32_t> vip(ip, ip + ng), vim(im, im + ng);
   if ((rc = upload(c, &g->d_ip, vip)) || (rc = upload(c, &g->d_im, vim)) ||
+      (rc = upload(c, &g->d_glist, glist)) ||
       (rc = upload(c, &g->d_tiles, tiles)) || (rc = upload(c, &g->d_row_start, row_start)) ||
       (rc = upload(c, &g->d_rc_ivec, rc_ivec)) || (rc = upload(c, &g->d_rc_x, rc_x)) ||
       (rc = upload(c, &g->d_act_row, act_row)) || (rc = upload(c, &g->d_colmap, colmap)) ||
@@ -396,7 +428,7 @@ int qbg_grid_destroy(qbg_grid* g) {
 // pass launchers refuse to run a batch larger than g->ws_pairs.
 int qbg::grid_ensure_ws(qbg_grid* g, int npairs) {
   if (npairs <= g->ws_pairs) return QBG_OK;
-  const size_t zb = (size_t)npairs * g->nvec * g->np2 * 16;
+  const size_t zb = (size_t)npairs * g->ncols * g->np2 * 16;
   const size_t xb = (size_t)npairs * g->nact * g->np0 * g->np2 * 16;
   qbg_ctx* c = g->ctx;
   (void)hipStreamSynchronize(c->stream);

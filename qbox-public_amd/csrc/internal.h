@@ -51,9 +51,11 @@ struct qbg_ctx {
   }
 };
 
-// column tile of the z pass: columns [col0, col0+ncol), G range [g0, g1)
+// column tile of the z pass: ncol workspace columns, the first np of them
+// at [col0, col0+np) (+G columns), the rest at [colm, colm+ncol-np) (-G
+// columns); its G vectors are glist[g0, g1)
 struct qbg_tile {
-  int col0, ncol, g0, g1;
+  int col0, ncol, g0, g1, np, colm;
 };
 
 namespace qbg {
@@ -83,14 +85,16 @@ struct qbg_grid {
   int ytile = 8;       // x values per y-pass block
   qbg::FftPlan plan0, plan1, plan2;
   // device tables
-  int32_t* d_ip = nullptr;
-  int32_t* d_im = nullptr;
+  int ncols = 0;       // workspace columns per plane (2 * rods)
+  int32_t* d_ip = nullptr;          // tile-local slots of +G (offset by col0*np2)
+  int32_t* d_im = nullptr;          // tile-local slots of -G
+  int32_t* d_glist = nullptr;       // G indices of the tiles, tile by tile
   qbg_tile* d_tiles = nullptr;
   int32_t* d_row_start = nullptr;   // nact+1
-  int32_t* d_rc_ivec = nullptr;     // nvec, columns sorted by (row, x)
+  int32_t* d_rc_ivec = nullptr;     // nvec, workspace columns sorted by (row, x)
   int32_t* d_rc_x = nullptr;        // nvec
   int32_t* d_act_row = nullptr;     // nact: y index of active row r
-  int32_t* d_colmap = nullptr;      // nact*np0: column at (row, x) or -1
+  int32_t* d_colmap = nullptr;      // nact*np0: workspace column at (row, x) or -1
   int32_t* d_jrow = nullptr;        // np1: active row of y or -1
   const qbg::F4Ops* f4[3] = {nullptr, nullptr, nullptr};   // per axis, or generic
   double* d_tw64[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i k / n)

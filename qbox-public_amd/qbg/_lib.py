@@ -22,7 +22,7 @@ QBG_E_NOMEM = -2
 QBG_E_HIP = -3
 QBG_E_NOT_SPD = -4
 QBG_E_STATE = -5
-PREC = {"fp64": 0, "fp32": 1}
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x2": 3}
 
 _vp = C.c_void_p
 _dp = C.POINTER(C.c_double)
