@@ -67,6 +67,10 @@ struct qbg_ef {
   double *d_tau = nullptr, *d_b = nullptr, *d_sftab = nullptr, *d_sfac = nullptr;
   double *d_vion = nullptr, *d_dvion = nullptr, *d_rhopst = nullptr;
   int32_t *d_sp_first = nullptr, *d_atom_sp = nullptr;
+  // local forces: atom groups (first, count, species) and vtemp per species
+  int32_t* d_lfgrp = nullptr;
+  int nlfgrp = 0;
+  double* d_vtemp = nullptr;
   // device: state
   double *d_rhor = nullptr, *d_vxc = nullptr, *d_vr = nullptr, *d_rhog = nullptr;
   double *d_rhoelg = nullptr, *d_rhogt = nullptr, *d_vlocal = nullptr, *d_occ = nullptr;
@@ -110,7 +114,7 @@ static void ef_free(qbg_ef* e) {
                e->d_m_ipr, e->d_m_npr, e->d_m_stride, e->d_m_dbase, e->d_m_wt, e->d_dmat,
                e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work,
                e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
-               e->b_y32, e->b_fout, e->b_w32, e->b_part};
+               e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->gw) qbg_grid_destroy(e->gw);
@@ -352,6 +356,18 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     krange = std::max(krange, e->vb.idxmax[i] - e->vb.idxmin[i] + 1);
   }
   e->krange = krange;
+  // atom groups of the local-force kernel: consecutive atoms of one species
+  std::vector<int32_t> lfgrp;
+  {
+    const int ag = local_forces_group_size(krange);
+    for (int is = 0; is < nsp; is++)
+      for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a += ag) {
+        lfgrp.push_back(a);
+        lfgrp.push_back(std::min(ag, e->sp_first[is + 1] - a));
+        lfgrp.push_back(is);
+      }
+    e->nlfgrp = (int)lfgrp.size() / 3;
+  }
   std::vector<double> bm(9);
   for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
   std::vector<int32_t> atom_sp(e->nat), spf(e->sp_first.begin(), e->sp_first.end());
@@ -380,6 +396,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       (rc = dput(c, &e->d_dmat, dmat)) || (rc = dput(c, &e->d_a_m0, a_m0)) ||
       (rc = dput(c, &e->d_a_npr, a_npr)) || (rc = dput(c, &e->d_a_stride, a_stride)) ||
       (rc = dput(c, &e->d_dtwnl, dtwnl)) || (rc = dput(c, &e->d_dt_off, dt_off)) ||
+      (rc = dput(c, &e->d_lfgrp, lfgrp)) || (rc = dalloc(c, &e->d_vtemp, (size_t)2 * nsp * ngv)) ||
       (rc = dput(c, &e->d_sp_npr, sp_npr)) || (rc = dput(c, &e->d_fac2, fac2)) ||
       (rc = dalloc(c, &e->d_tau, 3 * e->nat)) ||
       (rc = dalloc(c, &e->d_sftab, (size_t)e->nat * 3 * krange * 2)) ||
@@ -528,8 +545,8 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_begin(QBG_K_NL);
   const bool f32 = prec == QBG_PREC_FP32;
   float* anl32 = reinterpret_cast<float*>(e->d_anl);
-  launch_anl(ngw, M, 1, K, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l,
-             e->d_twnl, e->d_anl, f32 ? 1 : 0, s);
+  launch_anl(ngw, e->nat, K, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
+             e->d_m_twnl, e->d_m_l, e->d_twnl, e->d_anl, f32 ? 1 : 0, s);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_GEMM);
   const int grc = f32 ? gemm_tn_splitk(anl32, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M,
@@ -628,8 +645,8 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   const int ngw = e->wb.size, nst = e->nst, M = e->M, Mp = e->Mp, Np = e->Np;
   const long K = 2L * ngw, Kp = e->Kp;
   ctx->prof_begin(QBG_K_NL);
-  launch_anl(ngw, M, 1, Kp, e->d_gxw, e->d_tau, e->d_m_atom, e->d_m_twnl, e->d_m_l, e->d_twnl,
-             e->b_anl, 1, s);
+  launch_anl(ngw, e->nat, Kp, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
+             e->d_m_twnl, e->d_m_l, e->d_twnl, e->b_anl, 1, s);
   convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_GEMM);
@@ -717,9 +734,10 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
   if (compute_forces) {
     ctx->prof_begin(QBG_K_OTHER);
     const int nchunk = 64;
-    launch_local_forces(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->d_atom_sp,
-                        e->nat, e->d_vps, e->d_rhops, e->d_g2iv, e->d_gxv, e->d_rhogt,
-                        e->d_rhoelg, e->omega, e->d_fesr, e->d_partial, nchunk, e->d_fion, s);
+    launch_local_forces(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->d_lfgrp,
+                        e->nlfgrp, e->nsp, e->d_vps, e->d_rhops, e->d_g2iv, e->d_gxv, e->d_rhogt,
+                        e->d_rhoelg, e->omega, e->d_fesr, e->nat, e->d_vtemp, e->d_partial,
+                        nchunk, e->d_fion, s);
     ctx->prof_end();
   }
   // scalars back to the host
@@ -825,8 +843,7 @@ int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
   };
   c->prof_begin(QBG_K_GRAM);
   cleanup(cc, ldc, nst, s);
-  if (gemm_tn_splitk(cc, 2L * ldc, cc, 2L * ldc, nst, nst, K, 2.0, -1.0, S, nst, W, work,
-                     QBG_PREC_FP64, s)) {
+  if (gemm_syrk(cc, 2L * ldc, nst, K, 2.0, -1.0, S, nst, W, work, s)) {
     (void)hipStreamSynchronize(s);
     release();
     return c->fail(QBG_E_INVAL, "gram workspace");
@@ -845,7 +862,7 @@ int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
   }
   c->prof_begin(QBG_K_GRAM);
   trtri(S, nst, nst, X, s);
-  gemm_nn(cc, 2L * ldc, X, nst, K, nst, nst, T, K, 0, QBG_PREC_FP64, s);
+  gemm_nn_triu(cc, 2L * ldc, X, nst, K, nst, T, K, s);
   (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
                          nst, hipMemcpyDeviceToDevice, s);
   c->prof_end();

@@ -25,15 +25,20 @@ void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, con
                  int nsp, const int32_t* sp_first, const double* vps, const double* dvps,
                  const double* rhops, double* sfac, double* vion, double* dvion, double* rhopst,
                  hipStream_t s);
+// local forces; grp[3*g + {0,1,2}] = first atom, atoms (<= the group size),
+// species of group g; vt holds 2*nsp*ngv doubles
+int local_forces_group_size(int krange);
 void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int krange,
-                         const int* kmin, const int32_t* atom_sp, int nat, const double* vps,
-                         const double* rhops, const double* g2i, const double* gx,
-                         const double* rhogt, const double* rhoelg, double omega,
-                         const double* fesr, double* partial, int nchunk, double* fion,
-                         hipStream_t s);
-void launch_anl(int ngw, int M, int nmodes, long lda, const double* gx, const double* tau,
-                const int32_t* m_atom, const int32_t* m_twnl, const int32_t* m_l,
-                const double* twnl, void* A, int prec, hipStream_t s);
+                         const int* kmin, const int32_t* grp, int ngrp, int nsp,
+                         const double* vps, const double* rhops, const double* g2i,
+                         const double* gx, const double* rhogt, const double* rhoelg,
+                         double omega, const double* fesr, int nat, double* vt, double* partial,
+                         int nchunk, double* fion, hipStream_t s);
+// anl columns of every atom's projectors (FP64 if prec == 0, else FP32)
+void launch_anl(int ngw, int nat, long lda, const double* gx, const double* tau,
+                const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                const int32_t* m_twnl, const int32_t* m_l, const double* twnl, void* A, int prec,
+                hipStream_t s);
 int launch_dmat_enl(const double* F, int M, int nst, const double* occ, const int32_t* m_ipr,
                     const int32_t* m_npr, const int32_t* m_stride, const int32_t* m_dbase,
                     const double* dmat, const double* m_wt, double omega_inv, double* fout,
@@ -69,6 +74,14 @@ int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, 
 int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
                    size_t work_doubles, int prec, hipStream_t s);
+// symmetric TN (FP64): out = alpha A^T A + gamma a0 a0^T, only the lower
+// tiles formed, the upper part mirrored
+int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gamma, double* out,
+              long ldo, double* work, size_t work_doubles, hipStream_t s);
+// NN (FP64) with F upper triangular (F[m + n*ldf] = 0 for m > n), N x N:
+// C[k + n*ldc] = sum_{m <= n} A[k + m*lda] F[m + n*ldf]
+void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
+                  long ldc, hipStream_t s);
 // NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf]  (K rows, N cols, M sum)
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
              double* C, long ldc, int accumulate, int prec, hipStream_t s);

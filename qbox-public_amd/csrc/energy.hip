@@ -6,6 +6,7 @@
 // writes one partial per block; a second kernel adds the partials in block
 // order.  Results are therefore bit-reproducible run to run.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "energy.h"
 
 namespace qbg {
@@ -25,6 +26,26 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __re
     __syncthreads();
   }
   if (threadIdx.x < NV) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// the same sums through wave shuffles, then the 4 wave totals in order
+template <int NV>
+__device__ __forceinline__ void block_reduce_shfl(double (&v)[NV], double* __restrict__ out) {
+  __shared__ double red[NV][4];
+#pragma unroll
+  for (int q = 0; q < NV; q++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) red[q][w] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int q = threadIdx.x;
+    out[q] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+  }
 }
 
 // out[q] = sum_b partial[b*nv + q] in block order; one block of 256 threads
@@ -298,38 +319,69 @@ void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, con
 
 // ---------------------------------------------------------- local forces
 // src/EnergyFunctional.cpp:841-918: per atom a of species s,
-//   vtemp = 4 pi rhops g2i conj(rhogt) + vps conj(rhoelg)
-//   f_a,j = -2 omega sum_G (Re e * Im vtemp + Im e * Re vtemp) G_j
-// grid (chunks, atoms); partial[(a*nchunk + chunk)*3 + j]
-__global__ __launch_bounds__(256) void k_local_forces(
-    const int32_t* __restrict__ idx, int ngv, const double* __restrict__ tab, int krange,
-    int kmin0, int kmin1, int kmin2, const int32_t* __restrict__ atom_sp,
-    const double* __restrict__ vps, const double* __restrict__ rhops,
-    const double* __restrict__ g2i, const double* __restrict__ gx,
-    const double* __restrict__ rhogt, const double* __restrict__ rhoelg, int chunk,
-    double* __restrict__ partial) {
-  const int a = blockIdx.y;
-  const int is = atom_sp[a];
-  const int g0 = blockIdx.x * chunk, g1 = min(ngv, g0 + chunk);
+//   vtemp_s = 4 pi rhops_s g2i conj(rhogt) + vps_s conj(rhoelg)
+//   f_a,j = -2 omega sum_G (Re e * Im vtemp_s + Im e * Re vtemp_s) G_j
+// vtemp_s is formed once per species; a block then takes a chunk of G and a
+// group of up to LF_AG atoms of one species, with the atoms' per-axis phase
+// tables in LDS, so the G data is read once per group instead of per atom.
+constexpr int LF_AG = 8;
+
+__global__ __launch_bounds__(256) void k_vtemp(int ngv, int nsp, const double* __restrict__ vps,
+                                               const double* __restrict__ rhops,
+                                               const double* __restrict__ g2i,
+                                               const double* __restrict__ rhogt,
+                                               const double* __restrict__ rhoelg,
+                                               double* __restrict__ vt) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngv) return;
   const double fpi = 4.0 * M_PI;
-  double v[3] = {0, 0, 0};
-  for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
+  const double gtr = rhogt[2 * ig], gti = rhogt[2 * ig + 1];
+  const double elr = rhoelg[2 * ig], eli = rhoelg[2 * ig + 1];
+  for (int is = 0; is < nsp; is++) {
     const long o = (long)is * ngv + ig;
     const double tmp = fpi * rhops[o] * g2i[ig];
-    const double gtr = rhogt[2 * ig], gti = rhogt[2 * ig + 1];
-    const double elr = rhoelg[2 * ig], eli = rhoelg[2 * ig + 1];
     // conj: (x + iy)* = x - iy
-    const double vtr = tmp * gtr + vps[o] * elr;
-    const double vti = -tmp * gti - vps[o] * eli;
-    double er, ei;
-    eigr_from_tables(tab, a, krange, idx[3 * ig] - kmin0, idx[3 * ig + 1] - kmin1,
-                     idx[3 * ig + 2] - kmin2, er, ei);
-    const double t = er * vti + ei * vtr;
-    v[0] += t * gx[ig];
-    v[1] += t * gx[ngv + ig];
-    v[2] += t * gx[2 * ngv + ig];
+    vt[2 * o] = tmp * gtr + vps[o] * elr;
+    vt[2 * o + 1] = -tmp * gti - vps[o] * eli;
   }
-  block_reduce_store<3>(v, partial + ((long)a * gridDim.x + blockIdx.x) * 3);
+}
+
+// grid (chunks, groups); partial[(a*nchunk + chunk)*3 + j]
+__global__ __launch_bounds__(256) void k_local_forces(
+    const int32_t* __restrict__ idx, int ngv, const double* __restrict__ tab, int krange,
+    int kmin0, int kmin1, int kmin2, const int32_t* __restrict__ grp, const double* __restrict__ vt,
+    const double* __restrict__ gx, int chunk, double* __restrict__ partial) {
+  extern __shared__ __align__(16) double ltab[];   // [LF_AG][3][krange][2]
+  const int a0 = grp[3 * blockIdx.y], na = grp[3 * blockIdx.y + 1], is = grp[3 * blockIdx.y + 2];
+  const int tsz = 3 * krange * 2;
+  for (int i = threadIdx.x; i < na * tsz; i += 256) ltab[i] = tab[(long)a0 * tsz + i];
+  __syncthreads();
+  const int g0 = blockIdx.x * chunk, g1 = min(ngv, g0 + chunk);
+  double v[3 * LF_AG];
+#pragma unroll
+  for (int q = 0; q < 3 * LF_AG; q++) v[q] = 0.0;
+  const double* vts = vt + 2 * (long)is * ngv;
+  for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
+    const double vtr = vts[2 * ig], vti = vts[2 * ig + 1];
+    const double gxx = gx[ig], gyy = gx[ngv + ig], gzz = gx[2 * ngv + ig];
+    const int kx = idx[3 * ig] - kmin0, ky = idx[3 * ig + 1] - kmin1, kz = idx[3 * ig + 2] - kmin2;
+#pragma unroll
+    for (int q = 0; q < LF_AG; q++) {
+      if (q < na) {
+        double er, ei;
+        eigr_from_tables(ltab + q * tsz, 0, krange, kx, ky, kz, er, ei);
+        const double t = er * vti + ei * vtr;
+        v[3 * q] += t * gxx;
+        v[3 * q + 1] += t * gyy;
+        v[3 * q + 2] += t * gzz;
+      }
+    }
+  }
+  __shared__ double res[3 * LF_AG];
+  block_reduce_shfl<3 * LF_AG>(v, res);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * na; i += 256)
+    partial[((long)(a0 + i / 3) * gridDim.x + blockIdx.x) * 3 + i % 3] = res[i];
 }
 
 // sum chunks per atom: f[3a+j] = fion_esr[3a+j] - 2 omega sum_chunk partial
@@ -344,70 +396,73 @@ __global__ void k_local_forces_final(const double* __restrict__ partial, int nch
   fion[t] += fesr[t] + (-2.0 * omega) * s;
 }
 
+// atoms per group: at most LF_AG, and their tables within 64 KiB of LDS
+int local_forces_group_size(int krange) {
+  return std::max(1, std::min(LF_AG, 65536 / (int)(sizeof(double) * 6 * krange)));
+}
+
 void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int krange,
-                         const int* kmin, const int32_t* atom_sp, int nat, const double* vps,
-                         const double* rhops, const double* g2i, const double* gx,
-                         const double* rhogt, const double* rhoelg, double omega,
-                         const double* fesr, double* partial, int nchunk, double* fion,
-                         hipStream_t s) {
+                         const int* kmin, const int32_t* grp, int ngrp, int nsp,
+                         const double* vps, const double* rhops, const double* g2i,
+                         const double* gx, const double* rhogt, const double* rhoelg,
+                         double omega, const double* fesr, int nat, double* vt, double* partial,
+                         int nchunk, double* fion, hipStream_t s) {
+  hipLaunchKernelGGL(k_vtemp, dim3((ngv + 255) / 256), dim3(256), 0, s, ngv, nsp, vps, rhops, g2i,
+                     rhogt, rhoelg, vt);
   const int chunk = (ngv + nchunk - 1) / nchunk;
-  hipLaunchKernelGGL(k_local_forces, dim3(nchunk, nat), dim3(256), 0, s, idx, ngv, tab, krange,
-                     kmin[0], kmin[1], kmin[2], atom_sp, vps, rhops, g2i, gx, rhogt, rhoelg,
-                     chunk, partial);
+  const size_t lds = sizeof(double) * (size_t)local_forces_group_size(krange) * 3 * krange * 2;
+  hipLaunchKernelGGL(k_local_forces, dim3(nchunk, ngrp), dim3(256), lds, s, idx, ngv, tab,
+                     krange, kmin[0], kmin[1], kmin[2], grp, vt, gx, chunk, partial);
   hipLaunchKernelGGL(k_local_forces_final, dim3((3 * nat + 255) / 256), dim3(256), 0, s,
                      partial, nchunk, nat, omega, fesr, fion);
 }
 
 // ------------------------------------------------------- projectors anl
-// src/NonLocalPotential.cpp:1314-1395 (anl = twnl (-i)^l e^{-iG.tau}) and
-// :1642-1698 (anl_j = -i G_j anl) for l = 0, 1.  Column m of block `mode`
-// (0: anl, 1..3: direction j = mode-1) holds atom a = m_atom[m] and
-// projector ipr = m_ipr[m]; A[(mode*M + m)*lda + 2*ig + {0,1}].
+// src/NonLocalPotential.cpp:1314-1395: anl = twnl (-i)^l e^{-iG.tau} for
+// l = 0, 1.  One thread per (G, atom) forms the phase once and writes the
+// atom's projector columns m = a_m0[a] + ipr*a_stride[a]:
+// A[m*lda + 2*ig + {0,1}].
 template <typename T>
-__global__ __launch_bounds__(256) void k_anl(int ngw, int M, int nmodes, long lda,
-                                             const double* __restrict__ gx,
+__global__ __launch_bounds__(256) void k_anl(int ngw, long lda, const double* __restrict__ gx,
                                              const double* __restrict__ tau,
-                                             const int32_t* __restrict__ m_atom,
+                                             const int32_t* __restrict__ a_m0,
+                                             const int32_t* __restrict__ a_npr,
+                                             const int32_t* __restrict__ a_stride,
                                              const int32_t* __restrict__ m_twnl,
                                              const int32_t* __restrict__ m_l,
                                              const double* __restrict__ twnl,
                                              T* __restrict__ A) {
   const int ig = blockIdx.x * 256 + threadIdx.x;
-  const int m = blockIdx.y;
+  const int a = blockIdx.y;
   if (ig >= ngw) return;
-  const int a = m_atom[m];
-  const int l = m_l[m];
-  const double t = twnl[(long)m_twnl[m] + ig];
   const double gxx = gx[ig], gyy = gx[ngw + ig], gzz = gx[2 * ngw + ig];
   const double arg = -(gxx * tau[3 * a] + gyy * tau[3 * a + 1] + gzz * tau[3 * a + 2]);
   double s, c;
   sincos(arg, &s, &c);
-  for (int mode = 0; mode < nmodes; mode++) {
+  const int m0 = a_m0[a], npr = a_npr[a], st = a_stride[a];
+  for (int ipr = 0; ipr < npr; ipr++) {
+    const int m = m0 + ipr * st;
+    const double t = twnl[(long)m_twnl[m] + ig];
     double re, im;
-    if (mode == 0) {
-      if (l == 0) { re = t * c; im = t * s; }
-      else { re = t * s; im = -t * c; }
-    } else {
-      const double gj = mode == 1 ? gxx : (mode == 2 ? gyy : gzz);
-      if (l == 0) { const double tt = gj * t; re = tt * s; im = -tt * c; }
-      else { const double tt = -gj * t; re = tt * c; im = tt * s; }
-    }
-    T* col = A + ((long)mode * M + m) * lda;
+    if (m_l[m] == 0) { re = t * c; im = t * s; }
+    else { re = t * s; im = -t * c; }
+    T* col = A + (long)m * lda;
     col[2 * ig] = (T)re;
     col[2 * ig + 1] = (T)im;
   }
 }
 
-void launch_anl(int ngw, int M, int nmodes, long lda, const double* gx, const double* tau,
-                const int32_t* m_atom, const int32_t* m_twnl, const int32_t* m_l,
-                const double* twnl, void* A, int prec, hipStream_t s) {
-  dim3 grid((ngw + 255) / 256, M);
+void launch_anl(int ngw, int nat, long lda, const double* gx, const double* tau,
+                const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
+                const int32_t* m_twnl, const int32_t* m_l, const double* twnl, void* A, int prec,
+                hipStream_t s) {
+  dim3 grid((ngw + 255) / 256, nat);
   if (prec == 0)
-    hipLaunchKernelGGL(k_anl<double>, grid, dim3(256), 0, s, ngw, M, nmodes, lda, gx, tau, m_atom,
-                       m_twnl, m_l, twnl, (double*)A);
+    hipLaunchKernelGGL(k_anl<double>, grid, dim3(256), 0, s, ngw, lda, gx, tau, a_m0, a_npr,
+                       a_stride, m_twnl, m_l, twnl, (double*)A);
   else
-    hipLaunchKernelGGL(k_anl<float>, grid, dim3(256), 0, s, ngw, M, nmodes, lda, gx, tau, m_atom,
-                       m_twnl, m_l, twnl, (float*)A);
+    hipLaunchKernelGGL(k_anl<float>, grid, dim3(256), 0, s, ngw, lda, gx, tau, a_m0, a_npr,
+                       a_stride, m_twnl, m_l, twnl, (float*)A);
 }
 
 // ------------------------------------------------- D matrix, enl, forces

@@ -2,8 +2,10 @@
 // compile-time length (N1 x N2), and their launchers.  Instantiated per line
 // length in fft4_sizes.hip; dispatched from fft.hip through f4_ops().
 //
-// Index conventions (as in fft.hip): zbuf[pair][k][ivec], xbuf[pair][k][r][x],
-// real-space grids i + np0*(j + np1*k).  Within a line of length N,
+// Workspace layouts (blocked so that every block moves long contiguous
+// runs): zbuf[pair][kb][col][kk] holds plane k = KB*kb + kk of workspace
+// column col; xbuf[pair][k][xb][r][xi] holds x = XB*xb + xi of active row r
+// of plane k.  Real-space grids are i + np0*(j + np1*k).  Within a line of length N,
 // phase A holds x[N2*n1 + n2] (thread n2) and phase B holds X[k1 + N1*k2]
 // (thread k1); a forward transform starting from the phase-B layout
 // (thread k1 holding y = k1 + N1*k2) ends in the phase-A layout.
@@ -12,6 +14,14 @@
 
 namespace qbg {
 namespace f4 {
+
+constexpr int KB = 16;   // planes per zbuf block
+constexpr int XB = 16;   // x per xbuf block
+__host__ __device__ constexpr int nblk(int n, int b) { return (n + b - 1) / b; }
+// threads of an x or y block: 16 lines of max(N1, N2) threads, whole waves
+__host__ __device__ constexpr int xthreads(int n1, int n2) {
+  return ((16 * (n1 > n2 ? n1 : n2) + 63) / 64) * 64;
+}
 
 // ------------------------------------------------------------------ z-bwd
 template <typename T, int N1, int N2>
@@ -62,15 +72,29 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
     for (int k1 = 0; k1 < N1; k1++) A[(k1 * N2 + q) * TC + col] = a[k1];
   }
   __syncthreads();
-  if (col < t.ncol && q < N1) {           // phase B, q = k1
-    cpx<T> b[N2];
+  cpx<T> b[N2];
+  const bool vb = col < t.ncol && q < N1;
+  if (vb) {                               // phase B, q = k1
 #pragma unroll
     for (int n2 = 0; n2 < N2; n2++) b[n2] = A[(q * N2 + n2) * TC + col];
     rdft<T, N2, +1>(b);
-    const int gcol = col < t.np ? t.col0 + col : t.colm + (col - t.np);
-    cpx<T>* zp = zbuf + (long)pair * ncols * N + gcol;
+  }
+  __syncthreads();
+  if (vb) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) zp[(long)(q + N1 * k2) * ncols] = b[k2];
+    for (int k2 = 0; k2 < N2; k2++) A[(q + N1 * k2) * TC + col] = b[k2];
+  }
+  __syncthreads();
+  // store KB-plane chunks: the tile's +G and -G columns are two runs
+  constexpr int NKB = nblk(N, KB);
+  cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
+  for (int i = tid; i < NKB * t.ncol * KB; i += 256) {
+    const int kk = i % KB, rest = i / KB, cl = rest % t.ncol, kb = rest / t.ncol;
+    const int z = kb * KB + kk;
+    if (z < N) {
+      const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
+      zb[((long)kb * ncols + gcol) * KB + kk] = A[z * TC + cl];
+    }
   }
 }
 
@@ -93,17 +117,31 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   const int pair = blockIdx.y;
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
   __syncthreads();
+  // load KB-plane chunks of the tile's columns
+  constexpr int NKB = nblk(N, KB);
+  const cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
+  for (int i = tid; i < NKB * t.ncol * KB; i += 256) {
+    const int kk = i % KB, rest = i / KB, cl = rest % t.ncol, kb = rest / t.ncol;
+    const int z = kb * KB + kk;
+    if (z < N) {
+      const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
+      A[z * TC + cl] = zb[((long)kb * ncols + gcol) * KB + kk];
+    }
+  }
+  __syncthreads();
   const int col = tid % TC, q = tid / TC;
-  if (col < t.ncol && q < N1) {           // forward phase A', q = k1
-    const int gcol = col < t.np ? t.col0 + col : t.colm + (col - t.np);
-    const cpx<T>* zp = zbuf + (long)pair * ncols * N + gcol;
-    cpx<T> b[N2];
+  cpx<T> b[N2];
+  const bool va = col < t.ncol && q < N1;
+  if (va) {                               // forward phase A', q = k1
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) b[k2] = zp[(long)(q + N1 * k2) * ncols];
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = A[(q + N1 * k2) * TC + col];
     rdft<T, N2, -1>(b);
 #pragma unroll
     for (int m1 = 1; m1 < N2; m1++)
       if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+  }
+  __syncthreads();
+  if (va) {
 #pragma unroll
     for (int m1 = 0; m1 < N2; m1++) A[(m1 * N1 + q) * TC + col] = b[m1];
   }
@@ -153,146 +191,151 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
 }
 
 // ------------------------------------------------------------------ x-bwd
-// colmap[r*N + x] = column (ivec) at (row r, x) or -1
+// Block (plane block kb, active row r, pair): the 16 x-lines of row r in the
+// planes of kb, line kk = plane KB*kb + kk.  Lane kk is fastest so that a
+// wave reads whole KB-plane chunks of adjacent columns.  colmap[r*N + x] =
+// workspace column at (row r, x) or -1.  The result is staged in LDS and
+// stored as XB-wide runs of xbuf.
 template <typename T, int N1, int N2>
-__global__ __launch_bounds__(256) void kx_bwd(const cpx<T>* __restrict__ zbuf,
-                                              cpx<T>* __restrict__ xbuf, int np2, int ncols,
-                                              int nact, int ngroups, int gpb,
-                                              const int32_t* __restrict__ colmap,
-                                              const cpx<T>* __restrict__ tw_g) {
-  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
-  __shared__ cpx<T> ex[L * NP];
+__global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
+    const cpx<T>* __restrict__ zbuf, cpx<T>* __restrict__ xbuf, int np2, int ncols, int nact,
+    const int32_t* __restrict__ colmap, const cpx<T>* __restrict__ tw_g) {
+  constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2), NXB = nblk(N, XB);
+  __shared__ cpx<T> ex[KB * NP];
   __shared__ cpx<T> tw[N];
   const int tid = threadIdx.x;
-  // block = (plane k, run of gpb row groups): the runs of one plane share its
-  // column data through the CU's cache
-  const int nruns = (ngroups + gpb - 1) / gpb;
-  const int k = blockIdx.x / nruns;
-  const int gr0 = (blockIdx.x - k * nruns) * gpb, gr1 = min(ngroups, gr0 + gpb);
+  const int nkb = nblk(np2, KB);
+  const int kb = blockIdx.x % nkb, r = blockIdx.x / nkb;
   const int pair = blockIdx.y;
-  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  const cpx<T>* zp = zbuf + ((long)pair * np2 + k) * ncols;
-  for (int gr = gr0; gr < gr1; gr++) {
-  const int r0 = gr * L;
-  const int nr = min(L, nact - r0);
-  __syncthreads();                       // tw loaded; ex free for this group
-  {
-    const int n2 = tid % N2, line = tid / N2;
-    if (line < nr) {
-      const int32_t* cm = colmap + (long)(r0 + line) * N;
-      cpx<T> a[N1];
+  const int nk = min(KB, np2 - kb * KB);           // valid planes of this block
+  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  __syncthreads();
+  const int kk = tid % KB, p = tid / KB;
+  const int32_t* cm = colmap + (long)r * N;
+  if (p < N2 && kk < nk) {                // phase A, p = n2
+    const cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
+    cpx<T> a[N1];
 #pragma unroll
-      for (int n1 = 0; n1 < N1; n1++) {
-        const int iv = cm[N2 * n1 + n2];
-        a[n1] = iv >= 0 ? zp[iv] : cpx<T>{T(0), T(0)};
-      }
-      rdft<T, N1, +1>(a);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; k1++)
-        if (n2) a[k1] = mul(a[k1], twl<T, +1>(tw, n2 * k1));
-#pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) ex[line * NP + k1 * N2 + n2] = a[k1];
+    for (int n1 = 0; n1 < N1; n1++) {
+      const int c = cm[N2 * n1 + p];
+      a[n1] = c >= 0 ? zb[(long)c * KB] : cpx<T>{T(0), T(0)};
     }
+    rdft<T, N1, +1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; k1++)
+      if (p) a[k1] = mul(a[k1], twl<T, +1>(tw, p * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) ex[kk * NP + k1 * N2 + p] = a[k1];
   }
   __syncthreads();
-  {
-    const int k1 = tid % N1, line = tid / N1;
-    if (line < nr) {
-      cpx<T> b[N2];
+  cpx<T> b[N2];
+  const bool vb = p < N1 && kk < nk;
+  if (vb) {                               // phase B, p = k1
 #pragma unroll
-      for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[line * NP + k1 * N2 + n2];
-      rdft<T, N2, +1>(b);
-      cpx<T>* xp = xbuf + (((long)pair * np2 + k) * nact + r0 + line) * N;
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) xp[k1 + N1 * k2] = b[k2];
-    }
+    for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[kk * NP + p * N2 + n2];
+    rdft<T, N2, +1>(b);
   }
+  __syncthreads();
+  if (vb) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) ex[kk * NP + p + N1 * k2] = b[k2];
+  }
+  __syncthreads();
+  // xbuf[pair][k][xb][r][xi]: runs of XB elements
+  const long kplane = (long)pair * np2 + kb * KB;
+  for (int i = tid; i < nk * NXB * XB; i += NT) {
+    const int xi = i % XB, xb = (i / XB) % NXB, l = i / (XB * NXB);
+    const int x = xb * XB + xi;
+    if (x < N) xbuf[(((kplane + l) * NXB + xb) * nact + r) * XB + xi] = ex[l * NP + x];
   }
 }
 
 // ------------------------------------------------------------------ x-fwd
 template <typename T, int N1, int N2>
-__global__ __launch_bounds__(256) void kx_fwd(const cpx<T>* __restrict__ xbuf,
-                                              cpx<T>* __restrict__ zbuf, int np2, int ncols,
-                                              int nact, int ngroups, int gpb,
-                                              const int32_t* __restrict__ colmap,
-                                              const cpx<T>* __restrict__ tw_g) {
-  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
-  __shared__ cpx<T> ex[L * NP];
+__global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
+    const cpx<T>* __restrict__ xbuf, cpx<T>* __restrict__ zbuf, int np2, int ncols, int nact,
+    const int32_t* __restrict__ colmap, const cpx<T>* __restrict__ tw_g) {
+  constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2), NXB = nblk(N, XB);
+  __shared__ cpx<T> ex[KB * NP];
   __shared__ cpx<T> tw[N];
   const int tid = threadIdx.x;
-  const int nruns = (ngroups + gpb - 1) / gpb;
-  const int k = blockIdx.x / nruns;
-  const int gr0 = (blockIdx.x - k * nruns) * gpb, gr1 = min(ngroups, gr0 + gpb);
+  const int nkb = nblk(np2, KB);
+  const int kb = blockIdx.x % nkb, r = blockIdx.x / nkb;
   const int pair = blockIdx.y;
-  for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
-  for (int gr = gr0; gr < gr1; gr++) {
-  const int r0 = gr * L;
-  const int nr = min(L, nact - r0);
-  __syncthreads();                       // tw loaded; ex free for this group
-  {
-    const int k1 = tid % N1, line = tid / N1;
-    if (line < nr) {
-      const cpx<T>* xp = xbuf + (((long)pair * np2 + k) * nact + r0 + line) * N;
-      cpx<T> b[N2];
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) b[k2] = xp[k1 + N1 * k2];
-      rdft<T, N2, -1>(b);
-#pragma unroll
-      for (int m1 = 1; m1 < N2; m1++)
-        if (k1) b[m1] = mul(b[m1], twl<T, -1>(tw, k1 * m1));
-#pragma unroll
-      for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + k1] = b[m1];
-    }
+  const int nk = min(KB, np2 - kb * KB);
+  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  const long kplane = (long)pair * np2 + kb * KB;
+  for (int i = tid; i < nk * NXB * XB; i += NT) {
+    const int xi = i % XB, xb = (i / XB) % NXB, l = i / (XB * NXB);
+    const int x = xb * XB + xi;
+    if (x < N) ex[l * NP + x] = xbuf[(((kplane + l) * NXB + xb) * nact + r) * XB + xi];
   }
   __syncthreads();
-  {
-    const int m1 = tid % N2, line = tid / N2;
-    if (line < nr) {
-      cpx<T> cv[N1];
+  const int kk = tid % KB, p = tid / KB;
+  cpx<T> b[N2];
+  const bool va = p < N1 && kk < nk;
+  if (va) {                               // forward phase A', p = k1
 #pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + m1 * N1 + k1];
-      rdft<T, N1, -1>(cv);
-      cpx<T>* zp = zbuf + ((long)pair * np2 + k) * ncols;
-      const int32_t* cm = colmap + (long)(r0 + line) * N;
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = ex[kk * NP + p + N1 * k2];
+    rdft<T, N2, -1>(b);
 #pragma unroll
-      for (int m2 = 0; m2 < N1; m2++) {
-        const int iv = cm[m1 + N2 * m2];
-        if (iv >= 0) zp[iv] = cv[m2];
-      }
-    }
+    for (int m1 = 1; m1 < N2; m1++)
+      if (p) b[m1] = mul(b[m1], twl<T, -1>(tw, p * m1));
   }
+  __syncthreads();
+  if (va) {
+#pragma unroll
+    for (int m1 = 0; m1 < N2; m1++) ex[kk * NP + m1 * N1 + p] = b[m1];
+  }
+  __syncthreads();
+  if (p < N2 && kk < nk) {                // phase B', p = m1
+    cpx<T> cv[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[kk * NP + p * N1 + k1];
+    rdft<T, N1, -1>(cv);
+    cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
+    const int32_t* cm = colmap + (long)r * N;
+#pragma unroll
+    for (int m2 = 0; m2 < N1; m2++) {
+      const int c = cm[p + N2 * m2];
+      if (c >= 0) zb[(long)c * KB] = cv[m2];
+    }
   }
 }
 
 // ------------------------------------------------------------------ y pass
-// block: L consecutive x of plane k; threads: line = tid % L (x fastest for
-// coalescing), q = tid / L.  jrow[y] = active row of y or -1.
+// Block (x block xb, plane k, chunk of PPB pairs): the XB y-lines at
+// x = XB*xb + xi, one per lane group; lane xi is fastest, so a wave reads and
+// writes whole XB-wide runs of xbuf (adjacent rows are adjacent runs) and of
+// the real-space grids.  jrow[y] = active row of y or -1.
 // MC: mode class, 0 = Y_VMUL, 1 = Y_DENSITY, 2 = the single full-grid
 // transforms (mode selects which); one instantiation per class keeps each
 // kernel's register allocation to what its mode needs.
 template <typename T, int N1, int N2, int MC>
-__global__ __launch_bounds__(256) void ky(cpx<T>* __restrict__ xbuf, int np0, int np2, int nact,
-                                          const int32_t* __restrict__ jrow, int nbatch, int PPB,
-                                          const cpx<T>* __restrict__ tw_g,
-                                          const double* __restrict__ vin,
-                                          const double* __restrict__ fac, double* __restrict__ rho,
-                                          double* __restrict__ fgrid, double scale, int mode) {
-  constexpr int N = N1 * N2, L = 256 / cmax(N1, N2), NP = N + 1;
-  __shared__ cpx<T> ex[L * NP];
+__global__ __launch_bounds__(xthreads(N1, N2)) void ky(
+    cpx<T>* __restrict__ xbuf, int np0, int np2, int nact, const int32_t* __restrict__ jrow,
+    int nbatch, int PPB, const cpx<T>* __restrict__ tw_g, const double* __restrict__ vin,
+    const double* __restrict__ fac, double* __restrict__ rho, double* __restrict__ fgrid,
+    double scale, int mode) {
+  constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2);
+  __shared__ cpx<T> ex[XB * NP];
   __shared__ cpx<T> tw[N];
   __shared__ int jr[N];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * L;
-  const int nx = min(L, np0 - x0);
-  const int k = blockIdx.y;
-  for (int i = tid; i < N; i += 256) { tw[i] = tw_g[i]; jr[i] = jrow[i]; }
+  const int xb = blockIdx.x, k = blockIdx.y;
+  const int nxb = nblk(np0, XB);
+  for (int i = tid; i < N; i += NT) { tw[i] = tw_g[i]; jr[i] = jrow[i]; }
   __syncthreads();
-  const int line = tid % L, q = tid / L;
-  const bool vA = line < nx && q < N2;    // phase A / B' role (N2 per line)
-  const bool vB = line < nx && q < N1;    // phase B / A' role (N1 per line)
-  const int x = x0 + line;
+  const int xi = tid % XB, q = tid / XB;
+  const int x = xb * XB + xi;
+  const bool vx = x < np0;
+  const bool vA = vx && q < N2;           // phase A / B' role (N2 per line)
+  const bool vB = vx && q < N1;           // phase B / A' role (N1 per line)
   const long plane = (long)k * np0 * N;
+  // run of row r of this block in pair p's xbuf
+  auto xrow = [&](int pair, int r) {
+    return xbuf + ((((long)pair * np2 + k) * nxb + xb) * nact + r) * XB + xi;
+  };
 
   if (MC == 0) mode = Y_VMUL;
   if (MC == 1) mode = Y_DENSITY;
@@ -310,19 +353,18 @@ __global__ __launch_bounds__(256) void ky(cpx<T>* __restrict__ xbuf, int np0, in
       for (int m1 = 1; m1 < N2; m1++)
         if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
 #pragma unroll
-      for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + q] = b[m1];
+      for (int m1 = 0; m1 < N2; m1++) ex[xi * NP + m1 * N1 + q] = b[m1];
     }
     __syncthreads();
     if (vA) {
       cpx<T> cv[N1];
 #pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + q * N1 + k1];
+      for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[xi * NP + q * N1 + k1];
       rdft<T, N1, -1>(cv);
-      cpx<T>* xp = xbuf + (long)k * nact * np0;
 #pragma unroll
       for (int m2 = 0; m2 < N1; m2++) {
         const int r = jr[q + N2 * m2];
-        if (r >= 0) xp[(long)r * np0 + x] = cv[m2];
+        if (r >= 0) *xrow(0, r) = cv[m2];
       }
     }
     return;
@@ -335,27 +377,26 @@ __global__ __launch_bounds__(256) void ky(cpx<T>* __restrict__ xbuf, int np0, in
   const int p0 = blockIdx.z * PPB;
   const int p1 = min(nbatch, p0 + PPB);
   for (int pair = p0; pair < p1; pair++) {
-    cpx<T>* xp = xbuf + ((long)pair * np2 + k) * (long)nact * np0;
     __syncthreads();                      // previous pair's readers of ex are done
     if (vA) {                             // phase A, q = n2 (bwd)
       cpx<T> a[N1];
 #pragma unroll
       for (int n1 = 0; n1 < N1; n1++) {
         const int r = jr[N2 * n1 + q];
-        a[n1] = r >= 0 ? xp[(long)r * np0 + x] : cpx<T>{T(0), T(0)};
+        a[n1] = r >= 0 ? *xrow(pair, r) : cpx<T>{T(0), T(0)};
       }
       rdft<T, N1, +1>(a);
 #pragma unroll
       for (int k1 = 1; k1 < N1; k1++)
         if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
 #pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) ex[line * NP + k1 * N2 + q] = a[k1];
+      for (int k1 = 0; k1 < N1; k1++) ex[xi * NP + k1 * N2 + q] = a[k1];
     }
     __syncthreads();
     cpx<T> b[N2];
     if (vB) {                             // phase B, q = k1: psi at y = q + N1*k2
 #pragma unroll
-      for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[line * NP + q * N2 + n2];
+      for (int n2 = 0; n2 < N2; n2++) b[n2] = ex[xi * NP + q * N2 + n2];
       rdft<T, N2, +1>(b);
     }
     if (MC == 0) {
@@ -374,18 +415,18 @@ __global__ __launch_bounds__(256) void ky(cpx<T>* __restrict__ xbuf, int np0, in
       __syncthreads();
       if (vB) {
 #pragma unroll
-        for (int m1 = 0; m1 < N2; m1++) ex[line * NP + m1 * N1 + q] = b[m1];
+        for (int m1 = 0; m1 < N2; m1++) ex[xi * NP + m1 * N1 + q] = b[m1];
       }
       __syncthreads();
       if (vA) {
         cpx<T> cv[N1];
 #pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[line * NP + q * N1 + k1];
+        for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[xi * NP + q * N1 + k1];
         rdft<T, N1, -1>(cv);
 #pragma unroll
         for (int m2 = 0; m2 < N1; m2++) {
           const int r = jr[q + N2 * m2];
-          if (r >= 0) xp[(long)r * np0 + x] = cv[m2];
+          if (r >= 0) *xrow(pair, r) = cv[m2];
         }
       }
     } else if (MC == 1) {
@@ -465,28 +506,26 @@ struct Launch {
   }
 
   static void xpass(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s) {
-    const int ngroups = (g->nact + L - 1) / L;
-    const int gpb = g->xgpb > 0 ? std::min(g->xgpb, ngroups) : ngroups;
-    const int nruns = (ngroups + gpb - 1) / gpb;
-    dim3 grid(g->np2 * nruns, nbatch);
+    constexpr int NT = xthreads(N1, N2);
+    dim3 grid(nblk(g->np2, KB) * g->nact, nbatch);
     if (prec == QBG_PREC_FP64) {
       if (!fwd)
-        hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2, g->ncols,
-                           g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
+        hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(NT), 0, s,
+                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2,
+                           g->ncols, g->nact, g->d_colmap, twp<double>(g, 0));
       else
-        hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2, g->ncols,
-                           g->nact, ngroups, gpb, g->d_colmap, twp<double>(g, 0));
+        hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(NT), 0, s,
+                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2,
+                           g->ncols, g->nact, g->d_colmap, twp<double>(g, 0));
     } else {
       if (!fwd)
-        hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2, g->ncols,
-                           g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
+        hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(NT), 0, s,
+                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2,
+                           g->ncols, g->nact, g->d_colmap, twp<float>(g, 0));
       else
-        hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(256), 0, s,
-                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2, g->ncols,
-                           g->nact, ngroups, gpb, g->d_colmap, twp<float>(g, 0));
+        hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(NT), 0, s,
+                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2,
+                           g->ncols, g->nact, g->d_colmap, twp<float>(g, 0));
     }
   }
 
@@ -494,7 +533,7 @@ struct Launch {
                     const double* fac, double* rho, double* fgrid, double scale, hipStream_t s) {
     const bool fwd = mode == Y_FWD_FULL || mode == Y_FWD_REAL;
     if (fwd) { nbatch = 1; ppb = 1; }
-    dim3 grid((g->np0 + L - 1) / L, g->np2, (nbatch + ppb - 1) / ppb);
+    dim3 grid(nblk(g->np0, XB), g->np2, (nbatch + ppb - 1) / ppb);
     const int mc = mode == Y_VMUL ? 0 : (mode == Y_DENSITY ? 1 : 2);
     if (prec == QBG_PREC_FP64) ylaunch<double>(mc, grid, g, nbatch, ppb, v, fac, rho, fgrid, scale, mode, s);
     else ylaunch<float>(mc, grid, g, nbatch, ppb, v, fac, rho, fgrid, scale, mode, s);
@@ -503,8 +542,9 @@ struct Launch {
   static void ylaunch(int mc, dim3 grid, qbg_grid* g, int nbatch, int ppb, const double* v,
                       const double* fac, double* rho, double* fgrid, double scale, int mode,
                       hipStream_t s) {
+    constexpr int NT = xthreads(N1, N2);
 #define QBG_KY(MC)                                                                            \
-  hipLaunchKernelGGL((ky<T, N1, N2, MC>), grid, dim3(256), 0, s, (cpx<T>*)g->d_xbuf, g->np0,    \
+  hipLaunchKernelGGL((ky<T, N1, N2, MC>), grid, dim3(NT), 0, s, (cpx<T>*)g->d_xbuf, g->np0,     \
                      g->np2, g->nact, g->d_jrow, nbatch, ppb, twp<T>(g, 1), v, fac, rho, fgrid, \
                      scale, mode)
     if (mc == 0) QBG_KY(0);

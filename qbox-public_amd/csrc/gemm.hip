@@ -63,6 +63,8 @@ struct GemmArgs {
   void* out;        // out[split*sstride + q + p*ldo]
   long ldo, sstride;
   int accumulate;
+  int tri;          // 1: only tiles with q >= p (symmetric product);
+                    // 2: P(r,p) = 0 for r > p (reduction stops at the tile's last p)
 };
 
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
@@ -82,7 +84,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
   else { iq = tt % g.tq; ip = tt / g.tq; }
   const int p0 = ip * TP, q0 = iq * TQ;
-  const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
+  if (g.tri == 1 && iq < ip) return;
+  const long rb = (long)split * g.rper;
+  long re = min(g.nr, rb + g.rper);
+  if (g.tri == 2) re = min(re, (long)p0 + TP);
   const TPm* __restrict__ Pp = (const TPm*)g.P;
   const TQm* __restrict__ Qp = (const TQm*)g.Q;
   TC pv[8], qv[8];
@@ -168,12 +173,15 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double* __restrict__
                                                       int nq, int np, double alpha, double gamma,
                                                       const TQm* __restrict__ Q, long ldq,
                                                       const TPm* __restrict__ P, long ldp,
-                                                      double* __restrict__ out, long ldo) {
+                                                      double* __restrict__ out, long ldo,
+                                                      int sym) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t >= (long)nq * np) return;
   const int q = (int)(t % nq), p = (int)(t / nq);
+  // symmetric products hold only q >= p: the upper part mirrors it
+  const long src = (sym && q < p) ? (long)q * nq + p : t;
   double s = 0.0;
-  for (int k = 0; k < S; k++) s += partial[(long)k * nq * np + t];
+  for (int k = 0; k < S; k++) s += partial[(long)k * nq * np + src];
   double v = alpha * s;
   if (gamma != 0.0) v += gamma * (double)Q[(long)q * ldq] * (double)P[(long)p * ldp];
   out[q + (long)p * ldo] = v;
@@ -187,17 +195,19 @@ void launch_gemm(const GemmArgs& g, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm<TPm, TQm, TC, QR, TO>), dim3(nblk), dim3(256), 0, s, g);
 }
 
-// split-K TN product; partials in FP64, reduced in a fixed order
+// split-K TN product; partials in FP64, reduced in a fixed order.  sym: the
+// product is symmetric (A = B, M = N): only the lower tiles are formed.
 template <typename TQm>
 int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, long K,
               double alpha, double gamma, double* out, long ldo, double* work,
-              size_t work_doubles, int prec, hipStream_t s) {
+              size_t work_doubles, int prec, int sym, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return -1;
   GemmArgs g{};
   g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
   g.np = N; g.nq = M; g.nr = K;
   g.tp = (N + TP - 1) / TP; g.tq = (M + TQ - 1) / TQ;
   g.pfast = 0;
+  g.tri = sym ? 1 : 0;
   const long ksteps = (K + KC - 1) / KC;
   const int ntile = g.tp * g.tq;
   // about four blocks per CU; partial traffic kept well below operand traffic
@@ -213,19 +223,21 @@ int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, l
   else launch_gemm<double, TQm, float, true, double>(g, s);
   const long tot = (long)M * N;
   hipLaunchKernelGGL((k_split_reduce<TQm, double>), dim3((unsigned)((tot + 255) / 256)),
-                     dim3(256), 0, s, work, (int)S, M, N, alpha, gamma, A, lda, B, ldb, out, ldo);
+                     dim3(256), 0, s, work, (int)S, M, N, alpha, gamma, A, lda, B, ldb, out, ldo,
+                     sym);
   return 0;
 }
 
 template <typename TQm, typename TO>
 void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M, TO* C, long ldc,
-        int accumulate, int prec, hipStream_t s) {
+        int accumulate, int prec, int tri, hipStream_t s) {
   if (K <= 0 || N <= 0 || M <= 0) return;
   GemmArgs g{};
   g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
   g.np = N; g.nq = (int)K; g.nr = M;
   g.tp = (N + TP - 1) / TP; g.tq = (int)((K + TQ - 1) / TQ);
   g.pfast = 1;
+  g.tri = tri ? 2 : 0;
   g.rper = ((M + KC - 1) / KC) * KC;
   g.out = C; g.ldo = ldc; g.sstride = 0; g.accumulate = accumulate;
   if (prec == QBG_PREC_FP64) launch_gemm<double, TQm, double, false, TO>(g, s);
@@ -237,24 +249,35 @@ void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M,
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
                    size_t work_doubles, int prec, hipStream_t s) {
-  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, s);
+  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, 0,
+                   s);
 }
 int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
                    size_t work_doubles, int prec, hipStream_t s) {
-  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, s);
+  return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, 0,
+                   s);
+}
+int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gamma, double* out,
+              long ldo, double* work, size_t work_doubles, hipStream_t s) {
+  return tn_splitk(A, lda, A, lda, N, N, K, alpha, gamma, out, ldo, work, work_doubles,
+                   QBG_PREC_FP64, 1, s);
+}
+void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
+                  long ldc, hipStream_t s) {
+  nn(A, lda, F, ldf, K, N, N, C, ldc, 0, QBG_PREC_FP64, 1, s);
 }
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
              double* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
 }
 void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N, int M,
              double* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
 }
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
              float* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, s);
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
 }
 
 // ------------------------------------------------------- Cholesky, inverse
@@ -296,32 +319,144 @@ __global__ __launch_bounds__(1024) void k_potrf(double* __restrict__ S, int n, i
   if (threadIdx.x == 0) info[0] = bad;
 }
 
-// X = L^{-1} written row-major (X[i*n + c]) by forward substitution on the
-// identity, one work-group.  Row-major X is L^{-T} in column-major order.
-__global__ __launch_bounds__(1024) void k_trtri(const double* __restrict__ L, int n, int ldl,
-                                                double* __restrict__ X) {
-  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x)
-    X[t] = ((t / n) == (t % n)) ? 1.0 : 0.0;
+// Blocked lower Cholesky in one work-group: panels of PB columns.  Per
+// panel the PB x PB diagonal block is factored in LDS, the rows below are
+// solved against it (one thread per row) and kept in LDS, and the trailing
+// lower triangle is updated from LDS.  info as for k_potrf.
+constexpr int PB = 8;
+constexpr int POTRF_NMAX = 960;    // rows of the LDS panel
+
+__global__ __launch_bounds__(1024) void k_potrf_blk(double* __restrict__ S, int n, int lds,
+                                                    int* __restrict__ info) {
+  __shared__ double D[PB][PB + 1];
+  __shared__ double P[POTRF_NMAX * PB];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
   __syncthreads();
-  for (int j = 0; j < n; j++) {
-    const double d = L[j + (long)j * ldl];
-    for (int c = threadIdx.x; c <= j; c += blockDim.x) X[(long)j * n + c] /= d;
+  for (int j0 = 0; j0 < n; j0 += PB) {
+    const int nb = min(PB, n - j0), j1 = j0 + nb, rows = n - j1;
+    if (tid < nb * nb) D[tid / nb][tid % nb] = S[(j0 + tid / nb) + (long)(j0 + tid % nb) * lds];
     __syncthreads();
-    // rows i > j: X[i][c] -= L[i][j] X[j][c], c <= j
-    const int rows = n - j - 1, cols = j + 1;
-    for (long t = threadIdx.x; t < (long)rows * cols; t += blockDim.x) {
-      const int i = j + 1 + (int)(t / cols), c = (int)(t % cols);
-      X[(long)i * n + c] -= L[i + (long)j * ldl] * X[(long)j * n + c];
+    if (tid == 0) {                        // unblocked factor of the diagonal block
+      for (int c = 0; c < nb && !bad; c++) {
+        double d = D[c][c];
+        for (int t = 0; t < c; t++) d -= D[c][t] * D[c][t];
+        if (!(d > 0.0)) { bad = j0 + c + 1; break; }
+        d = sqrt(d);
+        D[c][c] = d;
+        for (int r = c + 1; r < nb; r++) {
+          double v = D[r][c];
+          for (int t = 0; t < c; t++) v -= D[r][t] * D[c][t];
+          D[r][c] = v / d;
+        }
+      }
     }
     __syncthreads();
+    if (bad) break;
+    if (tid < nb * nb && tid / nb >= tid % nb)
+      S[(j0 + tid / nb) + (long)(j0 + tid % nb) * lds] = D[tid / nb][tid % nb];
+    // rows below: L_i = A_i D^-T (forward substitution)
+    for (int i = tid; i < rows; i += blockDim.x) {
+      double x[PB];
+#pragma unroll
+      for (int c = 0; c < PB; c++) {
+        if (c < nb) {
+          double v = S[(j1 + i) + (long)(j0 + c) * lds];
+          for (int t = 0; t < c; t++) v -= x[t] * D[c][t];
+          x[c] = v / D[c][c];
+          S[(j1 + i) + (long)(j0 + c) * lds] = x[c];
+          P[i * PB + c] = x[c];
+        }
+      }
+    }
+    __syncthreads();
+    // trailing lower triangle: A_ik -= L_i . L_k, k <= i
+    for (long t = tid; t < (long)rows * rows; t += blockDim.x) {
+      const int i = (int)(t / rows), k = (int)(t - (long)i * rows);
+      if (k <= i) {
+        double v = 0.0;
+        for (int c = 0; c < nb; c++) v += P[i * PB + c] * P[k * PB + c];
+        S[(j1 + i) + (long)(j1 + k) * lds] -= v;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) info[0] = bad;
+}
+
+// X = L^{-1} (lower), written row-major (X[i*n + c], upper part zero):
+// block column J of TB columns per work-group, X_IJ = L_II^{-1} (E_IJ -
+// sum_{J <= K < I} L_IK X_KJ) for I = J, J+1, ... with 32 x 32 tiles staged
+// in LDS; one wave solves the diagonal-block system column by column.
+constexpr int TB = 32;
+__global__ __launch_bounds__(256) void k_trtri_blk(const double* __restrict__ L, int n, int ldl,
+                                                   double* __restrict__ X) {
+  __shared__ double R[TB][TB + 1];
+  __shared__ double La[TB][TB + 1];
+  __shared__ double Xb[TB][TB + 1];
+  const int tid = threadIdx.x, J = blockIdx.x, nbk = (n + TB - 1) / TB;
+  const int c0 = J * TB;
+  // upper blocks of this block column are zero
+  for (long t = tid; t < (long)c0 * TB; t += 256) {
+    const int r = (int)(t / TB), c = c0 + (int)(t % TB);
+    if (c < n) X[(long)r * n + c] = 0.0;
+  }
+  for (int I = J; I < nbk; I++) {
+    const int r0 = I * TB;
+    for (int e = tid; e < TB * TB; e += 256) {
+      const int r = e / TB, c = e % TB;
+      R[r][c] = (I == J && r == c) ? 1.0 : 0.0;
+    }
+    for (int K = J; K < I; K++) {
+      const int k0 = K * TB;
+      __syncthreads();
+      for (int e = tid; e < TB * TB; e += 256) {
+        const int r = e / TB, c = e % TB;
+        La[r][c] = (r0 + r < n && k0 + c < n) ? L[(r0 + r) + (long)(k0 + c) * ldl] : 0.0;
+        Xb[r][c] = (k0 + r < n && c0 + c < n) ? X[(long)(k0 + r) * n + c0 + c] : 0.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < TB * TB; e += 256) {
+        const int r = e / TB, c = e % TB;
+        double v = 0.0;
+#pragma unroll 8
+        for (int t = 0; t < TB; t++) v += La[r][t] * Xb[t][c];
+        R[r][c] -= v;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < TB * TB; e += 256) {
+      const int r = e / TB, c = e % TB;
+      La[r][c] = (r0 + r < n && r0 + c < n) ? L[(r0 + r) + (long)(r0 + c) * ldl]
+                                            : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (tid < TB) {                       // column tid: forward substitution
+      const int c = tid;
+      for (int r = 0; r < TB; r++) {
+        double v = R[r][c];
+        for (int t = 0; t < r; t++) v -= La[r][t] * R[t][c];
+        R[r][c] = v / La[r][r];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < TB * TB; e += 256) {
+      const int r = e / TB, c = e % TB;
+      if (r0 + r < n && c0 + c < n) X[(long)(r0 + r) * n + c0 + c] = R[r][c];
+    }
+    __syncthreads();                      // X of this step is read in the next
   }
 }
 
 void potrf(double* S, int n, int lds, int* info, hipStream_t s) {
-  hipLaunchKernelGGL(k_potrf, dim3(1), dim3(1024), 0, s, S, n, lds, info);
+  if (n <= POTRF_NMAX)
+    hipLaunchKernelGGL(k_potrf_blk, dim3(1), dim3(1024), 0, s, S, n, lds, info);
+  else
+    hipLaunchKernelGGL(k_potrf, dim3(1), dim3(1024), 0, s, S, n, lds, info);
 }
 void trtri(const double* L, int n, int ldl, double* X, hipStream_t s) {
-  hipLaunchKernelGGL(k_trtri, dim3(1), dim3(1024), 0, s, L, n, ldl, X);
+  hipLaunchKernelGGL(k_trtri_blk, dim3((n + TB - 1) / TB), dim3(256), 0, s, L, n, ldl, X);
 }
 
 // zero Im c(G=0) of every band (SlaterDet::cleanup, src/SlaterDet.cpp:1335-1356)

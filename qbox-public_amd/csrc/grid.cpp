@@ -262,13 +262,14 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
   g->f4[0] = use_f4 ? qbg::f4_ops(np0) : nullptr;
   g->f4[1] = use_f4 ? qbg::f4_ops(np1) : nullptr;
   g->f4[2] = use_f4 ? qbg::f4_ops(np2) : nullptr;
+  // the four-step kernels share blocked workspace layouts: all axes or none
+  if (!g->f4[0] || !g->f4[1] || !g->f4[2]) g->f4[0] = g->f4[1] = g->f4[2] = nullptr;
   if (g->f4[2]) {
     const int m = std::max(g->f4[2]->n1, g->f4[2]->n2);
     g->zcols = std::min(256 / m, 4096 / np2 - 1);
   }
-  // launch-shape overrides for tuning sweeps: QBG_XGPB (x-pass row groups
-  // per block, 0 = whole plane), QBG_YPPB (y-pass pairs per block, v multiply)
-  if (const char* e = std::getenv("QBG_XGPB")) g->xgpb = std::max(0, std::atoi(e));
+  // launch-shape override for tuning sweeps: QBG_YPPB (y-pass pairs per
+  // block of the v multiply)
   if (const char* e = std::getenv("QBG_YPPB")) g->yppb = std::max(1, std::atoi(e));
   g->xrows = lines(np0);
   g->ytile = lines(np1);
@@ -428,8 +429,11 @@ int qbg_grid_destroy(qbg_grid* g) {
 // pass launchers refuse to run a batch larger than g->ws_pairs.
 int qbg::grid_ensure_ws(qbg_grid* g, int npairs) {
   if (npairs <= g->ws_pairs) return QBG_OK;
-  const size_t zb = (size_t)npairs * g->ncols * g->np2 * 16;
-  const size_t xb = (size_t)npairs * g->nact * g->np0 * g->np2 * 16;
+  // sized for FP64 and for the blocked layouts of the four-step kernels
+  // (planes and x padded to multiples of 16)
+  const size_t np2p = (size_t)(g->np2 + 15) / 16 * 16, np0p = (size_t)(g->np0 + 15) / 16 * 16;
+  const size_t zb = (size_t)npairs * g->ncols * np2p * 16;
+  const size_t xb = (size_t)npairs * g->nact * np0p * g->np2 * 16;
   qbg_ctx* c = g->ctx;
   (void)hipStreamSynchronize(c->stream);
   if (g->d_zbuf) (void)hipFree(g->d_zbuf);

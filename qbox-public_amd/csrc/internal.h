@@ -107,7 +107,6 @@ struct qbg_grid {
   double* d_fac = nullptr;          // per-band density factors
   int fac_cap = 0;
   int batch = 64;                   // max pairs per batch
-  int xgpb = 0;                     // x-pass row groups per block (0: whole plane)
   int yppb = 4;                     // y-pass pairs per block for the v multiply
 };
 

@@ -76,12 +76,12 @@ def test_evaluate_fp64(system, ctx):
 
 
 # mixed precision (FP32 band FFTs; projector GEMMs on FP32 MFMA or as
-# BF16-split products; FP64 reductions) against the FP64 oracle.  FP32 and
-# the 3-part split: every energy term within 1e-6 Ha/atom, forces within
-# 1e-5 Ha/bohr, H*psi within 1e-5 relative.  The 2-part split keeps about
-# 16 bits per product: terms within 1e-5 Ha/atom, forces 1e-4, H*psi 1e-3.
+# BF16-split products; FP64 reductions) against the FP64 oracle: every energy
+# term within 1e-6 Ha/atom, forces within 1e-5 Ha/bohr, H*psi within 1e-5
+# relative.  The 2-part split keeps about 16 bits per product; its rounding
+# errors average out over the projector sums, and it meets the same bar.
 MIXED_TOL = {"fp32": (1e-6, 1e-5, 1e-5), "bf16x3": (1e-6, 1e-5, 1e-5),
-             "bf16x2": (1e-5, 1e-4, 1e-3)}
+             "bf16x2": (1e-6, 1e-5, 1e-5)}
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x2"])
@@ -171,7 +171,8 @@ def test_energy_state_errors(ctx):
     assert e.value.code == -5          # stress needs update_vhxc(compute_stress=1)
 
 
-@pytest.mark.parametrize("idx", [10, 0])
+# 4, 5, 16 and 128 bands: one or many Cholesky panels, ragged inverse blocks
+@pytest.mark.parametrize("idx", [10, 11, 0, 1])
 def test_gram_fp64(ctx, oracle_mod, idx):
     import qbg
     s = cf.config(idx)
