@@ -143,6 +143,67 @@ def run(ef, dc, hp, prec, steps, warmup, ctx):
     return res, wall, gpu_ms, prof.read()
 
 
+def bench_c4(args):
+    """Twelve independent systems of uneven size (the uneven-batching
+    workload): a loop evaluating one system after the other (each waits for
+    its results) against one HIP graph holding all twelve evaluations."""
+    import qbg
+    from qbg import configs as cf
+    ctx = qbg.Context(0)
+    prec = "bf16x2" if args.prec == "mixed" else "fp64"
+    systems = cf.config4_systems()
+    efs, dcs = [], []
+    for s in systems:
+        ef = qbg.EnergyFunctional(ctx, s)
+        c = cf.random_coefficients(cf.BASE_SEED + 60, ef.basis(0)["g2"], s.nst)
+        dc = ctx.to_device(c)
+        qbg.gram(ctx, dc, ef.ngw, ef.ngw, s.nst)
+        efs.append(ef)
+        dcs.append(dc)
+        print(f"[bench c4] {s.name}: grid {ef.np0}x{ef.np1}x{ef.np2} ngw {ef.ngw}",
+              file=sys.stderr, flush=True)
+    nb = sum(s.nst for s in systems)
+
+    def loop():
+        out = []
+        for ef, dc in zip(efs, dcs):
+            out.append(ef.evaluate(dc, None, forces=True, prec=prec)["etotal"])
+        return out
+
+    for _ in range(max(1, args.warmup)):
+        ref = loop()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loop()
+    t_loop = (time.perf_counter() - t0) / args.steps
+    for ef, dc in zip(efs, dcs):          # allocate before capture
+        ef.evaluate_async(dc, None, forces=True, prec=prec)
+        ef.results()
+    with qbg.Graph(ctx) as g:
+        for ef, dc in zip(efs, dcs):
+            ef.evaluate_async(dc, None, forces=True, prec=prec)
+    g.launch()
+    got = [ef.results()["etotal"] for ef in efs]
+    assert got == ref, "graph replay differs from the loop"
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    ctx.sync()
+    t_graph = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({
+        "metric": "uneven batch: H*psi band-applies/s of twelve systems per evaluation round",
+        "value": round(nb / t_graph, 2), "unit": "band-applies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t_graph, 3),
+        "higher_is_better": True, "dtype": "f32+f64" if prec != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "twelve Gamma-point systems, atoms in {8..128}, ecut 40-80 Ry",
+                   "systems": [s.name for s in systems], "bands_total": nb,
+                   "precision": prec},
+        "loop_value": round(nb / t_loop, 2), "loop_ms_per_step": round(1e3 * t_loop, 3),
+        "graph_over_loop": round(t_loop / t_graph, 3)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +213,8 @@ def main():
     ap.add_argument("--prec", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c4", action="store_true",
+                    help="only the uneven batch: twelve systems, per-system loop vs one graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -167,6 +230,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": nb, "copy_probe_bytes_written": nb,
                           "launches": 3}))
         return
+    if args.c4:
+        return bench_c4(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (band sharding + RCCL reduction of rho), so "

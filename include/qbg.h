@@ -50,6 +50,7 @@ extern "C" {
 typedef struct qbg_ctx qbg_ctx;
 typedef struct qbg_grid qbg_grid;
 typedef struct qbg_ef qbg_ef;
+typedef struct qbg_graph qbg_graph;
 
 /* ---------------------------------------------------------------- context */
 /* Replaces the MPI/BLACS context of a 1x1 run (src/Context.cpp:60-75). */
@@ -191,6 +192,31 @@ int qbg_ef_update_vhxc(qbg_ef* ef, int compute_stress);
 int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
                   double* hpsi, int ldh, int compute_forces, int compute_stress,
                   int prec, double* energies, double* fion, double* sigma);
+/* One SCF-iteration evaluation without host synchronization, so that it can
+ * be captured in a graph: update_density -> update_vhxc -> energy
+ * (src/BOSampleStepper.cpp:1207-1217) with the flags below; the results are
+ * staged on the device and returned by qbg_ef_results, which waits for the
+ * stream.  Workspaces are allocated by the first (uncaptured) call. */
+#define QBG_EF_HPSI 1
+#define QBG_EF_FORCES 2
+#define QBG_EF_STRESS 4
+int qbg_ef_evaluate_async(qbg_ef* ef, const double* c, int ldc, double* hpsi, int ldh,
+                          int flags, int prec);
+/* energies (14), fion (3*nat) and sigma (7x6) of the last launched
+ * evaluation, as qbg_ef_energy returns them */
+int qbg_ef_results(qbg_ef* ef, double* energies, double* fion, double* sigma);
+
+/* ---------------------------------------------------------------- graphs */
+/* Capture every call made on ctx between qbg_graph_begin and qbg_graph_end
+ * into one HIP graph (stream capture); qbg_graph_launch replays it on the
+ * context stream.  Calls that synchronize, allocate or copy from the host
+ * fail with QBG_E_STATE while capturing.  No counterpart in the reference:
+ * this is the per-SCF-iteration launch graph of the evaluation path. */
+int qbg_graph_begin(qbg_ctx* ctx);
+int qbg_graph_end(qbg_ctx* ctx, qbg_graph** out);
+int qbg_graph_launch(qbg_graph* graph);
+int qbg_graph_destroy(qbg_graph* graph);
+
 /* host copy of a device array: 0 rhor, 1 v_r, 2 vxc_r, 3 rhog,
  * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr */
 int qbg_ef_get(qbg_ef* ef, int what, double* host_out);

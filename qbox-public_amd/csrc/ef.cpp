@@ -98,6 +98,15 @@ struct qbg_ef {
   float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
         *b_w32 = nullptr, *b_part = nullptr;
   size_t b_part_floats = 0;
+  // per-band density factors occ/omega (2 * pairs) and the result staging:
+  // d_scal, rhoelg[0], vion[0], fion gathered by a kernel into pinned host
+  // memory, so an evaluation needs no host synchronization until collected
+  double* d_dfac = nullptr;
+  double* h_res = nullptr;
+  const double** d_seg_ptr = nullptr;
+  int* d_seg_n = nullptr;
+  int nres = 0;
+  int last_forces = 0, last_stress = 0, launched = 0;
   // host results of atoms_moved
   double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
   std::vector<double> fesr;
@@ -114,9 +123,11 @@ static void ef_free(qbg_ef* e) {
                e->d_m_ipr, e->d_m_npr, e->d_m_stride, e->d_m_dbase, e->d_m_wt, e->d_dmat,
                e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work,
                e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
-               e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp};
+               e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
+               e->d_seg_ptr, e->d_seg_n};
   for (void* q : p)
     if (q) (void)hipFree(q);
+  if (e->h_res) (void)hipHostFree(e->h_res);
   if (e->gw) qbg_grid_destroy(e->gw);
   if (e->gv) qbg_grid_destroy(e->gv);
 }
@@ -416,6 +427,28 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     delete e;
     return rc;
   }
+  // density factors and result staging
+  {
+    const int npairs = (nst + 1) / 2;
+    std::vector<double> dfac(2 * (size_t)npairs, 0.0);
+    for (int n = 0; n < nst; n++) dfac[n] = e->occ[n] / e->omega;
+    e->nres = S_COUNT + 4 + 3 * e->nat;
+    std::vector<const double*> segp = {e->d_scal, e->d_rhoelg, e->d_vion, e->d_fion};
+    std::vector<int> segn = {S_COUNT, 2, 2, 3 * e->nat};
+    if ((rc = dput(c, &e->d_dfac, dfac)) || (rc = dput(c, &e->d_seg_ptr, segp)) ||
+        (rc = dput(c, &e->d_seg_n, segn))) {
+      ef_free(e);
+      delete e;
+      return rc;
+    }
+    if (hipHostMalloc((void**)&e->h_res, sizeof(double) * e->nres, hipHostMallocDefault) !=
+        hipSuccess) {
+      e->h_res = nullptr;
+      ef_free(e);
+      delete e;
+      return c->fail(QBG_E_NOMEM, "pinned result buffer");
+    }
+  }
   if ((rc = ef_atoms_moved(e))) {
     ef_free(e);
     delete e;
@@ -474,10 +507,9 @@ int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
-  QBG_HIP(ctx, hipMemsetAsync(e->d_rhor, 0, sizeof(double) * e->N, s));
+  fill_zero(e->d_rhor, e->N, s);
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;
-  int rc = qbg_compute_density(e->gw, c, ldc, e->nst, e->occ.data(), 1.0, e->omega, e->d_rhor,
-                               fprec);
+  int rc = density_batches(e->gw, c, ldc, e->nst, e->d_dfac, e->d_rhor, fprec);
   if (rc) return rc;
   qbg_grid* g = e->gv;
   if ((rc = grid_ensure_ws(g, 1))) return rc;
@@ -574,7 +606,7 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     void* Y = f32 ? (void*)(anl32 + (size_t)M * K) : (void*)(e->d_anl + (size_t)M * K);
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces)
-      QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
+      fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
     launch_force_w(e->d_fout, e->d_occ, M, nst, W, s);
     ctx->prof_end();
     ctx->prof_begin(QBG_K_GEMM);
@@ -610,6 +642,9 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
 static int bf16_ensure(qbg_ef* e) {
   if (e->b_anl) return QBG_OK;
   qbg_ctx* c = e->ctx;
+  if (c->capturing)
+    return c->fail(QBG_E_STATE, "BF16 workspace must exist before graph capture "
+                                "(run the call once uncaptured)");
   auto up = [](long x, long m) { return (x + m - 1) / m * m; };
   e->Kp = up(2L * e->wb.size, 128);
   e->Mp = (int)up(e->M, 128);
@@ -674,7 +709,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces) {
-      QBG_HIP(ctx, hipMemsetAsync(e->d_fion, 0, sizeof(double) * 3 * e->nat, s));
+      fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
       launch_nl_forces_y(e->b_anl, (const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_a_m0,
                          e->d_a_npr, e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
     }
@@ -691,10 +726,9 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
 // Gamma, LDA.  energies (host, 14): ekin eps enl ehart ehart_e ehart_ep
 // ehart_p exc dxc esr eself ecoul etotal total_charge; fion (host, 3*nat)
 // species-major; sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr).
-int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
-                  int compute_forces, int compute_stress, int prec, double* energies,
-                  double* fion, double* sigma) {
-  if (!e || !c || !energies || ldc < e->wb.size) return QBG_E_INVAL;
+static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi,
+                     int ldh, int compute_forces, int compute_stress, int prec) {
+  if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
   if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
@@ -740,16 +774,26 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
                         nchunk, e->d_fion, s);
     ctx->prof_end();
   }
-  // scalars back to the host
-  double sc[S_COUNT];
-  QBG_HIP(ctx, hipMemcpyAsync(sc, e->d_scal, sizeof(sc), hipMemcpyDeviceToHost, s));
-  double g0[4];
-  QBG_HIP(ctx, hipMemcpyAsync(g0, e->d_rhoelg, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  QBG_HIP(ctx, hipMemcpyAsync(g0 + 2, e->d_vion, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (compute_forces && fion)
-    QBG_HIP(ctx, hipMemcpyAsync(fion, e->d_fion, sizeof(double) * 3 * e->nat,
-                                hipMemcpyDeviceToHost, s));
-  QBG_HIP(ctx, hipStreamSynchronize(s));
+  // stage the scalars, G=0 values and forces in pinned host memory
+  gather_segments(e->d_seg_ptr, e->d_seg_n, 4, e->h_res, s);
+  e->last_forces = compute_forces;
+  e->last_stress = compute_stress;
+  e->launched = 1;
+  hipError_t er = hipGetLastError();
+  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+  return QBG_OK;
+}
+
+// wait for the last launched evaluation and assemble its results
+static int ef_collect(qbg_ef* e, double* energies, double* fion, double* sigma) {
+  qbg_ctx* ctx = e->ctx;
+  if (ctx->capturing) return ctx->fail(QBG_E_STATE, "results during graph capture");
+  if (!e->launched) return ctx->fail(QBG_E_STATE, "no evaluation launched");
+  QBG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int compute_forces = e->last_forces, compute_stress = e->last_stress;
+  const double* sc = e->h_res;
+  const double* g0 = e->h_res + S_COUNT;   // rhoelg[0] (2), vion[0] (2)
+  if (compute_forces && fion) std::memcpy(fion, e->h_res + S_COUNT + 4, sizeof(double) * 3 * e->nat);
   const double omega = e->omega, fpi = 4.0 * M_PI, Nd = (double)e->N;
   const double exc = sc[S_LDA + 0] * omega / Nd, dxc = sc[S_LDA + 1] * omega / Nd;
   const double charge = sc[S_LDA + 2] * omega / Nd;
@@ -784,6 +828,32 @@ int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double*
     }
   }
   return QBG_OK;
+}
+
+
+int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
+                  int compute_forces, int compute_stress, int prec, double* energies,
+                  double* fion, double* sigma) {
+  if (!e || !c || !energies) return QBG_E_INVAL;
+  int rc = ef_launch(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress, prec);
+  if (rc) return rc;
+  return ef_collect(e, energies, fion, sigma);
+}
+
+int qbg_ef_evaluate_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int ldh, int flags,
+                          int prec) {
+  if (!e || !c) return QBG_E_INVAL;
+  const int hp = (flags & QBG_EF_HPSI) != 0, fo = (flags & QBG_EF_FORCES) != 0,
+            st = (flags & QBG_EF_STRESS) != 0;
+  int rc = qbg_ef_update_density(e, c, ldc, prec);
+  if (!rc) rc = qbg_ef_update_vhxc(e, st);
+  if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec);
+  return rc;
+}
+
+int qbg_ef_results(qbg_ef* e, double* energies, double* fion, double* sigma) {
+  if (!e || !energies) return QBG_E_INVAL;
+  return ef_collect(e, energies, fion, sigma);
 }
 
 // device arrays for tests: 0 rhor (N), 1 v_r (N), 2 vxc_r (N), 3 rhog

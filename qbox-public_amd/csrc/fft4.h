@@ -22,6 +22,36 @@ __host__ __device__ constexpr int split_n1(int n) {
   return 0;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+// x with a x = 1 (mod m), m > 1, gcd(a, m) = 1
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; x++)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+
+// Index maps of an N = N1 N2 line split into phase A (N1-point DFTs, thread
+// n2) and phase B (N2-point DFTs, thread k1).  For coprime N1, N2 the
+// Good-Thomas (prime factor) maps make the split exact without twiddles:
+// input n = (N2 n1 + N1 n2) mod N, output k = CRT(k1, k2) = (k1 E1 + k2 E2)
+// mod N with E1 = 1 (mod N1), 0 (mod N2) and E2 = 0 (mod N1), 1 (mod N2).
+// Otherwise the four-step maps n = N2 n1 + n2, k = k1 + N1 k2 with the
+// twiddle W_N^{n2 k1} between the phases.  A forward transform reads its
+// input in the output order of the backward one (out_b) and writes in the
+// input order (in_a, with n1 the N1-DFT output and n2 the N2-DFT output).
+template <int N1, int N2>
+struct Map {
+  static constexpr int N = N1 * N2;
+  static constexpr bool pfa = N1 > 1 && N2 > 1 && gcd_c(N1, N2) == 1;
+  static constexpr int E1 = pfa ? (N2 * inv_mod(N2 % N1, N1)) % N : 0;
+  static constexpr int E2 = pfa ? (N1 * inv_mod(N1 % N2, N2)) % N : 0;
+  static __host__ __device__ constexpr int in_a(int n1, int n2) {
+    return pfa ? (N2 * n1 + N1 * n2) % N : N2 * n1 + n2;
+  }
+  static __host__ __device__ constexpr int out_b(int k1, int k2) {
+    return pfa ? (k1 * E1 + k2 * E2) % N : k1 + N1 * k2;
+  }
+};
 constexpr int smallest_prime(int n) {
   for (int p = 2; p <= n; p++)
     if (n % p == 0) return p;
@@ -29,23 +59,23 @@ constexpr int smallest_prime(int n) {
 }
 
 template <typename T>
-__device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
-__device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
 
 // exp(SIGN 2 pi i m / R) from the constant tables
 template <typename T, int R, int SIGN>
-__device__ __forceinline__ cpx<T> wr(int m) {
+__host__ __device__ __forceinline__ cpx<T> wr(int m) {
   return {(T)Tw<R>::c[m], (T)(SIGN * Tw<R>::s[m])};
 }
 
 // in-register DFT of size R: v[k] <- sum_j v[j] exp(SIGN 2 pi i j k / R)
 template <typename T, int R, int SIGN>
-__device__ __forceinline__ void rdft(cpx<T>* v) {
+__host__ __device__ __forceinline__ void rdft(cpx<T>* v) {
   if constexpr (R == 1) {
     return;
   } else if constexpr (R == 2) {
@@ -87,6 +117,32 @@ __device__ __forceinline__ void rdft(cpx<T>* v) {
     }
 #pragma unroll
     for (int k = 0; k < R; k++) v[k] = out[k];
+  } else if constexpr (Map<(R % 4 == 0) ? 4 : smallest_prime(R),
+                           R / ((R % 4 == 0) ? 4 : smallest_prime(R))>::pfa) {
+    // R = p q coprime, prime factor algorithm: DFT_q over m of
+    // v[(q r + p m) mod R] for each r, then DFT_p over r, output at CRT
+    constexpr int p = (R % 4 == 0) ? 4 : smallest_prime(R);
+    constexpr int q = R / p;
+    using M = Map<p, q>;
+    cpx<T> t[R];
+#pragma unroll
+    for (int r = 0; r < p; r++) {
+      cpx<T> sv[q];
+#pragma unroll
+      for (int m = 0; m < q; m++) sv[m] = v[M::in_a(r, m)];
+      rdft<T, q, SIGN>(sv);
+#pragma unroll
+      for (int k = 0; k < q; k++) t[r * q + k] = sv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < q; k++) {
+      cpx<T> sv[p];
+#pragma unroll
+      for (int r = 0; r < p; r++) sv[r] = t[r * q + k];
+      rdft<T, p, SIGN>(sv);
+#pragma unroll
+      for (int u = 0; u < p; u++) v[M::out_b(u, k)] = sv[u];
+    }
   } else {
     // R = p q, decimation in time: X[k + q u] = sum_r W_p^{ru} W_R^{rk} Y_r[k],
     // Y_r = DFT_q of x[r + p m]

@@ -10,7 +10,7 @@ namespace qbg {
 
 static const F4Ops ops_a[] = {
     QBG_F4(11, 2),   // 22
-    QBG_F4(6, 4),   // 24
+    QBG_F4(8, 3),   // 24
     QBG_F4(7, 4),   // 28
     QBG_F4(8, 8),   // 64
     QBG_F4(10, 9),   // 90

@@ -10,7 +10,7 @@ namespace qbg {
 
 static const F4Ops ops_b[] = {
     QBG_F4(12, 8),   // 96
-    QBG_F4(14, 8),   // 112
+    QBG_F4(16, 7),   // 112
     QBG_F4(14, 9),   // 126
     QBG_F4(16, 8),   // 128
 };

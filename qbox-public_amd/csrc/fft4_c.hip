@@ -9,9 +9,9 @@ namespace qbg {
    &f4::Launch<A, B>::xpass, &f4::Launch<A, B>::ypass}
 
 static const F4Ops ops_c[] = {
-    QBG_F4(12, 12),   // 144
+    QBG_F4(16, 9),   // 144
     QBG_F4(16, 10),   // 160
-    QBG_F4(14, 12),   // 168
+    QBG_F4(21, 8),   // 168
 };
 
 const F4Ops* f4_ops_c(int n) {

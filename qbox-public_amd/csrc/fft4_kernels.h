@@ -60,14 +60,21 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
   }
   __syncthreads();
   const int col = tid % TC, q = tid / TC;
-  if (col < t.ncol && q < N2) {           // phase A, q = n2, in place
-    cpx<T> a[N1];
+  using Mp = Map<N1, N2>;
+  cpx<T> a[N1];
+  const bool va = col < t.ncol && q < N2;
+  if (va) {                               // phase A, q = n2
 #pragma unroll
-    for (int n1 = 0; n1 < N1; n1++) a[n1] = A[(N2 * n1 + q) * TC + col];
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = A[Mp::in_a(n1, q) * TC + col];
     rdft<T, N1, +1>(a);
+    if constexpr (!Mp::pfa) {
 #pragma unroll
-    for (int k1 = 1; k1 < N1; k1++)
-      if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+      for (int k1 = 1; k1 < N1; k1++)
+        if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+    }
+  }
+  __syncthreads();                        // the prime-factor reads cross threads
+  if (va) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) A[(k1 * N2 + q) * TC + col] = a[k1];
   }
@@ -82,7 +89,7 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
   __syncthreads();
   if (vb) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) A[(q + N1 * k2) * TC + col] = b[k2];
+    for (int k2 = 0; k2 < N2; k2++) A[Mp::out_b(q, k2) * TC + col] = b[k2];
   }
   __syncthreads();
   // store KB-plane chunks: the tile's +G and -G columns are two runs
@@ -130,15 +137,18 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   }
   __syncthreads();
   const int col = tid % TC, q = tid / TC;
+  using Mp = Map<N1, N2>;
   cpx<T> b[N2];
   const bool va = col < t.ncol && q < N1;
   if (va) {                               // forward phase A', q = k1
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) b[k2] = A[(q + N1 * k2) * TC + col];
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = A[Mp::out_b(q, k2) * TC + col];
     rdft<T, N2, -1>(b);
+    if constexpr (!Mp::pfa) {
 #pragma unroll
-    for (int m1 = 1; m1 < N2; m1++)
-      if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+      for (int m1 = 1; m1 < N2; m1++)
+        if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+    }
   }
   __syncthreads();
   if (va) {
@@ -156,7 +166,7 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   __syncthreads();
   if (vb) {
 #pragma unroll
-    for (int m2 = 0; m2 < N1; m2++) A[(q + N2 * m2) * TC + col] = cv[m2];
+    for (int m2 = 0; m2 < N1; m2++) A[Mp::in_a(m2, q) * TC + col] = cv[m2];
   }
   __syncthreads();
   // scale by 1/np012 (fz, src/FourierTransform.cpp:739-743) and unpack
@@ -211,19 +221,22 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
   for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
   __syncthreads();
   const int kk = tid % KB, p = tid / KB;
+  using Mp = Map<N1, N2>;
   const int32_t* cm = colmap + (long)r * N;
   if (p < N2 && kk < nk) {                // phase A, p = n2
     const cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
     cpx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; n1++) {
-      const int c = cm[N2 * n1 + p];
+      const int c = cm[Mp::in_a(n1, p)];
       a[n1] = c >= 0 ? zb[(long)c * KB] : cpx<T>{T(0), T(0)};
     }
     rdft<T, N1, +1>(a);
+    if constexpr (!Mp::pfa) {
 #pragma unroll
-    for (int k1 = 1; k1 < N1; k1++)
-      if (p) a[k1] = mul(a[k1], twl<T, +1>(tw, p * k1));
+      for (int k1 = 1; k1 < N1; k1++)
+        if (p) a[k1] = mul(a[k1], twl<T, +1>(tw, p * k1));
+    }
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) ex[kk * NP + k1 * N2 + p] = a[k1];
   }
@@ -238,7 +251,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
   __syncthreads();
   if (vb) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) ex[kk * NP + p + N1 * k2] = b[k2];
+    for (int k2 = 0; k2 < N2; k2++) ex[kk * NP + Mp::out_b(p, k2)] = b[k2];
   }
   __syncthreads();
   // xbuf[pair][k][xb][r][xi]: runs of XB elements
@@ -272,15 +285,18 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
   }
   __syncthreads();
   const int kk = tid % KB, p = tid / KB;
+  using Mp = Map<N1, N2>;
   cpx<T> b[N2];
   const bool va = p < N1 && kk < nk;
   if (va) {                               // forward phase A', p = k1
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) b[k2] = ex[kk * NP + p + N1 * k2];
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = ex[kk * NP + Mp::out_b(p, k2)];
     rdft<T, N2, -1>(b);
+    if constexpr (!Mp::pfa) {
 #pragma unroll
-    for (int m1 = 1; m1 < N2; m1++)
-      if (p) b[m1] = mul(b[m1], twl<T, -1>(tw, p * m1));
+      for (int m1 = 1; m1 < N2; m1++)
+        if (p) b[m1] = mul(b[m1], twl<T, -1>(tw, p * m1));
+    }
   }
   __syncthreads();
   if (va) {
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
     const int32_t* cm = colmap + (long)r * N;
 #pragma unroll
     for (int m2 = 0; m2 < N1; m2++) {
-      const int c = cm[p + N2 * m2];
+      const int c = cm[Mp::in_a(m2, p)];
       if (c >= 0) zb[(long)c * KB] = cv[m2];
     }
   }
@@ -332,6 +348,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   const bool vA = vx && q < N2;           // phase A / B' role (N2 per line)
   const bool vB = vx && q < N1;           // phase B / A' role (N1 per line)
   const long plane = (long)k * np0 * N;
+  using Mp = Map<N1, N2>;
   // run of row r of this block in pair p's xbuf
   auto xrow = [&](int pair, int r) {
     return xbuf + ((((long)pair * np2 + k) * nxb + xb) * nact + r) * XB + xi;
@@ -344,14 +361,18 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
     if (vB) {
 #pragma unroll
       for (int k2 = 0; k2 < N2; k2++) {
-        const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+        const long gi = plane + (long)Mp::out_b(q, k2) * np0 + x;
         b[k2] = mode == Y_FWD_FULL ? cpx<T>{T(fgrid[2 * gi]), T(fgrid[2 * gi + 1])}
                                    : cpx<T>{T(scale * vin[gi]), T(0)};
       }
       rdft<T, N2, -1>(b);
+      if constexpr (!Mp::pfa) {
+        if constexpr (!Mp::pfa) {
 #pragma unroll
-      for (int m1 = 1; m1 < N2; m1++)
-        if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+          for (int m1 = 1; m1 < N2; m1++)
+            if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+        }
+      }
 #pragma unroll
       for (int m1 = 0; m1 < N2; m1++) ex[xi * NP + m1 * N1 + q] = b[m1];
     }
@@ -363,7 +384,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       rdft<T, N1, -1>(cv);
 #pragma unroll
       for (int m2 = 0; m2 < N1; m2++) {
-        const int r = jr[q + N2 * m2];
+        const int r = jr[Mp::in_a(m2, q)];
         if (r >= 0) *xrow(0, r) = cv[m2];
       }
     }
@@ -382,13 +403,15 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       cpx<T> a[N1];
 #pragma unroll
       for (int n1 = 0; n1 < N1; n1++) {
-        const int r = jr[N2 * n1 + q];
+        const int r = jr[Mp::in_a(n1, q)];
         a[n1] = r >= 0 ? *xrow(pair, r) : cpx<T>{T(0), T(0)};
       }
       rdft<T, N1, +1>(a);
+      if constexpr (!Mp::pfa) {
 #pragma unroll
-      for (int k1 = 1; k1 < N1; k1++)
-        if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+        for (int k1 = 1; k1 < N1; k1++)
+          if (q) a[k1] = mul(a[k1], twl<T, +1>(tw, q * k1));
+      }
 #pragma unroll
       for (int k1 = 0; k1 < N1; k1++) ex[xi * NP + k1 * N2 + q] = a[k1];
     }
@@ -404,13 +427,15 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       if (vB) {
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
-          const T v = (T)vin[plane + (long)(q + N1 * k2) * np0 + x];
+          const T v = (T)vin[plane + (long)Mp::out_b(q, k2) * np0 + x];
           b[k2] = {b[k2].x * v, b[k2].y * v};
         }
         rdft<T, N2, -1>(b);
+        if constexpr (!Mp::pfa) {
 #pragma unroll
-        for (int m1 = 1; m1 < N2; m1++)
-          if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+          for (int m1 = 1; m1 < N2; m1++)
+            if (q) b[m1] = mul(b[m1], twl<T, -1>(tw, q * m1));
+        }
       }
       __syncthreads();
       if (vB) {
@@ -425,7 +450,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
         rdft<T, N1, -1>(cv);
 #pragma unroll
         for (int m2 = 0; m2 < N1; m2++) {
-          const int r = jr[q + N2 * m2];
+          const int r = jr[Mp::in_a(m2, q)];
           if (r >= 0) *xrow(pair, r) = cv[m2];
         }
       }
@@ -442,7 +467,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       if (vB) {
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
-          const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+          const long gi = plane + (long)Mp::out_b(q, k2) * np0 + x;
           fgrid[2 * gi] = (double)b[k2].x;
           fgrid[2 * gi + 1] = (double)b[k2].y;
         }
@@ -451,7 +476,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       if (vB) {
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
-          const long gi = plane + (long)(q + N1 * k2) * np0 + x;
+          const long gi = plane + (long)Mp::out_b(q, k2) * np0 + x;
           fgrid[gi] = vin[gi] + (double)b[k2].x;
         }
       }
@@ -459,7 +484,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   }
   if (MC == 1 && vB) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) rho[plane + (long)(q + N1 * k2) * np0 + x] += acc[k2 % NACC];
+    for (int k2 = 0; k2 < N2; k2++) rho[plane + (long)Mp::out_b(q, k2) * np0 + x] += acc[k2 % NACC];
   }
 }
 

@@ -42,14 +42,14 @@ hipEvent_t qbg_ctx::prof_event() {
 }
 
 void qbg_ctx::prof_begin(int kind) {
-  if (!prof) return;
+  if (!prof || capturing) return;
   Rec r{kind, prof_event(), prof_event()};
   (void)hipEventRecord(r.a, stream);
   recs.push_back(r);
 }
 
 void qbg_ctx::prof_end() {
-  if (!prof || recs.empty()) return;
+  if (!prof || capturing || recs.empty()) return;
   (void)hipEventRecord(recs.back().b, stream);
   if (recs.size() > 4096) prof_collect();
 }
@@ -128,12 +128,59 @@ const char* qbg_last_error(const qbg_ctx* c) { return c ? c->err.c_str() : "null
 
 int qbg_sync(qbg_ctx* c) {
   if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "sync during graph capture");
   QBG_HIP(c, hipStreamSynchronize(c->stream));
+  return QBG_OK;
+}
+
+int qbg_graph_begin(qbg_ctx* c) {
+  if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "graph capture already open");
+  (void)hipSetDevice(c->device);
+  c->prof_collect();
+  QBG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  return QBG_OK;
+}
+
+int qbg_graph_end(qbg_ctx* c, qbg_graph** out) {
+  if (!c || !out) return QBG_E_INVAL;
+  *out = nullptr;
+  if (!c->capturing) return c->fail(QBG_E_STATE, "no graph capture open");
+  c->capturing = false;
+  hipGraph_t gr = nullptr;
+  QBG_HIP(c, hipStreamEndCapture(c->stream, &gr));
+  qbg_graph* g = new (std::nothrow) qbg_graph;
+  if (!g) { (void)hipGraphDestroy(gr); return QBG_E_NOMEM; }
+  g->ctx = c;
+  g->graph = gr;
+  if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    delete g;
+    return c->fail(QBG_E_HIP, "hipGraphInstantiate failed");
+  }
+  *out = g;
+  return QBG_OK;
+}
+
+int qbg_graph_launch(qbg_graph* g) {
+  if (!g) return QBG_E_INVAL;
+  QBG_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+  return QBG_OK;
+}
+
+int qbg_graph_destroy(qbg_graph* g) {
+  if (!g) return QBG_E_INVAL;
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
   return QBG_OK;
 }
 
 int qbg_malloc(qbg_ctx* c, size_t bytes, void** p) {
   if (!c || !p) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   (void)hipSetDevice(c->device);
   if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
     *p = nullptr;
@@ -144,6 +191,7 @@ int qbg_malloc(qbg_ctx* c, size_t bytes, void** p) {
 
 int qbg_free(qbg_ctx* c, void* p) {
   if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   QBG_HIP(c, hipStreamSynchronize(c->stream));
   QBG_HIP(c, hipFree(p));
   return QBG_OK;
@@ -151,6 +199,7 @@ int qbg_free(qbg_ctx* c, void* p) {
 
 int qbg_memcpy_h2d(qbg_ctx* c, void* d, const void* h, size_t n) {
   if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   QBG_HIP(c, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, c->stream));
   QBG_HIP(c, hipStreamSynchronize(c->stream));
   return QBG_OK;
@@ -158,6 +207,7 @@ int qbg_memcpy_h2d(qbg_ctx* c, void* d, const void* h, size_t n) {
 
 int qbg_memcpy_d2h(qbg_ctx* c, void* h, const void* d, size_t n) {
   if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   QBG_HIP(c, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, c->stream));
   QBG_HIP(c, hipStreamSynchronize(c->stream));
   return QBG_OK;
@@ -165,6 +215,7 @@ int qbg_memcpy_d2h(qbg_ctx* c, void* h, const void* d, size_t n) {
 
 int qbg_memset(qbg_ctx* c, void* d, int v, size_t n) {
   if (!c) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   QBG_HIP(c, hipMemsetAsync(d, v, n, c->stream));
   return QBG_OK;
 }
@@ -177,6 +228,7 @@ int qbg_timer_start(qbg_ctx* c) {
 
 int qbg_timer_stop(qbg_ctx* c, float* ms) {
   if (!c || !ms) return QBG_E_INVAL;
+  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
   QBG_HIP(c, hipEventRecord(c->ev1, c->stream));
   QBG_HIP(c, hipEventSynchronize(c->ev1));
   QBG_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
@@ -429,6 +481,9 @@ int qbg_grid_destroy(qbg_grid* g) {
 // pass launchers refuse to run a batch larger than g->ws_pairs.
 int qbg::grid_ensure_ws(qbg_grid* g, int npairs) {
   if (npairs <= g->ws_pairs) return QBG_OK;
+  if (g->ctx->capturing)
+    return g->ctx->fail(QBG_E_STATE, "FFT workspace must exist before graph capture "
+                                     "(run the call once uncaptured)");
   // sized for FP64 and for the blocked layouts of the four-step kernels
   // (planes and x padded to multiples of 16)
   const size_t np2p = (size_t)(g->np2 + 15) / 16 * 16, np0p = (size_t)(g->np0 + 15) / 16 * 16;
@@ -451,6 +506,7 @@ static int ensure_ws(qbg_grid* g, int npairs) { return qbg::grid_ensure_ws(g, np
 
 static int ensure_fac(qbg_grid* g, int n) {
   if (n <= g->fac_cap) return QBG_OK;
+  if (g->ctx->capturing) return g->ctx->fail(QBG_E_STATE, "allocation during graph capture");
   (void)hipStreamSynchronize(g->ctx->stream);
   if (g->d_fac) (void)hipFree(g->d_fac);
   g->d_fac = nullptr;
@@ -540,9 +596,9 @@ int qbg_compute_density(qbg_grid* g, const double* c, int ldc, int nst, const do
   TRY(checkprec(g, prec));
   if (nst <= 0 || ldc < g->ng || !occ || !(omega > 0.0))
     return g->ctx->fail(QBG_E_INVAL, "compute_density: bad arguments");
+  if (g->ctx->capturing)
+    return g->ctx->fail(QBG_E_STATE, "compute_density copies host factors: not capturable");
   const int npairs = (nst + 1) / 2;
-  const int B = std::min(npairs, g->batch);
-  TRY(ensure_ws(g, B));
   TRY(ensure_fac(g, 2 * npairs));
   // fac = weight * occ / omega (src/SlaterDet.cpp:232-244); a missing second
   // band of the last pair contributes 0
@@ -551,19 +607,11 @@ int qbg_compute_density(qbg_grid* g, const double* c, int ldc, int nst, const do
   hipStream_t s = g->ctx->stream;
   QBG_HIP(g->ctx, hipMemcpyAsync(g->d_fac, fac.data(), sizeof(double) * fac.size(),
                                  hipMemcpyHostToDevice, s));
-  for (int p0 = 0; p0 < npairs; p0 += B) {
-    const int nb = std::min(B, npairs - p0);
-    const int band0 = 2 * p0;
-    const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
-    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
-    PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
-    PROF(QBG_K_YDENS, ypass(g, prec, Y_DENSITY, nb, nullptr, g->d_fac + 2 * p0, rho, nullptr, 1.0, s));
-  }
+  TRY(qbg::density_batches(g, c, ldc, nst, g->d_fac, rho, prec));
   // the host factor array must outlive the async copy
   QBG_HIP(g->ctx, hipStreamSynchronize(s));
-  return last_launch(g);
+  return QBG_OK;
 }
-
 int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc, int nst,
                    double* hpsi, int ldh, int prec) {
   TRY(checkprec(g, prec));

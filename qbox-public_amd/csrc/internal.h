@@ -32,6 +32,7 @@ struct cpx {
 struct qbg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool capturing = false;           // between qbg_graph_begin and qbg_graph_end
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   // per-kernel event profiler
@@ -132,7 +133,21 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
           double* rho, double* fgrid, double scale, hipStream_t s);
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s);
+// band-pair density passes with per-band factors already on the device
+// (fac[2*npairs], zero for a missing last band); no host synchronization
+int density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
+                    double* rho, int prec);
+// util.hip
+void fill_zero(double* p, size_t n, hipStream_t s);
+void gather_segments(const double* const* seg_ptr, const int* seg_n, int nseg, double* dst,
+                     hipStream_t s);
 }  // namespace qbg
+
+struct qbg_graph {
+  qbg_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 #define QBG_HIP(ctx, call)                                                   \
   do {                                                                       \

@@ -1,6 +1,7 @@
 // util.hip -- a plain streaming copy, used to calibrate HBM byte counters
 // (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "internal.h"
 
 namespace {
@@ -23,3 +24,35 @@ extern "C" int qbg_stream_copy(qbg_ctx* c, void* dst, const void* src, size_t by
   if (e != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(e));
   return QBG_OK;
 }
+
+namespace qbg {
+
+// zero fill (used instead of memset nodes, which graphs must not record)
+__global__ __launch_bounds__(256) void k_fill0(double* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    p[i] = 0.0;
+}
+void fill_zero(double* p, size_t n, hipStream_t s) {
+  if (!n) return;
+  const size_t nb = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill0, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+}
+
+// gather scattered device scalars into one (host-mapped) buffer:
+// dst[i] = src segment by segment, segments (ptr, count) given in seg
+__global__ __launch_bounds__(256) void k_gather(const double* const* __restrict__ seg_ptr,
+                                                const int* __restrict__ seg_n, int nseg,
+                                                double* __restrict__ dst) {
+  int off = 0;
+  for (int k = 0; k < nseg; k++) {
+    const double* p = seg_ptr[k];
+    for (int i = threadIdx.x; i < seg_n[k]; i += 256) dst[off + i] = p[i];
+    off += seg_n[k];
+  }
+}
+void gather_segments(const double* const* seg_ptr, const int* seg_n, int nseg, double* dst,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(1), dim3(256), 0, s, seg_ptr, seg_n, nseg, dst);
+}
+
+}  // namespace qbg

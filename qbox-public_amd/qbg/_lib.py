@@ -68,6 +68,12 @@ SIGNATURES = {
                       _dp, _dp, _dp],
     "qbg_ef_get": [_vp, C.c_int, _dp],
     "qbg_ef_set_vr": [_vp, _dp],
+    "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
+    "qbg_ef_results": [_vp, _vp, _vp, _vp],
+    "qbg_graph_begin": [_vp],
+    "qbg_graph_end": [_vp, C.POINTER(_vp)],
+    "qbg_graph_launch": [_vp],
+    "qbg_graph_destroy": [_vp],
 }
 
 KERNEL_KINDS = ["zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd", "yother", "nl", "gram",
@@ -162,6 +168,44 @@ class Context:
         ms = C.c_float()
         self.check(self.lib.qbg_timer_stop(self.h, C.byref(ms)))
         return ms.value
+
+
+class Graph:
+    """qbg_graph: the calls made on a context inside `with Graph(ctx) as g:`
+    are captured into one HIP graph; g.launch() replays them."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        self.h = None
+
+    def __enter__(self):
+        self.ctx.check(self.ctx.lib.qbg_graph_begin(self.ctx.h))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        h = _vp()
+        rc = self.ctx.lib.qbg_graph_end(self.ctx.h, C.byref(h))
+        if et is None:
+            self.ctx.check(rc)
+            self.h = h
+            self.ctx.adopt(self)
+        elif rc == QBG_OK and h:
+            self.ctx.lib.qbg_graph_destroy(h)
+        return False
+
+    def launch(self):
+        self.ctx.check(self.ctx.lib.qbg_graph_launch(self.h))
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class DArray:
@@ -376,6 +420,25 @@ class EnergyFunctional:
         self.ctx.check(self.ctx.lib.qbg_ef_energy(
             self.h, c.p, self.ngw, int(hpsi is not None), hpsi.p if hpsi is not None else None,
             self.ngw, int(forces), int(stress), PREC[prec], _pd(e), _pd(f), _pd(sig)))
+        out = dict(zip(ENERGY_NAMES, e.tolist()))
+        out["fion"] = f.reshape(-1, 3)
+        out["sigma"] = sig.reshape(7, 6)
+        return out
+
+    def evaluate_async(self, c: DArray, hpsi: DArray = None, forces=True, stress=False,
+                       prec="fp64"):
+        """qbg_ef_evaluate_async: enqueue one evaluation (capturable)."""
+        flags = (1 if hpsi is not None else 0) | (2 if forces else 0) | (4 if stress else 0)
+        self.ctx.check(self.ctx.lib.qbg_ef_evaluate_async(
+            self.h, c.p, self.ngw, hpsi.p if hpsi is not None else None, self.ngw, flags,
+            PREC[prec]))
+
+    def results(self):
+        """qbg_ef_results: wait and return the last evaluation's results."""
+        e = np.zeros(14)
+        f = np.zeros(3 * self.nat)
+        sig = np.zeros(42)
+        self.ctx.check(self.ctx.lib.qbg_ef_results(self.h, _pd(e), _pd(f), _pd(sig)))
         out = dict(zip(ENERGY_NAMES, e.tolist()))
         out["fion"] = f.reshape(-1, 3)
         out["sigma"] = sig.reshape(7, 6)
