@@ -93,14 +93,17 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
   }
   __syncthreads();
   // store KB-plane chunks: the tile's +G and -G columns are two runs
+  // lane = (column slot, plane kk): 16 slots x KB planes per pass
   constexpr int NKB = nblk(N, KB);
   cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
-  for (int i = tid; i < NKB * t.ncol * KB; i += 256) {
-    const int kk = i % KB, rest = i / KB, cl = rest % t.ncol, kb = rest / t.ncol;
-    const int z = kb * KB + kk;
-    if (z < N) {
-      const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
-      zb[((long)kb * ncols + gcol) * KB + kk] = A[z * TC + cl];
+  const int skk = tid % KB, scl = tid / KB;
+  for (int cl = scl; cl < t.ncol; cl += 256 / KB) {
+    const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
+    cpx<T>* zc = zb + gcol * KB + skk;
+#pragma unroll 2
+    for (int kb = 0; kb < NKB; kb++) {
+      const int z = kb * KB + skk;
+      if (z < N) zc[(long)kb * ncols * KB] = A[z * TC + cl];
     }
   }
 }
@@ -127,12 +130,16 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   // load KB-plane chunks of the tile's columns
   constexpr int NKB = nblk(N, KB);
   const cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
-  for (int i = tid; i < NKB * t.ncol * KB; i += 256) {
-    const int kk = i % KB, rest = i / KB, cl = rest % t.ncol, kb = rest / t.ncol;
-    const int z = kb * KB + kk;
-    if (z < N) {
+  {
+    const int skk = tid % KB, scl = tid / KB;
+    for (int cl = scl; cl < t.ncol; cl += 256 / KB) {
       const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
-      A[z * TC + cl] = zb[((long)kb * ncols + gcol) * KB + kk];
+      const cpx<T>* zc = zb + gcol * KB + skk;
+#pragma unroll 2
+      for (int kb = 0; kb < NKB; kb++) {
+        const int z = kb * KB + skk;
+        if (z < N) A[z * TC + cl] = zc[(long)kb * ncols * KB];
+      }
     }
   }
   __syncthreads();
@@ -254,12 +261,14 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
     for (int k2 = 0; k2 < N2; k2++) ex[kk * NP + Mp::out_b(p, k2)] = b[k2];
   }
   __syncthreads();
-  // xbuf[pair][k][xb][r][xi]: runs of XB elements
-  const long kplane = (long)pair * np2 + kb * KB;
-  for (int i = tid; i < nk * NXB * XB; i += NT) {
-    const int xi = i % XB, xb = (i / XB) % NXB, l = i / (XB * NXB);
-    const int x = xb * XB + xi;
-    if (x < N) xbuf[(((kplane + l) * NXB + xb) * nact + r) * XB + xi] = ex[l * NP + x];
+  // xbuf[pair][k][xb][r][xi]: runs of XB elements; 32-bit offsets from the
+  // block's first plane
+  cpx<T>* xo = xbuf + (((long)pair * np2 + kb * KB) * NXB * nact + r) * XB;
+  const int sxi = tid % XB, sq = tid / XB;
+  for (int j = sq; j < nk * NXB; j += NT / XB) {
+    const int l = j / NXB, xb = j - l * NXB;
+    const int x = xb * XB + sxi;
+    if (x < N) xo[(l * NXB + xb) * nact * XB + sxi] = ex[l * NP + x];
   }
 }
 
@@ -277,11 +286,14 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
   const int pair = blockIdx.y;
   const int nk = min(KB, np2 - kb * KB);
   for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
-  const long kplane = (long)pair * np2 + kb * KB;
-  for (int i = tid; i < nk * NXB * XB; i += NT) {
-    const int xi = i % XB, xb = (i / XB) % NXB, l = i / (XB * NXB);
-    const int x = xb * XB + xi;
-    if (x < N) ex[l * NP + x] = xbuf[(((kplane + l) * NXB + xb) * nact + r) * XB + xi];
+  {
+    const cpx<T>* xo = xbuf + (((long)pair * np2 + kb * KB) * NXB * nact + r) * XB;
+    const int sxi = tid % XB, sq = tid / XB;
+    for (int j = sq; j < nk * NXB; j += NT / XB) {
+      const int l = j / NXB, xb = j - l * NXB;
+      const int x = xb * XB + sxi;
+      if (x < N) ex[l * NP + x] = xo[(l * NXB + xb) * nact * XB + sxi];
+    }
   }
   __syncthreads();
   const int kk = tid % KB, p = tid / KB;
@@ -350,9 +362,9 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   const long plane = (long)k * np0 * N;
   using Mp = Map<N1, N2>;
   // run of row r of this block in pair p's xbuf
-  auto xrow = [&](int pair, int r) {
-    return xbuf + ((((long)pair * np2 + k) * nxb + xb) * nact + r) * XB + xi;
-  };
+  const long pstride = (long)np2 * nxb * nact * XB;
+  cpx<T>* const xb0 = xbuf + (((long)k * nxb + xb) * nact) * XB + xi;
+  auto xrow = [&](int pair, int r) { return xb0 + pair * pstride + r * XB; };
 
   if (MC == 0) mode = Y_VMUL;
   if (MC == 1) mode = Y_DENSITY;

@@ -545,6 +545,26 @@ static int last_launch(qbg_grid* g) {
   return QBG_OK;
 }
 
+// rho += sum over band pairs of fac[n] |psi_n(r)|^2, the pairs in batches of
+// up to g->batch (src/SlaterDet.cpp:232-283); fac is a device array of
+// 2 * ceil(nst / 2) factors, so the loop copies nothing from the host
+int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
+                         double* rho, int prec) {
+  const int npairs = (nst + 1) / 2;
+  const int B = std::min(npairs, g->batch);
+  TRY(ensure_ws(g, B));
+  hipStream_t s = g->ctx->stream;
+  for (int p0 = 0; p0 < npairs; p0 += B) {
+    const int nb = std::min(B, npairs - p0);
+    const int band0 = 2 * p0;
+    const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
+    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
+    PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
+    PROF(QBG_K_YDENS, ypass(g, prec, Y_DENSITY, nb, nullptr, fac + 2 * p0, rho, nullptr, 1.0, s));
+  }
+  return last_launch(g);
+}
+
 extern "C" {
 
 int qbg_fft_backward(qbg_grid* g, const double* c, double* f, int prec) {
