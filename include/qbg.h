@@ -51,6 +51,8 @@ typedef struct qbg_ctx qbg_ctx;
 typedef struct qbg_grid qbg_grid;
 typedef struct qbg_ef qbg_ef;
 typedef struct qbg_graph qbg_graph;
+typedef struct qbg_wfstep qbg_wfstep;
+typedef struct qbg_mixer qbg_mixer;
 
 /* ---------------------------------------------------------------- context */
 /* Replaces the MPI/BLACS context of a 1x1 run (src/Context.cpp:60-75). */
@@ -87,7 +89,9 @@ int qbg_timer_stop(qbg_ctx* ctx, float* ms);
 #define QBG_K_VGRID 10    /* single transforms on the density basis */
 #define QBG_K_GEMM 11     /* one projector GEMM (anl^T c, anl fout or c W):
                              2 * nproj * 2ngw * nst flops each */
-#define QBG_K_COUNT 12
+#define QBG_K_WFSTEP 12   /* wavefunction stepper update incl. orthogonalization */
+#define QBG_K_MIX 13      /* charge mixing incl. rhor from the mixed rhog */
+#define QBG_K_COUNT 14
 /* A plain device-to-device streaming copy of `bytes` (a multiple of 16):
  * the reference kernel of known traffic for calibrating HBM byte
  * counters.  No counterpart in the reference. */
@@ -202,6 +206,10 @@ int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
 #define QBG_EF_STRESS 4
 int qbg_ef_evaluate_async(qbg_ef* ef, const double* c, int ldc, double* hpsi, int ldh,
                           int flags, int prec);
+/* the energy step alone (EnergyFunctional::energy after update_density and
+ * update_vhxc), without host synchronization; results as below */
+int qbg_ef_energy_async(qbg_ef* ef, const double* c, int ldc, double* hpsi, int ldh,
+                        int flags, int prec);
 /* energies (14), fion (3*nat) and sigma (7x6) of the last launched
  * evaluation, as qbg_ef_energy returns them */
 int qbg_ef_results(qbg_ef* ef, double* energies, double* fion, double* sigma);
@@ -216,6 +224,63 @@ int qbg_graph_begin(qbg_ctx* ctx);
 int qbg_graph_end(qbg_ctx* ctx, qbg_graph** out);
 int qbg_graph_launch(qbg_graph* graph);
 int qbg_graph_destroy(qbg_graph* graph);
+
+/* ------------------------------------------------ SCF iteration updates */
+/* The wavefunction steppers of BOSampleStepper's SCF loop
+ * (src/BOSampleStepper.cpp:237-256, 936-970), all on the device for the
+ * bands of an energy functional:
+ *   QBG_WF_SD   SDWavefunctionStepper (src/SDWavefunctionStepper.cpp:36-51):
+ *               c -= param * dc, then gram; param = dt2bye;
+ *   QBG_WF_PSD  PSDWavefunctionStepper (src/PSDWavefunctionStepper.cpp:41-119):
+ *               residual dc - c (c^T dc), preconditioned step, gram;
+ *   QBG_WF_PSDA PSDAWavefunctionStepper (src/PSDAWavefunctionStepper.cpp:45-247):
+ *               preconditioned residual with Anderson extrapolation, then
+ *               SlaterDet::ortho_align against the previous bands
+ *               (src/SlaterDet.cpp:768-876, polar factor src/Matrix.cpp:2446-2517).
+ * For PSD and PSDA, param is ecutprec in Hartree (0: the adaptive
+ * Preconditioner of src/Preconditioner.cpp:48-143; the reference's
+ * "ecutprec" command stores 0.5 * Ry value). */
+#define QBG_WF_SD 0
+#define QBG_WF_PSD 1
+#define QBG_WF_PSDA 2
+int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out);
+int qbg_wfstep_destroy(qbg_wfstep* st);
+/* WavefunctionStepper::preprocess (PSDA: the next update does not extrapolate) */
+int qbg_wfstep_preprocess(qbg_wfstep* st);
+/* WavefunctionStepper::update(dwf): c (ldc) holds the bands, dc (ldd) H*c on
+ * entry and the stepper's direction on exit.  Also forms
+ * eigenvalue_sum = SlaterDet::dot(c, dc) of the entry values
+ * (src/BOSampleStepper.cpp:945-951).  Enqueued without host synchronization
+ * (capturable); results come from qbg_wfstep_results. */
+int qbg_wfstep_update(qbg_wfstep* st, double* c, int ldc, double* dc, int ldd);
+/* waits for the stream; out[4]: eigenvalue_sum, PSDA theta, polar iterations
+ * of ortho_align, Cholesky pivot of the failing orthogonalization (0 = ok).
+ * Returns QBG_E_NOT_SPD when the orthogonalization failed (the bands are
+ * then left as the step produced them, not orthonormalized). */
+int qbg_wfstep_results(qbg_wfstep* st, double* out);
+
+/* Charge mixing of BOSampleStepper (src/BOSampleStepper.cpp:319-406, 810-879)
+ * on the density of an energy functional: Kerker weights g2/(g2 + q0^2),
+ * q0 = 2 pi / rc_kerker (none if rc_kerker <= 0), least-squares row weights
+ * sqrt(g2/(g2 + q1^2)), q1 = 2 pi / rc1, and the AndersonMixer
+ * (src/AndersonMixer.cpp) over ndim previous vectors (ndim = 0: simple
+ * mixing; ndim <= 8).  qbg_mixer_update follows qbg_ef_update_density: the
+ * first call after create/restart memorizes rhog, later calls mix and
+ * recompute rhor from the mixed rhog (ChargeDensity::update_rhor,
+ * src/ChargeDensity.cpp:201-221).  Capturable; the iteration count is kept
+ * on the device. */
+int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, double rc1,
+                     qbg_mixer** out);
+int qbg_mixer_destroy(qbg_mixer* mx);
+int qbg_mixer_restart(qbg_mixer* mx);
+int qbg_mixer_update(qbg_mixer* mx);
+
+/* DoubleMatrix::polar (src/Matrix.cpp:2446-2517): x (device, n x n, column
+ * major) is replaced by its orthogonal polar factor; iterations stop when
+ * ||I - x^T x||_F <= tol or after maxiter (<= 64).  Synchronizes; *iters
+ * (host, may be null) receives the iteration count.  QBG_E_NOT_SPD if a
+ * Higham step met a singular matrix. */
+int qbg_polar(qbg_ctx* ctx, double* x, int n, double tol, int maxiter, int* iters);
 
 /* host copy of a device array: 0 rhor, 1 v_r, 2 vxc_r, 3 rhog,
  * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr */

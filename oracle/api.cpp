@@ -220,4 +220,49 @@ void qbo_cell(const double* a9, double* b9, double* vol) {
 }
 void qbo_fold_in_ws(const double* a9, double* v) { Cell c; c.set(a9); c.fold_in_ws(v); }
 
+// ------------------------------------------------------ SCF iteration pieces
+double qbo_sd_dot(const double* c, const double* d, int ngw, int nst) {
+  return sd_dot((const cplx*)c, (const cplx*)d, ngw, nst);
+}
+void qbo_prec_ekin(const double* c, const double* g2, int ngw, int nst, double* ekin) {
+  prec_ekin((const cplx*)c, g2, ngw, nst, ekin);
+}
+double qbo_prec_diag(double ekin, double g2, double ecutprec) {
+  return prec_diag(ekin, g2, ecutprec);
+}
+void qbo_residual(const double* c, double* d, int ngw, int nst, double* a) {
+  residual((const cplx*)c, (cplx*)d, ngw, nst, a);
+}
+int qbo_polar(double* x, int n, double tol, int maxiter) { return polar(x, n, tol, maxiter); }
+int qbo_ortho_align(double* c, const double* cl, int ngw, int nst) {
+  return ortho_align((cplx*)c, (const cplx*)cl, ngw, nst);
+}
+int qbo_wf_update(int kind, double param, const double* g2, double* c, double* d, int ngw,
+                  int nst, double* c_last, double* d_last, int* extrapolate) {
+  return wf_update(kind, param, g2, (cplx*)c, (cplx*)d, ngw, nst, (cplx*)c_last, (cplx*)d_last,
+                   extrapolate);
+}
+void* qbo_anderson_create(long m, int nmax) {
+  Anderson* a = new Anderson;
+  a->m = m;
+  a->nmax = nmax;
+  a->xs.assign((size_t)(nmax + 1) * m, 0.0);
+  a->fs.assign((size_t)(nmax + 1) * m, 0.0);
+  a->restart();
+  return a;
+}
+void qbo_anderson_update(void* a, const double* x, const double* f, double* xbar, double* fbar) {
+  ((Anderson*)a)->update(x, f, xbar, fbar);
+}
+void qbo_anderson_destroy(void* a) { delete (Anderson*)a; }
+void* qbo_mixer_create(void* h, double alpha, double rc_kerker, int ndim, double rc1) {
+  Mixer* m = new Mixer;
+  m->init(*(System*)h, alpha, rc_kerker, ndim, rc1);
+  return m;
+}
+void qbo_mixer_restart(void* m) { ((Mixer*)m)->restart(); }
+void qbo_mixer_update(void* m, void* h) { ((Mixer*)m)->update(*(System*)h); }
+void qbo_mixer_destroy(void* m) { delete (Mixer*)m; }
+void qbo_update_rhor(void* h) { ((System*)h)->update_rhor(); }
+
 }  // extern "C"

@@ -48,6 +48,27 @@ def lib():
                      "qbo_update_vhxc", "qbo_get", "qbo_set_vr", "qbo_species_eval"]:
             getattr(_lib, name).argtypes = None
         _lib.qbo_gram.argtypes = [_dp, C.c_int, C.c_int]
+        _lib.qbo_sd_dot.restype = C.c_double
+        _lib.qbo_sd_dot.argtypes = [_dp, _dp, C.c_int, C.c_int]
+        _lib.qbo_prec_ekin.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
+        _lib.qbo_prec_diag.restype = C.c_double
+        _lib.qbo_prec_diag.argtypes = [C.c_double, C.c_double, C.c_double]
+        _lib.qbo_residual.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
+        _lib.qbo_polar.argtypes = [_dp, C.c_int, C.c_double, C.c_int]
+        _lib.qbo_ortho_align.argtypes = [_dp, _dp, C.c_int, C.c_int]
+        _lib.qbo_wf_update.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int,
+                                       _dp, _dp, _ip]
+        _lib.qbo_anderson_create.restype = C.c_void_p
+        _lib.qbo_anderson_create.argtypes = [C.c_long, C.c_int]
+        _lib.qbo_anderson_update.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        _lib.qbo_anderson_destroy.argtypes = [C.c_void_p]
+        _lib.qbo_mixer_create.restype = C.c_void_p
+        _lib.qbo_mixer_create.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                          C.c_double]
+        _lib.qbo_mixer_restart.argtypes = [C.c_void_p]
+        _lib.qbo_mixer_update.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.qbo_mixer_destroy.argtypes = [C.c_void_p]
+        _lib.qbo_update_rhor.argtypes = [C.c_void_p]
         _lib.qbo_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
                                     _dp, _dp, _dp]
     return _lib
@@ -212,6 +233,13 @@ class Oracle:
         self.update_vhxc(stress)
         return self.energy(c, hpsi, forces, stress)
 
+    def update_rhor(self):
+        """ChargeDensity::update_rhor: rhor from the current rhog."""
+        lib().qbo_update_rhor(self.h)
+
+    def mixer(self, alpha=0.5, rc_kerker=10.0, ndim=3, rc1=3.0):
+        return Mixer(self, alpha, rc_kerker, ndim, rc1)
+
     def species_eval(self, isp, g, nop=2):
         g = np.ascontiguousarray(g, dtype=float)
         n = g.size
@@ -228,6 +256,122 @@ def compute_density_occ(o, c, occ):
     rho = np.zeros(o.N)
     lib().qbo_compute_density_occ(o.h, _ptr(c), c.shape[0], _ptr(occ), _ptr(rho))
     return rho
+
+
+class Mixer:
+    """Charge mixing of BOSampleStepper (Kerker weights, AndersonMixer when
+    ndim > 0) on an Oracle's rhog/rhor; update() follows update_density()."""
+
+    def __init__(self, o, alpha, rc_kerker, ndim, rc1):
+        self.o = o
+        self.h = lib().qbo_mixer_create(o.h, alpha, rc_kerker, ndim, rc1)
+
+    def restart(self):
+        lib().qbo_mixer_restart(self.h)
+
+    def update(self):
+        lib().qbo_mixer_update(self.h, self.o.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qbo_mixer_destroy(self.h)
+            self.h = None
+
+
+def _c2(c):
+    return np.array(c, dtype=np.complex128, order="C")
+
+
+def sd_dot(c, d):
+    """SlaterDet::dot for (nst, ngw) coefficient arrays (real basis)."""
+    c, d = _c2(c), _c2(d)
+    return lib().qbo_sd_dot(_ptr(c), _ptr(d), c.shape[1], c.shape[0])
+
+
+def prec_ekin(c, g2):
+    c = _c2(c)
+    g2 = np.ascontiguousarray(g2, dtype=float)
+    e = np.zeros(c.shape[0])
+    lib().qbo_prec_ekin(_ptr(c), _ptr(g2), c.shape[1], c.shape[0], _ptr(e))
+    return e
+
+
+def prec_diag(ekin, g2, ecutprec=0.0):
+    return lib().qbo_prec_diag(ekin, g2, ecutprec)
+
+
+def residual(c, d):
+    """(d - c A, A) with A = 2 c^T d - c0 d0^T (PSD/PSDA residual)."""
+    c, d = _c2(c), _c2(d)
+    nst = c.shape[0]
+    a = np.zeros(nst * nst)
+    lib().qbo_residual(_ptr(c), _ptr(d), c.shape[1], nst, _ptr(a))
+    return d, a.reshape(nst, nst).T
+
+
+def polar(x, tol=1e-6, maxiter=20):
+    """DoubleMatrix::polar of a square matrix; returns (factor, iterations)."""
+    xc = np.array(np.asarray(x, dtype=float).T, order="C")   # column major
+    it = lib().qbo_polar(_ptr(xc), xc.shape[0], tol, maxiter)
+    if it < 0:
+        raise RuntimeError("polar: singular matrix")
+    return xc.T.copy(), it
+
+
+def ortho_align(c, cl):
+    c, cl = _c2(c), _c2(cl)
+    rc = lib().qbo_ortho_align(_ptr(c), _ptr(cl), c.shape[1], c.shape[0])
+    if rc != 0:
+        raise RuntimeError(f"ortho_align failed ({rc})")
+    return c
+
+
+class WfStepper:
+    """SD (kind 0, param dt2bye), PSD (1) and PSDA (2, param ecutprec)
+    wavefunction steppers; PSDA state is kept between updates."""
+    KINDS = {"sd": 0, "psd": 1, "psda": 2}
+
+    def __init__(self, kind, param, g2):
+        self.kind = self.KINDS[kind] if isinstance(kind, str) else int(kind)
+        self.param = float(param)
+        self.g2 = np.ascontiguousarray(g2, dtype=float)
+        self.c_last = None
+        self.d_last = None
+        self.extrapolate = C.c_int(0)
+
+    def preprocess(self):
+        self.extrapolate.value = 0
+
+    def update(self, c, d):
+        """Returns (c_new, d_new): d holds H c on entry."""
+        c, d = _c2(c), _c2(d)
+        if self.c_last is None:
+            self.c_last = np.zeros_like(c)
+            self.d_last = np.zeros_like(c)
+        rc = lib().qbo_wf_update(self.kind, self.param, _ptr(self.g2), _ptr(c), _ptr(d),
+                                 c.shape[1], c.shape[0], _ptr(self.c_last), _ptr(self.d_last),
+                                 C.byref(self.extrapolate))
+        if rc != 0:
+            raise RuntimeError(f"wavefunction update failed ({rc})")
+        return c, d
+
+
+class Anderson:
+    def __init__(self, m, nmax):
+        self.m = m
+        self.h = lib().qbo_anderson_create(m, nmax)
+
+    def update(self, x, f):
+        x = np.ascontiguousarray(x, dtype=float)
+        f = np.ascontiguousarray(f, dtype=float)
+        xb, fb = np.zeros(self.m), np.zeros(self.m)
+        lib().qbo_anderson_update(self.h, _ptr(x), _ptr(f), _ptr(xb), _ptr(fb))
+        return xb, fb
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qbo_anderson_destroy(self.h)
+            self.h = None
 
 
 def gram(c):

@@ -148,6 +148,7 @@ struct System {
   void atoms_moved();                   // :1047-1197
   void update_twnl();                   // src/NonLocalPotential.cpp:235-1255
   void update_density(const cplx* c);   // src/ChargeDensity.cpp:136-198
+  void update_rhor();                   // src/ChargeDensity.cpp:201-221
   void update_vhxc(bool compute_stress);// src/EnergyFunctional.cpp:231-394
   double energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                 bool compute_forces, bool compute_stress, Results& r);  // :397
@@ -163,5 +164,35 @@ void rs_mul_add(const FFT3& ft, const double* v, const cplx* c, int ngw,
                 int nst, cplx* hpsi);
 int gram(cplx* c, int ngw, int nst);   // returns 0 or the failing pivot+1
 void cleanup(cplx* c, int ngw, int nst);
+
+// SCF iteration pieces (stepper.cpp)
+double sd_dot(const cplx* c, const cplx* d, int ngw, int nst);
+void prec_ekin(const cplx* c, const double* g2, int ngw, int nst, double* ekin);
+double prec_diag(double ekin_n, double g2, double ecutprec);
+void residual(const cplx* c, cplx* d, int ngw, int nst, double* a_out);
+int polar(double* x, int n, double tol, int maxiter);
+int ortho_align(cplx* c, const cplx* cl, int ngw, int nst);
+int wf_update(int kind, double param, const double* g2, cplx* c, cplx* d, int ngw, int nst,
+              cplx* c_last, cplx* d_last, int* extrapolate);
+
+struct Anderson {
+  long m = 0;
+  int nmax = 0, n_ = -1, k_ = -1;
+  std::vector<double> xs, fs;        // (nmax+1) x m ring buffers
+  void restart();
+  void update(const double* x, const double* f, double* xbar, double* fbar);
+};
+
+struct Mixer {
+  double alpha = 0.5;
+  bool anderson = true;
+  int itscf = 0;
+  std::vector<double> wkerker, wls;
+  std::vector<cplx> rhog_in;
+  Anderson am;
+  void init(const System& s, double alpha, double rc_kerker, int ndim, double rc1);
+  void restart();
+  void update(System& s);            // after update_density
+};
 
 }  // namespace qbo

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include "ef.h"
 #include "energy.h"
 #include "internal.h"
 #include "setup.h"
@@ -44,75 +45,6 @@ int dput(qbg_ctx* c, T** p, const std::vector<T>& h) {
 
 }  // namespace
 
-struct qbg_ef {
-  qbg_ctx* ctx = nullptr;
-  host::Cell cell;
-  double ecut = 0, omega = 0;
-  std::vector<host::Species> sp;
-  std::vector<int> na, sp_first;
-  int nsp = 0, nat = 0, nst = 0;
-  std::vector<double> tau, occ;
-  host::Basis wb, vb;
-  host::Maps wm, vm;
-  int np[3] = {0, 0, 0};
-  long N = 0;
-  qbg_grid* gw = nullptr;
-  qbg_grid* gv = nullptr;
-  int kmin[3] = {0, 0, 0}, krange = 0;
-  // device: bases and tables
-  double *d_g2w = nullptr, *d_gxw = nullptr;
-  int32_t* d_idxv = nullptr;
-  double *d_g2iv = nullptr, *d_giv = nullptr, *d_gxv = nullptr;
-  double *d_vps = nullptr, *d_dvps = nullptr, *d_rhops = nullptr, *d_twnl = nullptr;
-  double *d_tau = nullptr, *d_b = nullptr, *d_sftab = nullptr, *d_sfac = nullptr;
-  double *d_vion = nullptr, *d_dvion = nullptr, *d_rhopst = nullptr;
-  int32_t *d_sp_first = nullptr, *d_atom_sp = nullptr;
-  // local forces: atom groups (first, count, species) and vtemp per species
-  int32_t* d_lfgrp = nullptr;
-  int nlfgrp = 0;
-  double* d_vtemp = nullptr;
-  // device: state
-  double *d_rhor = nullptr, *d_vxc = nullptr, *d_vr = nullptr, *d_rhog = nullptr;
-  double *d_rhoelg = nullptr, *d_rhogt = nullptr, *d_vlocal = nullptr, *d_occ = nullptr;
-  double *d_fesr = nullptr, *d_fion = nullptr, *d_scal = nullptr, *d_partial = nullptr;
-  size_t partial_doubles = 0;
-  // nonlocal
-  int M = 0;
-  int32_t *d_m_atom = nullptr, *d_m_twnl = nullptr, *d_m_l = nullptr, *d_m_ipr = nullptr,
-          *d_m_npr = nullptr, *d_m_stride = nullptr, *d_m_dbase = nullptr;
-  double *d_m_wt = nullptr, *d_dmat = nullptr;
-  int32_t *d_a_m0 = nullptr, *d_a_npr = nullptr, *d_a_stride = nullptr;
-  // stress: dtwnl[s] at dt_off[s], layout [ipr][ij][ig]; npr per species;
-  // 2 (rcps/2)^2 per species
-  double* d_dtwnl = nullptr;
-  int64_t* d_dt_off = nullptr;
-  int32_t* d_sp_npr = nullptr;
-  double* d_fac2 = nullptr;
-  double *d_anl = nullptr, *d_fnl = nullptr, *d_fout = nullptr, *d_work = nullptr;
-  size_t work_doubles = 0;
-  // BF16-split GEMM workspace (allocated on first use): FP32 arrays padded
-  // to Kp rows (2 ngw up to a multiple of 128), Mp projector and Np band
-  // columns (multiples of 128), zero padding
-  int Mp = 0, Np = 0;
-  long Kp = 0;
-  float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
-        *b_w32 = nullptr, *b_part = nullptr;
-  size_t b_part_floats = 0;
-  // per-band density factors occ/omega (2 * pairs) and the result staging:
-  // d_scal, rhoelg[0], vion[0], fion gathered by a kernel into pinned host
-  // memory, so an evaluation needs no host synchronization until collected
-  double* d_dfac = nullptr;
-  double* h_res = nullptr;
-  const double** d_seg_ptr = nullptr;
-  int* d_seg_n = nullptr;
-  int nres = 0;
-  int last_forces = 0, last_stress = 0, launched = 0;
-  // host results of atoms_moved
-  double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
-  std::vector<double> fesr;
-  bool have_rho = false, have_vhxc = false, vhxc_stress = false;
-  std::vector<void*> allocs;
-};
 
 static void ef_free(qbg_ef* e) {
   void* p[] = {e->d_g2w, e->d_gxw, e->d_idxv, e->d_g2iv, e->d_giv, e->d_gxv, e->d_vps, e->d_dvps,
@@ -849,6 +781,15 @@ int qbg_ef_evaluate_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int
   if (!rc) rc = qbg_ef_update_vhxc(e, st);
   if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec);
   return rc;
+}
+
+// EnergyFunctional::energy alone, without host synchronization (after
+// update_density / update_vhxc of the same iteration); results as above
+int qbg_ef_energy_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int ldh, int flags,
+                        int prec) {
+  if (!e || !c) return QBG_E_INVAL;
+  return ef_launch(e, c, ldc, (flags & QBG_EF_HPSI) != 0, hpsi, ldh, (flags & QBG_EF_FORCES) != 0,
+                   (flags & QBG_EF_STRESS) != 0, prec);
 }
 
 int qbg_ef_results(qbg_ef* e, double* energies, double* fion, double* sigma) {

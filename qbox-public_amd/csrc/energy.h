@@ -68,23 +68,26 @@ void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double
 // type; operands of either type are converted as they are staged.
 // TN split-K: out[m + n*ldo] = alpha sum_k A[k + m*lda] B[k + n*ldb]
 //                              + gamma A[m*lda] B[n*ldb]   (partials in FP64)
+// skip: a device flag; when it reads nonzero the launches do nothing
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
-                   size_t work_doubles, int prec, hipStream_t s);
+                   size_t work_doubles, int prec, hipStream_t s, const int* skip = nullptr);
 int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
                    size_t work_doubles, int prec, hipStream_t s);
 // symmetric TN (FP64): out = alpha A^T A + gamma a0 a0^T, only the lower
 // tiles formed, the upper part mirrored
 int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gamma, double* out,
-              long ldo, double* work, size_t work_doubles, hipStream_t s);
+              long ldo, double* work, size_t work_doubles, hipStream_t s,
+              const int* skip = nullptr);
 // NN (FP64) with F upper triangular (F[m + n*ldf] = 0 for m > n), N x N:
 // C[k + n*ldc] = sum_{m <= n} A[k + m*lda] F[m + n*ldf]
 void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
                   long ldc, hipStream_t s);
 // NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf]  (K rows, N cols, M sum)
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
-             double* C, long ldc, int accumulate, int prec, hipStream_t s);
+             double* C, long ldc, int accumulate, int prec, hipStream_t s,
+             const int* skip = nullptr);
 void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N, int M,
              double* C, long ldc, int accumulate, int prec, hipStream_t s);
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,

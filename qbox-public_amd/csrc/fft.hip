@@ -381,11 +381,12 @@ __global__ __launch_bounds__(256) void k_ypass(
       }
       __syncthreads();
     } else if (MODE == Y_BWD_REAL_ADD) {
-      // v_r = vxc_r + Re(vft->backward(vlocal_g)) (src/EnergyFunctional.cpp:380)
+      // v_r = vxc_r + Re(vft->backward(vlocal_g)) (src/EnergyFunctional.cpp:380);
+      // without vin: rhor = Re(vft->backward(rhog)) * scale (ChargeDensity::update_rhor)
       for (int i = threadIdx.x; i < nel; i += blockDim.x) {
         const int j = i / nx, xl = i - j * nx;
         const long gi = plane + (long)j * np0 + x0 + xl;
-        fgrid[gi] = vin[gi] + (double)res[j * XT + xl].x;
+        fgrid[gi] = (vin ? vin[gi] : 0.0) + (double)res[j * XT + xl].x * scale;
       }
       __syncthreads();
     }
@@ -540,7 +541,7 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
   if (!ws_ok(g, nbatch)) return QBG_E_STATE;
   const bool bad = (mode == Y_VMUL && !v) || (mode == Y_DENSITY && (!fac || !rho)) ||
                    ((mode == Y_BWD_FULL || mode == Y_FWD_FULL) && !fgrid) ||
-                   (mode == Y_BWD_REAL_ADD && (!v || !fgrid)) || (mode == Y_FWD_REAL && !v);
+                   (mode == Y_BWD_REAL_ADD && !fgrid) || (mode == Y_FWD_REAL && !v);
   if (bad) return QBG_E_INVAL;
   if (g->f4[1]) {
     const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? std::min(g->yppb, nbatch) : 1);

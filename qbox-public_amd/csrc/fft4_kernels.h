@@ -489,7 +489,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) {
           const long gi = plane + (long)Mp::out_b(q, k2) * np0 + x;
-          fgrid[gi] = vin[gi] + (double)b[k2].x;
+          fgrid[gi] = (vin ? vin[gi] : 0.0) + (double)b[k2].x * scale;
         }
       }
     }

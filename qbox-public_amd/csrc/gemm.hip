@@ -65,6 +65,8 @@ struct GemmArgs {
   int accumulate;
   int tri;          // 1: only tiles with q >= p (symmetric product);
                     // 2: P(r,p) = 0 for r > p (reduction stops at the tile's last p)
+  const int* skip;  // if set and *skip != 0 the launch does nothing (a step of
+                    // a captured iteration that has already converged)
 };
 
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
@@ -72,6 +74,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   typedef Mfma<TC> MF;
   __shared__ TC Ps[KC][TP + PAD];
   __shared__ TC Qs[KC][TQ + PAD];
+  if (g.skip && *g.skip) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // blocks are dealt to the 8 XCDs round robin: renumber so that runs of
   // consecutive tiles (which share operand panels) stay on one XCD's L2
@@ -174,7 +177,8 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double* __restrict__
                                                       const TQm* __restrict__ Q, long ldq,
                                                       const TPm* __restrict__ P, long ldp,
                                                       double* __restrict__ out, long ldo,
-                                                      int sym) {
+                                                      int sym, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t >= (long)nq * np) return;
   const int q = (int)(t % nq), p = (int)(t / nq);
@@ -200,9 +204,10 @@ void launch_gemm(const GemmArgs& g, hipStream_t s) {
 template <typename TQm>
 int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, long K,
               double alpha, double gamma, double* out, long ldo, double* work,
-              size_t work_doubles, int prec, int sym, hipStream_t s) {
+              size_t work_doubles, int prec, int sym, hipStream_t s, const int* skip) {
   if (M <= 0 || N <= 0 || K <= 0) return -1;
   GemmArgs g{};
+  g.skip = skip;
   g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
   g.np = N; g.nq = M; g.nr = K;
   g.tp = (N + TP - 1) / TP; g.tq = (M + TQ - 1) / TQ;
@@ -224,15 +229,16 @@ int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, l
   const long tot = (long)M * N;
   hipLaunchKernelGGL((k_split_reduce<TQm, double>), dim3((unsigned)((tot + 255) / 256)),
                      dim3(256), 0, s, work, (int)S, M, N, alpha, gamma, A, lda, B, ldb, out, ldo,
-                     sym);
+                     sym, skip);
   return 0;
 }
 
 template <typename TQm, typename TO>
 void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M, TO* C, long ldc,
-        int accumulate, int prec, int tri, hipStream_t s) {
+        int accumulate, int prec, int tri, hipStream_t s, const int* skip) {
   if (K <= 0 || N <= 0 || M <= 0) return;
   GemmArgs g{};
+  g.skip = skip;
   g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
   g.np = N; g.nq = (int)K; g.nr = M;
   g.tp = (N + TP - 1) / TP; g.tq = (int)((K + TQ - 1) / TQ);
@@ -248,36 +254,36 @@ void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M,
 
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
-                   size_t work_doubles, int prec, hipStream_t s) {
+                   size_t work_doubles, int prec, hipStream_t s, const int* skip) {
   return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, 0,
-                   s);
+                   s, skip);
 }
 int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
                    size_t work_doubles, int prec, hipStream_t s) {
   return tn_splitk(A, lda, B, ldb, M, N, K, alpha, gamma, out, ldo, work, work_doubles, prec, 0,
-                   s);
+                   s, nullptr);
 }
 int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gamma, double* out,
-              long ldo, double* work, size_t work_doubles, hipStream_t s) {
+              long ldo, double* work, size_t work_doubles, hipStream_t s, const int* skip) {
   return tn_splitk(A, lda, A, lda, N, N, K, alpha, gamma, out, ldo, work, work_doubles,
-                   QBG_PREC_FP64, 1, s);
+                   QBG_PREC_FP64, 1, s, skip);
 }
 void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
                   long ldc, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, N, C, ldc, 0, QBG_PREC_FP64, 1, s);
+  nn(A, lda, F, ldf, K, N, N, C, ldc, 0, QBG_PREC_FP64, 1, s, nullptr);
 }
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
-             double* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
+             double* C, long ldc, int accumulate, int prec, hipStream_t s, const int* skip) {
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s, skip);
 }
 void gemm_nn(const float* A, long lda, const double* F, long ldf, long K, int N, int M,
              double* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s, nullptr);
 }
 void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
              float* C, long ldc, int accumulate, int prec, hipStream_t s) {
-  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s);
+  nn(A, lda, F, ldf, K, N, M, C, ldc, accumulate, prec, 0, s, nullptr);
 }
 
 // ------------------------------------------------------- Cholesky, inverse

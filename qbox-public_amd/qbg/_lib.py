@@ -69,7 +69,18 @@ SIGNATURES = {
     "qbg_ef_get": [_vp, C.c_int, _dp],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
+    "qbg_ef_energy_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_ef_results": [_vp, _vp, _vp, _vp],
+    "qbg_wfstep_create": [_vp, C.c_int, C.c_double, C.POINTER(_vp)],
+    "qbg_wfstep_destroy": [_vp],
+    "qbg_wfstep_preprocess": [_vp],
+    "qbg_wfstep_update": [_vp, _vp, C.c_int, _vp, C.c_int],
+    "qbg_wfstep_results": [_vp, _dp],
+    "qbg_mixer_create": [_vp, C.c_double, C.c_double, C.c_int, C.c_double, C.POINTER(_vp)],
+    "qbg_mixer_destroy": [_vp],
+    "qbg_mixer_restart": [_vp],
+    "qbg_mixer_update": [_vp],
+    "qbg_polar": [_vp, _vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)],
     "qbg_graph_begin": [_vp],
     "qbg_graph_end": [_vp, C.POINTER(_vp)],
     "qbg_graph_launch": [_vp],
@@ -77,7 +88,7 @@ SIGNATURES = {
 }
 
 KERNEL_KINDS = ["zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd", "yother", "nl", "gram",
-                "other", "vgrid", "gemm"]
+                "other", "vgrid", "gemm", "wfstep", "mix"]
 ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p", "exc", "dxc",
                 "esr", "eself", "ecoul", "etotal", "total_charge"]
 
@@ -433,6 +444,14 @@ class EnergyFunctional:
             self.h, c.p, self.ngw, hpsi.p if hpsi is not None else None, self.ngw, flags,
             PREC[prec]))
 
+    def energy_async(self, c: DArray, hpsi: DArray = None, forces=False, stress=False,
+                     prec="fp64"):
+        """qbg_ef_energy_async: enqueue EnergyFunctional::energy alone."""
+        flags = (1 if hpsi is not None else 0) | (2 if forces else 0) | (4 if stress else 0)
+        self.ctx.check(self.ctx.lib.qbg_ef_energy_async(
+            self.h, c.p, self.ngw, hpsi.p if hpsi is not None else None, self.ngw, flags,
+            PREC[prec]))
+
     def results(self):
         """qbg_ef_results: wait and return the last evaluation's results."""
         e = np.zeros(14)
@@ -463,3 +482,88 @@ class EnergyFunctional:
     def set_vr(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         self.ctx.check(self.ctx.lib.qbg_ef_set_vr(self.h, _pd(v)))
+
+
+class WfStepper:
+    """qbg_wfstep: the SD, PSD and PSDA wavefunction steppers
+    (src/SDWavefunctionStepper.cpp, src/PSDWavefunctionStepper.cpp,
+    src/PSDAWavefunctionStepper.cpp).  param: dt2bye for SD, ecutprec
+    (Hartree, 0 = adaptive) for PSD and PSDA."""
+    KINDS = {"sd": 0, "psd": 1, "psda": 2}
+
+    def __init__(self, ef: EnergyFunctional, kind, param=0.0):
+        self.ctx = ef.ctx
+        self.ef = ef
+        k = self.KINDS[kind] if isinstance(kind, str) else int(kind)
+        h = _vp()
+        self.ctx.check(self.ctx.lib.qbg_wfstep_create(ef.h, k, float(param), C.byref(h)))
+        self.h = h
+        self.ctx.adopt(self)
+
+    def preprocess(self):
+        self.ctx.check(self.ctx.lib.qbg_wfstep_preprocess(self.h))
+
+    def update(self, c: DArray, dc: DArray):
+        """Enqueue WavefunctionStepper::update(dwf): dc holds H c on entry."""
+        self.ctx.check(self.ctx.lib.qbg_wfstep_update(self.h, c.p, self.ef.ngw, dc.p,
+                                                      self.ef.ngw))
+
+    def results(self):
+        out = np.zeros(4)
+        self.ctx.check(self.ctx.lib.qbg_wfstep_results(self.h, _pd(out)))
+        return dict(eigenvalue_sum=out[0], theta=out[1], polar_iterations=int(out[2]),
+                    status=int(out[3]))
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_wfstep_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Mixer:
+    """qbg_mixer: BOSampleStepper's charge mixing (Kerker weights, Anderson
+    acceleration over ndim previous densities; ndim = 0 is simple mixing)."""
+
+    def __init__(self, ef: EnergyFunctional, alpha=0.5, rc_kerker=10.0, ndim=3, rc1=3.0):
+        self.ctx = ef.ctx
+        h = _vp()
+        self.ctx.check(self.ctx.lib.qbg_mixer_create(ef.h, float(alpha), float(rc_kerker),
+                                                     int(ndim), float(rc1), C.byref(h)))
+        self.h = h
+        self.ctx.adopt(self)
+
+    def restart(self):
+        self.ctx.check(self.ctx.lib.qbg_mixer_restart(self.h))
+
+    def update(self):
+        self.ctx.check(self.ctx.lib.qbg_mixer_update(self.h))
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_mixer_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def polar(ctx: Context, x, tol=1e-6, maxiter=20):
+    """DoubleMatrix::polar of a square host matrix on the device; returns
+    (orthogonal factor, iterations)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[0]
+    d = ctx.to_device(np.ascontiguousarray(x.T))          # column major
+    it = C.c_int(0)
+    ctx.check(ctx.lib.qbg_polar(ctx.h, d.p, n, float(tol), int(maxiter), C.byref(it)))
+    u = d.download().reshape(n, n).T.copy()
+    d.release()
+    return u, it.value
