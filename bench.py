@@ -204,6 +204,57 @@ def bench_c4(args):
         "graph_over_loop": round(t_loop / t_graph, 3)}))
 
 
+def bench_scf(args):
+    """One SCF iteration of BOSampleStepper on the device (update_density,
+    Anderson/Kerker charge mixing, update_vhxc, energy with H*psi, PSDA
+    update with ortho_align), replayed from two captured graphs per
+    iteration; the energy step in the chosen precision, the stepper in FP64."""
+    import qbg
+    from qbg import configs as cf
+    from qbg.scf import ScfLoop
+    sysdef = cf.config(args.config)
+    ctx = qbg.Context(0)
+    ef = qbg.EnergyFunctional(ctx, sysdef)
+    b = ef.basis(0)
+    c = cf.random_coefficients(cf.BASE_SEED + 50 + args.config, b["g2"], sysdef.nst)
+    dc = ctx.to_device(c)
+    qbg.gram(ctx, dc, ef.ngw, ef.ngw, sysdef.nst)
+    hp = ctx.array(c.shape, np.complex128)
+    prec = "bf16x2" if args.prec == "mixed" else "fp64"
+    print(f"[bench scf] {sysdef.name} prec {prec}", file=sys.stderr, flush=True)
+    # kernel breakdown from eager iterations with the profiler on
+    loop = ScfLoop(ef, dc, hp, wf_dyn="psda", prec=prec, graphs=False)
+    prof = qbg.Profiler(ctx)
+    loop.run(1, 1)
+    prof.reset()
+    prof.enable(True)
+    loop.run(2, 1)
+    prof.enable(False)
+    kt = prof.read()
+    loop.release()
+    # timed: graph replays after the first (eager) iteration
+    dc.upload(c)
+    qbg.gram(ctx, dc, ef.ngw, ef.ngw, sysdef.nst)
+    loop = ScfLoop(ef, dc, hp, wf_dyn="psda", prec=prec, graphs=True)
+    nit = 1 + args.warmup + args.steps
+    hist = loop.run(nit, 1)
+    secs = [h["seconds"] for h in hist[1 + args.warmup:]]
+    t = float(np.mean(secs))
+    loop.release()
+    print(json.dumps({
+        "metric": "SCF iterations/s on 1 GPU (density, charge mixing, vhxc, H*psi, PSDA "
+                  "update with ortho_align), one graph pair per iteration",
+        "value": round(1.0 / t, 3), "unit": "iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3),
+        "higher_is_better": True, "dtype": "f32+f64" if prec != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": sysdef.name, "bands": sysdef.nst, "precision": prec,
+                   "stepper": "PSDA (FP64)", "mixer": "Anderson ndim 3, Kerker rc 10"},
+        "band_updates_per_s": round(sysdef.nst / t, 2),
+        "etotal_per_iteration": [h["etotal"] for h in hist],
+        "eager_kernel_ms_per_iteration": {k: round(v[0] / 2, 4) for k, v in kt.items()}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c4", action="store_true",
                     help="only the uneven batch: twelve systems, per-system loop vs one graph")
+    ap.add_argument("--scf", action="store_true",
+                    help="only time whole SCF iterations (PSDA stepper, Anderson mixing)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -232,6 +285,8 @@ def main():
         return
     if args.c4:
         return bench_c4(args)
+    if args.scf:
+        return bench_scf(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (band sharding + RCCL reduction of rho), so "

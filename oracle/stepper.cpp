@@ -19,7 +19,7 @@ namespace {
 // a[i + j*n] = alpha sum_k p_i[k] q_j[k] + gamma p_i[0] q_j[0]  (real views
 // of length m; DoubleMatrix::gemm('t','n') followed by ger on row 0)
 void tn(const double* p, const double* q, long m, int n, double alpha, double gamma, double* a) {
-#pragma omp parallel for schedule(static) collapse(2)
+#pragma omp parallel for schedule(static) collapse(2) if ((long)n * n * m > (1L << 22))
   for (int j = 0; j < n; j++)
     for (int i = 0; i < n; i++) {
       const double* x = p + (long)i * m;
@@ -32,7 +32,7 @@ void tn(const double* p, const double* q, long m, int n, double alpha, double ga
 
 // c (m x n) = a (m x kk) b (kk x n), column major
 void nn(const double* a, long m, const double* b, int kk, int n, double* c) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)n * kk * m > (1L << 22))
   for (int j = 0; j < n; j++) {
     double* cj = c + (long)j * m;
     for (long r = 0; r < m; r++) cj[r] = 0.0;
@@ -64,7 +64,7 @@ int potrf_lower(double* s, int n) {
 // solve L X = B (trans = 0) or L^T X = B (trans = 1) in place, L lower
 // (dtrtrs('l', 'n'/'t', 'n'), src/Matrix.cpp:1959-1989)
 void trtrs_lower(const double* l, int n, double* b, int nrhs, int trans) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((long)n * n * nrhs > (1L << 22))
   for (int c = 0; c < nrhs; c++) {
     double* x = b + (long)c * n;
     if (!trans) {

@@ -23,6 +23,7 @@ enum {
   S_ENL = 32,     // 1
   S_NLS = 33,     // 6: NL stress sums tsum_ij
   S_HPS = 39,     // 6: species part of the Hartree stress
+  S_FLAGS = 45,   // 2: forces, stress computed by the staged evaluation
   S_COUNT = 48
 };
 
@@ -706,6 +707,10 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
                         nchunk, e->d_fion, s);
     ctx->prof_end();
   }
+  // what this evaluation computed travels with its results, so a replayed
+  // graph is read correctly after other evaluations ran in between
+  fill_value(e->d_scal + S_FLAGS, 1, compute_forces ? 1.0 : 0.0, s);
+  fill_value(e->d_scal + S_FLAGS + 1, 1, compute_stress ? 1.0 : 0.0, s);
   // stage the scalars, G=0 values and forces in pinned host memory
   gather_segments(e->d_seg_ptr, e->d_seg_n, 4, e->h_res, s);
   e->last_forces = compute_forces;
@@ -722,8 +727,8 @@ static int ef_collect(qbg_ef* e, double* energies, double* fion, double* sigma) 
   if (ctx->capturing) return ctx->fail(QBG_E_STATE, "results during graph capture");
   if (!e->launched) return ctx->fail(QBG_E_STATE, "no evaluation launched");
   QBG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int compute_forces = e->last_forces, compute_stress = e->last_stress;
   const double* sc = e->h_res;
+  const int compute_forces = sc[S_FLAGS] != 0.0, compute_stress = sc[S_FLAGS + 1] != 0.0;
   const double* g0 = e->h_res + S_COUNT;   // rhoelg[0] (2), vion[0] (2)
   if (compute_forces && fion) std::memcpy(fion, e->h_res + S_COUNT + 4, sizeof(double) * 3 * e->nat);
   const double omega = e->omega, fpi = 4.0 * M_PI, Nd = (double)e->N;

@@ -139,6 +139,7 @@ int density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double
                     double* rho, int prec);
 // util.hip
 void fill_zero(double* p, size_t n, hipStream_t s);
+void fill_value(double* p, size_t n, double v, hipStream_t s);
 void gather_segments(const double* const* seg_ptr, const int* seg_n, int nseg, double* dst,
                      hipStream_t s);
 }  // namespace qbg

@@ -27,16 +27,17 @@ extern "C" int qbg_stream_copy(qbg_ctx* c, void* dst, const void* src, size_t by
 
 namespace qbg {
 
-// zero fill (used instead of memset nodes, which graphs must not record)
-__global__ __launch_bounds__(256) void k_fill0(double* __restrict__ p, size_t n) {
+// fill (used instead of memset nodes, which graphs must not record)
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, size_t n, double v) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-    p[i] = 0.0;
+    p[i] = v;
 }
-void fill_zero(double* p, size_t n, hipStream_t s) {
+void fill_value(double* p, size_t n, double v, hipStream_t s) {
   if (!n) return;
   const size_t nb = std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill0, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
 }
+void fill_zero(double* p, size_t n, hipStream_t s) { fill_value(p, n, 0.0, s); }
 
 // gather scattered device scalars into one (host-mapped) buffer:
 // dst[i] = src segment by segment, segments (ptr, count) given in seg

@@ -16,6 +16,8 @@ replayed for every later iteration.
 """
 from __future__ import annotations
 
+import time
+
 from ._lib import Context, DArray, EnergyFunctional, Graph, Mixer, WfStepper
 
 
@@ -75,6 +77,7 @@ class ScfLoop:
         ehart = delta_ehart = 0.0
         for itscf in range(nitscf):
             self.itscf = itscf
+            t0 = time.perf_counter()
             self._run(("pot", nite > 0), self._potential, nite > 0)
             eigs = []
             res = None
@@ -94,7 +97,8 @@ class ScfLoop:
                 if ite > 0 and abs(eigs[-1] - eigs[-2]) < delta_ratio * delta_ehart:
                     break
             history.append(dict(etotal=res["etotal"], ehart=res["ehart"], eigenvalue_sum=eigs,
-                                steps=len(eigs), energies=res))
+                                steps=len(eigs), energies=res,
+                                seconds=time.perf_counter() - t0))
         return history
 
     def release(self):
