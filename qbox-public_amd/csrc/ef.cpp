@@ -864,7 +864,7 @@ int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
     release();
     return c->fail(QBG_E_INVAL, "gram workspace");
   }
-  potrf(S, nst, nst, info, s);
+  potrf(S, nst, nst, info, s, W);
   c->prof_end();
   // the coefficients are only touched once the factorization succeeded
   // (the reference aborts in potrf, src/Matrix.cpp:2324-2327)

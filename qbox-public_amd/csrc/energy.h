@@ -80,6 +80,11 @@ int gemm_tn_splitk(const float* A, long lda, const double* B, long ldb, int M, i
 int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gamma, double* out,
               long ldo, double* work, size_t work_doubles, hipStream_t s,
               const int* skip = nullptr);
+// NN (FP64) split over the reduction for small outputs (partials in work):
+// C[k + n*ldc] = sum_m A[k + m*lda] F[m + n*ldf].  Returns -1 on a bad shape.
+int gemm_nn_splitk(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
+                   double* C, long ldc, double* work, size_t work_doubles, hipStream_t s,
+                   const int* skip = nullptr);
 // NN (FP64) with F upper triangular (F[m + n*ldf] = 0 for m > n), N x N:
 // C[k + n*ldc] = sum_{m <= n} A[k + m*lda] F[m + n*ldf]
 void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
@@ -112,7 +117,10 @@ void convert_f32(const double* c, long ldc2, long K, int nst, float* c32, long l
 void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32, long ldf,
               float* w32, long ldw, hipStream_t s);
 
-void potrf(double* S, int n, int lds, int* info, hipStream_t s);
+// lower Cholesky in place; info[0] = 0 or the failing pivot + 1.  With a
+// workspace of 64 * n doubles (ws) the factorization runs blocked over the
+// whole device, otherwise in one work-group
+void potrf(double* S, int n, int lds, int* info, hipStream_t s, double* ws = nullptr);
 void trtri(const double* L, int n, int ldl, double* X, hipStream_t s);
 void cleanup(double* c, long ldc, int nst, hipStream_t s);
 

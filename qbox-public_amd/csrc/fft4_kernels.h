@@ -218,25 +218,31 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
     const cpx<T>* __restrict__ zbuf, cpx<T>* __restrict__ xbuf, int np2, int ncols, int nact,
     const int32_t* __restrict__ colmap, const cpx<T>* __restrict__ tw_g) {
   constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2), NXB = nblk(N, XB);
+  using Mp = Map<N1, N2>;
   __shared__ cpx<T> ex[KB * NP];
-  __shared__ cpx<T> tw[N];
+  __shared__ cpx<T> tw[Mp::pfa ? 1 : N];
   const int tid = threadIdx.x;
   const int nkb = nblk(np2, KB);
   const int kb = blockIdx.x % nkb, r = blockIdx.x / nkb;
   const int pair = blockIdx.y;
   const int nk = min(KB, np2 - kb * KB);           // valid planes of this block
-  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
-  __syncthreads();
+  if constexpr (!Mp::pfa) {
+    for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+    __syncthreads();
+  }
   const int kk = tid % KB, p = tid / KB;
-  using Mp = Map<N1, N2>;
-  const int32_t* cm = colmap + (long)r * N;
+  // the row's column map staged in LDS: one coalesced load per block
+  // instead of a dependent global load per element
+  __shared__ int32_t cml[N];
+  for (int i = tid; i < N; i += NT) cml[i] = colmap[(long)r * N + i];
+  __syncthreads();
   if (p < N2 && kk < nk) {                // phase A, p = n2
     const cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
     cpx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; n1++) {
-      const int c = cm[Mp::in_a(n1, p)];
-      a[n1] = c >= 0 ? zb[(long)c * KB] : cpx<T>{T(0), T(0)};
+      const int c = cml[Mp::in_a(n1, p)];
+      a[n1] = c >= 0 ? zb[c * KB] : cpx<T>{T(0), T(0)};
     }
     rdft<T, N1, +1>(a);
     if constexpr (!Mp::pfa) {
@@ -278,14 +284,18 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
     const cpx<T>* __restrict__ xbuf, cpx<T>* __restrict__ zbuf, int np2, int ncols, int nact,
     const int32_t* __restrict__ colmap, const cpx<T>* __restrict__ tw_g) {
   constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2), NXB = nblk(N, XB);
+  using Mp = Map<N1, N2>;
   __shared__ cpx<T> ex[KB * NP];
-  __shared__ cpx<T> tw[N];
+  __shared__ cpx<T> tw[Mp::pfa ? 1 : N];
   const int tid = threadIdx.x;
   const int nkb = nblk(np2, KB);
   const int kb = blockIdx.x % nkb, r = blockIdx.x / nkb;
   const int pair = blockIdx.y;
   const int nk = min(KB, np2 - kb * KB);
-  for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  if constexpr (!Mp::pfa)
+    for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  __shared__ int32_t cml[N];
+  for (int i = tid; i < N; i += NT) cml[i] = colmap[(long)r * N + i];
   {
     const cpx<T>* xo = xbuf + (((long)pair * np2 + kb * KB) * NXB * nact + r) * XB;
     const int sxi = tid % XB, sq = tid / XB;
@@ -297,7 +307,6 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
   }
   __syncthreads();
   const int kk = tid % KB, p = tid / KB;
-  using Mp = Map<N1, N2>;
   cpx<T> b[N2];
   const bool va = p < N1 && kk < nk;
   if (va) {                               // forward phase A', p = k1
@@ -322,11 +331,10 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
     for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[kk * NP + p * N1 + k1];
     rdft<T, N1, -1>(cv);
     cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
-    const int32_t* cm = colmap + (long)r * N;
 #pragma unroll
     for (int m2 = 0; m2 < N1; m2++) {
-      const int c = cm[Mp::in_a(m2, p)];
-      if (c >= 0) zb[(long)c * KB] = cv[m2];
+      const int c = cml[Mp::in_a(m2, p)];
+      if (c >= 0) zb[c * KB] = cv[m2];
     }
   }
 }

@@ -215,10 +215,12 @@ int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, l
   g.tri = sym ? 1 : 0;
   const long ksteps = (K + KC - 1) / KC;
   const int ntile = g.tp * g.tq;
-  // about four blocks per CU; partial traffic kept well below operand traffic
+  // about four blocks per CU; for large outputs the partial traffic is kept
+  // well below the operand traffic (small outputs split freely)
   long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
   S = std::min<long>(S, std::max<long>(1, ksteps / 4));
-  S = std::min<long>(S, std::max<long>(1, (long)((double)K * (M + N) / (4.0 * M * N))));
+  if ((long)M * N > (1L << 20))
+    S = std::min<long>(S, std::max<long>(1, (long)((double)K * (M + N) / (4.0 * M * N))));
   while (S > 1 && (size_t)S * M * N > work_doubles) S--;
   if ((size_t)S * M * N > work_doubles) return -1;
   g.rper = ((ksteps + S - 1) / S) * KC;
@@ -251,6 +253,37 @@ void nn(const TQm* A, long lda, const double* F, long ldf, long K, int N, int M,
 }
 
 }  // namespace
+
+// NN with the reduction (M) split over blocks, partials in work, for products
+// whose output has too few 128x128 tiles to fill the device:
+// C[k + n*ldc] = sum_m A[k + m*lda] F[m + n*ldf]
+int gemm_nn_splitk(const double* A, long lda, const double* F, long ldf, long K, int N, int M,
+                   double* C, long ldc, double* work, size_t work_doubles, hipStream_t s,
+                   const int* skip) {
+  if (K <= 0 || N <= 0 || M <= 0) return -1;
+  GemmArgs g{};
+  g.skip = skip;
+  g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
+  g.np = N; g.nq = (int)K; g.nr = M;
+  g.tp = (N + TP - 1) / TP; g.tq = (int)((K + TQ - 1) / TQ);
+  g.pfast = 1;
+  g.tri = 0;
+  const long msteps = (M + KC - 1) / KC;
+  const int ntile = g.tp * g.tq;
+  long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
+  S = std::min<long>(S, std::max<long>(1, msteps / 4));
+  while (S > 1 && (size_t)S * K * N > work_doubles) S--;
+  if ((size_t)S * K * N > work_doubles) return -1;
+  g.rper = ((msteps + S - 1) / S) * KC;
+  S = (M + g.rper - 1) / g.rper;
+  g.out = work; g.ldo = K; g.sstride = K * N; g.accumulate = 0;
+  launch_gemm<double, double, double, false, double>(g, s);
+  const long tot = K * N;
+  hipLaunchKernelGGL((k_split_reduce<double, double>), dim3((unsigned)((tot + 255) / 256)),
+                     dim3(256), 0, s, work, (int)S, (int)K, N, 1.0, 0.0, A, lda, F, ldf, C, ldc,
+                     0, skip);
+  return 0;
+}
 
 int gemm_tn_splitk(const double* A, long lda, const double* B, long ldb, int M, int N, long K,
                    double alpha, double gamma, double* out, long ldo, double* work,
@@ -391,6 +424,89 @@ __global__ __launch_bounds__(1024) void k_potrf_blk(double* __restrict__ S, int 
   if (tid == 0) info[0] = bad;
 }
 
+// Right-looking blocked Cholesky over the whole device, panels of CB
+// columns: the diagonal block factored in LDS by one work-group, the rows
+// below solved against it (one row per thread), and the trailing matrix
+// updated by an MFMA GEMM with the panel's negated transpose kept in ws.
+// info as for k_potrf; after a failure the remaining launches do nothing.
+constexpr int CB = 64;
+
+__global__ void k_fill_int0(int* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ S, int lds, int j0,
+                                                   int jb, int* __restrict__ info) {
+  if (*info) return;
+  __shared__ double D[CB][CB + 1];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < jb * jb; e += 256) D[e % jb][e / jb] = S[(j0 + e % jb) + (long)(j0 + e / jb) * lds];
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int c = 0; c < jb; c++) {
+    const double d = D[c][c];
+    if (!(d > 0.0)) {                      // uniform: every thread read D[c][c]
+      if (tid == 0) *info = j0 + c + 1;
+      return;
+    }
+    const double sd = sqrt(d);
+    __syncthreads();
+    if (tid == 0) D[c][c] = sd;
+    for (int r = c + 1 + tid; r < jb; r += 256) D[r][c] /= sd;
+    __syncthreads();
+    // D[r][k] -= D[r][c] D[k][c], c < k <= r
+    const int m = jb - c - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int r = c + 1 + e / m, k = c + 1 + e % m;
+      if (k <= r) D[r][k] -= D[r][c] * D[k][c];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < jb * jb; e += 256) {
+    const int r = e % jb, k = e / jb;
+    if (k <= r) S[(j0 + r) + (long)(j0 + k) * lds] = D[r][k];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_chol_panel(double* __restrict__ S, int n, int lds, int j0,
+                                                   int jb, const int* __restrict__ info,
+                                                   double* __restrict__ lt) {
+  if (*info) return;
+  __shared__ double D[CB][CB + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < jb * jb; e += 64) D[e % jb][e / jb] = S[(j0 + e % jb) + (long)(j0 + e / jb) * lds];
+  __syncthreads();
+  const int j1 = j0 + jb, i = blockIdx.x * 64 + tid;
+  if (j1 + i >= n) return;
+  double x[CB];
+#pragma unroll
+  for (int c = 0; c < CB; c++) {
+    if (c < jb) {
+      double v = S[(j1 + i) + (long)(j0 + c) * lds];
+#pragma unroll
+      for (int t = 0; t < c; t++) v -= x[t] * D[c][t];
+      x[c] = v / D[c][c];
+      S[(j1 + i) + (long)(j0 + c) * lds] = x[c];
+      lt[c + (long)i * CB] = -x[c];
+    }
+  }
+}
+
+static void potrf_blocked(double* S, int n, int lds, int* info, double* ws, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int0, dim3(1), dim3(64), 0, s, info);
+  for (int j0 = 0; j0 < n; j0 += CB) {
+    const int jb = std::min(CB, n - j0), rows = n - j0 - jb;
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, S, lds, j0, jb, info);
+    if (rows <= 0) break;
+    hipLaunchKernelGGL(k_chol_panel, dim3((rows + 63) / 64), dim3(64), 0, s, S, n, lds, j0, jb,
+                       info, ws);
+    // A22 += L21 (-L21^T)
+    nn(S + (j0 + jb) + (long)j0 * lds, lds, ws, CB, rows, rows, jb, S + (j0 + jb) +
+       (long)(j0 + jb) * lds, lds, 1, QBG_PREC_FP64, 0, s, info);
+  }
+}
+
 // X = L^{-1} (lower), written row-major (X[i*n + c], upper part zero):
 // block column J of TB columns per work-group, X_IJ = L_II^{-1} (E_IJ -
 // sum_{J <= K < I} L_IK X_KJ) for I = J, J+1, ... with 32 x 32 tiles staged
@@ -455,7 +571,11 @@ __global__ __launch_bounds__(256) void k_trtri_blk(const double* __restrict__ L,
   }
 }
 
-void potrf(double* S, int n, int lds, int* info, hipStream_t s) {
+void potrf(double* S, int n, int lds, int* info, hipStream_t s, double* ws) {
+  if (ws && n > 2 * CB) {
+    potrf_blocked(S, n, lds, info, ws, s);
+    return;
+  }
   if (n <= POTRF_NMAX)
     hipLaunchKernelGGL(k_potrf_blk, dim3(1), dim3(1024), 0, s, S, n, lds, info);
   else

@@ -9,9 +9,12 @@
 namespace qbg {
 
 // per band n of c (real view, ld ldc2 doubles) and d:
-//   ekin[n] (Preconditioner::update) and dotp[n] = 2 c_n.d_n - Re c_n(0) Re d_n(0)
+//   ekin[n] (Preconditioner::update) and dotp[n] = 2 c_n.d_n - Re c_n(0) Re d_n(0);
+// part holds band_partials(nst) doubles
+int band_partials(int nst);
 void launch_band_sums(const double* c, long ldc2, const double* d, long ldd2, const double* g2,
-                      int ngw, int nst, double* ekin, double* dotp, hipStream_t s);
+                      int ngw, int nst, double* part, double* ekin, double* dotp,
+                      hipStream_t s);
 // out[0] = sum_n dotp[n] in band order
 void launch_sum(const double* v, int n, double* out, hipStream_t s);
 // c -= diag(n, g) d  (PSDWavefunctionStepper::update)
@@ -32,7 +35,8 @@ void launch_psda_update(double* c, long ldc2, const double* d, long ldd2, double
                         const double* theta, int ngw, int nst, hipStream_t s);
 // *p = v
 void launch_set_int(int* p, int v, hipStream_t s);
-// dst (columns of 2 ngw doubles, ld lddst) = src (ld ldsrc) if *info == 0
+// dst (columns of `rows` doubles, ld lddst) = src (ld ldsrc) if *info == 0;
+// rows and both leading dimensions even
 void launch_copy_cols_if(double* dst, long lddst, const double* src, long ldsrc, long rows,
                          int cols, const int* info, hipStream_t s);
 

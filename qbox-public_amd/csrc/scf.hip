@@ -59,19 +59,23 @@ __device__ __forceinline__ double pdiag(double ekin_n, double g2, double ecutpre
 
 // ------------------------------------------------------------ band sums
 // Preconditioner::update (src/Preconditioner.cpp:88-143) and the terms of
-// SlaterDet::dot (src/SlaterDet.cpp:1122-1148), one block per band
-__global__ __launch_bounds__(256) void k_band_sums(const double* __restrict__ c, long ldc2,
-                                                   const double* __restrict__ d, long ldd2,
-                                                   const double* __restrict__ g2, int ngw,
-                                                   double* __restrict__ ekin,
-                                                   double* __restrict__ dotp) {
-  const int n = blockIdx.x;
-  const double* cn = c + ldc2 * n;
-  const double* dn = d + ldd2 * n;
+// SlaterDet::dot (src/SlaterDet.cpp:1122-1148): BS_CHUNKS blocks per band
+// write partial sums, one thread per band combines them in a fixed order
+constexpr int BS_CHUNKS = 8;
+
+__global__ __launch_bounds__(256) void k_band_partials(const double* __restrict__ c, long ldc2,
+                                                       const double* __restrict__ d, long ldd2,
+                                                       const double* __restrict__ g2, int ngw,
+                                                       double* __restrict__ part) {
+  const int n = blockIdx.y;
+  const double2* cn = reinterpret_cast<const double2*>(c + ldc2 * n);
+  const double2* dn = reinterpret_cast<const double2*>(d + ldd2 * n);
+  const int per = (ngw + BS_CHUNKS - 1) / BS_CHUNKS;
+  const int g0 = blockIdx.x * per, g1 = min(ngw, g0 + per);
   double v[3] = {0.0, 0.0, 0.0};
-  for (int g = threadIdx.x; g < ngw; g += 256) {
-    const double2 cc = *reinterpret_cast<const double2*>(cn + 2 * g);
-    const double2 dd = *reinterpret_cast<const double2*>(dn + 2 * g);
+  for (int g = g0 + threadIdx.x; g < g1; g += 256) {
+    const double2 cc = cn[g];
+    const double2 dd = dn[g];
     const double p2 = cc.x * cc.x + cc.y * cc.y;
     v[0] += p2;
     v[1] += p2 * g2[g];
@@ -79,18 +83,44 @@ __global__ __launch_bounds__(256) void k_band_sums(const double* __restrict__ c,
   }
   bsum<3>(v);
   if (threadIdx.x == 0) {
-    const double c0r = cn[0], c0i = cn[1];
-    const double bn = 2.0 * (v[0] - 0.5 * (c0r * c0r + c0i * c0i));
-    const double be = 2.0 * v[1];
-    ekin[n] = 0.5 * bn > 0.0 ? 0.5 * be / bn : 0.0;
-    dotp[n] = 2.0 * v[2] - c0r * dn[0];
+    double* o = part + 3 * ((long)n * BS_CHUNKS + blockIdx.x);
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
   }
 }
 
+__global__ __launch_bounds__(256) void k_band_finish(const double* __restrict__ part,
+                                                     const double* __restrict__ c, long ldc2,
+                                                     const double* __restrict__ d, long ldd2,
+                                                     int nst, double* __restrict__ ekin,
+                                                     double* __restrict__ dotp) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= nst) return;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  for (int k = 0; k < BS_CHUNKS; k++) {
+    const double* o = part + 3 * ((long)n * BS_CHUNKS + k);
+    v0 += o[0];
+    v1 += o[1];
+    v2 += o[2];
+  }
+  const double* cn = c + ldc2 * n;
+  const double c0r = cn[0], c0i = cn[1];
+  const double bn = 2.0 * (v0 - 0.5 * (c0r * c0r + c0i * c0i));
+  const double be = 2.0 * v1;
+  ekin[n] = 0.5 * bn > 0.0 ? 0.5 * be / bn : 0.0;
+  dotp[n] = 2.0 * v2 - c0r * d[ldd2 * n];
+}
+
+int band_partials(int nst) { return 3 * BS_CHUNKS * nst; }
+
 void launch_band_sums(const double* c, long ldc2, const double* d, long ldd2, const double* g2,
-                      int ngw, int nst, double* ekin, double* dotp, hipStream_t s) {
-  hipLaunchKernelGGL(k_band_sums, dim3(nst), dim3(256), 0, s, c, ldc2, d, ldd2, g2, ngw, ekin,
-                     dotp);
+                      int ngw, int nst, double* part, double* ekin, double* dotp,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_band_partials, dim3(BS_CHUNKS, nst), dim3(256), 0, s, c, ldc2, d, ldd2,
+                     g2, ngw, part);
+  hipLaunchKernelGGL(k_band_finish, dim3((nst + 255) / 256), dim3(256), 0, s, part, c, ldc2, d,
+                     ldd2, nst, ekin, dotp);
 }
 
 __global__ __launch_bounds__(256) void k_sum(const double* __restrict__ v, int n,
@@ -114,11 +144,13 @@ __global__ __launch_bounds__(256) void k_psd_update(double* __restrict__ c, long
   const int g = blockIdx.x * 256 + threadIdx.x, n = blockIdx.y;
   if (g >= ngw) return;
   const double f = pdiag(ekin[n], g2[g], ecutprec);
-  double* cp = c + ldc2 * n + 2 * g;
-  const double* dp = d + ldd2 * n + 2 * g;
+  double2* cp = reinterpret_cast<double2*>(c + ldc2 * n) + g;
+  const double2 dv = reinterpret_cast<const double2*>(d + ldd2 * n)[g];
+  double2 cv = *cp;
   // c -= f d as two rounded operations (src/PSDWavefunctionStepper.cpp:99-105)
-  cp[0] = __dsub_rn(cp[0], __dmul_rn(f, dp[0]));
-  cp[1] = __dsub_rn(cp[1], __dmul_rn(f, dp[1]));
+  cv.x = __dsub_rn(cv.x, __dmul_rn(f, dv.x));
+  cv.y = __dsub_rn(cv.y, __dmul_rn(f, dv.y));
+  *cp = cv;
 }
 
 void launch_psd_update(double* c, long ldc2, const double* d, long ldd2, const double* g2,
@@ -132,10 +164,12 @@ __global__ __launch_bounds__(256) void k_sd_update(double* __restrict__ c, long 
                                                    int ngw, double alpha) {
   const int g = blockIdx.x * 256 + threadIdx.x, n = blockIdx.y;
   if (g >= ngw) return;
-  double* cp = c + ldc2 * n + 2 * g;
-  const double* dp = d + ldd2 * n + 2 * g;
-  cp[0] = cp[0] + (-alpha) * dp[0];
-  cp[1] = cp[1] + (-alpha) * dp[1];
+  double2* cp = reinterpret_cast<double2*>(c + ldc2 * n) + g;
+  const double2 dv = reinterpret_cast<const double2*>(d + ldd2 * n)[g];
+  double2 cv = *cp;
+  cv.x = cv.x + (-alpha) * dv.x;
+  cv.y = cv.y + (-alpha) * dv.y;
+  *cp = cv;
 }
 
 void launch_sd_update(double* c, long ldc2, const double* d, long ldd2, int ngw, int nst,
@@ -160,12 +194,12 @@ __global__ __launch_bounds__(256) void k_psda_prec(double* __restrict__ d, long 
     const int g = blockIdx.x * 1024 + j * 256 + threadIdx.x;
     if (g < ngw) {
       const double f = pdiag(ekin[n], g2[g], ecutprec);
-      double* dp = d + ldd2 * n + 2 * g;
-      const double* lp = dl + 2L * ngw * n + 2 * g;
-      const double f0 = -f * dp[0], f1 = -f * dp[1];
-      dp[0] = f0;
-      dp[1] = f1;
-      const double d0 = f0 - lp[0], d1 = f1 - lp[1];
+      double2* dp = reinterpret_cast<double2*>(d + ldd2 * n) + g;
+      const double2 lv = reinterpret_cast<const double2*>(dl + 2L * ngw * n)[g];
+      const double2 dv = *dp;
+      const double f0 = -f * dv.x, f1 = -f * dv.y;
+      *dp = double2{f0, f1};
+      const double d0 = f0 - lv.x, d1 = f1 - lv.y;
       // G and -G count twice, G = 0 once (src/PSDAWavefunctionStepper.cpp:140-163)
       const double w = g == 0 ? 1.0 : 2.0;
       v[0] += w * (f0 * d0 + f1 * d1);
@@ -223,19 +257,17 @@ __global__ __launch_bounds__(256) void k_psda_update(double* __restrict__ c, lon
   const int g = blockIdx.x * 256 + threadIdx.x, n = blockIdx.y;
   if (g >= ngw) return;
   const double th = *theta;
-  double* cp = c + ldc2 * n + 2 * g;
-  const double* dp = d + ldd2 * n + 2 * g;
-  double* lp = cl + 2L * ngw * n + 2 * g;
-  double* fp = dl + 2L * ngw * n + 2 * g;
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const double x = cp[k], xl = lp[k], f = dp[k], fl = fp[k];
-    const double xbar = x + th * (x - xl);
-    const double fbar = f + th * (f - fl);
-    cp[k] = xbar + fbar;
-    lp[k] = x;
-    fp[k] = f;
-  }
+  double2* cp = reinterpret_cast<double2*>(c + ldc2 * n) + g;
+  double2* lp = reinterpret_cast<double2*>(cl + 2L * ngw * n) + g;
+  double2* fp = reinterpret_cast<double2*>(dl + 2L * ngw * n) + g;
+  const double2 x = *cp, xl = *lp, f = reinterpret_cast<const double2*>(d + ldd2 * n)[g],
+                fl = *fp;
+  double2 out;
+  out.x = (x.x + th * (x.x - xl.x)) + (f.x + th * (f.x - fl.x));
+  out.y = (x.y + th * (x.y - xl.y)) + (f.y + th * (f.y - fl.y));
+  *cp = out;
+  *lp = x;
+  *fp = f;
 }
 
 void launch_psda_update(double* c, long ldc2, const double* d, long ldd2, double* cl, double* dl,
@@ -244,19 +276,22 @@ void launch_psda_update(double* c, long ldc2, const double* d, long ldd2, double
                      ldd2, cl, dl, theta, ngw);
 }
 
+// rows, lddst, ldsrc even: columns of complex values copied as double2
 __global__ __launch_bounds__(256) void k_copy_cols_if(double* __restrict__ dst, long lddst,
                                                       const double* __restrict__ src, long ldsrc,
                                                       long rows, const int* __restrict__ info) {
   if (*info) return;
   const long r = (long)blockIdx.x * 256 + threadIdx.x;
   const int j = blockIdx.y;
-  if (r < rows) dst[lddst * j + r] = src[ldsrc * j + r];
+  if (2 * r < rows)
+    reinterpret_cast<double2*>(dst + lddst * j)[r] =
+        reinterpret_cast<const double2*>(src + ldsrc * j)[r];
 }
 
 void launch_copy_cols_if(double* dst, long lddst, const double* src, long ldsrc, long rows,
                          int cols, const int* info, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_cols_if, dim3((unsigned)((rows + 255) / 256), cols), dim3(256), 0, s,
-                     dst, lddst, src, ldsrc, rows, info);
+  hipLaunchKernelGGL(k_copy_cols_if, dim3((unsigned)((rows / 2 + 255) / 256), cols), dim3(256), 0,
+                     s, dst, lddst, src, ldsrc, rows, info);
 }
 
 __global__ void k_set_int(int* p, int v) {

@@ -63,9 +63,11 @@ int polar_enqueue(const PolarWs& w, double* Y, int n, double tol, int maxiter, d
     launch_polar_norm(w.Q, n, tol, w.flags, w.qnrm, s);
     // Bjorck-Bowie: Y <- Y (I + q/2 (I + 3q/4))
     launch_ident_axpy(w.T, w.Q, n, 0.75, w.flags + 1, s);
-    gemm_nn(w.Q, n, w.T, n, n, n, n, w.QT, n, 0, QBG_PREC_FP64, s, w.flags + 1);
+    if (gemm_nn_splitk(w.Q, n, w.T, n, n, n, n, w.QT, n, W, wsize, s, w.flags + 1))
+      return QBG_E_INVAL;
     launch_ident_axpy(w.T, w.QT, n, 0.5, w.flags + 1, s);
-    gemm_nn(Y, n, w.T, n, n, n, n, w.XP, n, 0, QBG_PREC_FP64, s, w.flags + 1);
+    if (gemm_nn_splitk(Y, n, w.T, n, n, n, n, w.XP, n, W, wsize, s, w.flags + 1))
+      return QBG_E_INVAL;
     launch_copy_if_not(Y, w.XP, n2, w.flags + 1, s);
     // Higham: Y <- (Y + Y^-T) / 2
     launch_polar_higham(Y, n, w.aug, w.flags, s);
@@ -90,6 +92,7 @@ struct qbg_wfstep {
   size_t wsize = 0;
   double *cl = nullptr, *dl = nullptr;   // PSDA: previous bands and directions
   double *ekin = nullptr, *dotp = nullptr, *partial = nullptr, *scal = nullptr;
+  double* bpart = nullptr;               // per-band partial sums
   int *info = nullptr, *extrap = nullptr;
   int npart = 0;
   PolarWs pw;
@@ -117,7 +120,7 @@ int gram_enqueue(qbg_wfstep* w, double* c, int ldc, hipStream_t s) {
   const int n = w->nst;
   cleanup(c, ldc, n, s);
   if (gemm_syrk(c, ldc2, n, K, 2.0, -1.0, w->S, n, w->W, w->wsize, s)) return QBG_E_INVAL;
-  potrf(w->S, n, n, w->info, s);
+  potrf(w->S, n, n, w->info, s, w->W);
   trtri(w->S, n, n, w->X, s);
   gemm_nn_triu(c, ldc2, w->X, n, K, n, w->Tc, K, s);
   launch_copy_cols_if(c, ldc2, w->Tc, K, K, n, w->info, s);
@@ -131,7 +134,7 @@ int ortho_align_enqueue(qbg_wfstep* w, double* c, int ldc, hipStream_t s) {
   const int n = w->nst;
   cleanup(c, ldc, n, s);
   if (gemm_syrk(c, ldc2, n, K, 2.0, -1.0, w->S, n, w->W, w->wsize, s)) return QBG_E_INVAL;
-  potrf(w->S, n, n, w->info, s);
+  potrf(w->S, n, n, w->info, s, w->W);
   trtri(w->S, n, n, w->X, s);                       // X = L^-T (column major)
   if (gemm_tn_splitk(c, ldc2, w->cl, K, n, n, K, 2.0, -1.0, w->B, n, w->W, w->wsize,
                      QBG_PREC_FP64, s))
@@ -142,7 +145,8 @@ int ortho_align_enqueue(qbg_wfstep* w, double* c, int ldc, hipStream_t s) {
     return QBG_E_INVAL;
   int rc = polar_enqueue(w->pw, w->Y, n, 1.e-6, 20, w->W, w->wsize, s);
   if (rc) return rc;
-  gemm_nn(w->X, n, w->Y, n, n, n, n, w->Z, n, 0, QBG_PREC_FP64, s);   // Z = L^-T Q
+  if (gemm_nn_splitk(w->X, n, w->Y, n, n, n, n, w->Z, n, w->W, w->wsize, s))   // Z = L^-T Q
+    return QBG_E_INVAL;
   gemm_nn(c, ldc2, w->Z, n, K, n, n, w->Tc, K, 0, QBG_PREC_FP64, s);  // c Z
   launch_copy_cols_if(c, ldc2, w->Tc, K, K, n, w->info, s);
   return QBG_OK;
@@ -178,6 +182,7 @@ int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
   if ((rc = a.get(c, &w->S, n2)) || (rc = a.get(c, &w->X, n2)) || (rc = a.get(c, &w->Tc, K * w->nst)) ||
       (rc = a.get(c, &w->W, w->wsize)) || (rc = a.get(c, &w->ekin, w->nst)) ||
       (rc = a.get(c, &w->dotp, w->nst)) || (rc = a.get(c, &w->scal, 4)) ||
+      (rc = a.get(c, &w->bpart, band_partials(w->nst))) ||
       (rc = a.get(c, &w->info, 1)) || (rc = a.get(c, &w->extrap, 1)) ||
       (kind != QBG_WF_SD && (rc = a.get(c, &w->A, n2))) ||
       (kind == QBG_WF_PSDA &&
@@ -240,7 +245,7 @@ int qbg_wfstep_update(qbg_wfstep* w, double* c, int ldc, double* dc, int ldd) {
   const long K = 2L * ngw, ldc2 = 2L * ldc, ldd2 = 2L * ldd;
   int rc = QBG_OK;
   ctx->prof_begin(QBG_K_WFSTEP);
-  launch_band_sums(c, ldc2, dc, ldd2, w->ef->d_g2w, ngw, n, w->ekin, w->dotp, s);
+  launch_band_sums(c, ldc2, dc, ldd2, w->ef->d_g2w, ngw, n, w->bpart, w->ekin, w->dotp, s);
   launch_sum(w->dotp, n, w->scal, s);
   if (w->kind == QBG_WF_SD) {
     launch_sd_update(c, ldc2, dc, ldd2, ngw, n, w->param, s);

@@ -189,15 +189,36 @@ def test_gram_fp64(ctx, oracle_mod, idx):
     assert np.abs(S - np.eye(s.nst)).max() < 1e-12
 
 
-def test_gram_not_spd(ctx):
+@pytest.mark.parametrize("nst", [129, 300])
+def test_gram_blocked_cholesky(ctx, oracle_mod, nst):
+    """More than two 64-column panels: the Cholesky runs panel by panel over
+    the device (diagonal block, panel solve, trailing GEMM)."""
     import qbg
-    s = cf.config(10)
+    import dataclasses
+    s = dataclasses.replace(cf.config(1), nst=nst)
+    o = oracle_mod.Oracle(cf.config(1))
+    b = o.basis(0)
+    c = cf.random_coefficients(88 + nst, b["g2"], nst)
+    ref = oracle_mod.gram(c)
+    dc = ctx.to_device(c)
+    qbg.gram(ctx, dc, o.ngw, o.ngw, nst)
+    got = dc.download()
+    assert relerr(got, ref) < 1e-10
+    P = got.view(np.float64).reshape(nst, -1)
+    S = 2 * P @ P.T - np.outer(P[:, 0], P[:, 0])
+    assert np.abs(S - np.eye(nst)).max() < 1e-12
+
+
+@pytest.mark.parametrize("nst", [4, 200])
+def test_gram_not_spd(ctx, nst):
+    import qbg
     rng = np.random.default_rng(0)
-    c = rng.standard_normal((4, 183)) + 1j * rng.standard_normal((4, 183))
-    c[3] = 0.0                        # zero band: singular overlap, pivot exactly 0
+    ngw = 183 if nst == 4 else 1500
+    c = rng.standard_normal((nst, ngw)) + 1j * rng.standard_normal((nst, ngw))
+    c[nst - 1] = 0.0                  # zero band: singular overlap, pivot exactly 0
     dc = ctx.to_device(c)
     with pytest.raises(qbg.QbgError) as e:
-        qbg.gram(ctx, dc, 183, 183, 4)
+        qbg.gram(ctx, dc, ngw, ngw, nst)
     assert e.value.code == -4         # QBG_E_NOT_SPD
     got = dc.download()
     c[:, 0] = c[:, 0].real

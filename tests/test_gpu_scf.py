@@ -45,12 +45,19 @@ def _hpsi(oracle_mod, o, c):
     return r["hpsi"]
 
 
-@pytest.mark.parametrize("idx,kind,param", [(11, "sd", 0.3), (11, "psd", 0.0), (12, "psd", 2.0),
-                                            (11, "psda", 0.0), (0, "psda", 0.0)])
-def test_stepper_update(ctx, oracle_mod, idx, kind, param):
+@pytest.mark.parametrize("idx,kind,param,nst", [(11, "sd", 0.3, None), (11, "psd", 0.0, None),
+                                                (12, "psd", 2.0, None), (11, "psda", 0.0, None),
+                                                (0, "psda", 0.0, None), (1, "psda", 0.0, 160),
+                                                (1, "psd", 0.0, 150)])
+def test_stepper_update(ctx, oracle_mod, idx, kind, param, nst):
+    """nst above 128 runs the device-wide blocked Cholesky inside gram and
+    ortho_align."""
+    import dataclasses
     import qbg
-    print(f"[case] stepper {kind} config {idx}", flush=True)
+    print(f"[case] stepper {kind} config {idx} nst {nst}", flush=True)
     s = cf.config(idx)
+    if nst is not None:
+        s = dataclasses.replace(s, nst=nst)
     ef = qbg.EnergyFunctional(ctx, s)
     o = oracle_mod.Oracle(s)
     b = ef.basis(0)
