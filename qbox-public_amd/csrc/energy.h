@@ -121,7 +121,11 @@ void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32,
 // workspace of 64 * n doubles (ws) the factorization runs blocked over the
 // whole device, otherwise in one work-group
 void potrf(double* S, int n, int lds, int* info, hipStream_t s, double* ws = nullptr);
-void trtri(const double* L, int n, int ldl, double* X, hipStream_t s);
+// the blocked factorization for any n (ws: 64 * n doubles); info = -1 and
+// nothing done when *skip
+void potrf_device(double* S, int n, int lds, int* info, double* ws, hipStream_t s,
+                  const int* skip);
+void trtri(const double* L, int n, int ldl, double* X, hipStream_t s, const int* skip = nullptr);
 void cleanup(double* c, long ldc, int nst, hipStream_t s);
 
 }  // namespace qbg

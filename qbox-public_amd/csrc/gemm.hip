@@ -70,7 +70,7 @@ struct GemmArgs {
 };
 
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+__global__ __launch_bounds__(256, 2) void k_gemm(GemmArgs g) {
   typedef Mfma<TC> MF;
   __shared__ TC Ps[KC][TP + PAD];
   __shared__ TC Qs[KC][TQ + PAD];
@@ -431,8 +431,10 @@ __global__ __launch_bounds__(1024) void k_potrf_blk(double* __restrict__ S, int 
 // info as for k_potrf; after a failure the remaining launches do nothing.
 constexpr int CB = 64;
 
-__global__ void k_fill_int0(int* p) {
-  if (threadIdx.x == 0) *p = 0;
+// info = 0, or -1 when *skip (every later launch of the factorization then
+// does nothing)
+__global__ void k_init_info(int* p, const int* skip) {
+  if (threadIdx.x == 0) *p = (skip && *skip) ? -1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ S, int lds, int j0,
@@ -493,8 +495,9 @@ __global__ __launch_bounds__(64) void k_chol_panel(double* __restrict__ S, int n
   }
 }
 
-static void potrf_blocked(double* S, int n, int lds, int* info, double* ws, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_int0, dim3(1), dim3(64), 0, s, info);
+static void potrf_blocked(double* S, int n, int lds, int* info, double* ws, hipStream_t s,
+                          const int* skip) {
+  hipLaunchKernelGGL(k_init_info, dim3(1), dim3(64), 0, s, info, skip);
   for (int j0 = 0; j0 < n; j0 += CB) {
     const int jb = std::min(CB, n - j0), rows = n - j0 - jb;
     hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, S, lds, j0, jb, info);
@@ -507,13 +510,20 @@ static void potrf_blocked(double* S, int n, int lds, int* info, double* ws, hipS
   }
 }
 
+void potrf_device(double* S, int n, int lds, int* info, double* ws, hipStream_t s,
+                  const int* skip) {
+  potrf_blocked(S, n, lds, info, ws, s, skip);
+}
+
 // X = L^{-1} (lower), written row-major (X[i*n + c], upper part zero):
 // block column J of TB columns per work-group, X_IJ = L_II^{-1} (E_IJ -
 // sum_{J <= K < I} L_IK X_KJ) for I = J, J+1, ... with 32 x 32 tiles staged
 // in LDS; one wave solves the diagonal-block system column by column.
 constexpr int TB = 32;
 __global__ __launch_bounds__(256) void k_trtri_blk(const double* __restrict__ L, int n, int ldl,
-                                                   double* __restrict__ X) {
+                                                   double* __restrict__ X,
+                                                   const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double R[TB][TB + 1];
   __shared__ double La[TB][TB + 1];
   __shared__ double Xb[TB][TB + 1];
@@ -573,7 +583,7 @@ __global__ __launch_bounds__(256) void k_trtri_blk(const double* __restrict__ L,
 
 void potrf(double* S, int n, int lds, int* info, hipStream_t s, double* ws) {
   if (ws && n > 2 * CB) {
-    potrf_blocked(S, n, lds, info, ws, s);
+    potrf_blocked(S, n, lds, info, ws, s, nullptr);
     return;
   }
   if (n <= POTRF_NMAX)
@@ -581,8 +591,8 @@ void potrf(double* S, int n, int lds, int* info, hipStream_t s, double* ws) {
   else
     hipLaunchKernelGGL(k_potrf, dim3(1), dim3(1024), 0, s, S, n, lds, info);
 }
-void trtri(const double* L, int n, int ldl, double* X, hipStream_t s) {
-  hipLaunchKernelGGL(k_trtri_blk, dim3((n + TB - 1) / TB), dim3(256), 0, s, L, n, ldl, X);
+void trtri(const double* L, int n, int ldl, double* X, hipStream_t s, const int* skip) {
+  hipLaunchKernelGGL(k_trtri_blk, dim3((n + TB - 1) / TB), dim3(256), 0, s, L, n, ldl, X, skip);
 }
 
 // zero Im c(G=0) of every band (SlaterDet::cleanup, src/SlaterDet.cpp:1335-1356)

@@ -42,7 +42,7 @@ void launch_copy_cols_if(double* dst, long lddst, const double* src, long ldsrc,
 
 // DoubleMatrix::polar pieces on an n x n column-major matrix.
 // flags: [0] done, [1] skip Bjorck-Bowie step, [2] skip Higham step,
-//        [3] iterations, [4] singular
+//        [3] iterations, [4] singular, [5] Cholesky status of a Higham step
 void launch_polar_init(int* flags, hipStream_t s);
 // q = q_in + I (q_in = -x^T x), its Frobenius norm; sets flags for this step
 void launch_polar_norm(double* q, int n, double tol, int* flags, double* qnrm, hipStream_t s);
@@ -52,10 +52,13 @@ void launch_ident_axpy(double* t, const double* q, int n, double a, const int* s
 // dst = src (n x n), skipped when *skip
 void launch_copy_if_not(double* dst, const double* src, long count, const int* skip,
                         hipStream_t s);
-// Higham step x <- (x + x^-T)/2 by Gauss-Jordan in one work-group;
-// aug holds 2 n^2 doubles
+// Higham step helpers (x^-T = x (x^T x)^-1 through the blocked Cholesky):
+// dst = transpose of src (n x n), x = (x + z) / 2, flags[4] = 1 if *info > 0;
+// the first two skipped when *skip
 constexpr int POLAR_NMAX = 4096;
-void launch_polar_higham(double* x, int n, double* aug, int* flags, hipStream_t s);
+void launch_transpose_if(double* dst, const double* src, int n, const int* skip, hipStream_t s);
+void launch_higham_update(double* x, const double* z, int n, const int* skip, hipStream_t s);
+void launch_polar_singular(const int* info, int* flags, hipStream_t s);
 
 // out[4] = eigenvalue sum, theta, polar iterations, orthogonalization status
 void launch_wf_results(const double* scal, const int* flags, const int* info, double* out,

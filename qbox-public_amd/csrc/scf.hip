@@ -375,79 +375,49 @@ void launch_copy_if_not(double* dst, const double* src, long count, const int* s
                      src, count, skip);
 }
 
-// x <- (x + x^-T) / 2 (src/Matrix.cpp:2500-2513).  (x^T)^-1 by Gauss-Jordan
-// elimination with partial pivoting on [x^T | I] (column major, n x 2n, in
-// aug), one work-group; the branch is taken only far from orthogonality.
-
-__global__ __launch_bounds__(1024) void k_polar_higham(double* __restrict__ x, int n,
-                                                       double* __restrict__ aug,
-                                                       int* __restrict__ flags) {
-  if (flags[2]) return;
-  __shared__ double colk[POLAR_NMAX];
-  __shared__ double rv[1024];
-  __shared__ int ri[1024];
-  const int tid = threadIdx.x;
-  const long nn = (long)n * n;
-  for (long i = tid; i < nn; i += 1024) {
-    const long col = i / n, row = i - col * n;
-    aug[i] = x[col + row * n];                           // x^T
-    aug[nn + i] = row == col ? 1.0 : 0.0;                // I
-  }
+// pieces of the Higham step x <- (x + x^-T) / 2 (src/Matrix.cpp:2500-2513),
+// with x^-T = x (x^T x)^-1 from the device-wide Cholesky of x^T x:
+// g = transpose of a row-major lower inverse (so g^T g = (x^T x)^-1),
+// x = (x + z) / 2, and the singular flag from the factorization
+__global__ __launch_bounds__(256) void k_transpose_if(double* __restrict__ dst,
+                                                      const double* __restrict__ src, int n,
+                                                      const int* __restrict__ skip) {
+  if (*skip) return;
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int j = ty; j < 32; j += 8)
+    if (by + j < n && bx + tx < n) tile[j][tx] = src[(long)(by + j) * n + bx + tx];
   __syncthreads();
-  for (int k = 0; k < n; k++) {
-    // pivot: largest |aug(i, k)|, i >= k (first index on ties)
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + tid; i < n; i += 1024) {
-      const double v = fabs(aug[i + (long)k * n]);
-      if (v > best) { best = v; bi = i; }
-    }
-    rv[tid] = best;
-    ri[tid] = bi;
-    __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
-      if (tid < off) {
-        const double o = rv[tid + off];
-        const int oi = ri[tid + off];
-        if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
-      }
-      __syncthreads();
-    }
-    const int p = ri[0];
-    if (!(rv[0] > 0.0)) {
-      if (tid == 0) flags[4] = 1;
-      return;                                            // uniform: every thread read rv[0]
-    }
-    // swap rows p and k over columns k .. 2n-1 (columns < k are unit vectors
-    // with zeros in both rows)
-    if (p != k)
-      for (int j = k + tid; j < 2 * n; j += 1024) {
-        const double a = aug[k + (long)j * n];
-        aug[k + (long)j * n] = aug[p + (long)j * n];
-        aug[p + (long)j * n] = a;
-      }
-    __syncthreads();
-    const double dinv = 1.0 / aug[k + (long)k * n];
-    __syncthreads();
-    for (int j = k + tid; j < 2 * n; j += 1024) aug[k + (long)j * n] *= dinv;
-    __syncthreads();
-    for (int i = tid; i < n; i += 1024) colk[i] = aug[i + (long)k * n];
-    __syncthreads();
-    // eliminate column k from every other row, columns k .. 2n-1
-    const long cnt = (long)n * (2 * n - k);
-    for (long e = tid; e < cnt; e += 1024) {
-      const long jj = e / n;
-      const int i = (int)(e - jj * n);
-      const long j = k + jj;
-      if (i != k) aug[i + j * n] -= colk[i] * aug[k + j * n];
-    }
-    __syncthreads();
-  }
-  for (long i = tid; i < nn; i += 1024) x[i] = 0.5 * (x[i] + aug[nn + i]);
+  for (int j = ty; j < 32; j += 8)
+    if (bx + j < n && by + tx < n) dst[(long)(bx + j) * n + by + tx] = tile[tx][j];
 }
 
-void launch_polar_higham(double* x, int n, double* aug, int* flags, hipStream_t s) {
-  hipLaunchKernelGGL(k_polar_higham, dim3(1), dim3(1024), 0, s, x, n, aug, flags);
+void launch_transpose_if(double* dst, const double* src, int n, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose_if, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, dst, src,
+                     n, skip);
+}
+
+__global__ __launch_bounds__(256) void k_higham_update(double* __restrict__ x,
+                                                       const double* __restrict__ z, long count,
+                                                       const int* __restrict__ skip) {
+  if (*skip) return;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < count) x[i] = 0.5 * (x[i] + z[i]);
+}
+
+void launch_higham_update(double* x, const double* z, int n, const int* skip, hipStream_t s) {
+  const long nn = (long)n * n;
+  hipLaunchKernelGGL(k_higham_update, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, x, z,
+                     nn, skip);
+}
+
+__global__ void k_polar_singular(const int* __restrict__ info, int* __restrict__ flags) {
+  if (threadIdx.x == 0 && *info > 0) flags[4] = 1;
+}
+
+void launch_polar_singular(const int* info, int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_polar_singular, dim3(1), dim3(64), 0, s, info, flags);
 }
 
 // out[0..3] = eigenvalue sum, theta, polar iterations, orthogonalization

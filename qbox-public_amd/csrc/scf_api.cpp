@@ -69,8 +69,22 @@ int polar_enqueue(const PolarWs& w, double* Y, int n, double tol, int maxiter, d
     if (gemm_nn_splitk(Y, n, w.T, n, n, n, n, w.XP, n, W, wsize, s, w.flags + 1))
       return QBG_E_INVAL;
     launch_copy_if_not(Y, w.XP, n2, w.flags + 1, s);
-    // Higham: Y <- (Y + Y^-T) / 2
-    launch_polar_higham(Y, n, w.aug, w.flags, s);
+    // Higham: Y <- (Y + Y^-T) / 2, Y^-T = Y (Y^T Y)^-1 with Y^T Y = I - q
+    // (reusing the Bjorck-Bowie buffers, idle in a Higham step)
+    double* M = w.T;
+    double* Linv = w.QT;
+    double* G = w.XP;
+    double* P = w.aug;
+    double* Z = w.aug + n2;
+    launch_ident_axpy(M, w.Q, n, -1.0, w.flags + 2, s);
+    potrf_device(M, n, n, w.flags + 5, W, s, w.flags + 2);
+    trtri(M, n, n, Linv, s, w.flags + 2);
+    launch_transpose_if(G, Linv, n, w.flags + 2, s);
+    if (gemm_syrk(G, n, n, n, 1.0, 0.0, P, n, W, wsize, s, w.flags + 2)) return QBG_E_INVAL;
+    if (gemm_nn_splitk(Y, n, P, n, n, n, n, Z, n, W, wsize, s, w.flags + 2))
+      return QBG_E_INVAL;
+    launch_higham_update(Y, Z, n, w.flags + 2, s);
+    launch_polar_singular(w.flags + 5, w.flags, s);
   }
   return QBG_OK;
 }
