@@ -144,12 +144,21 @@ int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc,
 int qbg_gram(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst, int prec);
 
 /* ---------------------------------------------------- EnergyFunctional */
-/* A norm-conserving semilocal species as Qbox reads it
- * (norm_conserving_semilocal_pseudopotential, src/SpeciesHandler.cpp:200-241)
- * with one projector per l = 0..lmax (lmax <= 1): radial local potential
- * vloc[nmesh] and projectors proj[(lmax+1)*nmesh] on the mesh i*deltar,
- * D_l in dl[lmax+1]; rcps is the pseudocharge radius (1.5 in Qbox,
- * src/AtomSet.cpp:43-44).  Host arrays. */
+/* A species as Qbox holds it, on the radial mesh i*deltar (host arrays);
+ * rcps is the pseudocharge radius (1.5 in Qbox, src/AtomSet.cpp:43-44).
+ *   type QBG_SPECIES_SLPP: norm_conserving_semilocal_pseudopotential
+ *     (src/SpeciesHandler.cpp:200-241, Species::initialize_slpp) with one
+ *     projector per l = 0..lmax (lmax <= 1): local potential vloc[nmesh],
+ *     projectors proj[(lmax+1)*nmesh], D_l in dl[lmax+1];
+ *   type QBG_SPECIES_NCPP: norm_conserving_pseudopotential in the
+ *     Kleinman-Bylander form (nquad = 0; Species::initialize_ncpp,
+ *     src/Species.cpp:71-507): semilocal potentials vps[(lmax+1)*nmesh]
+ *     and pseudo-wavefunctions phi[(lmax+1)*nmesh] (the row of the local
+ *     channel llocal may hold anything); lmax <= 3, every nonlocal channel
+ *     l != llocal must have l <= 1.
+ * Fields after dl were added for NCPP; a zeroed tail selects SLPP. */
+#define QBG_SPECIES_SLPP 0
+#define QBG_SPECIES_NCPP 1
 typedef struct {
   int zval;
   int lmax;
@@ -159,6 +168,10 @@ typedef struct {
   const double* vloc;
   const double* proj;
   const double* dl;
+  int type;
+  int llocal;
+  const double* vps;
+  const double* phi;
 } qbg_species_desc;
 
 /* EnergyFunctional(Sample&, ChargeDensity&) for one Gamma-point system

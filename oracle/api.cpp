@@ -18,6 +18,10 @@ typedef struct {
   const double* vloc;   // nmesh
   const double* proj;   // (lmax+1) * nmesh
   const double* dl;     // lmax+1
+  int type;             // 0 SLPP, 1 NCPP (vps, phi, llocal)
+  int llocal;
+  const double* vps;    // (lmax+1) * nmesh
+  const double* phi;    // (lmax+1) * nmesh
 } qbo_species_desc;
 
 static std::string g_err;
@@ -36,12 +40,16 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
       Species q;
       q.zval = sp[is].zval; q.lmax = sp[is].lmax; q.deltar = sp[is].deltar;
       q.rcps = sp[is].rcps;
-      q.vlocal.assign(sp[is].vloc, sp[is].vloc + sp[is].nmesh);
-      q.proj.resize(q.lmax + 1);
-      for (int l = 0; l <= q.lmax; l++)
-        q.proj[l].assign(sp[is].proj + (size_t)l * sp[is].nmesh,
-                         sp[is].proj + (size_t)(l + 1) * sp[is].nmesh);
-      q.dl.assign(sp[is].dl, sp[is].dl + q.lmax + 1);
+      if (sp[is].type == 1) {
+        ncpp_species(q, sp[is].lmax, sp[is].llocal, sp[is].nmesh, sp[is].vps, sp[is].phi);
+      } else {
+        q.vlocal.assign(sp[is].vloc, sp[is].vloc + sp[is].nmesh);
+        q.proj.resize(q.lmax + 1);
+        for (int l = 0; l <= q.lmax; l++)
+          q.proj[l].assign(sp[is].proj + (size_t)l * sp[is].nmesh,
+                           sp[is].proj + (size_t)(l + 1) * sp[is].nmesh);
+        q.dl.assign(sp[is].dl, sp[is].dl + q.lmax + 1);
+      }
       s->sp.push_back(q);
       s->tau.emplace_back(tau + 3 * off, tau + 3 * (off + na[is]));
       off += na[is];
@@ -57,6 +65,25 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
 }
 
 void qbo_destroy(void* h) { delete (System*)h; }
+
+// the Kleinman-Bylander form of an NCPP species: proj[(lkb+1)*nmesh], dl[lkb+1]
+// (room for lmax+1 rows); returns lkb or -1
+int qbo_ncpp_kb(int lmax, int llocal, int nmesh, double deltar, const double* vps,
+                const double* phi, double* proj, double* dl) {
+  try {
+    Species q;
+    q.deltar = deltar;
+    ncpp_species(q, lmax, llocal, nmesh, vps, phi);
+    for (int l = 0; l <= q.lmax; l++) {
+      std::memcpy(proj + (size_t)l * nmesh, q.proj[l].data(), 8 * (size_t)nmesh);
+      dl[l] = q.dl[l];
+    }
+    return q.lmax;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
 
 // out[0..]: ngw, ngv, np0, np1, np2, nvec_w, ntrans0_w, nvec_v, ntrans0_v,
 //           nrods_w, nrods_v, ndft(species 0)

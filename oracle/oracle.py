@@ -22,7 +22,23 @@ _ip = C.POINTER(C.c_int)
 class SpeciesDesc(C.Structure):
     _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
                 ("deltar", C.c_double), ("rcps", C.c_double),
-                ("vloc", _dp), ("proj", _dp), ("dl", _dp)]
+                ("vloc", _dp), ("proj", _dp), ("dl", _dp),
+                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp)]
+
+
+def species_desc(sp, keep):
+    """SpeciesDesc of a configs.SpeciesDef (SLPP form) or an NCPP species
+    (semilocal potentials vps, pseudo-wavefunctions phi, llocal)."""
+    if getattr(sp, "kind", "") == "ncpp":
+        vps = np.ascontiguousarray(sp.vps, dtype=float)
+        phi = np.ascontiguousarray(sp.phi, dtype=float)
+        keep += [vps, phi]
+        return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
+                           1, sp.llocal, _ptr(vps), _ptr(phi))
+    vloc, proj, dl = sp.radial()
+    keep += [vloc, proj, dl]
+    return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
+                       _ptr(vloc), _ptr(proj), _ptr(dl), 0, 0, None, None)
 
 
 def _ptr(a, t=_dp):
@@ -42,6 +58,7 @@ def lib():
         _lib.qbo_create.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
                                     _ip, _dp, C.c_int, _dp]
         _lib.qbo_last_error.restype = C.c_char_p
+        _lib.qbo_ncpp_kb.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         for name in ["qbo_destroy", "qbo_sizes", "qbo_basis", "qbo_rods", "qbo_fft_backward",
                      "qbo_fft_backward_pair", "qbo_fft_forward", "qbo_fft_forward_pair",
                      "qbo_compute_density", "qbo_rs_mul_add", "qbo_update_density",
@@ -90,10 +107,7 @@ class Oracle:
         self._keep = []
         descs = (SpeciesDesc * len(sysdef.species))()
         for i, sp in enumerate(sysdef.species):
-            vloc, proj, dl = sp.radial()
-            self._keep += [vloc, proj, dl]
-            descs[i] = SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
-                                   _ptr(vloc), _ptr(proj), _ptr(dl))
+            descs[i] = species_desc(sp, self._keep)
         cell = np.ascontiguousarray(sysdef.cell, dtype=float).ravel()
         na = np.array(sysdef.na, dtype=np.int32)
         tau = np.ascontiguousarray(sysdef.tau, dtype=float).ravel()
@@ -247,6 +261,20 @@ class Oracle:
         vn, dvn = np.zeros(nop * n), np.zeros(nop * n)
         lib().qbo_species_eval(self.h, isp, n, _ptr(g), _ptr(v), _ptr(dv), _ptr(vn), _ptr(dvn))
         return v, dv, vn.reshape(nop, n), dvn.reshape(nop, n)
+
+
+def ncpp_kb(sp):
+    """(projectors (lkb+1, nmesh), D_l) of an NCPP species in the
+    one-projector-per-l form (Species::initialize_ncpp, nquad = 0)."""
+    vps = np.ascontiguousarray(sp.vps, dtype=float)
+    phi = np.ascontiguousarray(sp.phi, dtype=float)
+    proj = np.zeros((sp.lmax + 1) * sp.nmesh)
+    dl = np.zeros(sp.lmax + 1)
+    lkb = lib().qbo_ncpp_kb(sp.lmax, sp.llocal, sp.nmesh, sp.deltar, _ptr(vps), _ptr(phi),
+                            _ptr(proj), _ptr(dl))
+    if lkb < 0:
+        raise RuntimeError(lib().qbo_last_error().decode())
+    return proj[:(lkb + 1) * sp.nmesh].reshape(lkb + 1, sp.nmesh), dl[:lkb + 1]
 
 
 def compute_density_occ(o, c, occ):

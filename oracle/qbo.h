@@ -102,6 +102,11 @@ void splint(int n, const double* xa, const double* ya, const double* y2a,
 void splintd(int n, const double* xa, const double* ya, const double* y2a,
              double x, double* y, double* dy);             // spline.cpp:138-168
 
+// Kleinman-Bylander species from semilocal potentials and pseudo-wavefunctions
+// (src/Species.cpp:71-330); fills vlocal, proj, dl, lmax of q (deltar set)
+void ncpp_species(Species& q, int lmax, int llocal, int nmesh, const double* vps,
+                  const double* phi);
+
 // LDA (Perdew-Zunger/Ceperley-Alder), src/LDAFunctional.cpp:92-151
 void lda_unpolarized(double rh, double& ee, double& vv);
 

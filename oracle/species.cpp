@@ -276,4 +276,42 @@ void lda_unpolarized(double rh, double& ee, double& vv) {
   }
 }
 
+// Species::initialize_ncpp, Kleinman-Bylander branch (src/Species.cpp:71-330,
+// nquad = 0), restated as the equivalent one-projector-per-l species: local
+// potential vps[llocal]; nonlocal l: projector dv_l phi_l with D_l = w_l,
+// w_l = 1 / (deltar * simpsn(dv_l (phi_l r)^2)), dv_l = vps_l - vps_llocal,
+// integrated on the 40-bohr transform mesh (zero beyond the input mesh).
+void ncpp_species(Species& q, int lmax, int llocal, int nmesh, const double* vps,
+                  const double* phi) {
+  if (lmax < 0 || lmax > 3 || llocal < 0 || llocal > lmax)
+    throw std::runtime_error("ncpp: bad lmax/llocal");
+  int ndft = 1;
+  while (ndft * q.deltar < 40.0) ndft *= 2;
+  int lkb = 0;
+  for (int l = 0; l <= lmax; l++)
+    if (l != llocal) {
+      if (l > 1) throw std::runtime_error("ncpp: nonlocal channel with l > 1");
+      lkb = std::max(lkb, l);
+    }
+  const double* vl = vps + (size_t)llocal * nmesh;
+  q.vlocal.assign(vl, vl + nmesh);
+  q.lmax = lkb;
+  q.proj.assign(lkb + 1, std::vector<double>(nmesh, 0.0));
+  q.dl.assign(lkb + 1, 0.0);
+  for (int l = 0; l <= lkb; l++) {
+    if (l == llocal) continue;
+    const double* v = vps + (size_t)l * nmesh;
+    const double* f = phi + (size_t)l * nmesh;
+    std::vector<double> w(ndft, 0.0);
+    for (int i = 0; i < nmesh && i < ndft; i++) {
+      const double fr = f[i] * (q.deltar * i);
+      w[i] = (v[i] - vl[i]) * fr * fr;
+    }
+    const double integral = simpsn(ndft, w.data());
+    if (integral == 0.0) throw std::runtime_error("ncpp: zero projector norm");
+    q.dl[l] = 1.0 / (q.deltar * integral);
+    for (int i = 0; i < nmesh; i++) q.proj[l][i] = (v[i] - vl[i]) * f[i];
+  }
+}
+
 }  // namespace qbo

@@ -158,12 +158,27 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
   e->sp_first.assign(1, 0);
   for (int is = 0; is < nsp; is++) {
     host::Species q;
-    q.zval = spd[is].zval; q.lmax = spd[is].lmax; q.nmesh = spd[is].nmesh;
-    q.deltar = spd[is].deltar; q.rcps = spd[is].rcps;
-    q.vloc.assign(spd[is].vloc, spd[is].vloc + q.nmesh);
-    q.proj.assign(spd[is].proj, spd[is].proj + (size_t)(q.lmax + 1) * q.nmesh);
-    q.dl.assign(spd[is].dl, spd[is].dl + q.lmax + 1);
-    if (!q.init()) { delete e; return c->fail(QBG_E_INVAL, "species: unsupported definition"); }
+    const qbg_species_desc& d = spd[is];
+    q.zval = d.zval; q.nmesh = d.nmesh; q.deltar = d.deltar; q.rcps = d.rcps;
+    bool ok;
+    if (d.type == QBG_SPECIES_NCPP) {
+      ok = d.vps && d.phi && d.nmesh > 0 &&
+           host::ncpp_to_kb(d.lmax, d.llocal, d.nmesh, d.deltar, d.vps, d.phi, q.vloc, q.proj,
+                            q.dl, q.lmax);
+    } else {
+      ok = d.type == QBG_SPECIES_SLPP && d.vloc && d.proj && d.dl && d.lmax >= 0 &&
+           d.lmax <= 1 && d.nmesh > 0;
+      if (ok) {
+        q.lmax = d.lmax;
+        q.vloc.assign(d.vloc, d.vloc + q.nmesh);
+        q.proj.assign(d.proj, d.proj + (size_t)(q.lmax + 1) * q.nmesh);
+        q.dl.assign(d.dl, d.dl + q.lmax + 1);
+      }
+    }
+    if (!ok || !q.init()) {
+      delete e;
+      return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 1?)");
+    }
     e->sp.push_back(q);
     e->na.push_back(na[is]);
     e->sp_first.push_back(e->sp_first.back() + na[is]);

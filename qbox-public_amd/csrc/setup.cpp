@@ -259,6 +259,43 @@ static void splintd(int n, const double* xa, const double* ya, const double* y2a
   dy = (ya[kh] - ya[kl]) / h + h * (((1.0 / 6.0) - 0.5 * a * a) * y2a[kl] + (0.5 * b * b - (1.0 / 6.0)) * y2a[kh]);
 }
 
+bool ncpp_to_kb(int lmax, int llocal, int nmesh, double deltar, const double* vps,
+                const double* phi, std::vector<double>& vloc, std::vector<double>& proj,
+                std::vector<double>& dl, int& lmax_kb) {
+  if (lmax < 0 || lmax > 3 || llocal < 0 || llocal > lmax || nmesh < 8 || !(deltar > 0.0))
+    return false;
+  int ndft = 1;
+  while (ndft * deltar < 40.0) ndft *= 2;
+  const double* vl = vps + (size_t)llocal * nmesh;
+  lmax_kb = 0;
+  for (int l = 0; l <= lmax; l++)
+    if (l != llocal) {
+      if (l > 1) return false;
+      lmax_kb = std::max(lmax_kb, l);
+    }
+  vloc.assign(vl, vl + nmesh);
+  proj.assign((size_t)(lmax_kb + 1) * nmesh, 0.0);
+  dl.assign(lmax_kb + 1, 0.0);
+  std::vector<double> fint(ndft, 0.0);
+  for (int l = 0; l <= lmax_kb; l++) {
+    if (l == llocal) continue;
+    const double* v = vps + (size_t)l * nmesh;
+    const double* f = phi + (size_t)l * nmesh;
+    // weight 1 / (deltar * simpson((v_l - v_loc) (phi_l r)^2)); both vanish
+    // beyond the mesh (src/Species.cpp:272-289)
+    std::fill(fint.begin(), fint.end(), 0.0);
+    for (int i = 0; i < std::min(nmesh, ndft); i++) {
+      const double t = f[i] * (i * deltar);
+      fint[i] = (v[i] - vl[i]) * t * t;
+    }
+    const double tmp = simpsn(ndft, fint.data());
+    if (!(tmp != 0.0)) return false;
+    dl[l] = 1.0 / (deltar * tmp);
+    for (int i = 0; i < nmesh; i++) proj[(size_t)l * nmesh + i] = (v[i] - vl[i]) * f[i];
+  }
+  return true;
+}
+
 bool Species::init() {
   if (lmax < 0 || lmax > 1 || nmesh < 8 || !(deltar > 0.0)) return false;
   const double fpi = 4.0 * M_PI;

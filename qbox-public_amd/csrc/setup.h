@@ -50,5 +50,16 @@ struct Species {
   double eself() const;
 };
 
+// Kleinman-Bylander form of a norm-conserving species given by semilocal
+// potentials vps[l] and pseudo-wavefunctions phi[l] (src/Species.cpp:71-507,
+// nquad = 0): local potential vps[llocal]; for each nonlocal l the projector
+// (vps[l] - vps[llocal]) phi[l] with weight 1 / <phi_l|vps_l - vps_llocal|phi_l>
+// (Simpson on the 40-bohr transform mesh).  Output in the one-projector-per-l
+// form of Species (lmax_kb = largest nonlocal l, channels without a projector
+// zero); false if a nonlocal channel has l > 1 or a weight is undefined.
+bool ncpp_to_kb(int lmax, int llocal, int nmesh, double deltar, const double* vps,
+                const double* phi, std::vector<double>& vloc, std::vector<double>& proj,
+                std::vector<double>& dl, int& lmax_kb);
+
 }  // namespace host
 }  // namespace qbg

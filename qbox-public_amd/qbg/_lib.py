@@ -97,7 +97,27 @@ class SpeciesDesc(C.Structure):
     """qbg_species_desc (include/qbg.h)."""
     _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
                 ("deltar", C.c_double), ("rcps", C.c_double),
-                ("vloc", _dp), ("proj", _dp), ("dl", _dp)]
+                ("vloc", _dp), ("proj", _dp), ("dl", _dp),
+                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp)]
+
+
+def species_desc(sp, keep):
+    """qbg_species_desc of a configs.SpeciesDef (semilocal form with explicit
+    projectors) or of a species.NcppSpecies (semilocal potentials and
+    pseudo-wavefunctions, turned into Kleinman-Bylander projectors by the
+    library)."""
+    if getattr(sp, "kind", "") == "ncpp":
+        if sp.nquad != 0:
+            raise ValueError("semilocal quadrature species (nquad > 0) are not supported")
+        vps = np.ascontiguousarray(sp.vps, dtype=np.float64)
+        phi = np.ascontiguousarray(sp.phi, dtype=np.float64)
+        keep += [vps, phi]
+        return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
+                           1, sp.llocal, _pd(vps), _pd(phi))
+    vloc, proj, dl = sp.radial()
+    keep += [vloc, proj, dl]
+    return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, _pd(vloc), _pd(proj),
+                       _pd(dl), 0, 0, None, None)
 
 _lib = None
 
@@ -363,10 +383,7 @@ class EnergyFunctional:
         descs = (SpeciesDesc * len(sysdef.species))()
         self._keep = [descs]
         for i, sp in enumerate(sysdef.species):
-            vloc, proj, dl = sp.radial()
-            self._keep += [vloc, proj, dl]
-            descs[i] = SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
-                                   _pd(vloc), _pd(proj), _pd(dl))
+            descs[i] = species_desc(sp, self._keep)
         cell = np.ascontiguousarray(sysdef.cell, dtype=np.float64).ravel()
         na = np.ascontiguousarray(sysdef.na, dtype=np.int32)
         tau = np.ascontiguousarray(sysdef.tau, dtype=np.float64).ravel()

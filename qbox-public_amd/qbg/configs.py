@@ -189,3 +189,20 @@ def gram_numpy(c: np.ndarray) -> np.ndarray:
     L = np.linalg.cholesky(S)
     X = np.linalg.solve(L, P)                             # L X = P  -> rows
     return np.ascontiguousarray(X).view(np.complex128).reshape(c.shape)
+
+
+def si4gs(si) -> SystemDef:
+    """The system of test/si4gs/test.i: four Si atoms in a 16 bohr cube,
+    ecut 6 Ry, 8 bands; ``si`` is the species read from Si_VBC_LDA-1.0.xml."""
+    tau = np.array([[3.7, 0.0, 0.0], [0.0, 2.2, 0.0], [-3.7, 0.0, 0.0], [0.0, -2.2, 0.0]])
+    return SystemDef("si4gs: Si4 L=16 ecut=6Ry 8 bands", cubic(16.0), 6.0, [si], [4], 8,
+                     0, 0.0, tau=tau)
+
+
+def h2ogs(o, h, ecut_ry=70.0) -> SystemDef:
+    """The system of test/h2ogs/test.i: one water molecule in a 16 bohr cube,
+    4 bands (ecut 70 Ry there); species from O_HSCV_PBE-1.0.xml and
+    H_HSCV_PBE-1.0.xml (evaluated here with the LDA functional)."""
+    tau = np.array([[0.0, 0.0, 0.0], [1.128, 1.444, 0.0], [1.128, -1.444, 0.0]])
+    return SystemDef(f"h2ogs: H2O L=16 ecut={ecut_ry:g}Ry 4 bands", cubic(16.0), ecut_ry, [o, h],
+                     [1, 2], 4, 0, 0.0, tau=tau)

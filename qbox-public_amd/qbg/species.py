@@ -1,0 +1,107 @@
+"""Qbox species files (the fpmd ``species`` XML of test/potentials/*.xml,
+read by src/SpeciesReader.cpp and src/SpeciesHandler.cpp) as inputs of the
+energy functional.
+
+``read_species(path)`` returns an :class:`NcppSpecies` for a
+``norm_conserving_pseudopotential`` (semilocal potentials and
+pseudo-wavefunctions per l, Kleinman-Bylander form when nquad = 0); the
+library turns it into projectors itself (Species::initialize_ncpp,
+src/Species.cpp:71-507).  ``rcps`` is 1.5 bohr as in src/AtomSet.cpp:43-44.
+"""
+from __future__ import annotations
+
+import xml.etree.ElementTree as ET
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RCPS = 1.5
+
+
+@dataclass
+class NcppSpecies:
+    symbol: str
+    zval: int
+    lmax: int
+    llocal: int
+    deltar: float
+    vps: np.ndarray            # (lmax+1, nmesh)
+    phi: np.ndarray            # (lmax+1, nmesh); zero rows where absent
+    nquad: int = 0
+    rquad: float = 0.0
+    mass: float = 0.0
+    atomic_number: int = 0
+    rcps: float = RCPS
+    description: str = field(default="", repr=False)
+
+    kind = "ncpp"
+
+    @property
+    def nmesh(self) -> int:
+        return self.vps.shape[1]
+
+
+def _local(tag):
+    return tag.rsplit("}", 1)[-1]
+
+
+def _numbers(text):
+    return np.array([float(x) for x in text.split()], dtype=np.float64)
+
+
+def parse_species(text: str) -> NcppSpecies:
+    """Parse the XML text of a species file (fpmd species, version 1.0)."""
+    root = ET.fromstring(text)
+    vals = {}
+    ncpp = None
+    for el in root.iter():
+        name = _local(el.tag)
+        if name in ("symbol", "atomic_number", "mass", "description"):
+            vals[name] = (el.text or "").strip()
+        if name == "norm_conserving_pseudopotential":
+            ncpp = el
+        if name == "norm_conserving_semilocal_pseudopotential":
+            raise ValueError("semilocal (SLPP) species files are given as configs.SpeciesDef")
+    if ncpp is None:
+        raise ValueError("no norm_conserving_pseudopotential element")
+    p = {}
+    proj = {}
+    for el in ncpp:
+        name = _local(el.tag)
+        if name == "projector":
+            l = int(el.attrib["l"])
+            size = int(el.attrib["size"])
+            d = {"size": size}
+            for sub in el:
+                d[_local(sub.tag)] = _numbers(sub.text or "")
+            proj[l] = d
+        else:
+            p[name] = (el.text or "").strip()
+    lmax = int(p["lmax"])
+    llocal = int(p["llocal"])
+    nquad = int(p.get("nquad", "0"))
+    sizes = {d["size"] for d in proj.values()}
+    if len(sizes) != 1:
+        raise ValueError("projectors of different sizes")
+    n = sizes.pop()
+    vps = np.zeros((lmax + 1, n))
+    phi = np.zeros((lmax + 1, n))
+    for l in range(lmax + 1):
+        if l not in proj:
+            raise ValueError(f"missing projector l={l}")
+        vps[l] = proj[l]["radial_potential"][:n]
+        if "radial_function" in proj[l]:
+            phi[l] = proj[l]["radial_function"][:n]
+        elif l != llocal and nquad == 0:
+            raise ValueError(f"projector l={l} has no radial_function")
+    return NcppSpecies(symbol=vals.get("symbol", ""), zval=int(p["valence_charge"]),
+                       lmax=lmax, llocal=llocal, deltar=float(p["mesh_spacing"]), vps=vps,
+                       phi=phi, nquad=nquad, rquad=float(p.get("rquad", "0")),
+                       mass=float(vals.get("mass", "0") or 0),
+                       atomic_number=int(vals.get("atomic_number", "0") or 0),
+                       description=vals.get("description", ""))
+
+
+def read_species(path) -> NcppSpecies:
+    with open(path, "r") as f:
+        return parse_species(f.read())

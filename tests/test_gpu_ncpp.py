@@ -1,0 +1,96 @@
+"""Norm-conserving species from the reference's test potentials on the
+device: the systems of test/si4gs and test/h2ogs evaluated against the FP64
+oracle (energies, H*psi, forces to 1e-10; the mixed modes to the 1e-6
+Ha/atom bar), the si4gs SCF run (PSDA, ecutprec 2 Ry, 200 iterations)
+reaching the oracle's minimum, and species the library does not take."""
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from qbg import configs as cf
+from qbg.species import read_species
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+TERMS = ["ekin", "eps", "enl", "ehart", "exc", "esr", "eself", "etotal", "total_charge"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import qbg
+    c = qbg.Context(0)
+    yield c
+    c.close()
+
+
+def relerr(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+def systems():
+    si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
+    o = read_species(GOLD / "O_HSCV_PBE-1.0.xml")
+    h = read_species(GOLD / "H_HSCV_PBE-1.0.xml")
+    return {"si4gs": cf.si4gs(si), "h2ogs": cf.h2ogs(o, h, ecut_ry=70.0)}
+
+
+@pytest.mark.parametrize("name", ["si4gs", "h2ogs"])
+@pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
+def test_evaluate(ctx, oracle_mod, name, prec):
+    import qbg
+    s = systems()[name]
+    print(f"[case] {name} {prec}", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(11, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        if prec == "fp64":
+            assert abs(got[t] - ref[t]) <= 1e-10 * max(abs(ref[t]), 1e-12), (t, got[t], ref[t])
+        else:
+            assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    ftol, htol = (1e-10, 1e-10) if prec == "fp64" else (1e-5, 1e-5)
+    assert np.abs(got["fion"] - ref["fion"]).max() <= ftol * max(1.0, np.abs(ref["fion"]).max())
+    assert relerr(hp.download(), ref["hpsi"]) < htol
+    stol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.abs(got["sigma"] - ref["sigma"]).max() < stol * np.abs(ref["sigma"][0]).max()
+    ef.release()
+
+
+def test_si4gs_scf(ctx, oracle_mod):
+    """test/si4gs/test.i: PSDA, ecutprec 2 Ry, run 0 200, from random
+    coefficients; the oracle's loop ends at -15.3729448 Ha."""
+    import qbg
+    from qbg.scf import ScfLoop
+    s = systems()["si4gs"]
+    print("[case] si4gs scf", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    b = ef.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(4, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    dh = ctx.array(c.shape, np.complex128)
+    loop = ScfLoop(ef, dc, dh, wf_dyn="psda", param=1.0, graphs=True)
+    hist = loop.run(200, 0)
+    e = [h["etotal"] for h in hist]
+    assert abs(e[-1] - e[-2]) < 1e-7
+    assert abs(e[-1] - (-15.3729448)) < 1e-6, e[-1]
+    loop.release()
+    ef.release()
+
+
+def test_nonlocal_d_channel_refused(ctx):
+    """A d projector (nonlocal l = 2) is outside what the library builds."""
+    import qbg
+    si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
+    bad = dataclasses.replace(si, llocal=0)
+    s = cf.si4gs(bad)
+    with pytest.raises(qbg.QbgError) as e:
+        qbg.EnergyFunctional(ctx, s)
+    assert e.value.code == -1

@@ -1,0 +1,117 @@
+"""Norm-conserving species read from the reference's own test potentials
+(test/potentials/*.xml, copied under tests/golden/) and their
+Kleinman-Bylander form in the oracle (Species::initialize_ncpp,
+src/Species.cpp:71-507), pinned by independent quadratures:
+
+  - the weight of each nonlocal channel is 1 / <phi_l| v_l - v_loc |phi_l>;
+  - the projector's transform equals 4 pi int r^2 j_l(Gr) (v_l - v_loc) phi_l dr
+    (direct quadrature with scipy's spherical Bessel functions), which checks
+    the sine/cosine-transform route of the reference;
+  - the KB operator reproduces the semilocal channel on its own reference
+    state: w_l <phi_l|dv phi_l>^2 = <phi_l|dv|phi_l>;
+and whole systems of the reference's tests (test/si4gs, test/h2ogs) hold
+their electron count and a variational SCF minimum."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.integrate import simpson
+from scipy.special import spherical_jn
+
+from qbg import configs as cf
+from qbg.species import read_species
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def si():
+    return read_species(GOLD / "Si_VBC_LDA-1.0.xml")
+
+
+@pytest.fixture(scope="module")
+def oxy():
+    return read_species(GOLD / "O_HSCV_PBE-1.0.xml")
+
+
+@pytest.fixture(scope="module")
+def hyd():
+    return read_species(GOLD / "H_HSCV_PBE-1.0.xml")
+
+
+def test_read_species(si, oxy, hyd):
+    assert (si.symbol, si.zval, si.lmax, si.llocal, si.nmesh, si.deltar) == \
+        ("Si", 4, 2, 2, 1000, 0.01)
+    assert (oxy.zval, oxy.lmax, oxy.llocal, oxy.nmesh) == (6, 1, 1, 2208)
+    assert (hyd.zval, hyd.lmax, hyd.llocal, hyd.nmesh) == (1, 0, 0, 2537)
+    # the first and last mesh values as printed in the file
+    assert si.vps[0][0] == 6.01180353958
+    r = np.arange(si.nmesh) * si.deltar
+    # pseudo-wavefunctions are normalized: int phi^2 r^2 dr = 1
+    for l in (0, 1):
+        assert abs(simpson(si.phi[l] ** 2 * r * r, x=r) - 1.0) < 1e-3
+    # tails approach -Z/r
+    assert abs(si.vps[2][-1] * r[-1] + si.zval) < 1e-2
+
+
+@pytest.mark.parametrize("name", ["si", "oxy"])
+def test_kb_weights_and_identity(oracle_mod, name, request):
+    sp = request.getfixturevalue(name)
+    proj, dl = oracle_mod.ncpp_kb(sp)
+    r = np.arange(sp.nmesh) * sp.deltar
+    vl = sp.vps[sp.llocal]
+    for l in range(proj.shape[0]):
+        if l == sp.llocal:
+            assert dl[l] == 0.0 and not proj[l].any()
+            continue
+        dv = sp.vps[l] - vl
+        e = simpson(dv * sp.phi[l] ** 2 * r * r, x=r)
+        assert abs(dl[l] * e - 1.0) < 1e-6, (l, dl[l], e)
+        assert np.allclose(proj[l], dv * sp.phi[l], rtol=0, atol=1e-15)
+        # KB acting on its reference state reproduces the semilocal value
+        beta_phi = simpson(proj[l] * sp.phi[l] * r * r, x=r)
+        assert abs(dl[l] * beta_phi ** 2 - e) < 1e-6 * abs(e)
+
+
+def test_projector_transform_by_quadrature(oracle_mod, si):
+    """The oracle's projector tables w_l(G) (sinft/cosft route) against direct
+    quadrature of 4 pi int r^2 j_l(Gr) f_l(r) dr."""
+    s = cf.si4gs(si)
+    o = oracle_mod.Oracle(s)
+    proj, dl = oracle_mod.ncpp_kb(si)
+    r = np.arange(si.nmesh) * si.deltar
+    g = np.array([0.3, 1.1, 2.7, 4.0, 6.5])
+    v, dv, vn, dvn = o.species_eval(0, g)
+    for l in (0, 1):
+        direct = np.array([4 * np.pi * simpson(r * r * spherical_jn(l, gg * r) * proj[l], x=r)
+                           for gg in g])
+        assert np.allclose(vn[l], direct, rtol=2e-5, atol=1e-6), (l, vn[l], direct)
+
+
+def test_si4_and_h2o_charge(oracle_mod, si, oxy, hyd):
+    for s in (cf.si4gs(si), cf.h2ogs(oxy, hyd, ecut_ry=30.0)):
+        o = oracle_mod.Oracle(s)
+        b = o.basis(0)
+        c = cf.gram_numpy(cf.random_coefficients(3, b["g2"], s.nst))
+        r = o.evaluate(c, hpsi=True, forces=True)
+        assert abs(r["total_charge"] - 2 * s.nst) < 1e-8
+        assert np.isfinite(r["etotal"])
+
+
+def test_si4_scf_is_variational(oracle_mod, si):
+    """The settings of test/si4gs/test.i: PSDA, ecutprec 2 Ry (1 Ha),
+    run 0 200 (200 SCF iterations of one step, no charge mixing); the
+    minimum reached is -15.3729448 Ha."""
+    from test_oracle_scf import perturbed_energy, scf
+    s = cf.si4gs(si)
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(4, b["g2"], s.nst))
+    c, es = scf(oracle_mod, s, o, c, "psda", 1.0, 200, 0)
+    assert abs(es[-1] - es[-2]) < 1e-7, es[-4:]
+    assert abs(es[-1] - (-15.3729448)) < 1e-6
+    o.update_density(c)
+    o.update_vhxc(False)
+    e0 = o.energy(c, hpsi=False, forces=False)["etotal"]
+    for t in range(2):
+        assert perturbed_energy(oracle_mod, o, c, 1e-2, t) > e0
