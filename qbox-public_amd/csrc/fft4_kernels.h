@@ -415,6 +415,13 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   double acc[NACC];
 #pragma unroll
   for (int i = 0; i < NACC; i++) acc[i] = 0.0;
+  // the potential of this thread's points, read once for all pairs of the block
+  constexpr int NV = MC == 0 ? N2 : 1;
+  T vreg[NV];
+  if (MC == 0 && vB) {
+#pragma unroll
+    for (int k2 = 0; k2 < NV; k2++) vreg[k2] = (T)vin[plane + (long)Mp::out_b(q, k2) * np0 + x];
+  }
   const int p0 = blockIdx.z * PPB;
   const int p1 = min(nbatch, p0 + PPB);
   for (int pair = p0; pair < p1; pair++) {
@@ -446,10 +453,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
       // tmp *= v (src/SlaterDet.cpp:432-434), then forward along y
       if (vB) {
 #pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) {
-          const T v = (T)vin[plane + (long)Mp::out_b(q, k2) * np0 + x];
-          b[k2] = {b[k2].x * v, b[k2].y * v};
-        }
+        for (int k2 = 0; k2 < NV; k2++) b[k2] = {b[k2].x * vreg[k2], b[k2].y * vreg[k2]};
         rdft<T, N2, -1>(b);
         if constexpr (!Mp::pfa) {
 #pragma unroll

@@ -108,7 +108,7 @@ struct qbg_grid {
   double* d_fac = nullptr;          // per-band density factors
   int fac_cap = 0;
   int batch = 64;                   // max pairs per batch
-  int yppb = 4;                     // y-pass pairs per block for the v multiply
+  int yppb = 16;                     // y-pass pairs per block for the v multiply
 };
 
 // launchers (fft.hip)
