@@ -45,16 +45,22 @@ __device__ __forceinline__ unsigned short bf16_rne(float x, float& back) {
   return (unsigned short)(u >> 16);
 }
 
-// split 8 values into NS parts of 8 bf16 each
+// split 8 values into NS parts of 8 bf16 each; the conversions are the
+// hardware's round-to-nearest-even FP32 -> BF16 (packed, two per instruction)
 template <int NS>
 __device__ __forceinline__ void split8(const float* v, u16x8 (&out)[NS]) {
+  float r[8];
 #pragma unroll
-  for (int e = 0; e < 8; e++) {
-    float r = v[e], b;
+  for (int e = 0; e < 8; e++) r[e] = v[e];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      out[s][e] = bf16_rne(r, b);
-      r -= b;
+  for (int s = 0; s < NS; s++) {
+    bf16x8 h;
+#pragma unroll
+    for (int e = 0; e < 8; e++) h[e] = (__bf16)r[e];
+    out[s] = __builtin_bit_cast(u16x8, h);
+    if (s + 1 < NS) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) r[e] -= (float)h[e];
     }
   }
 }
