@@ -171,6 +171,32 @@ __host__ __device__ __forceinline__ void rdft(cpx<T>* v) {
   }
 }
 
+// Raw buffer resource over [base, base + bytes): a load at a byte offset at
+// or past `bytes` returns zero and a store there is dropped (the hardware's
+// range check of an unswizzled buffer), so the elements a line does not
+// hold need no branch.  `bytes` stays below OOB.
+constexpr uint32_t OOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_res(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  else
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, cpx<T> v) {
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, (int)off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, (int)off, 0, 0);
+}
+
 // twiddle W_N^{SIGN m} from the LDS table tw[m] = exp(-2 pi i m / N)
 template <typename T, int SIGN>
 __device__ __forceinline__ cpx<T> twl(const cpx<T>* tw, int m) {

@@ -94,16 +94,20 @@ __global__ __launch_bounds__(256) void kz_bwd(const double* __restrict__ c, long
   __syncthreads();
   // store KB-plane chunks: the tile's +G and -G columns are two runs
   // lane = (column slot, plane kk): 16 slots x KB planes per pass
+  // (all NKB chunks of a column unrolled: the LDS reads and the stores
+  // issue back to back; the chunk past the line end is dropped)
   constexpr int NKB = nblk(N, KB);
-  cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
+  constexpr uint32_t SZ = sizeof(cpx<T>);
+  const auto zr = buf_res(zbuf + (long)pair * NKB * ncols * KB, (uint32_t)(NKB * KB) * ncols * SZ);
   const int skk = tid % KB, scl = tid / KB;
   for (int cl = scl; cl < t.ncol; cl += 256 / KB) {
     const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
-    cpx<T>* zc = zb + gcol * KB + skk;
-#pragma unroll 2
+    const uint32_t cb = (uint32_t)(gcol * KB + skk) * SZ;
+#pragma unroll
     for (int kb = 0; kb < NKB; kb++) {
       const int z = kb * KB + skk;
-      if (z < N) zc[(long)kb * ncols * KB] = A[z * TC + cl];
+      const cpx<T> v = A[(z < N ? z : 0) * TC + cl];
+      buf_store<T>(zr, z < N ? cb + (uint32_t)(kb * ncols * KB) * SZ : OOB, v);
     }
   }
 }
@@ -128,17 +132,26 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   for (int i = tid; i < N; i += 256) tw[i] = tw_g[i];
   __syncthreads();
   // load KB-plane chunks of the tile's columns
+  // (all NKB chunks of a column in flight before the LDS writes)
   constexpr int NKB = nblk(N, KB);
-  const cpx<T>* zb = zbuf + (long)pair * NKB * ncols * KB;
+  constexpr uint32_t SZ = sizeof(cpx<T>);
   {
+    const auto zr = buf_res(zbuf + (long)pair * NKB * ncols * KB,
+                            (uint32_t)(NKB * KB) * ncols * SZ);
     const int skk = tid % KB, scl = tid / KB;
     for (int cl = scl; cl < t.ncol; cl += 256 / KB) {
       const int gcol = cl < t.np ? t.col0 + cl : t.colm + (cl - t.np);
-      const cpx<T>* zc = zb + gcol * KB + skk;
-#pragma unroll 2
+      const uint32_t cb = (uint32_t)(gcol * KB + skk) * SZ;
+      cpx<T> st[NKB];
+#pragma unroll
       for (int kb = 0; kb < NKB; kb++) {
         const int z = kb * KB + skk;
-        if (z < N) A[z * TC + cl] = zc[(long)kb * ncols * KB];
+        st[kb] = buf_load<T>(zr, z < N ? cb + (uint32_t)(kb * ncols * KB) * SZ : OOB);
+      }
+#pragma unroll
+      for (int kb = 0; kb < NKB; kb++) {
+        const int z = kb * KB + skk;
+        if (z < N) A[z * TC + cl] = st[kb];
       }
     }
   }
@@ -231,19 +244,23 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
     __syncthreads();
   }
   const int kk = tid % KB, p = tid / KB;
-  // the row's column map staged in LDS: one coalesced load per block
-  // instead of a dependent global load per element
-  __shared__ int32_t cml[N];
-  for (int i = tid; i < N; i += NT) cml[i] = colmap[(long)r * N + i];
+  // the row's column map as byte offsets into the block's zbuf run, stored
+  // in the order phase A reads them (thread p, element n1): one ds_read with
+  // an immediate offset per element; OOB where the row has no column at x
+  __shared__ uint32_t cmo[N];
+  for (int i = tid; i < N; i += NT) {
+    const int pp = i / N1, n1 = i - pp * N1;
+    const int c = colmap[(long)r * N + Mp::in_a(n1, pp)];
+    cmo[i] = c >= 0 ? (uint32_t)(c * KB) * (uint32_t)sizeof(cpx<T>) : OOB;
+  }
   __syncthreads();
   if (p < N2 && kk < nk) {                // phase A, p = n2
-    const cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
+    const auto zr = buf_res(zbuf + ((long)pair * nkb + kb) * ncols * KB,
+                            (uint32_t)ncols * KB * sizeof(cpx<T>));
+    const uint32_t lo = kk * (uint32_t)sizeof(cpx<T>);
     cpx<T> a[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; n1++) {
-      const int c = cml[Mp::in_a(n1, p)];
-      a[n1] = c >= 0 ? zb[c * KB] : cpx<T>{T(0), T(0)};
-    }
+    for (int n1 = 0; n1 < N1; n1++) a[n1] = buf_load<T>(zr, cmo[p * N1 + n1] + lo);
     rdft<T, N1, +1>(a);
     if constexpr (!Mp::pfa) {
 #pragma unroll
@@ -267,14 +284,21 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_bwd(
     for (int k2 = 0; k2 < N2; k2++) ex[kk * NP + Mp::out_b(p, k2)] = b[k2];
   }
   __syncthreads();
-  // xbuf[pair][k][xb][r][xi]: runs of XB elements; 32-bit offsets from the
-  // block's first plane
+  // xbuf[pair][k][xb][r][xi]: runs of XB elements (run j = l*NXB + xb of
+  // plane l at j*nact*XB from the block's first); unrolled so that the LDS
+  // reads and stores of all runs issue back to back
+  constexpr int SQ = NT / XB, NIT = nblk(KB * NXB, SQ);
   cpx<T>* xo = xbuf + (((long)pair * np2 + kb * KB) * NXB * nact + r) * XB;
+  const auto xr = buf_res(xo, (uint32_t)(KB * NXB * XB) * nact * (uint32_t)sizeof(cpx<T>));
   const int sxi = tid % XB, sq = tid / XB;
-  for (int j = sq; j < nk * NXB; j += NT / XB) {
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int j = sq + it * SQ;
     const int l = j / NXB, xb = j - l * NXB;
     const int x = xb * XB + sxi;
-    if (x < N) xo[(l * NXB + xb) * nact * XB + sxi] = ex[l * NP + x];
+    const bool ok = j < nk * NXB && x < N;
+    const cpx<T> v = ex[ok ? l * NP + x : 0];
+    buf_store<T>(xr, ok ? (uint32_t)(j * nact * XB + sxi) * (uint32_t)sizeof(cpx<T>) : OOB, v);
   }
 }
 
@@ -294,15 +318,34 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
   const int nk = min(KB, np2 - kb * KB);
   if constexpr (!Mp::pfa)
     for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
-  __shared__ int32_t cml[N];
-  for (int i = tid; i < N; i += NT) cml[i] = colmap[(long)r * N + i];
+  // byte offsets of the row's columns in phase B' order (as in kx_bwd)
+  __shared__ uint32_t cmo[N];
+  for (int i = tid; i < N; i += NT) {
+    const int pp = i / N1, n1 = i - pp * N1;
+    const int c = colmap[(long)r * N + Mp::in_a(n1, pp)];
+    cmo[i] = c >= 0 ? (uint32_t)(c * KB) * (uint32_t)sizeof(cpx<T>) : OOB;
+  }
   {
+    // all runs' loads in flight before the LDS writes (layout as in kx_bwd)
+    constexpr int SQ = NT / XB, NIT = nblk(KB * NXB, SQ);
     const cpx<T>* xo = xbuf + (((long)pair * np2 + kb * KB) * NXB * nact + r) * XB;
+    const auto xr = buf_res(xo, (uint32_t)(KB * NXB * XB) * nact * (uint32_t)sizeof(cpx<T>));
     const int sxi = tid % XB, sq = tid / XB;
-    for (int j = sq; j < nk * NXB; j += NT / XB) {
+    cpx<T> st[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int j = sq + it * SQ;
+      const int xb = j % NXB;
+      const bool ok = j < nk * NXB && xb * XB + sxi < N;
+      st[it] = buf_load<T>(xr, ok ? (uint32_t)(j * nact * XB + sxi) * (uint32_t)sizeof(cpx<T>)
+                                  : OOB);
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int j = sq + it * SQ;
       const int l = j / NXB, xb = j - l * NXB;
       const int x = xb * XB + sxi;
-      if (x < N) ex[l * NP + x] = xo[(l * NXB + xb) * nact * XB + sxi];
+      if (j < nk * NXB && x < N) ex[l * NP + x] = st[it];
     }
   }
   __syncthreads();
@@ -330,12 +373,11 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void kx_fwd(
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[kk * NP + p * N1 + k1];
     rdft<T, N1, -1>(cv);
-    cpx<T>* zb = zbuf + ((long)pair * nkb + kb) * ncols * KB + kk;
+    const auto zr = buf_res(zbuf + ((long)pair * nkb + kb) * ncols * KB,
+                            (uint32_t)ncols * KB * sizeof(cpx<T>));
+    const uint32_t lo = kk * (uint32_t)sizeof(cpx<T>);
 #pragma unroll
-    for (int m2 = 0; m2 < N1; m2++) {
-      const int c = cml[Mp::in_a(m2, p)];
-      if (c >= 0) zb[c * KB] = cv[m2];
-    }
+    for (int m2 = 0; m2 < N1; m2++) buf_store<T>(zr, cmo[p * N1 + m2] + lo, cv[m2]);
   }
 }
 
@@ -354,13 +396,23 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
     const double* __restrict__ fac, double* __restrict__ rho, double* __restrict__ fgrid,
     double scale, int mode) {
   constexpr int N = N1 * N2, NP = N + 1, NT = xthreads(N1, N2);
+  using Mp = Map<N1, N2>;
   __shared__ cpx<T> ex[XB * NP];
-  __shared__ cpx<T> tw[N];
-  __shared__ int jr[N];
+  __shared__ cpx<T> tw[Mp::pfa ? 1 : N];
+  // byte offsets of the active rows within a pair's run of this block, in
+  // the order phase A reads them (thread q, element n1); OOB for the rows
+  // outside the active band: one ds_read per element, no branch
+  __shared__ uint32_t jo[N];
   const int tid = threadIdx.x;
   const int xb = blockIdx.x, k = blockIdx.y;
   const int nxb = nblk(np0, XB);
-  for (int i = tid; i < N; i += NT) { tw[i] = tw_g[i]; jr[i] = jrow[i]; }
+  if constexpr (!Mp::pfa)
+    for (int i = tid; i < N; i += NT) tw[i] = tw_g[i];
+  for (int i = tid; i < N; i += NT) {
+    const int qq = i / N1, n1 = i - qq * N1;
+    const int r = jrow[Mp::in_a(n1, qq)];
+    jo[i] = r >= 0 ? (uint32_t)(r * XB) * (uint32_t)sizeof(cpx<T>) : OOB;
+  }
   __syncthreads();
   const int xi = tid % XB, q = tid / XB;
   const int x = xb * XB + xi;
@@ -368,11 +420,12 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   const bool vA = vx && q < N2;           // phase A / B' role (N2 per line)
   const bool vB = vx && q < N1;           // phase B / A' role (N1 per line)
   const long plane = (long)k * np0 * N;
-  using Mp = Map<N1, N2>;
-  // run of row r of this block in pair p's xbuf
+  // pair p's run of this block: nact rows of XB elements
   const long pstride = (long)np2 * nxb * nact * XB;
-  cpx<T>* const xb0 = xbuf + (((long)k * nxb + xb) * nact) * XB + xi;
-  auto xrow = [&](int pair, int r) { return xb0 + pair * pstride + r * XB; };
+  cpx<T>* const run0 = xbuf + (((long)k * nxb + xb) * nact) * XB;
+  const uint32_t runbytes = (uint32_t)nact * XB * sizeof(cpx<T>);
+  const uint32_t lo = xi * (uint32_t)sizeof(cpx<T>);
+  const uint32_t* const jq = jo + q * N1;
 
   if (MC == 0) mode = Y_VMUL;
   if (MC == 1) mode = Y_DENSITY;
@@ -402,11 +455,9 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
 #pragma unroll
       for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[xi * NP + q * N1 + k1];
       rdft<T, N1, -1>(cv);
+      const auto xr = buf_res(run0, runbytes);
 #pragma unroll
-      for (int m2 = 0; m2 < N1; m2++) {
-        const int r = jr[Mp::in_a(m2, q)];
-        if (r >= 0) *xrow(0, r) = cv[m2];
-      }
+      for (int m2 = 0; m2 < N1; m2++) buf_store<T>(xr, jq[m2] + lo, cv[m2]);
     }
     return;
   }
@@ -426,13 +477,11 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
   const int p1 = min(nbatch, p0 + PPB);
   for (int pair = p0; pair < p1; pair++) {
     __syncthreads();                      // previous pair's readers of ex are done
+    const auto xr = buf_res(run0 + pair * pstride, runbytes);
     if (vA) {                             // phase A, q = n2 (bwd)
       cpx<T> a[N1];
 #pragma unroll
-      for (int n1 = 0; n1 < N1; n1++) {
-        const int r = jr[Mp::in_a(n1, q)];
-        a[n1] = r >= 0 ? *xrow(pair, r) : cpx<T>{T(0), T(0)};
-      }
+      for (int n1 = 0; n1 < N1; n1++) a[n1] = buf_load<T>(xr, jq[n1] + lo);
       rdft<T, N1, +1>(a);
       if constexpr (!Mp::pfa) {
 #pragma unroll
@@ -473,10 +522,7 @@ __global__ __launch_bounds__(xthreads(N1, N2)) void ky(
         for (int k1 = 0; k1 < N1; k1++) cv[k1] = ex[xi * NP + q * N1 + k1];
         rdft<T, N1, -1>(cv);
 #pragma unroll
-        for (int m2 = 0; m2 < N1; m2++) {
-          const int r = jr[Mp::in_a(m2, q)];
-          if (r >= 0) *xrow(pair, r) = cv[m2];
-        }
+        for (int m2 = 0; m2 < N1; m2++) buf_store<T>(xr, jq[m2] + lo, cv[m2]);
       }
     } else if (MC == 1) {
       if (vB) {
