@@ -152,7 +152,7 @@ def bench_c4(args):
     ctx = qbg.Context(0)
     prec = "bf16x2" if args.prec == "mixed" else "fp64"
     systems = cf.config4_systems()
-    efs, dcs = [], []
+    efs, dcs, hps = [], [], []
     for s in systems:
         ef = qbg.EnergyFunctional(ctx, s)
         c = cf.random_coefficients(cf.BASE_SEED + 60, ef.basis(0)["g2"], s.nst)
@@ -160,14 +160,15 @@ def bench_c4(args):
         qbg.gram(ctx, dc, ef.ngw, ef.ngw, s.nst)
         efs.append(ef)
         dcs.append(dc)
+        hps.append(ctx.array(c.shape, np.complex128))
         print(f"[bench c4] {s.name}: grid {ef.np0}x{ef.np1}x{ef.np2} ngw {ef.ngw}",
               file=sys.stderr, flush=True)
     nb = sum(s.nst for s in systems)
 
     def loop():
         out = []
-        for ef, dc in zip(efs, dcs):
-            out.append(ef.evaluate(dc, None, forces=True, prec=prec)["etotal"])
+        for ef, dc, hp in zip(efs, dcs, hps):
+            out.append(ef.evaluate(dc, hp, forces=True, prec=prec)["etotal"])
         return out
 
     for _ in range(max(1, args.warmup)):
@@ -177,12 +178,12 @@ def bench_c4(args):
     for _ in range(args.steps):
         loop()
     t_loop = (time.perf_counter() - t0) / args.steps
-    for ef, dc in zip(efs, dcs):          # allocate before capture
-        ef.evaluate_async(dc, None, forces=True, prec=prec)
+    for ef, dc, hp in zip(efs, dcs, hps):          # allocate before capture
+        ef.evaluate_async(dc, hp, forces=True, prec=prec)
         ef.results()
     with qbg.Graph(ctx) as g:
-        for ef, dc in zip(efs, dcs):
-            ef.evaluate_async(dc, None, forces=True, prec=prec)
+        for ef, dc, hp in zip(efs, dcs, hps):
+            ef.evaluate_async(dc, hp, forces=True, prec=prec)
     g.launch()
     got = [ef.results()["etotal"] for ef in efs]
     assert got == ref, "graph replay differs from the loop"

@@ -220,7 +220,15 @@ int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
 int qbg_ef_evaluate_async(qbg_ef* ef, const double* c, int ldc, double* hpsi, int ldh,
                           int flags, int prec);
 /* the energy step alone (EnergyFunctional::energy after update_density and
- * update_vhxc), without host synchronization; results as below */
+ * update_vhxc), without host synchronization; results as below.  With
+ * QBG_EF_SAME_C in flags the caller guarantees that c is the array the last
+ * qbg_ef_update_density read and that it has not changed since: the H*psi
+ * pass then starts from the backward transforms update_density kept
+ * (resident in device memory when they fit in half of the free memory)
+ * instead of repeating them; the result is the same.  Without the flag, or
+ * when nothing was kept, the transforms are recomputed.  qbg_ef_evaluate_async
+ * always reuses them (one call, same c). */
+#define QBG_EF_SAME_C 8
 int qbg_ef_energy_async(qbg_ef* ef, const double* c, int ldc, double* hpsi, int ldh,
                         int flags, int prec);
 /* energies (14), fion (3*nat) and sigma (7x6) of the last launched

@@ -449,7 +449,10 @@ int qbg_ef_set_positions(qbg_ef* e, const double* tau) {
 
 // ChargeDensity::update_density (src/ChargeDensity.cpp:136-198):
 // rhor = sum_n occ_n/omega |psi_n|^2; rhog = vft.forward(omega * rhor)
-int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
+// The backward z and x transforms of the bands stay in the band grid's
+// resident array when it fits (keep), for an H*psi of the same bands
+// (ef_launch with reuse); the density itself is the same either way.
+static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool keep) {
   if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
   (void)hipSetDevice(ctx->device);
@@ -457,8 +460,15 @@ int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
   fill_zero(e->d_rhor, e->N, s);
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;
-  int rc = density_batches(e->gw, c, ldc, e->nst, e->d_dfac, e->d_rhor, fprec);
+  e->xres_c = nullptr;
+  const bool res = keep && grid_ensure_xres(e->gw, (e->nst + 1) / 2);
+  int rc = density_batches(e->gw, c, ldc, e->nst, e->d_dfac, e->d_rhor, fprec, res);
   if (rc) return rc;
+  if (res) {
+    e->xres_c = c;
+    e->xres_ldc = ldc;
+    e->xres_prec = fprec;
+  }
   qbg_grid* g = e->gv;
   if ((rc = grid_ensure_ws(g, 1))) return rc;
   int r1, r2, r3;
@@ -477,6 +487,10 @@ int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
   hipError_t er = hipGetLastError();
   if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
   return QBG_OK;
+}
+
+int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
+  return ef_update_density(e, c, ldc, prec, true);
 }
 
 // EnergyFunctional::update_vhxc (src/EnergyFunctional.cpp:231-394), LDA
@@ -674,8 +688,11 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
 // Gamma, LDA.  energies (host, 14): ekin eps enl ehart ehart_e ehart_ep
 // ehart_p exc dxc esr eself ecoul etotal total_charge; fion (host, 3*nat)
 // species-major; sigma (host, 7x6: total, ekin, eps, enl, ehart, exc, esr).
+// reuse: the caller guarantees that c holds what the last update_density
+// read; H*psi then starts from the transforms update_density kept
 static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi,
-                     int ldh, int compute_forces, int compute_stress, int prec) {
+                     int ldh, int compute_forces, int compute_stress, int prec,
+                     bool reuse = false) {
   if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
@@ -710,7 +727,13 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   }
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
   if (compute_hpsi) {
-    int rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, fprec);
+    int rc;
+    if (reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec) {
+      rc = hpsi_resident(e->gw, e->d_vr, nst, hpsi, ldh, fprec);
+      e->xres_c = nullptr;   // transformed in place: consumed
+    } else {
+      rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, fprec);
+    }
     if (rc) return rc;
   }
   if (compute_forces) {
@@ -797,9 +820,9 @@ int qbg_ef_evaluate_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int
   if (!e || !c) return QBG_E_INVAL;
   const int hp = (flags & QBG_EF_HPSI) != 0, fo = (flags & QBG_EF_FORCES) != 0,
             st = (flags & QBG_EF_STRESS) != 0;
-  int rc = qbg_ef_update_density(e, c, ldc, prec);
+  int rc = ef_update_density(e, c, ldc, prec, hp != 0);
   if (!rc) rc = qbg_ef_update_vhxc(e, st);
-  if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec);
+  if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec, true);
   return rc;
 }
 
@@ -809,7 +832,7 @@ int qbg_ef_energy_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int l
                         int prec) {
   if (!e || !c) return QBG_E_INVAL;
   return ef_launch(e, c, ldc, (flags & QBG_EF_HPSI) != 0, hpsi, ldh, (flags & QBG_EF_FORCES) != 0,
-                   (flags & QBG_EF_STRESS) != 0, prec);
+                   (flags & QBG_EF_STRESS) != 0, prec, (flags & QBG_EF_SAME_C) != 0);
 }
 
 int qbg_ef_results(qbg_ef* e, double* energies, double* fion, double* sigma) {

@@ -73,5 +73,10 @@ struct qbg_ef {
   double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
   std::vector<double> fesr;
   bool have_rho = false, have_vhxc = false, vhxc_stress = false;
+  // the bands (pointer, leading dimension, band-FFT precision) whose x-pass
+  // output the last update_density left in the band grid's resident array;
+  // xres_c = nullptr when there is none (consumed by an H*psi or not kept)
+  const double* xres_c = nullptr;
+  int xres_ldc = 0, xres_prec = -1;
   std::vector<void*> allocs;
 };

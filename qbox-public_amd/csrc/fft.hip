@@ -441,10 +441,10 @@ static int xpass_t(qbg_grid* g, bool fwd, int nbatch, hipStream_t s) {
   dim3 grid(g->np2 * ngroups, nbatch);
   if (!fwd)
     hipLaunchKernelGGL((k_xbwd<T>), grid, dim3(256), lds, s, (const cpx<T>*)g->d_zbuf,
-                       (cpx<T>*)g->d_xbuf, g->np0, g->np2, g->ncols, g->nact, g->xrows, ngroups,
+                       (cpx<T>*)g->xws, g->np0, g->np2, g->ncols, g->nact, g->xrows, ngroups,
                        g->d_row_start, g->d_rc_ivec, g->d_rc_x, g->plan0, tw_of<T>(g, 0));
   else
-    hipLaunchKernelGGL((k_xfwd<T>), grid, dim3(256), lds, s, (const cpx<T>*)g->d_xbuf,
+    hipLaunchKernelGGL((k_xfwd<T>), grid, dim3(256), lds, s, (const cpx<T>*)g->xws,
                        (cpx<T>*)g->d_zbuf, g->np0, g->np2, g->ncols, g->nact, g->xrows, ngroups,
                        g->d_row_start, g->d_rc_ivec, g->d_rc_x, g->plan0, tw_of<T>(g, 0));
   return hipGetLastError() == hipSuccess ? QBG_OK : QBG_E_HIP;
@@ -455,7 +455,7 @@ static void ylaunch(qbg_grid* g, int nbatch, int ppb, const double* v, const dou
                     double* rho, double* fgrid, double scale, hipStream_t s) {
   const size_t lds = 2 * sizeof(cpx<T>) * g->ytile * g->np1;
   dim3 grid((g->np0 + g->ytile - 1) / g->ytile, g->np2, (nbatch + ppb - 1) / ppb);
-  hipLaunchKernelGGL((k_ypass<T, MODE>), grid, dim3(256), lds, s, (cpx<T>*)g->d_xbuf, g->np0,
+  hipLaunchKernelGGL((k_ypass<T, MODE>), grid, dim3(256), lds, s, (cpx<T>*)g->xws, g->np0,
                      g->np1, g->np2, g->nact, g->ytile, g->d_act_row, nbatch, ppb, g->plan1,
                      tw_of<T>(g, 1), v, fac, rho, fgrid, scale);
 }
@@ -503,7 +503,7 @@ static int zfwd_t(qbg_grid* g, int mode, int outmode, double* out, long pstride,
 // Each one checks, before launching, that the workspace holds the batch and
 // that the arrays its mode dereferences are present.
 static bool ws_ok(const qbg_grid* g, int nbatch) {
-  return g && g->d_zbuf && g->d_xbuf && nbatch >= 1 && nbatch <= g->ws_pairs;
+  return g && g->d_zbuf && g->xws && nbatch >= 1 && nbatch <= g->ws_pairs;
 }
 
 static int launched() { return hipGetLastError() == hipSuccess ? QBG_OK : QBG_E_HIP; }

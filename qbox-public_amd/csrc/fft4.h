@@ -91,29 +91,47 @@ __host__ __device__ __forceinline__ void rdft(cpx<T>* v) {
     v[1] = add(a1, jb1);
     v[3] = sub(a1, jb1);
   } else if constexpr (smallest_prime(R) == R) {
-    // odd prime: symmetric form, out[k] and out[R-k] share the cosine sums
+    // odd prime: symmetric form over the pair sums ap_j = v_j + v_{R-j} and
+    // differences am_j = v_j - v_{R-j}; out[k] and out[R-k] share the cosine
+    // sums, and out[0] = v_0 + sum ap_j reuses the pair sums
     constexpr int H = (R - 1) / 2;
+    cpx<T> ap[H + 1], am[H + 1];
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+      ap[j] = add(v[j], v[R - j]);
+      am[j] = sub(v[j], v[R - j]);
+    }
     cpx<T> out[R];
     cpx<T> s0 = v[0];
 #pragma unroll
-    for (int j = 1; j < R; j++) s0 = add(s0, v[j]);
+    for (int j = 1; j <= H; j++) s0 = add(s0, ap[j]);
     out[0] = s0;
 #pragma unroll
     for (int k = 1; k <= H; k++) {
-      T ra = v[0].x, ia = v[0].y, rb = 0, ib = 0;
+      T ra = v[0].x, ia = v[0].y;
 #pragma unroll
       for (int j = 1; j <= H; j++) {
         const T c = (T)Tw<R>::c[(j * k) % R];
-        const T s = (T)Tw<R>::s[(j * k) % R];
-        const cpx<T> ap = add(v[j], v[R - j]), am = sub(v[j], v[R - j]);
-        ra += c * ap.x;
-        ia += c * ap.y;
-        rb += s * am.x;
-        ib += s * am.y;
+        ra += c * ap[j].x;
+        ia += c * ap[j].y;
       }
-      // SIGN i sum s am = (-SIGN ib, SIGN rb)
-      out[k] = {ra - SIGN * ib, ia + SIGN * rb};
-      out[R - k] = {ra + SIGN * ib, ia - SIGN * rb};
+      // SIGN i sum_j s_jk am_j = (-SIGN ib, SIGN rb)
+      if constexpr (H == 1) {
+        // one sine term: fused into the output multiply-adds
+        const T s = (T)(SIGN * Tw<R>::s[k % R]);
+        out[k] = {ra - s * am[1].y, ia + s * am[1].x};
+        out[R - k] = {ra + s * am[1].y, ia - s * am[1].x};
+      } else {
+        T rb = 0, ib = 0;
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+          const T s = (T)Tw<R>::s[(j * k) % R];
+          rb += s * am[j].x;
+          ib += s * am[j].y;
+        }
+        out[k] = {ra - SIGN * ib, ia + SIGN * rb};
+        out[R - k] = {ra + SIGN * ib, ia - SIGN * rb};
+      }
     }
 #pragma unroll
     for (int k = 0; k < R; k++) v[k] = out[k];

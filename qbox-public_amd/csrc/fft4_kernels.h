@@ -606,20 +606,20 @@ struct Launch {
     if (prec == QBG_PREC_FP64) {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<double, N1, N2>), grid, dim3(NT), 0, s,
-                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->d_xbuf, g->np2,
+                           (const cpx<double>*)g->d_zbuf, (cpx<double>*)g->xws, g->np2,
                            g->ncols, g->nact, g->d_colmap, twp<double>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<double, N1, N2>), grid, dim3(NT), 0, s,
-                           (const cpx<double>*)g->d_xbuf, (cpx<double>*)g->d_zbuf, g->np2,
+                           (const cpx<double>*)g->xws, (cpx<double>*)g->d_zbuf, g->np2,
                            g->ncols, g->nact, g->d_colmap, twp<double>(g, 0));
     } else {
       if (!fwd)
         hipLaunchKernelGGL((kx_bwd<float, N1, N2>), grid, dim3(NT), 0, s,
-                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->d_xbuf, g->np2,
+                           (const cpx<float>*)g->d_zbuf, (cpx<float>*)g->xws, g->np2,
                            g->ncols, g->nact, g->d_colmap, twp<float>(g, 0));
       else
         hipLaunchKernelGGL((kx_fwd<float, N1, N2>), grid, dim3(NT), 0, s,
-                           (const cpx<float>*)g->d_xbuf, (cpx<float>*)g->d_zbuf, g->np2,
+                           (const cpx<float>*)g->xws, (cpx<float>*)g->d_zbuf, g->np2,
                            g->ncols, g->nact, g->d_colmap, twp<float>(g, 0));
     }
   }
@@ -639,7 +639,7 @@ struct Launch {
                       hipStream_t s) {
     constexpr int NT = xthreads(N1, N2);
 #define QBG_KY(MC)                                                                            \
-  hipLaunchKernelGGL((ky<T, N1, N2, MC>), grid, dim3(NT), 0, s, (cpx<T>*)g->d_xbuf, g->np0,     \
+  hipLaunchKernelGGL((ky<T, N1, N2, MC>), grid, dim3(NT), 0, s, (cpx<T>*)g->xws, g->np0,     \
                      g->np2, g->nact, g->d_jrow, nbatch, ppb, twp<T>(g, 1), v, fac, rho, fgrid, \
                      scale, mode)
     if (mc == 0) QBG_KY(0);

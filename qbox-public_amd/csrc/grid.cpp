@@ -240,7 +240,7 @@ int qbg_timer_stop(qbg_ctx* c, float* ms) {
 static void grid_free(qbg_grid* g) {
   void* ptrs[] = {g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->d_row_start, g->d_rc_ivec, g->d_rc_x,
                   g->d_act_row, g->d_colmap, g->d_jrow, g->d_tw64[0], g->d_tw64[1], g->d_tw64[2], g->d_tw32[0],
-                  g->d_tw32[1], g->d_tw32[2], g->d_zbuf, g->d_xbuf, g->d_fac};
+                  g->d_tw32[1], g->d_tw32[2], g->d_zbuf, g->d_xbuf, g->d_fac, g->d_xres};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -496,11 +496,41 @@ int qbg::grid_ensure_ws(qbg_grid* g, int npairs) {
   g->d_zbuf = g->d_xbuf = nullptr;
   g->ws_bytes = 0;
   g->ws_pairs = 0;
+  g->xws = nullptr;
   if (hipMalloc(&g->d_zbuf, zb) != hipSuccess || hipMalloc(&g->d_xbuf, xb) != hipSuccess)
     return c->fail(QBG_E_NOMEM, "hipMalloc failed (FFT workspace)");
   g->ws_bytes = zb + xb;
   g->ws_pairs = npairs;
+  g->xws = g->d_xbuf;
   return QBG_OK;
+}
+
+// one pair's x workspace: nact rows x np0 (padded to the x blocks) x np2
+// planes; covers the four-step layout and the generic one (np0 unpadded)
+size_t qbg::xpair_bytes(const qbg_grid* g, int prec) {
+  const size_t np0p = (size_t)(g->np0 + 15) / 16 * 16;
+  return (size_t)g->nact * np0p * g->np2 * (prec == QBG_PREC_FP64 ? 16 : 8);
+}
+
+bool qbg::grid_ensure_xres(qbg_grid* g, int npairs) {
+  // sized for FP64 whatever the precision asked for now, so that a later
+  // FP64 evaluation does not reallocate under graphs captured earlier
+  const size_t need = (size_t)npairs * xpair_bytes(g, QBG_PREC_FP64);
+  if (need <= g->xres_bytes) return true;
+  if (g->ctx->capturing) return false;
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->d_xres) (void)hipFree(g->d_xres);
+  g->d_xres = nullptr;
+  g->xres_bytes = 0;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 2) return false;
+  if (hipMalloc(&g->d_xres, need) != hipSuccess) {
+    (void)hipGetLastError();
+    g->d_xres = nullptr;
+    return false;
+  }
+  g->xres_bytes = need;
+  return true;
 }
 static int ensure_ws(qbg_grid* g, int npairs) { return qbg::grid_ensure_ws(g, npairs); }
 
@@ -548,19 +578,54 @@ static int last_launch(qbg_grid* g) {
 // rho += sum over band pairs of fac[n] |psi_n(r)|^2, the pairs in batches of
 // up to g->batch (src/SlaterDet.cpp:232-283); fac is a device array of
 // 2 * ceil(nst / 2) factors, so the loop copies nothing from the host
+namespace {
+// points the pass launchers at a slice of the resident x array for one
+// batch and back at the batch workspace when the scope ends (also on error)
+struct XwsScope {
+  qbg_grid* g;
+  XwsScope(qbg_grid* g_, void* p) : g(g_) { if (p) g->xws = p; }
+  ~XwsScope() { g->xws = g->d_xbuf; }
+};
+}  // namespace
+
 int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
-                         double* rho, int prec) {
+                         double* rho, int prec, bool resident) {
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
+  const size_t pb = xpair_bytes(g, prec);
+  if (resident && g->xres_bytes < (size_t)npairs * pb)
+    return g->ctx->fail(QBG_E_STATE, "resident x array too small");
   hipStream_t s = g->ctx->stream;
   for (int p0 = 0; p0 < npairs; p0 += B) {
     const int nb = std::min(B, npairs - p0);
     const int band0 = 2 * p0;
     const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
+    XwsScope xs(g, resident ? (char*)g->d_xres + (size_t)p0 * pb : nullptr);
     PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
     PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
     PROF(QBG_K_YDENS, ypass(g, prec, Y_DENSITY, nb, nullptr, fac + 2 * p0, rho, nullptr, 1.0, s));
+  }
+  return last_launch(g);
+}
+
+int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec) {
+  const int npairs = (nst + 1) / 2;
+  const int B = std::min(npairs, g->batch);
+  TRY(ensure_ws(g, B));
+  const size_t pb = xpair_bytes(g, prec);
+  if (g->xres_bytes < (size_t)npairs * pb)
+    return g->ctx->fail(QBG_E_STATE, "resident x array too small");
+  hipStream_t s = g->ctx->stream;
+  for (int p0 = 0; p0 < npairs; p0 += B) {
+    const int nb = std::min(B, npairs - p0);
+    const int band0 = 2 * p0;
+    const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
+    XwsScope xs(g, (char*)g->d_xres + (size_t)p0 * pb);
+    PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
+    PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
+    PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh,
+                          last_has2, nb, s));
   }
   return last_launch(g);
 }

@@ -105,6 +105,14 @@ struct qbg_grid {
   void* d_xbuf = nullptr;
   size_t ws_bytes = 0;
   int ws_pairs = 0;                 // pairs the workspace holds
+  // the x workspace the pass launchers use: d_xbuf, or a batch's slice of
+  // the resident array while a resident density or H*psi loop runs
+  void* xws = nullptr;
+  // resident x-pass output of every pair of a band set (grown on demand):
+  // the backward z and x transforms of update_density, kept for the H*psi
+  // of the same coefficients
+  void* d_xres = nullptr;
+  size_t xres_bytes = 0;
   double* d_fac = nullptr;          // per-band density factors
   int fac_cap = 0;
   int batch = 64;                   // max pairs per batch
@@ -125,6 +133,13 @@ enum OutMode { OUT_STORE = 0, OUT_ACCUM = 1 };
 // that many already); every launcher below returns QBG_E_STATE without
 // launching when the workspace is missing or smaller than its batch.
 int grid_ensure_ws(qbg_grid* g, int npairs);
+// bytes of one pair's x workspace at precision prec (both pass families)
+size_t xpair_bytes(const qbg_grid* g, int prec);
+// allocate the resident x array for npairs pairs (sized for FP64); returns
+// false (allocating nothing) when it would take more than half of the free
+// device memory or during graph capture, and the caller keeps the
+// recomputing path
+bool grid_ensure_xres(qbg_grid* g, int npairs);
 int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s);
 int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s);
@@ -134,9 +149,15 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
          int last_has2, int nbatch, hipStream_t s);
 // band-pair density passes with per-band factors already on the device
-// (fac[2*npairs], zero for a missing last band); no host synchronization
+// (fac[2*npairs], zero for a missing last band); no host synchronization.
+// resident: the x-pass output of every pair stays in g->d_xres (which must
+// hold (nst+1)/2 pairs at prec) for hpsi_resident
 int density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
-                    double* rho, int prec);
+                    double* rho, int prec, bool resident = false);
+// hpsi += the local-potential part (rs_mul_add) of the bands whose x-pass
+// output density_batches(..., resident = true) left in g->d_xres; consumes
+// it (the y pass transforms it in place)
+int hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec);
 // util.hip
 void fill_zero(double* p, size_t n, hipStream_t s);
 void fill_value(double* p, size_t n, double v, hipStream_t s);

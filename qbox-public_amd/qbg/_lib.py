@@ -462,9 +462,12 @@ class EnergyFunctional:
             PREC[prec]))
 
     def energy_async(self, c: DArray, hpsi: DArray = None, forces=False, stress=False,
-                     prec="fp64"):
-        """qbg_ef_energy_async: enqueue EnergyFunctional::energy alone."""
+                     prec="fp64", same_c=False):
+        """qbg_ef_energy_async: enqueue EnergyFunctional::energy alone.
+        same_c: c is unchanged since the last update_density (QBG_EF_SAME_C),
+        so H*psi reuses the backward transforms update_density kept."""
         flags = (1 if hpsi is not None else 0) | (2 if forces else 0) | (4 if stress else 0)
+        flags |= 8 if same_c else 0
         self.ctx.check(self.ctx.lib.qbg_ef_energy_async(
             self.h, c.p, self.ngw, hpsi.p if hpsi is not None else None, self.ngw, flags,
             PREC[prec]))
@@ -481,10 +484,10 @@ class EnergyFunctional:
         return out
 
     def evaluate(self, c: DArray, hpsi: DArray = None, forces=True, stress=False, prec="fp64"):
-        """density -> update_vhxc -> energy (src/BOSampleStepper.cpp:1207-1217)."""
-        self.update_density(c, prec)
-        self.update_vhxc(stress)
-        return self.energy(c, hpsi, forces, stress, prec)
+        """density -> update_vhxc -> energy (src/BOSampleStepper.cpp:1207-1217)
+        as one call (qbg_ef_evaluate_async, then qbg_ef_results)."""
+        self.evaluate_async(c, hpsi, forces, stress, prec)
+        return self.results()
 
     def get(self, what):
         n = {0: self.N, 1: self.N, 2: self.N, 3: 2 * self.ngv, 4: 2 * self.ngv,

@@ -34,7 +34,6 @@ class ScfLoop:
         self.mixer = Mixer(ef, alpha, rc_kerker, ndim, rc1)
         self.graphs = graphs
         self._g_pot = {}
-        self._g_step = None
         self.itscf = 0
 
     # the two launch sequences of one SCF iteration
@@ -46,26 +45,21 @@ class ScfLoop:
         if mix:
             self.stepper.preprocess()
 
-    def _step(self):
-        self.ef.energy_async(self.c, self.hpsi, forces=False, stress=False, prec=self.prec)
+    def _step(self, first: bool):
+        # the first step of an iteration sees the bands update_density read
+        self.ef.energy_async(self.c, self.hpsi, forces=False, stress=False, prec=self.prec,
+                             same_c=first)
         self.stepper.update(self.c, self.hpsi)
 
     def _run(self, key, fn, *args):
         if not self.graphs or self.itscf == 0:
             fn(*args)
             return
-        if key == "step":
-            if self._g_step is None:
-                with Graph(self.ctx) as g:
-                    fn(*args)
-                self._g_step = g
-            self._g_step.launch()
-        else:
-            if key not in self._g_pot:
-                with Graph(self.ctx) as g:
-                    fn(*args)
-                self._g_pot[key] = g
-            self._g_pot[key].launch()
+        if key not in self._g_pot:
+            with Graph(self.ctx) as g:
+                fn(*args)
+            self._g_pot[key] = g
+        self._g_pot[key].launch()
 
     def run(self, nitscf: int, nite: int, delta_ratio: float = 0.01):
         """Returns one dict per SCF iteration: etotal (of the last energy
@@ -82,7 +76,7 @@ class ScfLoop:
             eigs = []
             res = None
             for ite in range(max(nite, 1)):
-                self._run("step", self._step)
+                self._run(("step", ite == 0), self._step, ite == 0)
                 res = self.ef.results()
                 w = self.stepper.results()
                 if ite == 0:
@@ -102,9 +96,8 @@ class ScfLoop:
         return history
 
     def release(self):
-        for g in list(self._g_pot.values()) + ([self._g_step] if self._g_step else []):
+        for g in self._g_pot.values():
             g.release()
         self._g_pot = {}
-        self._g_step = None
         self.stepper.release()
         self.mixer.release()
