@@ -608,9 +608,13 @@ static int bf16_ensure(qbg_ef* e) {
     return c->fail(QBG_E_STATE, "BF16 workspace must exist before graph capture "
                                 "(run the call once uncaptured)");
   auto up = [](long x, long m) { return (x + m - 1) / m * m; };
-  e->Kp = up(2L * e->wb.size, 128);
-  e->Mp = (int)up(e->M, 128);
-  e->Np = (int)up(e->nst, 128);
+  // 256-edge tiles of the split GEMMs when the projector and band counts
+  // padded to 128 are multiples of 256 already (no extra padded work);
+  // otherwise 128
+  const long edge = (up(e->M, 128) % 256 == 0 && up(e->nst, 128) % 256 == 0) ? 256 : 128;
+  e->Kp = up(2L * e->wb.size, edge);
+  e->Mp = (int)up(e->M, edge);
+  e->Np = (int)up(e->nst, edge);
   const int ntile = (e->Mp / 128) * (e->Np / 128);
   e->b_part_floats = (size_t)(1024 + ntile) * 128 * 128;
   const size_t n_anl = (size_t)e->Mp * e->Kp, n_c = (size_t)e->Np * e->Kp;

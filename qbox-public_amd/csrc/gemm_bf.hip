@@ -10,16 +10,19 @@
 // FP32 product; NS = 2 keeps about 16 bits.
 //
 // Same product shape as gemm.hip: D[p][q] = sum_r P(r,p) Q(r,q) on a
-// 128 x 128 tile per block of 256 threads (2 x 2 waves of 64 x 64), with
+// 128 x 128 tile per block of 256 threads (2 x 2 waves of 64 x 64), or, when
+// both output extents are multiples of 256 and NS = 2, a 256 x 256 tile per
+// block of 512 threads (k_gemm_bfb), with
 // P(r,p) = P[r + p*ldp] and Q(r,q) = Q[r + q*ldq] (TN) or Q[q + r*ldq] (NN).
-// All operand extents are padded (rows to BK, columns to 128, leading
+// All operand extents are padded (rows to BK, columns to the tile edge, leading
 // dimensions to multiples of 4) and the padding holds zeros, so loads need
 // no bounds checks; stores are bounded by the valid extents.
 //
 // LDS holds the parts as [part][row][k] with k contiguous: the MFMA lane l
 // reads row l%16 and the 8 consecutive k at 8*(l/16) as one 16-byte load
 // (both operands use the same k order, so the products pair k correctly).
-// Accumulator e of lane l is D[4(l/16) + e][l%16].
+// The Q fragment is the MFMA's first operand, so accumulator e of lane l is
+// D[p = l%16][q = 4(l/16) + e]: four consecutive q, stored as one vector.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "energy.h"
@@ -78,6 +81,7 @@ struct BfArgs {
   void* out;           // out[split*sstride + q + p*ldo]
   long ldo, sstride;
   int accumulate;
+  int vec;             // out and ldo allow the 4-wide vector stores
 };
 
 template <int NS, bool QR, typename TO>
@@ -96,9 +100,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf(BfArgs g) {
   else { iq = tt % g.tq; ip = tt / g.tq; }
   const long p0 = (long)ip * BT, q0 = (long)iq * BT;
   const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
-  // staging: r-contiguous operands as (row = tid/2, 16 k at 16*(tid&1));
+  // staging: r-contiguous operands as (row = 16 rows per 16 lanes, 16 k at
+  // 16*((tid>>4)&1)): each 16-lane group writes 16 consecutive rows at one k
+  // offset, the pattern of the fragment reads (no bank conflicts), and lanes
+  // l, l+16 read the two halves of one 128-byte row segment;
   // q-contiguous Q as (q = tid&127, 16 k at 16*(tid>>7))
-  const int srow = tid >> 1, sk = (tid & 1) * 16;
+  const int srow = (tid & 15) | ((tid >> 5) << 4), sk = ((tid >> 4) & 1) * 16;
   const int qrow = QR ? srow : (tid & 127), qk = QR ? sk : (tid >> 7) * 16;
   const float* Pp = g.P + (p0 + srow) * g.ldp + sk;
   const float* Qp = QR ? g.Q + (q0 + srow) * g.ldq + sk : g.Q + q0 + qrow + (long)qk * g.ldq;
@@ -162,32 +169,198 @@ __global__ __launch_bounds__(256) void k_gemm_bf(BfArgs g) {
         v4f c = acc[i][j];
         // smallest part products first
         if constexpr (NS == 3) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], bq[0][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], bq[1][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[2][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[2][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[1][j], a[1][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[2][j], a[0][i], c, 0, 0, 0);
         }
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], bq[0][j], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[1][j], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], bq[0][j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[1][j], a[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[0][i], c, 0, 0, 0);
         acc[i][j] = c;
       }
     __syncthreads();
   }
+  // lane: row p = .. + fr, four consecutive q from 4*(lane/16); whole
+  // vectors when all four are inside the valid extent and the output is
+  // aligned for them (g.vec: 16-byte base, ldo a multiple of 4; q is one)
   TO* o = (TO*)g.out + (long)split * g.sstride;
+  const int fq = 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const long p = p0 + wp + 16 * i + 4 * (lane >> 4) + e;
-        const long q = q0 + wq + 16 * j + fr;
-        if (p < g.np && q < g.nq) {
-          TO* d = o + q + p * g.ldo;
-          const TO v = (TO)acc[i][j][e];
-          *d = g.accumulate ? (TO)(*d + v) : v;
+    for (int j = 0; j < 4; j++) {
+      const long p = p0 + wp + 16 * i + fr;
+      const long q = q0 + wq + 16 * j + fq;
+      if (p >= g.np) continue;
+      TO* d = o + q + p * g.ldo;
+      if (g.vec && q + 3 < g.nq) {
+        if constexpr (sizeof(TO) == 4) {
+          float4 v = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+          if (g.accumulate) {
+            const float4 u = *reinterpret_cast<const float4*>(d);
+            v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
+          }
+          *reinterpret_cast<float4*>(d) = v;
+        } else {
+          double2 v0 = {(double)acc[i][j][0], (double)acc[i][j][1]};
+          double2 v1 = {(double)acc[i][j][2], (double)acc[i][j][3]};
+          if (g.accumulate) {
+            const double2 u0 = *reinterpret_cast<const double2*>(d);
+            const double2 u1 = *reinterpret_cast<const double2*>(d + 2);
+            v0.x = u0.x + v0.x; v0.y = u0.y + v0.y;
+            v1.x = u1.x + v1.x; v1.y = u1.y + v1.y;
+          }
+          *reinterpret_cast<double2*>(d) = v0;
+          *reinterpret_cast<double2*>(d + 2) = v1;
         }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (q + e < g.nq) {
+            const TO v = (TO)acc[i][j][e];
+            d[e] = g.accumulate ? (TO)(d[e] + v) : v;
+          }
       }
+    }
+}
+
+// 256 x 256 tiles for large products: 512 threads = 8 waves of 128 (p) x 64
+// (q), so each staged operand row serves twice as many outputs as in the
+// 128 x 128 kernel (half the operand traffic from L2 and the LDS writes per
+// MFMA); fragments of P are read one 16-row group at a time to keep the
+// registers of 2 waves per SIMD
+constexpr int BB = 256;
+template <int NS, bool QR, typename TO>
+__global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
+  __shared__ __align__(16) unsigned short sP[NS][BB][LR];
+  __shared__ __align__(16) unsigned short sQ[NS][BB][LR];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nblk = gridDim.x;
+  int b = blockIdx.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int ntile = g.tp * g.tq;
+  const int split = b / ntile, tt = b - split * ntile;
+  int ip, iq;
+  if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
+  else { iq = tt % g.tq; ip = tt / g.tq; }
+  const long p0 = (long)ip * BB, q0 = (long)iq * BB;
+  const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
+  const int srow = (tid & 15) | ((tid >> 5) << 4), sk = ((tid >> 4) & 1) * 16;
+  const int qrow = QR ? srow : (tid & 255), qk = QR ? sk : (tid >> 8) * 16;
+  const float* Pp = g.P + (p0 + srow) * g.ldp + sk;
+  const float* Qp = QR ? g.Q + (q0 + srow) * g.ldq + sk : g.Q + q0 + qrow + (long)qk * g.ldq;
+  float pv[16], qv[16];
+  auto load = [&](long r0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float4 t = *reinterpret_cast<const float4*>(Pp + r0 + 4 * i);
+      pv[4 * i] = t.x; pv[4 * i + 1] = t.y; pv[4 * i + 2] = t.z; pv[4 * i + 3] = t.w;
+    }
+    if (QR) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float4 t = *reinterpret_cast<const float4*>(Qp + r0 + 4 * i);
+        qv[4 * i] = t.x; qv[4 * i + 1] = t.y; qv[4 * i + 2] = t.z; qv[4 * i + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) qv[i] = Qp[(r0 + i) * g.ldq];
+    }
+  };
+  auto store = [&]() {
+    u16x8 h[NS];
+    split8<NS>(pv, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sP[s][srow][sk]) = h[s];
+    split8<NS>(pv + 8, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sP[s][srow][sk + 8]) = h[s];
+    split8<NS>(qv, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sQ[s][qrow][qk]) = h[s];
+    split8<NS>(qv + 8, h);
+#pragma unroll
+    for (int s = 0; s < NS; s++) *reinterpret_cast<u16x8*>(&sQ[s][qrow][qk + 8]) = h[s];
+  };
+  const int wp = (w & 1) * 128, wq = (w >> 1) * 64;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  v4f acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  load(rb);
+  for (long r0 = rb; r0 < re; r0 += BK) {
+    store();
+    __syncthreads();
+    if (r0 + BK < re) load(r0 + BK);
+    bf16x8 bq[NS][4];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        bq[s][j] = *reinterpret_cast<const bf16x8*>(&sQ[s][wq + 16 * j + fr][fk]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      bf16x8 a[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        a[s] = *reinterpret_cast<const bf16x8*>(&sP[s][wp + 16 * i + fr][fk]);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v4f c = acc[i][j];
+        if constexpr (NS == 3) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[2], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[1][j], a[1], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[2][j], a[0], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[1][j], a[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[0], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    }
+    __syncthreads();
+  }
+  TO* o = (TO*)g.out + (long)split * g.sstride;
+  const int fq = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const long p = p0 + wp + 16 * i + fr;
+      const long q = q0 + wq + 16 * j + fq;
+      if (p >= g.np) continue;
+      TO* d = o + q + p * g.ldo;
+      if (g.vec && q + 3 < g.nq) {
+        if constexpr (sizeof(TO) == 4) {
+          float4 v = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+          if (g.accumulate) {
+            const float4 u = *reinterpret_cast<const float4*>(d);
+            v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
+          }
+          *reinterpret_cast<float4*>(d) = v;
+        } else {
+          double2 v0 = {(double)acc[i][j][0], (double)acc[i][j][1]};
+          double2 v1 = {(double)acc[i][j][2], (double)acc[i][j][3]};
+          if (g.accumulate) {
+            const double2 u0 = *reinterpret_cast<const double2*>(d);
+            const double2 u1 = *reinterpret_cast<const double2*>(d + 2);
+            v0.x = u0.x + v0.x; v0.y = u0.y + v0.y;
+            v1.x = u1.x + v1.x; v1.y = u1.y + v1.y;
+          }
+          *reinterpret_cast<double2*>(d) = v0;
+          *reinterpret_cast<double2*>(d + 2) = v1;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (q + e < g.nq) {
+            const TO v = (TO)acc[i][j][e];
+            d[e] = g.accumulate ? (TO)(d[e] + v) : v;
+          }
+      }
+    }
 }
 
 // out[q + p*ldo] = alpha sum_s partial[s][q + p*pitch] + gamma Q[q*ldq] P[p*ldp]
@@ -207,6 +380,48 @@ __global__ __launch_bounds__(256) void k_bf_reduce(const float* __restrict__ par
   out[q + (long)p * ldo] = v;
 }
 
+// blocks of k_gemm_bf<NS, true, float> the current device runs at once
+// (CUs x resident blocks per CU), cached per device
+template <int NS>
+int tn_slots() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  if (!cache[dev]) {
+    int ncu = 0, occ = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void*>(&k_gemm_bf<NS, true, float>), 256, 0) != hipSuccess)
+      occ = 2;
+    cache[dev] = std::max(1, ncu) * std::max(1, occ);
+  }
+  return cache[dev];
+}
+
+// blocks of k_gemm_bfb<2, true, float> the current device runs at once
+int tnb_slots() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cache[dev]) {
+    int ncu = 0, occ = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void*>(&k_gemm_bfb<2, true, float>), 512, 0) != hipSuccess)
+      occ = 1;
+    cache[dev] = std::max(1, ncu) * std::max(1, occ);
+  }
+  return cache[dev];
+}
+
+template <int NS, bool QR, typename TO>
+void launchb(const BfArgs& g, int nsplit, hipStream_t s) {
+  const int nblk = g.tp * g.tq * nsplit;
+  hipLaunchKernelGGL((k_gemm_bfb<NS, QR, TO>), dim3(nblk), dim3(512), 0, s, g);
+}
+
 template <int NS, bool QR, typename TO>
 void launch(const BfArgs& g, int nsplit, hipStream_t s) {
   const int nblk = g.tp * g.tq * nsplit;
@@ -222,12 +437,17 @@ int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int N
   if (Mp % BT || Np % BT || Kp % BK || lda % 4 || ldb % 4 || M > Mp || N > Np) return -1;
   BfArgs g{};
   g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
-  g.tp = Np / BT; g.tq = Mp / BT;
+  const bool big = parts == 2 && Mp % BB == 0 && Np % BB == 0;
+  const int bt = big ? BB : BT;
+  g.tp = Np / bt; g.tq = Mp / bt;
   g.np = Np; g.nq = Mp;
   g.pfast = 0;
   const long ksteps = Kp / BK;
   const int ntile = g.tp * g.tq;
-  long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
+  // split the reduction so the blocks fill the device exactly once (a
+  // second, partial round would run at a fraction of the occupancy)
+  const int slots = big ? tnb_slots() : parts == 3 ? tn_slots<3>() : tn_slots<2>();
+  long S = std::max<long>(1, slots / ntile);
   S = std::min<long>(S, std::max<long>(1, ksteps / 8));
   while (S > 1 && (size_t)S * Mp * Np > work_floats) S--;
   if ((size_t)S * Mp * Np > work_floats) return -1;
@@ -235,7 +455,9 @@ int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int N
   g.rper = ((ksteps + S - 1) / S) * BK;
   S = (Kp + g.rper - 1) / g.rper;
   g.out = work; g.ldo = Mp; g.sstride = (long)Mp * Np; g.accumulate = 0;
-  if (parts == 3) launch<3, true, float>(g, (int)S, s);
+  g.vec = ((uintptr_t)work % 16 == 0) ? 1 : 0;
+  if (big) launchb<2, true, float>(g, (int)S, s);
+  else if (parts == 3) launch<3, true, float>(g, (int)S, s);
   else launch<2, true, float>(g, (int)S, s);
   const long tot = (long)M * N;
   hipLaunchKernelGGL(k_bf_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, work,
@@ -250,13 +472,17 @@ static int bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, in
   if (Kp % BT || Np % BT || Mp % BK || lda % 4 || ldf % 4) return -1;
   BfArgs g{};
   g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
-  g.tp = Np / BT; g.tq = (int)(Kp / BT);
+  const bool big = parts == 2 && Kp % BB == 0 && Np % BB == 0;
+  const int bt = big ? BB : BT;
+  g.tp = Np / bt; g.tq = (int)(Kp / bt);
   g.np = N; g.nq = (int)K;
   g.pfast = 1;
   g.nr = Mp;
   g.rper = Mp;
   g.out = C; g.ldo = ldc; g.sstride = 0; g.accumulate = accumulate;
-  if (parts == 3) launch<3, false, TO>(g, 1, s);
+  g.vec = ((uintptr_t)C % 16 == 0 && ldc % 4 == 0) ? 1 : 0;
+  if (big) launchb<2, false, TO>(g, 1, s);
+  else if (parts == 3) launch<3, false, TO>(g, 1, s);
   else launch<2, false, TO>(g, 1, s);
   return 0;
 }

@@ -223,3 +223,26 @@ def test_gram_not_spd(ctx, nst):
     got = dc.download()
     c[:, 0] = c[:, 0].real
     assert np.array_equal(got, c)     # coefficients untouched apart from cleanup
+
+
+def test_mixed_256_tiles(ctx, oracle_mod):
+    """256 bands and 256 projectors: the BF16-split products run on the
+    256 x 256 tiles (projections, H*psi and the force product); the same
+    bars as the 128-tile runs."""
+    import dataclasses
+    import qbg
+    s = dataclasses.replace(cf.config(1), nst=256)
+    print(f"[case] 256 tiles: {s.name} nst 256", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    c = cf.gram_numpy(cf.random_coefficients(4242, b["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, prec="bf16x2")
+    ref = o.evaluate(c, hpsi=True, forces=True)
+    for t in TERMS:
+        assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    assert np.abs(got["fion"] - ref["fion"]).max() < 1e-5
+    assert relerr(hp.download(), ref["hpsi"]) < 1e-5
+    ef.release()
