@@ -36,9 +36,8 @@ MIXED_MODES = ("fp32", "bf16x3", "bf16x2")
 DE_TOL = 1e-6      # Ha/atom, every energy term
 F_TOL = 1e-5       # Ha/bohr
 FFT_KINDS = ("zbwd", "xbwd", "yvmul", "ydens", "xfwd", "zfwd")
-# band-pair passes of each kind in one step: density (z, x backward, y
-# density) and H*psi (z, x backward, y v-multiply, x, z forward)
-PASSES_PER_STEP = {"zbwd": 2, "xbwd": 2, "ydens": 1, "yvmul": 1, "xfwd": 1, "zfwd": 1}
+# band pairs per pass launch (the library's batch)
+PAIRS_PER_BATCH = 64
 TERMS = ["ekin", "eps", "enl", "ehart", "exc", "esr", "etotal"]
 
 
@@ -47,9 +46,10 @@ def pair_bytes(ef, kind, sc):
 
     z-bwd reads two bands (2 ngw complex FP64) and writes the columns
     (nvec np2 elements of sc bytes); x-bwd reads the columns and writes the
-    active rows A = 2 ntrans0 np0 np2; the y pass reads and writes A and
-    reads v (N doubles); x-fwd and z-fwd mirror the backward passes, z-fwd
-    read-modify-writes the two H*psi bands.
+    active rows A = 2 ntrans0 np0 np2; the y pass reads and writes A (v(r),
+    shared by the pairs, is counted once per launch in fft_roofline); x-fwd
+    and z-fwd mirror the backward passes, z-fwd read-modify-writes the two
+    H*psi bands.
     """
     ngw, nvec, np2 = ef.ngw, ef.nvec, ef.np2
     cols = nvec * np2 * sc
@@ -57,7 +57,7 @@ def pair_bytes(ef, kind, sc):
     return {
         "zbwd": 2 * ngw * 16 + cols,
         "xbwd": cols + A,
-        "yvmul": 2 * A + ef.N * 8,
+        "yvmul": 2 * A,
         "ydens": A,
         "xfwd": A + cols,
         "zfwd": cols + 2 * 2 * ngw * 16,
@@ -68,10 +68,12 @@ def fft_roofline(ef, kind, kt, steps, nst, sc):
     tot_ms, nlaunch = kt[kind]
     launches_per_step = nlaunch / steps
     npairs = (nst + 1) // 2
-    pairs_per_launch = npairs * PASSES_PER_STEP[kind] / launches_per_step
+    pairs_per_launch = npairs / -(-npairs // PAIRS_PER_BATCH)
     per_launch = pair_bytes(ef, kind, sc) * pairs_per_launch
     if kind == "ydens":
         per_launch += 2 * ef.N * 8                 # rho read-modify-write
+    if kind == "yvmul":
+        per_launch += ef.N * 8                     # v(r), read once per launch
     avg_ms = tot_ms / nlaunch
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1),
