@@ -163,7 +163,8 @@ void qbo_update_vhxc(void* h, int stress) { ((System*)h)->update_vhxc(stress != 
 // arrays: 0 rhor, 1 v_r, 2 vxc_r (N doubles); 3 rhog, 4 vion_local_g,
 // 5 rhopst (ngv complex); 6 vps, 7 rhops, 8 dvps of species `is` (ngv);
 // 9 twnl of species is (npr*ngw); 10 sfac of species is (ngv complex);
-// 11 fion_esr (3 nat); 12 dtwnl (npr*6*ngw); 13 dvion_local_g
+// 11 fion_esr (3 nat); 12 dtwnl (npr*6*ngw); 13 dvion_local_g;
+// 14 rhor, 15 v_r, 16 vxc_r of spin down (nspin 2; arrays 0-2 are spin up)
 void qbo_get(void* h, int what, int is, double* out) {
   System* s = (System*)h;
   auto cpy = [&](const void* p, size_t bytes) { std::memcpy(out, p, bytes); };
@@ -182,7 +183,25 @@ void qbo_get(void* h, int what, int is, double* out) {
     case 11: cpy(s->fion_esr.data(), 8 * s->fion_esr.size()); break;
     case 12: cpy(s->dtwnl[is].data(), 8 * s->dtwnl[is].size()); break;
     case 13: cpy(s->dvion_local_g.data(), 16 * s->dvion_local_g.size()); break;
+    case 14: cpy(s->rhor_dn.data(), 8 * s->rhor_dn.size()); break;
+    case 15: cpy(s->v_r_dn.data(), 8 * s->v_r_dn.size()); break;
+    case 16: cpy(s->vxc_r_dn.data(), 8 * s->vxc_r_dn.size()); break;
   }
+}
+
+// nspin 2 with bands [0, nst_up) up (Wavefunction::set_nspin)
+int qbo_set_nspin(void* h, int nspin, int nst_up) {
+  System* s = (System*)h;
+  if (nspin == 1) { s->nspin = 1; s->nst_up = 0; return 0; }
+  if (nspin != 2 || nst_up < 0 || nst_up > s->nst) return -1;
+  s->nspin = 2;
+  s->nst_up = nst_up;
+  return 0;
+}
+
+void qbo_lsda(int n, const double* rup, const double* rdn, double* e, double* vup,
+              double* vdn) {
+  for (int i = 0; i < n; i++) lsda(rup[i], rdn[i], e[i], vup[i], vdn[i]);
 }
 void qbo_set_vr(void* h, const double* v) {
   System* s = (System*)h;

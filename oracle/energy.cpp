@@ -31,6 +31,9 @@ void System::setup() {
   rhor.assign(N, 0.0);
   v_r.assign(N, 0.0);
   vxc_r.assign(N, 0.0);
+  rhor_dn.assign(N, 0.0);
+  v_r_dn.assign(N, 0.0);
+  vxc_r_dn.assign(N, 0.0);
   rhog.assign(vb.size, 0.0);
   rhogt.assign(vb.size, 0.0);
   rhoelg.assign(vb.size, 0.0);
@@ -367,16 +370,24 @@ int gram(cplx* c, int ngw, int nst) {
 
 // ---------------------------------------------------------- ChargeDensity
 // src/ChargeDensity.cpp:136-198
+// (nspin 2: one density per spin from its bands; rhog is the transform of the
+// total, rhog[0] + rhog[1] of src/EnergyFunctional.cpp:256-261)
 void System::update_density(const cplx* c) {
   std::fill(rhor.begin(), rhor.end(), 0.0);
-  compute_density(ft, c, wb.size, nst, occ.data(), 1.0, cell.volume, rhor.data());
+  std::fill(rhor_dn.begin(), rhor_dn.end(), 0.0);
+  const int nup = nspin == 2 ? nst_up : nst;
+  compute_density(ft, c, wb.size, nup, occ.data(), 1.0, cell.volume, rhor.data());
+  if (nspin == 2 && nst > nup)
+    compute_density(ft, c + (size_t)nup * wb.size, wb.size, nst - nup, occ.data() + nup, 1.0,
+                    cell.volume, rhor_dn.data());
   const long N = ft.np012();
   const double omega = cell.volume;
   std::vector<cplx> rhotmp(N);
   double sum = 0.0;
   for (long i = 0; i < N; i++) {
-    sum += rhor[i];
-    rhotmp[i] = cplx(omega * rhor[i], 0.0);
+    const double r = nspin == 2 ? rhor[i] + rhor_dn[i] : rhor[i];
+    sum += r;
+    rhotmp[i] = cplx(omega * r, 0.0);
   }
   total_charge = sum * omega / (double)N;
   vft.forward(rhotmp.data(), rhog.data());
@@ -388,14 +399,26 @@ void System::update_vhxc(bool compute_stress) {
   const double omega = cell.volume, omega_inv = 1.0 / omega;
   const long N = vft.np012();
   for (int ig = 0; ig < ngv; ig++) rhoelg[ig] = omega_inv * rhog[ig];
-  // LDA
+  // LDA, or LSDA for nspin 2 (XCPotential.cpp:181-226)
   double esum = 0.0, dsum = 0.0;
-  for (long i = 0; i < N; i++) {
-    double e, v;
-    lda_unpolarized(rhor[i], e, v);
-    esum += rhor[i] * e;
-    dsum += rhor[i] * (e - v);
-    vxc_r[i] = v;
+  if (nspin == 2) {
+    for (long i = 0; i < N; i++) {
+      double e, vu, vd;
+      lsda(rhor[i], rhor_dn[i], e, vu, vd);
+      const double r = rhor[i] + rhor_dn[i];
+      esum += r * e;
+      dsum += r * e - rhor[i] * vu - rhor_dn[i] * vd;
+      vxc_r[i] = vu;
+      vxc_r_dn[i] = vd;
+    }
+  } else {
+    for (long i = 0; i < N; i++) {
+      double e, v;
+      lda_unpolarized(rhor[i], e, v);
+      esum += rhor[i] * e;
+      dsum += rhor[i] * (e - v);
+      vxc_r[i] = v;
+    }
   }
   exc = esum * omega / (double)N;
   dxc = dsum * omega / (double)N;
@@ -429,6 +452,8 @@ void System::update_vhxc(bool compute_stress) {
   for (int ig = 0; ig < ngv; ig++) vlocal_g[ig] = vion_local_g[ig] + fpi * rhogt[ig] * g2i[ig];
   vft.backward(vlocal_g.data(), tmp_r.data());
   for (long i = 0; i < N; i++) v_r[i] = vxc_r[i] + tmp_r[i].real();
+  if (nspin == 2)                               // :383-389
+    for (long i = 0; i < N; i++) v_r_dn[i] = vxc_r_dn[i] + tmp_r[i].real();
 }
 
 // ---------------------------------------------------- blocked GEMM helpers
@@ -699,7 +724,12 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
     for (int n = 0; n < nst; n++)
       for (int ig = 0; ig < ngw; ig++)
         hpsi[ig + (size_t)n * ngw] += 0.5 * wb.g2[ig] * c[ig + (size_t)n * ngw];
-    rs_mul_add(ft, v_r.data(), c, ngw, nst, hpsi);
+    // rs_mul_add with the potential of each band's spin (:831)
+    const int nup = nspin == 2 ? nst_up : nst;
+    rs_mul_add(ft, v_r.data(), c, ngw, nup, hpsi);
+    if (nspin == 2 && nst > nup)
+      rs_mul_add(ft, v_r_dn.data(), c + (size_t)nup * ngw, ngw, nst - nup,
+                 hpsi + (size_t)nup * ngw);
   }
   // local forces (:841-930), eigr as products of per-axis tables
   if (compute_forces) {

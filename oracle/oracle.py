@@ -129,6 +129,9 @@ class Oracle:
          self.nvec_v, self.ntrans0_v, self.nrods, self.nrods_v, self.ndft) = [int(x) for x in s]
         self.N = self.np0 * self.np1 * self.np2
         self.nst = sysdef.nst
+        if getattr(sysdef, "nspin", 1) == 2:
+            if L.qbo_set_nspin(self.h, 2, int(sysdef.nst_up)) != 0:
+                raise RuntimeError("bad nspin / nst_up")
 
     def __del__(self):
         try:
@@ -215,7 +218,8 @@ class Oracle:
                  6: (self.ngv, float), 7: (self.ngv, float), 8: (self.ngv, float),
                  9: (npr * self.ngw, float), 10: (self.ngv, complex),
                  11: (3 * self.sysdef.nat, float), 12: (npr * 6 * self.ngw, float),
-                 13: (self.ngv, complex)}
+                 13: (self.ngv, complex), 14: (self.N, float), 15: (self.N, float),
+                 16: (self.N, float)}
         n, t = sizes[what]
         out = np.zeros(n, dtype=np.complex128 if t is complex else np.float64)
         lib().qbo_get(self.h, what, isp, out.ctypes.data_as(_dp))
@@ -417,6 +421,15 @@ def lda(rho):
     e, v = np.zeros_like(rho), np.zeros_like(rho)
     lib().qbo_lda(rho.size, _ptr(rho), _ptr(e), _ptr(v))
     return e, v
+
+
+def lsda(rup, rdn):
+    """Spin-polarized LDA per point: (exc, v_up, v_dn)."""
+    rup = np.ascontiguousarray(rup, dtype=float)
+    rdn = np.ascontiguousarray(rdn, dtype=float)
+    e, vu, vd = np.zeros_like(rup), np.zeros_like(rup), np.zeros_like(rup)
+    lib().qbo_lsda(rup.size, _ptr(rup), _ptr(rdn), _ptr(e), _ptr(vu), _ptr(vd))
+    return e, vu, vd
 
 
 def sinft(y):

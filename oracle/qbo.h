@@ -109,6 +109,10 @@ void ncpp_species(Species& q, int lmax, int llocal, int nmesh, const double* vps
 
 // LDA (Perdew-Zunger/Ceperley-Alder), src/LDAFunctional.cpp:92-151
 void lda_unpolarized(double rh, double& ee, double& vv);
+// fully polarized fit (src/LDAFunctional.cpp:153-213) and the spin
+// interpolation (:42-89)
+void lda_ferro(double rh, double& ee, double& vv);
+void lsda(double rup, double rdn, double& ee, double& vup, double& vdn);
 
 // --------------------------------------------------------- whole evaluation
 struct Results {
@@ -128,6 +132,9 @@ struct System {
   std::vector<std::vector<double>> tau;   // tau[is][3*ia]
   int nst = 0;
   std::vector<double> occ;
+  // nspin 2: bands [0, nst_up) spin up, [nst_up, nst) spin down
+  // (Wavefunction nst_[ispin], src/Wavefunction.cpp:326-336)
+  int nspin = 1, nst_up = 0;
   // wavefunction and density bases and grids
   Basis wb, vb;
   FFT3 ft;      // wf basis on the density grid (ChargeDensity ft_[0])
@@ -141,9 +148,10 @@ struct System {
   std::vector<double> fion_esr; double esr = 0.0; double sigma_esr[6];
   double eself_total = 0.0;
   // state
-  std::vector<double> rhor;           // density, real space
-  std::vector<cplx> rhog;             // density G
-  std::vector<double> v_r, vxc_r;
+  std::vector<double> rhor;           // density, real space (spin up if nspin 2)
+  std::vector<cplx> rhog;             // total density G
+  std::vector<double> v_r, vxc_r;     // (spin up if nspin 2)
+  std::vector<double> rhor_dn, v_r_dn, vxc_r_dn;   // spin down (nspin 2)
   std::vector<cplx> rhoelg, rhogt;
   double exc = 0, dxc = 0, eps = 0, ehart = 0, ehart_e = 0, ehart_ep = 0,
          ehart_p = 0, total_charge = 0, sigma_exc[6];

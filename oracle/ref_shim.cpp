@@ -25,6 +25,16 @@ void ref_lda(int n, const double* rho, double* e, double* v) {
   for (int i = 0; i < n; i++) { e[i] = f.exc[i]; v[i] = f.vxc1[i]; }
 }
 
+void ref_lsda(int n, const double* rup, const double* rdn, double* e, double* vup,
+              double* vdn) {
+  std::vector<std::vector<double> > rhoe(2);
+  rhoe[0].assign(rup, rup + n);
+  rhoe[1].assign(rdn, rdn + n);
+  LDAFunctional f(rhoe);
+  f.setxc();
+  for (int i = 0; i < n; i++) { e[i] = f.exc[i]; vup[i] = f.vxc1_up[i]; vdn[i] = f.vxc1_dn[i]; }
+}
+
 void ref_cell(const double* a9, double* b9, double* vol) {
   UnitCell c(D3vector(a9[0], a9[1], a9[2]), D3vector(a9[3], a9[4], a9[5]),
              D3vector(a9[6], a9[7], a9[8]));

@@ -276,6 +276,59 @@ void lda_unpolarized(double rh, double& ee, double& vv) {
   }
 }
 
+// Perdew-Zunger fit of the fully polarized electron gas (exchange constant
+// 2^(1/3) times the paramagnetic one, the polarized correlation parameters;
+// C and D from continuity at rs = 1), src/LDAFunctional.cpp:153-213
+void lda_ferro(double rh, double& ee, double& vv) {
+  const double c1 = 0.6203504908994001;
+  const double cx = -0.769669463118;
+  const double A = 0.01555, B = -0.0269, b1 = 1.3981, b2 = 0.2611, G = -0.0843;
+  const double D = G / (1.0 + b1 + b2) - B;
+  const double C = -A - D - G * ((b1 / 2.0 + b2) / ((1.0 + b1 + b2) * (1.0 + b1 + b2)));
+  ee = 0.0;
+  vv = 0.0;
+  if (!(rh > 0.0)) return;
+  const double rs = c1 / std::cbrt(rh);
+  const double vx = cx / rs, ex = 0.75 * vx;
+  double ec, vc;
+  if (rs < 1.0) {
+    const double l = std::log(rs);
+    ec = A * l + B + C * rs * l + D * rs;
+    vc = A * l + (B - A / 3.0) + (2.0 / 3.0) * C * rs * l + ((2.0 * D - C) / 3.0) * rs;
+  } else {
+    const double q = std::sqrt(rs);
+    const double den = 1.0 + b1 * q + b2 * rs;
+    ec = G / den;
+    vc = ec * (1.0 + (7.0 / 6.0) * b1 * q + (4.0 / 3.0) * b2 * rs) / den;
+  }
+  ee = ex + ec;
+  vv = vx + vc;
+}
+
+// spin-polarized LDA (src/LDAFunctional.cpp:42-89): energy per electron and
+// the two spin potentials by von Barth-Hedin interpolation between the
+// paramagnetic and ferromagnetic fits, f(z) = ((1+z)^(4/3) + (1-z)^(4/3) - 2)
+// / (2^(4/3) - 2); zero where the total density is not positive
+void lsda(double rup, double rdn, double& ee, double& vup, double& vdn) {
+  ee = vup = vdn = 0.0;
+  const double r = rup + rdn;
+  if (!(r > 0.0)) return;
+  const double kf = 1.0 / (2.0 * std::cbrt(2.0) - 2.0);
+  const double z = (rup - rdn) / r;
+  const double a = 1.0 + z, b = 1.0 - z;
+  const double a3 = std::cbrt(a), b3 = std::cbrt(b);
+  const double f = kf * (a3 * a + b3 * b - 2.0);
+  const double df = (4.0 / 3.0) * kf * (a3 - b3);
+  double eu, vu, ep, vp;
+  lda_unpolarized(r, eu, vu);
+  lda_ferro(r, ep, vp);
+  const double de = ep - eu;
+  ee = eu + f * de;
+  const double v = vu + f * (vp - vu);
+  vup = v + de * (1.0 - z) * df;
+  vdn = v + de * (-1.0 - z) * df;
+}
+
 // Species::initialize_ncpp, Kleinman-Bylander branch (src/Species.cpp:71-330,
 // nquad = 0), restated as the equivalent one-projector-per-l species: local
 // potential vps[llocal]; nonlocal l: projector dv_l phi_l with D_l = w_l,

@@ -69,6 +69,12 @@ class SystemDef:
     seed: int
     min_sep: float
     tau: np.ndarray = field(default=None)   # (nat, 3), species-major
+    # nspin 2: bands [0, nst_up) are spin up, the rest spin down; occupations
+    # per band (at most 1 with two spins), Wavefunction nst_[ispin] and
+    # SlaterDet occ_ (src/Wavefunction.cpp:326-336, 424-447)
+    nspin: int = 1
+    nst_up: int = 0
+    occ_bands: np.ndarray = field(default=None)
 
     @property
     def ecut(self) -> float:         # Hartree (src/Ecut.h:38-57)
@@ -80,6 +86,8 @@ class SystemDef:
 
     @property
     def occ(self) -> np.ndarray:
+        if self.occ_bands is not None:
+            return np.asarray(self.occ_bands, dtype=float)
         return np.full(self.nst, 2.0)
 
     def build_positions(self) -> "SystemDef":
@@ -206,3 +214,21 @@ def h2ogs(o, h, ecut_ry=70.0) -> SystemDef:
     tau = np.array([[0.0, 0.0, 0.0], [1.128, 1.444, 0.0], [1.128, -1.444, 0.0]])
     return SystemDef(f"h2ogs: H2O L=16 ecut={ecut_ry:g}Ry 4 bands", cubic(16.0), ecut_ry, [o, h],
                      [1, 2], 4, 0, 0.0, tau=tau)
+
+
+def spin_polarized(s: SystemDef, nel: int, delta_spin: int = 1, nempty: int = 0) -> SystemDef:
+    """The nspin = 2 form of a system with nel electrons: nst_up = (nel+1)/2 +
+    delta_spin + nempty bands up and nel/2 - delta_spin + nempty down, the
+    lowest (nel+1)/2 + delta_spin up and nel/2 - delta_spin down occupied
+    with 1 (Wavefunction::set_nspin / update_occ, src/Wavefunction.cpp:326-336,
+    424-447)."""
+    import dataclasses
+    nup_occ = (nel + 1) // 2 + delta_spin
+    ndn_occ = nel // 2 - delta_spin
+    nst_up = nup_occ + nempty
+    nst_dn = ndn_occ + nempty
+    occ = np.zeros(nst_up + nst_dn)
+    occ[:nup_occ] = 1.0
+    occ[nst_up:nst_up + ndn_occ] = 1.0
+    return dataclasses.replace(s, name=s.name + f" nspin 2 (up {nst_up}, down {nst_dn})",
+                               nst=nst_up + nst_dn, nspin=2, nst_up=nst_up, occ_bands=occ)

@@ -1,0 +1,248 @@
+"""Pinning the oracle's energy functional (a6-a16) independently of its own
+formulas:
+
+  - LDA, spline/splintd and UnitCell against the reference's own sources
+    compiled into oracle/_ref (src/LDAFunctional.cpp, src/spline.cpp,
+    src/UnitCell.cpp), and sinft/cosft1 against direct trigonometric sums;
+  - the spin-polarized LDA against the reference's LDAFunctional with two
+    densities, against its own derivative (v_s = d(rho e)/d rho_s by central
+    differences) and its zeta = 0 reduction;
+  - H*psi is the gradient of the total energy: dE/de along a direction d
+    equals sum_n occ_n 2 <d_n|H psi_n> (full-sphere inner product), by
+    central differences of complete evaluations;
+  - forces are -dE/dtau (central differences at fixed coefficients), the
+    short-range ion forces sum to zero (Newton's third law), the total
+    charge is the sum of the occupations;
+  - nspin 2: identical up and down band sets with occupation 1 reproduce
+    nspin 1 with occupation 2; swapping the spin sets leaves the energy
+    unchanged; the gradient and force identities hold per spin."""
+import ctypes as C
+import dataclasses
+
+import numpy as np
+import pytest
+
+from qbg import configs as cf
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _need_ref(oracle_mod):
+    r = oracle_mod.reflib()
+    if r is None:
+        pytest.skip("reference sources not compiled (oracle/_ref)")
+    return r
+
+
+def full_dot(a, b):
+    """Re sum over the full sphere of conj(a) b for half-sphere (Gamma)
+    coefficients of real functions: 2 Re sum_half - Re(a0 b0)."""
+    return 2.0 * np.real(np.vdot(a, b)) - np.real(np.conj(a[0]) * b[0])
+
+
+# ------------------------------------------------------------ reference
+def test_lda_matches_reference(oracle_mod):
+    ref = _need_ref(oracle_mod)
+    rng = np.random.default_rng(1)
+    rho = np.concatenate([10 ** rng.uniform(-8, 1.5, 4000), [0.0, 1e-30, 0.2387]])
+    e, v = oracle_mod.lda(rho)
+    er, vr = np.zeros_like(rho), np.zeros_like(rho)
+    ref.ref_lda(C.c_int(rho.size), _p(rho), _p(er), _p(vr))
+    assert np.allclose(e, er, rtol=1e-13, atol=0)
+    assert np.allclose(v, vr, rtol=1e-13, atol=0)
+
+
+def test_lsda_matches_reference(oracle_mod):
+    ref = _need_ref(oracle_mod)
+    rng = np.random.default_rng(2)
+    n = 4000
+    r = 10 ** rng.uniform(-8, 1.5, n)
+    z = rng.uniform(-1, 1, n)
+    z[:20] = 1.0
+    z[20:40] = -1.0
+    rup, rdn = 0.5 * r * (1 + z), 0.5 * r * (1 - z)
+    rup[40], rdn[40] = 0.0, 0.0
+    e, vu, vd = oracle_mod.lsda(rup, rdn)
+    er, vur, vdr = np.zeros(n), np.zeros(n), np.zeros(n)
+    ref.ref_lsda(C.c_int(n), _p(rup), _p(rdn), _p(er), _p(vur), _p(vdr))
+    for a, b in ((e, er), (vu, vur), (vd, vdr)):
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def test_lsda_derivatives_and_limits(oracle_mod):
+    rng = np.random.default_rng(3)
+    n = 200
+    r = 10 ** rng.uniform(-4, 1, n)
+    z = rng.uniform(-0.95, 0.95, n)
+    rup, rdn = 0.5 * r * (1 + z), 0.5 * r * (1 - z)
+    e, vu, vd = oracle_mod.lsda(rup, rdn)
+    for which, v in ((0, vu), (1, vd)):
+        h = 1e-6 * r
+        d = [h if which == 0 else 0 * h, h if which == 1 else 0 * h]
+        ep, _, _ = oracle_mod.lsda(rup + d[0], rdn + d[1])
+        em, _, _ = oracle_mod.lsda(rup - d[0], rdn - d[1])
+        fd = ((r + h) * ep - (r - h) * em) / (2 * h)
+        assert np.allclose(v, fd, rtol=2e-6, atol=1e-9), which
+    # unpolarized limit is the LDA exactly
+    e0, vu0, vd0 = oracle_mod.lsda(r / 2, r / 2)
+    el, vl = oracle_mod.lda(r)
+    assert np.array_equal(e0, el) and np.array_equal(vu0, vl) and np.array_equal(vd0, vl)
+    # fully polarized exchange is 2^(1/3) times the paramagnetic exchange:
+    # at high density (rs -> 0) e_x dominates, e(zeta=1)/e(zeta=0) -> 2^(1/3)
+    rr = np.array([1e6])
+    e1, _, _ = oracle_mod.lsda(rr, 0 * rr)
+    ex, _ = oracle_mod.lda(rr)
+    assert abs(e1[0] / ex[0] - 2 ** (1 / 3)) < 2e-2
+
+
+def test_spline_matches_reference(oracle_mod):
+    ref = _need_ref(oracle_mod)
+    rng = np.random.default_rng(4)
+    x = np.cumsum(rng.uniform(0.05, 0.2, 60))
+    y = np.sin(x) * np.exp(-0.1 * x)
+    for bl, br, yp1, ypn in ((0, 0, 0.3, -0.2), (1, 1, 0.0, 0.0), (0, 1, 1.0, 0.0)):
+        y2 = oracle_mod.spline(x, y, yp1, ypn, bl, br)
+        y2r = np.zeros_like(x)
+        ref.ref_spline(C.c_int(x.size), _p(x.copy()), _p(y.copy()), C.c_double(yp1),
+                       C.c_double(ypn), C.c_int(bl), C.c_int(br), _p(y2r))
+        assert np.allclose(y2, y2r, rtol=1e-12, atol=1e-14)
+        for t in rng.uniform(x[0], x[-1], 20):
+            a, da = oracle_mod.splintd(x, y, y2, t)
+            yy, dy = C.c_double(), C.c_double()
+            ref.ref_splintd(C.c_int(x.size), _p(x.copy()), _p(y.copy()), _p(y2r.copy()),
+                            C.c_double(t), C.byref(yy), C.byref(dy))
+            assert abs(a - yy.value) < 1e-13 and abs(da - dy.value) < 1e-12
+
+
+def test_unit_cell_matches_reference(oracle_mod):
+    ref = _need_ref(oracle_mod)
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        a = np.diag(rng.uniform(6, 12, 3)) + rng.uniform(-1.5, 1.5, (3, 3))
+        b, vol = oracle_mod.cell(a)
+        br, volr = np.zeros(9), C.c_double()
+        ref.ref_cell(_p(np.ascontiguousarray(a.ravel())), _p(br), C.byref(volr))
+        assert np.allclose(b.ravel(), br, rtol=1e-13, atol=1e-14)
+        assert abs(vol - volr.value) < 1e-10 * abs(vol)
+        # b = 2 pi A^-T
+        assert np.allclose(b, 2 * np.pi * np.linalg.inv(a).T, rtol=1e-12)
+        v = rng.uniform(-20, 20, 3)
+        w = oracle_mod.fold_in_ws(a, v)
+        wr = v.copy()
+        ref.ref_fold_in_ws(_p(np.ascontiguousarray(a.ravel())), _p(wr))
+        assert np.allclose(w, wr, atol=1e-12)
+
+
+def test_sine_cosine_transforms_direct(oracle_mod):
+    n = 64
+    rng = np.random.default_rng(6)
+    y = rng.standard_normal(n)
+    y[0] = 0.0
+    s = oracle_mod.sinft(y)
+    k = np.arange(n)
+    direct = np.array([np.sum(y * np.sin(np.pi * j * k / n)) for j in range(n)])
+    assert np.allclose(s, direct, atol=1e-12)
+    yc = rng.standard_normal(n + 1)
+    c = oracle_mod.cosft1(yc)
+    kk = np.arange(1, n)
+    dc = np.array([0.5 * (yc[0] + (-1) ** j * yc[n]) + np.sum(yc[1:n] * np.cos(np.pi * j * kk / n))
+                   for j in range(n)])
+    assert np.allclose(c[:n], dc, atol=1e-12)
+
+
+# ------------------------------------------------------- energy identities
+def _system(idx, spin=None):
+    s = cf.config(idx)
+    if spin is not None:
+        nel, dspin, nempty = spin
+        s = cf.spin_polarized(s, nel, dspin, nempty)
+    return s
+
+
+def _coeffs(oracle_mod, s, seed):
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    c = cf.random_coefficients(seed, b["g2"], s.nst)
+    if getattr(s, "nspin", 1) == 2:
+        u = s.nst_up
+        c = np.concatenate([cf.gram_numpy(c[:u]), cf.gram_numpy(c[u:])])
+    else:
+        c = cf.gram_numpy(c)
+    return o, c
+
+
+CASES = [(11, None), (12, None), (11, (9, 1, 1)), (10, (5, 0, 1))]
+
+
+@pytest.mark.parametrize("idx,spin", CASES)
+def test_hpsi_is_energy_gradient(oracle_mod, idx, spin):
+    s = _system(idx, spin)
+    o, c = _coeffs(oracle_mod, s, 300 + idx)
+    r = o.evaluate(c, hpsi=True, forces=False)
+    assert abs(r["total_charge"] - s.occ.sum()) < 1e-9
+    rng = np.random.default_rng(7)
+    d = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    d[:, 0] = d[:, 0].real
+    d *= 1e-4 / np.abs(d).max()
+    pred = sum(s.occ[n] * 2.0 * full_dot(d[n], r["hpsi"][n]) for n in range(s.nst))
+    h = 1.0
+    ep = o.evaluate(c + h * d, hpsi=False, forces=False)["etotal"]
+    em = o.evaluate(c - h * d, hpsi=False, forces=False)["etotal"]
+    fd = (ep - em) / (2 * h)
+    print(f"  dE/de fd {fd:.10e} pred {pred:.10e}")
+    assert abs(fd - pred) <= 1e-6 * abs(pred)
+
+
+@pytest.mark.parametrize("idx,spin", CASES)
+def test_forces_are_energy_gradient(oracle_mod, idx, spin):
+    s = _system(idx, spin)
+    o, c = _coeffs(oracle_mod, s, 400 + idx)
+    r = o.evaluate(c, hpsi=False, forces=True)
+    assert np.abs(o.get(11).reshape(-1, 3).sum(axis=0)).max() < 1e-10   # esr pairs
+    h = 1e-4
+    for a, j in ((0, 0), (1, 2), (s.nat - 1, 1)):
+        e = []
+        for sgn in (1.0, -1.0):
+            t = np.array(s.tau, dtype=float)
+            t[a, j] += sgn * h
+            o2 = oracle_mod.Oracle(dataclasses.replace(s, tau=t))
+            e.append(o2.evaluate(c, hpsi=False, forces=False)["etotal"])
+        fd = -(e[0] - e[1]) / (2 * h)
+        print(f"  f[{a},{j}] fd {fd:.10e} oracle {r['fion'][a, j]:.10e}")
+        assert abs(fd - r["fion"][a, j]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_spin_identical_sets_equal_unpolarized(oracle_mod):
+    s1 = cf.config(11)
+    o1, c = _coeffs(oracle_mod, s1, 500)
+    r1 = o1.evaluate(c, hpsi=True, forces=True, stress=True)
+    s2 = dataclasses.replace(s1, nst=2 * s1.nst, nspin=2, nst_up=s1.nst,
+                             occ_bands=np.ones(2 * s1.nst))
+    o2 = oracle_mod.Oracle(s2)
+    r2 = o2.evaluate(np.concatenate([c, c]), hpsi=True, forces=True, stress=True)
+    for t in ["ekin", "eps", "enl", "ehart", "exc", "dxc", "etotal", "total_charge"]:
+        assert abs(r2[t] - r1[t]) <= 1e-11 * max(1.0, abs(r1[t])), t
+    assert np.allclose(r2["fion"], r1["fion"], rtol=0, atol=1e-11)
+    assert np.allclose(r2["sigma"], r1["sigma"], rtol=0, atol=1e-11)
+    n = s1.nst
+    assert np.allclose(r2["hpsi"][:n], r1["hpsi"], rtol=0, atol=1e-11)
+    assert np.allclose(r2["hpsi"][n:], r1["hpsi"], rtol=0, atol=1e-11)
+    assert np.allclose(o2.get(0), o2.get(14), rtol=0, atol=1e-14)        # rho up = down
+
+
+def test_spin_swap_symmetry(oracle_mod):
+    s = _system(11, (9, 1, 1))
+    o, c = _coeffs(oracle_mod, s, 600)
+    r = o.evaluate(c, hpsi=True, forces=True)
+    u = s.nst_up
+    occ = s.occ
+    sw = dataclasses.replace(s, nst_up=s.nst - u, occ_bands=np.concatenate([occ[u:], occ[:u]]))
+    osw = oracle_mod.Oracle(sw)
+    rsw = osw.evaluate(np.concatenate([c[u:], c[:u]]), hpsi=True, forces=True)
+    assert abs(rsw["etotal"] - r["etotal"]) <= 1e-11 * abs(r["etotal"])
+    assert np.allclose(rsw["fion"], r["fion"], atol=1e-11)
+    assert np.allclose(rsw["hpsi"][s.nst - u:], r["hpsi"][:u], atol=1e-11)
+    # the polarized system differs from its unpolarized density's LDA
+    assert np.abs(o.get(0) - o.get(14)).max() > 1e-3
