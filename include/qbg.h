@@ -209,6 +209,19 @@ int qbg_ef_update_vhxc(qbg_ef* ef, int compute_stress);
 int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
                   double* hpsi, int ldh, int compute_forces, int compute_stress,
                   int prec, double* energies, double* fion, double* sigma);
+/* Spin polarization: Wavefunction::set_nspin (src/Wavefunction.cpp:365-390)
+ * with the coefficient columns [0, nst_up) spin up and [nst_up, nst) spin
+ * down (Qbox keeps one SlaterDet per spin; here they are column ranges of one
+ * array), occupations as given at creation (at most 1 per band with two
+ * spins).  update_density forms one density per spin (ChargeDensity
+ * rhor[ispin]), update_vhxc the spin-polarized LDA and v_r[ispin]
+ * (src/EnergyFunctional.cpp:248-389, src/LDAFunctional.cpp:42-89), energy
+ * applies each band's spin potential.  nspin 1 restores the unpolarized
+ * functional.  The wavefunction steppers and the mixer take nspin 1 only.
+ * qbg_ef_get: what 0-2 are the spin-up arrays, 8 rhor, 9 v_r, 10 vxc_r of
+ * spin down. */
+int qbg_ef_set_nspin(qbg_ef* ef, int nspin, int nst_up);
+
 /* One SCF-iteration evaluation without host synchronization, so that it can
  * be captured in a graph: update_density -> update_vhxc -> energy
  * (src/BOSampleStepper.cpp:1207-1217) with the flags below; the results are
@@ -304,7 +317,8 @@ int qbg_mixer_update(qbg_mixer* mx);
 int qbg_polar(qbg_ctx* ctx, double* x, int n, double tol, int maxiter, int* iters);
 
 /* host copy of a device array: 0 rhor, 1 v_r, 2 vxc_r, 3 rhog,
- * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr */
+ * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr; with nspin 2, 0-2 are the
+ * spin-up arrays and 8 rhor, 9 v_r, 10 vxc_r the spin-down ones */
 int qbg_ef_get(qbg_ef* ef, int what, double* host_out);
 /* replace v_r (host array of np012 doubles) */
 int qbg_ef_set_vr(qbg_ef* ef, const double* host_v);

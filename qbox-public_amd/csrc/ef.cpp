@@ -47,6 +47,28 @@ int dput(qbg_ctx* c, T** p, const std::vector<T>& h) {
 }  // namespace
 
 
+// band range and first pair slot of spin s (pairs never straddle the spins)
+static void spin_bands(const qbg_ef* e, int s, int& b0, int& nb, int& pair0) {
+  const int nup = e->nspin == 2 ? e->nst_up : e->nst;
+  b0 = s == 0 ? 0 : nup;
+  nb = s == 0 ? nup : e->nst - nup;
+  pair0 = s == 0 ? 0 : (nup + 1) / 2;
+}
+
+// density factors occ_n / omega in pair order, spin by spin (a missing
+// second band of a spin's last pair contributes 0)
+static std::vector<double> density_factors(const qbg_ef* e) {
+  std::vector<double> f;
+  for (int s = 0; s < e->nspin; s++) {
+    int b0, nb, p0;
+    spin_bands(e, s, b0, nb, p0);
+    const size_t base = f.size();
+    f.resize(base + 2 * (size_t)((nb + 1) / 2), 0.0);
+    for (int n = 0; n < nb; n++) f[base + n] = e->occ[b0 + n] / e->omega;
+  }
+  return f;
+}
+
 static void ef_free(qbg_ef* e) {
   void* p[] = {e->d_g2w, e->d_gxw, e->d_idxv, e->d_g2iv, e->d_giv, e->d_gxv, e->d_vps, e->d_dvps,
                e->d_rhops, e->d_twnl, e->d_tau, e->d_b, e->d_sftab, e->d_sfac, e->d_vion,
@@ -57,7 +79,8 @@ static void ef_free(qbg_ef* e) {
                e->d_a_m0, e->d_a_npr, e->d_a_stride, e->d_anl, e->d_fnl, e->d_fout, e->d_work,
                e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
                e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
-               e->d_seg_ptr, e->d_seg_n};
+               e->d_seg_ptr, e->d_seg_n, e->d_rhor2, e->d_vxc2, e->d_vr2, e->d_vloc,
+               e->d_rhot};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
@@ -377,9 +400,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
   }
   // density factors and result staging
   {
-    const int npairs = (nst + 1) / 2;
-    std::vector<double> dfac(2 * (size_t)npairs, 0.0);
-    for (int n = 0; n < nst; n++) dfac[n] = e->occ[n] / e->omega;
+    std::vector<double> dfac = density_factors(e);
     e->nres = S_COUNT + 4 + 3 * e->nat;
     std::vector<const double*> segp = {e->d_scal, e->d_rhoelg, e->d_vion, e->d_fion};
     std::vector<int> segn = {S_COUNT, 2, 2, 3 * e->nat};
@@ -459,21 +480,42 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
   hipStream_t s = ctx->stream;
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
   fill_zero(e->d_rhor, e->N, s);
+  if (e->nspin == 2) fill_zero(e->d_rhor2, e->N, s);
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;
   e->xres_c = nullptr;
-  const bool res = keep && grid_ensure_xres(e->gw, (e->nst + 1) / 2);
-  int rc = density_batches(e->gw, c, ldc, e->nst, e->d_dfac, e->d_rhor, fprec, res);
+  int pairs = 0;
+  for (int sp = 0; sp < e->nspin; sp++) {
+    int b0, nb, p0;
+    spin_bands(e, sp, b0, nb, p0);
+    pairs = p0 + (nb + 1) / 2;
+  }
+  const bool res = keep && grid_ensure_xres(e->gw, pairs);
+  int rc = QBG_OK;
+  // one density per spin (src/ChargeDensity.cpp:136-160)
+  for (int sp = 0; sp < e->nspin && !rc; sp++) {
+    int b0, nb, p0;
+    spin_bands(e, sp, b0, nb, p0);
+    if (nb > 0)
+      rc = density_batches(e->gw, c + 2L * ldc * b0, ldc, nb, e->d_dfac + 2 * p0,
+                           sp == 0 ? e->d_rhor : e->d_rhor2, fprec, res, p0);
+  }
   if (rc) return rc;
   if (res) {
     e->xres_c = c;
     e->xres_ldc = ldc;
     e->xres_prec = fprec;
   }
+  // rhog of the total density (rhog[0] + rhog[1], src/EnergyFunctional.cpp:256-261)
+  const double* rtot = e->d_rhor;
+  if (e->nspin == 2) {
+    add2(e->d_rhor, e->d_rhor2, e->d_rhot, e->N, s);
+    rtot = e->d_rhot;
+  }
   qbg_grid* g = e->gv;
   if ((rc = grid_ensure_ws(g, 1))) return rc;
   int r1, r2, r3;
   ctx->prof_begin(QBG_K_VGRID);
-  r1 = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, e->d_rhor, nullptr, nullptr, nullptr, e->omega, s);
+  r1 = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, rtot, nullptr, nullptr, nullptr, e->omega, s);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_VGRID);
   r2 = r1 ? r1 : xfwd(g, QBG_PREC_FP64, 1, s);
@@ -501,7 +543,11 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   ctx->prof_begin(QBG_K_OTHER);
-  launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
+  if (e->nspin == 2)
+    launch_lsda(e->d_rhor, e->d_rhor2, e->N, e->d_vxc, e->d_vxc2, e->d_partial,
+                e->d_scal + S_LDA, s);
+  else
+    launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
   launch_vhxc_g(e->d_rhog, e->vb.size, 1.0 / e->omega, e->d_vion, e->d_dvion, e->d_rhopst,
                 e->d_g2iv, e->d_giv, e->d_gxv, e->d_rhoelg, e->d_rhogt, e->d_vlocal,
                 compute_stress, e->d_partial, e->d_scal + S_VHXC, s);
@@ -517,8 +563,18 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
   r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_VGRID);
-  r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr, e->d_vr,
-                       1.0, s);
+  // v_r[ispin] = vxc_r[ispin] + Re(vlocal) (src/EnergyFunctional.cpp:372-389)
+  if (e->nspin == 2) {
+    r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
+                         e->d_vloc, 1.0, s);
+    if (!r3) {
+      add2(e->d_vxc, e->d_vloc, e->d_vr, e->N, s);
+      add2(e->d_vxc2, e->d_vloc, e->d_vr2, e->N, s);
+    }
+  } else {
+    r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr,
+                         e->d_vr, 1.0, s);
+  }
   ctx->prof_end();
   if (r3) return ctx->fail(r3, "update_vhxc: density-grid transform refused");
   e->have_vhxc = true;
@@ -731,14 +787,19 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   }
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
   if (compute_hpsi) {
-    int rc;
-    if (reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec) {
-      rc = hpsi_resident(e->gw, e->d_vr, nst, hpsi, ldh, fprec);
-      e->xres_c = nullptr;   // transformed in place: consumed
-    } else {
-      rc = qbg_rs_mul_add(e->gw, e->d_vr, c, ldc, nst, hpsi, ldh, fprec);
+    // with the potential of each band's spin (src/EnergyFunctional.cpp:831)
+    const bool kept = reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec;
+    for (int sp = 0; sp < e->nspin; sp++) {
+      int b0, nb, p0;
+      spin_bands(e, sp, b0, nb, p0);
+      if (nb == 0) continue;
+      const double* v = sp == 0 ? e->d_vr : e->d_vr2;
+      const int rc = kept ? hpsi_resident(e->gw, v, nb, hpsi + 2L * ldh * b0, ldh, fprec, p0)
+                          : qbg_rs_mul_add(e->gw, v, c + 2L * ldc * b0, ldc, nb,
+                                           hpsi + 2L * ldh * b0, ldh, fprec);
+      if (rc) return rc;
     }
-    if (rc) return rc;
+    if (kept) e->xres_c = nullptr;   // transformed in place: consumed
   }
   if (compute_forces) {
     ctx->prof_begin(QBG_K_OTHER);
@@ -860,11 +921,46 @@ int qbg_ef_get(qbg_ef* e, int what, double* host) {
     case 5: src = e->d_rhopst; n = 2 * (size_t)ngv; break;
     case 6: src = e->d_sfac; n = 2 * (size_t)e->nsp * ngv; break;
     case 7: std::memcpy(host, e->fesr.data(), sizeof(double) * e->fesr.size()); return QBG_OK;
+    case 8: src = e->d_rhor2; n = e->N; break;
+    case 9: src = e->d_vr2; n = e->N; break;
+    case 10: src = e->d_vxc2; n = e->N; break;
     default: return QBG_E_INVAL;
   }
+  if (!src) return e->ctx->fail(QBG_E_STATE, "spin-down arrays exist with nspin 2 only");
   QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
                                  e->ctx->stream));
   QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+  return QBG_OK;
+}
+
+// Wavefunction::set_nspin (src/Wavefunction.cpp:365-390) with the bands
+// [0, nst_up) spin up: per-spin densities and potentials (ChargeDensity
+// rhor[ispin], EnergyFunctional v_r[ispin], src/EnergyFunctional.h:78-79)
+int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
+  if (!e) return QBG_E_INVAL;
+  qbg_ctx* c = e->ctx;
+  if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
+  if (nspin == 1) nst_up = 0;
+  if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
+    return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
+  (void)hipSetDevice(c->device);
+  QBG_HIP(c, hipStreamSynchronize(c->stream));
+  if (nspin == 2 && !e->d_rhor2) {
+    int rc;
+    if ((rc = dalloc(c, &e->d_rhor2, e->N)) || (rc = dalloc(c, &e->d_vxc2, e->N)) ||
+        (rc = dalloc(c, &e->d_vr2, e->N)) || (rc = dalloc(c, &e->d_vloc, e->N)) ||
+        (rc = dalloc(c, &e->d_rhot, e->N)))
+      return rc;
+  }
+  e->nspin = nspin;
+  e->nst_up = nst_up;
+  std::vector<double> dfac = density_factors(e);
+  if (e->d_dfac) (void)hipFree(e->d_dfac);
+  e->d_dfac = nullptr;
+  int rc = dput(c, &e->d_dfac, dfac);
+  if (rc) return rc;
+  e->have_rho = e->have_vhxc = false;
+  e->xres_c = nullptr;
   return QBG_OK;
 }
 

@@ -73,6 +73,12 @@ struct qbg_ef {
   double esr = 0, sigma_esr[6] = {0, 0, 0, 0, 0, 0}, eself = 0;
   std::vector<double> fesr;
   bool have_rho = false, have_vhxc = false, vhxc_stress = false;
+  // nspin 2 (qbg_ef_set_nspin): bands [0, nst_up) spin up, [nst_up, nst)
+  // spin down; d_rhor, d_vxc, d_vr are then the spin-up arrays and these the
+  // spin-down ones, d_vloc = Re(vlocal) on the grid, d_rhot the total density
+  int nspin = 1, nst_up = 0;
+  double *d_rhor2 = nullptr, *d_vxc2 = nullptr, *d_vr2 = nullptr, *d_vloc = nullptr,
+         *d_rhot = nullptr;
   // the bands (pointer, leading dimension, band-FFT precision) whose x-pass
   // output the last update_density left in the band grid's resident array;
   // xres_c = nullptr when there is none (consumed by an H*psi or not kept)

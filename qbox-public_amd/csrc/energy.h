@@ -14,6 +14,9 @@ int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ng
                    double* partial, double* out8, hipStream_t s);
 int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
                hipStream_t s);
+// spin-polarized LDA: vup, vdn and the same four partial sums as launch_lda
+int launch_lsda(const double* rup, const double* rdn, long n, double* vup, double* vdn,
+                double* partial, double* out4, hipStream_t s);
 void launch_lda_points(const double* rho, int n, double* e, double* v, hipStream_t s);
 int launch_vhxc_g(const double* rhog, int ngv, double omega_inv, const double* vion,
                   const double* dvion, const double* rhopst, const double* g2i, const double* gi,

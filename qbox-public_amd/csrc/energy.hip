@@ -169,6 +169,83 @@ int launch_lda(const double* rho, long n, double* vxc, double* partial, double* 
   return nb;
 }
 
+// fully polarized Perdew-Zunger fit (src/LDAFunctional.cpp:153-213):
+// exchange 2^(1/3) times the paramagnetic one, the polarized correlation
+// parameters, C and D by continuity at rs = 1
+__device__ __forceinline__ void lda_pz_ferro(double rh, double& ee, double& vv) {
+  const double c1 = 0.6203504908994001;
+  const double cx = -0.769669463118;
+  const double A = 0.01555, B = -0.0269, b1 = 1.3981, b2 = 0.2611, G = -0.0843;
+  const double D = G / (1.0 + b1 + b2) - B;
+  const double C = -A - D - G * ((b1 / 2.0 + b2) / ((1.0 + b1 + b2) * (1.0 + b1 + b2)));
+  ee = 0.0;
+  vv = 0.0;
+  if (rh > 0.0) {
+    const double rs = c1 / cbrt(rh);
+    const double vx = cx / rs, ex = 0.75 * vx;
+    double ec, vc;
+    if (rs < 1.0) {
+      const double lr = log(rs);
+      ec = A * lr + B + C * rs * lr + D * rs;
+      vc = A * lr + (B - A / 3.0) + (2.0 / 3.0) * C * rs * lr + ((2.0 * D - C) / 3.0) * rs;
+    } else {
+      const double sr = sqrt(rs);
+      const double den = 1.0 + b1 * sr + b2 * rs;
+      ec = G / den;
+      vc = ec * (1.0 + (7.0 / 6.0) * b1 * sr + (4.0 / 3.0) * b2 * rs) / den;
+    }
+    ee = ex + ec;
+    vv = vx + vc;
+  }
+}
+
+// spin-polarized LDA (src/LDAFunctional.cpp:42-89): interpolation between
+// the paramagnetic and ferromagnetic fits with f(zeta); partial sums
+// [sum rho e, sum (rho e - rho_up v_up - rho_dn v_dn), sum rho, 0]
+// (src/XCPotential.cpp:206-218)
+__global__ __launch_bounds__(256) void k_lsda(const double* __restrict__ rup,
+                                              const double* __restrict__ rdn, long n,
+                                              double* __restrict__ vup,
+                                              double* __restrict__ vdn,
+                                              double* __restrict__ partial) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  double v[4] = {0, 0, 0, 0};
+  if (i < n) {
+    const double ru = rup[i], rd = rdn[i], r = ru + rd;
+    double e = 0.0, vu = 0.0, vd = 0.0;
+    if (r > 0.0) {
+      const double kf = 1.0 / (2.0 * cbrt(2.0) - 2.0);
+      const double z = (ru - rd) / r;
+      const double a = 1.0 + z, b = 1.0 - z;
+      const double a3 = cbrt(a), b3 = cbrt(b);
+      const double f = kf * (a3 * a + b3 * b - 2.0);
+      const double df = (4.0 / 3.0) * kf * (a3 - b3);
+      double eu, vvu, ep, vvp;
+      lda_pz(r, eu, vvu);
+      lda_pz_ferro(r, ep, vvp);
+      const double de = ep - eu;
+      e = eu + f * de;
+      const double vm = vvu + f * (vvp - vvu);
+      vu = vm + de * (1.0 - z) * df;
+      vd = vm + de * (-1.0 - z) * df;
+    }
+    vup[i] = vu;
+    vdn[i] = vd;
+    v[0] = r * e;
+    v[1] = r * e - ru * vu - rd * vd;
+    v[2] = r;
+  }
+  block_reduce_store<4>(v, partial + (long)blockIdx.x * 4);
+}
+
+int launch_lsda(const double* rup, const double* rdn, long n, double* vup, double* vdn,
+                double* partial, double* out4, hipStream_t s) {
+  const int nb = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_lsda, dim3(nb), dim3(256), 0, s, rup, rdn, n, vup, vdn, partial);
+  reduce_partials(partial, nb, 4, out4, s);
+  return nb;
+}
+
 // LDA pointwise values for a test vector (no reduction)
 __global__ void k_lda_points(const double* __restrict__ rho, int n, double* e, double* v) {
   const int i = blockIdx.x * 256 + threadIdx.x;

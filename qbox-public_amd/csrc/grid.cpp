@@ -589,19 +589,19 @@ struct XwsScope {
 }  // namespace
 
 int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
-                         double* rho, int prec, bool resident) {
+                         double* rho, int prec, bool resident, int pair_base) {
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
   const size_t pb = xpair_bytes(g, prec);
-  if (resident && g->xres_bytes < (size_t)npairs * pb)
+  if (resident && g->xres_bytes < (size_t)(pair_base + npairs) * pb)
     return g->ctx->fail(QBG_E_STATE, "resident x array too small");
   hipStream_t s = g->ctx->stream;
   for (int p0 = 0; p0 < npairs; p0 += B) {
     const int nb = std::min(B, npairs - p0);
     const int band0 = 2 * p0;
     const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
-    XwsScope xs(g, resident ? (char*)g->d_xres + (size_t)p0 * pb : nullptr);
+    XwsScope xs(g, resident ? (char*)g->d_xres + (size_t)(pair_base + p0) * pb : nullptr);
     PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
     PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
     PROF(QBG_K_YDENS, ypass(g, prec, Y_DENSITY, nb, nullptr, fac + 2 * p0, rho, nullptr, 1.0, s));
@@ -609,19 +609,20 @@ int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const d
   return last_launch(g);
 }
 
-int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec) {
+int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec,
+                       int pair_base) {
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
   const size_t pb = xpair_bytes(g, prec);
-  if (g->xres_bytes < (size_t)npairs * pb)
+  if (g->xres_bytes < (size_t)(pair_base + npairs) * pb)
     return g->ctx->fail(QBG_E_STATE, "resident x array too small");
   hipStream_t s = g->ctx->stream;
   for (int p0 = 0; p0 < npairs; p0 += B) {
     const int nb = std::min(B, npairs - p0);
     const int band0 = 2 * p0;
     const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
-    XwsScope xs(g, (char*)g->d_xres + (size_t)p0 * pb);
+    XwsScope xs(g, (char*)g->d_xres + (size_t)(pair_base + p0) * pb);
     PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
     PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
     PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh,

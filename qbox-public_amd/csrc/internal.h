@@ -150,17 +150,20 @@ int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride
          int last_has2, int nbatch, hipStream_t s);
 // band-pair density passes with per-band factors already on the device
 // (fac[2*npairs], zero for a missing last band); no host synchronization.
-// resident: the x-pass output of every pair stays in g->d_xres (which must
-// hold (nst+1)/2 pairs at prec) for hpsi_resident
+// resident: the x-pass output of every pair stays in g->d_xres, from pair
+// slot pair_base on (the array must hold pair_base + (nst+1)/2 pairs at
+// prec), for hpsi_resident
 int density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double* fac,
-                    double* rho, int prec, bool resident = false);
+                    double* rho, int prec, bool resident = false, int pair_base = 0);
 // hpsi += the local-potential part (rs_mul_add) of the bands whose x-pass
-// output density_batches(..., resident = true) left in g->d_xres; consumes
-// it (the y pass transforms it in place)
-int hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec);
+// output density_batches(..., resident = true, pair_base) left in g->d_xres;
+// consumes it (the y pass transforms it in place)
+int hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec,
+                  int pair_base = 0);
 // util.hip
 void fill_zero(double* p, size_t n, hipStream_t s);
 void fill_value(double* p, size_t n, double v, hipStream_t s);
+void add2(const double* a, const double* b, double* out, size_t n, hipStream_t s);
 void gather_segments(const double* const* seg_ptr, const int* seg_n, int nseg, double* dst,
                      hipStream_t s);
 }  // namespace qbg

@@ -176,6 +176,8 @@ int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
   if (!ef) return QBG_E_INVAL;
   qbg_ctx* c = ef->ctx;
   if (kind < QBG_WF_SD || kind > QBG_WF_PSDA) return c->fail(QBG_E_INVAL, "wfstep: unknown kind");
+  if (ef->nspin != 1)
+    return c->fail(QBG_E_INVAL, "wfstep: the steppers take nspin 1 (one band set)");
   if (!(param >= 0.0)) return c->fail(QBG_E_INVAL, "wfstep: negative parameter");
   if (c->capturing) return c->fail(QBG_E_STATE, "wfstep_create during graph capture");
   if (ef->nst > POLAR_NMAX) return c->fail(QBG_E_INVAL, "wfstep: more than 4096 bands");
@@ -253,6 +255,8 @@ int qbg_wfstep_update(qbg_wfstep* w, double* c, int ldc, double* dc, int ldd) {
   if (!w || !c || !dc) return QBG_E_INVAL;
   qbg_ctx* ctx = w->ctx;
   if (ldc < w->ngw || ldd < w->ngw) return ctx->fail(QBG_E_INVAL, "wfstep_update: leading dimension");
+  if (w->ef->nspin != 1)
+    return ctx->fail(QBG_E_STATE, "wfstep_update: the functional was switched to nspin 2");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   const int n = w->nst, ngw = w->ngw;
@@ -311,6 +315,7 @@ int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, doubl
   if (!ef) return QBG_E_INVAL;
   qbg_ctx* c = ef->ctx;
   if (ndim < 0 || ndim > MIX_NMAX) return c->fail(QBG_E_INVAL, "mixer: ndim must be 0..8");
+  if (ef->nspin != 1) return c->fail(QBG_E_INVAL, "mixer: nspin 1 only");
   if (!(alpha >= 0.0) || !(rc1 >= 0.0)) return c->fail(QBG_E_INVAL, "mixer: bad parameters");
   if (c->capturing) return c->fail(QBG_E_STATE, "mixer_create during graph capture");
   (void)hipSetDevice(c->device);
@@ -390,6 +395,7 @@ int qbg_mixer_update(qbg_mixer* m) {
   qbg_ef* e = m->ef;
   qbg_ctx* ctx = m->ctx;
   if (!e->have_rho) return ctx->fail(QBG_E_STATE, "mixer_update before update_density");
+  if (e->nspin != 1) return ctx->fail(QBG_E_STATE, "mixer_update: nspin 1 only");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   qbg_grid* g = e->gv;

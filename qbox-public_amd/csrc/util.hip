@@ -56,4 +56,15 @@ void gather_segments(const double* const* seg_ptr, const int* seg_n, int nseg, d
   hipLaunchKernelGGL(k_gather, dim3(1), dim3(256), 0, s, seg_ptr, seg_n, nseg, dst);
 }
 
+// out[i] = a[i] + b[i]
+__global__ void k_add2(const double* __restrict__ a, const double* __restrict__ b,
+                       double* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+void add2(const double* a, const double* b, double* out, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_add2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out, n);
+}
+
 }  // namespace qbg

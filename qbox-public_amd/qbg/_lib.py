@@ -67,6 +67,7 @@ SIGNATURES = {
     "qbg_ef_energy": [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
                       _dp, _dp, _dp],
     "qbg_ef_get": [_vp, C.c_int, _dp],
+    "qbg_ef_set_nspin": [_vp, C.c_int, C.c_int],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_ef_energy_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
@@ -404,6 +405,8 @@ class EnergyFunctional:
          self.nproj, self.nst) = [int(x) for x in s]
         self.N = self.np0 * self.np1 * self.np2
         ctx.adopt(self)
+        if getattr(sysdef, "nspin", 1) == 2:
+            self.set_nspin(2, sysdef.nst_up)
 
     def release(self):
         if self.h and self.ctx.h:
@@ -492,12 +495,16 @@ class EnergyFunctional:
     def get(self, what):
         n = {0: self.N, 1: self.N, 2: self.N, 3: 2 * self.ngv, 4: 2 * self.ngv,
              5: 2 * self.ngv, 6: 2 * len(self.sysdef.species) * self.ngv,
-             7: 3 * self.nat}[what]
+             7: 3 * self.nat, 8: self.N, 9: self.N, 10: self.N}[what]
         out = np.zeros(n)
         self.ctx.check(self.ctx.lib.qbg_ef_get(self.h, what, _pd(out)))
         if what in (3, 4, 5, 6):
             return out.view(np.complex128)
         return out
+
+    def set_nspin(self, nspin, nst_up=0):
+        """qbg_ef_set_nspin: bands [0, nst_up) spin up, the rest spin down."""
+        self.ctx.check(self.ctx.lib.qbg_ef_set_nspin(self.h, int(nspin), int(nst_up)))
 
     def set_vr(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
