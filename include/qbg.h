@@ -222,6 +222,13 @@ int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
  * spin down. */
 int qbg_ef_set_nspin(qbg_ef* ef, int nspin, int nst_up);
 
+/* The wavefunction basis on its own grid Basis::np (the FourierTransform of
+ * SlaterDet::print, src/SlaterDet.cpp:1385, and of the sample reader,
+ * src/WavefunctionHandler.cpp:411): the grid of the <grid_function> elements
+ * of a sample file.  dims (host, 3, may be null) receives nx, ny, nz; the
+ * grid belongs to ef (do not destroy it) and takes the qbg_fft_* calls. */
+int qbg_ef_wf_grid(qbg_ef* ef, int* dims, qbg_grid** out);
+
 /* One SCF-iteration evaluation without host synchronization, so that it can
  * be captured in a graph: update_density -> update_vhxc -> energy
  * (src/BOSampleStepper.cpp:1207-1217) with the flags below; the results are

@@ -4,6 +4,9 @@
 #include "spline.h"
 #include "LDAFunctional.h"
 #include "UnitCell.h"
+#include "Base64Transcoder.h"
+#include <sstream>
+#include <cstring>
 #include <vector>
 
 extern "C" {
@@ -33,6 +36,28 @@ void ref_lsda(int n, const double* rup, const double* rdn, double* e, double* vu
   LDAFunctional f(rhoe);
   f.setxc();
   for (int i = 0; i < n; i++) { e[i] = f.exc[i]; vup[i] = f.vxc1_up[i]; vdn[i] = f.vxc1_dn[i]; }
+}
+
+// base64 of nbytes bytes as Qbox writes it in a grid_function (encode, then
+// print with 72-character lines); returns the length written to out (cap)
+long ref_b64_print(long nbytes, const unsigned char* in, char* out, long cap) {
+  Base64Transcoder x;
+  const size_t n = x.nchars((size_t)nbytes);
+  std::vector<char> b(n + 1);
+  x.encode((size_t)nbytes, in, b.data());
+  std::ostringstream os;
+  x.print(n, b.data(), os);
+  const std::string s = os.str();
+  if ((long)s.size() > cap) return -1;
+  std::memcpy(out, s.data(), s.size());
+  return (long)s.size();
+}
+
+// decode base64 text (whitespace removed by the caller) into nbytes bytes
+long ref_b64_decode(long nchars, const char* in, unsigned char* out, long nbytes) {
+  Base64Transcoder x;
+  (void)nbytes;
+  return (long)x.decode((size_t)nchars, in, out);
 }
 
 void ref_cell(const double* a9, double* b9, double* vol) {

@@ -85,6 +85,7 @@ static void ef_free(qbg_ef* e) {
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
   if (e->gw) qbg_grid_destroy(e->gw);
+  if (e->gwf) qbg_grid_destroy(e->gwf);
   if (e->gv) qbg_grid_destroy(e->gv);
 }
 
@@ -930,6 +931,28 @@ int qbg_ef_get(qbg_ef* e, int what, double* host) {
   QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
                                  e->ctx->stream));
   QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+  return QBG_OK;
+}
+
+// FourierTransform(basis, basis.np(0), basis.np(1), basis.np(2)) of the
+// wavefunction basis (src/SlaterDet.cpp:1385, src/WavefunctionHandler.cpp:411):
+// the grid of the grid functions in sample files
+int qbg_ef_wf_grid(qbg_ef* e, int* dims, qbg_grid** out) {
+  if (!e || !out) return QBG_E_INVAL;
+  *out = nullptr;
+  qbg_ctx* c = e->ctx;
+  if (!e->gwf) {
+    if (c->capturing) return c->fail(QBG_E_STATE, "wf_grid during graph capture");
+    qbg::host::Maps m;
+    if (!m.build(e->wb, e->wb.np[0], e->wb.np[1], e->wb.np[2]))
+      return c->fail(QBG_E_INVAL, "wf_grid: basis does not fit its grid");
+    int rc = qbg_grid_create(c, e->wb.np[0], e->wb.np[1], e->wb.np[2], e->wb.size, m.nvec,
+                             m.ip.data(), m.im.data(), m.zvec_to_val.data(), &e->gwf);
+    if (rc) return rc;
+  }
+  if (dims)
+    for (int i = 0; i < 3; i++) dims[i] = e->wb.np[i];
+  *out = e->gwf;
   return QBG_OK;
 }
 

@@ -19,6 +19,9 @@ struct qbg_ef {
   int np[3] = {0, 0, 0};
   long N = 0;
   qbg_grid* gw = nullptr;
+  // the wavefunction basis on its own grid (Basis::np), for sample I/O;
+  // built on first use by qbg_ef_wf_grid
+  qbg_grid* gwf = nullptr;
   qbg_grid* gv = nullptr;
   int kmin[3] = {0, 0, 0}, krange = 0;
   // device: bases and tables

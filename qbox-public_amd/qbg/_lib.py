@@ -68,6 +68,7 @@ SIGNATURES = {
                       _dp, _dp, _dp],
     "qbg_ef_get": [_vp, C.c_int, _dp],
     "qbg_ef_set_nspin": [_vp, C.c_int, C.c_int],
+    "qbg_ef_wf_grid": [_vp, _ip, C.POINTER(_vp)],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_ef_energy_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
@@ -342,6 +343,20 @@ class Grid:
                                                    PREC[prec]))
 
 
+class _BorrowedGrid(Grid):
+    """A grid owned by an energy functional (qbg_ef_wf_grid)."""
+
+    def __init__(self, ctx: Context, h, dims):
+        self.ctx = ctx
+        self.h = h
+        self.np = tuple(dims)
+        self.ng = None
+        self.N = dims[0] * dims[1] * dims[2]
+
+    def release(self):
+        self.h = None
+
+
 def _pd(a):
     return a.ctypes.data_as(_dp)
 
@@ -501,6 +516,16 @@ class EnergyFunctional:
         if what in (3, 4, 5, 6):
             return out.view(np.complex128)
         return out
+
+    def wf_grid(self):
+        """qbg_ef_wf_grid: the wavefunction basis on its own grid (sample I/O)."""
+        if getattr(self, "_wfg", None) is None:
+            h = _vp()
+            dims = np.zeros(3, dtype=np.int32)
+            self.ctx.check(self.ctx.lib.qbg_ef_wf_grid(self.h, dims.ctypes.data_as(_ip),
+                                                       C.byref(h)))
+            self._wfg = _BorrowedGrid(self.ctx, h, [int(x) for x in dims])
+        return self._wfg
 
     def set_nspin(self, nspin, nst_up=0):
         """qbg_ef_set_nspin: bands [0, nst_up) spin up, the rest spin down."""

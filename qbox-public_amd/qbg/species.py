@@ -51,7 +51,12 @@ def _numbers(text):
 
 def parse_species(text: str) -> NcppSpecies:
     """Parse the XML text of a species file (fpmd species, version 1.0)."""
-    root = ET.fromstring(text)
+    return parse_species_element(ET.fromstring(text))
+
+
+def parse_species_element(root) -> NcppSpecies:
+    """A species element: the root of a species file or a <species> of a
+    sample's atomset (SpeciesReader, src/SpeciesHandler.cpp)."""
     vals = {}
     ncpp = None
     for el in root.iter():
@@ -90,7 +95,7 @@ def parse_species(text: str) -> NcppSpecies:
         if l not in proj:
             raise ValueError(f"missing projector l={l}")
         vps[l] = proj[l]["radial_potential"][:n]
-        if "radial_function" in proj[l]:
+        if proj[l].get("radial_function") is not None and proj[l]["radial_function"].size:
             phi[l] = proj[l]["radial_function"][:n]
         elif l != llocal and nquad == 0:
             raise ValueError(f"projector l={l} has no radial_function")
@@ -105,3 +110,32 @@ def parse_species(text: str) -> NcppSpecies:
 def read_species(path) -> NcppSpecies:
     with open(path, "r") as f:
         return parse_species(f.read())
+
+
+def species_xml(sp: NcppSpecies, name: str) -> str:
+    """The expanded <species> element of a norm-conserving species, as
+    Species::print writes it (src/Species.cpp:1228-1268)."""
+    out = [f'<species name="{name}">',
+           f"<description>{sp.description}</description>",
+           f"<symbol>{sp.symbol}</symbol>",
+           f"<atomic_number>{sp.atomic_number}</atomic_number>",
+           f"<mass>{sp.mass:.12e}</mass>",
+           "<norm_conserving_pseudopotential>",
+           f"<valence_charge>{sp.zval}</valence_charge>",
+           f"<lmax>{sp.lmax}</lmax>",
+           f"<llocal>{sp.llocal}</llocal>",
+           f"<nquad>{sp.nquad}</nquad>",
+           f"<rquad>{sp.rquad:.12e}</rquad>",
+           f"<mesh_spacing>{sp.deltar:.12e}</mesh_spacing>"]
+    for l in range(sp.lmax + 1):
+        out.append(f'<projector l="{l}" size="{sp.nmesh}">')
+        out.append("<radial_potential>")
+        out += [repr(float(x)) for x in sp.vps[l]]
+        out.append("</radial_potential>")
+        out.append("<radial_function>")
+        if np.any(sp.phi[l]):
+            out += [repr(float(x)) for x in sp.phi[l]]
+        out.append("</radial_function>")
+        out.append("</projector>")
+    out += ["</norm_conserving_pseudopotential>", "</species>"]
+    return "\n".join(out) + "\n"
