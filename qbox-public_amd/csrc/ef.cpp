@@ -80,7 +80,7 @@ static void ef_free(qbg_ef* e) {
                e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
                e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
                e->d_seg_ptr, e->d_seg_n, e->d_rhor2, e->d_vxc2, e->d_vr2, e->d_vloc,
-               e->d_rhot};
+               e->d_rhot, e->b_fpart, e->b_fred};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
@@ -689,6 +689,12 @@ static int bf16_ensure(qbg_ef* e) {
     e->b_part = nullptr;
     return c->fail(QBG_E_NOMEM, "bf16 workspace");
   }
+  if (edge == 256) {
+    int rc;
+    if ((rc = dalloc(c, &e->b_fpart, (size_t)e->M * 3 * (e->Kp / 256))) ||
+        (rc = dalloc(c, &e->b_fred, (size_t)e->M * 3)))
+      return rc;
+  }
   return QBG_OK;
 }
 
@@ -724,6 +730,22 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     rc = gemm_bf_nn(e->b_anl, Kp, e->b_fout, Mp, Kp, Np, Mp, K, nst, hpsi, 2L * ldh, 1, parts, s);
     ctx->prof_end();
     if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
+  }
+  // forces alone on 256-tile shapes: the force reduction runs in the product's
+  // epilogue (Y is not stored)
+  if (compute_forces && !compute_stress && parts == 2 && e->b_fpart) {
+    ctx->prof_begin(QBG_K_GEMM);
+    rc = gemm_bf_nn_forces(e->b_c32, Kp, e->b_w32, Np, Kp, Mp, Np, M, e->b_anl, Kp, e->d_gxw,
+                           ngw, e->b_fpart, s);
+    ctx->prof_end();
+    if (rc == 0) {
+      ctx->prof_begin(QBG_K_NL);
+      fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
+      launch_nl_forces_tiles(e->b_fpart, M, (int)(Kp / 256), e->b_fred, e->d_a_m0, e->d_a_npr,
+                             e->d_a_stride, e->nat, e->d_fion, s);
+      ctx->prof_end();
+      return QBG_OK;
+    }
   }
   if (needy) {
     ctx->prof_begin(QBG_K_GEMM);

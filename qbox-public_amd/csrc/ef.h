@@ -63,6 +63,9 @@ struct qbg_ef {
   float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
         *b_w32 = nullptr, *b_part = nullptr;
   size_t b_part_floats = 0;
+  // fused force product (256-tile shapes): per-tile partials (M x 3 x Kp/256)
+  // and their per-projector sums
+  double *b_fpart = nullptr, *b_fred = nullptr;
   // per-band density factors occ/omega (2 * pairs) and the result staging:
   // d_scal, rhoelg[0], vion[0], fion gathered by a kernel into pinned host
   // memory, so an evaluation needs no host synchronization until collected

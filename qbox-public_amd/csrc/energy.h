@@ -109,6 +109,15 @@ int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int N
                double* out, long ldo, float* work, size_t work_floats, int parts,
                hipStream_t s);
 // NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf], k < K, n < N stored
+// force product with the fused force epilogue (256-tile shapes, 2-part split);
+// -1 when the shape does not allow it
+int gemm_bf_nn_forces(const float* A, long lda, const float* F, long ldf, long Kp, int Np,
+                      int Mp, int N, const float* anl, long lda_anl, const double* gx, int ngw,
+                      double* fpart, hipStream_t s);
+// fion[3a+j] -= 4 sum over the atom's projectors and the ntq tiles of fpart
+void launch_nl_forces_tiles(const double* fpart, int M, int ntq, double* red,
+                            const int32_t* a_m0, const int32_t* a_npr,
+                            const int32_t* a_stride, int nat, double* fion, hipStream_t s);
 int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
                long K, int N, double* C, long ldc, int accumulate, int parts, hipStream_t s);
 int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,

@@ -671,6 +671,35 @@ __global__ void k_nl_forces_final(const double* __restrict__ partial, int nchunk
   fion[t] -= 4.0 * s;
 }
 
+// the fused force product's partials: sum over the row tiles in a fixed-order
+// tree (one block per projector and component), then over each atom's
+// projectors: fion[3a+j] -= 4 sum (src/NonLocalPotential.cpp:1560-1770)
+__global__ __launch_bounds__(256) void k_fpart_sum(const double* __restrict__ fpart, int ntq,
+                                                   double* __restrict__ red) {
+  double v[1] = {0.0};
+  for (int t = threadIdx.x; t < ntq; t += 256) v[0] += fpart[(long)blockIdx.x * ntq + t];
+  block_reduce_store<1>(v, red + blockIdx.x);
+}
+__global__ void k_nl_forces_proj(const double* __restrict__ red,
+                                 const int32_t* __restrict__ a_m0,
+                                 const int32_t* __restrict__ a_npr,
+                                 const int32_t* __restrict__ a_stride, int nat,
+                                 double* __restrict__ fion) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * nat) return;
+  const int a = t / 3, j = t - 3 * a;
+  double sum = 0.0;
+  for (int ip = 0; ip < a_npr[a]; ip++) sum += red[(long)(a_m0[a] + ip * a_stride[a]) * 3 + j];
+  fion[t] -= 4.0 * sum;
+}
+void launch_nl_forces_tiles(const double* fpart, int M, int ntq, double* red,
+                            const int32_t* a_m0, const int32_t* a_npr,
+                            const int32_t* a_stride, int nat, double* fion, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpart_sum, dim3(3 * M), dim3(256), 0, s, fpart, ntq, red);
+  hipLaunchKernelGGL(k_nl_forces_proj, dim3((3 * nat + 255) / 256), dim3(256), 0, s, red, a_m0,
+                     a_npr, a_stride, nat, fion);
+}
+
 template <typename T>
 void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double* gx,
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,

@@ -82,6 +82,14 @@ struct BfArgs {
   long ldo, sstride;
   int accumulate;
   int vec;             // out and ldo allow the 4-wide vector stores
+  // force epilogue (k_gemm_bfb with FE): D = Y(K, m) of the force product is
+  // not stored; fpart[(m*3 + j)*ntq + q tile] = sum over the tile's G of
+  // gx_j(G) Im(conj(anl(m, G)) Y(m, G))
+  const float* fa;     // anl, fa[q + m*lda_fa]
+  long lda_fa;
+  const double* gx;    // gx[j*ngw + ig]
+  int ngw, ntq;
+  double* fpart;
 };
 
 template <int NS, bool QR, typename TO>
@@ -230,7 +238,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf(BfArgs g) {
 // MFMA); fragments of P are read one 16-row group at a time to keep the
 // registers of 2 waves per SIMD
 constexpr int BB = 256;
-template <int NS, bool QR, typename TO>
+template <int NS, bool QR, typename TO, bool FE = false>
 __global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
   __shared__ __align__(16) unsigned short sP[NS][BB][LR];
   __shared__ __align__(16) unsigned short sQ[NS][BB][LR];
@@ -321,6 +329,58 @@ __global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
       }
     }
     __syncthreads();
+  }
+  if constexpr (FE) {
+    // one projector fragment i at a time (3 double sums live): lane p =
+    // p0 + wp + 16 i + fr, G vectors ig, ig+1 of each q fragment (q even:
+    // re/im pairs); then the four lane groups sharing p, through LDS the
+    // four waves sharing wp
+    const int fq = 4 * (lane >> 4);
+    const int wg = w >> 1;
+    __syncthreads();                      // LDS tiles are free now
+    double* red = reinterpret_cast<double*>(&sP[0][0][0]);   // [4][256][3]
+    // the tile's G vectors q0/2 .. q0/2 + 127, staged once: gxs[c][ig - q0/2]
+    double* gxs = red + 4 * BB * 3;
+    for (int t = tid; t < 3 * (BB / 2); t += 512) {
+      const int c = t / (BB / 2), k = t - c * (BB / 2);
+      const long ig = (q0 >> 1) + k;
+      gxs[t] = ig < g.ngw ? g.gx[(long)c * g.ngw + ig] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const long p = p0 + wp + 16 * i + fr;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const long q = q0 + wq + 16 * j + fq;
+        const int k = (int)((q - q0) >> 1);   // local G index (zero beyond ngw)
+        const float4 a = *reinterpret_cast<const float4*>(g.fa + q + p * g.lda_fa);
+        const double t0 = (double)a.y * acc[i][j][0] - (double)a.x * acc[i][j][1];
+        const double t1 = (double)a.w * acc[i][j][2] - (double)a.z * acc[i][j][3];
+        v0 += gxs[k] * t0 + gxs[k + 1] * t1;
+        v1 += gxs[BB / 2 + k] * t0 + gxs[BB / 2 + k + 1] * t1;
+        v2 += gxs[BB + k] * t0 + gxs[BB + k + 1] * t1;
+      }
+      v0 += __shfl_xor(v0, 16); v0 += __shfl_xor(v0, 32);
+      v1 += __shfl_xor(v1, 16); v1 += __shfl_xor(v1, 32);
+      v2 += __shfl_xor(v2, 16); v2 += __shfl_xor(v2, 32);
+      if (lane < 16) {
+        double* r = red + (wg * BB + wp + 16 * i + lane) * 3;
+        r[0] = v0; r[1] = v1; r[2] = v2;
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < BB * 3; t += 512) {
+      const int pl = t / 3, c = t - 3 * pl;
+      const long p = p0 + pl;
+      if (p < g.np) {
+        const double sum = ((red[(0 * BB + pl) * 3 + c] + red[(1 * BB + pl) * 3 + c]) +
+                            red[(2 * BB + pl) * 3 + c]) + red[(3 * BB + pl) * 3 + c];
+        g.fpart[(p * 3 + c) * g.ntq + iq] = sum;
+      }
+    }
+    return;
   }
   TO* o = (TO*)g.out + (long)split * g.sstride;
   const int fq = 4 * (lane >> 4);
@@ -484,6 +544,26 @@ static int bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, in
   if (big) launchb<2, false, TO>(g, 1, s);
   else if (parts == 3) launch<3, false, TO>(g, 1, s);
   else launch<2, false, TO>(g, 1, s);
+  return 0;
+}
+
+// the force product Y = A F (A = c, K x Mp, F = W) with the force epilogue:
+// fpart[(m*3 + j)*ntq + t] for the ntq = Kp/256 row tiles; 256-tile shapes
+// and the 2-part split only (returns -1 otherwise: the caller materializes Y)
+int gemm_bf_nn_forces(const float* A, long lda, const float* F, long ldf, long Kp, int Np,
+                      int Mp, int N, const float* anl, long lda_anl, const double* gx, int ngw,
+                      double* fpart, hipStream_t s) {
+  if (Kp % BB || Np % BB || Mp % BK || lda % 4 || ldf % 4 || lda_anl % 4) return -1;
+  BfArgs g{};
+  g.P = F; g.ldp = ldf; g.Q = A; g.ldq = lda;
+  g.tp = Np / BB; g.tq = (int)(Kp / BB);
+  g.np = N; g.nq = (int)Kp;
+  g.pfast = 1;
+  g.nr = Mp;
+  g.rper = Mp;
+  g.fa = anl; g.lda_fa = lda_anl; g.gx = gx; g.ngw = ngw; g.ntq = g.tq; g.fpart = fpart;
+  const int nblk = g.tp * g.tq;
+  hipLaunchKernelGGL((k_gemm_bfb<2, false, float, true>), dim3(nblk), dim3(512), 0, s, g);
   return 0;
 }
 
