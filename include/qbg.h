@@ -285,6 +285,11 @@ int qbg_graph_destroy(qbg_graph* graph);
 #define QBG_WF_PSD 1
 #define QBG_WF_PSDA 2
 int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out);
+/* A stepper over a band set of nst bands (1 <= nst <= the functional's
+ * bands): one per spin with nspin 2, as Qbox steps each spin's SlaterDet
+ * (src/BOSampleStepper.cpp:939-944, wf_stepper->update over the spins);
+ * qbg_wfstep_update then takes that spin's columns. */
+int qbg_wfstep_create_bands(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep** out);
 int qbg_wfstep_destroy(qbg_wfstep* st);
 /* WavefunctionStepper::preprocess (PSDA: the next update does not extrapolate) */
 int qbg_wfstep_preprocess(qbg_wfstep* st);

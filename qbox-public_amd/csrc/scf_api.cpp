@@ -99,6 +99,7 @@ struct qbg_wfstep {
   int kind = 0;
   double param = 0.0;
   int ngw = 0, nst = 0;
+  bool bands = false;     // created for a band set (qbg_wfstep_create_bands)
   // nst x nst matrices: residual A, overlap S / Cholesky L, X = L^-T,
   // B = c^T c_last, Y = L^-1 B (polar), Z = L^-T Q
   double *A = nullptr, *S = nullptr, *X = nullptr, *B = nullptr, *Y = nullptr, *Z = nullptr;
@@ -170,17 +171,22 @@ int ortho_align_enqueue(qbg_wfstep* w, double* c, int ldc, hipStream_t s) {
 
 extern "C" {
 
-int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
+// nst = 0: all bands of ef (nspin 1); nst > 0: a band set of that size, as
+// one spin's SlaterDet (the caller passes that spin's columns to update)
+static int wfstep_create(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep** out) {
   if (!out) return QBG_E_INVAL;
   *out = nullptr;
   if (!ef) return QBG_E_INVAL;
   qbg_ctx* c = ef->ctx;
   if (kind < QBG_WF_SD || kind > QBG_WF_PSDA) return c->fail(QBG_E_INVAL, "wfstep: unknown kind");
-  if (ef->nspin != 1)
-    return c->fail(QBG_E_INVAL, "wfstep: the steppers take nspin 1 (one band set)");
+  if (nst == 0 && ef->nspin != 1)
+    return c->fail(QBG_E_INVAL, "wfstep: with nspin 2 create one stepper per spin "
+                                "(qbg_wfstep_create_bands)");
+  if (nst < 0 || nst > ef->nst) return c->fail(QBG_E_INVAL, "wfstep: band count");
   if (!(param >= 0.0)) return c->fail(QBG_E_INVAL, "wfstep: negative parameter");
   if (c->capturing) return c->fail(QBG_E_STATE, "wfstep_create during graph capture");
-  if (ef->nst > POLAR_NMAX) return c->fail(QBG_E_INVAL, "wfstep: more than 4096 bands");
+  if ((nst ? nst : ef->nst) > POLAR_NMAX)
+    return c->fail(QBG_E_INVAL, "wfstep: more than 4096 bands");
   (void)hipSetDevice(c->device);
   qbg_wfstep* w = new (std::nothrow) qbg_wfstep;
   if (!w) return QBG_E_NOMEM;
@@ -189,7 +195,8 @@ int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
   w->kind = kind;
   w->param = param;
   w->ngw = ef->wb.size;
-  w->nst = ef->nst;
+  w->nst = nst ? nst : ef->nst;
+  w->bands = nst > 0;
   const size_t n2 = (size_t)w->nst * w->nst, K = 2 * (size_t)w->ngw;
   w->wsize = gemm_work(w->nst);
   w->npart = psda_partials(w->ngw, w->nst);
@@ -234,6 +241,18 @@ int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
   return QBG_OK;
 }
 
+int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
+  return wfstep_create(ef, kind, param, 0, out);
+}
+
+int qbg_wfstep_create_bands(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep** out) {
+  if (nst <= 0) {
+    if (out) *out = nullptr;
+    return ef ? ef->ctx->fail(QBG_E_INVAL, "wfstep: band count") : QBG_E_INVAL;
+  }
+  return wfstep_create(ef, kind, param, nst, out);
+}
+
 int qbg_wfstep_destroy(qbg_wfstep* w) {
   if (!w) return QBG_E_INVAL;
   (void)hipSetDevice(w->ctx->device);
@@ -255,7 +274,7 @@ int qbg_wfstep_update(qbg_wfstep* w, double* c, int ldc, double* dc, int ldd) {
   if (!w || !c || !dc) return QBG_E_INVAL;
   qbg_ctx* ctx = w->ctx;
   if (ldc < w->ngw || ldd < w->ngw) return ctx->fail(QBG_E_INVAL, "wfstep_update: leading dimension");
-  if (w->ef->nspin != 1)
+  if (!w->bands && w->ef->nspin != 1)
     return ctx->fail(QBG_E_STATE, "wfstep_update: the functional was switched to nspin 2");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;

@@ -68,6 +68,7 @@ SIGNATURES = {
                       _dp, _dp, _dp],
     "qbg_ef_get": [_vp, C.c_int, _dp],
     "qbg_ef_set_nspin": [_vp, C.c_int, C.c_int],
+    "qbg_wfstep_create_bands": [_vp, C.c_int, C.c_double, C.c_int, C.POINTER(_vp)],
     "qbg_ef_wf_grid": [_vp, _ip, C.POINTER(_vp)],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
@@ -543,22 +544,31 @@ class WfStepper:
     (Hartree, 0 = adaptive) for PSD and PSDA."""
     KINDS = {"sd": 0, "psd": 1, "psda": 2}
 
-    def __init__(self, ef: EnergyFunctional, kind, param=0.0):
+    def __init__(self, ef: EnergyFunctional, kind, param=0.0, nst=None):
+        """nst: a band set of that size (one spin's bands with nspin 2);
+        None: all bands of ef."""
         self.ctx = ef.ctx
         self.ef = ef
         k = self.KINDS[kind] if isinstance(kind, str) else int(kind)
         h = _vp()
-        self.ctx.check(self.ctx.lib.qbg_wfstep_create(ef.h, k, float(param), C.byref(h)))
+        if nst is None:
+            self.ctx.check(self.ctx.lib.qbg_wfstep_create(ef.h, k, float(param), C.byref(h)))
+        else:
+            self.ctx.check(self.ctx.lib.qbg_wfstep_create_bands(ef.h, k, float(param), int(nst),
+                                                                C.byref(h)))
         self.h = h
         self.ctx.adopt(self)
 
     def preprocess(self):
         self.ctx.check(self.ctx.lib.qbg_wfstep_preprocess(self.h))
 
-    def update(self, c: DArray, dc: DArray):
-        """Enqueue WavefunctionStepper::update(dwf): dc holds H c on entry."""
-        self.ctx.check(self.ctx.lib.qbg_wfstep_update(self.h, c.p, self.ef.ngw, dc.p,
-                                                      self.ef.ngw))
+    def update(self, c: DArray, dc: DArray, band0=0):
+        """Enqueue WavefunctionStepper::update(dwf): dc holds H c on entry;
+        band0: first column of this stepper's band set in c and dc."""
+        off = 16 * self.ef.ngw * band0
+        cp = c.offset(off) if band0 else c.p
+        dp = dc.offset(off) if band0 else dc.p
+        self.ctx.check(self.ctx.lib.qbg_wfstep_update(self.h, cp, self.ef.ngw, dp, self.ef.ngw))
 
     def results(self):
         out = np.zeros(4)

@@ -30,8 +30,19 @@ class ScfLoop:
         self.c = c
         self.hpsi = hpsi
         self.prec = prec
-        self.stepper = WfStepper(ef, wf_dyn, param)
-        self.mixer = Mixer(ef, alpha, rc_kerker, ndim, rc1)
+        # one stepper per spin (each spin's bands form their own SlaterDet)
+        sd = ef.sysdef
+        if getattr(sd, "nspin", 1) == 2:
+            u = int(sd.nst_up)
+            self.sets = [(0, u), (u, sd.nst - u)]
+            self.steppers = [WfStepper(ef, wf_dyn, param, nst=n) for _, n in self.sets if n > 0]
+            self.sets = [x for x in self.sets if x[1] > 0]
+            self.mixer = None              # charge mixing of two spins is not built
+        else:
+            self.sets = [(0, sd.nst)]
+            self.steppers = [WfStepper(ef, wf_dyn, param)]
+            self.mixer = Mixer(ef, alpha, rc_kerker, ndim, rc1)
+        self.stepper = self.steppers[0]
         self.graphs = graphs
         self._g_pot = {}
         self.itscf = 0
@@ -43,13 +54,15 @@ class ScfLoop:
             self.mixer.update()
         self.ef.update_vhxc(False)
         if mix:
-            self.stepper.preprocess()
+            for st in self.steppers:
+                st.preprocess()
 
     def _step(self, first: bool):
         # the first step of an iteration sees the bands update_density read
         self.ef.energy_async(self.c, self.hpsi, forces=False, stress=False, prec=self.prec,
                              same_c=first)
-        self.stepper.update(self.c, self.hpsi)
+        for st, (b0, _) in zip(self.steppers, self.sets):
+            st.update(self.c, self.hpsi, band0=b0)
 
     def _run(self, key, fn, *args):
         if not self.graphs or self.itscf == 0:
@@ -64,8 +77,12 @@ class ScfLoop:
     def run(self, nitscf: int, nite: int, delta_ratio: float = 0.01):
         """Returns one dict per SCF iteration: etotal (of the last energy
         evaluated), ehart, the eigenvalue sums and the number of steps."""
-        self.mixer.restart()
-        self.stepper.preprocess()
+        if nite > 0 and self.mixer is None:
+            raise ValueError("charge mixing (nite > 0) takes nspin 1")
+        if self.mixer is not None:
+            self.mixer.restart()
+        for st in self.steppers:
+            st.preprocess()
         self.itscf = 0
         history = []
         ehart = delta_ehart = 0.0
@@ -78,7 +95,8 @@ class ScfLoop:
             for ite in range(max(nite, 1)):
                 self._run(("step", ite == 0), self._step, ite == 0)
                 res = self.ef.results()
-                w = self.stepper.results()
+                ws = [st.results() for st in self.steppers]
+                w = dict(ws[0], eigenvalue_sum=sum(x["eigenvalue_sum"] for x in ws))
                 if ite == 0:
                     if itscf == 0:
                         ehart, delta_ehart = res["ehart"], 0.0
@@ -99,5 +117,7 @@ class ScfLoop:
         for g in self._g_pot.values():
             g.release()
         self._g_pot = {}
-        self.stepper.release()
-        self.mixer.release()
+        for st in self.steppers:
+            st.release()
+        if self.mixer is not None:
+            self.mixer.release()

@@ -128,3 +128,29 @@ def test_spin_refused_by_steppers(ctx):
     with pytest.raises(qbg.QbgError):
         ef.set_nspin(2, s.nst + 1)
     ef.release()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_spin_scf_matches_oracle(ctx, oracle_mod, graphs):
+    """nspin 2 SCF on the device (one stepper per spin, no charge mixing)
+    against the oracle's loop, per iteration; graph replays included."""
+    import qbg
+    from qbg.scf import ScfLoop
+    from test_oracle_energy import scf_spin
+    s = cf.spin_polarized(cf.config(11), 9, 1, 1)
+    print(f"[case] spin scf graphs={graphs}", flush=True)
+    o = oracle_mod.Oracle(s)
+    c = spin_coeffs(s, o.basis(0)["g2"], 950)
+    co, es = scf_spin(oracle_mod, s, o, c, "psda", 0.0, 6)
+    ef = qbg.EnergyFunctional(ctx, s)
+    dc = ctx.to_device(c)
+    dh = ctx.array(c.shape, np.complex128)
+    loop = ScfLoop(ef, dc, dh, wf_dyn="psda", param=0.0, graphs=graphs)
+    hist = loop.run(6, 0)
+    ed = [h["etotal"] for h in hist]
+    assert np.allclose(ed, es, rtol=1e-10, atol=0), (ed, es)
+    assert relerr(dc.download(), co) < 1e-8
+    with pytest.raises(ValueError):
+        loop.run(2, 1)                    # charge mixing takes nspin 1
+    loop.release()
+    ef.release()

@@ -246,3 +246,60 @@ def test_spin_swap_symmetry(oracle_mod):
     assert np.allclose(rsw["hpsi"][s.nst - u:], r["hpsi"][:u], atol=1e-11)
     # the polarized system differs from its unpolarized density's LDA
     assert np.abs(o.get(0) - o.get(14)).max() > 1e-3
+
+
+# ---------------------------------------------------------- nspin 2 SCF
+def scf_spin(oracle_mod, s, o, c, kind, param, nitscf):
+    """BOSampleStepper's loop without charge mixing (nite = 0) with one
+    wavefunction stepper per spin (src/BOSampleStepper.cpp:782-977, the
+    spin loop of wf_stepper->update); returns bands and energies."""
+    g2 = o.basis(0)["g2"]
+    sets = [(0, s.nst_up), (s.nst_up, s.nst - s.nst_up)] if s.nspin == 2 else [(0, s.nst)]
+    sts = [oracle_mod.WfStepper(kind, param, g2) for _ in sets]
+    for st in sts:
+        st.preprocess()
+    es = []
+    c = c.copy()
+    for _ in range(nitscf):
+        o.update_density(c)
+        o.update_vhxc(False)
+        r = o.energy(c, hpsi=True, forces=False)
+        for st, (b0, n) in zip(sts, sets):
+            cn, _ = st.update(c[b0:b0 + n], r["hpsi"][b0:b0 + n])
+            c[b0:b0 + n] = cn
+        es.append(r["etotal"])
+    return c, es
+
+
+def test_spin_scf_identical_sets_follow_nspin1(oracle_mod):
+    s1 = cf.config(10)
+    o1, c = _coeffs(oracle_mod, s1, 900)
+    _, e1 = scf_spin(oracle_mod, s1, o1, c, "psda", 0.0, 8)
+    s2 = dataclasses.replace(s1, nst=2 * s1.nst, nspin=2, nst_up=s1.nst,
+                             occ_bands=np.ones(2 * s1.nst))
+    o2 = oracle_mod.Oracle(s2)
+    _, e2 = scf_spin(oracle_mod, s2, o2, np.concatenate([c, c]), "psda", 0.0, 8)
+    assert np.allclose(e1, e2, rtol=1e-11, atol=0), (e1, e2)
+
+
+def test_spin_scf_reaches_stationary_point(oracle_mod):
+    """A spin-polarized SCF (PSDA, 40 iterations) converges, and random
+    perturbations of the final bands (kept orthonormal per spin) raise the
+    energy."""
+    s = _system(11, (9, 1, 1))
+    o, c = _coeffs(oracle_mod, s, 901)
+    c, es = scf_spin(oracle_mod, s, o, c, "psda", 0.0, 60)
+    assert abs(es[-1] - es[-2]) < 1e-7 * abs(es[-1])
+    o.update_density(c)
+    o.update_vhxc(False)
+    e0 = o.energy(c, hpsi=False, forces=False)["etotal"]
+    u = s.nst_up
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        d = 1e-3 * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+        d[:, 0] = d[:, 0].real
+        cp = c + d
+        cp = np.concatenate([cf.gram_numpy(cp[:u]), cf.gram_numpy(cp[u:])])
+        o.update_density(cp)
+        o.update_vhxc(False)
+        assert o.energy(cp, hpsi=False, forces=False)["etotal"] > e0
