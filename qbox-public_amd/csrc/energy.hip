@@ -499,8 +499,13 @@ void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int kra
 // l = 0, 1.  One thread per (G, atom) forms the phase once and writes the
 // atom's projector columns m = a_m0[a] + ipr*a_stride[a]:
 // A[m*lda + 2*ig + {0,1}].
+constexpr int ANL_ATOMS = 8;
 template <typename T>
-__global__ __launch_bounds__(256) void k_anl(int ngw, long lda, const double* __restrict__ gx,
+struct alignas(2 * sizeof(T)) cpx2 {
+  T x, y;
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_anl(int ngw, int nat, long lda, const double* __restrict__ gx,
                                              const double* __restrict__ tau,
                                              const int32_t* __restrict__ a_m0,
                                              const int32_t* __restrict__ a_npr,
@@ -509,23 +514,25 @@ __global__ __launch_bounds__(256) void k_anl(int ngw, long lda, const double* __
                                              const int32_t* __restrict__ m_l,
                                              const double* __restrict__ twnl,
                                              T* __restrict__ A) {
+  // ANL_ATOMS atoms per thread: G read once, the atoms' stores in flight
+  // together; each complex entry is one vector store
   const int ig = blockIdx.x * 256 + threadIdx.x;
-  const int a = blockIdx.y;
   if (ig >= ngw) return;
+  const int a0 = blockIdx.y * ANL_ATOMS, a1 = min(nat, a0 + ANL_ATOMS);
   const double gxx = gx[ig], gyy = gx[ngw + ig], gzz = gx[2 * ngw + ig];
-  const double arg = -(gxx * tau[3 * a] + gyy * tau[3 * a + 1] + gzz * tau[3 * a + 2]);
-  double s, c;
-  sincos(arg, &s, &c);
-  const int m0 = a_m0[a], npr = a_npr[a], st = a_stride[a];
-  for (int ipr = 0; ipr < npr; ipr++) {
-    const int m = m0 + ipr * st;
-    const double t = twnl[(long)m_twnl[m] + ig];
-    double re, im;
-    if (m_l[m] == 0) { re = t * c; im = t * s; }
-    else { re = t * s; im = -t * c; }
-    T* col = A + (long)m * lda;
-    col[2 * ig] = (T)re;
-    col[2 * ig + 1] = (T)im;
+  for (int a = a0; a < a1; a++) {
+    const double arg = -(gxx * tau[3 * a] + gyy * tau[3 * a + 1] + gzz * tau[3 * a + 2]);
+    double s, c;
+    sincos(arg, &s, &c);
+    const int m0 = a_m0[a], npr = a_npr[a], st = a_stride[a];
+    for (int ipr = 0; ipr < npr; ipr++) {
+      const int m = m0 + ipr * st;
+      const double t = twnl[(long)m_twnl[m] + ig];
+      cpx2<T> v;
+      if (m_l[m] == 0) { v.x = (T)(t * c); v.y = (T)(t * s); }
+      else { v.x = (T)(t * s); v.y = (T)(-t * c); }
+      reinterpret_cast<cpx2<T>*>(A + (long)m * lda)[ig] = v;
+    }
   }
 }
 
@@ -533,13 +540,13 @@ void launch_anl(int ngw, int nat, long lda, const double* gx, const double* tau,
                 const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,
                 const int32_t* m_twnl, const int32_t* m_l, const double* twnl, void* A, int prec,
                 hipStream_t s) {
-  dim3 grid((ngw + 255) / 256, nat);
+  dim3 grid((ngw + 255) / 256, (nat + ANL_ATOMS - 1) / ANL_ATOMS);
   if (prec == 0)
-    hipLaunchKernelGGL(k_anl<double>, grid, dim3(256), 0, s, ngw, lda, gx, tau, a_m0, a_npr,
-                       a_stride, m_twnl, m_l, twnl, (double*)A);
+    hipLaunchKernelGGL(k_anl<double>, grid, dim3(256), 0, s, ngw, nat, lda, gx, tau, a_m0,
+                       a_npr, a_stride, m_twnl, m_l, twnl, (double*)A);
   else
-    hipLaunchKernelGGL(k_anl<float>, grid, dim3(256), 0, s, ngw, lda, gx, tau, a_m0, a_npr,
-                       a_stride, m_twnl, m_l, twnl, (float*)A);
+    hipLaunchKernelGGL(k_anl<float>, grid, dim3(256), 0, s, ngw, nat, lda, gx, tau, a_m0,
+                       a_npr, a_stride, m_twnl, m_l, twnl, (float*)A);
 }
 
 // ------------------------------------------------- D matrix, enl, forces
