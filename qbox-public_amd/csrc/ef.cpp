@@ -361,7 +361,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
   const size_t nbmax = (size_t)std::max<long>((N + 255) / 256, (ngv + 255) / 256);
   e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
-  e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)((ngw + 255) / 256) * 8);
+  e->partial_doubles = std::max<size_t>(e->partial_doubles,
+                                        (size_t)((ngw + 255) / 256) * ((nst + 31) / 32) * 8);
   e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
   // split-K partials of the fnl product: up to 40 splits of M x nst
   e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
@@ -701,7 +702,7 @@ static int bf16_ensure(qbg_ef* e) {
 // the nonlocal part with the projector GEMMs as BF16-split products
 // (gemm_bf.hip) on FP32 copies of anl, c, fout and W
 static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
-                   int compute_forces, int compute_stress, int parts) {
+                   int compute_forces, int compute_stress, int parts, bool c32_ready) {
   qbg_ctx* ctx = e->ctx;
   hipStream_t s = ctx->stream;
   int rc = bf16_ensure(e);
@@ -711,7 +712,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_begin(QBG_K_NL);
   launch_anl(ngw, e->nat, Kp, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
              e->d_m_twnl, e->d_m_l, e->d_twnl, e->b_anl, 1, s);
-  convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s);
+  if (!c32_ready) convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_GEMM);
   rc = gemm_bf_tn(e->b_anl, Kp, e->b_c32, Kp, Mp, Np, Kp, M, nst, c, 2L * ldc, 2.0, -1.0,
@@ -787,10 +788,17 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   const int ngw = e->wb.size, nst = e->nst, M = e->M;
-  // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf)
+  // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf);
+  // in the split modes the same pass writes the FP32 copy of c for the GEMMs
+  const bool split = prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2;
+  if (split) {
+    const int rc = bf16_ensure(e);
+    if (rc) return rc;
+  }
   ctx->prof_begin(QBG_K_OTHER);
   launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
-                 e->d_partial, e->d_scal + S_KIN, s);
+                 e->d_partial, e->d_scal + S_KIN, s, split ? e->b_c32 : nullptr,
+                 split ? e->Kp : 0);
   ctx->prof_end();
   // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
   // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
@@ -798,7 +806,7 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   const long K = 2L * ngw;
   int nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
                 ? nl_bf16(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec == QBG_PREC_BF16X3 ? 3 : 2)
+                          prec == QBG_PREC_BF16X3 ? 3 : 2, true)
                 : nl_mfma(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
                           prec);
   if (nrc) return nrc;

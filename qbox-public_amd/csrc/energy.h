@@ -11,7 +11,8 @@ void reduce_partials(const double* partial, int nblocks, int nv, double* out, hi
 // returns the number of partial blocks used (partial needs nb*8 doubles)
 int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ngw,
                    const double* g2, const double* gx, double* hpsi, long ldh, int store,
-                   double* partial, double* out8, hipStream_t s);
+                   double* partial, double* out8, hipStream_t s,
+                   float* c32 = nullptr, long ldk = 0);
 int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
                hipStream_t s);
 // spin-polarized LDA: vup, vdn and the same four partial sums as launch_lda

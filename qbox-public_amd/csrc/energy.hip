@@ -74,24 +74,32 @@ void reduce_partials(const double* partial, int nblocks, int nv, double* out, hi
 // src/EnergyFunctional.cpp:441-571 (ekin, sigma_ekin) and :818-827
 // (cp += 0.5 kpg2 c); psi2sum(G) = sum_n occ_n |c_nG|^2 (fac = 1, real basis)
 // partial layout per block: [ekin, s_xx, s_yy, s_zz, s_xy, s_yz, s_xz, 0]
+// blockIdx.y: a chunk of KIN_BANDS bands (the sums below are linear in the
+// bands, so the chunks' partials add up); c32 (if given): the FP32 copy of
+// the coefficients for the split GEMMs, c32[2*ig + {0,1} + n*ldk], written in
+// the same pass over c
+constexpr int KIN_BANDS = 32;
 __global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ c, long ldc, int nst,
                                                  const double* __restrict__ occ, int ngw,
                                                  const double* __restrict__ g2,
                                                  const double* __restrict__ gx,
                                                  double* __restrict__ hpsi, long ldh, int store,
+                                                 float* __restrict__ c32, long ldk,
                                                  double* __restrict__ partial) {
   const int ig = blockIdx.x * 256 + threadIdx.x;
+  const int n0 = blockIdx.y * KIN_BANDS, n1 = min(nst, n0 + KIN_BANDS);
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (ig < ngw) {
     const double gg = g2[ig];
     double p = 0.0;
-    for (int n = 0; n < nst; n++) {
-      const double re = c[2 * (ig + ldc * n)], im = c[2 * (ig + ldc * n) + 1];
-      p += occ[n] * (re * re + im * im);
-      if (store) {
-        hpsi[2 * (ig + ldh * n)] = 0.5 * gg * re;
-        hpsi[2 * (ig + ldh * n) + 1] = 0.5 * gg * im;
-      }
+#pragma unroll 4
+    for (int n = n0; n < n1; n++) {
+      const double2 cv = *reinterpret_cast<const double2*>(c + 2 * (ig + ldc * n));
+      p += occ[n] * (cv.x * cv.x + cv.y * cv.y);
+      if (store)
+        *reinterpret_cast<double2*>(hpsi + 2 * (ig + ldh * n)) =
+            double2{0.5 * gg * cv.x, 0.5 * gg * cv.y};
+      if (c32) *reinterpret_cast<float2*>(c32 + 2 * ig + ldk * n) = float2{(float)cv.x, (float)cv.y};
     }
     const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
     const double f = 2.0 * p;
@@ -99,17 +107,17 @@ __global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ c, l
     v[1] = f * x * x; v[2] = f * y * y; v[3] = f * z * z;
     v[4] = f * x * y; v[5] = f * y * z; v[6] = f * x * z;
   }
-  block_reduce_store<8>(v, partial + (long)blockIdx.x * 8);
+  block_reduce_store<8>(v, partial + ((long)blockIdx.y * gridDim.x + blockIdx.x) * 8);
 }
 
 int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ngw,
                    const double* g2, const double* gx, double* hpsi, long ldh, int store,
-                   double* partial, double* out8, hipStream_t s) {
-  const int nb = (ngw + 255) / 256;
-  hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, s, c, ldc, nst, occ, ngw, g2, gx, hpsi,
-                     ldh, store, partial);
-  reduce_partials(partial, nb, 8, out8, s);
-  return nb;
+                   double* partial, double* out8, hipStream_t s, float* c32, long ldk) {
+  const int nb = (ngw + 255) / 256, nch = (nst + KIN_BANDS - 1) / KIN_BANDS;
+  hipLaunchKernelGGL(k_kinetic, dim3(nb, nch), dim3(256), 0, s, c, ldc, nst, occ, ngw, g2, gx,
+                     hpsi, ldh, store, c32, ldk, partial);
+  reduce_partials(partial, nb * nch, 8, out8, s);
+  return nb * nch;
 }
 
 // ------------------------------------------------------------------ LDA
