@@ -30,6 +30,24 @@ sys.path.insert(0, str(ROOT / "qbox-public_amd"))
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and, in a run
+# of its own, WRITE_SIZE (KiB per dispatch), scaled by what the counters read
+# for a plain 1 GiB copy (--copy-probe 1024: FETCH_SIZE 0.5 GiB, WRITE_SIZE
+# 1 GiB, so bytes = 1024 (2 FETCH_SIZE + WRITE_SIZE)); commands and the
+# per-kernel figures are in profiles/c3_hbm_traffic.md.  Keyed by workload and
+# precision; "gemm" is the mean of the three projector products of a step.
+MEASURED_TRAFFIC = {
+    ("c3: 256 atoms L=36 ecut=70Ry 512 bands", "bf16x2"): {
+        "zbwd": 1527674880, "xbwd": 3053592576, "yvmul": 5073543168, "ydens": 2539485184,
+        "xfwd": 3134738432, "zfwd": 2055003136,
+        "gemm": (5058459648 + 6154072064 + 3480879104) // 3,
+    },
+}
+
+
+def measured_traffic(ef, mode, kind):
+    return MEASURED_TRAFFIC.get((ef.sysdef.name, mode), {}).get(kind)
 # mixed modes: FP32 band FFTs with the projector GEMMs on FP32 MFMA, or as
 # 3-part / 2-part BF16-split products; the bar of the headline metric
 MIXED_MODES = ("fp32", "bf16x3", "bf16x2")
@@ -64,7 +82,7 @@ def pair_bytes(ef, kind, sc):
     }[kind]
 
 
-def fft_roofline(ef, kind, kt, steps, nst, sc):
+def fft_roofline(ef, kind, kt, steps, nst, sc, mode):
     tot_ms, nlaunch = kt[kind]
     launches_per_step = nlaunch / steps
     npairs = (nst + 1) // 2
@@ -78,7 +96,8 @@ def fft_roofline(ef, kind, kt, steps, nst, sc):
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "algorithmic_bytes_per_launch": int(per_launch),
+            "traffic": measured_traffic(ef, mode, kind),
+            "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches_per_step}
 
 
@@ -95,7 +114,8 @@ def gemm_roofline(ef, kt, nst, mode):
                       "bf16x2": ("bf16", 3)}[mode]
     peak = MFMA_PEAK_TFLOPS[kind]
     return {"kernel": "gemm", "bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": measured_traffic(ef, mode, "gemm"),
             "mfma_type": kind, "products_per_flop": products,
             "mfma_busy_frac": round(achieved * products / peak, 4),
             "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(avg_ms, 4)}
@@ -371,7 +391,7 @@ def main():
     value64 = nst / (wall64 / args.steps)
     sc = 8 if args.prec == "mixed" else 16
     fft_kind = max((k for k in FFT_KINDS if k in kt), key=lambda k: kt[k][0])
-    froof = fft_roofline(ef, fft_kind, kt, args.steps, nst, sc)
+    froof = fft_roofline(ef, fft_kind, kt, args.steps, nst, sc, best)
     groof = gemm_roofline(ef, kt, nst, best)
     roof = froof if kt[fft_kind][0] >= kt["gemm"][0] else groof
 

@@ -67,6 +67,7 @@ struct GemmArgs {
                     // 2: P(r,p) = 0 for r > p (reduction stops at the tile's last p)
   const int* skip;  // if set and *skip != 0 the launch does nothing (a step of
                     // a captured iteration that has already converged)
+  int nblk;         // tiles x splits; the grid is that rounded up to 8
 };
 
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
@@ -77,10 +78,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm(GemmArgs g) {
   if (g.skip && *g.skip) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // blocks are dealt to the 8 XCDs round robin: renumber so that runs of
-  // consecutive tiles (which share operand panels) stay on one XCD's L2
-  const int nblk = gridDim.x;
-  int b = blockIdx.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  // consecutive tiles (which share operand panels) stay on one XCD's L2; the
+  // grid is nblk rounded up to a multiple of 8, the blocks past nblk return
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (b >= g.nblk) return;
   const int ntile = g.tp * g.tq;
   const int split = b / ntile, tt = b - split * ntile;
   int ip, iq;
@@ -194,9 +195,10 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double* __restrict__
 namespace {
 
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
-void launch_gemm(const GemmArgs& g, hipStream_t s) {
-  const int nblk = g.tp * g.tq * (int)((g.nr + g.rper - 1) / g.rper);
-  hipLaunchKernelGGL((k_gemm<TPm, TQm, TC, QR, TO>), dim3(nblk), dim3(256), 0, s, g);
+void launch_gemm(GemmArgs g, hipStream_t s) {
+  g.nblk = g.tp * g.tq * (int)((g.nr + g.rper - 1) / g.rper);
+  const unsigned grid = (unsigned)((g.nblk + 7) & ~7);
+  hipLaunchKernelGGL((k_gemm<TPm, TQm, TC, QR, TO>), dim3(grid), dim3(256), 0, s, g);
 }
 
 // split-K TN product; partials in FP64, reduced in a fixed order.  sym: the

@@ -90,17 +90,30 @@ struct BfArgs {
   const double* gx;    // gx[j*ngw + ig]
   int ngw, ntq;
   double* fpart;
+  int nblk;            // tiles x splits; the grid is that rounded up to 8
 };
+
+// the hardware deals consecutive blocks of a grid out to the 8 XCDs round
+// robin: renumber them so that each XCD runs one contiguous range of the
+// nblk logical blocks (tiles sharing an operand slice stay on one L2); the
+// grid is nblk rounded up to a multiple of 8 and the blocks past nblk return
+__device__ __forceinline__ int xcd_block(int nblk) {
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  return b < nblk ? b : -1;
+}
+
+inline unsigned xcd_grid(BfArgs& g, int nblk) {
+  g.nblk = nblk;
+  return (unsigned)((nblk + 7) & ~7);
+}
 
 template <int NS, bool QR, typename TO>
 __global__ __launch_bounds__(256) void k_gemm_bf(BfArgs g) {
   __shared__ __align__(16) unsigned short sP[NS][BT][LR];
   __shared__ __align__(16) unsigned short sQ[NS][BT][LR];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // renumber blocks so runs of consecutive tiles stay on one XCD (shared L2)
-  const int nblk = gridDim.x;
-  int b = blockIdx.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int b = xcd_block(g.nblk);
+  if (b < 0) return;
   const int ntile = g.tp * g.tq;
   const int split = b / ntile, tt = b - split * ntile;
   int ip, iq;
@@ -243,9 +256,8 @@ __global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
   __shared__ __align__(16) unsigned short sP[NS][BB][LR];
   __shared__ __align__(16) unsigned short sQ[NS][BB][LR];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int nblk = gridDim.x;
-  int b = blockIdx.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int b = xcd_block(g.nblk);
+  if (b < 0) return;
   const int ntile = g.tp * g.tq;
   const int split = b / ntile, tt = b - split * ntile;
   int ip, iq;
@@ -477,15 +489,15 @@ int tnb_slots() {
 }
 
 template <int NS, bool QR, typename TO>
-void launchb(const BfArgs& g, int nsplit, hipStream_t s) {
-  const int nblk = g.tp * g.tq * nsplit;
-  hipLaunchKernelGGL((k_gemm_bfb<NS, QR, TO>), dim3(nblk), dim3(512), 0, s, g);
+void launchb(BfArgs g, int nsplit, hipStream_t s) {
+  const unsigned grid = xcd_grid(g, g.tp * g.tq * nsplit);
+  hipLaunchKernelGGL((k_gemm_bfb<NS, QR, TO>), dim3(grid), dim3(512), 0, s, g);
 }
 
 template <int NS, bool QR, typename TO>
-void launch(const BfArgs& g, int nsplit, hipStream_t s) {
-  const int nblk = g.tp * g.tq * nsplit;
-  hipLaunchKernelGGL((k_gemm_bf<NS, QR, TO>), dim3(nblk), dim3(256), 0, s, g);
+void launch(BfArgs g, int nsplit, hipStream_t s) {
+  const unsigned grid = xcd_grid(g, g.tp * g.tq * nsplit);
+  hipLaunchKernelGGL((k_gemm_bf<NS, QR, TO>), dim3(grid), dim3(256), 0, s, g);
 }
 
 }  // namespace
@@ -562,8 +574,8 @@ int gemm_bf_nn_forces(const float* A, long lda, const float* F, long ldf, long K
   g.nr = Mp;
   g.rper = Mp;
   g.fa = anl; g.lda_fa = lda_anl; g.gx = gx; g.ngw = ngw; g.ntq = g.tq; g.fpart = fpart;
-  const int nblk = g.tp * g.tq;
-  hipLaunchKernelGGL((k_gemm_bfb<2, false, float, true>), dim3(nblk), dim3(512), 0, s, g);
+  const unsigned grid = xcd_grid(g, g.tp * g.tq);
+  hipLaunchKernelGGL((k_gemm_bfb<2, false, float, true>), dim3(grid), dim3(512), 0, s, g);
   return 0;
 }
 
