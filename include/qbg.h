@@ -217,7 +217,8 @@ int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
  * rhor[ispin]), update_vhxc the spin-polarized LDA and v_r[ispin]
  * (src/EnergyFunctional.cpp:248-389, src/LDAFunctional.cpp:42-89), energy
  * applies each band's spin potential.  nspin 1 restores the unpolarized
- * functional.  The wavefunction steppers and the mixer take nspin 1 only.
+ * functional.  With nspin 2 each spin takes its own stepper
+ * (qbg_wfstep_create_bands); the mixer mixes both spins' densities.
  * qbg_ef_get: what 0-2 are the spin-up arrays, 8 rhor, 9 v_r, 10 vxc_r of
  * spin down. */
 int qbg_ef_set_nspin(qbg_ef* ef, int nspin, int nst_up);
@@ -313,8 +314,12 @@ int qbg_wfstep_results(qbg_wfstep* st, double* out);
  * mixing; ndim <= 8).  qbg_mixer_update follows qbg_ef_update_density: the
  * first call after create/restart memorizes rhog, later calls mix and
  * recompute rhor from the mixed rhog (ChargeDensity::update_rhor,
- * src/ChargeDensity.cpp:201-221).  Capturable; the iteration count is kept
- * on the device. */
+ * src/ChargeDensity.cpp:201-221).  nspin 2 (the functional's at creation):
+ * the vector mixed holds rhog of spin up then spin down, each the transform
+ * of that spin's rhor (src/BOSampleStepper.cpp:318-327); after mixing each
+ * rhor is the inverse transform of its mixed rhog and the total rhog their
+ * sum.  An update after the functional's nspin changed is refused.
+ * Capturable; the iteration count is kept on the device. */
 int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, double rc1,
                      qbg_mixer** out);
 int qbg_mixer_destroy(qbg_mixer* mx);

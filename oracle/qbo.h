@@ -162,6 +162,10 @@ struct System {
   void update_twnl();                   // src/NonLocalPotential.cpp:235-1255
   void update_density(const cplx* c);   // src/ChargeDensity.cpp:136-198
   void update_rhor();                   // src/ChargeDensity.cpp:201-221
+  // one spin's density in G (the forward transform of update_density) and
+  // back: rhor = Re(backward(rhog)) / omega (src/ChargeDensity.cpp:201-221)
+  void rhog_of(const double* r, cplx* g);
+  void rhor_of(const cplx* g, double* r);
   void update_vhxc(bool compute_stress);// src/EnergyFunctional.cpp:231-394
   double energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                 bool compute_forces, bool compute_stress, Results& r);  // :397

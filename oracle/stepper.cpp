@@ -428,45 +428,76 @@ void Mixer::init(const System& s, double alpha_, double rc_kerker, int ndim, dou
     for (int i = 0; i < ng; i++) wkerker[i] = g2[i] != 0.0 ? g2[i] / (g2[i] + q02) : 1.0;
   }
   anderson = ndim > 0;
-  am.m = 2L * ng;
+  // both spins in one vector (src/BOSampleStepper.cpp:318-327)
+  am.m = 2L * s.nspin * ng;
   am.nmax = ndim;
   am.xs.assign((size_t)(ndim + 1) * am.m, 0.0);
   am.fs.assign((size_t)(ndim + 1) * am.m, 0.0);
   am.restart();
-  rhog_in.assign(ng, cplx(0.0));
+  rhog_in.assign((size_t)s.nspin * ng, cplx(0.0));
   itscf = 0;
 }
 
 void Mixer::restart() { am.restart(); itscf = 0; }
 
 void Mixer::update(System& s) {
-  const int ng = s.vb.size;
-  if (itscf == 0) {
-    rhog_in.assign(s.rhog.begin(), s.rhog.begin() + ng);
+  const int ng = s.vb.size, ns = s.nspin;
+  const size_t n = (size_t)ns * ng;
+  // the mixed vector: rhog of each spin, spin-major
+  std::vector<cplx> rg(n);
+  if (ns == 1) {
+    std::copy(s.rhog.begin(), s.rhog.begin() + ng, rg.begin());
   } else {
-    std::vector<cplx> drhog(ng);
-    for (int i = 0; i < ng; i++) drhog[i] = s.rhog[i] - rhog_in[i];
+    s.rhog_of(s.rhor.data(), rg.data());
+    s.rhog_of(s.rhor_dn.data(), rg.data() + ng);
+  }
+  if (itscf == 0) {
+    rhog_in = rg;
+  } else {
+    std::vector<cplx> drhog(n);
+    for (size_t i = 0; i < n; i++) drhog[i] = rg[i] - rhog_in[i];
     if (anderson) {
-      for (int i = 0; i < ng; i++) drhog[i] /= wls[i];
-      std::vector<cplx> rhobar(ng), drhobar(ng);
+      for (size_t i = 0; i < n; i++) drhog[i] /= wls[i % ng];
+      std::vector<cplx> rhobar(n), drhobar(n);
       am.update(reinterpret_cast<const double*>(rhog_in.data()),
                 reinterpret_cast<const double*>(drhog.data()),
                 reinterpret_cast<double*>(rhobar.data()), reinterpret_cast<double*>(drhobar.data()));
-      for (int i = 0; i < ng; i++) {
-        drhobar[i] *= wls[i];
-        rhog_in[i] = rhobar[i] + alpha * drhobar[i] * wkerker[i];
+      for (size_t i = 0; i < n; i++) {
+        drhobar[i] *= wls[i % ng];
+        rhog_in[i] = rhobar[i] + alpha * drhobar[i] * wkerker[i % ng];
       }
     } else {
-      for (int i = 0; i < ng; i++) rhog_in[i] += alpha * drhog[i] * wkerker[i];
+      for (size_t i = 0; i < n; i++) rhog_in[i] += alpha * drhog[i] * wkerker[i % ng];
     }
-    for (int i = 0; i < ng; i++) s.rhog[i] = rhog_in[i];
-    s.update_rhor();
+    if (ns == 1) {
+      for (int i = 0; i < ng; i++) s.rhog[i] = rhog_in[i];
+      s.update_rhor();
+    } else {
+      // rhor of each spin; rhog = rhog[0] + rhog[1] (src/EnergyFunctional.cpp:256-261)
+      s.rhor_of(rhog_in.data(), s.rhor.data());
+      s.rhor_of(rhog_in.data() + ng, s.rhor_dn.data());
+      for (int i = 0; i < ng; i++) s.rhog[i] = rhog_in[i] + rhog_in[i + ng];
+    }
   }
   itscf++;
 }
 
 // ChargeDensity::update_rhor (src/ChargeDensity.cpp:201-221):
 // rhor = Re(vft.backward(rhog)) / omega
+void System::rhog_of(const double* r, cplx* g) {
+  const long N = vft.np012();
+  std::vector<cplx> t(N);
+  for (long i = 0; i < N; i++) t[i] = cplx(cell.volume * r[i], 0.0);
+  vft.forward(t.data(), g);
+}
+
+void System::rhor_of(const cplx* g, double* r) {
+  std::vector<cplx> f(vft.np012());
+  vft.backward(g, f.data());
+  const double oi = 1.0 / cell.volume;
+  for (size_t i = 0; i < f.size(); i++) r[i] = f[i].real() * oi;
+}
+
 void System::update_rhor() {
   std::vector<cplx> f(vft.np012());
   vft.backward(rhog.data(), f.data());

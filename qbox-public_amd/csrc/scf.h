@@ -70,8 +70,8 @@ constexpr int MIX_NMAX = 8;
 void launch_mix_restart(int* st, hipStream_t s);
 void launch_mix_begin(int* st, int nmax, int anderson, hipStream_t s);
 void launch_mix_stage(double* rhog, double* rhog_in, const double* wls, const double* wk,
-                      double* xs, double* fs, const int* st, int ng, double alpha, int anderson,
-                      hipStream_t s);
+                      double* xs, double* fs, const int* st, int ng, int nw, double alpha,
+                      int anderson, hipStream_t s);
 int mix_partials();
 void launch_mix_dots(const double* fs, const int* st, int nmax, long m, double* partial,
                      hipStream_t s);
@@ -79,7 +79,7 @@ void launch_mix_solve(const double* partial, const int* st, int anderson, double
                       hipStream_t s);
 void launch_mix_apply(double* rhog, double* rhog_in, const double* wls, const double* wk,
                       const double* xs, const double* fs, const int* st, const double* theta,
-                      int nmax, int ng, double alpha, int anderson, hipStream_t s);
+                      int nmax, int ng, int nw, double alpha, int anderson, hipStream_t s);
 void launch_mix_select(double* rhor, const double* rhor_new, long n, const int* st,
                        hipStream_t s);
 

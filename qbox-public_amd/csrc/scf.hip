@@ -463,7 +463,9 @@ void launch_mix_begin(int* st, int nmax, int anderson, hipStream_t s) {
 
 // first call: rhog_in = rhog.  Later: drhog = rhog - rhog_in; Anderson
 // stores x = rhog_in, f = drhog / wls in ring slot k; simple mixing
-// rhog_in += alpha drhog wk and rhog = rhog_in
+// rhog_in += alpha drhog wk and rhog = rhog_in.  ng complex entries: the
+// densities of the spins one after the other, each of nw entries (the
+// weights wls, wk, of length nw, repeat per spin)
 __global__ __launch_bounds__(256) void k_mix_stage(double* __restrict__ rhog,
                                                    double* __restrict__ rhog_in,
                                                    const double* __restrict__ wls,
@@ -471,9 +473,10 @@ __global__ __launch_bounds__(256) void k_mix_stage(double* __restrict__ rhog,
                                                    double* __restrict__ xs,
                                                    double* __restrict__ fs,
                                                    const int* __restrict__ st, int ng,
-                                                   double alpha, int anderson) {
+                                                   int nw, double alpha, int anderson) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= ng) return;
+  const int iw = i % nw;
   const double2 r = reinterpret_cast<const double2*>(rhog)[i];
   const double2 ri = reinterpret_cast<const double2*>(rhog_in)[i];
   if (!st[3]) {
@@ -485,10 +488,10 @@ __global__ __launch_bounds__(256) void k_mix_stage(double* __restrict__ rhog,
     const long m = 2L * ng, k = st[2];
     xs[k * m + 2 * i] = ri.x;
     xs[k * m + 2 * i + 1] = ri.y;
-    fs[k * m + 2 * i] = dx / wls[i];
-    fs[k * m + 2 * i + 1] = dy / wls[i];
+    fs[k * m + 2 * i] = dx / wls[iw];
+    fs[k * m + 2 * i + 1] = dy / wls[iw];
   } else {
-    const double w = wk[i];
+    const double w = wk[iw];
     const double2 rin = {ri.x + (alpha * dx) * w, ri.y + (alpha * dy) * w};
     reinterpret_cast<double2*>(rhog_in)[i] = rin;
     reinterpret_cast<double2*>(rhog)[i] = rin;
@@ -496,10 +499,10 @@ __global__ __launch_bounds__(256) void k_mix_stage(double* __restrict__ rhog,
 }
 
 void launch_mix_stage(double* rhog, double* rhog_in, const double* wls, const double* wk,
-                      double* xs, double* fs, const int* st, int ng, double alpha, int anderson,
-                      hipStream_t s) {
+                      double* xs, double* fs, const int* st, int ng, int nw, double alpha,
+                      int anderson, hipStream_t s) {
   hipLaunchKernelGGL(k_mix_stage, dim3((ng + 255) / 256), dim3(256), 0, s, rhog, rhog_in, wls,
-                     wk, xs, fs, st, ng, alpha, anderson);
+                     wk, xs, fs, st, ng, nw, alpha, anderson);
 }
 
 // Anderson sums a_ij = sum (f_k - f_i)(f_k - f_j) (j <= i), b_i = sum (f_k - f_i) f_k
@@ -634,10 +637,11 @@ __global__ __launch_bounds__(256) void k_mix_apply(double* __restrict__ rhog,
                                                    const double* __restrict__ fs,
                                                    const int* __restrict__ st,
                                                    const double* __restrict__ theta, int nmax,
-                                                   int ng, double alpha, int anderson) {
+                                                   int ng, int nw, double alpha, int anderson) {
   if (!st[3] || !anderson) return;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= ng) return;
+  const int iw = i % nw;
   const int nv = st[1], k = st[2];
   const long m = 2L * ng;
   double out[2];
@@ -652,8 +656,8 @@ __global__ __launch_bounds__(256) void k_mix_apply(double* __restrict__ rhog,
       fb -= th * (fk - fs[kmi * m + l]);
       xb -= th * (xk - xs[kmi * m + l]);
     }
-    const double drb = fb * wls[i];
-    out[cpt] = xb + (alpha * drb) * wk[i];
+    const double drb = fb * wls[iw];
+    out[cpt] = xb + (alpha * drb) * wk[iw];
   }
   const double2 rin = {out[0], out[1]};
   reinterpret_cast<double2*>(rhog_in)[i] = rin;
@@ -662,9 +666,9 @@ __global__ __launch_bounds__(256) void k_mix_apply(double* __restrict__ rhog,
 
 void launch_mix_apply(double* rhog, double* rhog_in, const double* wls, const double* wk,
                       const double* xs, const double* fs, const int* st, const double* theta,
-                      int nmax, int ng, double alpha, int anderson, hipStream_t s) {
+                      int nmax, int ng, int nw, double alpha, int anderson, hipStream_t s) {
   hipLaunchKernelGGL(k_mix_apply, dim3((ng + 255) / 256), dim3(256), 0, s, rhog, rhog_in, wls, wk,
-                     xs, fs, st, theta, nmax, ng, alpha, anderson);
+                     xs, fs, st, theta, nmax, ng, nw, alpha, anderson);
 }
 
 __global__ __launch_bounds__(256) void k_mix_select(double* __restrict__ rhor,

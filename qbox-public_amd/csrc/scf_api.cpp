@@ -119,9 +119,10 @@ struct qbg_mixer {
   qbg_ef* ef = nullptr;
   qbg_ctx* ctx = nullptr;
   double alpha = 0.5;
-  int nmax = 0, anderson = 0, ng = 0;
+  int nmax = 0, anderson = 0, ng = 0, nsp = 1;
   double *wk = nullptr, *wls = nullptr, *rhog_in = nullptr, *xs = nullptr, *fs = nullptr;
   double *partial = nullptr, *theta = nullptr, *rhor_new = nullptr;
+  double *rhog_sp = nullptr, *rhog_tot = nullptr;   // nspin 2: rhog per spin, their sum
   int* st = nullptr;
   Allocs mem;
 };
@@ -334,7 +335,6 @@ int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, doubl
   if (!ef) return QBG_E_INVAL;
   qbg_ctx* c = ef->ctx;
   if (ndim < 0 || ndim > MIX_NMAX) return c->fail(QBG_E_INVAL, "mixer: ndim must be 0..8");
-  if (ef->nspin != 1) return c->fail(QBG_E_INVAL, "mixer: nspin 1 only");
   if (!(alpha >= 0.0) || !(rc1 >= 0.0)) return c->fail(QBG_E_INVAL, "mixer: bad parameters");
   if (c->capturing) return c->fail(QBG_E_STATE, "mixer_create during graph capture");
   (void)hipSetDevice(c->device);
@@ -346,7 +346,8 @@ int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, doubl
   m->nmax = ndim;
   m->anderson = ndim > 0 ? 1 : 0;
   m->ng = ef->vb.size;
-  const int ng = m->ng;
+  m->nsp = ef->nspin;
+  const int ng = m->ng, nsp = m->nsp;
   // Kerker and least-squares row weights (src/BOSampleStepper.cpp:341-406)
   std::vector<double> wk(ng, 1.0), wls(ng, 1.0);
   const std::vector<double>& g2 = ef->vb.g2;
@@ -359,10 +360,13 @@ int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, doubl
     for (int i = 0; i < ng; i++) wk[i] = g2[i] != 0.0 ? g2[i] / (g2[i] + q02) : 1.0;
   }
   Allocs& a = m->mem;
-  const size_t mlen = 2 * (size_t)ng;
+  // the mixed vector holds the densities of both spins (src/BOSampleStepper.cpp:318-327)
+  const size_t mlen = 2 * (size_t)nsp * ng;
   int rc;
   if ((rc = a.get(c, &m->wk, ng)) || (rc = a.get(c, &m->wls, ng)) ||
-      (rc = a.get(c, &m->rhog_in, mlen)) || (rc = a.get(c, &m->rhor_new, ef->N)) ||
+      (rc = a.get(c, &m->rhog_in, mlen)) || (rc = a.get(c, &m->rhor_new, nsp * ef->N)) ||
+      (nsp == 2 && ((rc = a.get(c, &m->rhog_sp, mlen)) ||
+                    (rc = a.get(c, &m->rhog_tot, 2 * (size_t)ng)))) ||
       (rc = a.get(c, &m->st, 4)) || (rc = a.get(c, &m->theta, MIX_NMAX)) ||
       (m->anderson && ((rc = a.get(c, &m->xs, (m->nmax + 1) * mlen)) ||
                        (rc = a.get(c, &m->fs, (m->nmax + 1) * mlen)) ||
@@ -375,7 +379,7 @@ int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, doubl
   bool ok = hipMemcpyAsync(m->wk, wk.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(m->wls, wls.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess;
   fill_zero(m->rhog_in, mlen, s);
-  fill_zero(m->rhor_new, ef->N, s);
+  fill_zero(m->rhor_new, nsp * ef->N, s);
   fill_zero(m->theta, MIX_NMAX, s);
   if (m->anderson) {
     fill_zero(m->xs, (m->nmax + 1) * mlen, s);
@@ -408,36 +412,59 @@ int qbg_mixer_restart(qbg_mixer* m) {
 }
 
 // BOSampleStepper's mixing step (src/BOSampleStepper.cpp:810-879) and
-// ChargeDensity::update_rhor (src/ChargeDensity.cpp:201-221)
+// ChargeDensity::update_rhor (src/ChargeDensity.cpp:201-221).  nspin 2: the
+// vector mixed is rhog of the up spin followed by rhog of the down spin (each
+// the forward transform of that spin's rhor); after mixing, each rhor is the
+// inverse transform of its mixed rhog and the total rhog their sum
+// (src/EnergyFunctional.cpp:256-261)
 int qbg_mixer_update(qbg_mixer* m) {
   if (!m) return QBG_E_INVAL;
   qbg_ef* e = m->ef;
   qbg_ctx* ctx = m->ctx;
   if (!e->have_rho) return ctx->fail(QBG_E_STATE, "mixer_update before update_density");
-  if (e->nspin != 1) return ctx->fail(QBG_E_STATE, "mixer_update: nspin 1 only");
+  if (e->nspin != m->nsp)
+    return ctx->fail(QBG_E_STATE, "mixer_update: nspin changed since the mixer was created");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   qbg_grid* g = e->gv;
   int rc = grid_ensure_ws(g, 1);
   if (rc) return rc;
+  const int ng = m->ng, nsp = m->nsp, nv = nsp * ng;
+  const long N = e->N;
+  double* rhor[2] = {e->d_rhor, e->d_rhor2};
+  double* rg = nsp == 2 ? m->rhog_sp : e->d_rhog;
+  int r = 0;
   ctx->prof_begin(QBG_K_MIX);
-  launch_mix_begin(m->st, m->nmax, m->anderson, s);
-  launch_mix_stage(e->d_rhog, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, m->ng, m->alpha,
-                   m->anderson, s);
-  if (m->anderson) {
-    launch_mix_dots(m->fs, m->st, m->nmax, 2L * m->ng, m->partial, s);
-    launch_mix_solve(m->partial, m->st, m->anderson, m->theta, s);
-    launch_mix_apply(e->d_rhog, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, m->theta,
-                     m->nmax, m->ng, m->alpha, m->anderson, s);
+  for (int sp = 0; nsp == 2 && sp < 2 && !r; sp++) {
+    r = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, rhor[sp], nullptr, nullptr, nullptr, e->omega, s);
+    r = r ? r : xfwd(g, QBG_PREC_FP64, 1, s);
+    r = r ? r : zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
   }
-  // rhor = Re(backward(rhog)) / omega, kept only when this call mixed
-  int r1 = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_rhog, 0, 0, 0, 1, s);
-  int r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
-  int r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
-                           m->rhor_new, 1.0 / e->omega, s);
-  launch_mix_select(e->d_rhor, m->rhor_new, e->N, m->st, s);
+  if (!r) {
+    launch_mix_begin(m->st, m->nmax, m->anderson, s);
+    launch_mix_stage(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, nv, ng, m->alpha,
+                     m->anderson, s);
+    if (m->anderson) {
+      launch_mix_dots(m->fs, m->st, m->nmax, 2L * nv, m->partial, s);
+      launch_mix_solve(m->partial, m->st, m->anderson, m->theta, s);
+      launch_mix_apply(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, m->theta, m->nmax,
+                       nv, ng, m->alpha, m->anderson, s);
+    }
+  }
+  // rhor = Re(backward(rhog)) / omega per spin, kept only when this call mixed
+  for (int sp = 0; sp < nsp && !r; sp++) {
+    r = zbwd(g, QBG_PREC_FP64, Z_SINGLE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
+    r = r ? r : xbwd(g, QBG_PREC_FP64, 1, s);
+    r = r ? r : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
+                      m->rhor_new + sp * N, 1.0 / e->omega, s);
+    if (!r) launch_mix_select(rhor[sp], m->rhor_new + sp * N, N, m->st, s);
+  }
+  if (!r && nsp == 2) {
+    add2(m->rhog_sp, m->rhog_sp + 2L * ng, m->rhog_tot, 2L * ng, s);
+    launch_mix_select(e->d_rhog, m->rhog_tot, 2L * ng, m->st, s);
+  }
   ctx->prof_end();
-  if (r3) return ctx->fail(r3, "mixer_update: density-grid transform refused");
+  if (r) return ctx->fail(r, "mixer_update: density-grid transform refused");
   e->have_vhxc = false;
   hipError_t er = hipGetLastError();
   if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));

@@ -216,6 +216,17 @@ def h2ogs(o, h, ecut_ry=70.0) -> SystemDef:
                      [1, 2], 4, 0, 0.0, tau=tau)
 
 
+def o2gs(o, ecut_ry=70.0) -> SystemDef:
+    """The system of test/o2gs/test.i: the oxygen dimer (atoms at +-1.16 bohr
+    along x) in an 11 bohr cube, triplet: nspin 2, delta_spin 1, nempty 4,
+    12 electrons (ecut 70 Ry there); species from O_HSCV_PBE-1.0.xml
+    (evaluated here with the LDA functional, as in the other tests)."""
+    tau = np.array([[1.16, 0.0, 0.0], [-1.16, 0.0, 0.0]])
+    s = SystemDef(f"o2gs: O2 L=11 ecut={ecut_ry:g}Ry", cubic(11.0), ecut_ry, [o], [2], 6,
+                  0, 0.0, tau=tau)
+    return spin_polarized(s, 12, 1, 4)
+
+
 def spin_polarized(s: SystemDef, nel: int, delta_spin: int = 1, nempty: int = 0) -> SystemDef:
     """The nspin = 2 form of a system with nel electrons: nst_up = (nel+1)/2 +
     delta_spin + nempty bands up and nel/2 - delta_spin + nempty down, the

@@ -37,11 +37,11 @@ class ScfLoop:
             self.sets = [(0, u), (u, sd.nst - u)]
             self.steppers = [WfStepper(ef, wf_dyn, param, nst=n) for _, n in self.sets if n > 0]
             self.sets = [x for x in self.sets if x[1] > 0]
-            self.mixer = None              # charge mixing of two spins is not built
         else:
             self.sets = [(0, sd.nst)]
             self.steppers = [WfStepper(ef, wf_dyn, param)]
-            self.mixer = Mixer(ef, alpha, rc_kerker, ndim, rc1)
+        # both spins' densities are mixed as one vector
+        self.mixer = Mixer(ef, alpha, rc_kerker, ndim, rc1)
         self.stepper = self.steppers[0]
         self.graphs = graphs
         self._g_pot = {}
@@ -77,10 +77,7 @@ class ScfLoop:
     def run(self, nitscf: int, nite: int, delta_ratio: float = 0.01):
         """Returns one dict per SCF iteration: etotal (of the last energy
         evaluated), ehart, the eigenvalue sums and the number of steps."""
-        if nite > 0 and self.mixer is None:
-            raise ValueError("charge mixing (nite > 0) takes nspin 1")
-        if self.mixer is not None:
-            self.mixer.restart()
+        self.mixer.restart()
         for st in self.steppers:
             st.preprocess()
         self.itscf = 0

@@ -1,5 +1,6 @@
 """Norm-conserving species from the reference's test potentials on the
-device: the systems of test/si4gs and test/h2ogs evaluated against the FP64
+device: the systems of test/si4gs, test/h2ogs and test/o2gs (nspin 2)
+evaluated against the FP64
 oracle (energies, H*psi, forces to 1e-10; the mixed modes to the 1e-6
 Ha/atom bar), the si4gs SCF run (PSDA, ecutprec 2 Ry, 200 iterations)
 reaching the oracle's minimum, and species the library does not take."""
@@ -34,10 +35,11 @@ def systems():
     si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
     o = read_species(GOLD / "O_HSCV_PBE-1.0.xml")
     h = read_species(GOLD / "H_HSCV_PBE-1.0.xml")
-    return {"si4gs": cf.si4gs(si), "h2ogs": cf.h2ogs(o, h, ecut_ry=70.0)}
+    return {"si4gs": cf.si4gs(si), "h2ogs": cf.h2ogs(o, h, ecut_ry=70.0),
+            "o2gs": cf.o2gs(o, ecut_ry=70.0)}
 
 
-@pytest.mark.parametrize("name", ["si4gs", "h2ogs"])
+@pytest.mark.parametrize("name", ["si4gs", "h2ogs", "o2gs"])
 @pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
 def test_evaluate(ctx, oracle_mod, name, prec):
     import qbg
@@ -46,7 +48,12 @@ def test_evaluate(ctx, oracle_mod, name, prec):
     ef = qbg.EnergyFunctional(ctx, s)
     o = oracle_mod.Oracle(s)
     b = o.basis(0)
-    c = cf.gram_numpy(cf.random_coefficients(11, b["g2"], s.nst))
+    c = cf.random_coefficients(11, b["g2"], s.nst)
+    if s.nspin == 2:
+        u = s.nst_up
+        c = np.concatenate([cf.gram_numpy(c[:u]), cf.gram_numpy(c[u:])])
+    else:
+        c = cf.gram_numpy(c)
     dc = ctx.to_device(c)
     hp = ctx.array(c.shape, np.complex128)
     got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
@@ -94,3 +101,30 @@ def test_nonlocal_d_channel_refused(ctx):
     with pytest.raises(qbg.QbgError) as e:
         qbg.EnergyFunctional(ctx, s)
     assert e.value.code == -1
+
+
+def test_o2gs_scf_with_mixing(ctx, oracle_mod):
+    """test/o2gs/test.i's run (nspin 2, charge mixing, five steps per SCF
+    iteration at fixed potential) at ecut 30 Ry for six SCF iterations
+    against the oracle's loop, per iteration."""
+    import qbg
+    from qbg.scf import ScfLoop
+    from test_oracle_energy import scf_spin
+    o_sp = read_species(GOLD / "O_HSCV_PBE-1.0.xml")
+    s = cf.o2gs(o_sp, ecut_ry=30.0)
+    print("[case] o2gs scf", flush=True)
+    o = oracle_mod.Oracle(s)
+    c = cf.random_coefficients(12, o.basis(0)["g2"], s.nst)
+    u = s.nst_up
+    c = np.concatenate([cf.gram_numpy(c[:u]), cf.gram_numpy(c[u:])])
+    co, es = scf_spin(oracle_mod, s, o, c, "psda", 0.0, 6, 5)
+    ef = qbg.EnergyFunctional(ctx, s)
+    dc = ctx.to_device(c)
+    dh = ctx.array(c.shape, np.complex128)
+    loop = ScfLoop(ef, dc, dh, wf_dyn="psda", param=0.0, graphs=True)
+    hist = loop.run(6, 5)
+    ed = [h["etotal"] for h in hist]
+    assert np.allclose(ed, es, rtol=1e-9, atol=0), (ed, es)
+    assert ed[-1] < ed[0]
+    loop.release()
+    ef.release()

@@ -2,8 +2,9 @@
 densities and potentials, the spin-polarized LDA, energies, forces, stress
 and H*psi to 1e-10 in FP64 and to the mixed-precision bar in the split modes;
 the H*psi pass from the kept transforms of both spins equal to the
-recomputing pass; identical up and down sets equal to nspin 1; the steppers
-and the mixer refuse nspin 2."""
+recomputing pass; identical up and down sets equal to nspin 1; the SCF loop
+with one stepper per spin, without and with charge mixing of both spins,
+against the oracle."""
 import dataclasses
 
 import numpy as np
@@ -117,14 +118,23 @@ def test_spin_identical_sets_equal_nspin1(ctx):
     ef2.release()
 
 
-def test_spin_refused_by_steppers(ctx):
+def test_spin_refusals(ctx):
+    """A stepper over all bands refuses nspin 2 (one per spin is needed), a
+    mixer made for nspin 2 refuses a functional switched back to nspin 1,
+    and nst_up beyond nst is refused."""
     import qbg
     s = cf.spin_polarized(cf.config(10), 5, 0, 1)
     ef = qbg.EnergyFunctional(ctx, s)
-    for make in (lambda: qbg.WfStepper(ef, "psd", 0.0), lambda: qbg.Mixer(ef, 0.5, 10.0, 3, 3.0)):
-        with pytest.raises(qbg.QbgError) as e:
-            make()
-        assert e.value.code == -1
+    with pytest.raises(qbg.QbgError) as e:
+        qbg.WfStepper(ef, "psd", 0.0)
+    assert e.value.code == -1
+    mx = qbg.Mixer(ef, 0.5, 10.0, 3, 3.0)
+    c = spin_coeffs(s, ef.basis(0)["g2"], 806)
+    ef.set_nspin(1)
+    ef.update_density(ctx.to_device(c), "fp64")
+    with pytest.raises(qbg.QbgError):
+        mx.update()
+    mx.release()
     with pytest.raises(qbg.QbgError):
         ef.set_nspin(2, s.nst + 1)
     ef.release()
@@ -150,7 +160,32 @@ def test_spin_scf_matches_oracle(ctx, oracle_mod, graphs):
     ed = [h["etotal"] for h in hist]
     assert np.allclose(ed, es, rtol=1e-10, atol=0), (ed, es)
     assert relerr(dc.download(), co) < 1e-8
-    with pytest.raises(ValueError):
-        loop.run(2, 1)                    # charge mixing takes nspin 1
+    loop.release()
+    ef.release()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_spin_scf_mixing_matches_oracle(ctx, oracle_mod, graphs):
+    """nspin 2 SCF with charge mixing (both spins' densities in one Anderson
+    vector, three steps per iteration at fixed potential) against the
+    oracle's loop, per iteration; graph replays included."""
+    import qbg
+    from qbg.scf import ScfLoop
+    from test_oracle_energy import scf_spin
+    s = cf.spin_polarized(cf.config(11), 9, 1, 1)
+    print(f"[case] spin scf mixing graphs={graphs}", flush=True)
+    o = oracle_mod.Oracle(s)
+    c = spin_coeffs(s, o.basis(0)["g2"], 951)
+    co, es = scf_spin(oracle_mod, s, o, c, "psda", 0.0, 5, 3)
+    ef = qbg.EnergyFunctional(ctx, s)
+    dc = ctx.to_device(c)
+    dh = ctx.array(c.shape, np.complex128)
+    loop = ScfLoop(ef, dc, dh, wf_dyn="psda", param=0.0, graphs=graphs)
+    hist = loop.run(5, 3)
+    ed = [h["etotal"] for h in hist]
+    assert np.allclose(ed, es, rtol=1e-9, atol=0), (ed, es)
+    assert relerr(dc.download(), co) < 1e-7
+    for dev, orc in ((0, 0), (8, 14)):
+        assert relerr(ef.get(dev), o.get(orc)) < 1e-9, (dev, orc)
     loop.release()
     ef.release()

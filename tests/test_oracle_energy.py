@@ -249,24 +249,47 @@ def test_spin_swap_symmetry(oracle_mod):
 
 
 # ---------------------------------------------------------- nspin 2 SCF
-def scf_spin(oracle_mod, s, o, c, kind, param, nitscf):
-    """BOSampleStepper's loop without charge mixing (nite = 0) with one
-    wavefunction stepper per spin (src/BOSampleStepper.cpp:782-977, the
-    spin loop of wf_stepper->update); returns bands and energies."""
+def scf_spin(oracle_mod, s, o, c, kind, param, nitscf, nite=0, ndim=3, delta_ratio=0.01):
+    """BOSampleStepper's loop (src/BOSampleStepper.cpp:782-977) with one
+    wavefunction stepper per spin (the spin loop of wf_stepper->update):
+    charge mixing of both spins' densities in one vector when nite > 0, then
+    up to max(nite, 1) steps at fixed potential, stopped early when the
+    eigenvalue sum changes by less than delta_ratio times the change of the
+    Hartree energy; returns bands and energies."""
     g2 = o.basis(0)["g2"]
     sets = [(0, s.nst_up), (s.nst_up, s.nst - s.nst_up)] if s.nspin == 2 else [(0, s.nst)]
     sts = [oracle_mod.WfStepper(kind, param, g2) for _ in sets]
+    mx = o.mixer(0.5, 10.0, ndim, 3.0) if nite > 0 else None
     for st in sts:
         st.preprocess()
     es = []
     c = c.copy()
-    for _ in range(nitscf):
+    ehart = delta_ehart = 0.0
+    for itscf in range(nitscf):
         o.update_density(c)
+        if mx is not None:
+            mx.update()
         o.update_vhxc(False)
-        r = o.energy(c, hpsi=True, forces=False)
-        for st, (b0, n) in zip(sts, sets):
-            cn, _ = st.update(c[b0:b0 + n], r["hpsi"][b0:b0 + n])
-            c[b0:b0 + n] = cn
+        if nite > 0:
+            for st in sts:
+                st.preprocess()
+        eig_m = None
+        for ite in range(max(nite, 1)):
+            r = o.energy(c, hpsi=True, forces=False)
+            if ite == 0:
+                if itscf == 0:
+                    ehart, delta_ehart = r["ehart"], 0.0
+                elif itscf == 1:
+                    ehart, delta_ehart = r["ehart"], abs(r["ehart"] - ehart)
+                else:
+                    ehart, delta_ehart = r["ehart"], min(delta_ehart, abs(r["ehart"] - ehart))
+            eig = oracle_mod.sd_dot(c, r["hpsi"])
+            for st, (b0, n) in zip(sts, sets):
+                cn, _ = st.update(c[b0:b0 + n], r["hpsi"][b0:b0 + n])
+                c[b0:b0 + n] = cn
+            if ite > 0 and abs(eig - eig_m) < delta_ratio * delta_ehart:
+                break
+            eig_m = eig
         es.append(r["etotal"])
     return c, es
 
@@ -280,6 +303,52 @@ def test_spin_scf_identical_sets_follow_nspin1(oracle_mod):
     o2 = oracle_mod.Oracle(s2)
     _, e2 = scf_spin(oracle_mod, s2, o2, np.concatenate([c, c]), "psda", 0.0, 8)
     assert np.allclose(e1, e2, rtol=1e-11, atol=0), (e1, e2)
+
+
+def test_spin_mixing_identical_sets_follow_nspin1(oracle_mod):
+    """Charge mixing of two identical spin densities (both in one Anderson
+    vector) follows the nspin 1 SCF with mixing: the spin halves carry half
+    the total, the Anderson sums scale by one half and its solution does not
+    change."""
+    from test_oracle_scf import scf
+    s1 = cf.config(10)
+    o1, c = _coeffs(oracle_mod, s1, 903)
+    _, e1 = scf(oracle_mod, s1, o1, c, "psda", 0.0, 6, 3)
+    s2 = dataclasses.replace(s1, nst=2 * s1.nst, nspin=2, nst_up=s1.nst,
+                             occ_bands=np.ones(2 * s1.nst))
+    o2 = oracle_mod.Oracle(s2)
+    _, e2 = scf_spin(oracle_mod, s2, o2, np.concatenate([c, c]), "psda", 0.0, 6, 3)
+    assert np.allclose(e1, e2, rtol=1e-10, atol=0), (e1, e2)
+
+
+def test_spin_mixing_first_step_is_kerker(oracle_mod):
+    """The second mixer call with ndim 3 takes the Kerker step on each spin,
+    rhog_s = rin_s + alpha wk (rout_s - rin_s): the total rhog follows the
+    same step; a spin whose bands did not change keeps its density; the spin
+    densities in r add up to the inverse transform of the mixed total."""
+    s = _system(11, (9, 1, 1))
+    o, c = _coeffs(oracle_mod, s, 904)
+    _, c2 = _coeffs(oracle_mod, s, 905)
+    u = s.nst_up
+    c2[:u] = c[:u]                        # only the down bands change
+    alpha, rck = 0.4, 6.0
+    g2 = o.basis(1)["g2"]
+    q0 = 2 * np.pi / rck
+    wk = np.where(g2 != 0.0, g2 / (g2 + q0 * q0), 1.0)
+    o.update_density(c2)
+    g_out, dn_out = o.get(3).copy(), o.get(14).copy()
+    mx = o.mixer(alpha, rck, 3, 3.0)
+    o.update_density(c)
+    g_in, up_in, dn_in = o.get(3).copy(), o.get(0).copy(), o.get(14).copy()
+    mx.update()                           # memorizes
+    o.update_density(c2)
+    mx.update()                           # mixes
+    up, dn, g = o.get(0).copy(), o.get(14).copy(), o.get(3).copy()
+    assert np.allclose(g, g_in + alpha * wk * (g_out - g_in), rtol=1e-11, atol=1e-14)
+    assert np.abs(up - up_in).max() < 1e-12 * np.abs(up_in).max()
+    assert np.abs(dn - dn_in).max() > 1e-3 * np.abs(dn_out - dn_in).max()
+    o.update_rhor()                       # rhor (up slot) = inverse transform of the total
+    assert np.abs(o.get(0) - (up + dn)).max() < 1e-12 * np.abs(up + dn).max()
 
 
 def test_spin_scf_reaches_stationary_point(oracle_mod):

@@ -98,6 +98,21 @@ def test_si4_and_h2o_charge(oracle_mod, si, oxy, hyd):
         assert np.isfinite(r["etotal"])
 
 
+def test_o2gs_triplet_charge_and_moment(oracle_mod, oxy):
+    """test/o2gs/test.i: 12 electrons, delta_spin 1, nempty 4: 11 bands up
+    (7 occupied), 9 down (5 occupied); the densities integrate to 7 and 5."""
+    s = cf.o2gs(oxy, ecut_ry=30.0)
+    assert (s.nst, s.nst_up) == (20, 11)
+    assert s.occ.sum() == 12 and s.occ[:11].sum() == 7
+    o = oracle_mod.Oracle(s)
+    c = cf.random_coefficients(5, o.basis(0)["g2"], s.nst)
+    c = np.concatenate([cf.gram_numpy(c[:11]), cf.gram_numpy(c[11:])])
+    r = o.evaluate(c, hpsi=True, forces=True)
+    assert abs(r["total_charge"] - 12) < 1e-8
+    dv = abs(np.linalg.det(np.asarray(s.cell))) / o.get(0).size
+    assert abs(o.get(0).sum() * dv - 7) < 1e-8 and abs(o.get(14).sum() * dv - 5) < 1e-8
+
+
 def test_si4_scf_is_variational(oracle_mod, si):
     """The settings of test/si4gs/test.i: PSDA, ecutprec 2 Ry (1 Ha),
     run 0 200 (200 SCF iterations of one step, no charge mixing); the
