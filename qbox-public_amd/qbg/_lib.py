@@ -590,7 +590,9 @@ class WfStepper:
 
 class Mixer:
     """qbg_mixer: BOSampleStepper's charge mixing (Kerker weights, Anderson
-    acceleration over ndim previous densities; ndim = 0 is simple mixing)."""
+    acceleration over ndim previous densities; ndim = 0 is simple mixing).
+    With a spin-polarized functional both spins' densities are mixed as one
+    vector; the functional's nspin must not change afterwards."""
 
     def __init__(self, ef: EnergyFunctional, alpha=0.5, rc_kerker=10.0, ndim=3, rc1=3.0):
         self.ctx = ef.ctx
