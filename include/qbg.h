@@ -112,6 +112,13 @@ int qbg_grid_create(qbg_ctx* ctx, int np0, int np1, int np2, int ng, int nvec,
                     const int32_t* ip, const int32_t* im,
                     const int32_t* zvec_to_val, qbg_grid** out);
 int qbg_grid_destroy(qbg_grid* g);
+/* FourierTransform's size accessors np0(), np1(), np2() and the basis size
+ * (src/FourierTransform.h:123-137), plus the kernels that serve the grid:
+ * out[0..2] np0 np1 np2, [3] ng, [4] nvec, [5] ntrans0, [6] 1 if the
+ * compiled four-step / prime-factor kernels run every axis (0: the generic
+ * Stockham kernels), [7..12] the split n1 x n2 of axes 0, 1, 2 (0 if
+ * generic).  out has room for 13 ints. */
+int qbg_grid_info(const qbg_grid* g, int* out);
 
 /* FourierTransform::backward(c, f) and backward(c1, c2, f)
  * (src/FourierTransform.cpp:175-199): f has np012 complex entries. */
@@ -196,6 +203,18 @@ int qbg_ef_basis(const qbg_ef* ef, int which, int32_t* idx, double* g2,
                  double* gx, int32_t* ip, int32_t* im, int32_t* zvec_to_val);
 /* EnergyFunctional::atoms_moved() with new positions (host tau[3*nat]) */
 int qbg_ef_set_positions(qbg_ef* ef, const double* tau);
+/* Wavefunction::resize(cell, refcell, ecut) followed by
+ * EnergyFunctional::cell_moved() (src/EnergyFunctional.cpp:1200-1245): the
+ * basis vectors of both bases for the new cell cell9 (host, a0 a1 a2), with
+ * the membership and grid fixed by the reference cell given at creation; the
+ * local tables vps, dvps, rhops; NonLocalPotential::update_twnl; then
+ * atoms_moved at the unchanged Cartesian positions (move them afterwards
+ * with qbg_ef_set_positions if they scale with the cell).  The density must
+ * be updated again before update_vhxc.  Without a reference cell the basis
+ * follows the cell, as in the reference: QBG_E_INVAL (nothing changed) if
+ * the new cell would change the basis membership or the grid; create the
+ * functional again then. */
+int qbg_ef_cell_moved(qbg_ef* ef, const double* cell9);
 /* ChargeDensity::update_density() from device coefficients c */
 int qbg_ef_update_density(qbg_ef* ef, const double* c, int ldc, int prec);
 /* EnergyFunctional::update_vhxc(compute_stress) */
@@ -222,6 +241,30 @@ int qbg_ef_energy(qbg_ef* ef, const double* c, int ldc, int compute_hpsi,
  * qbg_ef_get: what 0-2 are the spin-up arrays, 8 rhor, 9 v_r, 10 vxc_r of
  * spin down. */
 int qbg_ef_set_nspin(qbg_ef* ef, int nspin, int nst_up);
+
+/* NonLocalPotential(atoms, sd) of the SlaterDet of spin ispin (bands
+ * [0, nst_up) or [nst_up, nst) of the functional; all bands with nspin 1),
+ * as EnergyFunctional constructs it (src/EnergyFunctional.cpp:113-118,
+ * src/NonLocalPotential.h:61-68).  The handle uses the functional's
+ * positions (atoms_moved), projector tables and workspace; it is not
+ * reentrant with the functional's own calls (one host thread, one stream).
+ * qbg_nl_update_twnl: NonLocalPotential::update_twnl()
+ * (src/NonLocalPotential.cpp:235-1255) for the functional's current basis
+ * (qbg_ef_cell_moved already does it).
+ * qbg_nl_energy: NonLocalPotential::energy(compute_hpsi, dsd,
+ * compute_forces, fion, compute_stress, sigma_enl)
+ * (src/NonLocalPotential.cpp:1258-2069): c (device, ld ldc complex) holds
+ * the spin's bands; H*psi_NL is ADDED into dsd (device, ld ldd); fion (host,
+ * 3 nat, species-major) and sigma_enl (host, 6: xx yy zz xy yz xz) are
+ * zeroed, then filled when asked for; *enl receives the nonlocal energy.
+ * Synchronous (returns after the results are on the host). */
+typedef struct qbg_nl qbg_nl;
+int qbg_nl_create(qbg_ef* ef, int ispin, qbg_nl** out);
+int qbg_nl_destroy(qbg_nl* nl);
+int qbg_nl_update_twnl(qbg_nl* nl);
+int qbg_nl_energy(qbg_nl* nl, const double* c, int ldc, int compute_hpsi, double* dsd, int ldd,
+                  int compute_forces, double* fion, int compute_stress, double* sigma_enl,
+                  int prec, double* enl);
 
 /* The wavefunction basis on its own grid Basis::np (the FourierTransform of
  * SlaterDet::print, src/SlaterDet.cpp:1385, and of the sample reader,

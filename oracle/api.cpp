@@ -29,10 +29,11 @@ const char* qbo_last_error(void) { return g_err.c_str(); }
 
 void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
                  int nsp, const qbo_species_desc* sp, const int* na,
-                 const double* tau, int nst, const double* occ) {
+                 const double* tau, int nst, const double* occ, const int* grid3) {
   try {
     System* s = new System;
     s->cell.set(cell9);
+    if (grid3) for (int i = 0; i < 3; i++) s->grid[i] = grid3[i];
     if (refcell9) { s->has_ref = true; s->refcell.set(refcell9); }
     s->ecut = ecut_ha;
     int off = 0;
@@ -238,6 +239,19 @@ int qbo_energy(void* h, const double* c, int compute_hpsi, double* hpsi,
 
 // species radial tables of species is at n points g: v, dv (local), and
 // vnl, dvnl for each projector (l-major)
+// NonLocalPotential::energy alone (enl returned; hpsi += H*psi_NL)
+double qbo_nl_energy(void* h, const double* c, int compute_hpsi, double* hpsi, int forces,
+                     int stress, double* fion, double* sigma6) {
+  System* s = (System*)h;
+  std::vector<double> f;
+  double sig[6];
+  const double enl = s->nl_energy((const cplx*)c, compute_hpsi != 0, (cplx*)hpsi, forces != 0,
+                                  f, stress != 0, sig);
+  if (fion) std::memcpy(fion, f.data(), sizeof(double) * f.size());
+  if (sigma6) std::memcpy(sigma6, sig, sizeof(sig));
+  return enl;
+}
+
 void qbo_species_eval(void* h, int is, int n, const double* g, double* vloc,
                       double* dvloc, double* vnl, double* dvnl) {
   System* s = (System*)h;

@@ -20,6 +20,11 @@ void System::setup() {
   for (int i = 0; i < 3; i++) {              // src/ChargeDensity.cpp:44-49
     n[i] = vb.np[i] + 2;
     while (!factorizable(n[i])) n[i] += 2;
+    if (grid[i] > 0) {                       // FourierTransform(basis, np0, np1, np2)
+      if (grid[i] < vb.np[i] || !factorizable(grid[i]))
+        throw std::invalid_argument("forced grid does not hold the density basis");
+      n[i] = grid[i];
+    }
   }
   vft.init(vb, n[0], n[1], n[2]);
   ft.init(wb, n[0], n[1], n[2]);             // src/ChargeDensity.cpp:98

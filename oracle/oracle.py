@@ -56,7 +56,7 @@ def lib():
         _lib = C.CDLL(str(LIB))
         _lib.qbo_create.restype = C.c_void_p
         _lib.qbo_create.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
-                                    _ip, _dp, C.c_int, _dp]
+                                    _ip, _dp, C.c_int, _dp, _ip]
         _lib.qbo_last_error.restype = C.c_char_p
         _lib.qbo_ncpp_kb.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         for name in ["qbo_destroy", "qbo_sizes", "qbo_basis", "qbo_rods", "qbo_fft_backward",
@@ -88,6 +88,9 @@ def lib():
         _lib.qbo_update_rhor.argtypes = [C.c_void_p]
         _lib.qbo_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
                                     _dp, _dp, _dp]
+        _lib.qbo_nl_energy.restype = C.c_double
+        _lib.qbo_nl_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
+                                       _dp, _dp]
     return _lib
 
 
@@ -98,12 +101,17 @@ ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p",
 class Oracle:
     """One Qbox-like system on the CPU (EnergyFunctional + ChargeDensity)."""
 
-    def __init__(self, sysdef, refcell=None):
+    def __init__(self, sysdef, refcell=None, grid=None):
         """refcell (3x3, rows a0 a1 a2): the reference cell that fixes the
         basis membership and grid sizes (src/Basis.cpp:314-325), so that
-        strained copies of a system share one basis."""
+        strained copies of a system share one basis.  grid (np0, np1, np2):
+        the density grid, as FourierTransform(basis, np0, np1, np2) takes it
+        (src/FourierTransform.h:105), instead of ChargeDensity's rule; it must
+        hold the density basis."""
         L = lib()
         self.sysdef = sysdef
+        if refcell is None:
+            refcell = getattr(sysdef, "refcell", None)
         self._keep = []
         descs = (SpeciesDesc * len(sysdef.species))()
         for i, sp in enumerate(sysdef.species):
@@ -117,9 +125,14 @@ class Oracle:
         if refcell is not None:
             ref = np.ascontiguousarray(refcell, dtype=float).ravel()
             self._keep.append(ref)
+        g3 = None
+        if grid is not None:
+            g3 = np.array(grid, dtype=np.int32)
+            self._keep.append(g3)
         h = L.qbo_create(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
                          len(sysdef.species), descs,
-                         _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ))
+                         _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ),
+                         _ptr(g3, _ip) if g3 is not None else None)
         if not h:
             raise RuntimeError(L.qbo_last_error().decode())
         self.h = C.c_void_p(h)
@@ -244,6 +257,17 @@ class Oracle:
         out["sigma"] = sig.reshape(7, 6)
         out["hpsi"] = h
         return out
+
+    def nl_energy(self, c, hpsi=None, forces=True, stress=True):
+        """NonLocalPotential::energy: (enl, fion (nat x 3), sigma_enl[6],
+        hpsi + H*psi_NL or None)."""
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        h = None if hpsi is None else np.array(hpsi, dtype=np.complex128)
+        f = np.zeros(3 * self.sysdef.nat)
+        sig = np.zeros(6)
+        enl = lib().qbo_nl_energy(self.h, _ptr(c), int(h is not None), _ptr(h), int(forces),
+                                  int(stress), _ptr(f), _ptr(sig))
+        return enl, f.reshape(-1, 3), sig, h
 
     def evaluate(self, c, hpsi=True, forces=True, stress=False):
         """density -> update_vhxc -> energy (src/BOSampleStepper.cpp:1207-1217)."""

@@ -127,6 +127,9 @@ struct Results {
 struct System {
   // inputs
   Cell cell; bool has_ref = false; Cell refcell;
+  // density grid forced by the caller (0: ChargeDensity's rule); the basis
+  // must fit (n >= the density basis's own grid on every axis)
+  int grid[3] = {0, 0, 0};
   double ecut = 0.0;                  // Hartree
   std::vector<Species> sp;
   std::vector<std::vector<double>> tau;   // tau[is][3*ia]

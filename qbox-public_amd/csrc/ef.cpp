@@ -44,6 +44,14 @@ int dput(qbg_ctx* c, T** p, const std::vector<T>& h) {
   return QBG_OK;
 }
 
+// copy host values into an existing device array of the same size
+template <typename T>
+int dset(qbg_ctx* c, T* d, const std::vector<T>& h) {
+  if (!h.empty())
+    QBG_HIP(c, hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return QBG_OK;
+}
+
 }  // namespace
 
 
@@ -143,101 +151,15 @@ static void ef_esr(qbg_ef* e) {
   for (double& x : e->sigma_esr) x *= -1.0 / e->omega;
 }
 
-static int ef_atoms_moved(qbg_ef* e) {
-  qbg_ctx* c = e->ctx;
-  hipStream_t s = c->stream;
-  QBG_HIP(c, hipMemcpyAsync(e->d_tau, e->tau.data(), sizeof(double) * e->tau.size(),
-                            hipMemcpyHostToDevice, s));
-  c->prof_begin(QBG_K_OTHER);
-  launch_sf_tables(e->d_tau, e->nat, e->d_b, e->kmin, e->krange, e->d_sftab, s);
-  launch_sfac(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->nsp, e->d_sp_first,
-              e->d_vps, e->d_dvps, e->d_rhops, e->d_sfac, e->d_vion, e->d_dvion, e->d_rhopst, s);
-  c->prof_end();
-  ef_esr(e);
-  QBG_HIP(c, hipMemcpyAsync(e->d_fesr, e->fesr.data(), sizeof(double) * e->fesr.size(),
-                            hipMemcpyHostToDevice, s));
-  QBG_HIP(c, hipStreamSynchronize(s));
-  e->have_vhxc = false;
-  return QBG_OK;
-}
-
-extern "C" {
-
-int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut, int nsp,
-                  const qbg_species_desc* spd, const int* na, const double* tau, int nst,
-                  const double* occ, qbg_ef** out) {
-  if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
-    return QBG_E_INVAL;
-  *out = nullptr;
-  (void)hipSetDevice(c->device);
-  qbg_ef* e = new (std::nothrow) qbg_ef;
-  if (!e) return QBG_E_NOMEM;
-  e->ctx = c;
-  e->cell.set(cell9);
-  e->omega = e->cell.volume;
-  e->ecut = ecut;
-  e->nsp = nsp;
-  e->nst = nst;
-  e->occ.assign(occ, occ + nst);
-  e->sp_first.assign(1, 0);
-  for (int is = 0; is < nsp; is++) {
-    host::Species q;
-    const qbg_species_desc& d = spd[is];
-    q.zval = d.zval; q.nmesh = d.nmesh; q.deltar = d.deltar; q.rcps = d.rcps;
-    bool ok;
-    if (d.type == QBG_SPECIES_NCPP) {
-      ok = d.vps && d.phi && d.nmesh > 0 &&
-           host::ncpp_to_kb(d.lmax, d.llocal, d.nmesh, d.deltar, d.vps, d.phi, q.vloc, q.proj,
-                            q.dl, q.lmax);
-    } else {
-      ok = d.type == QBG_SPECIES_SLPP && d.vloc && d.proj && d.dl && d.lmax >= 0 &&
-           d.lmax <= 1 && d.nmesh > 0;
-      if (ok) {
-        q.lmax = d.lmax;
-        q.vloc.assign(d.vloc, d.vloc + q.nmesh);
-        q.proj.assign(d.proj, d.proj + (size_t)(q.lmax + 1) * q.nmesh);
-        q.dl.assign(d.dl, d.dl + q.lmax + 1);
-      }
-    }
-    if (!ok || !q.init()) {
-      delete e;
-      return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 1?)");
-    }
-    e->sp.push_back(q);
-    e->na.push_back(na[is]);
-    e->sp_first.push_back(e->sp_first.back() + na[is]);
-  }
-  e->nat = e->sp_first.back();
-  e->tau.assign(tau, tau + 3 * e->nat);
-  for (int is = 0; is < nsp; is++) e->eself += e->na[is] * e->sp[is].eself();
-  // bases and grid (src/ChargeDensity.cpp:34-98)
-  host::Cell def = e->cell;
-  if (refcell9) def.set(refcell9);
-  e->wb.build(def, e->cell, ecut);
-  e->vb.build(def, e->cell, 4.0 * ecut);
-  for (int i = 0; i < 3; i++) {
-    int n = e->vb.np[i] + 2;
-    while (!host::factorizable(n)) n += 2;
-    e->np[i] = n;
-  }
-  e->N = (long)e->np[0] * e->np[1] * e->np[2];
-  if (!e->wm.build(e->wb, e->np[0], e->np[1], e->np[2]) ||
-      !e->vm.build(e->vb, e->np[0], e->np[1], e->np[2])) {
-    delete e;
-    return c->fail(QBG_E_INVAL, "basis does not fit in the grid");
-  }
-  int rc;
-  if ((rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->wb.size, e->wm.nvec,
-                            e->wm.ip.data(), e->wm.im.data(), e->wm.zvec_to_val.data(), &e->gw)) ||
-      (rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->vb.size, e->vm.nvec,
-                            e->vm.ip.data(), e->vm.im.data(), e->vm.zvec_to_val.data(), &e->gv))) {
-    ef_free(e);
-    delete e;
-    return rc;
-  }
-  const int ngw = e->wb.size, ngv = e->vb.size;
-  // cell_moved tables (src/EnergyFunctional.cpp:1200-1245)
-  std::vector<double> vps((size_t)nsp * ngv), dvps((size_t)nsp * ngv), rhops((size_t)nsp * ngv);
+// EnergyFunctional::cell_moved's tables on the density basis of the current
+// cell (src/EnergyFunctional.cpp:1200-1245): rhops = rhopsg/omega,
+// vps = vlocg/omega, dvps = dvlocg/omega, species-major [is][ig]
+static void ef_local_tables(const qbg_ef* e, std::vector<double>& vps, std::vector<double>& dvps,
+                            std::vector<double>& rhops) {
+  const int nsp = e->nsp, ngv = e->vb.size;
+  vps.assign((size_t)nsp * ngv, 0.0);
+  dvps.assign((size_t)nsp * ngv, 0.0);
+  rhops.assign((size_t)nsp * ngv, 0.0);
   for (int is = 0; is < nsp; is++)
     for (int ig = 0; ig < ngv; ig++) {
       double v, dv;
@@ -247,22 +169,24 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       vps[(size_t)is * ngv + ig] = v * oi;
       dvps[(size_t)is * ngv + ig] = dv * oi;
     }
-  // projectors twnl (src/NonLocalPotential.cpp:235-520, KB l = 0, 1)
-  std::vector<double> twnl;
-  std::vector<int> twnl_off(nsp);
-  std::vector<int32_t> m_atom, m_twnl, m_l, m_ipr, m_npr, m_stride, m_dbase, a_m0(e->nat),
-      a_npr(e->nat), a_stride(e->nat);
-  std::vector<double> m_wt, dmat, dtwnl, fac2(nsp);
-  std::vector<int64_t> dt_off(nsp);
-  std::vector<int32_t> sp_npr(nsp);
+}
+
+// NonLocalPotential::update_twnl on the wavefunction basis of the current
+// cell (src/NonLocalPotential.cpp:235-520, KB l = 0, 1): twnl[is] =
+// [ipr][ig] at offset sum_{js<is} npr_js ngw, dtwnl[is] = [ipr][ij][ig] at
+// six times that offset
+static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<double>& dtwnl) {
+  const int ngw = e->wb.size;
+  twnl.clear();
+  dtwnl.clear();
   const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
-  for (int is = 0; is < nsp; is++) {
+  for (int is = 0; is < e->nsp; is++) {
     const host::Species& q = e->sp[is];
     const int npr = q.nlm;
-    twnl_off[is] = (int)twnl.size();
-    twnl.resize(twnl.size() + (size_t)npr * ngw);
-    double* t = &twnl[twnl_off[is]];
-    std::vector<int> lp(npr);
+    const size_t toff = twnl.size(), doff = dtwnl.size();
+    twnl.resize(toff + (size_t)npr * ngw);
+    dtwnl.resize(doff + (size_t)npr * 6 * ngw);
+    double* t = &twnl[toff];
     for (int ig = 0; ig < ngw; ig++) {
       double v, dv;
       q.dvnlg(0, e->wb.g[ig], v, dv);
@@ -274,9 +198,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       }
     }
     // dtwnl (src/NonLocalPotential.cpp:283-306 for l = 0, :426-512 for l = 1)
-    dt_off[is] = (int64_t)dtwnl.size();
-    dtwnl.resize(dtwnl.size() + (size_t)npr * 6 * ngw);
-    double* d = &dtwnl[dt_off[is]];
+    double* d = &dtwnl[doff];
     for (int ig = 0; ig < ngw; ig++) {
       double v, dv;
       const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
@@ -303,171 +225,351 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
         d[(size_t)6 * 3 * ngw + 5 * ngw + ig] += v * yv[0];   // m=z: xz
       }
     }
-    sp_npr[is] = npr;
-    fac2[is] = 2.0 * 0.25 * q.rcps * q.rcps;
-    lp[0] = 0;
-    for (int j = 1; j < npr; j++) lp[j] = 1;
-    // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
-    const int dbase = (int)dmat.size();
-    for (int i = 0; i < npr; i++)
-      for (int j = 0; j < npr; j++) dmat.push_back(i == j ? q.dl[lp[i]] : 0.0);
-    const int Ms = (int)m_atom.size();
-    for (int ipr = 0; ipr < npr; ipr++)
-      for (int ia = 0; ia < e->na[is]; ia++) {
-        const int a = e->sp_first[is] + ia;
-        m_atom.push_back(a);
-        m_twnl.push_back(twnl_off[is] + ipr * ngw);
-        m_l.push_back(lp[ipr]);
-        m_ipr.push_back(ipr);
-        m_npr.push_back(npr);
-        m_stride.push_back(e->na[is]);
-        m_dbase.push_back(dbase);
-        m_wt.push_back(1.0);   // wsg = 1 for SLPP (src/Species.cpp:733)
+  }
+}
+
+static int ef_atoms_moved(qbg_ef* e) {
+  qbg_ctx* c = e->ctx;
+  hipStream_t s = c->stream;
+  QBG_HIP(c, hipMemcpyAsync(e->d_tau, e->tau.data(), sizeof(double) * e->tau.size(),
+                            hipMemcpyHostToDevice, s));
+  c->prof_begin(QBG_K_OTHER);
+  launch_sf_tables(e->d_tau, e->nat, e->d_b, e->kmin, e->krange, e->d_sftab, s);
+  launch_sfac(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->nsp, e->d_sp_first,
+              e->d_vps, e->d_dvps, e->d_rhops, e->d_sfac, e->d_vion, e->d_dvion, e->d_rhopst, s);
+  c->prof_end();
+  ef_esr(e);
+  QBG_HIP(c, hipMemcpyAsync(e->d_fesr, e->fesr.data(), sizeof(double) * e->fesr.size(),
+                            hipMemcpyHostToDevice, s));
+  QBG_HIP(c, hipStreamSynchronize(s));
+  e->have_vhxc = false;
+  return QBG_OK;
+}
+
+extern "C" {
+
+int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut, int nsp,
+                  const qbg_species_desc* spd, const int* na, const double* tau, int nst,
+                  const double* occ, qbg_ef** out) {
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
+      return QBG_E_INVAL;
+    *out = nullptr;
+    QBG_HIP(c, hipSetDevice(c->device));
+    qbg_ef* e = new (std::nothrow) qbg_ef;
+    if (!e) return QBG_E_NOMEM;
+    e->ctx = c;
+    e->cell.set(cell9);
+    e->omega = e->cell.volume;
+    e->ecut = ecut;
+    e->nsp = nsp;
+    e->nst = nst;
+    e->occ.assign(occ, occ + nst);
+    e->sp_first.assign(1, 0);
+    for (int is = 0; is < nsp; is++) {
+      host::Species q;
+      const qbg_species_desc& d = spd[is];
+      q.zval = d.zval; q.nmesh = d.nmesh; q.deltar = d.deltar; q.rcps = d.rcps;
+      bool ok;
+      if (d.type == QBG_SPECIES_NCPP) {
+        ok = d.vps && d.phi && d.nmesh > 0 &&
+             host::ncpp_to_kb(d.lmax, d.llocal, d.nmesh, d.deltar, d.vps, d.phi, q.vloc, q.proj,
+                              q.dl, q.lmax);
+      } else {
+        ok = d.type == QBG_SPECIES_SLPP && d.vloc && d.proj && d.dl && d.lmax >= 0 &&
+             d.lmax <= 1 && d.nmesh > 0;
+        if (ok) {
+          q.lmax = d.lmax;
+          q.vloc.assign(d.vloc, d.vloc + q.nmesh);
+          q.proj.assign(d.proj, d.proj + (size_t)(q.lmax + 1) * q.nmesh);
+          q.dl.assign(d.dl, d.dl + q.lmax + 1);
+        }
       }
-    for (int ia = 0; ia < e->na[is]; ia++) {
-      const int a = e->sp_first[is] + ia;
-      a_m0[a] = Ms + ia;
-      a_npr[a] = npr;
-      a_stride[a] = e->na[is];
+      if (!ok || !q.init()) {
+        delete e;
+        return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 1?)");
+      }
+      e->sp.push_back(q);
+      e->na.push_back(na[is]);
+      e->sp_first.push_back(e->sp_first.back() + na[is]);
     }
-  }
-  e->M = (int)m_atom.size();
-  // structure factor table extents (src/StructureFactor.cpp:27-39)
-  int krange = 0;
-  for (int i = 0; i < 3; i++) {
-    e->kmin[i] = e->vb.idxmin[i];
-    krange = std::max(krange, e->vb.idxmax[i] - e->vb.idxmin[i] + 1);
-  }
-  e->krange = krange;
-  // atom groups of the local-force kernel: consecutive atoms of one species
-  std::vector<int32_t> lfgrp;
-  {
-    const int ag = local_forces_group_size(krange);
-    for (int is = 0; is < nsp; is++)
-      for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a += ag) {
-        lfgrp.push_back(a);
-        lfgrp.push_back(std::min(ag, e->sp_first[is + 1] - a));
-        lfgrp.push_back(is);
-      }
-    e->nlfgrp = (int)lfgrp.size() / 3;
-  }
-  std::vector<double> bm(9);
-  for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
-  std::vector<int32_t> atom_sp(e->nat), spf(e->sp_first.begin(), e->sp_first.end());
-  for (int is = 0; is < nsp; is++)
-    for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a++) atom_sp[a] = is;
-  std::vector<double> g2iv(e->vb.g2i), giv(e->vb.gi);
-  const long N = e->N;
-  const size_t nbmax = (size_t)std::max<long>((N + 255) / 256, (ngv + 255) / 256);
-  e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
-  e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
-  e->partial_doubles = std::max<size_t>(e->partial_doubles,
-                                        (size_t)((ngw + 255) / 256) * ((nst + 31) / 32) * 8);
-  e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
-  // split-K partials of the fnl product: up to 40 splits of M x nst
-  e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
-  if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
-      (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
-      (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
-      (rc = dput(c, &e->d_vps, vps)) || (rc = dput(c, &e->d_dvps, dvps)) ||
-      (rc = dput(c, &e->d_rhops, rhops)) || (rc = dput(c, &e->d_twnl, twnl)) ||
-      (rc = dput(c, &e->d_b, bm)) || (rc = dput(c, &e->d_sp_first, spf)) ||
-      (rc = dput(c, &e->d_atom_sp, atom_sp)) || (rc = dput(c, &e->d_occ, e->occ)) ||
-      (rc = dput(c, &e->d_m_atom, m_atom)) || (rc = dput(c, &e->d_m_twnl, m_twnl)) ||
-      (rc = dput(c, &e->d_m_l, m_l)) || (rc = dput(c, &e->d_m_ipr, m_ipr)) ||
-      (rc = dput(c, &e->d_m_npr, m_npr)) || (rc = dput(c, &e->d_m_stride, m_stride)) ||
-      (rc = dput(c, &e->d_m_dbase, m_dbase)) || (rc = dput(c, &e->d_m_wt, m_wt)) ||
-      (rc = dput(c, &e->d_dmat, dmat)) || (rc = dput(c, &e->d_a_m0, a_m0)) ||
-      (rc = dput(c, &e->d_a_npr, a_npr)) || (rc = dput(c, &e->d_a_stride, a_stride)) ||
-      (rc = dput(c, &e->d_dtwnl, dtwnl)) || (rc = dput(c, &e->d_dt_off, dt_off)) ||
-      (rc = dput(c, &e->d_lfgrp, lfgrp)) || (rc = dalloc(c, &e->d_vtemp, (size_t)2 * nsp * ngv)) ||
-      (rc = dput(c, &e->d_sp_npr, sp_npr)) || (rc = dput(c, &e->d_fac2, fac2)) ||
-      (rc = dalloc(c, &e->d_tau, 3 * e->nat)) ||
-      (rc = dalloc(c, &e->d_sftab, (size_t)e->nat * 3 * krange * 2)) ||
-      (rc = dalloc(c, &e->d_sfac, (size_t)2 * nsp * ngv)) ||
-      (rc = dalloc(c, &e->d_vion, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_dvion, (size_t)2 * ngv)) ||
-      (rc = dalloc(c, &e->d_rhopst, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhor, N)) ||
-      (rc = dalloc(c, &e->d_vxc, N)) || (rc = dalloc(c, &e->d_vr, N)) ||
-      (rc = dalloc(c, &e->d_rhog, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhoelg, (size_t)2 * ngv)) ||
-      (rc = dalloc(c, &e->d_rhogt, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_vlocal, (size_t)2 * ngv)) ||
-      (rc = dalloc(c, &e->d_fesr, 3 * e->nat)) || (rc = dalloc(c, &e->d_fion, 3 * e->nat)) ||
-      (rc = dalloc(c, &e->d_scal, S_COUNT)) || (rc = dalloc(c, &e->d_partial, e->partial_doubles)) ||
-      (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngw)) ||
-      (rc = dalloc(c, &e->d_fnl, (size_t)2 * e->M * nst)) ||
-      (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
-      (rc = dalloc(c, &e->d_work, e->work_doubles))) {
-    ef_free(e);
-    delete e;
-    return rc;
-  }
-  // density factors and result staging
-  {
-    std::vector<double> dfac = density_factors(e);
-    e->nres = S_COUNT + 4 + 3 * e->nat;
-    std::vector<const double*> segp = {e->d_scal, e->d_rhoelg, e->d_vion, e->d_fion};
-    std::vector<int> segn = {S_COUNT, 2, 2, 3 * e->nat};
-    if ((rc = dput(c, &e->d_dfac, dfac)) || (rc = dput(c, &e->d_seg_ptr, segp)) ||
-        (rc = dput(c, &e->d_seg_n, segn))) {
+    e->nat = e->sp_first.back();
+    e->tau.assign(tau, tau + 3 * e->nat);
+    for (int is = 0; is < nsp; is++) e->eself += e->na[is] * e->sp[is].eself();
+    // bases and grid (src/ChargeDensity.cpp:34-98)
+    host::Cell def = e->cell;
+    if (refcell9) def.set(refcell9);
+    e->defcell = def;
+    e->has_ref = refcell9 != nullptr;
+    e->wb.build(def, e->cell, ecut);
+    e->vb.build(def, e->cell, 4.0 * ecut);
+    for (int i = 0; i < 3; i++) {
+      int n = e->vb.np[i] + 2;
+      while (!host::factorizable(n)) n += 2;
+      e->np[i] = n;
+    }
+    e->N = (long)e->np[0] * e->np[1] * e->np[2];
+    if (!e->wm.build(e->wb, e->np[0], e->np[1], e->np[2]) ||
+        !e->vm.build(e->vb, e->np[0], e->np[1], e->np[2])) {
+      delete e;
+      return c->fail(QBG_E_INVAL, "basis does not fit in the grid");
+    }
+    int rc;
+    if ((rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->wb.size, e->wm.nvec,
+                              e->wm.ip.data(), e->wm.im.data(), e->wm.zvec_to_val.data(), &e->gw)) ||
+        (rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->vb.size, e->vm.nvec,
+                              e->vm.ip.data(), e->vm.im.data(), e->vm.zvec_to_val.data(), &e->gv))) {
       ef_free(e);
       delete e;
       return rc;
     }
-    if (hipHostMalloc((void**)&e->h_res, sizeof(double) * e->nres, hipHostMallocDefault) !=
-        hipSuccess) {
-      e->h_res = nullptr;
+    const int ngw = e->wb.size, ngv = e->vb.size;
+    // cell_moved tables (src/EnergyFunctional.cpp:1200-1245)
+    std::vector<double> vps, dvps, rhops, twnl, dtwnl;
+    ef_local_tables(e, vps, dvps, rhops);
+    ef_twnl(e, twnl, dtwnl);
+    // projector layout: species-major, ipr-major within a species, atom-minor
+    // (src/NonLocalPotential.cpp:1341-1395)
+    std::vector<int> twnl_off(nsp);
+    std::vector<int32_t> m_atom, m_twnl, m_l, m_ipr, m_npr, m_stride, m_dbase, a_m0(e->nat),
+        a_npr(e->nat), a_stride(e->nat);
+    std::vector<double> m_wt, dmat, fac2(nsp);
+    std::vector<int64_t> dt_off(nsp);
+    std::vector<int32_t> sp_npr(nsp);
+    for (int is = 0, tw = 0; is < nsp; is++) {
+      const host::Species& q = e->sp[is];
+      const int npr = q.nlm;
+      twnl_off[is] = tw;
+      dt_off[is] = (int64_t)6 * tw;
+      tw += npr * ngw;
+      std::vector<int> lp(npr);
+      sp_npr[is] = npr;
+      fac2[is] = 2.0 * 0.25 * q.rcps * q.rcps;
+      lp[0] = 0;
+      for (int j = 1; j < npr; j++) lp[j] = 1;
+      // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
+      const int dbase = (int)dmat.size();
+      for (int i = 0; i < npr; i++)
+        for (int j = 0; j < npr; j++) dmat.push_back(i == j ? q.dl[lp[i]] : 0.0);
+      const int Ms = (int)m_atom.size();
+      for (int ipr = 0; ipr < npr; ipr++)
+        for (int ia = 0; ia < e->na[is]; ia++) {
+          const int a = e->sp_first[is] + ia;
+          m_atom.push_back(a);
+          m_twnl.push_back(twnl_off[is] + ipr * ngw);
+          m_l.push_back(lp[ipr]);
+          m_ipr.push_back(ipr);
+          m_npr.push_back(npr);
+          m_stride.push_back(e->na[is]);
+          m_dbase.push_back(dbase);
+          m_wt.push_back(1.0);   // wsg = 1 for SLPP (src/Species.cpp:733)
+        }
+      for (int ia = 0; ia < e->na[is]; ia++) {
+        const int a = e->sp_first[is] + ia;
+        a_m0[a] = Ms + ia;
+        a_npr[a] = npr;
+        a_stride[a] = e->na[is];
+      }
+    }
+    e->M = (int)m_atom.size();
+    // structure factor table extents (src/StructureFactor.cpp:27-39)
+    int krange = 0;
+    for (int i = 0; i < 3; i++) {
+      e->kmin[i] = e->vb.idxmin[i];
+      krange = std::max(krange, e->vb.idxmax[i] - e->vb.idxmin[i] + 1);
+    }
+    e->krange = krange;
+    // atom groups of the local-force kernel: consecutive atoms of one species
+    std::vector<int32_t> lfgrp;
+    {
+      const int ag = local_forces_group_size(krange);
+      for (int is = 0; is < nsp; is++)
+        for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a += ag) {
+          lfgrp.push_back(a);
+          lfgrp.push_back(std::min(ag, e->sp_first[is + 1] - a));
+          lfgrp.push_back(is);
+        }
+      e->nlfgrp = (int)lfgrp.size() / 3;
+    }
+    std::vector<double> bm(9);
+    for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
+    std::vector<int32_t> atom_sp(e->nat), spf(e->sp_first.begin(), e->sp_first.end());
+    for (int is = 0; is < nsp; is++)
+      for (int a = e->sp_first[is]; a < e->sp_first[is + 1]; a++) atom_sp[a] = is;
+    std::vector<double> g2iv(e->vb.g2i), giv(e->vb.gi);
+    const long N = e->N;
+    const size_t nbmax = (size_t)std::max<long>((N + 255) / 256, (ngv + 255) / 256);
+    e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
+    e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
+    e->partial_doubles = std::max<size_t>(e->partial_doubles,
+                                          (size_t)((ngw + 255) / 256) * ((nst + 31) / 32) * 8);
+    e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
+    // split-K partials of the fnl product: up to 40 splits of M x nst
+    e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
+    if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
+        (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
+        (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
+        (rc = dput(c, &e->d_vps, vps)) || (rc = dput(c, &e->d_dvps, dvps)) ||
+        (rc = dput(c, &e->d_rhops, rhops)) || (rc = dput(c, &e->d_twnl, twnl)) ||
+        (rc = dput(c, &e->d_b, bm)) || (rc = dput(c, &e->d_sp_first, spf)) ||
+        (rc = dput(c, &e->d_atom_sp, atom_sp)) || (rc = dput(c, &e->d_occ, e->occ)) ||
+        (rc = dput(c, &e->d_m_atom, m_atom)) || (rc = dput(c, &e->d_m_twnl, m_twnl)) ||
+        (rc = dput(c, &e->d_m_l, m_l)) || (rc = dput(c, &e->d_m_ipr, m_ipr)) ||
+        (rc = dput(c, &e->d_m_npr, m_npr)) || (rc = dput(c, &e->d_m_stride, m_stride)) ||
+        (rc = dput(c, &e->d_m_dbase, m_dbase)) || (rc = dput(c, &e->d_m_wt, m_wt)) ||
+        (rc = dput(c, &e->d_dmat, dmat)) || (rc = dput(c, &e->d_a_m0, a_m0)) ||
+        (rc = dput(c, &e->d_a_npr, a_npr)) || (rc = dput(c, &e->d_a_stride, a_stride)) ||
+        (rc = dput(c, &e->d_dtwnl, dtwnl)) || (rc = dput(c, &e->d_dt_off, dt_off)) ||
+        (rc = dput(c, &e->d_lfgrp, lfgrp)) || (rc = dalloc(c, &e->d_vtemp, (size_t)2 * nsp * ngv)) ||
+        (rc = dput(c, &e->d_sp_npr, sp_npr)) || (rc = dput(c, &e->d_fac2, fac2)) ||
+        (rc = dalloc(c, &e->d_tau, 3 * e->nat)) ||
+        (rc = dalloc(c, &e->d_sftab, (size_t)e->nat * 3 * krange * 2)) ||
+        (rc = dalloc(c, &e->d_sfac, (size_t)2 * nsp * ngv)) ||
+        (rc = dalloc(c, &e->d_vion, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_dvion, (size_t)2 * ngv)) ||
+        (rc = dalloc(c, &e->d_rhopst, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhor, N)) ||
+        (rc = dalloc(c, &e->d_vxc, N)) || (rc = dalloc(c, &e->d_vr, N)) ||
+        (rc = dalloc(c, &e->d_rhog, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_rhoelg, (size_t)2 * ngv)) ||
+        (rc = dalloc(c, &e->d_rhogt, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_vlocal, (size_t)2 * ngv)) ||
+        (rc = dalloc(c, &e->d_fesr, 3 * e->nat)) || (rc = dalloc(c, &e->d_fion, 3 * e->nat)) ||
+        (rc = dalloc(c, &e->d_scal, S_COUNT)) || (rc = dalloc(c, &e->d_partial, e->partial_doubles)) ||
+        (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngw)) ||
+        (rc = dalloc(c, &e->d_fnl, (size_t)2 * e->M * nst)) ||
+        (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
+        (rc = dalloc(c, &e->d_work, e->work_doubles))) {
       ef_free(e);
       delete e;
-      return c->fail(QBG_E_NOMEM, "pinned result buffer");
+      return rc;
     }
-  }
-  if ((rc = ef_atoms_moved(e))) {
-    ef_free(e);
-    delete e;
-    return rc;
-  }
-  *out = e;
-  return QBG_OK;
+    // density factors and result staging
+    {
+      std::vector<double> dfac = density_factors(e);
+      e->nres = S_COUNT + 4 + 3 * e->nat;
+      std::vector<const double*> segp = {e->d_scal, e->d_rhoelg, e->d_vion, e->d_fion};
+      std::vector<int> segn = {S_COUNT, 2, 2, 3 * e->nat};
+      if ((rc = dput(c, &e->d_dfac, dfac)) || (rc = dput(c, &e->d_seg_ptr, segp)) ||
+          (rc = dput(c, &e->d_seg_n, segn))) {
+        ef_free(e);
+        delete e;
+        return rc;
+      }
+      if (hipHostMalloc((void**)&e->h_res, sizeof(double) * e->nres, hipHostMallocDefault) !=
+          hipSuccess) {
+        e->h_res = nullptr;
+        ef_free(e);
+        delete e;
+        return c->fail(QBG_E_NOMEM, "pinned result buffer");
+      }
+    }
+    if ((rc = ef_atoms_moved(e))) {
+      ef_free(e);
+      delete e;
+      return rc;
+    }
+    *out = e;
+    return QBG_OK;
+  });
 }
 
 int qbg_ef_destroy(qbg_ef* e) {
-  if (!e) return QBG_E_INVAL;
-  (void)hipSetDevice(e->ctx->device);
-  (void)hipStreamSynchronize(e->ctx->stream);
-  ef_free(e);
-  delete e;
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e) return QBG_E_INVAL;
+    QBG_HIP(e->ctx, hipSetDevice(e->ctx->device));
+    (void)hipStreamSynchronize(e->ctx->stream);
+    ef_free(e);
+    delete e;
+    return QBG_OK;
+  });
 }
 
 // out: ngw, ngv, np0, np1, np2, nvec_w, nvec_v, nat, M, nst
 int qbg_ef_sizes(const qbg_ef* e, int* out) {
-  if (!e || !out) return QBG_E_INVAL;
-  const int v[10] = {e->wb.size, e->vb.size, e->np[0], e->np[1], e->np[2], e->wm.nvec,
-                     e->vm.nvec, e->nat, e->M, e->nst};
-  std::memcpy(out, v, sizeof(v));
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !out) return QBG_E_INVAL;
+    const int v[10] = {e->wb.size, e->vb.size, e->np[0], e->np[1], e->np[2], e->wm.nvec,
+                       e->vm.nvec, e->nat, e->M, e->nst};
+    std::memcpy(out, v, sizeof(v));
+    return QBG_OK;
+  });
 }
 
 // host copies of a basis (which 0: wavefunctions, 1: density):
 // idx[3*ng], g2[ng], gx[3*ng] (each may be null), and the maps
 int qbg_ef_basis(const qbg_ef* e, int which, int32_t* idx, double* g2, double* gx, int32_t* ip,
                  int32_t* im, int32_t* zvec_to_val) {
-  if (!e) return QBG_E_INVAL;
-  const host::Basis& b = which ? e->vb : e->wb;
-  const host::Maps& m = which ? e->vm : e->wm;
-  if (idx) std::memcpy(idx, b.idx.data(), sizeof(int32_t) * b.idx.size());
-  if (g2) std::memcpy(g2, b.g2.data(), sizeof(double) * b.size);
-  if (gx) std::memcpy(gx, b.gx.data(), sizeof(double) * b.gx.size());
-  if (ip) std::memcpy(ip, m.ip.data(), sizeof(int32_t) * m.ip.size());
-  if (im) std::memcpy(im, m.im.data(), sizeof(int32_t) * m.im.size());
-  if (zvec_to_val) std::memcpy(zvec_to_val, m.zvec_to_val.data(), sizeof(int32_t) * m.nvec);
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e) return QBG_E_INVAL;
+    const host::Basis& b = which ? e->vb : e->wb;
+    const host::Maps& m = which ? e->vm : e->wm;
+    if (idx) std::memcpy(idx, b.idx.data(), sizeof(int32_t) * b.idx.size());
+    if (g2) std::memcpy(g2, b.g2.data(), sizeof(double) * b.size);
+    if (gx) std::memcpy(gx, b.gx.data(), sizeof(double) * b.gx.size());
+    if (ip) std::memcpy(ip, m.ip.data(), sizeof(int32_t) * m.ip.size());
+    if (im) std::memcpy(im, m.im.data(), sizeof(int32_t) * m.im.size());
+    if (zvec_to_val) std::memcpy(zvec_to_val, m.zvec_to_val.data(), sizeof(int32_t) * m.nvec);
+    return QBG_OK;
+  });
 }
 
 int qbg_ef_set_positions(qbg_ef* e, const double* tau) {
-  if (!e || !tau) return QBG_E_INVAL;
-  (void)hipSetDevice(e->ctx->device);
-  e->tau.assign(tau, tau + 3 * e->nat);
-  return ef_atoms_moved(e);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !tau) return QBG_E_INVAL;
+    QBG_HIP(e->ctx, hipSetDevice(e->ctx->device));
+    e->tau.assign(tau, tau + 3 * e->nat);
+    return ef_atoms_moved(e);
+  });
+}
+
+// Wavefunction::resize to a new cell over the same reference cell, then
+// EnergyFunctional::cell_moved (src/EnergyFunctional.cpp:1200-1245):
+// basis vectors of both bases (membership and grid unchanged), the local
+// tables, update_twnl, and atoms_moved at the current Cartesian positions
+int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !cell9) return QBG_E_INVAL;
+    qbg_ctx* c = e->ctx;
+    if (c->capturing) return c->fail(QBG_E_STATE, "cell_moved during graph capture");
+    host::Cell cell;
+    cell.set(cell9);
+    if (!(cell.volume > 0.0)) return c->fail(QBG_E_INVAL, "cell_moved: singular cell");
+    // without a reference cell the basis follows the cell (defcell = cell,
+    // src/Basis.cpp:314-325); a change of membership or grid needs a new
+    // functional
+    const host::Cell& def = e->has_ref ? e->defcell : cell;
+    host::Basis wb, vb;
+    wb.build(def, cell, e->ecut);
+    vb.build(def, cell, 4.0 * e->ecut);
+    bool same = wb.idx == e->wb.idx && vb.idx == e->vb.idx;
+    for (int i = 0; i < 3 && same; i++) {
+      int n = vb.np[i] + 2;
+      while (!host::factorizable(n)) n += 2;
+      same = n == e->np[i];
+    }
+    if (!same)
+      return c->fail(QBG_E_INVAL, "cell_moved: the basis or grid would change (no reference "
+                                  "cell covers the new cell); create the functional again");
+    QBG_HIP(c, hipSetDevice(c->device));
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    e->cell = cell;
+    e->omega = cell.volume;
+    e->wb = std::move(wb);
+    e->vb = std::move(vb);
+    std::vector<double> vps, dvps, rhops, twnl, dtwnl;
+    ef_local_tables(e, vps, dvps, rhops);
+    ef_twnl(e, twnl, dtwnl);
+    std::vector<double> bm(9);
+    for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
+    int rc;
+    if ((rc = dset(c, e->d_g2w, e->wb.g2)) || (rc = dset(c, e->d_gxw, e->wb.gx)) ||
+        (rc = dset(c, e->d_g2iv, e->vb.g2i)) || (rc = dset(c, e->d_giv, e->vb.gi)) ||
+        (rc = dset(c, e->d_gxv, e->vb.gx)) || (rc = dset(c, e->d_vps, vps)) ||
+        (rc = dset(c, e->d_dvps, dvps)) || (rc = dset(c, e->d_rhops, rhops)) ||
+        (rc = dset(c, e->d_twnl, twnl)) || (rc = dset(c, e->d_dtwnl, dtwnl)) ||
+        (rc = dset(c, e->d_b, bm)) || (rc = dset(c, e->d_dfac, density_factors(e))))
+      return rc;
+    e->have_rho = false;
+    e->xres_c = nullptr;
+    return ef_atoms_moved(e);
+  });
 }
 
 // ChargeDensity::update_density (src/ChargeDensity.cpp:136-198):
@@ -478,7 +580,7 @@ int qbg_ef_set_positions(qbg_ef* e, const double* tau) {
 static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool keep) {
   if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
-  (void)hipSetDevice(ctx->device);
+  QBG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
   fill_zero(e->d_rhor, e->N, s);
@@ -534,65 +636,72 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
 }
 
 int qbg_ef_update_density(qbg_ef* e, const double* c, int ldc, int prec) {
-  return ef_update_density(e, c, ldc, prec, true);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    return ef_update_density(e, c, ldc, prec, true);
+  });
 }
 
 // EnergyFunctional::update_vhxc (src/EnergyFunctional.cpp:231-394), LDA
 int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
-  if (!e) return QBG_E_INVAL;
-  qbg_ctx* ctx = e->ctx;
-  if (!e->have_rho) return ctx->fail(QBG_E_STATE, "update_vhxc before update_density");
-  (void)hipSetDevice(ctx->device);
-  hipStream_t s = ctx->stream;
-  ctx->prof_begin(QBG_K_OTHER);
-  if (e->nspin == 2)
-    launch_lsda(e->d_rhor, e->d_rhor2, e->N, e->d_vxc, e->d_vxc2, e->d_partial,
-                e->d_scal + S_LDA, s);
-  else
-    launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
-  launch_vhxc_g(e->d_rhog, e->vb.size, 1.0 / e->omega, e->d_vion, e->d_dvion, e->d_rhopst,
-                e->d_g2iv, e->d_giv, e->d_gxv, e->d_rhoelg, e->d_rhogt, e->d_vlocal,
-                compute_stress, e->d_partial, e->d_scal + S_VHXC, s);
-  ctx->prof_end();
-  qbg_grid* g = e->gv;
-  int rc = grid_ensure_ws(g, 1);
-  if (rc) return rc;
-  int r1, r2, r3;
-  ctx->prof_begin(QBG_K_VGRID);
-  r1 = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_vlocal, 0, 0, 0, 1, s);
-  ctx->prof_end();
-  ctx->prof_begin(QBG_K_VGRID);
-  r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
-  ctx->prof_end();
-  ctx->prof_begin(QBG_K_VGRID);
-  // v_r[ispin] = vxc_r[ispin] + Re(vlocal) (src/EnergyFunctional.cpp:372-389)
-  if (e->nspin == 2) {
-    r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
-                         e->d_vloc, 1.0, s);
-    if (!r3) {
-      add2(e->d_vxc, e->d_vloc, e->d_vr, e->N, s);
-      add2(e->d_vxc2, e->d_vloc, e->d_vr2, e->N, s);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e) return QBG_E_INVAL;
+    qbg_ctx* ctx = e->ctx;
+    if (!e->have_rho) return ctx->fail(QBG_E_STATE, "update_vhxc before update_density");
+    QBG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    ctx->prof_begin(QBG_K_OTHER);
+    if (e->nspin == 2)
+      launch_lsda(e->d_rhor, e->d_rhor2, e->N, e->d_vxc, e->d_vxc2, e->d_partial,
+                  e->d_scal + S_LDA, s);
+    else
+      launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
+    launch_vhxc_g(e->d_rhog, e->vb.size, 1.0 / e->omega, e->d_vion, e->d_dvion, e->d_rhopst,
+                  e->d_g2iv, e->d_giv, e->d_gxv, e->d_rhoelg, e->d_rhogt, e->d_vlocal,
+                  compute_stress, e->d_partial, e->d_scal + S_VHXC, s);
+    ctx->prof_end();
+    qbg_grid* g = e->gv;
+    int rc = grid_ensure_ws(g, 1);
+    if (rc) return rc;
+    int r1, r2, r3;
+    ctx->prof_begin(QBG_K_VGRID);
+    r1 = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_vlocal, 0, 0, 0, 1, s);
+    ctx->prof_end();
+    ctx->prof_begin(QBG_K_VGRID);
+    r2 = r1 ? r1 : xbwd(g, QBG_PREC_FP64, 1, s);
+    ctx->prof_end();
+    ctx->prof_begin(QBG_K_VGRID);
+    // v_r[ispin] = vxc_r[ispin] + Re(vlocal) (src/EnergyFunctional.cpp:372-389)
+    if (e->nspin == 2) {
+      r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
+                           e->d_vloc, 1.0, s);
+      if (!r3) {
+        add2(e->d_vxc, e->d_vloc, e->d_vr, e->N, s);
+        add2(e->d_vxc2, e->d_vloc, e->d_vr2, e->N, s);
+      }
+    } else {
+      r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr,
+                           e->d_vr, 1.0, s);
     }
-  } else {
-    r3 = r2 ? r2 : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, e->d_vxc, nullptr, nullptr,
-                         e->d_vr, 1.0, s);
-  }
-  ctx->prof_end();
-  if (r3) return ctx->fail(r3, "update_vhxc: density-grid transform refused");
-  e->have_vhxc = true;
-  e->vhxc_stress = compute_stress != 0;
-  hipError_t er = hipGetLastError();
-  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
-  return QBG_OK;
+    ctx->prof_end();
+    if (r3) return ctx->fail(r3, "update_vhxc: density-grid transform refused");
+    e->have_vhxc = true;
+    e->vhxc_stress = compute_stress != 0;
+    hipError_t er = hipGetLastError();
+    if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+    return QBG_OK;
+  });
 }
 
 // EnergyFunctional::energy's nonlocal part (src/NonLocalPotential.cpp:1258-2069)
 // with the projector GEMMs on FP64 or FP32 MFMA (gemm.hip)
+// over the bands [b0, b0 + nst) of the functional (c and hpsi point at band
+// b0; their occupations are occ[b0 + n])
 static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
-                   int compute_forces, int compute_stress, int prec) {
+                   int compute_forces, int compute_stress, int prec, int b0, int nst) {
   qbg_ctx* ctx = e->ctx;
   hipStream_t s = ctx->stream;
-  const int ngw = e->wb.size, nst = e->nst, M = e->M;
+  const int ngw = e->wb.size, M = e->M;
+  const double* occ = e->d_occ + b0;
   const long K = 2L * ngw;
   ctx->prof_begin(QBG_K_NL);
   const bool f32 = prec == QBG_PREC_FP32;
@@ -608,7 +717,7 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_end();
   if (grc) return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
   ctx->prof_begin(QBG_K_NL);
-  launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
+  launch_dmat_enl(e->d_fnl, M, nst, occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
                   e->d_scal + S_ENL, s);
   ctx->prof_end();
@@ -627,7 +736,7 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces)
       fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
-    launch_force_w(e->d_fout, e->d_occ, M, nst, W, s);
+    launch_force_w(e->d_fout, occ, M, nst, W, s);
     ctx->prof_end();
     ctx->prof_begin(QBG_K_GEMM);
     if (f32) gemm_nn(c, 2L * ldc, W, nst, K, M, nst, (float*)Y, K, 0, prec, s);
@@ -701,18 +810,22 @@ static int bf16_ensure(qbg_ef* e) {
 
 // the nonlocal part with the projector GEMMs as BF16-split products
 // (gemm_bf.hip) on FP32 copies of anl, c, fout and W
+// (bands [b0, b0 + nst) as nl_mfma; band columns [nst, e->nst) of the FP32
+// copies are zeroed, so the padded band sums see zeros there)
 static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
-                   int compute_forces, int compute_stress, int parts, bool c32_ready) {
+                   int compute_forces, int compute_stress, int parts, bool c32_ready, int b0,
+                   int nst) {
   qbg_ctx* ctx = e->ctx;
   hipStream_t s = ctx->stream;
   int rc = bf16_ensure(e);
   if (rc) return rc;
-  const int ngw = e->wb.size, nst = e->nst, M = e->M, Mp = e->Mp, Np = e->Np;
+  const int ngw = e->wb.size, M = e->M, Mp = e->Mp, Np = e->Np;
+  const double* occ = e->d_occ + b0;
   const long K = 2L * ngw, Kp = e->Kp;
   ctx->prof_begin(QBG_K_NL);
   launch_anl(ngw, e->nat, Kp, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
              e->d_m_twnl, e->d_m_l, e->d_twnl, e->b_anl, 1, s);
-  if (!c32_ready) convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s);
+  if (!c32_ready) convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s, e->nst);
   ctx->prof_end();
   ctx->prof_begin(QBG_K_GEMM);
   rc = gemm_bf_tn(e->b_anl, Kp, e->b_c32, Kp, Mp, Np, Kp, M, nst, c, 2L * ldc, 2.0, -1.0,
@@ -720,11 +833,12 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_end();
   if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
   ctx->prof_begin(QBG_K_NL);
-  launch_dmat_enl(e->d_fnl, M, nst, e->d_occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
+  launch_dmat_enl(e->d_fnl, M, nst, occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
                   e->d_scal + S_ENL, s);
   const bool needy = compute_forces || compute_stress;
-  fout_f32(e->d_fout, e->d_occ, M, nst, e->b_fout, Mp, needy ? e->b_w32 : nullptr, Np, s);
+  fout_f32(e->d_fout, occ, M, nst, e->b_fout, Mp, needy ? e->b_w32 : nullptr, Np, s,
+           e->nst);
   ctx->prof_end();
   if (compute_hpsi) {
     ctx->prof_begin(QBG_K_GEMM);
@@ -785,7 +899,7 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;   // band FFTs
-  (void)hipSetDevice(ctx->device);
+  QBG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const int ngw = e->wb.size, nst = e->nst, M = e->M;
   // kinetic energy and hpsi = 0.5 kpg2 c (this store replaces the clear of dwf);
@@ -806,9 +920,9 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   const long K = 2L * ngw;
   int nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
                 ? nl_bf16(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec == QBG_PREC_BF16X3 ? 3 : 2, true)
+                          prec == QBG_PREC_BF16X3 ? 3 : 2, true, 0, nst)
                 : nl_mfma(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec);
+                          prec, 0, nst);
   if (nrc) return nrc;
   if (compute_stress) {
     ctx->prof_begin(QBG_K_OTHER);
@@ -905,178 +1019,291 @@ static int ef_collect(qbg_ef* e, double* energies, double* fion, double* sigma) 
 int qbg_ef_energy(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
                   int compute_forces, int compute_stress, int prec, double* energies,
                   double* fion, double* sigma) {
-  if (!e || !c || !energies) return QBG_E_INVAL;
-  int rc = ef_launch(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress, prec);
-  if (rc) return rc;
-  return ef_collect(e, energies, fion, sigma);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !c || !energies) return QBG_E_INVAL;
+    int rc = ef_launch(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress, prec);
+    if (rc) return rc;
+    return ef_collect(e, energies, fion, sigma);
+  });
 }
 
 int qbg_ef_evaluate_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int ldh, int flags,
                           int prec) {
-  if (!e || !c) return QBG_E_INVAL;
-  const int hp = (flags & QBG_EF_HPSI) != 0, fo = (flags & QBG_EF_FORCES) != 0,
-            st = (flags & QBG_EF_STRESS) != 0;
-  int rc = ef_update_density(e, c, ldc, prec, hp != 0);
-  if (!rc) rc = qbg_ef_update_vhxc(e, st);
-  if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec, true);
-  return rc;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !c) return QBG_E_INVAL;
+    const int hp = (flags & QBG_EF_HPSI) != 0, fo = (flags & QBG_EF_FORCES) != 0,
+              st = (flags & QBG_EF_STRESS) != 0;
+    int rc = ef_update_density(e, c, ldc, prec, hp != 0);
+    if (!rc) rc = qbg_ef_update_vhxc(e, st);
+    if (!rc) rc = ef_launch(e, c, ldc, hp, hpsi, ldh, fo, st, prec, true);
+    return rc;
+  });
 }
 
 // EnergyFunctional::energy alone, without host synchronization (after
 // update_density / update_vhxc of the same iteration); results as above
 int qbg_ef_energy_async(qbg_ef* e, const double* c, int ldc, double* hpsi, int ldh, int flags,
                         int prec) {
-  if (!e || !c) return QBG_E_INVAL;
-  return ef_launch(e, c, ldc, (flags & QBG_EF_HPSI) != 0, hpsi, ldh, (flags & QBG_EF_FORCES) != 0,
-                   (flags & QBG_EF_STRESS) != 0, prec, (flags & QBG_EF_SAME_C) != 0);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !c) return QBG_E_INVAL;
+    return ef_launch(e, c, ldc, (flags & QBG_EF_HPSI) != 0, hpsi, ldh, (flags & QBG_EF_FORCES) != 0,
+                     (flags & QBG_EF_STRESS) != 0, prec, (flags & QBG_EF_SAME_C) != 0);
+  });
+}
+
+// NonLocalPotential of one spin's SlaterDet of the functional
+// (src/NonLocalPotential.h:61-68; constructed by EnergyFunctional,
+// src/EnergyFunctional.cpp:113-118)
+struct qbg_nl {
+  qbg_ef* ef = nullptr;
+  qbg_ctx* ctx = nullptr;
+  int ispin = 0;
+};
+
+int qbg_nl_create(qbg_ef* e, int ispin, qbg_nl** out) {
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !out) return QBG_E_INVAL;
+    *out = nullptr;
+    if (ispin < 0 || ispin >= e->nspin) return e->ctx->fail(QBG_E_INVAL, "nl: ispin");
+    qbg_nl* nl = new qbg_nl;
+    nl->ef = e;
+    nl->ctx = e->ctx;
+    nl->ispin = ispin;
+    *out = nl;
+    return QBG_OK;
+  });
+}
+
+int qbg_nl_destroy(qbg_nl* nl) {
+  if (!nl) return QBG_E_INVAL;
+  delete nl;
+  return QBG_OK;
+}
+
+// NonLocalPotential::update_twnl (src/NonLocalPotential.cpp:235-1255): the
+// projector tables twnl and dtwnl of the functional's current basis
+int qbg_nl_update_twnl(qbg_nl* nl) {
+  return qbg::guarded((nl ? nl->ctx : nullptr), [&]() -> int {
+    if (!nl) return QBG_E_INVAL;
+    qbg_ef* e = nl->ef;
+    qbg_ctx* c = e->ctx;
+    if (c->capturing) return c->fail(QBG_E_STATE, "update_twnl during graph capture");
+    QBG_HIP(c, hipSetDevice(c->device));
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> twnl, dtwnl;
+    ef_twnl(e, twnl, dtwnl);
+    int rc;
+    if ((rc = dset(c, e->d_twnl, twnl)) || (rc = dset(c, e->d_dtwnl, dtwnl))) return rc;
+    return QBG_OK;
+  });
+}
+
+// NonLocalPotential::energy (src/NonLocalPotential.cpp:1258-2069) for the
+// bands of the handle's spin: returns enl in *enl; fion (host, 3 nat,
+// species-major) and sigma_enl (host, 6: xx yy zz xy yz xz) are zeroed on
+// entry and hold the nonlocal forces and stress when asked for; H*psi_NL is
+// added into dsd (device, ld ldd complex).  c points at the spin's first band.
+int qbg_nl_energy(qbg_nl* nl, const double* c, int ldc, int compute_hpsi, double* dsd, int ldd,
+                  int compute_forces, double* fion, int compute_stress, double* sigma_enl,
+                  int prec, double* enl) {
+  return qbg::guarded((nl ? nl->ctx : nullptr), [&]() -> int {
+    if (!nl || !c || !enl) return QBG_E_INVAL;
+    qbg_ef* e = nl->ef;
+    qbg_ctx* ctx = e->ctx;
+    if (ctx->capturing) return ctx->fail(QBG_E_STATE, "nl energy during graph capture");
+    if (ldc < e->wb.size || (compute_hpsi && (!dsd || ldd < e->wb.size)))
+      return ctx->fail(QBG_E_INVAL, "nl energy: leading dimension");
+    if ((compute_forces && !fion) || (compute_stress && !sigma_enl))
+      return ctx->fail(QBG_E_INVAL, "nl energy: missing output");
+    if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "prec");
+    QBG_HIP(ctx, hipSetDevice(ctx->device));
+    int b0, nb, p0;
+    spin_bands(e, nl->ispin, b0, nb, p0);
+    if (fion) std::memset(fion, 0, sizeof(double) * 3 * e->nat);
+    if (sigma_enl) std::memset(sigma_enl, 0, sizeof(double) * 6);
+    *enl = 0.0;
+    if (nb == 0) return QBG_OK;
+    const int rc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
+                       ? nl_bf16(e, c, ldc, compute_hpsi, dsd, ldd, compute_forces,
+                                 compute_stress, prec == QBG_PREC_BF16X3 ? 3 : 2, false, b0, nb)
+                       : nl_mfma(e, c, ldc, compute_hpsi, dsd, ldd, compute_forces,
+                                 compute_stress, prec, b0, nb);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    double sc[7];
+    QBG_HIP(ctx, hipMemcpyAsync(sc, e->d_scal + S_ENL, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
+    if (compute_forces)
+      QBG_HIP(ctx, hipMemcpyAsync(fion, e->d_fion, sizeof(double) * 3 * e->nat,
+                                  hipMemcpyDeviceToHost, s));
+    QBG_HIP(ctx, hipStreamSynchronize(s));
+    *enl = sc[0];
+    if (compute_stress) {
+      // src/NonLocalPotential.cpp:2057-2066
+      const double oi = 1.0 / e->omega;
+      for (int i = 0; i < 6; i++) sigma_enl[i] = ((i < 3 ? sc[0] : 0.0) + sc[1 + i]) * oi;
+    }
+    return QBG_OK;
+  });
 }
 
 int qbg_ef_results(qbg_ef* e, double* energies, double* fion, double* sigma) {
-  if (!e || !energies) return QBG_E_INVAL;
-  return ef_collect(e, energies, fion, sigma);
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !energies) return QBG_E_INVAL;
+    return ef_collect(e, energies, fion, sigma);
+  });
 }
 
 // device arrays for tests: 0 rhor (N), 1 v_r (N), 2 vxc_r (N), 3 rhog
 // (2*ngv), 4 vion_local_g, 5 rhopst, 6 sfac (2*nsp*ngv), 7 fion_esr (3 nat)
 int qbg_ef_get(qbg_ef* e, int what, double* host) {
-  if (!e || !host) return QBG_E_INVAL;
-  const int ngv = e->vb.size;
-  const double* src = nullptr;
-  size_t n = 0;
-  switch (what) {
-    case 0: src = e->d_rhor; n = e->N; break;
-    case 1: src = e->d_vr; n = e->N; break;
-    case 2: src = e->d_vxc; n = e->N; break;
-    case 3: src = e->d_rhog; n = 2 * (size_t)ngv; break;
-    case 4: src = e->d_vion; n = 2 * (size_t)ngv; break;
-    case 5: src = e->d_rhopst; n = 2 * (size_t)ngv; break;
-    case 6: src = e->d_sfac; n = 2 * (size_t)e->nsp * ngv; break;
-    case 7: std::memcpy(host, e->fesr.data(), sizeof(double) * e->fesr.size()); return QBG_OK;
-    case 8: src = e->d_rhor2; n = e->N; break;
-    case 9: src = e->d_vr2; n = e->N; break;
-    case 10: src = e->d_vxc2; n = e->N; break;
-    default: return QBG_E_INVAL;
-  }
-  if (!src) return e->ctx->fail(QBG_E_STATE, "spin-down arrays exist with nspin 2 only");
-  QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                 e->ctx->stream));
-  QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !host) return QBG_E_INVAL;
+    const int ngv = e->vb.size;
+    const double* src = nullptr;
+    size_t n = 0;
+    switch (what) {
+      case 0: src = e->d_rhor; n = e->N; break;
+      case 1: src = e->d_vr; n = e->N; break;
+      case 2: src = e->d_vxc; n = e->N; break;
+      case 3: src = e->d_rhog; n = 2 * (size_t)ngv; break;
+      case 4: src = e->d_vion; n = 2 * (size_t)ngv; break;
+      case 5: src = e->d_rhopst; n = 2 * (size_t)ngv; break;
+      case 6: src = e->d_sfac; n = 2 * (size_t)e->nsp * ngv; break;
+      case 7: std::memcpy(host, e->fesr.data(), sizeof(double) * e->fesr.size()); return QBG_OK;
+      case 8: src = e->d_rhor2; n = e->N; break;
+      case 9: src = e->d_vr2; n = e->N; break;
+      case 10: src = e->d_vxc2; n = e->N; break;
+      default: return QBG_E_INVAL;
+    }
+    if (!src) return e->ctx->fail(QBG_E_STATE, "spin-down arrays exist with nspin 2 only");
+    QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                   e->ctx->stream));
+    QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return QBG_OK;
+  });
 }
 
 // FourierTransform(basis, basis.np(0), basis.np(1), basis.np(2)) of the
 // wavefunction basis (src/SlaterDet.cpp:1385, src/WavefunctionHandler.cpp:411):
 // the grid of the grid functions in sample files
 int qbg_ef_wf_grid(qbg_ef* e, int* dims, qbg_grid** out) {
-  if (!e || !out) return QBG_E_INVAL;
-  *out = nullptr;
-  qbg_ctx* c = e->ctx;
-  if (!e->gwf) {
-    if (c->capturing) return c->fail(QBG_E_STATE, "wf_grid during graph capture");
-    qbg::host::Maps m;
-    if (!m.build(e->wb, e->wb.np[0], e->wb.np[1], e->wb.np[2]))
-      return c->fail(QBG_E_INVAL, "wf_grid: basis does not fit its grid");
-    int rc = qbg_grid_create(c, e->wb.np[0], e->wb.np[1], e->wb.np[2], e->wb.size, m.nvec,
-                             m.ip.data(), m.im.data(), m.zvec_to_val.data(), &e->gwf);
-    if (rc) return rc;
-  }
-  if (dims)
-    for (int i = 0; i < 3; i++) dims[i] = e->wb.np[i];
-  *out = e->gwf;
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !out) return QBG_E_INVAL;
+    *out = nullptr;
+    qbg_ctx* c = e->ctx;
+    if (!e->gwf) {
+      if (c->capturing) return c->fail(QBG_E_STATE, "wf_grid during graph capture");
+      qbg::host::Maps m;
+      if (!m.build(e->wb, e->wb.np[0], e->wb.np[1], e->wb.np[2]))
+        return c->fail(QBG_E_INVAL, "wf_grid: basis does not fit its grid");
+      int rc = qbg_grid_create(c, e->wb.np[0], e->wb.np[1], e->wb.np[2], e->wb.size, m.nvec,
+                               m.ip.data(), m.im.data(), m.zvec_to_val.data(), &e->gwf);
+      if (rc) return rc;
+    }
+    if (dims)
+      for (int i = 0; i < 3; i++) dims[i] = e->wb.np[i];
+    *out = e->gwf;
+    return QBG_OK;
+  });
 }
 
 // Wavefunction::set_nspin (src/Wavefunction.cpp:365-390) with the bands
 // [0, nst_up) spin up: per-spin densities and potentials (ChargeDensity
 // rhor[ispin], EnergyFunctional v_r[ispin], src/EnergyFunctional.h:78-79)
 int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
-  if (!e) return QBG_E_INVAL;
-  qbg_ctx* c = e->ctx;
-  if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
-  if (nspin == 1) nst_up = 0;
-  if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
-    return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
-  (void)hipSetDevice(c->device);
-  QBG_HIP(c, hipStreamSynchronize(c->stream));
-  if (nspin == 2 && !e->d_rhor2) {
-    int rc;
-    if ((rc = dalloc(c, &e->d_rhor2, e->N)) || (rc = dalloc(c, &e->d_vxc2, e->N)) ||
-        (rc = dalloc(c, &e->d_vr2, e->N)) || (rc = dalloc(c, &e->d_vloc, e->N)) ||
-        (rc = dalloc(c, &e->d_rhot, e->N)))
-      return rc;
-  }
-  e->nspin = nspin;
-  e->nst_up = nst_up;
-  std::vector<double> dfac = density_factors(e);
-  if (e->d_dfac) (void)hipFree(e->d_dfac);
-  e->d_dfac = nullptr;
-  int rc = dput(c, &e->d_dfac, dfac);
-  if (rc) return rc;
-  e->have_rho = e->have_vhxc = false;
-  e->xres_c = nullptr;
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e) return QBG_E_INVAL;
+    qbg_ctx* c = e->ctx;
+    if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
+    if (nspin == 1) nst_up = 0;
+    if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
+      return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
+    QBG_HIP(c, hipSetDevice(c->device));
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    if (nspin == 2 && !e->d_rhor2) {
+      int rc;
+      if ((rc = dalloc(c, &e->d_rhor2, e->N)) || (rc = dalloc(c, &e->d_vxc2, e->N)) ||
+          (rc = dalloc(c, &e->d_vr2, e->N)) || (rc = dalloc(c, &e->d_vloc, e->N)) ||
+          (rc = dalloc(c, &e->d_rhot, e->N)))
+        return rc;
+    }
+    e->nspin = nspin;
+    e->nst_up = nst_up;
+    std::vector<double> dfac = density_factors(e);
+    if (e->d_dfac) (void)hipFree(e->d_dfac);
+    e->d_dfac = nullptr;
+    int rc = dput(c, &e->d_dfac, dfac);
+    if (rc) return rc;
+    e->have_rho = e->have_vhxc = false;
+    e->xres_c = nullptr;
+    return QBG_OK;
+  });
 }
 
 // the potential v_r used by H*psi may be replaced (tests of rs_mul_add)
 int qbg_ef_set_vr(qbg_ef* e, const double* host_v) {
-  if (!e || !host_v) return QBG_E_INVAL;
-  QBG_HIP(e->ctx, hipMemcpyAsync(e->d_vr, host_v, sizeof(double) * e->N, hipMemcpyHostToDevice,
-                                 e->ctx->stream));
-  QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
-  return QBG_OK;
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !host_v) return QBG_E_INVAL;
+    QBG_HIP(e->ctx, hipMemcpyAsync(e->d_vr, host_v, sizeof(double) * e->N, hipMemcpyHostToDevice,
+                                   e->ctx->stream));
+    QBG_HIP(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return QBG_OK;
+  });
 }
 
 // SlaterDet::gram (src/SlaterDet.cpp:480-534), real basis:
 // cleanup; S = 2 P^T P - p0 p0^T; S = L L^T; C <- C L^-T
 int qbg_gram(qbg_ctx* c, double* cc, int ldc, int ngw, int nst, int prec) {
-  if (!c || !cc || ngw <= 0 || nst <= 0 || ldc < ngw) return QBG_E_INVAL;
-  if (prec != QBG_PREC_FP64) return c->fail(QBG_E_INVAL, "gram: FP64 only");
-  (void)hipSetDevice(c->device);
-  hipStream_t s = c->stream;
-  const long K = 2L * ngw;
-  const int tm = (nst + 63) / 64;
-  const size_t work = (size_t)nst * nst * std::max(1, 2048 / (tm * tm) + 1);
-  double *S = nullptr, *X = nullptr, *T = nullptr, *W = nullptr;
-  int* info = nullptr;
-  int rc = QBG_OK;
-  if ((rc = dalloc(c, &S, (size_t)nst * nst)) || (rc = dalloc(c, &X, (size_t)nst * nst)) ||
-      (rc = dalloc(c, &T, (size_t)K * nst)) || (rc = dalloc(c, &W, work)) ||
-      (rc = dalloc(c, &info, 1))) {
-    (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
-    return rc;
-  }
-  auto release = [&]() {
-    (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
-  };
-  c->prof_begin(QBG_K_GRAM);
-  cleanup(cc, ldc, nst, s);
-  if (gemm_syrk(cc, 2L * ldc, nst, K, 2.0, -1.0, S, nst, W, work, s)) {
-    (void)hipStreamSynchronize(s);
-    release();
-    return c->fail(QBG_E_INVAL, "gram workspace");
-  }
-  potrf(S, nst, nst, info, s, W);
-  c->prof_end();
-  // the coefficients are only touched once the factorization succeeded
-  // (the reference aborts in potrf, src/Matrix.cpp:2324-2327)
-  int hinfo = 0;
-  (void)hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
-  hipError_t er = hipStreamSynchronize(s);
-  if (er != hipSuccess || hinfo) {
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !cc || ngw <= 0 || nst <= 0 || ldc < ngw) return QBG_E_INVAL;
+    if (prec != QBG_PREC_FP64) return c->fail(QBG_E_INVAL, "gram: FP64 only");
+    QBG_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const long K = 2L * ngw;
+    const int tm = (nst + 63) / 64;
+    const size_t work = (size_t)nst * nst * std::max(1, 2048 / (tm * tm) + 1);
+    double *S = nullptr, *X = nullptr, *T = nullptr, *W = nullptr;
+    int* info = nullptr;
+    int rc = QBG_OK;
+    if ((rc = dalloc(c, &S, (size_t)nst * nst)) || (rc = dalloc(c, &X, (size_t)nst * nst)) ||
+        (rc = dalloc(c, &T, (size_t)K * nst)) || (rc = dalloc(c, &W, work)) ||
+        (rc = dalloc(c, &info, 1))) {
+      (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
+      return rc;
+    }
+    auto release = [&]() {
+      (void)hipFree(S); (void)hipFree(X); (void)hipFree(T); (void)hipFree(W); (void)hipFree(info);
+    };
+    c->prof_begin(QBG_K_GRAM);
+    cleanup(cc, ldc, nst, s);
+    if (gemm_syrk(cc, 2L * ldc, nst, K, 2.0, -1.0, S, nst, W, work, s)) {
+      (void)hipStreamSynchronize(s);
+      release();
+      return c->fail(QBG_E_INVAL, "gram workspace");
+    }
+    potrf(S, nst, nst, info, s, W);
+    c->prof_end();
+    // the coefficients are only touched once the factorization succeeded
+    // (the reference aborts in potrf, src/Matrix.cpp:2324-2327)
+    int hinfo = 0;
+    (void)hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipError_t er = hipStreamSynchronize(s);
+    if (er != hipSuccess || hinfo) {
+      release();
+      if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+      return c->fail(QBG_E_NOT_SPD, "gram: overlap matrix not positive definite");
+    }
+    c->prof_begin(QBG_K_GRAM);
+    trtri(S, nst, nst, X, s);
+    gemm_nn_triu(cc, 2L * ldc, X, nst, K, nst, T, K, s);
+    (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
+                           nst, hipMemcpyDeviceToDevice, s);
+    c->prof_end();
+    er = hipStreamSynchronize(s);
     release();
     if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
-    return c->fail(QBG_E_NOT_SPD, "gram: overlap matrix not positive definite");
-  }
-  c->prof_begin(QBG_K_GRAM);
-  trtri(S, nst, nst, X, s);
-  gemm_nn_triu(cc, 2L * ldc, X, nst, K, nst, T, K, s);
-  (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
-                         nst, hipMemcpyDeviceToDevice, s);
-  c->prof_end();
-  er = hipStreamSynchronize(s);
-  release();
-  if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
-  return rc;
+    return rc;
+  });
 }
 
 }  // extern "C"

@@ -9,6 +9,10 @@
 struct qbg_ef {
   qbg_ctx* ctx = nullptr;
   qbg::host::Cell cell;
+  // the cell that fixes basis membership and grid (Wavefunction's refcell,
+  // src/Basis.cpp:314-325; the cell itself when none was given)
+  qbg::host::Cell defcell;
+  bool has_ref = false;
   double ecut = 0, omega = 0;
   std::vector<qbg::host::Species> sp;
   std::vector<int> na, sp_first;

@@ -123,12 +123,14 @@ int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int 
                long K, int N, double* C, long ldc, int accumulate, int parts, hipStream_t s);
 int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int Np, int Mp,
                long K, int N, float* C, long ldc, int accumulate, int parts, hipStream_t s);
-// c (FP64, real view ld ldc2) -> c32[k + n*ldk]
+// c (FP64, real view ld ldc2) -> c32[k + n*ldk]; band columns [nst, nzero)
+// set to zero
 void convert_f32(const double* c, long ldc2, long K, int nst, float* c32, long ldk,
-                 hipStream_t s);
-// fout32[m + n*ldf] = fout[m + n*M]; w32[n + m*ldw] = occ_n fout[m + n*M] (if w32)
+                 hipStream_t s, int nzero = 0);
+// fout32[m + n*ldf] = fout[m + n*M]; w32[n + m*ldw] = occ_n fout[m + n*M] (if w32);
+// band columns [nst, nzero) set to zero
 void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32, long ldf,
-              float* w32, long ldw, hipStream_t s);
+              float* w32, long ldw, hipStream_t s, int nzero = 0);
 
 // lower Cholesky in place; info[0] = 0 or the failing pivot + 1.  With a
 // workspace of 64 * n doubles (ws) the factorization runs blocked over the

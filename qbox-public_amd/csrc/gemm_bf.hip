@@ -589,37 +589,41 @@ int gemm_bf_nn(const float* A, long lda, const float* F, long ldf, long Kp, int 
 }
 
 // ------------------------------------------------ FP32 operand staging
-// c (FP64 complex, ld ldc complex) -> c32[k + n*ldk], real view, K = 2 ngw
-__global__ void k_to_f32(const double* __restrict__ c, long ldc2, long K, int nst,
+// c (FP64 complex, ld ldc complex) -> c32[k + n*ldk], real view, K = 2 ngw;
+// columns [nst, nzero) are written as zeros
+__global__ void k_to_f32(const double* __restrict__ c, long ldc2, long K, int nst, int nzero,
                          float* __restrict__ c32, long ldk) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= K * nst) return;
+  if (t >= K * nzero) return;
   const long k = t % K, n = t / K;
-  c32[k + n * ldk] = (float)c[k + n * ldc2];
+  c32[k + n * ldk] = n < nst ? (float)c[k + n * ldc2] : 0.0f;
 }
 void convert_f32(const double* c, long ldc2, long K, int nst, float* c32, long ldk,
-                 hipStream_t s) {
-  const long tot = K * nst;
+                 hipStream_t s, int nzero) {
+  nzero = nzero > nst ? nzero : nst;
+  const long tot = K * nzero;
   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c, ldc2, K,
-                     nst, c32, ldk);
+                     nst, nzero, c32, ldk);
 }
 
-// fout (FP64, M x nst, ld M) -> fout32[m + n*ldf] and W32[n + m*ldw] = occ_n fout[m,n]
+// fout (FP64, M x nst, ld M) -> fout32[m + n*ldf] and W32[n + m*ldw] = occ_n fout[m,n];
+// band columns [nst, nzero) are written as zeros
 __global__ void k_fout_f32(const double* __restrict__ fout, const double* __restrict__ occ, int M,
-                           int nst, float* __restrict__ f32, long ldf, float* __restrict__ w32,
-                           long ldw) {
+                           int nst, int nzero, float* __restrict__ f32, long ldf,
+                           float* __restrict__ w32, long ldw) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)M * nst) return;
+  if (t >= (long)M * nzero) return;
   const int m = (int)(t % M), n = (int)(t / M);
-  const double v = fout[t];
+  const double v = n < nst ? fout[t] : 0.0;
   f32[m + n * ldf] = (float)v;
-  if (w32) w32[n + m * ldw] = (float)(occ[n] * v);
+  if (w32) w32[n + m * ldw] = n < nst ? (float)(occ[n] * v) : 0.0f;
 }
 void fout_f32(const double* fout, const double* occ, int M, int nst, float* f32, long ldf,
-              float* w32, long ldw, hipStream_t s) {
-  const long tot = (long)M * nst;
+              float* w32, long ldw, hipStream_t s, int nzero) {
+  nzero = nzero > nst ? nzero : nst;
+  const long tot = (long)M * nzero;
   hipLaunchKernelGGL(k_fout_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, fout, occ,
-                     M, nst, f32, ldf, w32, ldw);
+                     M, nst, nzero, f32, ldf, w32, ldw);
 }
 
 }  // namespace qbg

@@ -72,50 +72,61 @@ void qbg_ctx::prof_collect() {
 extern "C" {
 
 int qbg_prof_enable(qbg_ctx* c, int on) {
-  if (!c) return QBG_E_INVAL;
-  c->prof_collect();
-  c->prof = on != 0;
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    c->prof_collect();
+    c->prof = on != 0;
+    return QBG_OK;
+  });
 }
 
 int qbg_prof_reset(qbg_ctx* c) {
-  if (!c) return QBG_E_INVAL;
-  c->prof_collect();
-  for (int k = 0; k < QBG_K_COUNT; k++) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    c->prof_collect();
+    for (int k = 0; k < QBG_K_COUNT; k++) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    return QBG_OK;
+  });
 }
 
 int qbg_prof_read(qbg_ctx* c, int kind, double* ms, int* n) {
-  if (!c || kind < 0 || kind >= QBG_K_COUNT) return QBG_E_INVAL;
-  c->prof_collect();
-  if (ms) *ms = c->prof_ms[kind];
-  if (n) *n = c->prof_n[kind];
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || kind < 0 || kind >= QBG_K_COUNT) return QBG_E_INVAL;
+    c->prof_collect();
+    if (ms) *ms = c->prof_ms[kind];
+    if (n) *n = c->prof_n[kind];
+    return QBG_OK;
+  });
 }
 
 int qbg_init(int device, qbg_ctx** out) {
-  if (!out) return QBG_E_INVAL;
-  *out = nullptr;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return QBG_E_HIP;
-  qbg_ctx* c = new (std::nothrow) qbg_ctx;
-  if (!c) return QBG_E_NOMEM;
-  c->device = device;
-  if (hipSetDevice(device) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    delete c;
-    return QBG_E_HIP;
-  }
-  *out = c;
-  return QBG_OK;
+  return qbg::guarded(nullptr, [&]() -> int {
+    if (!out) return QBG_E_INVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return QBG_E_HIP;
+    qbg_ctx* c = new (std::nothrow) qbg_ctx;
+    if (!c) return QBG_E_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      delete c;
+      return QBG_E_HIP;
+    }
+    *out = c;
+    return QBG_OK;
+  });
 }
 
 int qbg_finalize(qbg_ctx* c) {
   if (!c) return QBG_E_INVAL;
-  (void)hipSetDevice(c->device);
+  if (hipSetDevice(c->device) != hipSuccess) return QBG_E_HIP;
   (void)hipStreamSynchronize(c->stream);
-  c->prof_collect();
+  try {
+    c->prof_collect();
+  } catch (...) {
+  }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -127,112 +138,136 @@ int qbg_finalize(qbg_ctx* c) {
 const char* qbg_last_error(const qbg_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int qbg_sync(qbg_ctx* c) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "sync during graph capture");
-  QBG_HIP(c, hipStreamSynchronize(c->stream));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "sync during graph capture");
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_graph_begin(qbg_ctx* c) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "graph capture already open");
-  (void)hipSetDevice(c->device);
-  c->prof_collect();
-  QBG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  c->capturing = true;
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "graph capture already open");
+    QBG_HIP(c, hipSetDevice(c->device));
+    c->prof_collect();
+    QBG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    return QBG_OK;
+  });
 }
 
 int qbg_graph_end(qbg_ctx* c, qbg_graph** out) {
-  if (!c || !out) return QBG_E_INVAL;
-  *out = nullptr;
-  if (!c->capturing) return c->fail(QBG_E_STATE, "no graph capture open");
-  c->capturing = false;
-  hipGraph_t gr = nullptr;
-  QBG_HIP(c, hipStreamEndCapture(c->stream, &gr));
-  qbg_graph* g = new (std::nothrow) qbg_graph;
-  if (!g) { (void)hipGraphDestroy(gr); return QBG_E_NOMEM; }
-  g->ctx = c;
-  g->graph = gr;
-  if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) {
-    (void)hipGraphDestroy(gr);
-    delete g;
-    return c->fail(QBG_E_HIP, "hipGraphInstantiate failed");
-  }
-  *out = g;
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !out) return QBG_E_INVAL;
+    *out = nullptr;
+    if (!c->capturing) return c->fail(QBG_E_STATE, "no graph capture open");
+    c->capturing = false;
+    hipGraph_t gr = nullptr;
+    QBG_HIP(c, hipStreamEndCapture(c->stream, &gr));
+    qbg_graph* g = new (std::nothrow) qbg_graph;
+    if (!g) { (void)hipGraphDestroy(gr); return QBG_E_NOMEM; }
+    g->ctx = c;
+    g->graph = gr;
+    if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      delete g;
+      return c->fail(QBG_E_HIP, "hipGraphInstantiate failed");
+    }
+    *out = g;
+    return QBG_OK;
+  });
 }
 
 int qbg_graph_launch(qbg_graph* g) {
-  if (!g) return QBG_E_INVAL;
-  QBG_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
-  return QBG_OK;
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    if (!g) return QBG_E_INVAL;
+    QBG_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_graph_destroy(qbg_graph* g) {
-  if (!g) return QBG_E_INVAL;
-  (void)hipStreamSynchronize(g->ctx->stream);
-  if (g->exec) (void)hipGraphExecDestroy(g->exec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  delete g;
-  return QBG_OK;
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    if (!g) return QBG_E_INVAL;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return QBG_OK;
+  });
 }
 
 int qbg_malloc(qbg_ctx* c, size_t bytes, void** p) {
-  if (!c || !p) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  (void)hipSetDevice(c->device);
-  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
-    *p = nullptr;
-    return c->fail(QBG_E_NOMEM, "hipMalloc failed");
-  }
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !p) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipSetDevice(c->device));
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+      *p = nullptr;
+      return c->fail(QBG_E_NOMEM, "hipMalloc failed");
+    }
+    return QBG_OK;
+  });
 }
 
 int qbg_free(qbg_ctx* c, void* p) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  QBG_HIP(c, hipStreamSynchronize(c->stream));
-  QBG_HIP(c, hipFree(p));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    QBG_HIP(c, hipFree(p));
+    return QBG_OK;
+  });
 }
 
 int qbg_memcpy_h2d(qbg_ctx* c, void* d, const void* h, size_t n) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  QBG_HIP(c, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, c->stream));
-  QBG_HIP(c, hipStreamSynchronize(c->stream));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, c->stream));
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_memcpy_d2h(qbg_ctx* c, void* h, const void* d, size_t n) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  QBG_HIP(c, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, c->stream));
-  QBG_HIP(c, hipStreamSynchronize(c->stream));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, c->stream));
+    QBG_HIP(c, hipStreamSynchronize(c->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_memset(qbg_ctx* c, void* d, int v, size_t n) {
-  if (!c) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  QBG_HIP(c, hipMemsetAsync(d, v, n, c->stream));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipMemsetAsync(d, v, n, c->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_timer_start(qbg_ctx* c) {
-  if (!c) return QBG_E_INVAL;
-  QBG_HIP(c, hipEventRecord(c->ev0, c->stream));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c) return QBG_E_INVAL;
+    QBG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    return QBG_OK;
+  });
 }
 
 int qbg_timer_stop(qbg_ctx* c, float* ms) {
-  if (!c || !ms) return QBG_E_INVAL;
-  if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
-  QBG_HIP(c, hipEventRecord(c->ev1, c->stream));
-  QBG_HIP(c, hipEventSynchronize(c->ev1));
-  QBG_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !ms) return QBG_E_INVAL;
+    if (c->capturing) return c->fail(QBG_E_STATE, "not allowed during graph capture");
+    QBG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    QBG_HIP(c, hipEventSynchronize(c->ev1));
+    QBG_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return QBG_OK;
+  });
 }
 
 }  // extern "C"
@@ -259,220 +294,238 @@ extern "C" {
 
 int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
                     const int32_t* ip, const int32_t* im, const int32_t* z2v, qbg_grid** out) {
-  if (!c || !out || !ip || !im || !z2v) return QBG_E_INVAL;
-  *out = nullptr;
-  if (np0 <= 0 || np1 <= 0 || np2 <= 0 || ng <= 0 || nvec <= 0 || nvec % 2 == 0)
-    return c->fail(QBG_E_INVAL, "grid: bad dimensions (real basis needs odd nvec)");
-  FftPlan p0 = make_plan(np0), p1 = make_plan(np1), p2 = make_plan(np2);
-  if (p0.nstage < 0 || p1.nstage < 0 || p2.nstage < 0)
-    return c->fail(QBG_E_INVAL, "grid: FFT length not a product of 2,3,5,7,11");
-  const long nslot = (long)nvec * np2;
-  // structure of the real-basis maps (src/BasisMapping.cpp:103-152):
-  // rod 0 on column 0, rod r>0 on columns 2r-1 (+G) and 2r (-G), rods in G order
-  std::vector<int> gfirst((nvec + 1) / 2, -1);
-  int prev_grp = 0;
-  for (int ig = 0; ig < ng; ig++) {
-    if (ip[ig] < 0 || ip[ig] >= nslot || im[ig] < 0 || im[ig] >= nslot)
-      return c->fail(QBG_E_INVAL, "grid: map entry out of range");
-    const int cp = ip[ig] / np2, cm = im[ig] / np2;
-    int grp;
-    if (cp == 0 && cm == 0) grp = 0;
-    else if (cp % 2 == 1 && cm == cp + 1) grp = (cp + 1) / 2;
-    else return c->fail(QBG_E_INVAL, "grid: maps do not have the real-basis rod structure");
-    if (grp < prev_grp) return c->fail(QBG_E_INVAL, "grid: G vectors not grouped by rod");
-    if (gfirst[grp] < 0) gfirst[grp] = ig;
-    prev_grp = grp;
-  }
-  if (gfirst[0] != 0) return c->fail(QBG_E_INVAL, "grid: G=0 rod must come first");
-  for (size_t r = 0; r < gfirst.size(); r++)
-    if (gfirst[r] < 0) return c->fail(QBG_E_INVAL, "grid: empty rod");
-  // slots must be distinct except +G/-G of G=0 (second write wins)
-  {
-    std::vector<unsigned char> used(nslot, 0);
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !out || !ip || !im || !z2v) return QBG_E_INVAL;
+    *out = nullptr;
+    if (np0 <= 0 || np1 <= 0 || np2 <= 0 || ng <= 0 || nvec <= 0 || nvec % 2 == 0)
+      return c->fail(QBG_E_INVAL, "grid: bad dimensions (real basis needs odd nvec)");
+    FftPlan p0 = make_plan(np0), p1 = make_plan(np1), p2 = make_plan(np2);
+    if (p0.nstage < 0 || p1.nstage < 0 || p2.nstage < 0)
+      return c->fail(QBG_E_INVAL, "grid: FFT length not a product of 2,3,5,7,11");
+    const long nslot = (long)nvec * np2;
+    // structure of the real-basis maps (src/BasisMapping.cpp:103-152):
+    // rod 0 on column 0, rod r>0 on columns 2r-1 (+G) and 2r (-G), rods in G order
+    std::vector<int> gfirst((nvec + 1) / 2, -1);
+    int prev_grp = 0;
     for (int ig = 0; ig < ng; ig++) {
-      if (used[ip[ig]]) return c->fail(QBG_E_INVAL, "grid: duplicate map slot");
-      used[ip[ig]] = 1;
-      if (im[ig] != ip[ig]) {
-        if (used[im[ig]]) return c->fail(QBG_E_INVAL, "grid: duplicate map slot");
-        used[im[ig]] = 1;
-      }
+      if (ip[ig] < 0 || ip[ig] >= nslot || im[ig] < 0 || im[ig] >= nslot)
+        return c->fail(QBG_E_INVAL, "grid: map entry out of range");
+      const int cp = ip[ig] / np2, cm = im[ig] / np2;
+      int grp;
+      if (cp == 0 && cm == 0) grp = 0;
+      else if (cp % 2 == 1 && cm == cp + 1) grp = (cp + 1) / 2;
+      else return c->fail(QBG_E_INVAL, "grid: maps do not have the real-basis rod structure");
+      if (grp < prev_grp) return c->fail(QBG_E_INVAL, "grid: G vectors not grouped by rod");
+      if (gfirst[grp] < 0) gfirst[grp] = ig;
+      prev_grp = grp;
     }
-  }
-  qbg_grid* g = new (std::nothrow) qbg_grid;
-  if (!g) return QBG_E_NOMEM;
-  g->ctx = c;
-  g->np0 = np0; g->np1 = np1; g->np2 = np2; g->ng = ng; g->nvec = nvec;
-  g->plan0 = p0; g->plan1 = p1; g->plan2 = p2;
-  // LDS: two buffers of (lines x n) complex doubles within 64 KiB
-  auto lines = [](int n) { return std::max(1, std::min(8, 65536 / (2 * 16 * n))); };
-  g->zcols = lines(np2);
-  // four-step kernels where compiled for the axis length (fft4_*.hip);
-  // their z tile: zcols*N + N complex doubles of LDS, zcols*max(n1,n2) threads
-  // (QBG_GENERIC_FFT=1 selects the generic Stockham kernels everywhere)
-  const char* gen = std::getenv("QBG_GENERIC_FFT");
-  const bool use_f4 = !(gen && gen[0] == '1');
-  g->f4[0] = use_f4 ? qbg::f4_ops(np0) : nullptr;
-  g->f4[1] = use_f4 ? qbg::f4_ops(np1) : nullptr;
-  g->f4[2] = use_f4 ? qbg::f4_ops(np2) : nullptr;
-  // the four-step kernels share blocked workspace layouts: all axes or none
-  if (!g->f4[0] || !g->f4[1] || !g->f4[2]) g->f4[0] = g->f4[1] = g->f4[2] = nullptr;
-  if (g->f4[2]) {
-    const int m = std::max(g->f4[2]->n1, g->f4[2]->n2);
-    g->zcols = std::min(256 / m, 4096 / np2 - 1);
-  }
-  // launch-shape override for tuning sweeps: QBG_YPPB (y-pass pairs per
-  // block of the v multiply)
-  if (const char* e = std::getenv("QBG_YPPB")) g->yppb = std::max(1, std::atoi(e));
-  g->xrows = lines(np0);
-  g->ytile = lines(np1);
-  if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);
-  if (g->zcols < 3) { delete g; return c->fail(QBG_E_INVAL, "grid: np2 too large"); }
-  // active rows: y in [0, ntrans0) and [np1 - ntrans0, np1)
-  int nt0 = 0;
-  for (int iv = 0; iv < nvec; iv++) {
-    const int xy = z2v[iv];
-    if (xy < 0 || xy >= np0 * np1) { delete g; return c->fail(QBG_E_INVAL, "grid: zvec_to_val out of range"); }
-    const int y = xy / np0;
-    const int k = y == 0 ? 0 : std::min(y, np1 - y);
-    nt0 = std::max(nt0, k + 1);
-  }
-  if (2 * nt0 > np1) { delete g; return c->fail(QBG_E_INVAL, "grid: basis does not fit in grid (y)"); }
-  g->ntrans0 = nt0;
-  g->nact = 2 * nt0;
-  std::vector<int32_t> act_row(g->nact), jrow(np1, -1);
-  for (int r = 0; r < g->nact; r++) {
-    act_row[r] = r < nt0 ? r : np1 - 2 * nt0 + r;
-    jrow[act_row[r]] = r;
-  }
-  // Column numbering of the pass workspace.  Qbox numbers columns by rod
-  // (rods sorted by size); here the rods are ranked by the (row, x) of
-  // their +G column, rod of rank i owns column i (+G) and column nrods + i
-  // (-G; unused for rod (0,0)).  The columns of one grid row then form two
-  // runs in x order (ascending +G, descending -G), so the x passes read and
-  // write coalesced; a z tile is a run of ranks [t0, t1) and owns columns
-  // [t0, t1) and [nrods + t0, nrods + t1), its G vectors listed in glist.
-  const int nrods = (nvec + 1) / 2;
-  g->ncols = 2 * nrods;
-  std::vector<int> rank(nrods), byrank(nrods);
-  {
-    auto pcol = [&](int r) { return r == 0 ? 0 : 2 * r - 1; };
-    for (int r = 0; r < nrods; r++) byrank[r] = r;
-    std::stable_sort(byrank.begin(), byrank.end(), [&](int a, int b) {
-      const int ya = jrow[z2v[pcol(a)] / np0], yb = jrow[z2v[pcol(b)] / np0];
-      if (ya != yb) return ya < yb;
-      return z2v[pcol(a)] % np0 < z2v[pcol(b)] % np0;
-    });
-    for (int i = 0; i < nrods; i++) rank[byrank[i]] = i;
-  }
-  auto qcol = [&](int iv) {   // workspace column of Qbox column iv
-    if (iv == 0) return rank[0];
-    const int r = (iv + 1) / 2;
-    return iv % 2 == 1 ? rank[r] : nrods + rank[r];
-  };
-  auto gend = [&](int r) { return r + 1 < nrods ? gfirst[r + 1] : ng; };
-  std::vector<qbg_tile> tiles;
-  std::vector<int32_t> glist;
-  glist.reserve(ng);
-  std::vector<int32_t> vip(ng), vim(ng);
-  {
-    const int rmax = std::max(1, g->zcols / 2);
-    for (int t0 = 0; t0 < nrods; t0 += rmax) {
-      const int t1 = std::min(nrods, t0 + rmax), nr = t1 - t0;
-      qbg_tile t;
-      t.col0 = t0;
-      t.ncol = 2 * nr;
-      t.np = nr;
-      t.colm = nrods + t0;
-      t.g0 = (int)glist.size();
-      for (int i = t0; i < t1; i++) {
-        const int r = byrank[i];
-        for (int ig = gfirst[r]; ig < gend(r); ig++) {
-          glist.push_back(ig);
-          // tile-local slots offset by col0: local column (i - t0) for +G,
-          // nr + (i - t0) for -G (rod (0,0): both in its +G column)
-          vip[ig] = i * np2 + ip[ig] % np2;
-          vim[ig] = (r == 0 ? i : nr + i) * np2 + im[ig] % np2;
+    if (gfirst[0] != 0) return c->fail(QBG_E_INVAL, "grid: G=0 rod must come first");
+    for (size_t r = 0; r < gfirst.size(); r++)
+      if (gfirst[r] < 0) return c->fail(QBG_E_INVAL, "grid: empty rod");
+    // slots must be distinct except +G/-G of G=0 (second write wins)
+    {
+      std::vector<unsigned char> used(nslot, 0);
+      for (int ig = 0; ig < ng; ig++) {
+        if (used[ip[ig]]) return c->fail(QBG_E_INVAL, "grid: duplicate map slot");
+        used[ip[ig]] = 1;
+        if (im[ig] != ip[ig]) {
+          if (used[im[ig]]) return c->fail(QBG_E_INVAL, "grid: duplicate map slot");
+          used[im[ig]] = 1;
         }
       }
-      t.g1 = (int)glist.size();
-      tiles.push_back(t);
     }
-  }
-  g->ntiles = (int)tiles.size();
-  std::vector<int32_t> row_start(g->nact + 1, 0), rc_ivec(nvec), rc_x(nvec);
-  {
-    std::vector<int> rowof(nvec);
+    qbg_grid* g = new (std::nothrow) qbg_grid;
+    if (!g) return QBG_E_NOMEM;
+    g->ctx = c;
+    g->np0 = np0; g->np1 = np1; g->np2 = np2; g->ng = ng; g->nvec = nvec;
+    g->plan0 = p0; g->plan1 = p1; g->plan2 = p2;
+    // LDS: two buffers of (lines x n) complex doubles within 64 KiB
+    auto lines = [](int n) { return std::max(1, std::min(8, 65536 / (2 * 16 * n))); };
+    g->zcols = lines(np2);
+    // four-step kernels where compiled for the axis length (fft4_*.hip);
+    // their z tile: zcols*N + N complex doubles of LDS, zcols*max(n1,n2) threads
+    // (QBG_GENERIC_FFT=1 selects the generic Stockham kernels everywhere)
+    const char* gen = std::getenv("QBG_GENERIC_FFT");
+    const bool use_f4 = !(gen && gen[0] == '1');
+    g->f4[0] = use_f4 ? qbg::f4_ops(np0) : nullptr;
+    g->f4[1] = use_f4 ? qbg::f4_ops(np1) : nullptr;
+    g->f4[2] = use_f4 ? qbg::f4_ops(np2) : nullptr;
+    // the four-step kernels share blocked workspace layouts: all axes or none
+    if (!g->f4[0] || !g->f4[1] || !g->f4[2]) g->f4[0] = g->f4[1] = g->f4[2] = nullptr;
+    if (g->f4[2]) {
+      const int m = std::max(g->f4[2]->n1, g->f4[2]->n2);
+      g->zcols = std::min(256 / m, 4096 / np2 - 1);
+    }
+    // launch-shape override for tuning sweeps: QBG_YPPB (y-pass pairs per
+    // block of the v multiply)
+    if (const char* e = std::getenv("QBG_YPPB")) g->yppb = std::max(1, std::atoi(e));
+    g->xrows = lines(np0);
+    g->ytile = lines(np1);
+    if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);
+    if (g->zcols < 3) { delete g; return c->fail(QBG_E_INVAL, "grid: np2 too large"); }
+    // active rows: y in [0, ntrans0) and [np1 - ntrans0, np1)
+    int nt0 = 0;
     for (int iv = 0; iv < nvec; iv++) {
-      const int y = z2v[iv] / np0;
-      rowof[iv] = jrow[y];
-      row_start[rowof[iv] + 1]++;
+      const int xy = z2v[iv];
+      if (xy < 0 || xy >= np0 * np1) { delete g; return c->fail(QBG_E_INVAL, "grid: zvec_to_val out of range"); }
+      const int y = xy / np0;
+      const int k = y == 0 ? 0 : std::min(y, np1 - y);
+      nt0 = std::max(nt0, k + 1);
     }
-    for (int r = 0; r < g->nact; r++) row_start[r + 1] += row_start[r];
-    std::vector<int> order(nvec);
-    for (int iv = 0; iv < nvec; iv++) order[iv] = iv;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      if (rowof[a] != rowof[b]) return rowof[a] < rowof[b];
-      return z2v[a] % np0 < z2v[b] % np0;
-    });
-    for (int e = 0; e < nvec; e++) {
-      rc_ivec[e] = qcol(order[e]);
-      rc_x[e] = z2v[order[e]] % np0;
+    if (2 * nt0 > np1) { delete g; return c->fail(QBG_E_INVAL, "grid: basis does not fit in grid (y)"); }
+    g->ntrans0 = nt0;
+    g->nact = 2 * nt0;
+    std::vector<int32_t> act_row(g->nact), jrow(np1, -1);
+    for (int r = 0; r < g->nact; r++) {
+      act_row[r] = r < nt0 ? r : np1 - 2 * nt0 + r;
+      jrow[act_row[r]] = r;
     }
-    // columns map to distinct grid points
-    for (int e = 1; e < nvec; e++)
-      if (rowof[order[e]] == rowof[order[e - 1]] && rc_x[e] == rc_x[e - 1]) {
-        delete g;
-        return c->fail(QBG_E_INVAL, "grid: two columns on one grid point");
+    // Column numbering of the pass workspace.  Qbox numbers columns by rod
+    // (rods sorted by size); here the rods are ranked by the (row, x) of
+    // their +G column, rod of rank i owns column i (+G) and column nrods + i
+    // (-G; unused for rod (0,0)).  The columns of one grid row then form two
+    // runs in x order (ascending +G, descending -G), so the x passes read and
+    // write coalesced; a z tile is a run of ranks [t0, t1) and owns columns
+    // [t0, t1) and [nrods + t0, nrods + t1), its G vectors listed in glist.
+    const int nrods = (nvec + 1) / 2;
+    g->ncols = 2 * nrods;
+    std::vector<int> rank(nrods), byrank(nrods);
+    {
+      auto pcol = [&](int r) { return r == 0 ? 0 : 2 * r - 1; };
+      for (int r = 0; r < nrods; r++) byrank[r] = r;
+      std::stable_sort(byrank.begin(), byrank.end(), [&](int a, int b) {
+        const int ya = jrow[z2v[pcol(a)] / np0], yb = jrow[z2v[pcol(b)] / np0];
+        if (ya != yb) return ya < yb;
+        return z2v[pcol(a)] % np0 < z2v[pcol(b)] % np0;
+      });
+      for (int i = 0; i < nrods; i++) rank[byrank[i]] = i;
+    }
+    auto qcol = [&](int iv) {   // workspace column of Qbox column iv
+      if (iv == 0) return rank[0];
+      const int r = (iv + 1) / 2;
+      return iv % 2 == 1 ? rank[r] : nrods + rank[r];
+    };
+    auto gend = [&](int r) { return r + 1 < nrods ? gfirst[r + 1] : ng; };
+    std::vector<qbg_tile> tiles;
+    std::vector<int32_t> glist;
+    glist.reserve(ng);
+    std::vector<int32_t> vip(ng), vim(ng);
+    {
+      const int rmax = std::max(1, g->zcols / 2);
+      for (int t0 = 0; t0 < nrods; t0 += rmax) {
+        const int t1 = std::min(nrods, t0 + rmax), nr = t1 - t0;
+        qbg_tile t;
+        t.col0 = t0;
+        t.ncol = 2 * nr;
+        t.np = nr;
+        t.colm = nrods + t0;
+        t.g0 = (int)glist.size();
+        for (int i = t0; i < t1; i++) {
+          const int r = byrank[i];
+          for (int ig = gfirst[r]; ig < gend(r); ig++) {
+            glist.push_back(ig);
+            // tile-local slots offset by col0: local column (i - t0) for +G,
+            // nr + (i - t0) for -G (rod (0,0): both in its +G column)
+            vip[ig] = i * np2 + ip[ig] % np2;
+            vim[ig] = (r == 0 ? i : nr + i) * np2 + im[ig] % np2;
+          }
+        }
+        t.g1 = (int)glist.size();
+        tiles.push_back(t);
       }
-  }
-  // dense (row, x) -> workspace column map for the four-step x passes
-  std::vector<int32_t> colmap((size_t)g->nact * np0, -1);
-  for (int iv = 0; iv < nvec; iv++) {
-    const int x = z2v[iv] % np0, y = z2v[iv] / np0;
-    colmap[(size_t)jrow[y] * np0 + x] = qcol(iv);
-  }
-  (void)hipSetDevice(c->device);
-  int rc = QBG_OK;
-  if ((rc = upload(c, &g->d_ip, vip)) || (rc = upload(c, &g->d_im, vim)) ||
-      (rc = upload(c, &g->d_glist, glist)) ||
-      (rc = upload(c, &g->d_tiles, tiles)) || (rc = upload(c, &g->d_row_start, row_start)) ||
-      (rc = upload(c, &g->d_rc_ivec, rc_ivec)) || (rc = upload(c, &g->d_rc_x, rc_x)) ||
-      (rc = upload(c, &g->d_act_row, act_row)) || (rc = upload(c, &g->d_colmap, colmap)) ||
-      (rc = upload(c, &g->d_jrow, jrow))) {
-    grid_free(g);
-    delete g;
-    return rc;
-  }
-  const int ns[3] = {np0, np1, np2};
-  for (int a = 0; a < 3; a++) {
-    const int n = ns[a];
-    std::vector<double> t64(2 * n);
-    std::vector<float> t32(2 * n);
-    const long double pi = 3.14159265358979323846264338327950288L;
-    for (int k = 0; k < n; k++) {
-      const long double ang = -2.0L * pi * k / n;
-      t64[2 * k] = (double)cosl(ang);
-      t64[2 * k + 1] = (double)sinl(ang);
-      t32[2 * k] = (float)cosl(ang);
-      t32[2 * k + 1] = (float)sinl(ang);
     }
-    if ((rc = upload(c, &g->d_tw64[a], t64)) || (rc = upload(c, &g->d_tw32[a], t32))) {
+    g->ntiles = (int)tiles.size();
+    std::vector<int32_t> row_start(g->nact + 1, 0), rc_ivec(nvec), rc_x(nvec);
+    {
+      std::vector<int> rowof(nvec);
+      for (int iv = 0; iv < nvec; iv++) {
+        const int y = z2v[iv] / np0;
+        rowof[iv] = jrow[y];
+        row_start[rowof[iv] + 1]++;
+      }
+      for (int r = 0; r < g->nact; r++) row_start[r + 1] += row_start[r];
+      std::vector<int> order(nvec);
+      for (int iv = 0; iv < nvec; iv++) order[iv] = iv;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (rowof[a] != rowof[b]) return rowof[a] < rowof[b];
+        return z2v[a] % np0 < z2v[b] % np0;
+      });
+      for (int e = 0; e < nvec; e++) {
+        rc_ivec[e] = qcol(order[e]);
+        rc_x[e] = z2v[order[e]] % np0;
+      }
+      // columns map to distinct grid points
+      for (int e = 1; e < nvec; e++)
+        if (rowof[order[e]] == rowof[order[e - 1]] && rc_x[e] == rc_x[e - 1]) {
+          delete g;
+          return c->fail(QBG_E_INVAL, "grid: two columns on one grid point");
+        }
+    }
+    // dense (row, x) -> workspace column map for the four-step x passes
+    std::vector<int32_t> colmap((size_t)g->nact * np0, -1);
+    for (int iv = 0; iv < nvec; iv++) {
+      const int x = z2v[iv] % np0, y = z2v[iv] / np0;
+      colmap[(size_t)jrow[y] * np0 + x] = qcol(iv);
+    }
+    QBG_HIP(c, hipSetDevice(c->device));
+    int rc = QBG_OK;
+    if ((rc = upload(c, &g->d_ip, vip)) || (rc = upload(c, &g->d_im, vim)) ||
+        (rc = upload(c, &g->d_glist, glist)) ||
+        (rc = upload(c, &g->d_tiles, tiles)) || (rc = upload(c, &g->d_row_start, row_start)) ||
+        (rc = upload(c, &g->d_rc_ivec, rc_ivec)) || (rc = upload(c, &g->d_rc_x, rc_x)) ||
+        (rc = upload(c, &g->d_act_row, act_row)) || (rc = upload(c, &g->d_colmap, colmap)) ||
+        (rc = upload(c, &g->d_jrow, jrow))) {
       grid_free(g);
       delete g;
       return rc;
     }
-  }
-  *out = g;
-  return QBG_OK;
+    const int ns[3] = {np0, np1, np2};
+    for (int a = 0; a < 3; a++) {
+      const int n = ns[a];
+      std::vector<double> t64(2 * n);
+      std::vector<float> t32(2 * n);
+      const long double pi = 3.14159265358979323846264338327950288L;
+      for (int k = 0; k < n; k++) {
+        const long double ang = -2.0L * pi * k / n;
+        t64[2 * k] = (double)cosl(ang);
+        t64[2 * k + 1] = (double)sinl(ang);
+        t32[2 * k] = (float)cosl(ang);
+        t32[2 * k + 1] = (float)sinl(ang);
+      }
+      if ((rc = upload(c, &g->d_tw64[a], t64)) || (rc = upload(c, &g->d_tw32[a], t32))) {
+        grid_free(g);
+        delete g;
+        return rc;
+      }
+    }
+    *out = g;
+    return QBG_OK;
+  });
+}
+
+int qbg_grid_info(const qbg_grid* g, int* out) {
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    if (!g || !out) return QBG_E_INVAL;
+    out[0] = g->np0; out[1] = g->np1; out[2] = g->np2;
+    out[3] = g->ng; out[4] = g->nvec; out[5] = g->ntrans0;
+    out[6] = g->f4[0] ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+      out[7 + 2 * a] = g->f4[a] ? g->f4[a]->n1 : 0;
+      out[8 + 2 * a] = g->f4[a] ? g->f4[a]->n2 : 0;
+    }
+    return QBG_OK;
+  });
 }
 
 int qbg_grid_destroy(qbg_grid* g) {
-  if (!g) return QBG_E_INVAL;
-  (void)hipSetDevice(g->ctx->device);
-  (void)hipStreamSynchronize(g->ctx->stream);
-  grid_free(g);
-  delete g;
-  return QBG_OK;
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    if (!g) return QBG_E_INVAL;
+    QBG_HIP(g->ctx, hipSetDevice(g->ctx->device));
+    (void)hipStreamSynchronize(g->ctx->stream);
+    grid_free(g);
+    delete g;
+    return QBG_OK;
+  });
 }
 
 }  // extern "C"
@@ -565,7 +618,7 @@ static int checkprec(qbg_grid* g, int prec) {
   if (!g) return QBG_E_INVAL;
   if (prec != QBG_PREC_FP64 && prec != QBG_PREC_FP32)
     return g->ctx->fail(QBG_E_INVAL, "unknown precision");
-  (void)hipSetDevice(g->ctx->device);
+  QBG_HIP(g->ctx, hipSetDevice(g->ctx->device));
   return QBG_OK;
 }
 
@@ -634,90 +687,102 @@ int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int 
 extern "C" {
 
 int qbg_fft_backward(qbg_grid* g, const double* c, double* f, int prec) {
-  TRY(checkprec(g, prec));
-  TRY(ensure_ws(g, 1));
-  hipStream_t s = g->ctx->stream;
-  PROF(QBG_K_ZBWD, zbwd(g, prec, Z_SINGLE, c, 0, 0, 0, 1, s));
-  PROF(QBG_K_XBWD, xbwd(g, prec, 1, s));
-  PROF(QBG_K_YOTHER, ypass(g, prec, Y_BWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
-  return last_launch(g);
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    TRY(ensure_ws(g, 1));
+    hipStream_t s = g->ctx->stream;
+    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_SINGLE, c, 0, 0, 0, 1, s));
+    PROF(QBG_K_XBWD, xbwd(g, prec, 1, s));
+    PROF(QBG_K_YOTHER, ypass(g, prec, Y_BWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
+    return last_launch(g);
+  });
 }
 
 int qbg_fft_backward_pair(qbg_grid* g, const double* c1, const double* c2, double* f, int prec) {
-  TRY(checkprec(g, prec));
-  TRY(ensure_ws(g, 1));
-  hipStream_t s = g->ctx->stream;
-  const long off2 = (long)(c2 - c1) / 2;
-  if ((c2 - c1) % 2) return g->ctx->fail(QBG_E_INVAL, "misaligned band pointers");
-  PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c1, 0, off2, 1, 1, s));
-  PROF(QBG_K_XBWD, xbwd(g, prec, 1, s));
-  PROF(QBG_K_YOTHER, ypass(g, prec, Y_BWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
-  return last_launch(g);
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    TRY(ensure_ws(g, 1));
+    hipStream_t s = g->ctx->stream;
+    const long off2 = (long)(c2 - c1) / 2;
+    if ((c2 - c1) % 2) return g->ctx->fail(QBG_E_INVAL, "misaligned band pointers");
+    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c1, 0, off2, 1, 1, s));
+    PROF(QBG_K_XBWD, xbwd(g, prec, 1, s));
+    PROF(QBG_K_YOTHER, ypass(g, prec, Y_BWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
+    return last_launch(g);
+  });
 }
 
 int qbg_fft_forward(qbg_grid* g, double* f, double* c, int prec) {
-  TRY(checkprec(g, prec));
-  TRY(ensure_ws(g, 1));
-  hipStream_t s = g->ctx->stream;
-  PROF(QBG_K_YOTHER, ypass(g, prec, Y_FWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
-  PROF(QBG_K_XFWD, xfwd(g, prec, 1, s));
-  PROF(QBG_K_ZFWD, zfwd(g, prec, Z_SINGLE, OUT_STORE, c, 0, 0, 0, 1, s));
-  return last_launch(g);
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    TRY(ensure_ws(g, 1));
+    hipStream_t s = g->ctx->stream;
+    PROF(QBG_K_YOTHER, ypass(g, prec, Y_FWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
+    PROF(QBG_K_XFWD, xfwd(g, prec, 1, s));
+    PROF(QBG_K_ZFWD, zfwd(g, prec, Z_SINGLE, OUT_STORE, c, 0, 0, 0, 1, s));
+    return last_launch(g);
+  });
 }
 
 int qbg_fft_forward_pair(qbg_grid* g, double* f, double* c1, double* c2, int prec) {
-  TRY(checkprec(g, prec));
-  TRY(ensure_ws(g, 1));
-  hipStream_t s = g->ctx->stream;
-  if ((c2 - c1) % 2) return g->ctx->fail(QBG_E_INVAL, "misaligned band pointers");
-  const long off2 = (long)(c2 - c1) / 2;
-  PROF(QBG_K_YOTHER, ypass(g, prec, Y_FWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
-  PROF(QBG_K_XFWD, xfwd(g, prec, 1, s));
-  PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_STORE, c1, 0, off2, 1, 1, s));
-  return last_launch(g);
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    TRY(ensure_ws(g, 1));
+    hipStream_t s = g->ctx->stream;
+    if ((c2 - c1) % 2) return g->ctx->fail(QBG_E_INVAL, "misaligned band pointers");
+    const long off2 = (long)(c2 - c1) / 2;
+    PROF(QBG_K_YOTHER, ypass(g, prec, Y_FWD_FULL, 1, nullptr, nullptr, nullptr, f, 1.0, s));
+    PROF(QBG_K_XFWD, xfwd(g, prec, 1, s));
+    PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_STORE, c1, 0, off2, 1, 1, s));
+    return last_launch(g);
+  });
 }
 
 int qbg_compute_density(qbg_grid* g, const double* c, int ldc, int nst, const double* occ,
                         double weight, double omega, double* rho, int prec) {
-  TRY(checkprec(g, prec));
-  if (nst <= 0 || ldc < g->ng || !occ || !(omega > 0.0))
-    return g->ctx->fail(QBG_E_INVAL, "compute_density: bad arguments");
-  if (g->ctx->capturing)
-    return g->ctx->fail(QBG_E_STATE, "compute_density copies host factors: not capturable");
-  const int npairs = (nst + 1) / 2;
-  TRY(ensure_fac(g, 2 * npairs));
-  // fac = weight * occ / omega (src/SlaterDet.cpp:232-244); a missing second
-  // band of the last pair contributes 0
-  std::vector<double> fac(2 * npairs, 0.0);
-  for (int n = 0; n < nst; n++) fac[n] = weight * (1.0 / omega) * occ[n];
-  hipStream_t s = g->ctx->stream;
-  QBG_HIP(g->ctx, hipMemcpyAsync(g->d_fac, fac.data(), sizeof(double) * fac.size(),
-                                 hipMemcpyHostToDevice, s));
-  TRY(qbg::density_batches(g, c, ldc, nst, g->d_fac, rho, prec));
-  // the host factor array must outlive the async copy
-  QBG_HIP(g->ctx, hipStreamSynchronize(s));
-  return QBG_OK;
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    if (nst <= 0 || ldc < g->ng || !occ || !(omega > 0.0))
+      return g->ctx->fail(QBG_E_INVAL, "compute_density: bad arguments");
+    if (g->ctx->capturing)
+      return g->ctx->fail(QBG_E_STATE, "compute_density copies host factors: not capturable");
+    const int npairs = (nst + 1) / 2;
+    TRY(ensure_fac(g, 2 * npairs));
+    // fac = weight * occ / omega (src/SlaterDet.cpp:232-244); a missing second
+    // band of the last pair contributes 0
+    std::vector<double> fac(2 * npairs, 0.0);
+    for (int n = 0; n < nst; n++) fac[n] = weight * (1.0 / omega) * occ[n];
+    hipStream_t s = g->ctx->stream;
+    QBG_HIP(g->ctx, hipMemcpyAsync(g->d_fac, fac.data(), sizeof(double) * fac.size(),
+                                   hipMemcpyHostToDevice, s));
+    TRY(qbg::density_batches(g, c, ldc, nst, g->d_fac, rho, prec));
+    // the host factor array must outlive the async copy
+    QBG_HIP(g->ctx, hipStreamSynchronize(s));
+    return QBG_OK;
+  });
 }
 int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc, int nst,
                    double* hpsi, int ldh, int prec) {
-  TRY(checkprec(g, prec));
-  if (nst <= 0 || ldc < g->ng || ldh < g->ng)
-    return g->ctx->fail(QBG_E_INVAL, "rs_mul_add: bad arguments");
-  const int npairs = (nst + 1) / 2;
-  const int B = std::min(npairs, g->batch);
-  TRY(ensure_ws(g, B));
-  hipStream_t s = g->ctx->stream;
-  for (int p0 = 0; p0 < npairs; p0 += B) {
-    const int nb = std::min(B, npairs - p0);
-    const int band0 = 2 * p0;
-    const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
-    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
-    PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
-    PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
-    PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
-    PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh, last_has2, nb, s));
-  }
-  return last_launch(g);
+  return qbg::guarded((g ? g->ctx : nullptr), [&]() -> int {
+    TRY(checkprec(g, prec));
+    if (nst <= 0 || ldc < g->ng || ldh < g->ng)
+      return g->ctx->fail(QBG_E_INVAL, "rs_mul_add: bad arguments");
+    const int npairs = (nst + 1) / 2;
+    const int B = std::min(npairs, g->batch);
+    TRY(ensure_ws(g, B));
+    hipStream_t s = g->ctx->stream;
+    for (int p0 = 0; p0 < npairs; p0 += B) {
+      const int nb = std::min(B, npairs - p0);
+      const int band0 = 2 * p0;
+      const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
+      PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
+      PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
+      PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
+      PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
+      PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh, last_has2, nb, s));
+    }
+    return last_launch(g);
+  });
 }
 
 }  // extern "C"

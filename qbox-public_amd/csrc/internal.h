@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 #include "qbg.h"
@@ -173,6 +175,35 @@ struct qbg_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
+
+namespace qbg {
+// Body of an exported entry point: no C++ exception crosses the C ABI.  A
+// failed host allocation returns QBG_E_NOMEM, any other exception
+// QBG_E_INVAL, each with its text in the context's last error (when there
+// is a context to hold it).
+template <class F>
+int guarded(qbg_ctx* c, F&& body) noexcept {
+  const char* what = nullptr;
+  int code = QBG_E_INVAL;
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    code = QBG_E_NOMEM;
+    what = "host allocation failed";
+  } catch (const std::exception& e) {
+    what = e.what();
+  } catch (...) {
+    what = "unknown exception";
+  }
+  if (c) {
+    try {
+      c->err = what;
+    } catch (...) {
+    }
+  }
+  return code;
+}
+}  // namespace qbg
 
 #define QBG_HIP(ctx, call)                                                   \
   do {                                                                       \

@@ -188,7 +188,7 @@ static int wfstep_create(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep
   if (c->capturing) return c->fail(QBG_E_STATE, "wfstep_create during graph capture");
   if ((nst ? nst : ef->nst) > POLAR_NMAX)
     return c->fail(QBG_E_INVAL, "wfstep: more than 4096 bands");
-  (void)hipSetDevice(c->device);
+  QBG_HIP(c, hipSetDevice(c->device));
   qbg_wfstep* w = new (std::nothrow) qbg_wfstep;
   if (!w) return QBG_E_NOMEM;
   w->ef = ef;
@@ -243,172 +243,190 @@ static int wfstep_create(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep
 }
 
 int qbg_wfstep_create(qbg_ef* ef, int kind, double param, qbg_wfstep** out) {
-  return wfstep_create(ef, kind, param, 0, out);
+  return qbg::guarded((ef ? ef->ctx : nullptr), [&]() -> int {
+    return wfstep_create(ef, kind, param, 0, out);
+  });
 }
 
 int qbg_wfstep_create_bands(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep** out) {
-  if (nst <= 0) {
-    if (out) *out = nullptr;
-    return ef ? ef->ctx->fail(QBG_E_INVAL, "wfstep: band count") : QBG_E_INVAL;
-  }
-  return wfstep_create(ef, kind, param, nst, out);
+  return qbg::guarded((ef ? ef->ctx : nullptr), [&]() -> int {
+    if (nst <= 0) {
+      if (out) *out = nullptr;
+      return ef ? ef->ctx->fail(QBG_E_INVAL, "wfstep: band count") : QBG_E_INVAL;
+    }
+    return wfstep_create(ef, kind, param, nst, out);
+  });
 }
 
 int qbg_wfstep_destroy(qbg_wfstep* w) {
-  if (!w) return QBG_E_INVAL;
-  (void)hipSetDevice(w->ctx->device);
-  (void)hipStreamSynchronize(w->ctx->stream);
-  if (w->h_out) (void)hipHostFree(w->h_out);
-  w->mem.release();
-  delete w;
-  return QBG_OK;
+  return qbg::guarded((w ? w->ctx : nullptr), [&]() -> int {
+    if (!w) return QBG_E_INVAL;
+    QBG_HIP(w->ctx, hipSetDevice(w->ctx->device));
+    (void)hipStreamSynchronize(w->ctx->stream);
+    if (w->h_out) (void)hipHostFree(w->h_out);
+    w->mem.release();
+    delete w;
+    return QBG_OK;
+  });
 }
 
 int qbg_wfstep_preprocess(qbg_wfstep* w) {
-  if (!w) return QBG_E_INVAL;
-  // a kernel rather than a memset node, so the call can be captured
-  launch_set_int(w->extrap, 0, w->ctx->stream);
-  return QBG_OK;
+  return qbg::guarded((w ? w->ctx : nullptr), [&]() -> int {
+    if (!w) return QBG_E_INVAL;
+    // a kernel rather than a memset node, so the call can be captured
+    launch_set_int(w->extrap, 0, w->ctx->stream);
+    return QBG_OK;
+  });
 }
 
 int qbg_wfstep_update(qbg_wfstep* w, double* c, int ldc, double* dc, int ldd) {
-  if (!w || !c || !dc) return QBG_E_INVAL;
-  qbg_ctx* ctx = w->ctx;
-  if (ldc < w->ngw || ldd < w->ngw) return ctx->fail(QBG_E_INVAL, "wfstep_update: leading dimension");
-  if (!w->bands && w->ef->nspin != 1)
-    return ctx->fail(QBG_E_STATE, "wfstep_update: the functional was switched to nspin 2");
-  (void)hipSetDevice(ctx->device);
-  hipStream_t s = ctx->stream;
-  const int n = w->nst, ngw = w->ngw;
-  const long K = 2L * ngw, ldc2 = 2L * ldc, ldd2 = 2L * ldd;
-  int rc = QBG_OK;
-  ctx->prof_begin(QBG_K_WFSTEP);
-  launch_band_sums(c, ldc2, dc, ldd2, w->ef->d_g2w, ngw, n, w->bpart, w->ekin, w->dotp, s);
-  launch_sum(w->dotp, n, w->scal, s);
-  if (w->kind == QBG_WF_SD) {
-    launch_sd_update(c, ldc2, dc, ldd2, ngw, n, w->param, s);
-    rc = gram_enqueue(w, c, ldc, s);
-  } else {
-    // residual: A = 2 c^T dc - c0 dc0^T, dc <- dc - c A (stored as -A)
-    if (gemm_tn_splitk(c, ldc2, dc, ldd2, n, n, K, -2.0, 1.0, w->A, n, w->W, w->wsize,
-                       QBG_PREC_FP64, s))
-      rc = QBG_E_INVAL;
-    else
-      gemm_nn(c, ldc2, w->A, n, K, n, n, dc, ldd2, 1, QBG_PREC_FP64, s);
-    if (!rc && w->kind == QBG_WF_PSD) {
-      launch_psd_update(c, ldc2, dc, ldd2, w->ef->d_g2w, w->ekin, ngw, n, w->param, s);
+  return qbg::guarded((w ? w->ctx : nullptr), [&]() -> int {
+    if (!w || !c || !dc) return QBG_E_INVAL;
+    qbg_ctx* ctx = w->ctx;
+    if (ldc < w->ngw || ldd < w->ngw) return ctx->fail(QBG_E_INVAL, "wfstep_update: leading dimension");
+    if (!w->bands && w->ef->nspin != 1)
+      return ctx->fail(QBG_E_STATE, "wfstep_update: the functional was switched to nspin 2");
+    QBG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int n = w->nst, ngw = w->ngw;
+    const long K = 2L * ngw, ldc2 = 2L * ldc, ldd2 = 2L * ldd;
+    int rc = QBG_OK;
+    ctx->prof_begin(QBG_K_WFSTEP);
+    launch_band_sums(c, ldc2, dc, ldd2, w->ef->d_g2w, ngw, n, w->bpart, w->ekin, w->dotp, s);
+    launch_sum(w->dotp, n, w->scal, s);
+    if (w->kind == QBG_WF_SD) {
+      launch_sd_update(c, ldc2, dc, ldd2, ngw, n, w->param, s);
       rc = gram_enqueue(w, c, ldc, s);
-    } else if (!rc) {
-      launch_psda_prec(dc, ldd2, w->dl, w->ef->d_g2w, w->ekin, ngw, n, w->param, w->partial, s);
-      launch_psda_theta(w->partial, w->npart, w->extrap, w->scal + 1, s);
-      launch_psda_update(c, ldc2, dc, ldd2, w->cl, w->dl, w->scal + 1, ngw, n, s);
-      rc = ortho_align_enqueue(w, c, ldc, s);
+    } else {
+      // residual: A = 2 c^T dc - c0 dc0^T, dc <- dc - c A (stored as -A)
+      if (gemm_tn_splitk(c, ldc2, dc, ldd2, n, n, K, -2.0, 1.0, w->A, n, w->W, w->wsize,
+                         QBG_PREC_FP64, s))
+        rc = QBG_E_INVAL;
+      else
+        gemm_nn(c, ldc2, w->A, n, K, n, n, dc, ldd2, 1, QBG_PREC_FP64, s);
+      if (!rc && w->kind == QBG_WF_PSD) {
+        launch_psd_update(c, ldc2, dc, ldd2, w->ef->d_g2w, w->ekin, ngw, n, w->param, s);
+        rc = gram_enqueue(w, c, ldc, s);
+      } else if (!rc) {
+        launch_psda_prec(dc, ldd2, w->dl, w->ef->d_g2w, w->ekin, ngw, n, w->param, w->partial, s);
+        launch_psda_theta(w->partial, w->npart, w->extrap, w->scal + 1, s);
+        launch_psda_update(c, ldc2, dc, ldd2, w->cl, w->dl, w->scal + 1, ngw, n, s);
+        rc = ortho_align_enqueue(w, c, ldc, s);
+      }
     }
-  }
-  if (!rc) launch_wf_results(w->scal, w->kind == QBG_WF_PSDA ? w->pw.flags : nullptr, w->info,
-                             w->h_out, s);
-  ctx->prof_end();
-  if (rc) return ctx->fail(rc, "wfstep_update: workspace too small");
-  hipError_t er = hipGetLastError();
-  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
-  return QBG_OK;
+    if (!rc) launch_wf_results(w->scal, w->kind == QBG_WF_PSDA ? w->pw.flags : nullptr, w->info,
+                               w->h_out, s);
+    ctx->prof_end();
+    if (rc) return ctx->fail(rc, "wfstep_update: workspace too small");
+    hipError_t er = hipGetLastError();
+    if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+    return QBG_OK;
+  });
 }
 
 int qbg_wfstep_results(qbg_wfstep* w, double* out) {
-  if (!w || !out) return QBG_E_INVAL;
-  qbg_ctx* ctx = w->ctx;
-  if (ctx->capturing) return ctx->fail(QBG_E_STATE, "wfstep_results during graph capture");
-  (void)hipSetDevice(ctx->device);
-  hipError_t er = hipStreamSynchronize(ctx->stream);
-  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
-  for (int i = 0; i < 4; i++) out[i] = w->h_out[i];
-  if (w->h_out[3] != 0.0)
-    return ctx->fail(QBG_E_NOT_SPD, "wfstep: orthogonalization failed (overlap not positive definite)");
-  return QBG_OK;
+  return qbg::guarded((w ? w->ctx : nullptr), [&]() -> int {
+    if (!w || !out) return QBG_E_INVAL;
+    qbg_ctx* ctx = w->ctx;
+    if (ctx->capturing) return ctx->fail(QBG_E_STATE, "wfstep_results during graph capture");
+    QBG_HIP(ctx, hipSetDevice(ctx->device));
+    hipError_t er = hipStreamSynchronize(ctx->stream);
+    if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+    for (int i = 0; i < 4; i++) out[i] = w->h_out[i];
+    if (w->h_out[3] != 0.0)
+      return ctx->fail(QBG_E_NOT_SPD, "wfstep: orthogonalization failed (overlap not positive definite)");
+    return QBG_OK;
+  });
 }
 
 // ----------------------------------------------------------------- mixer
 int qbg_mixer_create(qbg_ef* ef, double alpha, double rc_kerker, int ndim, double rc1,
                      qbg_mixer** out) {
-  if (!out) return QBG_E_INVAL;
-  *out = nullptr;
-  if (!ef) return QBG_E_INVAL;
-  qbg_ctx* c = ef->ctx;
-  if (ndim < 0 || ndim > MIX_NMAX) return c->fail(QBG_E_INVAL, "mixer: ndim must be 0..8");
-  if (!(alpha >= 0.0) || !(rc1 >= 0.0)) return c->fail(QBG_E_INVAL, "mixer: bad parameters");
-  if (c->capturing) return c->fail(QBG_E_STATE, "mixer_create during graph capture");
-  (void)hipSetDevice(c->device);
-  qbg_mixer* m = new (std::nothrow) qbg_mixer;
-  if (!m) return QBG_E_NOMEM;
-  m->ef = ef;
-  m->ctx = c;
-  m->alpha = alpha;
-  m->nmax = ndim;
-  m->anderson = ndim > 0 ? 1 : 0;
-  m->ng = ef->vb.size;
-  m->nsp = ef->nspin;
-  const int ng = m->ng, nsp = m->nsp;
-  // Kerker and least-squares row weights (src/BOSampleStepper.cpp:341-406)
-  std::vector<double> wk(ng, 1.0), wls(ng, 1.0);
-  const std::vector<double>& g2 = ef->vb.g2;
-  if (rc1 != 0.0) {
-    const double q1 = 2.0 * M_PI / rc1;
-    for (int i = 0; i < ng; i++) wls[i] = g2[i] != 0.0 ? std::sqrt(g2[i] / (g2[i] + q1 * q1)) : 1.0;
-  }
-  if (rc_kerker > 0.0) {
-    const double q0 = 2 * M_PI / rc_kerker, q02 = q0 * q0;
-    for (int i = 0; i < ng; i++) wk[i] = g2[i] != 0.0 ? g2[i] / (g2[i] + q02) : 1.0;
-  }
-  Allocs& a = m->mem;
-  // the mixed vector holds the densities of both spins (src/BOSampleStepper.cpp:318-327)
-  const size_t mlen = 2 * (size_t)nsp * ng;
-  int rc;
-  if ((rc = a.get(c, &m->wk, ng)) || (rc = a.get(c, &m->wls, ng)) ||
-      (rc = a.get(c, &m->rhog_in, mlen)) || (rc = a.get(c, &m->rhor_new, nsp * ef->N)) ||
-      (nsp == 2 && ((rc = a.get(c, &m->rhog_sp, mlen)) ||
-                    (rc = a.get(c, &m->rhog_tot, 2 * (size_t)ng)))) ||
-      (rc = a.get(c, &m->st, 4)) || (rc = a.get(c, &m->theta, MIX_NMAX)) ||
-      (m->anderson && ((rc = a.get(c, &m->xs, (m->nmax + 1) * mlen)) ||
-                       (rc = a.get(c, &m->fs, (m->nmax + 1) * mlen)) ||
-                       (rc = a.get(c, &m->partial, mix_partials()))))) {
-    a.release();
-    delete m;
-    return rc;
-  }
-  hipStream_t s = c->stream;
-  bool ok = hipMemcpyAsync(m->wk, wk.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(m->wls, wls.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess;
-  fill_zero(m->rhog_in, mlen, s);
-  fill_zero(m->rhor_new, nsp * ef->N, s);
-  fill_zero(m->theta, MIX_NMAX, s);
-  if (m->anderson) {
-    fill_zero(m->xs, (m->nmax + 1) * mlen, s);
-    fill_zero(m->fs, (m->nmax + 1) * mlen, s);
-  }
-  launch_mix_restart(m->st, s);
-  ok = ok && hipStreamSynchronize(s) == hipSuccess;
-  if (!ok) {
-    a.release();
-    delete m;
-    return c->fail(QBG_E_HIP, "mixer_create: initialization failed");
-  }
-  *out = m;
-  return QBG_OK;
+  return qbg::guarded((ef ? ef->ctx : nullptr), [&]() -> int {
+    if (!out) return QBG_E_INVAL;
+    *out = nullptr;
+    if (!ef) return QBG_E_INVAL;
+    qbg_ctx* c = ef->ctx;
+    if (ndim < 0 || ndim > MIX_NMAX) return c->fail(QBG_E_INVAL, "mixer: ndim must be 0..8");
+    if (!(alpha >= 0.0) || !(rc1 >= 0.0)) return c->fail(QBG_E_INVAL, "mixer: bad parameters");
+    if (c->capturing) return c->fail(QBG_E_STATE, "mixer_create during graph capture");
+    QBG_HIP(c, hipSetDevice(c->device));
+    qbg_mixer* m = new (std::nothrow) qbg_mixer;
+    if (!m) return QBG_E_NOMEM;
+    m->ef = ef;
+    m->ctx = c;
+    m->alpha = alpha;
+    m->nmax = ndim;
+    m->anderson = ndim > 0 ? 1 : 0;
+    m->ng = ef->vb.size;
+    m->nsp = ef->nspin;
+    const int ng = m->ng, nsp = m->nsp;
+    // Kerker and least-squares row weights (src/BOSampleStepper.cpp:341-406)
+    std::vector<double> wk(ng, 1.0), wls(ng, 1.0);
+    const std::vector<double>& g2 = ef->vb.g2;
+    if (rc1 != 0.0) {
+      const double q1 = 2.0 * M_PI / rc1;
+      for (int i = 0; i < ng; i++) wls[i] = g2[i] != 0.0 ? std::sqrt(g2[i] / (g2[i] + q1 * q1)) : 1.0;
+    }
+    if (rc_kerker > 0.0) {
+      const double q0 = 2 * M_PI / rc_kerker, q02 = q0 * q0;
+      for (int i = 0; i < ng; i++) wk[i] = g2[i] != 0.0 ? g2[i] / (g2[i] + q02) : 1.0;
+    }
+    Allocs& a = m->mem;
+    // the mixed vector holds the densities of both spins (src/BOSampleStepper.cpp:318-327)
+    const size_t mlen = 2 * (size_t)nsp * ng;
+    int rc;
+    if ((rc = a.get(c, &m->wk, ng)) || (rc = a.get(c, &m->wls, ng)) ||
+        (rc = a.get(c, &m->rhog_in, mlen)) || (rc = a.get(c, &m->rhor_new, nsp * ef->N)) ||
+        (nsp == 2 && ((rc = a.get(c, &m->rhog_sp, mlen)) ||
+                      (rc = a.get(c, &m->rhog_tot, 2 * (size_t)ng)))) ||
+        (rc = a.get(c, &m->st, 4)) || (rc = a.get(c, &m->theta, MIX_NMAX)) ||
+        (m->anderson && ((rc = a.get(c, &m->xs, (m->nmax + 1) * mlen)) ||
+                         (rc = a.get(c, &m->fs, (m->nmax + 1) * mlen)) ||
+                         (rc = a.get(c, &m->partial, mix_partials()))))) {
+      a.release();
+      delete m;
+      return rc;
+    }
+    hipStream_t s = c->stream;
+    bool ok = hipMemcpyAsync(m->wk, wk.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(m->wls, wls.data(), 8 * ng, hipMemcpyHostToDevice, s) == hipSuccess;
+    fill_zero(m->rhog_in, mlen, s);
+    fill_zero(m->rhor_new, nsp * ef->N, s);
+    fill_zero(m->theta, MIX_NMAX, s);
+    if (m->anderson) {
+      fill_zero(m->xs, (m->nmax + 1) * mlen, s);
+      fill_zero(m->fs, (m->nmax + 1) * mlen, s);
+    }
+    launch_mix_restart(m->st, s);
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) {
+      a.release();
+      delete m;
+      return c->fail(QBG_E_HIP, "mixer_create: initialization failed");
+    }
+    *out = m;
+    return QBG_OK;
+  });
 }
 
 int qbg_mixer_destroy(qbg_mixer* m) {
-  if (!m) return QBG_E_INVAL;
-  (void)hipSetDevice(m->ctx->device);
-  (void)hipStreamSynchronize(m->ctx->stream);
-  m->mem.release();
-  delete m;
-  return QBG_OK;
+  return qbg::guarded((m ? m->ctx : nullptr), [&]() -> int {
+    if (!m) return QBG_E_INVAL;
+    QBG_HIP(m->ctx, hipSetDevice(m->ctx->device));
+    (void)hipStreamSynchronize(m->ctx->stream);
+    m->mem.release();
+    delete m;
+    return QBG_OK;
+  });
 }
 
 int qbg_mixer_restart(qbg_mixer* m) {
-  if (!m) return QBG_E_INVAL;
-  launch_mix_restart(m->st, m->ctx->stream);
-  return QBG_OK;
+  return qbg::guarded((m ? m->ctx : nullptr), [&]() -> int {
+    if (!m) return QBG_E_INVAL;
+    launch_mix_restart(m->st, m->ctx->stream);
+    return QBG_OK;
+  });
 }
 
 // BOSampleStepper's mixing step (src/BOSampleStepper.cpp:810-879) and
@@ -418,83 +436,87 @@ int qbg_mixer_restart(qbg_mixer* m) {
 // inverse transform of its mixed rhog and the total rhog their sum
 // (src/EnergyFunctional.cpp:256-261)
 int qbg_mixer_update(qbg_mixer* m) {
-  if (!m) return QBG_E_INVAL;
-  qbg_ef* e = m->ef;
-  qbg_ctx* ctx = m->ctx;
-  if (!e->have_rho) return ctx->fail(QBG_E_STATE, "mixer_update before update_density");
-  if (e->nspin != m->nsp)
-    return ctx->fail(QBG_E_STATE, "mixer_update: nspin changed since the mixer was created");
-  (void)hipSetDevice(ctx->device);
-  hipStream_t s = ctx->stream;
-  qbg_grid* g = e->gv;
-  int rc = grid_ensure_ws(g, 1);
-  if (rc) return rc;
-  const int ng = m->ng, nsp = m->nsp, nv = nsp * ng;
-  const long N = e->N;
-  double* rhor[2] = {e->d_rhor, e->d_rhor2};
-  double* rg = nsp == 2 ? m->rhog_sp : e->d_rhog;
-  int r = 0;
-  ctx->prof_begin(QBG_K_MIX);
-  for (int sp = 0; nsp == 2 && sp < 2 && !r; sp++) {
-    r = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, rhor[sp], nullptr, nullptr, nullptr, e->omega, s);
-    r = r ? r : xfwd(g, QBG_PREC_FP64, 1, s);
-    r = r ? r : zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
-  }
-  if (!r) {
-    launch_mix_begin(m->st, m->nmax, m->anderson, s);
-    launch_mix_stage(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, nv, ng, m->alpha,
-                     m->anderson, s);
-    if (m->anderson) {
-      launch_mix_dots(m->fs, m->st, m->nmax, 2L * nv, m->partial, s);
-      launch_mix_solve(m->partial, m->st, m->anderson, m->theta, s);
-      launch_mix_apply(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, m->theta, m->nmax,
-                       nv, ng, m->alpha, m->anderson, s);
+  return qbg::guarded((m ? m->ctx : nullptr), [&]() -> int {
+    if (!m) return QBG_E_INVAL;
+    qbg_ef* e = m->ef;
+    qbg_ctx* ctx = m->ctx;
+    if (!e->have_rho) return ctx->fail(QBG_E_STATE, "mixer_update before update_density");
+    if (e->nspin != m->nsp)
+      return ctx->fail(QBG_E_STATE, "mixer_update: nspin changed since the mixer was created");
+    QBG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    qbg_grid* g = e->gv;
+    int rc = grid_ensure_ws(g, 1);
+    if (rc) return rc;
+    const int ng = m->ng, nsp = m->nsp, nv = nsp * ng;
+    const long N = e->N;
+    double* rhor[2] = {e->d_rhor, e->d_rhor2};
+    double* rg = nsp == 2 ? m->rhog_sp : e->d_rhog;
+    int r = 0;
+    ctx->prof_begin(QBG_K_MIX);
+    for (int sp = 0; nsp == 2 && sp < 2 && !r; sp++) {
+      r = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, rhor[sp], nullptr, nullptr, nullptr, e->omega, s);
+      r = r ? r : xfwd(g, QBG_PREC_FP64, 1, s);
+      r = r ? r : zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
     }
-  }
-  // rhor = Re(backward(rhog)) / omega per spin, kept only when this call mixed
-  for (int sp = 0; sp < nsp && !r; sp++) {
-    r = zbwd(g, QBG_PREC_FP64, Z_SINGLE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
-    r = r ? r : xbwd(g, QBG_PREC_FP64, 1, s);
-    r = r ? r : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
-                      m->rhor_new + sp * N, 1.0 / e->omega, s);
-    if (!r) launch_mix_select(rhor[sp], m->rhor_new + sp * N, N, m->st, s);
-  }
-  if (!r && nsp == 2) {
-    add2(m->rhog_sp, m->rhog_sp + 2L * ng, m->rhog_tot, 2L * ng, s);
-    launch_mix_select(e->d_rhog, m->rhog_tot, 2L * ng, m->st, s);
-  }
-  ctx->prof_end();
-  if (r) return ctx->fail(r, "mixer_update: density-grid transform refused");
-  e->have_vhxc = false;
-  hipError_t er = hipGetLastError();
-  if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
-  return QBG_OK;
+    if (!r) {
+      launch_mix_begin(m->st, m->nmax, m->anderson, s);
+      launch_mix_stage(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, nv, ng, m->alpha,
+                       m->anderson, s);
+      if (m->anderson) {
+        launch_mix_dots(m->fs, m->st, m->nmax, 2L * nv, m->partial, s);
+        launch_mix_solve(m->partial, m->st, m->anderson, m->theta, s);
+        launch_mix_apply(rg, m->rhog_in, m->wls, m->wk, m->xs, m->fs, m->st, m->theta, m->nmax,
+                         nv, ng, m->alpha, m->anderson, s);
+      }
+    }
+    // rhor = Re(backward(rhog)) / omega per spin, kept only when this call mixed
+    for (int sp = 0; sp < nsp && !r; sp++) {
+      r = zbwd(g, QBG_PREC_FP64, Z_SINGLE, rg + 2L * sp * ng, 0, 0, 0, 1, s);
+      r = r ? r : xbwd(g, QBG_PREC_FP64, 1, s);
+      r = r ? r : ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr,
+                        m->rhor_new + sp * N, 1.0 / e->omega, s);
+      if (!r) launch_mix_select(rhor[sp], m->rhor_new + sp * N, N, m->st, s);
+    }
+    if (!r && nsp == 2) {
+      add2(m->rhog_sp, m->rhog_sp + 2L * ng, m->rhog_tot, 2L * ng, s);
+      launch_mix_select(e->d_rhog, m->rhog_tot, 2L * ng, m->st, s);
+    }
+    ctx->prof_end();
+    if (r) return ctx->fail(r, "mixer_update: density-grid transform refused");
+    e->have_vhxc = false;
+    hipError_t er = hipGetLastError();
+    if (er != hipSuccess) return ctx->fail(QBG_E_HIP, hipGetErrorString(er));
+    return QBG_OK;
+  });
 }
 
 // ----------------------------------------------------------------- polar
 int qbg_polar(qbg_ctx* c, double* x, int n, double tol, int maxiter, int* iters) {
-  if (!c || !x || n <= 0 || maxiter < 0 || maxiter > 64 || !(tol >= 0.0)) return QBG_E_INVAL;
-  if (n > POLAR_NMAX) return c->fail(QBG_E_INVAL, "polar: n > 4096");
-  if (c->capturing) return c->fail(QBG_E_STATE, "polar allocates: not capturable");
-  (void)hipSetDevice(c->device);
-  Allocs a;
-  PolarWs w;
-  double* W = nullptr;
-  const size_t wsize = gemm_work(n);
-  int rc = w.alloc(c, a, n);
-  if (!rc) rc = a.get(c, &W, wsize);
-  if (rc) { a.release(); return rc; }
-  hipStream_t s = c->stream;
-  rc = polar_enqueue(w, x, n, tol, maxiter, W, wsize, s);
-  int flags[5] = {0, 0, 0, 0, 0};
-  hipError_t er = hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s);
-  if (er == hipSuccess) er = hipStreamSynchronize(s);
-  a.release();
-  if (rc) return c->fail(rc, "polar: workspace");
-  if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
-  if (iters) *iters = flags[3];
-  if (flags[4]) return c->fail(QBG_E_NOT_SPD, "polar: singular matrix in a Higham step");
-  return QBG_OK;
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !x || n <= 0 || maxiter < 0 || maxiter > 64 || !(tol >= 0.0)) return QBG_E_INVAL;
+    if (n > POLAR_NMAX) return c->fail(QBG_E_INVAL, "polar: n > 4096");
+    if (c->capturing) return c->fail(QBG_E_STATE, "polar allocates: not capturable");
+    QBG_HIP(c, hipSetDevice(c->device));
+    Allocs a;
+    PolarWs w;
+    double* W = nullptr;
+    const size_t wsize = gemm_work(n);
+    int rc = w.alloc(c, a, n);
+    if (!rc) rc = a.get(c, &W, wsize);
+    if (rc) { a.release(); return rc; }
+    hipStream_t s = c->stream;
+    rc = polar_enqueue(w, x, n, tol, maxiter, W, wsize, s);
+    int flags[5] = {0, 0, 0, 0, 0};
+    hipError_t er = hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s);
+    if (er == hipSuccess) er = hipStreamSynchronize(s);
+    a.release();
+    if (rc) return c->fail(rc, "polar: workspace");
+    if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+    if (iters) *iters = flags[3];
+    if (flags[4]) return c->fail(QBG_E_NOT_SPD, "polar: singular matrix in a Higham step");
+    return QBG_OK;
+  });
 }
 
 }  // extern "C"

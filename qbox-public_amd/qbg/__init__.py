@@ -4,5 +4,5 @@ The compute lives in libqbg.so (hand-written HIP kernels for gfx950 behind
 the C ABI of include/qbg.h); this package is the Python harness over it.
 """
 from . import configs  # noqa: F401
-from ._lib import (Context, DArray, EnergyFunctional, Graph, Grid, Mixer, Profiler,  # noqa: F401
-                   QbgError, WfStepper, ENERGY_NAMES, gram, load, polar)
+from ._lib import (Context, DArray, EnergyFunctional, Graph, Grid, Mixer, NonLocalPotential,  # noqa: F401,E501
+                   Profiler, QbgError, WfStepper, ENERGY_NAMES, gram, load, polar)

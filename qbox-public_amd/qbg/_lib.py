@@ -44,6 +44,7 @@ SIGNATURES = {
     "qbg_grid_create": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip,
                         C.POINTER(_vp)],
     "qbg_grid_destroy": [_vp],
+    "qbg_grid_info": [_vp, _ip],
     "qbg_fft_backward": [_vp, _vp, _vp, C.c_int],
     "qbg_fft_backward_pair": [_vp, _vp, _vp, _vp, C.c_int],
     "qbg_fft_forward": [_vp, _vp, _vp, C.c_int],
@@ -62,6 +63,12 @@ SIGNATURES = {
     "qbg_ef_sizes": [_vp, _ip],
     "qbg_ef_basis": [_vp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip],
     "qbg_ef_set_positions": [_vp, _dp],
+    "qbg_ef_cell_moved": [_vp, _dp],
+    "qbg_nl_create": [_vp, C.c_int, C.POINTER(_vp)],
+    "qbg_nl_destroy": [_vp],
+    "qbg_nl_update_twnl": [_vp],
+    "qbg_nl_energy": [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _dp, C.c_int, _dp,
+                      C.c_int, _dp],
     "qbg_ef_update_density": [_vp, _vp, C.c_int, C.c_int],
     "qbg_ef_update_vhxc": [_vp, C.c_int],
     "qbg_ef_energy": [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -310,6 +317,18 @@ class Grid:
         self.N = np0 * np1 * np2
         ctx.adopt(self)
 
+    def info(self) -> dict:
+        """qbg_grid_info: sizes, ntrans0 and the line splits that serve the grid."""
+        out = np.zeros(13, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.qbg_grid_info(self.h, out.ctypes.data_as(_ip)))
+        return dict(np=tuple(int(x) for x in out[:3]), ng=int(out[3]), nvec=int(out[4]),
+                    ntrans0=int(out[5]), four_step=bool(out[6]),
+                    splits=[(int(out[7 + 2 * a]), int(out[8 + 2 * a])) for a in range(3)])
+
+    @property
+    def four_step(self) -> bool:
+        return self.info()["four_step"]
+
     def release(self):
         if self.h and self.ctx.h:
             self.ctx.lib.qbg_grid_destroy(self.h)
@@ -394,9 +413,13 @@ class EnergyFunctional:
 
     def __init__(self, ctx: Context, sysdef, refcell=None):
         """refcell (3x3, rows a0 a1 a2): Wavefunction's reference cell, which
-        fixes the basis membership and the grids (src/Basis.cpp:314-325)."""
+        fixes the basis membership and the grids (src/Basis.cpp:314-325);
+        sysdef.refcell when not given."""
         self.ctx = ctx
         self.sysdef = sysdef
+        if refcell is None:
+            refcell = getattr(sysdef, "refcell", None)
+        self.refcell = None if refcell is None else np.asarray(refcell, dtype=np.float64)
         descs = (SpeciesDesc * len(sysdef.species))()
         self._keep = [descs]
         for i, sp in enumerate(sysdef.species):
@@ -453,6 +476,15 @@ class EnergyFunctional:
     def set_positions(self, tau):
         tau = np.ascontiguousarray(tau, dtype=np.float64).ravel()
         self.ctx.check(self.ctx.lib.qbg_ef_set_positions(self.h, _pd(tau)))
+
+    def cell_moved(self, cell):
+        """qbg_ef_cell_moved: Wavefunction::resize + EnergyFunctional::cell_moved
+        for the new cell (rows a0 a1 a2) over the creation's reference cell;
+        Cartesian positions unchanged (move them with set_positions)."""
+        import dataclasses
+        cell = np.ascontiguousarray(cell, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.qbg_ef_cell_moved(self.h, _pd(cell.ravel())))
+        self.sysdef = dataclasses.replace(self.sysdef, cell=cell.copy())
 
     def update_density(self, c: DArray, prec="fp64"):
         self.ctx.check(self.ctx.lib.qbg_ef_update_density(self.h, c.p, self.ngw, PREC[prec]))
@@ -535,6 +567,48 @@ class EnergyFunctional:
     def set_vr(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         self.ctx.check(self.ctx.lib.qbg_ef_set_vr(self.h, _pd(v)))
+
+
+class NonLocalPotential:
+    """qbg_nl: the NonLocalPotential of one spin's SlaterDet of an energy
+    functional (src/NonLocalPotential.h:61-68)."""
+
+    def __init__(self, ef: EnergyFunctional, ispin=0):
+        self.ef = ef
+        self.ctx = ef.ctx
+        h = _vp()
+        self.ctx.check(self.ctx.lib.qbg_nl_create(ef.h, int(ispin), C.byref(h)))
+        self.h = h
+        self.ctx.adopt(self)
+
+    def update_twnl(self):
+        self.ctx.check(self.ctx.lib.qbg_nl_update_twnl(self.h))
+
+    def energy(self, c: DArray, dsd: DArray = None, forces=False, stress=False, prec="fp64",
+               ldc=None, ldd=None):
+        """NonLocalPotential::energy: returns (enl, fion (nat x 3), sigma_enl[6]);
+        H*psi_NL is added into dsd when given."""
+        f = np.zeros(3 * self.ef.nat)
+        sig = np.zeros(6)
+        enl = C.c_double(0.0)
+        ldc = ldc or self.ef.ngw
+        ldd = ldd or self.ef.ngw
+        self.ctx.check(self.ctx.lib.qbg_nl_energy(
+            self.h, c.p if isinstance(c, DArray) else c, ldc, int(dsd is not None),
+            (dsd.p if isinstance(dsd, DArray) else dsd) if dsd is not None else None, ldd,
+            int(forces), _pd(f), int(stress), _pd(sig), PREC[prec], C.byref(enl)))
+        return enl.value, f.reshape(-1, 3), sig
+
+    def release(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.qbg_nl_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class WfStepper:

@@ -42,8 +42,9 @@ def sigma_matrix(v):
 
 
 class QboxModel:
-    """One system on the device: coefficients resident, the energy functional
-    rebuilt when the cell changes and moved when only positions change."""
+    """One system on the device: coefficients resident, one energy functional
+    moved to each new cell (qbg_ef_cell_moved over the reference cell) and to
+    each new set of positions (atoms_moved)."""
 
     def __init__(self, ctx: Context, sysdef, coefficients: np.ndarray, refcell=None,
                  prec: str = "fp64"):
@@ -56,14 +57,15 @@ class QboxModel:
         self._cell = None
 
     def _functional(self, cell, tau):
-        if self._ef is None or not np.array_equal(cell, self._cell):
-            if self._ef is not None:
-                self._ef.release()
+        if self._ef is None:
             sd = dataclasses.replace(self.sysdef, cell=cell, tau=tau)
             self._ef = EnergyFunctional(self.ctx, sd, refcell=self.refcell)
             self._cell = cell.copy()
-        else:
-            self._ef.set_positions(tau)
+            return self._ef
+        if not np.array_equal(cell, self._cell):
+            self._ef.cell_moved(cell)
+            self._cell = cell.copy()
+        self._ef.set_positions(tau)
         return self._ef
 
     def evaluate(self, tau: np.ndarray, eps: np.ndarray):

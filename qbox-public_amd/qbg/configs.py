@@ -75,6 +75,9 @@ class SystemDef:
     nspin: int = 1
     nst_up: int = 0
     occ_bands: np.ndarray = field(default=None)
+    # Wavefunction's reference cell (rows a0 a1 a2), which fixes the basis
+    # membership and the grids (src/Basis.cpp:314-325); None: the cell itself
+    refcell: np.ndarray = field(default=None)
 
     @property
     def ecut(self) -> float:         # Hartree (src/Ecut.h:38-57)
@@ -158,6 +161,29 @@ def config(index: int) -> SystemDef:
     else:
         raise ValueError(index)
     return s.build_positions()
+
+
+# line lengths with compiled four-step / prime-factor kernels (csrc/fft4_*.hip)
+FFT_LENGTHS = (22, 24, 28, 64, 90, 96, 112, 126, 128, 144, 160, 168, 198, 210, 224)
+
+
+def length_grid_system(k: int, nst: int = 5):
+    """Grid k of the fifteen that put every compiled line length on every
+    axis: (L[k], L[k+5], L[k+10]) with indices mod 15.  The cell has edges
+    0.3 bohr per grid point and the cutoff is as large as the forced grid
+    allows (density-basis half-width 0.45 of each axis), so the wavefunction
+    sphere fills the share of the active rows a natural grid has (c3: 96 of
+    210).  One atom of the first species at the origin.  Returns
+    (SystemDef, grid)."""
+    L = FFT_LENGTHS
+    n = len(L)
+    grid = (L[k % n], L[(k + 5) % n], L[(k + 10) % n])
+    h, x = 0.3, 0.45
+    ecut_ha = (2.0 * math.pi * x / h) ** 2 / 8.0
+    s = SystemDef(f"lengths[{k}]: grid {grid[0]}x{grid[1]}x{grid[2]}",
+                  np.diag([h * m for m in grid]), 2.0 * ecut_ha, [SPECIES_A], [1], nst,
+                  BASE_SEED + 200 + k, 0.0, tau=np.zeros((1, 3)))
+    return s, grid
 
 
 def config4_systems():

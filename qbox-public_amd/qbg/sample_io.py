@@ -326,13 +326,17 @@ def wavefunction_data(ef, c: np.ndarray, sysdef, nel: int, nempty: int = 0) -> W
     else:
         sds = [SlaterDetData(None, np.zeros(3), 1.0, occ, gf)]
     g = ef.wf_grid()
+    ref = getattr(sysdef, "refcell", None)
     return WavefunctionData(ecut=sysdef.ecut, nspin=nspin, nel=nel, nempty=nempty,
-                            cell=np.asarray(sysdef.cell, dtype=float), grid=g.np, sds=sds)
+                            cell=np.asarray(sysdef.cell, dtype=float), grid=g.np, sds=sds,
+                            refcell=None if ref is None else np.asarray(ref, dtype=float))
 
 
 def system_from_sample(s: Sample, name="sample"):
     """A system definition (species-major atoms, ecut, bands, occupations,
-    nspin) from a sample with an atomset and a wavefunction."""
+    nspin, and the wavefunction's reference cell when the sample has one, so
+    the functional is built on the basis the grid functions were written
+    for) from a sample with an atomset and a wavefunction."""
     from .configs import SystemDef
     if s.wf is None:
         raise ValueError("the sample has no wavefunction")
@@ -344,11 +348,20 @@ def system_from_sample(s: Sample, name="sample"):
     nst_up = s.wf.sds[0].occ.size if nspin == 2 else 0
     return SystemDef(name, np.asarray(s.cell, dtype=float), 2.0 * s.wf.ecut,
                      [s.species[sp] for sp in order], na, int(occ.size), 0, 0.0, tau=tau,
-                     nspin=nspin, nst_up=nst_up, occ_bands=occ)
+                     nspin=nspin, nst_up=nst_up, occ_bands=occ,
+                     refcell=None if s.wf.refcell is None else np.asarray(s.wf.refcell,
+                                                                          dtype=float))
 
 
 def sample_coefficients(ef, s: Sample) -> np.ndarray:
     """All bands of the sample's wavefunction (spin up first) as coefficients
-    of the functional ef."""
+    of the functional ef, which must have the basis the grid functions were
+    written for: the sample's reference cell (or its cell when it has none)
+    must be the functional's."""
+    want = s.wf.refcell if s.wf.refcell is not None else s.wf.cell
+    have = ef.refcell if ef.refcell is not None else ef.sysdef.cell
+    if not np.allclose(np.asarray(want, dtype=float), np.asarray(have, dtype=float),
+                       rtol=0, atol=1e-10):
+        raise ValueError("the functional's basis is not the sample's (reference cell differs)")
     gf = np.concatenate([sd.gf for sd in s.wf.sds])
     return coefficients(ef, gf)

@@ -1,5 +1,7 @@
 """Forces and stress as adjoints (qbg.autograd over the device path) against
-central finite differences of the device total energy at fixed coefficients:
+central finite differences of the device total energy at fixed coefficients,
+on two small cells and on the 128-atom orthorhombic 28 x 30 x 32 bohr
+workload (ecut 60 Ry, 256 bands):
 position steps of 1e-4 bohr and strain steps of 1e-4, with a reference cell
 fixing the basis (test/si2stress/check_stress.sh:20-90 for the strain
 convention)."""
@@ -20,12 +22,14 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("idx", [11, 12])
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("idx", [11, 12, 4])
 def test_autograd_forces_stress_vs_fd(ctx, idx):
     import qbg
     from qbg import autograd as qa
     s = cf.config(idx)
-    ref = s.cell * 1.04
+    # the 128-atom orthorhombic workload (144 x 160 x 168 grid) and two small cells
+    ref = s.cell * (1.0 if idx == 4 else 1.04)
     ef0 = qbg.EnergyFunctional(ctx, s, refcell=ref)
     c = cf.gram_numpy(cf.random_coefficients(321 + idx, ef0.basis(0)["g2"], s.nst))
     ef0.release()

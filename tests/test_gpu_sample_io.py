@@ -93,3 +93,39 @@ def test_sample_file_round_trip_energy(ctx, tmp_path, nspin):
     assert np.abs(e2["fion"] - e0["fion"]).max() < 1e-10
     ef.release()
     ef2.release()
+
+
+def test_sample_round_trip_with_reference_cell(ctx, tmp_path):
+    """A wavefunction on a strained cell whose basis is fixed by a reference
+    cell (Qbox writes <reference_domain>) reads back onto the same basis."""
+    import dataclasses
+    import qbg
+    s0 = _system(1)
+    strain = np.array([[1.02, 0.01, 0.0], [0.01, 0.99, 0.0], [0.0, 0.0, 1.01]])
+    s = dataclasses.replace(s0, cell=s0.cell @ strain.T, refcell=s0.cell * 1.05)
+    print("[case] sample file with a reference cell", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    c = _coeffs(s, ef.basis(0)["g2"], 41)
+    e0 = ef.evaluate(ctx.to_device(c), None, forces=True)
+    atoms = [sio.Atom(f"Si{i + 1}", "silicon", np.asarray(s.tau[i], dtype=float))
+             for i in range(s.nat)]
+    wf = sio.wavefunction_data(ef, c, s, nel=16)
+    sample = sio.Sample(cell=np.asarray(s.cell, dtype=float),
+                        species={"silicon": s.species[0]}, atoms=atoms, wf=wf)
+    path = tmp_path / "si4_strained.xml"
+    sio.write_sample(path, sample)
+    r = sio.read_sample(path)
+    assert r.wf.refcell is not None
+    s2 = sio.system_from_sample(r)
+    ef2 = qbg.EnergyFunctional(ctx, s2)
+    assert ef2.ngw == ef.ngw and np.array_equal(ef2.basis(0)["idx"], ef.basis(0)["idx"])
+    c2 = sio.sample_coefficients(ef2, r)
+    assert np.abs(c2 - c).max() < 1e-13 * np.abs(c).max()
+    e2 = ef2.evaluate(ctx.to_device(c2), None, forces=True)
+    assert abs(e2["etotal"] - e0["etotal"]) <= 1e-11 * abs(e0["etotal"])
+    # a functional on another basis is refused
+    ef3 = qbg.EnergyFunctional(ctx, dataclasses.replace(s2, refcell=None))
+    with pytest.raises(ValueError):
+        sio.sample_coefficients(ef3, r)
+    for x in (ef, ef2, ef3):
+        x.release()

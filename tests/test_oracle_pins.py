@@ -1,0 +1,189 @@
+"""External pins of the oracle's energy terms: results the mathematics fixes,
+independent of how the oracle computes them.
+
+- Local pseudopotential: the reference transforms v_loc(r) + Z erf(r/rcps)/r
+  (src/Species.cpp:648-700).  For v_loc = -Z erf(r/rc)/r the transform is
+  -4 pi Z (exp(-G^2 rc^2/4) - exp(-G^2 rcps^2/4)) / G^2, with the value
+  pi Z (rc^2 - rcps^2) at G = 0; checked with its derivative at spline
+  accuracy.
+- Projectors: 4 pi int r^2 j_l(Gr) p_l(r) dr (src/Species.cpp:782-855) of the
+  Gaussian projectors p_0 = exp(-r^2/(2 s^2)) and p_1 = r exp(-r^2/(2 s^2))
+  are (2 pi)^{3/2} s^3 exp(-G^2 s^2/2) and (2 pi)^{3/2} s^5 G exp(-G^2 s^2/2).
+- Nonlocal energy: one atom and one Gaussian orbital psi (s and p type) in a
+  large cell; enl = occ D_l <beta_lm|psi>^2 with beta_lm = p_l(r) Y_lm and the
+  overlap integral in closed form (src/NonLocalPotential.cpp:1469-1571).
+  This fixes D_l, the projector weights and the 1/omega of the reference.
+- Ion-ion energy: esr - eself + ehart_p - pi Q^2 / (2 omega eta^2) (eta =
+  1/(sqrt 2 rcps)) is the Ewald energy of the point ions in a neutralizing
+  background, computed here by an independent Ewald sum with another
+  splitting parameter (src/EnergyFunctional.cpp:1097-1192, :327-351).
+- Smearing width: for a neutral system etotal does not depend on rcps
+  (it moves energy between eps, ehart, esr and eself, src/AtomSet.cpp:43-44).
+"""
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+from qbg import configs as cf
+
+
+def _species_oracle(oracle_mod, sp, L=16.0, ecut_ry=30.0):
+    s = cf.SystemDef("one atom", cf.cubic(L), ecut_ry, [sp], [1], 1, 1, 0.0,
+                     tau=np.zeros((1, 3)))
+    return oracle_mod.Oracle(s)
+
+
+def _closed_forms(sp, g):
+    """(vloc, dvloc, p0, dp0, p1, dp1) of the closed forms at g > 0."""
+    Z, rc, rp, s = sp.zval, sp.rc, sp.rcps, sp.sigma
+    gg = g * g
+    a, b = np.exp(-gg * rc ** 2 / 4), np.exp(-gg * rp ** 2 / 4)
+    v = -4 * math.pi * Z * (a - b) / gg
+    dv = -4 * math.pi * Z * ((-rc ** 2 * g / 2 * a + rp ** 2 * g / 2 * b) / gg - 2 * (a - b) / (gg * g))
+    e = np.exp(-gg * s * s / 2)
+    p0 = (2 * math.pi) ** 1.5 * s ** 3 * e
+    p1 = (2 * math.pi) ** 1.5 * s ** 5 * g * e
+    return v, dv, p0, -(s * s) * g * p0, p1, (2 * math.pi) ** 1.5 * s ** 5 * e * (1 - gg * s * s)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("sp", [cf.SPECIES_A, cf.SPECIES_B])
+def test_species_transforms_closed_form(oracle_mod, sp):
+    """At the spline nodes g_i = i pi / (ndft dr) the tables are the
+    transforms themselves (1e-10); between nodes the cubic spline adds its
+    interpolation error (a few 1e-7 at dg = 0.077 / bohr)."""
+    o = _species_oracle(oracle_mod, sp)
+    dg = math.pi / (o.ndft * sp.deltar)
+    nodes = np.arange(1, int(12.0 / dg)) * dg
+    v, dv, vn, dvn = o.species_eval(0, nodes)
+    cv, cdv, p0, dp0, p1, dp1 = _closed_forms(sp, nodes)
+    assert _rel(v, cv) < 1e-10 and _rel(vn[0], p0) < 1e-10 and _rel(vn[1], p1) < 1e-10
+    v0 = o.species_eval(0, np.zeros(1))[0][0]
+    assert abs(v0 - math.pi * sp.zval * (sp.rc ** 2 - sp.rcps ** 2)) < 1e-10 * abs(v0)
+    mid = nodes + 0.5 * dg
+    v, dv, vn, dvn = o.species_eval(0, mid)
+    cv, cdv, p0, dp0, p1, dp1 = _closed_forms(sp, mid)
+    assert _rel(v, cv) < 1e-6 and _rel(vn[0], p0) < 1e-6 and _rel(vn[1], p1) < 1e-6
+    assert _rel(dv, cdv) < 1e-5 and _rel(dvn[0], dp0) < 1e-5 and _rel(dvn[1], dp1) < 1e-5
+
+
+def _gaussian_orbital(b, omega, a, kind):
+    """Plane-wave coefficients (half sphere) of a normalized Gaussian orbital
+    of width a at the origin: s = exp(-r^2/2a^2), p = x exp(-r^2/2a^2);
+    c(G) = omega^{-1/2} int psi e^{-iGr} d^3r."""
+    g2 = b["g2"]
+    gx = b["gx"][0]
+    fg = (2 * math.pi * a * a) ** 1.5 * np.exp(-g2 * a * a / 2)
+    if kind == "s":
+        n = (math.pi * a * a) ** -0.75
+        c = n * fg + 0j
+    else:
+        n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
+        c = n * (-1j * a * a * gx) * fg
+    return (c / math.sqrt(omega))[None, :]
+
+
+@pytest.mark.parametrize("kind", ["s", "p"])
+@pytest.mark.parametrize("sp", [cf.SPECIES_A, cf.SPECIES_B])
+def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
+    L, a = 22.0, 1.1
+    s = cf.SystemDef("one atom, one orbital", cf.cubic(L), 50.0, [sp], [1], 1, 1, 0.0,
+                     tau=np.zeros((1, 3)), occ_bands=np.array([2.0]))
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    c = _gaussian_orbital(b, L ** 3, a, kind)
+    # the orbital is normalized on the half sphere (Gamma convention) up to
+    # the basis truncation
+    norm = 2 * np.sum(np.abs(c[0]) ** 2) - abs(c[0, 0]) ** 2
+    assert abs(norm - 1) < 1e-12
+    enl = o.nl_energy(c, None, forces=False, stress=False)[0]
+    sg = sp.sigma
+    s2 = 1.0 / (1.0 / sg ** 2 + 1.0 / a ** 2)
+    if kind == "s":
+        ov = (4 * math.pi) ** -0.5 * (math.pi * a * a) ** -0.75 * (2 * math.pi * s2) ** 1.5
+        ref = 2.0 * sp.d[0] * ov ** 2
+    else:
+        n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
+        ov = math.sqrt(3 / (4 * math.pi)) * n * s2 * (2 * math.pi * s2) ** 1.5
+        ref = 2.0 * sp.d[1] * ov ** 2
+    assert abs(enl - ref) < 1e-6 * abs(ref), (enl, ref)   # spline accuracy of twnl
+
+
+def _ewald(cell, tau, z, eta):
+    """Ewald energy of point charges z at tau in a neutralizing background."""
+    A = np.asarray(cell, float)
+    omega = abs(np.linalg.det(A))
+    B = 2 * math.pi * np.linalg.inv(A).T
+    n = len(z)
+    Q = z.sum()
+    # real space
+    rmax = 40.0 / eta ** 0 if False else 7.0 / eta
+    nr = [int(rmax / np.linalg.norm(A[i])) + 2 for i in range(3)]
+    er = 0.0
+    for i in range(n):
+        for j in range(n):
+            d = tau[i] - tau[j]
+            for i0 in range(-nr[0], nr[0] + 1):
+                for i1 in range(-nr[1], nr[1] + 1):
+                    rr = d + i0 * A[0] + i1 * A[1] + np.outer(np.arange(-nr[2], nr[2] + 1), A[2])
+                    r = np.linalg.norm(rr, axis=1)
+                    if i == j and i0 == 0 and i1 == 0:
+                        r = r[r > 1e-12]
+                    er += 0.5 * z[i] * z[j] * np.sum(erfc(eta * r) / r)
+    # reciprocal space
+    gmax = 2 * eta * 6.5
+    ng = [int(gmax / np.linalg.norm(B[i])) + 2 for i in range(3)]
+    h = np.arange(-ng[0], ng[0] + 1)
+    k = np.arange(-ng[1], ng[1] + 1)
+    l_ = np.arange(-ng[2], ng[2] + 1)
+    H, K, Lg = np.meshgrid(h, k, l_, indexing="ij")
+    G = (H[..., None] * B[0] + K[..., None] * B[1] + Lg[..., None] * B[2]).reshape(-1, 3)
+    G2 = (G * G).sum(1)
+    G, G2 = G[G2 > 1e-12], G2[G2 > 1e-12]
+    S = (z[None, :] * np.exp(-1j * G @ tau.T)).sum(1)
+    erec = 2 * math.pi / omega * np.sum(np.abs(S) ** 2 * np.exp(-G2 / (4 * eta * eta)) / G2)
+    eself = eta / math.sqrt(math.pi) * np.sum(z * z)
+    ebg = math.pi * Q * Q / (2 * omega * eta * eta)
+    return er + erec - eself - ebg
+
+
+@pytest.mark.parametrize("idx", [0, 11, 12])
+def test_ion_energy_against_ewald(oracle_mod, idx):
+    s = cf.config(idx)
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(9, o.basis(0)["g2"], s.nst))
+    r = o.evaluate(c, hpsi=False, forces=False)
+    z = np.concatenate([[sp.zval] * n for sp, n in zip(s.species, s.na)]).astype(float)
+    omega = abs(np.linalg.det(s.cell))
+    eta = 1.0 / (math.sqrt(2.0) * cf.RCPS)
+    qbox = r["esr"] - r["eself"] + r["ehart_p"] - math.pi * z.sum() ** 2 / (2 * omega * eta ** 2)
+    for eta2 in (0.45, 0.7):
+        ref = _ewald(s.cell, s.tau, z, eta2)
+        assert abs(qbox - ref) < 1e-12 * abs(ref), (idx, eta2, qbox, ref)
+
+
+def _with_rcps(s, rcps):
+    return dataclasses.replace(s, species=[dataclasses.replace(sp, rcps=rcps)
+                                           for sp in s.species])
+
+
+@pytest.mark.parametrize("idx,nst", [(10, 4), (11, 7)])
+def test_etotal_independent_of_rcps(oracle_mod, idx, nst):
+    s = dataclasses.replace(cf.config(idx), nst=nst, ecut_ry=40.0)
+    assert 2 * nst == sum(sp.zval * n for sp, n in zip(s.species, s.na))     # neutral
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(21, o.basis(0)["g2"], nst))
+    out = {}
+    for rcps in (1.0, 1.2, 1.5):
+        r = oracle_mod.Oracle(_with_rcps(s, rcps)).evaluate(c, hpsi=False, forces=False)
+        out[rcps] = r
+    et = [out[k]["etotal"] for k in out]
+    # the parts do move with rcps; their sum does not
+    assert abs(out[1.0]["esr"] - out[1.5]["esr"]) > 1e-3
+    assert max(et) - min(et) < 1e-7, et
