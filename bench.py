@@ -199,53 +199,68 @@ def run(ef, dc, hp, prec, steps, warmup, ctx):
 
 def bench_c4(args):
     """Twelve independent systems of uneven size (the uneven-batching
-    workload): a loop evaluating one system after the other (each waits for
-    its results) against one HIP graph holding all twelve evaluations."""
+    workload), evaluated three ways: a loop over the systems (each waits for
+    its results), one HIP graph holding the twelve evaluations one after the
+    other, and one graph holding them as twelve parallel branches (one
+    context and stream per system, joined by qbg_stream_join) so the small
+    systems' launches run beside the large ones'."""
     import qbg
     from qbg import configs as cf
-    ctx = qbg.Context(0)
     prec = "bf16x2" if args.prec == "mixed" else "fp64"
     systems = cf.config4_systems()
-    efs, dcs, hps = [], [], []
+    main = qbg.Context(0)
+    ctxs, efs, dcs, hps = [], [], [], []
     for s in systems:
+        ctx = qbg.Context(0)
         ef = qbg.EnergyFunctional(ctx, s)
-        c = cf.random_coefficients(cf.BASE_SEED + 60, ef.basis(0)["g2"], s.nst)
-        dc = ctx.to_device(c)
-        qbg.gram(ctx, dc, ef.ngw, ef.ngw, s.nst)
+        c = cf.gram_numpy(cf.random_coefficients(cf.BASE_SEED + 60, ef.basis(0)["g2"], s.nst))
+        ctxs.append(ctx)
         efs.append(ef)
-        dcs.append(dc)
+        dcs.append(ctx.to_device(c))
         hps.append(ctx.array(c.shape, np.complex128))
         print(f"[bench c4] {s.name}: grid {ef.np0}x{ef.np1}x{ef.np2} ngw {ef.ngw}",
               file=sys.stderr, flush=True)
     nb = sum(s.nst for s in systems)
 
     def loop():
-        out = []
-        for ef, dc, hp in zip(efs, dcs, hps):
-            out.append(ef.evaluate(dc, hp, forces=True, prec=prec)["etotal"])
-        return out
+        return [ef.evaluate(dc, hp, forces=True, prec=prec)["etotal"]
+                for ef, dc, hp in zip(efs, dcs, hps)]
 
     for _ in range(max(1, args.warmup)):
         ref = loop()
-    ctx.sync()
+    for ctx in ctxs:
+        ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         loop()
     t_loop = (time.perf_counter() - t0) / args.steps
-    for ef, dc, hp in zip(efs, dcs, hps):          # allocate before capture
-        ef.evaluate_async(dc, hp, forces=True, prec=prec)
-        ef.results()
-    with qbg.Graph(ctx) as g:
-        for ef, dc, hp in zip(efs, dcs, hps):
-            ef.evaluate_async(dc, hp, forces=True, prec=prec)
-    g.launch()
-    got = [ef.results()["etotal"] for ef in efs]
-    assert got == ref, "graph replay differs from the loop"
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+    def graph(parallel):
+        with qbg.Graph(main) as g:
+            for ctx, ef, dc, hp in zip(ctxs, efs, dcs, hps):
+                qbg.stream_join(main, ctx)
+                ef.evaluate_async(dc, hp, forces=True, prec=prec)
+                if not parallel:
+                    qbg.stream_join(ctx, main)
+            if parallel:
+                for ctx in ctxs:
+                    qbg.stream_join(ctx, main)
+        return g
+
+    out = {}
+    for name, parallel in (("serial", False), ("parallel", True)):
+        g = graph(parallel)
         g.launch()
-    ctx.sync()
-    t_graph = (time.perf_counter() - t0) / args.steps
+        main.sync()
+        got = [ef.results()["etotal"] for ef in efs]
+        assert got == ref, f"graph ({name}) replay differs from the loop"
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.launch()
+        main.sync()
+        out[name] = (time.perf_counter() - t0) / args.steps
+        g.release()
+    t_graph = out["parallel"]
     print(json.dumps({
         "metric": "uneven batch: H*psi band-applies/s of twelve systems per evaluation round",
         "value": round(nb / t_graph, 2), "unit": "band-applies/s", "n_gpus": 1,
@@ -256,7 +271,9 @@ def bench_c4(args):
                    "systems": [s.name for s in systems], "bands_total": nb,
                    "precision": prec},
         "loop_value": round(nb / t_loop, 2), "loop_ms_per_step": round(1e3 * t_loop, 3),
-        "graph_over_loop": round(t_loop / t_graph, 3)}))
+        "serial_graph_ms_per_step": round(1e3 * out["serial"], 3),
+        "graph_over_loop": round(t_loop / t_graph, 3),
+        "parallel_over_serial_graph": round(out["serial"] / t_graph, 3)}))
 
 
 def bench_scf(args):

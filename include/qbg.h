@@ -305,6 +305,16 @@ int qbg_ef_results(qbg_ef* ef, double* energies, double* fion, double* sigma);
  * context stream.  Calls that synchronize, allocate or copy from the host
  * fail with QBG_E_STATE while capturing.  No counterpart in the reference:
  * this is the per-SCF-iteration launch graph of the evaluation path. */
+/* Order two contexts' streams (same device): work enqueued on `after` from
+ * now on waits for the work enqueued on `before` so far.  While `before`
+ * captures a graph, `after`'s stream joins the capture, and the calls made
+ * on `after` become a parallel branch of the graph; join the branch back
+ * with qbg_stream_join(after, before) before qbg_graph_end(before).  A graph
+ * launched on one context and holding other contexts' branches is complete
+ * when the launching context's stream is (qbg_sync it before reading the
+ * branches' results).  No counterpart in the reference (one sequence of work
+ * per MPI task); it serves several independent systems in one graph. */
+int qbg_stream_join(qbg_ctx* before, qbg_ctx* after);
 int qbg_graph_begin(qbg_ctx* ctx);
 int qbg_graph_end(qbg_ctx* ctx, qbg_graph** out);
 int qbg_graph_launch(qbg_graph* graph);

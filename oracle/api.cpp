@@ -98,6 +98,11 @@ void qbo_sizes(void* h, int* out) {
   out[11] = s->sp.empty() ? 0 : s->sp[0].ndft;
 }
 
+// projectors per atom of species is (one per (l, m), l <= lmax of the KB form)
+int qbo_nlm(void* h, int is) { return ((System*)h)->sp[is].nlm; }
+// radial projector channels of species is (lmax + 1 of the KB form)
+int qbo_nop(void* h, int is) { return ((System*)h)->sp[is].nop; }
+
 // which = 0: wavefunction basis (on the density grid), 1: density basis
 void qbo_basis(void* h, int which, int* idx, double* g2, double* gx, int* ip,
                int* im, int* zvec_to_val, int* np_basis) {

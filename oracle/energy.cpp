@@ -108,7 +108,7 @@ void System::update_twnl() {
           dt[4 * ngw + ig] = tmp * tgy * tgz;
           dt[5 * ngw + ig] = tmp * tgx * tgz;
         }
-      } else {
+      } else if (l == 1) {
         for (int m = 0; m < 3; m++) lproj[is][ilm + m] = 1;
         double* t1 = &twnl[is][(size_t)ngw * ilm];
         double* t2 = &twnl[is][(size_t)ngw * (ilm + 1)];
@@ -151,6 +151,49 @@ void System::update_twnl() {
           d3[3 * ngw + ig] = fac3 * tgx * tgy;
           d3[4 * ngw + ig] = fac3 * tgy * tgz + v * y2;
           d3[5 * ngw + ig] = fac3 * tgx * tgz + v * y1;
+        }
+      } else {   // l == 2 (:627-760): y4..y8 and their strain derivatives
+        const double s54pi = std::sqrt(5.0 / fpi), s3 = std::sqrt(3.0);
+        const double s20pi = std::sqrt(20.0 * M_PI), s20pi3 = std::sqrt(20.0 * M_PI / 3.0);
+        for (int m = 0; m < 5; m++) lproj[is][ilm + m] = 2;
+        double* t[5];
+        double* d[5];
+        for (int m = 0; m < 5; m++) {
+          t[m] = &twnl[is][(size_t)ngw * (ilm + m)];
+          d[m] = &dtwnl[is][(size_t)ngw * 6 * (ilm + m)];
+        }
+#pragma omp parallel for schedule(static)
+        for (int ig = 0; ig < ngw; ig++) {
+          double v, dv;
+          const double tg = kpg[ig];
+          s.dvnlg(iop, tg, v, dv);
+          const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
+          const double tgi = kpgi[ig], tgi2 = tgi * tgi;
+          // unit-vector products, ij order xx yy zz xy yz xz
+          const double uu[6] = {tgx * tgx * tgi2, tgy * tgy * tgi2, tgz * tgz * tgi2,
+                                tgx * tgy * tgi2, tgy * tgz * tgi2, tgx * tgz * tgi2};
+          const double y[5] = {s54pi * 0.5 * (3.0 * uu[2] - 1.0), s54pi * 0.5 * s3 * (uu[0] - uu[1]),
+                               s54pi * s3 * uu[3], s54pi * s3 * uu[4], s54pi * s3 * uu[5]};
+          const double y1x = s34pi * tgx * tgi, y1y = s34pi * tgy * tgi, y1z = s34pi * tgz * tgi;
+          // derivatives of the p harmonics: dX_ij = y1X uu_ij - (delta terms)
+          double dx[6], dy[6], dz[6];
+          for (int ij = 0; ij < 6; ij++) {
+            dx[ij] = y1x * uu[ij];
+            dy[ij] = y1y * uu[ij];
+            dz[ij] = y1z * uu[ij];
+          }
+          dx[0] -= y1x;
+          dy[1] -= y1y; dy[3] -= y1x;
+          dz[2] -= y1z; dz[4] -= y1y; dz[5] -= y1x;
+          for (int m = 0; m < 5; m++) t[m][ig] = y[m] * v;
+          for (int ij = 0; ij < 6; ij++) {
+            const double r = dv * tg * uu[ij];
+            d[0][ij * ngw + ig] = -(v * s20pi * dz[ij] * y1z - y[0] * r);
+            d[1][ij * ngw + ig] = -(v * s20pi3 * (y1x * dx[ij] - y1y * dy[ij]) - y[1] * r);
+            d[2][ij * ngw + ig] = -(v * s20pi3 * (dx[ij] * y1y + y1x * dy[ij]) - y[2] * r);
+            d[3][ij * ngw + ig] = -(v * s20pi3 * (dy[ij] * y1z + y1y * dz[ij]) - y[3] * r);
+            d[4][ij * ngw + ig] = -(v * s20pi3 * (dx[ij] * y1z + y1x * dz[ij]) - y[4] * r);
+          }
         }
       }
       ilm += 2 * l + 1;
@@ -509,6 +552,16 @@ static void gemm_nn_acc(long K, int N, int M, const double* A, long lda,
 // projectors with l <= 1.  The reference processes atoms in blocks of 32
 // (:1274); the block size only reorders independent columns and is not
 // restated.
+// t (-i)^l (c + i s) as (re, im)
+static inline void phase_l(int l, double t, double c, double s, double& re, double& im) {
+  switch (l & 3) {
+    case 0: re = t * c; im = t * s; break;
+    case 1: re = t * s; im = -t * c; break;
+    case 2: re = -t * c; im = -t * s; break;
+    default: re = -t * s; im = t * c; break;
+  }
+}
+
 double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                          bool compute_forces, std::vector<double>& fion_enl,
                          bool compute_stress, double* sigma_enl) {
@@ -553,18 +606,11 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
           const double* cc = &cgr[(size_t)ia * ngw];
           const double* ss = &sgr[(size_t)ia * ngw];
           for (int ig = 0; ig < ngw; ig++) {
-            if (dir < 0) {
-              if (l == 0) { a[2 * ig] = t[ig] * cc[ig]; a[2 * ig + 1] = t[ig] * ss[ig]; }
-              else { a[2 * ig] = t[ig] * ss[ig]; a[2 * ig + 1] = -t[ig] * cc[ig]; }
-            } else {
-              if (l == 0) {       // :1656-1665
-                const double tt = kx[dir][ig] * t[ig];
-                a[2 * ig] = tt * ss[ig]; a[2 * ig + 1] = -tt * cc[ig];
-              } else {            // :1667-1675
-                const double tt = -kx[dir][ig] * t[ig];
-                a[2 * ig] = tt * cc[ig]; a[2 * ig + 1] = tt * ss[ig];
-              }
-            }
+            // anl = t (-i)^l e^{-iG.tau} (:1341-1395); -i G_dir anl for the
+            // forces (:1656-1700): the phase of l + 1 with t G_dir
+            const int ll = dir < 0 ? l : l + 1;
+            const double tt = dir < 0 ? t[ig] : kx[dir][ig] * t[ig];
+            phase_l(ll, tt, cc[ig], ss[ig], a[2 * ig], a[2 * ig + 1]);
           }
         }
     };
@@ -627,10 +673,8 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
             double* a = &anl[(size_t)K * (ia + ipr * na)];
             const double* cc = &cgr[(size_t)ia * ngw];
             const double* ss = &sgr[(size_t)ia * ngw];
-            for (int ig = 0; ig < ngw; ig++) {
-              if (l == 0) { a[2 * ig] = d[ig] * cc[ig]; a[2 * ig + 1] = d[ig] * ss[ig]; }
-              else { a[2 * ig] = d[ig] * ss[ig]; a[2 * ig + 1] = -d[ig] * cc[ig]; }
-            }
+            for (int ig = 0; ig < ngw; ig++)
+              phase_l(l, d[ig], cc[ig], ss[ig], a[2 * ig], a[2 * ig + 1]);
           }
         // dfnl = 2 danl^T c, no G=0 correction (dtwnl(G=0) = 0)
         gemm_tn(M, nst, K, 2.0, anl.data(), K, cp, K, 0.0, dfnl.data(), M);

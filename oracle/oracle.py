@@ -88,6 +88,8 @@ def lib():
         _lib.qbo_update_rhor.argtypes = [C.c_void_p]
         _lib.qbo_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
                                     _dp, _dp, _dp]
+        _lib.qbo_nlm.argtypes = [C.c_void_p, C.c_int]
+        _lib.qbo_nop.argtypes = [C.c_void_p, C.c_int]
         _lib.qbo_nl_energy.restype = C.c_double
         _lib.qbo_nl_energy.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, C.c_int,
                                        _dp, _dp]
@@ -225,7 +227,7 @@ class Oracle:
         lib().qbo_update_vhxc(self.h, int(stress))
 
     def get(self, what, isp=0):
-        npr = 4
+        npr = lib().qbo_nlm(self.h, isp)
         sizes = {0: (self.N, float), 1: (self.N, float), 2: (self.N, float),
                  3: (self.ngv, complex), 4: (self.ngv, complex), 5: (self.ngv, complex),
                  6: (self.ngv, float), 7: (self.ngv, float), 8: (self.ngv, float),
@@ -282,9 +284,15 @@ class Oracle:
     def mixer(self, alpha=0.5, rc_kerker=10.0, ndim=3, rc1=3.0):
         return Mixer(self, alpha, rc_kerker, ndim, rc1)
 
-    def species_eval(self, isp, g, nop=2):
+    def species_eval(self, isp, g, nop=None):
+        """(vlocg, dvlocg, vnlg[l], dvnlg[l]) of species isp at |G| = g; one
+        row per projector channel l = 0 .. lmax."""
         g = np.ascontiguousarray(g, dtype=float)
         n = g.size
+        have = lib().qbo_nop(self.h, isp)
+        if nop is not None and nop != have:
+            raise ValueError(f"species {isp} has {have} projector channels")
+        nop = have
         v, dv = np.zeros(n), np.zeros(n)
         vn, dvn = np.zeros(nop * n), np.zeros(nop * n)
         lib().qbo_species_eval(self.h, isp, n, _ptr(g), _ptr(v), _ptr(dv), _ptr(vn), _ptr(dvn))

@@ -139,11 +139,11 @@ void splintd(int n, const double* xa, const double* ya, const double* y2a,
         h * (((1.0 / 6.0) - 0.5 * a * a) * y2a[kl] + (0.5 * b * b - (1.0 / 6.0)) * y2a[kh]);
 }
 
-// src/Species.cpp:510-1053 (SLPP; one projector per l, l <= 1)
+// src/Species.cpp:510-1053 (SLPP; one projector per l, l <= 2)
 void Species::initialize() {
   const double fpi = 4.0 * M_PI;
   const int np = (int)vlocal.size();
-  if (lmax > 1) throw std::runtime_error("oracle species: l > 1 not restated");
+  if (lmax > 2) throw std::runtime_error("oracle species: l > 2 not restated");
   nop = lmax + 1;
   nlm = 0;
   for (int l = 0; l <= lmax; l++) nlm += 2 * l + 1;
@@ -190,6 +190,22 @@ void Species::initialize() {
       sinft(ndft, v.data());
       v[0] = w0;
       for (int i = 1; i < ndft; i++) v[i] /= gspl[i];
+      spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 0, 1, vnlg2[iop].data());
+    } else if (iop == 2) {  // :872-940, j_2 in three transforms; 0 at G = 0
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = vnlr[i] / rps[i];
+      sinft(ndft, v.data());
+      fint[0] = 0.0;
+      for (int i = 1; i < ndft; i++) fint[i] = 3.0 * v[i] / (gspl[i] * gspl[i] * gspl[i]);
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i] * rps[i];
+      sinft(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] += -v[i] / gspl[i];
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] += -3.0 * v[i] / (gspl[i] * gspl[i]);
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = fint[i];
       spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 0, 1, vnlg2[iop].data());
     } else {  // l == 1, :812-855
       const double w0 = 0.0;
@@ -343,7 +359,7 @@ void ncpp_species(Species& q, int lmax, int llocal, int nmesh, const double* vps
   int lkb = 0;
   for (int l = 0; l <= lmax; l++)
     if (l != llocal) {
-      if (l > 1) throw std::runtime_error("ncpp: nonlocal channel with l > 1");
+      if (l > 2) throw std::runtime_error("ncpp: nonlocal channel with l > 2");
       lkb = std::max(lkb, l);
     }
   const double* vl = vps + (size_t)llocal * nmesh;

@@ -171,15 +171,38 @@ static void ef_local_tables(const qbg_ef* e, std::vector<double>& vps, std::vect
     }
 }
 
+// real spherical harmonic of projector ipr of a species (one projector per
+// l: ipr 0 s; 1-3 p as x, y, z; 4-8 d as 3z^2-1, x^2-y^2, xy, yz, xz, the
+// order of src/NonLocalPotential.cpp:250-760) at the unit vector u: value
+// and gradient with respect to u
+static int proj_l(int ipr) { return ipr == 0 ? 0 : (ipr < 4 ? 1 : 2); }
+static void harmonic(int ipr, const double* u, double& y, double* dy) {
+  const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
+  const double d5 = std::sqrt(5.0 / (4.0 * M_PI)), d53 = d5 * std::sqrt(3.0);
+  dy[0] = dy[1] = dy[2] = 0.0;
+  switch (ipr) {
+    case 0: y = s14pi; break;
+    case 1: case 2: case 3: y = s34pi * u[ipr - 1]; dy[ipr - 1] = s34pi; break;
+    case 4: y = 0.5 * d5 * (3.0 * u[2] * u[2] - 1.0); dy[2] = 3.0 * d5 * u[2]; break;
+    case 5: y = 0.5 * d53 * (u[0] * u[0] - u[1] * u[1]); dy[0] = d53 * u[0]; dy[1] = -d53 * u[1]; break;
+    case 6: y = d53 * u[0] * u[1]; dy[0] = d53 * u[1]; dy[1] = d53 * u[0]; break;
+    case 7: y = d53 * u[1] * u[2]; dy[1] = d53 * u[2]; dy[2] = d53 * u[1]; break;
+    default: y = d53 * u[0] * u[2]; dy[0] = d53 * u[2]; dy[2] = d53 * u[0]; break;
+  }
+}
+
 // NonLocalPotential::update_twnl on the wavefunction basis of the current
-// cell (src/NonLocalPotential.cpp:235-520, KB l = 0, 1): twnl[is] =
+// cell (src/NonLocalPotential.cpp:235-760, KB projectors l <= 2): twnl[is] =
 // [ipr][ig] at offset sum_{js<is} npr_js ngw, dtwnl[is] = [ipr][ij][ig] at
-// six times that offset
+// six times that offset.  twnl = y(u) v_l(|G|), u = G/|G|; the strain
+// derivative is dtwnl_ij = y dv |G| u_i u_j - v sum_a dy/du_a (u_a u_i u_j -
+// delta_aj u_i), ij = xx yy zz xy yz xz, which is the reference's
+// expressions for l = 0, 1, 2 (:283-306, :426-512, :627-760) written once.
 static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<double>& dtwnl) {
+  static const int IJ[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
   const int ngw = e->wb.size;
   twnl.clear();
   dtwnl.clear();
-  const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
   for (int is = 0; is < e->nsp; is++) {
     const host::Species& q = e->sp[is];
     const int npr = q.nlm;
@@ -187,42 +210,25 @@ static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<doub
     twnl.resize(toff + (size_t)npr * ngw);
     dtwnl.resize(doff + (size_t)npr * 6 * ngw);
     double* t = &twnl[toff];
-    for (int ig = 0; ig < ngw; ig++) {
-      double v, dv;
-      q.dvnlg(0, e->wb.g[ig], v, dv);
-      t[ig] = s14pi * v;
-      if (q.lmax >= 1) {
-        q.dvnlg(1, e->wb.g[ig], v, dv);
-        const double gi = e->wb.gi[ig];
-        for (int j = 0; j < 3; j++) t[(size_t)(1 + j) * ngw + ig] = s34pi * e->wb.gx[j * ngw + ig] * gi * v;
-      }
-    }
-    // dtwnl (src/NonLocalPotential.cpp:283-306 for l = 0, :426-512 for l = 1)
     double* d = &dtwnl[doff];
     for (int ig = 0; ig < ngw; ig++) {
-      double v, dv;
       const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
-      const double x = e->wb.gx[ig], y = e->wb.gx[ngw + ig], z = e->wb.gx[2 * ngw + ig];
-      const double gg[6] = {x * x, y * y, z * z, x * y, y * z, x * z};
-      q.dvnlg(0, tg, v, dv);
-      const double t0 = tgi * s14pi * dv;
-      for (int ij = 0; ij < 6; ij++) d[(size_t)ij * ngw + ig] = t0 * gg[ij];
-      if (q.lmax >= 1) {
-        q.dvnlg(1, tg, v, dv);
-        const double tgi2 = tgi * tgi;
-        const double yv[3] = {s34pi * x * tgi, s34pi * y * tgi, s34pi * z * tgi};
-        for (int m = 0; m < 3; m++) {
-          const double fac = -yv[m] * (v - tg * dv) * tgi2;
-          double* dm = d + (size_t)6 * (1 + m) * ngw;
-          for (int ij = 0; ij < 6; ij++) dm[(size_t)ij * ngw + ig] = fac * gg[ij];
+      const double u[3] = {e->wb.gx[ig] * tgi, e->wb.gx[ngw + ig] * tgi,
+                           e->wb.gx[2 * ngw + ig] * tgi};
+      double v[3], dv[3];
+      for (int l = 0; l <= q.lmax; l++) q.dvnlg(l, tg, v[l], dv[l]);
+      for (int ipr = 0; ipr < npr; ipr++) {
+        const int l = proj_l(ipr);
+        double y, dy[3];
+        harmonic(ipr, u, y, dy);
+        t[(size_t)ipr * ngw + ig] = y * v[l];
+        for (int ij = 0; ij < 6; ij++) {
+          const int a = IJ[ij][0], b = IJ[ij][1];
+          double sum = 0.0;
+          for (int c = 0; c < 3; c++)
+            if (dy[c] != 0.0) sum += dy[c] * (u[c] * u[a] * u[b] - (c == b ? u[a] : 0.0));
+          d[((size_t)ipr * 6 + ij) * ngw + ig] = y * dv[l] * tg * u[a] * u[b] - v[l] * sum;
         }
-        // the v*y terms of :484-511, as the reference writes them
-        d[(size_t)6 * 1 * ngw + 0 * ngw + ig] += v * yv[0];   // m=x: xx
-        d[(size_t)6 * 2 * ngw + 1 * ngw + ig] += v * yv[1];   // m=y: yy
-        d[(size_t)6 * 2 * ngw + 3 * ngw + ig] += v * yv[0];   // m=y: xy
-        d[(size_t)6 * 3 * ngw + 2 * ngw + ig] += v * yv[2];   // m=z: zz
-        d[(size_t)6 * 3 * ngw + 4 * ngw + ig] += v * yv[1];   // m=z: yz
-        d[(size_t)6 * 3 * ngw + 5 * ngw + ig] += v * yv[0];   // m=z: xz
       }
     }
   }
@@ -277,7 +283,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
                               q.dl, q.lmax);
       } else {
         ok = d.type == QBG_SPECIES_SLPP && d.vloc && d.proj && d.dl && d.lmax >= 0 &&
-             d.lmax <= 1 && d.nmesh > 0;
+             d.lmax <= 2 && d.nmesh > 0;
         if (ok) {
           q.lmax = d.lmax;
           q.vloc.assign(d.vloc, d.vloc + q.nmesh);
@@ -287,7 +293,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       }
       if (!ok || !q.init()) {
         delete e;
-        return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 1?)");
+        return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 2?)");
       }
       e->sp.push_back(q);
       e->na.push_back(na[is]);
@@ -345,8 +351,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       std::vector<int> lp(npr);
       sp_npr[is] = npr;
       fac2[is] = 2.0 * 0.25 * q.rcps * q.rcps;
-      lp[0] = 0;
-      for (int j = 1; j < npr; j++) lp[j] = 1;
+      for (int j = 0; j < npr; j++) lp[j] = proj_l(j);
       // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
       const int dbase = (int)dmat.size();
       for (int i = 0; i < npr; i++)

@@ -504,7 +504,7 @@ void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int kra
 
 // ------------------------------------------------------- projectors anl
 // src/NonLocalPotential.cpp:1314-1395: anl = twnl (-i)^l e^{-iG.tau} for
-// l = 0, 1.  One thread per (G, atom) forms the phase once and writes the
+// l = 0, 1, 2.  One thread per (G, atom) forms the phase once and writes the
 // atom's projector columns m = a_m0[a] + ipr*a_stride[a]:
 // A[m*lda + 2*ig + {0,1}].
 constexpr int ANL_ATOMS = 8;
@@ -536,9 +536,12 @@ __global__ __launch_bounds__(256) void k_anl(int ngw, int nat, long lda, const d
     for (int ipr = 0; ipr < npr; ipr++) {
       const int m = m0 + ipr * st;
       const double t = twnl[(long)m_twnl[m] + ig];
+      // (-i)^l (c + i s): l = 0 (c, s), 1 (s, -c), 2 (-c, -s)
       cpx2<T> v;
-      if (m_l[m] == 0) { v.x = (T)(t * c); v.y = (T)(t * s); }
-      else { v.x = (T)(t * s); v.y = (T)(-t * c); }
+      const int l = m_l[m];
+      if (l == 0) { v.x = (T)(t * c); v.y = (T)(t * s); }
+      else if (l == 1) { v.x = (T)(t * s); v.y = (T)(-t * c); }
+      else { v.x = (T)(-t * c); v.y = (T)(-t * s); }
       reinterpret_cast<cpx2<T>*>(A + (long)m * lda)[ig] = v;
     }
   }
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
     const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
-    double Z[4] = {0, 0, 0, 0};
+    double Z[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int a = a0; a < a1; a++) {
       double sn, cs;
       sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
@@ -768,8 +771,10 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
       for (int ipr = 0; ipr < npr; ipr++) {
         const T* yp = Y + (mb + ipr * st) * lda + 2 * ig;
         const double yr = (double)yp[0], yi = (double)yp[1];
-        // l = 0: phase (c, s); l = 1: (-i)(c + i s) = (s, -c)
-        Z[ipr] += ipr == 0 ? cs * yr + sn * yi : sn * yr - cs * yi;
+        // Re(conj(phase) Y) for the phase (-i)^l (c + i s) of projector ipr:
+        // l = 0 (c, s), l = 1 (ipr 1-3) (s, -c), l = 2 (ipr 4-8) (-c, -s)
+        Z[ipr] += ipr == 0 ? cs * yr + sn * yi
+                           : (ipr < 4 ? sn * yr - cs * yi : -(cs * yr + sn * yi));
       }
     }
     for (int ipr = 0; ipr < npr; ipr++)

@@ -130,6 +130,7 @@ int qbg_finalize(qbg_ctx* c) {
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return QBG_OK;
@@ -142,6 +143,29 @@ int qbg_sync(qbg_ctx* c) {
     if (!c) return QBG_E_INVAL;
     if (c->capturing) return c->fail(QBG_E_STATE, "sync during graph capture");
     QBG_HIP(c, hipStreamSynchronize(c->stream));
+    return QBG_OK;
+  });
+}
+
+int qbg_stream_join(qbg_ctx* before, qbg_ctx* after) {
+  return qbg::guarded(after, [&]() -> int {
+    if (!before || !after || before == after) return QBG_E_INVAL;
+    if (before->device != after->device)
+      return after->fail(QBG_E_INVAL, "stream_join: contexts on different devices");
+    QBG_HIP(after, hipSetDevice(after->device));
+    if (!before->join_ev)
+      QBG_HIP(after, hipEventCreateWithFlags(&before->join_ev, hipEventDisableTiming));
+    QBG_HIP(after, hipEventRecord(before->join_ev, before->stream));
+    QBG_HIP(after, hipStreamWaitEvent(after->stream, before->join_ev, 0));
+    if (before->capturing && !after->capturing) {
+      // after's stream is now part of before's capture
+      after->capturing = true;
+      after->capture_origin = before;
+    } else if (after->capturing && before->capture_origin == after) {
+      // a branch joined back into the capturing stream
+      before->capturing = false;
+      before->capture_origin = nullptr;
+    }
     return QBG_OK;
   });
 }
@@ -162,7 +186,8 @@ int qbg_graph_end(qbg_ctx* c, qbg_graph** out) {
   return qbg::guarded(c, [&]() -> int {
     if (!c || !out) return QBG_E_INVAL;
     *out = nullptr;
-    if (!c->capturing) return c->fail(QBG_E_STATE, "no graph capture open");
+    if (!c->capturing || c->capture_origin)
+      return c->fail(QBG_E_STATE, "no graph capture open on this context");
     c->capturing = false;
     hipGraph_t gr = nullptr;
     QBG_HIP(c, hipStreamEndCapture(c->stream, &gr));

@@ -35,6 +35,9 @@ struct qbg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool capturing = false;           // between qbg_graph_begin and qbg_graph_end
+  // the context whose capture this one's stream joined (qbg_stream_join)
+  qbg_ctx* capture_origin = nullptr;
+  hipEvent_t join_ev = nullptr;     // recorded by qbg_stream_join(this, ...)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   // per-kernel event profiler

@@ -270,7 +270,7 @@ bool ncpp_to_kb(int lmax, int llocal, int nmesh, double deltar, const double* vp
   lmax_kb = 0;
   for (int l = 0; l <= lmax; l++)
     if (l != llocal) {
-      if (l > 1) return false;
+      if (l > 2) return false;
       lmax_kb = std::max(lmax_kb, l);
     }
   vloc.assign(vl, vl + nmesh);
@@ -297,7 +297,7 @@ bool ncpp_to_kb(int lmax, int llocal, int nmesh, double deltar, const double* vp
 }
 
 bool Species::init() {
-  if (lmax < 0 || lmax > 1 || nmesh < 8 || !(deltar > 0.0)) return false;
+  if (lmax < 0 || lmax > 2 || nmesh < 8 || !(deltar > 0.0)) return false;
   const double fpi = 4.0 * M_PI;
   nlm = (lmax + 1) * (lmax + 1);
   ndft = 1;
@@ -335,7 +335,7 @@ bool Species::init() {
       v[0] = w0;
       for (int i = 1; i < ndft; i++) v[i] /= gspl[i];
       spline(ndft, gspl.data(), v.data(), false, vnlg2[l].data());
-    } else {        // (1/G^2) sum sin(Gr) w - (1/G) sum cos(Gr) r w
+    } else if (l == 1) {   // (1/G^2) sum sin(Gr) w - (1/G) sum cos(Gr) r w
       for (int i = 0; i < ndft; i++) v[i] = w[i];
       sinft(ndft, v.data());
       fint[0] = 0.0;
@@ -346,6 +346,24 @@ bool Species::init() {
       v[0] = 0.0;
       for (int i = 1; i < ndft; i++) v[i] = fint[i] - v[i] / gspl[i];
       spline(ndft, gspl.data(), v.data(), true, vnlg2[l].data());
+    } else {   // j_2: (3/G^3) sum sin(Gr) w/r - (1/G) sum sin(Gr) r w - (3/G^2) sum cos(Gr) w
+               // (src/Species.cpp:872-940); zero at G = 0
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = w[i] / r[i];
+      sinft(ndft, v.data());
+      fint[0] = 0.0;
+      for (int i = 1; i < ndft; i++) fint[i] = 3.0 * v[i] / (gspl[i] * gspl[i] * gspl[i]);
+      for (int i = 0; i < ndft; i++) v[i] = w[i] * r[i];
+      sinft(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] -= v[i] / gspl[i];
+      for (int i = 0; i < ndft; i++) v[i] = w[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] -= 3.0 * v[i] / (gspl[i] * gspl[i]);
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = fint[i];
+      // zero slope at G = 0, natural at the end
+      spline(ndft, gspl.data(), v.data(), false, vnlg2[l].data());
     }
   }
   return true;

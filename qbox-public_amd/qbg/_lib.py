@@ -91,6 +91,7 @@ SIGNATURES = {
     "qbg_mixer_restart": [_vp],
     "qbg_mixer_update": [_vp],
     "qbg_polar": [_vp, _vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)],
+    "qbg_stream_join": [_vp, _vp],
     "qbg_graph_begin": [_vp],
     "qbg_graph_end": [_vp, C.POINTER(_vp)],
     "qbg_graph_launch": [_vp],
@@ -401,6 +402,13 @@ class Profiler:
             if n.value:
                 out[name] = (ms.value, n.value)
         return out
+
+
+def stream_join(before: Context, after: Context):
+    """qbg_stream_join: work on `after` waits for the work enqueued on
+    `before`; inside a capture on `before`, `after` becomes a branch of the
+    graph (join it back before the capture ends)."""
+    after.check(after.lib.qbg_stream_join(before.h, after.h))
 
 
 def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):

@@ -56,6 +56,8 @@ class SpeciesDef:
 
 SPECIES_A = SpeciesDef(zval=4, rc=1.0, sigma=0.8)
 SPECIES_B = SpeciesDef(zval=6, rc=0.7, sigma=0.6)
+# with a d projector r^2 exp(-r^2 / (2 sigma^2)), D_2 = 0.7 Ha
+SPECIES_D = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7))
 
 
 @dataclass
@@ -153,6 +155,11 @@ def config(index: int) -> SystemDef:
     elif index == 11:
         s = SystemDef("t1: 2+1 atoms 7x8x9 ecut=16Ry 5 bands", np.diag([7.0, 8.0, 9.0]),
                       16.0, [SPECIES_A, SPECIES_B], [2, 1], 5, BASE_SEED + 11, 2.0)
+    elif index == 13:
+        # d projectors (l = 2) beside an s/p species, orthorhombic cell
+        s = SystemDef("t3: 2+1 atoms with d projectors 8x8.5x9 ecut=18Ry 6 bands",
+                      np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_D, SPECIES_B], [2, 1], 6,
+                      BASE_SEED + 13, 2.0)
     elif index == 12:
         # non-orthogonal (monoclinic) cell
         a = np.array([[9.0, 0.0, 0.0], [1.5, 8.5, 0.0], [0.0, 0.7, 8.0]])

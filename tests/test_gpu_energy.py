@@ -27,7 +27,7 @@ def relerr(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.fixture(scope="module", params=[10, 11, 12, 0, 1])
+@pytest.fixture(scope="module", params=[10, 11, 12, 13, 0, 1])
 def system(request, ctx, oracle_mod):
     import qbg
     idx = request.param

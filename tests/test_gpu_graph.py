@@ -91,3 +91,47 @@ def test_capture_refuses_sync(ctx):
             ef.evaluate_async(dc, None, forces=False)
             ef.results()              # waits on the stream: refused while capturing
     assert e.value.code == -5
+
+
+def test_parallel_branches_equal_eager():
+    """Several systems, one context (stream) each, captured as parallel
+    branches of one graph (qbg_stream_join): every replay gives each system's
+    eager results bit for bit, also after other work ran in between."""
+    import qbg
+    main = qbg.Context(0)
+    parts = []
+    for idx in (10, 11, 12, 0):
+        s = cf.config(idx)
+        ctx = qbg.Context(0)
+        ef = qbg.EnergyFunctional(ctx, s)
+        c = cf.gram_numpy(cf.random_coefficients(600 + idx, ef.basis(0)["g2"], s.nst))
+        dc = ctx.to_device(c)
+        hp = ctx.array(c.shape, np.complex128)
+        prec = "bf16x2" if idx == 0 else "fp64"
+        ref = ef.evaluate(dc, hp, forces=True, prec=prec)
+        parts.append((ctx, ef, dc, hp, prec, ref, hp.download()))
+    print("[case] parallel graph branches", flush=True)
+    with qbg.Graph(main) as g:
+        for ctx, ef, dc, hp, prec, _, _ in parts:
+            qbg.stream_join(main, ctx)
+            ef.evaluate_async(dc, hp, forces=True, prec=prec)
+        for ctx, *_ in parts:
+            qbg.stream_join(ctx, main)
+    for rep in range(3):
+        for ctx, _, _, hp, *_ in parts:
+            hp.zero()
+            ctx.sync()             # the graph runs on main's stream
+        g.launch()
+        main.sync()
+        for ctx, ef, dc, hp, prec, ref, href in parts:
+            got = ef.results()
+            assert got["etotal"] == ref["etotal"], rep
+            assert np.array_equal(got["fion"], ref["fion"])
+            assert np.array_equal(hp.download(), href)
+        # eager work on one branch's context between replays
+        parts[1][1].evaluate(parts[1][2], parts[1][3], forces=True)
+    g.release()
+    for ctx, ef, *_ in parts:
+        ef.release()
+        ctx.close()
+    main.close()

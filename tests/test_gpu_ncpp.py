@@ -3,7 +3,8 @@ device: the systems of test/si4gs, test/h2ogs and test/o2gs (nspin 2)
 evaluated against the FP64
 oracle (energies, H*psi, forces to 1e-10; the mixed modes to the 1e-6
 Ha/atom bar), the si4gs SCF run (PSDA, ecutprec 2 Ry, 200 iterations)
-reaching the oracle's minimum, and species the library does not take."""
+reaching the oracle's minimum, Si with nonlocal p and d channels, and
+species the library does not take (nonlocal f)."""
 import dataclasses
 from pathlib import Path
 
@@ -92,14 +93,44 @@ def test_si4gs_scf(ctx, oracle_mod):
     ef.release()
 
 
-def test_nonlocal_d_channel_refused(ctx):
-    """A d projector (nonlocal l = 2) is outside what the library builds."""
+@pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
+def test_d_channel_evaluate(ctx, oracle_mod, prec):
+    """Si with nonlocal p and d channels (a synthetic d pseudo-wavefunction,
+    tests/test_oracle_ncpp.py) in the si4gs cell against the oracle: the
+    l = 2 projector tables, the (-i)^2 phase and the d strain derivatives."""
+    import qbg
+    from test_oracle_ncpp import si_with_d
+    s = cf.si4gs(si_with_d(read_species(GOLD / "Si_VBC_LDA-1.0.xml")))
+    print(f"[case] si4gs with d projectors {prec}", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    assert ef.nproj == 4 * 9
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(13, o.basis(0)["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        if prec == "fp64":
+            assert abs(got[t] - ref[t]) <= 1e-10 * max(abs(ref[t]), 1e-12), (t, got[t], ref[t])
+        else:
+            assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    ftol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.abs(got["fion"] - ref["fion"]).max() <= ftol * max(1.0, np.abs(ref["fion"]).max())
+    assert relerr(hp.download(), ref["hpsi"]) < ftol
+    assert np.abs(got["sigma"] - ref["sigma"]).max() < ftol * np.abs(ref["sigma"][0]).max()
+    ef.release()
+
+
+def test_f_channel_refused(ctx):
+    """A nonlocal f channel (l = 3) is outside what the library builds."""
     import qbg
     si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
-    bad = dataclasses.replace(si, llocal=0)
-    s = cf.si4gs(bad)
+    vps = np.vstack([si.vps, si.vps[2:3]])
+    phi = np.vstack([si.phi, si.phi[1:2]])
+    bad = dataclasses.replace(si, lmax=3, llocal=0, vps=vps, phi=phi)
     with pytest.raises(qbg.QbgError) as e:
-        qbg.EnergyFunctional(ctx, s)
+        qbg.EnergyFunctional(ctx, cf.si4gs(bad))
     assert e.value.code == -1
 
 

@@ -73,19 +73,62 @@ def test_kb_weights_and_identity(oracle_mod, name, request):
         assert abs(dl[l] * beta_phi ** 2 - e) < 1e-6 * abs(e)
 
 
-def test_projector_transform_by_quadrature(oracle_mod, si):
-    """The oracle's projector tables w_l(G) (sinft/cosft route) against direct
-    quadrature of 4 pi int r^2 j_l(Gr) f_l(r) dr."""
-    s = cf.si4gs(si)
-    o = oracle_mod.Oracle(s)
-    proj, dl = oracle_mod.ncpp_kb(si)
+def si_with_d(si):
+    """Si_VBC_LDA with llocal = 0, so the p and d channels are nonlocal; the
+    file holds no d pseudo-wavefunction, so a normalized r^2 exp(-r^2/2.88)
+    stands in for it (a synthetic d channel for the l = 2 code paths)."""
+    import dataclasses
     r = np.arange(si.nmesh) * si.deltar
+    phi2 = r * r * np.exp(-r * r / 2.88)
+    phi2 /= np.sqrt(simpson(phi2 ** 2 * r * r, x=r))
+    phi = np.array(si.phi, dtype=float).copy()
+    phi[2] = phi2
+    return dataclasses.replace(si, llocal=0, phi=phi)
+
+
+@pytest.mark.parametrize("with_d", [False, True])
+def test_projector_transform_by_quadrature(oracle_mod, si, with_d):
+    """The oracle's projector tables w_l(G) (sinft/cosft route) against direct
+    quadrature of 4 pi int r^2 j_l(Gr) f_l(r) dr, for Si as given (s and p
+    nonlocal) and with nonlocal p and d channels."""
+    sp = si_with_d(si) if with_d else si
+    llocal = sp.llocal
+    s = cf.si4gs(sp)
+    o = oracle_mod.Oracle(s)
+    proj, dl = oracle_mod.ncpp_kb(sp)
+    r = np.arange(sp.nmesh) * sp.deltar
     g = np.array([0.3, 1.1, 2.7, 4.0, 6.5])
     v, dv, vn, dvn = o.species_eval(0, g)
-    for l in (0, 1):
+    for l in range(proj.shape[0]):
+        if l == llocal:
+            assert not vn[l].any()
+            continue
         direct = np.array([4 * np.pi * simpson(r * r * spherical_jn(l, gg * r) * proj[l], x=r)
                            for gg in g])
         assert np.allclose(vn[l], direct, rtol=2e-5, atol=1e-6), (l, vn[l], direct)
+
+
+def test_si_d_channel_system(oracle_mod, si):
+    """Si with nonlocal p and d channels in the si4gs cell: charge, and
+    forces as the gradient of the energy."""
+    import dataclasses
+    s = cf.si4gs(si_with_d(si))
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(6, o.basis(0)["g2"], s.nst))
+    r = o.evaluate(c, hpsi=False, forces=True)
+    assert abs(r["total_charge"] - 2 * s.nst) < 1e-8
+    assert abs(r["enl"]) > 1e-3
+    h = 1e-4
+    t = np.array(s.tau, dtype=float)
+    for a, j in ((0, 0), (1, 1)):
+        e = []
+        for sgn in (1.0, -1.0):
+            tt = t.copy()
+            tt[a, j] += sgn * h
+            e.append(oracle_mod.Oracle(dataclasses.replace(s, tau=tt)).evaluate(
+                c, hpsi=False, forces=False)["etotal"])
+        fd = -(e[0] - e[1]) / (2 * h)
+        assert abs(fd - r["fion"][a, j]) <= 1e-6 * max(1.0, abs(fd)), (a, j, fd)
 
 
 def test_si4_and_h2o_charge(oracle_mod, si, oxy, hyd):

@@ -6,9 +6,12 @@ independent of how the oracle computes them.
   -4 pi Z (exp(-G^2 rc^2/4) - exp(-G^2 rcps^2/4)) / G^2, with the value
   pi Z (rc^2 - rcps^2) at G = 0; checked with its derivative at spline
   accuracy.
-- Projectors: 4 pi int r^2 j_l(Gr) p_l(r) dr (src/Species.cpp:782-855) of the
-  Gaussian projectors p_0 = exp(-r^2/(2 s^2)) and p_1 = r exp(-r^2/(2 s^2))
-  are (2 pi)^{3/2} s^3 exp(-G^2 s^2/2) and (2 pi)^{3/2} s^5 G exp(-G^2 s^2/2).
+- Projectors: 4 pi int r^2 j_l(Gr) p_l(r) dr (src/Species.cpp:782-940) of the
+  Gaussian projectors p_l = r^l exp(-r^2/(2 s^2)) are
+  (2 pi)^{3/2} s^{3+2l} G^l exp(-G^2 s^2/2) (l = 0, 1, 2).
+- d harmonics: the five real l = 2 harmonics of twnl satisfy the addition
+  theorem sum_m y_m(u) y_m(u') = (5 / 4 pi) P_2(u.u')
+  (src/NonLocalPotential.cpp:627-760).
 - Nonlocal energy: one atom and one Gaussian orbital psi (s and p type) in a
   large cell; enl = occ D_l <beta_lm|psi>^2 with beta_lm = p_l(r) Y_lm and the
   overlap integral in closed form (src/NonLocalPotential.cpp:1469-1571).
@@ -73,6 +76,43 @@ def test_species_transforms_closed_form(oracle_mod, sp):
     assert _rel(dv, cdv) < 1e-5 and _rel(dvn[0], dp0) < 1e-5 and _rel(dvn[1], dp1) < 1e-5
 
 
+def test_d_projector_transform_closed_form(oracle_mod):
+    sp = cf.SPECIES_D
+    o = _species_oracle(oracle_mod, sp)
+    dg = math.pi / (o.ndft * sp.deltar)
+    s = sp.sigma
+    for g, tol in ((np.arange(1, int(12.0 / dg)) * dg, 1e-10),
+                   (np.arange(1, int(12.0 / dg)) * dg + 0.5 * dg, 1e-6)):
+        vn = o.species_eval(0, g, nop=3)[2]
+        e = np.exp(-g * g * s * s / 2)
+        assert _rel(vn[2], (2 * math.pi) ** 1.5 * s ** 7 * g * g * e) < tol
+        assert _rel(vn[0], (2 * math.pi) ** 1.5 * s ** 3 * e) < tol
+    assert o.species_eval(0, np.zeros(1), nop=3)[2][2][0] == 0.0
+
+
+def test_d_harmonics_addition_theorem(oracle_mod):
+    s = dataclasses.replace(cf.config(13), na=[1, 0], species=[cf.SPECIES_D, cf.SPECIES_B],
+                            tau=np.zeros((1, 3)))
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    ngw = o.ngw
+    tw = o.get(9, 0).reshape(9, ngw)
+    g = np.sqrt(b["g2"])
+    v = o.species_eval(0, g, nop=3)[2][2]
+    rng = np.random.default_rng(5)
+    i = rng.integers(1, ngw, 400)
+    j = rng.integers(1, ngw, 400)
+    u = (b["gx"] / np.where(g > 0, g, 1.0)).T
+    cos = np.einsum("ij,ij->i", u[i], u[j])
+    lhs = np.einsum("mi,mi->i", tw[4:9][:, i], tw[4:9][:, j])
+    rhs = v[i] * v[j] * 5 / (4 * math.pi) * 0.5 * (3 * cos * cos - 1)
+    assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(v).max() ** 2
+    # and the p block: sum_m y_m y_m' = (3 / 4 pi) u.u'
+    vp = o.species_eval(0, g, nop=3)[2][1]
+    lhs = np.einsum("mi,mi->i", tw[1:4][:, i], tw[1:4][:, j])
+    assert np.abs(lhs - vp[i] * vp[j] * 3 / (4 * math.pi) * cos).max() < 1e-12 * np.abs(vp).max() ** 2
+
+
 def _gaussian_orbital(b, omega, a, kind):
     """Plane-wave coefficients (half sphere) of a normalized Gaussian orbital
     of width a at the origin: s = exp(-r^2/2a^2), p = x exp(-r^2/2a^2);
@@ -83,14 +123,19 @@ def _gaussian_orbital(b, omega, a, kind):
     if kind == "s":
         n = (math.pi * a * a) ** -0.75
         c = n * fg + 0j
-    else:
+    elif kind == "p":
         n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
         c = n * (-1j * a * a * gx) * fg
+    else:                                   # d: x y exp(-r^2/2a^2)
+        gy = b["gx"][1]
+        n = ((a * a / 2) ** 2 * (math.pi * a * a) ** 1.5) ** -0.5
+        c = n * (-(a ** 4) * gx * gy) * fg + 0j
     return (c / math.sqrt(omega))[None, :]
 
 
-@pytest.mark.parametrize("kind", ["s", "p"])
-@pytest.mark.parametrize("sp", [cf.SPECIES_A, cf.SPECIES_B])
+@pytest.mark.parametrize("sp,kind", [(cf.SPECIES_A, "s"), (cf.SPECIES_A, "p"),
+                                     (cf.SPECIES_B, "s"), (cf.SPECIES_B, "p"),
+                                     (cf.SPECIES_D, "s"), (cf.SPECIES_D, "d")])
 def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
     L, a = 22.0, 1.1
     s = cf.SystemDef("one atom, one orbital", cf.cubic(L), 50.0, [sp], [1], 1, 1, 0.0,
@@ -108,10 +153,14 @@ def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
     if kind == "s":
         ov = (4 * math.pi) ** -0.5 * (math.pi * a * a) ** -0.75 * (2 * math.pi * s2) ** 1.5
         ref = 2.0 * sp.d[0] * ov ** 2
-    else:
+    elif kind == "p":
         n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
         ov = math.sqrt(3 / (4 * math.pi)) * n * s2 * (2 * math.pi * s2) ** 1.5
         ref = 2.0 * sp.d[1] * ov ** 2
+    else:   # beta = r^2 exp(-r^2/2s^2) sqrt(15/4pi) x y / r^2
+        n = ((a * a / 2) ** 2 * (math.pi * a * a) ** 1.5) ** -0.5
+        ov = math.sqrt(15 / (4 * math.pi)) * n * s2 ** 2 * (2 * math.pi * s2) ** 1.5
+        ref = 2.0 * sp.d[2] * ov ** 2
     assert abs(enl - ref) < 1e-6 * abs(ref), (enl, ref)   # spline accuracy of twnl
 
 
