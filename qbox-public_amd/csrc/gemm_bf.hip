@@ -435,6 +435,207 @@ __global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
     }
 }
 
+// 256 x 256 tiles with the 2-part split and two LDS stages (160 KB): the
+// loop computes step i from one stage while the split parts of step i+1,
+// loaded into registers during step i-1, go into the other stage and the
+// loads of step i+2 are issued; one barrier per step.  Q along q (NN) is
+// loaded as pairs of adjacent q (one 8-byte load per lane and k row).
+// Otherwise as k_gemm_bfb (same fragments, MFMA order, epilogues).
+template <bool QR, typename TO, bool FE = false>
+__global__ __launch_bounds__(512) void k_gemm_bfd(BfArgs g) {
+  __shared__ __align__(16) unsigned short sm[2][2][2][BB][LR];   // [stage][P/Q][part][row][k]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int b = xcd_block(g.nblk);
+  if (b < 0) return;
+  const int ntile = g.tp * g.tq;
+  const int split = b / ntile, tt = b - split * ntile;
+  int ip, iq;
+  if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
+  else { iq = tt % g.tq; ip = tt / g.tq; }
+  const long p0 = (long)ip * BB, q0 = (long)iq * BB;
+  const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
+  const long nsteps = (re - rb) / BK;
+  // P (and Q when r-contiguous): row srow, 16 k at sk
+  const int srow = (tid & 15) | ((tid >> 5) << 4), sk = ((tid >> 4) & 1) * 16;
+  // Q along q: rows qq, qq + 1 (adjacent q), 8 k at qk
+  const int qq = 2 * (tid & 127), qk = (tid >> 7) * 8;
+  const float* Pp = g.P + (p0 + srow) * g.ldp + sk;
+  const float* Qp = QR ? g.Q + (q0 + srow) * g.ldq + sk : g.Q + q0 + qq + (long)qk * g.ldq;
+  float pv[16], qv[16];
+  auto load = [&](long r0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float4 t = *reinterpret_cast<const float4*>(Pp + r0 + 4 * i);
+      pv[4 * i] = t.x; pv[4 * i + 1] = t.y; pv[4 * i + 2] = t.z; pv[4 * i + 3] = t.w;
+    }
+    if (QR) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float4 t = *reinterpret_cast<const float4*>(Qp + r0 + 4 * i);
+        qv[4 * i] = t.x; qv[4 * i + 1] = t.y; qv[4 * i + 2] = t.z; qv[4 * i + 3] = t.w;
+      }
+    } else {
+      // qv[i] = Q(r0 + qk + i, qq), qv[8 + i] = Q(r0 + qk + i, qq + 1)
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float2 t = *reinterpret_cast<const float2*>(Qp + (r0 + i) * g.ldq);
+        qv[i] = t.x; qv[8 + i] = t.y;
+      }
+    }
+  };
+  auto store = [&](int st) {
+    u16x8 h[2];
+    split8<2>(pv, h);
+    *reinterpret_cast<u16x8*>(&sm[st][0][0][srow][sk]) = h[0];
+    *reinterpret_cast<u16x8*>(&sm[st][0][1][srow][sk]) = h[1];
+    split8<2>(pv + 8, h);
+    *reinterpret_cast<u16x8*>(&sm[st][0][0][srow][sk + 8]) = h[0];
+    *reinterpret_cast<u16x8*>(&sm[st][0][1][srow][sk + 8]) = h[1];
+    if (QR) {
+      split8<2>(qv, h);
+      *reinterpret_cast<u16x8*>(&sm[st][1][0][srow][sk]) = h[0];
+      *reinterpret_cast<u16x8*>(&sm[st][1][1][srow][sk]) = h[1];
+      split8<2>(qv + 8, h);
+      *reinterpret_cast<u16x8*>(&sm[st][1][0][srow][sk + 8]) = h[0];
+      *reinterpret_cast<u16x8*>(&sm[st][1][1][srow][sk + 8]) = h[1];
+    } else {
+      split8<2>(qv, h);
+      *reinterpret_cast<u16x8*>(&sm[st][1][0][qq][qk]) = h[0];
+      *reinterpret_cast<u16x8*>(&sm[st][1][1][qq][qk]) = h[1];
+      split8<2>(qv + 8, h);
+      *reinterpret_cast<u16x8*>(&sm[st][1][0][qq + 1][qk]) = h[0];
+      *reinterpret_cast<u16x8*>(&sm[st][1][1][qq + 1][qk]) = h[1];
+    }
+  };
+  const int wp = (w & 1) * 128, wq = (w >> 1) * 64;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  v4f acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  if (nsteps > 0) {
+    load(rb);
+    store(0);
+    if (nsteps > 1) load(rb + BK);
+  }
+  __syncthreads();
+  for (long it = 0; it < nsteps; it++) {
+    const int cur = (int)(it & 1);
+    bf16x8 bq[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        bq[s][j] = *reinterpret_cast<const bf16x8*>(&sm[cur][1][s][wq + 16 * j + fr][fk]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      bf16x8 a[2];
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+        a[s] = *reinterpret_cast<const bf16x8*>(&sm[cur][0][s][wp + 16 * i + fr][fk]);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v4f c = acc[i][j];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[1][j], a[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[0][j], a[0], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    }
+    if (it + 1 < nsteps) {
+      store(cur ^ 1);
+      if (it + 2 < nsteps) load(rb + (it + 2) * BK);
+    }
+    __syncthreads();
+  }
+  if constexpr (FE) {
+    const int fq = 4 * (lane >> 4);
+    const int wg = w >> 1;
+    double* red = reinterpret_cast<double*>(&sm[0][0][0][0][0]);   // [4][256][3]
+    double* gxs = red + 4 * BB * 3;
+    for (int t = tid; t < 3 * (BB / 2); t += 512) {
+      const int c = t / (BB / 2), k = t - c * (BB / 2);
+      const long ig = (q0 >> 1) + k;
+      gxs[t] = ig < g.ngw ? g.gx[(long)c * g.ngw + ig] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const long p = p0 + wp + 16 * i + fr;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const long q = q0 + wq + 16 * j + fq;
+        const int k = (int)((q - q0) >> 1);
+        const float4 a = *reinterpret_cast<const float4*>(g.fa + q + p * g.lda_fa);
+        const double t0 = (double)a.y * acc[i][j][0] - (double)a.x * acc[i][j][1];
+        const double t1 = (double)a.w * acc[i][j][2] - (double)a.z * acc[i][j][3];
+        v0 += gxs[k] * t0 + gxs[k + 1] * t1;
+        v1 += gxs[BB / 2 + k] * t0 + gxs[BB / 2 + k + 1] * t1;
+        v2 += gxs[BB + k] * t0 + gxs[BB + k + 1] * t1;
+      }
+      v0 += __shfl_xor(v0, 16); v0 += __shfl_xor(v0, 32);
+      v1 += __shfl_xor(v1, 16); v1 += __shfl_xor(v1, 32);
+      v2 += __shfl_xor(v2, 16); v2 += __shfl_xor(v2, 32);
+      if (lane < 16) {
+        double* r = red + (wg * BB + wp + 16 * i + lane) * 3;
+        r[0] = v0; r[1] = v1; r[2] = v2;
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < BB * 3; t += 512) {
+      const int pl = t / 3, c = t - 3 * pl;
+      const long p = p0 + pl;
+      if (p < g.np) {
+        const double sum = ((red[(0 * BB + pl) * 3 + c] + red[(1 * BB + pl) * 3 + c]) +
+                            red[(2 * BB + pl) * 3 + c]) + red[(3 * BB + pl) * 3 + c];
+        g.fpart[(p * 3 + c) * g.ntq + iq] = sum;
+      }
+    }
+    return;
+  }
+  TO* o = (TO*)g.out + (long)split * g.sstride;
+  const int fq = 4 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const long p = p0 + wp + 16 * i + fr;
+      const long q = q0 + wq + 16 * j + fq;
+      if (p >= g.np) continue;
+      TO* d = o + q + p * g.ldo;
+      if (g.vec && q + 3 < g.nq) {
+        if constexpr (sizeof(TO) == 4) {
+          float4 v = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+          if (g.accumulate) {
+            const float4 u = *reinterpret_cast<const float4*>(d);
+            v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
+          }
+          *reinterpret_cast<float4*>(d) = v;
+        } else {
+          double2 v0 = {(double)acc[i][j][0], (double)acc[i][j][1]};
+          double2 v1 = {(double)acc[i][j][2], (double)acc[i][j][3]};
+          if (g.accumulate) {
+            const double2 u0 = *reinterpret_cast<const double2*>(d);
+            const double2 u1 = *reinterpret_cast<const double2*>(d + 2);
+            v0.x = u0.x + v0.x; v0.y = u0.y + v0.y;
+            v1.x = u1.x + v1.x; v1.y = u1.y + v1.y;
+          }
+          *reinterpret_cast<double2*>(d) = v0;
+          *reinterpret_cast<double2*>(d + 2) = v1;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (q + e < g.nq) {
+            const TO v = (TO)acc[i][j][e];
+            d[e] = g.accumulate ? (TO)(d[e] + v) : v;
+          }
+      }
+    }
+}
+
 // out[q + p*ldo] = alpha sum_s partial[s][q + p*pitch] + gamma Q[q*ldq] P[p*ldp]
 __global__ __launch_bounds__(256) void k_bf_reduce(const float* __restrict__ partial, int S,
                                                    int nq, int np, long pitch, long sstride,
@@ -481,7 +682,7 @@ int tnb_slots() {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       ncu = 256;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, reinterpret_cast<const void*>(&k_gemm_bfb<2, true, float>), 512, 0) != hipSuccess)
+            &occ, reinterpret_cast<const void*>(&k_gemm_bfd<true, float>), 512, 0) != hipSuccess)
       occ = 1;
     cache[dev] = std::max(1, ncu) * std::max(1, occ);
   }
@@ -490,8 +691,9 @@ int tnb_slots() {
 
 template <int NS, bool QR, typename TO>
 void launchb(BfArgs g, int nsplit, hipStream_t s) {
+  static_assert(NS == 2, "256-edge tiles run the 2-part split only");
   const unsigned grid = xcd_grid(g, g.tp * g.tq * nsplit);
-  hipLaunchKernelGGL((k_gemm_bfb<NS, QR, TO>), dim3(grid), dim3(512), 0, s, g);
+  hipLaunchKernelGGL((k_gemm_bfd<QR, TO>), dim3(grid), dim3(512), 0, s, g);
 }
 
 template <int NS, bool QR, typename TO>
@@ -575,7 +777,7 @@ int gemm_bf_nn_forces(const float* A, long lda, const float* F, long ldf, long K
   g.rper = Mp;
   g.fa = anl; g.lda_fa = lda_anl; g.gx = gx; g.ngw = ngw; g.ntq = g.tq; g.fpart = fpart;
   const unsigned grid = xcd_grid(g, g.tp * g.tq);
-  hipLaunchKernelGGL((k_gemm_bfb<2, false, float, true>), dim3(grid), dim3(512), 0, s, g);
+  hipLaunchKernelGGL((k_gemm_bfd<false, float, true>), dim3(grid), dim3(512), 0, s, g);
   return 0;
 }
 

