@@ -150,6 +150,22 @@ int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc,
  * which case c is left as cleaned up but otherwise unchanged. */
 int qbg_gram(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst, int prec);
 
+/* qbg_gram without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as qbg_gram (equal to rounding: the overlap's
+ * split-K partition may differ); the overlap, its inverse factor and the
+ * split-K partials live in one fixed device array per GPU of
+ * qbg_gram_noalloc_bytes() bytes, held until the process ends, and C is
+ * updated in place.  nst above QBG_GRAM_NOALLOC_MAX_NST is refused with
+ * QBG_E_INVAL.  Every call on a GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams or threads) race, so the caller
+ * serializes them. */
+#define QBG_GRAM_NOALLOC_MAX_NST 1024
+#define QBG_GRAM_NOALLOC_SPLITS 6
+int qbg_gram_noalloc(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst, int prec);
+size_t qbg_gram_noalloc_bytes(void);
+
 /* ---------------------------------------------------- EnergyFunctional */
 /* A species as Qbox holds it, on the radial mesh i*deltar (host arrays);
  * rcps is the pseudocharge radius (1.5 in Qbox, src/AtomSet.cpp:43-44).

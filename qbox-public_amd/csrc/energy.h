@@ -91,6 +91,9 @@ int gemm_nn_splitk(const double* A, long lda, const double* F, long ldf, long K,
                    const int* skip = nullptr);
 // NN (FP64) with F upper triangular (F[m + n*ldf] = 0 for m > n), N x N:
 // C[k + n*ldc] = sum_{m <= n} A[k + m*lda] F[m + n*ldf]
+// output columns (p) of one tile of the NN product: a gemm_nn call with N <= this
+// runs one tile per group of rows, which reads its rows before it writes them
+int gemm_nn_tile_p();
 void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
                   long ldc, hipStream_t s);
 // NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf]  (K rows, N cols, M sum)

@@ -304,6 +304,7 @@ int gemm_syrk(const double* A, long lda, int N, long K, double alpha, double gam
   return tn_splitk(A, lda, A, lda, N, N, K, alpha, gamma, out, ldo, work, work_doubles,
                    QBG_PREC_FP64, 1, s, skip);
 }
+int gemm_nn_tile_p() { return TP; }
 void gemm_nn_triu(const double* A, long lda, const double* F, long ldf, long K, int N, double* C,
                   long ldc, hipStream_t s) {
   nn(A, lda, F, ldf, K, N, N, C, ldc, 0, QBG_PREC_FP64, 1, s, nullptr);

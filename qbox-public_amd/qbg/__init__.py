@@ -5,5 +5,5 @@ the C ABI of include/qbg.h); this package is the Python harness over it.
 """
 from . import configs  # noqa: F401
 from ._lib import (Context, DArray, EnergyFunctional, Graph, Grid, Mixer, NonLocalPotential,  # noqa: F401,E501
-                   Profiler, QbgError, WfStepper, ENERGY_NAMES, gram, load, polar,
+                   Profiler, QbgError, WfStepper, ENERGY_NAMES, gram, gram_noalloc, load, polar,
                    stream_join)

@@ -57,6 +57,8 @@ SIGNATURES = {
     "qbg_prof_reset": [_vp],
     "qbg_prof_read": [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "qbg_gram": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "qbg_gram_noalloc": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "qbg_gram_noalloc_bytes": [],
     "qbg_ef_create": [_vp, _dp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
                       C.POINTER(_vp)],
     "qbg_ef_destroy": [_vp],
@@ -139,6 +141,9 @@ class QbgError(RuntimeError):
         self.code = code
 
 
+RESTYPES = {"qbg_last_error": C.c_char_p, "qbg_gram_noalloc_bytes": C.c_size_t}
+
+
 def load():
     """Load libqbg.so (raises if it has not been built)."""
     global _lib
@@ -151,7 +156,7 @@ def load():
             if f is None:
                 continue
             f.argtypes = args
-            f.restype = C.c_char_p if name == "qbg_last_error" else C.c_int
+            f.restype = RESTYPES.get(name, C.c_int)
         _lib = lib
     return _lib
 
@@ -414,6 +419,12 @@ def stream_join(before: Context, after: Context):
 def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
     """SlaterDet::gram on device coefficients (in place)."""
     ctx.check(ctx.lib.qbg_gram(ctx.h, c.p, ldc, ngw, nst, PREC[prec]))
+
+
+def gram_noalloc(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
+    """qbg_gram_noalloc: SlaterDet::gram with every temporary in the
+    library's fixed per-device array (no device allocation in the call)."""
+    ctx.check(ctx.lib.qbg_gram_noalloc(ctx.h, c.p, ldc, ngw, nst, PREC[prec]))
 
 
 class EnergyFunctional:
