@@ -119,6 +119,25 @@ int qbg_grid_destroy(qbg_grid* g);
  * Stockham kernels), [7..12] the split n1 x n2 of axes 0, 1, 2 (0 if
  * generic).  out has room for 13 ints. */
 int qbg_grid_info(const qbg_grid* g, int* out);
+/* Launch tuning of the batched transforms, in place of FourierTransform's
+ * FFTW_MEASURE planning (src/FourierTransform.cpp:45-54, 830-1062).  At the
+ * first batched call of a grid in a precision (compute_density, rs_mul_add,
+ * the energy functional's passes), outside graph capture, the library times
+ * every compiled line split of np0 (x pass) and np1 (y pass) and the y
+ * pass's band pairs per block (8, 16, 32) on the grid's own workspace, keeps
+ * the fastest, and caches the choice per device, grid shape and precision
+ * for the process.  qbg_grid_tuning reports it: out[0] 1 once tuned for prec
+ * (FP64, or any of the FP32-FFT modes), out[1..2] the x split n1 x n2,
+ * out[3..4] the y split, out[5] the pairs per block (6 ints).
+ * qbg_grid_plan fixes the choice instead (FFTW_ESTIMATE-like): x_n1, y_n1
+ * name a compiled split by its n1 (0: the default split), yppb the pairs per
+ * block (0: the default, 16); QBG_E_INVAL if no such split is compiled. */
+int qbg_grid_tuning(const qbg_grid* g, int prec, int* out);
+int qbg_grid_plan(qbg_grid* g, int prec, int x_n1, int y_n1, int yppb);
+/* the n1 of every compiled split n1 x n2 of line length n, default first,
+ * at most cap of them written to n1 (host); returns their number (0 if n
+ * has no compiled kernels: the generic ones serve it) */
+int qbg_fft_splits(int n, int* n1, int cap);
 
 /* FourierTransform::backward(c, f) and backward(c1, c2, f)
  * (src/FourierTransform.cpp:175-199): f has np012 complex entries. */

@@ -518,10 +518,19 @@ int zbwd(qbg_grid* g, int prec, int mode, const double* c, long pstride, long of
   return prec == QBG_PREC_FP64 ? zbwd_t<double>(g, mode, c, pstride, off2, last_has2, nbatch, s)
                                : zbwd_t<float>(g, mode, c, pstride, off2, last_has2, nbatch, s);
 }
+static inline int pix(int prec) { return prec == QBG_PREC_FP64 ? 0 : 1; }
+static inline const F4Ops* xops(const qbg_grid* g, int prec) {
+  const F4Ops* o = g->fx[pix(prec)];
+  return o ? o : g->f4[0];
+}
+static inline const F4Ops* yops(const qbg_grid* g, int prec) {
+  const F4Ops* o = g->fy[pix(prec)];
+  return o ? o : g->f4[1];
+}
 int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch)) return QBG_E_STATE;
   if (g->f4[0]) {
-    g->f4[0]->xpass(g, prec, false, nbatch, s);
+    xops(g, prec)->xpass(g, prec, false, nbatch, s);
     return launched();
   }
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, false, nbatch, s)
@@ -530,7 +539,7 @@ int xbwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
 int xfwd(qbg_grid* g, int prec, int nbatch, hipStream_t s) {
   if (!ws_ok(g, nbatch)) return QBG_E_STATE;
   if (g->f4[0]) {
-    g->f4[0]->xpass(g, prec, true, nbatch, s);
+    xops(g, prec)->xpass(g, prec, true, nbatch, s);
     return launched();
   }
   return prec == QBG_PREC_FP64 ? xpass_t<double>(g, true, nbatch, s)
@@ -544,8 +553,9 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
                    (mode == Y_BWD_REAL_ADD && !fgrid) || (mode == Y_FWD_REAL && !v);
   if (bad) return QBG_E_INVAL;
   if (g->f4[1]) {
-    const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? std::min(g->yppb, nbatch) : 1);
-    g->f4[1]->ypass(g, prec, mode, nbatch, ppb, v, fac, rho, fgrid, scale, s);
+    const int yp = g->yppb_t[pix(prec)] > 0 ? g->yppb_t[pix(prec)] : g->yppb;
+    const int ppb = mode == Y_DENSITY ? nbatch : (mode == Y_VMUL ? std::min(yp, nbatch) : 1);
+    yops(g, prec)->ypass(g, prec, mode, nbatch, ppb, v, fac, rho, fgrid, scale, s);
     return launched();
   }
   return prec == QBG_PREC_FP64 ? ypass_t<double>(g, mode, nbatch, v, fac, rho, fgrid, scale, s)

@@ -16,10 +16,9 @@ static const F4Ops ops_a[] = {
     QBG_F4(10, 9),   // 90
 };
 
-const F4Ops* f4_ops_a(int n) {
-  for (const F4Ops& o : ops_a)
-    if (o.n == n) return &o;
-  return nullptr;
+const F4Ops* f4_table_a(int* count) {
+  *count = (int)(sizeof(ops_a) / sizeof(ops_a[0]));
+  return ops_a;
 }
 
 }  // namespace qbg

@@ -15,10 +15,9 @@ static const F4Ops ops_b[] = {
     QBG_F4(16, 8),   // 128
 };
 
-const F4Ops* f4_ops_b(int n) {
-  for (const F4Ops& o : ops_b)
-    if (o.n == n) return &o;
-  return nullptr;
+const F4Ops* f4_table_b(int* count) {
+  *count = (int)(sizeof(ops_b) / sizeof(ops_b[0]));
+  return ops_b;
 }
 
 }  // namespace qbg

@@ -14,10 +14,9 @@ static const F4Ops ops_c[] = {
     QBG_F4(21, 8),   // 168
 };
 
-const F4Ops* f4_ops_c(int n) {
-  for (const F4Ops& o : ops_c)
-    if (o.n == n) return &o;
-  return nullptr;
+const F4Ops* f4_table_c(int* count) {
+  *count = (int)(sizeof(ops_c) / sizeof(ops_c[0]));
+  return ops_c;
 }
 
 }  // namespace qbg

@@ -14,10 +14,9 @@ static const F4Ops ops_d[] = {
     QBG_F4(16, 14),   // 224
 };
 
-const F4Ops* f4_ops_d(int n) {
-  for (const F4Ops& o : ops_d)
-    if (o.n == n) return &o;
-  return nullptr;
+const F4Ops* f4_table_d(int* count) {
+  *count = (int)(sizeof(ops_d) / sizeof(ops_d[0]));
+  return ops_d;
 }
 
 }  // namespace qbg

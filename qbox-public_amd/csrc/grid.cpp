@@ -383,7 +383,10 @@ int qbg_grid_create(qbg_ctx* c, int np0, int np1, int np2, int ng, int nvec,
     }
     // launch-shape override for tuning sweeps: QBG_YPPB (y-pass pairs per
     // block of the v multiply)
-    if (const char* e = std::getenv("QBG_YPPB")) g->yppb = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("QBG_YPPB")) {
+      g->yppb = std::max(1, std::atoi(e));
+      g->yppb_fixed = true;
+    }
     g->xrows = lines(np0);
     g->ytile = lines(np1);
     if (np1 * g->ytile > 256 * 16) g->ytile = std::max(1, 256 * 16 / np1);
@@ -671,6 +674,7 @@ int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const d
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
+  TRY(qbg::grid_tune(g, prec));
   const size_t pb = xpair_bytes(g, prec);
   if (resident && g->xres_bytes < (size_t)(pair_base + npairs) * pb)
     return g->ctx->fail(QBG_E_STATE, "resident x array too small");
@@ -692,6 +696,7 @@ int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int 
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
+  TRY(qbg::grid_tune(g, prec));
   const size_t pb = xpair_bytes(g, prec);
   if (g->xres_bytes < (size_t)(pair_base + npairs) * pb)
     return g->ctx->fail(QBG_E_STATE, "resident x array too small");
@@ -795,6 +800,7 @@ int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc, int n
     const int npairs = (nst + 1) / 2;
     const int B = std::min(npairs, g->batch);
     TRY(ensure_ws(g, B));
+    TRY(qbg::grid_tune(g, prec));
     hipStream_t s = g->ctx->stream;
     for (int p0 = 0; p0 < npairs; p0 += B) {
       const int nb = std::min(B, npairs - p0);

@@ -77,7 +77,9 @@ struct F4Ops {
   void (*ypass)(qbg_grid* g, int prec, int mode, int nbatch, int ppb, const double* v,
                 const double* fac, double* rho, double* fgrid, double scale, hipStream_t s);
 };
-const F4Ops* f4_ops(int n);   // nullptr if n has no compiled four-step kernels
+const F4Ops* f4_ops(int n);   // default split of n, nullptr if n has no compiled kernels
+// every compiled split of n (default first), at most cap; returns the count
+int f4_candidates(int n, const F4Ops** out, int cap);
 }  // namespace qbg
 
 struct qbg_grid {
@@ -122,6 +124,15 @@ struct qbg_grid {
   int fac_cap = 0;
   int batch = 64;                   // max pairs per batch
   int yppb = 16;                     // y-pass pairs per block for the v multiply
+  // the tuner's choices per precision (index 0 FP64, 1 FP32): the split of
+  // the x-pass (axis 0) and y-pass (axis 1) kernels among the compiled ones
+  // for the axis length, and the y-pass pairs per block of the v multiply;
+  // until tuned, the defaults (f4[0], f4[1], yppb)
+  const qbg::F4Ops* fx[2] = {nullptr, nullptr};
+  const qbg::F4Ops* fy[2] = {nullptr, nullptr};
+  int yppb_t[2] = {0, 0};
+  bool tuned[2] = {false, false};
+  bool yppb_fixed = false;           // set by QBG_YPPB: no tuning of yppb
 };
 
 // launchers (fft.hip)
@@ -138,6 +149,10 @@ enum OutMode { OUT_STORE = 0, OUT_ACCUM = 1 };
 // that many already); every launcher below returns QBG_E_STATE without
 // launching when the workspace is missing or smaller than its batch.
 int grid_ensure_ws(qbg_grid* g, int npairs);
+// time the compiled x- and y-pass variants on the grid's workspace and keep
+// the fastest for prec (once per grid and precision; not while capturing;
+// the workspace must exist and holds no data across this call)
+int grid_tune(qbg_grid* g, int prec);
 // bytes of one pair's x workspace at precision prec (both pass families)
 size_t xpair_bytes(const qbg_grid* g, int prec);
 // allocate the resident x array for npairs pairs (sized for FP64); returns

@@ -45,6 +45,9 @@ SIGNATURES = {
                         C.POINTER(_vp)],
     "qbg_grid_destroy": [_vp],
     "qbg_grid_info": [_vp, _ip],
+    "qbg_grid_tuning": [_vp, C.c_int, _ip],
+    "qbg_grid_plan": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "qbg_fft_splits": [C.c_int, _ip, C.c_int],
     "qbg_fft_backward": [_vp, _vp, _vp, C.c_int],
     "qbg_fft_backward_pair": [_vp, _vp, _vp, _vp, C.c_int],
     "qbg_fft_forward": [_vp, _vp, _vp, C.c_int],
@@ -335,6 +338,17 @@ class Grid:
     def four_step(self) -> bool:
         return self.info()["four_step"]
 
+    def tuning(self, prec="fp64") -> dict:
+        """qbg_grid_tuning: the launch choice of the batched passes for prec."""
+        out = np.zeros(6, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.qbg_grid_tuning(self.h, PREC[prec], out.ctypes.data_as(_ip)))
+        return dict(tuned=bool(out[0]), x=(int(out[1]), int(out[2])),
+                    y=(int(out[3]), int(out[4])), yppb=int(out[5]))
+
+    def plan(self, prec="fp64", x_n1=0, y_n1=0, yppb=0):
+        """qbg_grid_plan: fix the x/y splits (by n1, 0 = default) and pairs per block."""
+        self.ctx.check(self.ctx.lib.qbg_grid_plan(self.h, PREC[prec], x_n1, y_n1, yppb))
+
     def release(self):
         if self.h and self.ctx.h:
             self.ctx.lib.qbg_grid_destroy(self.h)
@@ -414,6 +428,15 @@ def stream_join(before: Context, after: Context):
     `before`; inside a capture on `before`, `after` becomes a branch of the
     graph (join it back before the capture ends)."""
     after.check(after.lib.qbg_stream_join(before.h, after.h))
+
+
+def fft_splits(n) -> list:
+    """qbg_fft_splits: n1 of every compiled split of line length n (default first)."""
+    out = np.zeros(8, dtype=np.int32)
+    k = load().qbg_fft_splits(int(n), out.ctypes.data_as(_ip), 8)
+    if k < 0:
+        raise QbgError(k, "qbg_fft_splits")
+    return [int(x) for x in out[:k]]
 
 
 def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):

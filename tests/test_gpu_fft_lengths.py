@@ -10,6 +10,11 @@ Per grid and precision: single backward/forward on the wavefunction and the
 density basis, pair backward/forward (src/FourierTransform.cpp:147-827),
 SlaterDet::compute_density and rs_mul_add (src/SlaterDet.cpp:225-298,
 410-477) with an odd band count.  FP64 to 1e-12 relative, FP32 to 2e-5.
+
+Lengths with more than one compiled split (the candidates of the launch
+tuner) run the pair transforms, the density and rs_mul_add again with every
+split fixed on the x axis (np0) and the y axis (np1) through qbg_grid_plan,
+and with 8 and 32 band pairs per y block.
 """
 import numpy as np
 import pytest
@@ -94,3 +99,24 @@ def test_every_length_on_every_axis(ctx, oracle_mod, k):
         hp = ctx.to_device(h0)
         g.rs_mul_add(dv, dcb, o.ngw, nst, hp, o.ngw, prec)
         assert relerr(hp.download() - h0, href) < tol, (prec, "rs_mul_add")
+        # every other compiled split of np0 (x pass) and np1 (y pass), fixed
+        xs, ys = qbg.fft_splits(o.np0), qbg.fft_splits(o.np1)
+        plans = [(xn1, 0, 0) for xn1 in xs[1:]] + [(0, yn1, 0) for yn1 in ys[1:]]
+        plans += [(0, 0, 8), (xs[-1], ys[-1], 32)]
+        for xn1, yn1, ppb in plans:
+            print(f"[case] {s.name} {prec} plan x {xn1} y {yn1} ppb {ppb}", flush=True)
+            g.plan(prec, xn1, yn1, ppb)
+            t = g.tuning(prec)
+            assert t["tuned"] and (xn1 == 0 or t["x"][0] == xn1) and (yn1 == 0 or t["y"][0] == yn1)
+            f = ctx.array(o.N, np.complex128)
+            g.fft_backward_pair(both.p, both.offset(16 * o.ngw), f, prec)
+            assert relerr(f.download(), fpair) < tol, (prec, "backward pair", xn1, yn1)
+            g.fft_forward_pair(ctx.to_device(xp), out.p, out.offset(16 * o.ngw), prec)
+            got = out.download()
+            assert relerr(got[0], r1) < tol and relerr(got[1], r2) < tol, (prec, "fwd pair", xn1, yn1)
+            rho = ctx.array(o.N).zero()
+            g.compute_density(dcb, o.ngw, nst, occ, 1.0, omega, rho, prec)
+            assert relerr(rho.download(), rho_ref) < tol, (prec, "density", xn1, yn1, ppb)
+            hp = ctx.to_device(h0)
+            g.rs_mul_add(dv, dcb, o.ngw, nst, hp, o.ngw, prec)
+            assert relerr(hp.download() - h0, href) < tol, (prec, "rs_mul_add", xn1, yn1, ppb)
