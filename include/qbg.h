@@ -307,6 +307,12 @@ int qbg_nl_energy(qbg_nl* nl, const double* c, int ldc, int compute_hpsi, double
  * of a sample file.  dims (host, 3, may be null) receives nx, ny, nz; the
  * grid belongs to ef (do not destroy it) and takes the qbg_fft_* calls. */
 int qbg_ef_wf_grid(qbg_ef* ef, int* dims, qbg_grid** out);
+/* The grid the band transforms of update_density and H*psi run on: the
+ * FourierTransform of the wavefunction basis on the density grid
+ * (src/ChargeDensity.cpp:44-49, 98; src/EnergyFunctional.cpp:831).  It
+ * belongs to ef (do not destroy it); qbg_grid_tuning reports its launch
+ * choice, qbg_grid_plan fixes one. */
+int qbg_ef_band_grid(qbg_ef* ef, qbg_grid** out);
 
 /* One SCF-iteration evaluation without host synchronization, so that it can
  * be captured in a graph: update_density -> update_vhxc -> energy

@@ -1212,6 +1212,16 @@ int qbg_ef_wf_grid(qbg_ef* e, int* dims, qbg_grid** out) {
   });
 }
 
+// the grid of the band transforms (ChargeDensity's FourierTransform on the
+// density grid, src/ChargeDensity.cpp:44-49, 98), owned by ef
+int qbg_ef_band_grid(qbg_ef* e, qbg_grid** out) {
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || !out) return QBG_E_INVAL;
+    *out = e->gw;
+    return e->gw ? QBG_OK : e->ctx->fail(QBG_E_STATE, "band grid not built");
+  });
+}
+
 // Wavefunction::set_nspin (src/Wavefunction.cpp:365-390) with the bands
 // [0, nst_up) spin up: per-spin densities and potentials (ChargeDensity
 // rhor[ispin], EnergyFunctional v_r[ispin], src/EnergyFunctional.h:78-79)

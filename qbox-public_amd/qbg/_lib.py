@@ -82,6 +82,7 @@ SIGNATURES = {
     "qbg_ef_set_nspin": [_vp, C.c_int, C.c_int],
     "qbg_wfstep_create_bands": [_vp, C.c_int, C.c_double, C.c_int, C.POINTER(_vp)],
     "qbg_ef_wf_grid": [_vp, _ip, C.POINTER(_vp)],
+    "qbg_ef_band_grid": [_vp, C.POINTER(_vp)],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_ef_energy_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
@@ -601,6 +602,12 @@ class EnergyFunctional:
                                                        C.byref(h)))
             self._wfg = _BorrowedGrid(self.ctx, h, [int(x) for x in dims])
         return self._wfg
+
+    def band_grid(self):
+        """qbg_ef_band_grid: the grid of the band transforms (owned by the functional)."""
+        h = _vp()
+        self.ctx.check(self.ctx.lib.qbg_ef_band_grid(self.h, C.byref(h)))
+        return _BorrowedGrid(self.ctx, h, [self.np0, self.np1, self.np2])
 
     def set_nspin(self, nspin, nst_up=0):
         """qbg_ef_set_nspin: bands [0, nst_up) spin up, the rest spin down."""
