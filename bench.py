@@ -482,6 +482,7 @@ def main():
         "gemm_roofline": groof,
         "fp64_gemm_roofline": gemm_roofline(ef, kt64, nst, "fp64"),
         "fft_launch_tuning": {p: ef.band_grid().tuning(p) for p in ("fp64", best)},
+        "gemm_launch_tuning": ef.gemm_plan(),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sysdef, dc.download(), args.cpu_seconds)

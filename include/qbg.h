@@ -313,6 +313,15 @@ int qbg_ef_wf_grid(qbg_ef* ef, int* dims, qbg_grid** out);
  * belongs to ef (do not destroy it); qbg_grid_tuning reports its launch
  * choice, qbg_grid_plan fixes one. */
 int qbg_ef_band_grid(qbg_ef* ef, qbg_grid** out);
+/* The GEMM half of the launch tuner (the grid's tuner covers the FFT
+ * passes): at the first 2-part BF16 evaluation, outside graph capture, the
+ * projection product anl^T c is timed with the 256-edge tiles at the default
+ * reduction split, at twice and half of it, and with the 128-edge tiles, on
+ * the functional's own operands; the fastest plan is kept (cached per device
+ * and padded shape).  out (host, 3 ints): [0] 1 once tuned, [1] tile edge,
+ * [2] reduction splits (0: the default rule).  No counterpart in the
+ * reference (its dgemm calls take no plan). */
+int qbg_ef_gemm_plan(const qbg_ef* ef, int* out);
 
 /* One SCF-iteration evaluation without host synchronization, so that it can
  * be captured in a graph: update_density -> update_vhxc -> energy

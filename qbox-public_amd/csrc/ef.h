@@ -3,6 +3,7 @@
 // and charge mixer (scf_api.cpp).
 #pragma once
 #include <vector>
+#include "energy.h"
 #include "internal.h"
 #include "setup.h"
 
@@ -67,6 +68,9 @@ struct qbg_ef {
   float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
         *b_w32 = nullptr, *b_part = nullptr;
   size_t b_part_floats = 0;
+  // launch plan of the TN projections, timed at the first 2-part BF16 use
+  qbg::BfTnPlan bf_tn;
+  bool bf_tuned = false;
   // fused force product (256-tile shapes): per-tile partials (M x 3 x Kp/256)
   // and their per-projector sums
   double *b_fpart = nullptr, *b_fred = nullptr;

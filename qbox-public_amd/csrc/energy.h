@@ -108,10 +108,19 @@ void gemm_nn(const double* A, long lda, const double* F, long ldf, long K, int N
 // the BF16 MFMA path; extents padded (Mp, Np, Kp multiples of 128, Kp of 32,
 // leading dimensions of 4) with zero padding.  Returns -1 on a bad shape.
 // TN: out[m + n*ldo] = alpha sum_k A[k + m*lda] B[k + n*ldb] + gamma A[m*lda] B64[n*ldb64]
+// launch plan of the TN product: edge 256 (the 256-tile kernel where the
+// padded extents allow it) or 128; split = reduction splits (0: enough to
+// fill the device once)
+struct BfTnPlan {
+  int edge = 256;
+  int split = 0;
+};
 int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int Np, long Kp,
                int M, int N, const double* B64, long ldb64, double alpha, double gamma,
                double* out, long ldo, float* work, size_t work_floats, int parts,
-               hipStream_t s);
+               hipStream_t s, const BfTnPlan* plan = nullptr);
+// the reduction splits the default plan of gemm_bf_tn uses for this shape
+int gemm_bf_tn_default_split(int Mp, int Np, long Kp, int parts, int edge);
 // NN: C[k + n*ldc] (+)= sum_m A[k + m*lda] F[m + n*ldf], k < K, n < N stored
 // force product with the fused force epilogue (256-tile shapes, 2-part split);
 // -1 when the shape does not allow it

@@ -704,25 +704,36 @@ void launch(BfArgs g, int nsplit, hipStream_t s) {
 
 }  // namespace
 
-int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int Np, long Kp,
-               int M, int N, const double* B64, long ldb64, double alpha, double gamma,
-               double* out, long ldo, float* work, size_t work_floats, int parts,
-               hipStream_t s) {
-  if (Mp % BT || Np % BT || Kp % BK || lda % 4 || ldb % 4 || M > Mp || N > Np) return -1;
-  BfArgs g{};
-  g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
-  const bool big = parts == 2 && Mp % BB == 0 && Np % BB == 0;
+int gemm_bf_tn_default_split(int Mp, int Np, long Kp, int parts, int edge) {
+  const bool big = parts == 2 && edge == BB && Mp % BB == 0 && Np % BB == 0;
   const int bt = big ? BB : BT;
-  g.tp = Np / bt; g.tq = Mp / bt;
-  g.np = Np; g.nq = Mp;
-  g.pfast = 0;
+  const int ntile = (Np / bt) * (Mp / bt);
   const long ksteps = Kp / BK;
-  const int ntile = g.tp * g.tq;
   // split the reduction so the blocks fill the device exactly once (a
   // second, partial round would run at a fraction of the occupancy)
   const int slots = big ? tnb_slots() : parts == 3 ? tn_slots<3>() : tn_slots<2>();
   long S = std::max<long>(1, slots / ntile);
   S = std::min<long>(S, std::max<long>(1, ksteps / 8));
+  return (int)S;
+}
+
+int gemm_bf_tn(const float* A, long lda, const float* B, long ldb, int Mp, int Np, long Kp,
+               int M, int N, const double* B64, long ldb64, double alpha, double gamma,
+               double* out, long ldo, float* work, size_t work_floats, int parts,
+               hipStream_t s, const BfTnPlan* plan) {
+  if (Mp % BT || Np % BT || Kp % BK || lda % 4 || ldb % 4 || M > Mp || N > Np) return -1;
+  BfArgs g{};
+  g.P = B; g.ldp = ldb; g.Q = A; g.ldq = lda;
+  const int edge = plan ? plan->edge : BB;
+  const bool big = parts == 2 && edge == BB && Mp % BB == 0 && Np % BB == 0;
+  const int bt = big ? BB : BT;
+  g.tp = Np / bt; g.tq = Mp / bt;
+  g.np = Np; g.nq = Mp;
+  g.pfast = 0;
+  const long ksteps = Kp / BK;
+  long S = (plan && plan->split > 0) ? plan->split
+                                      : gemm_bf_tn_default_split(Mp, Np, Kp, parts, edge);
+  S = std::min<long>(S, ksteps);
   while (S > 1 && (size_t)S * Mp * Np > work_floats) S--;
   if ((size_t)S * Mp * Np > work_floats) return -1;
   g.nr = Kp;

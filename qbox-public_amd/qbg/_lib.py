@@ -83,6 +83,7 @@ SIGNATURES = {
     "qbg_wfstep_create_bands": [_vp, C.c_int, C.c_double, C.c_int, C.POINTER(_vp)],
     "qbg_ef_wf_grid": [_vp, _ip, C.POINTER(_vp)],
     "qbg_ef_band_grid": [_vp, C.POINTER(_vp)],
+    "qbg_ef_gemm_plan": [_vp, _ip],
     "qbg_ef_set_vr": [_vp, _dp],
     "qbg_ef_evaluate_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_ef_energy_async": [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int],
@@ -608,6 +609,12 @@ class EnergyFunctional:
         h = _vp()
         self.ctx.check(self.ctx.lib.qbg_ef_band_grid(self.h, C.byref(h)))
         return _BorrowedGrid(self.ctx, h, [self.np0, self.np1, self.np2])
+
+    def gemm_plan(self) -> dict:
+        """qbg_ef_gemm_plan: the tuned launch plan of the BF16 projection product."""
+        out = np.zeros(3, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.qbg_ef_gemm_plan(self.h, out.ctypes.data_as(_ip)))
+        return dict(tuned=bool(out[0]), edge=int(out[1]), split=int(out[2]))
 
     def set_nspin(self, nspin, nst_up=0):
         """qbg_ef_set_nspin: bands [0, nst_up) spin up, the rest spin down."""

@@ -86,3 +86,29 @@ def test_plan_fixes_and_validates(ctx, oracle_mod):
     g.compute_density(dc, o.ngw, s.nst, s.occ, 1.0, abs(np.linalg.det(s.cell)), rho, "fp64")
     assert g.tuning("fp64") == t
     g.release()
+
+
+def test_gemm_plan_tuned_at_first_bf16_use(ctx):
+    """The projection product's plan (tile edge, reduction splits) is timed at
+    the first 2-part BF16 evaluation and kept; the tuned evaluation meets the
+    metric's bar against the FP64 evaluation of the same inputs."""
+    import dataclasses
+    import qbg
+    s = dataclasses.replace(cf.config(1), nst=256)
+    print(f"[case] gemm plan {s.name} nst 256", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    assert not ef.gemm_plan()["tuned"]
+    c = cf.gram_numpy(cf.random_coefficients(610, ef.basis(0)["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    r64 = ef.evaluate(dc, hp, forces=True, prec="fp64")
+    h64 = hp.download()
+    rbf = ef.evaluate(dc, hp, forces=True, prec="bf16x2")
+    p = ef.gemm_plan()
+    print(f"  plan {p}", flush=True)
+    assert p["tuned"] and p["edge"] in (128, 256) and p["split"] >= 0
+    for t in ("ekin", "eps", "enl", "ehart", "exc", "etotal"):
+        assert abs(rbf[t] - r64[t]) / s.nat < 1e-6, t
+    assert np.abs(rbf["fion"] - r64["fion"]).max() < 1e-5
+    assert np.abs(hp.download() - h64).max() / np.abs(h64).max() < 1e-5
+    ef.release()
