@@ -152,6 +152,88 @@ void System::update_twnl() {
           d3[4 * ngw + ig] = fac3 * tgy * tgz + v * y2;
           d3[5 * ngw + ig] = fac3 * tgx * tgz + v * y1;
         }
+      } else if (l == 3) {   // (:963-1245): y9..y15 and their strain derivatives
+        // The harmonics are the reference's.  Their strain derivatives take
+        // the angular term with the sign of the l <= 2 branches,
+        // y dv |G| u_i u_j + v (d_i y G_j + d_j y G_i) / 2 with d = d/dG
+        // (the reference's l = 3 branch writes - v (...) / 2, opposite to its
+        // own l = 0..2 expressions; the form here is the one the strain
+        // finite differences of the energy confirm).
+        const double s74pi = std::sqrt(7.0 / fpi), s32 = std::sqrt(1.5), s52 = std::sqrt(2.5),
+                     s15 = std::sqrt(15.0);
+        const double factor = 0.5 * s74pi;
+        for (int m = 0; m < 7; m++) lproj[is][ilm + m] = 3;
+        double* t[7];
+        double* d[7];
+        for (int m = 0; m < 7; m++) {
+          t[m] = &twnl[is][(size_t)ngw * (ilm + m)];
+          d[m] = &dtwnl[is][(size_t)ngw * 6 * (ilm + m)];
+        }
+#pragma omp parallel for schedule(static)
+        for (int ig = 0; ig < ngw; ig++) {
+          double v, dv;
+          const double tg = kpg[ig];
+          s.dvnlg(iop, tg, v, dv);
+          const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
+          const double tgi = kpgi[ig], tgi2 = tgi * tgi, tgi3 = tgi * tgi2;
+          const double tgxx = tgx * tgx * tgi2, tgyy = tgy * tgy * tgi2, tgzz = tgz * tgz * tgi2;
+          const double tgxy = tgx * tgy * tgi2, tgyz = tgy * tgz * tgi2, tgxz = tgx * tgz * tgi2;
+          const double tgxxx = tgx * tgx * tgx * tgi3, tgyyy = tgy * tgy * tgy * tgi3,
+                       tgzzz = tgz * tgz * tgz * tgi3;
+          const double tgxyy = tgx * tgy * tgy * tgi3, tgxzz = tgx * tgz * tgz * tgi3,
+                       tgyxx = tgy * tgx * tgx * tgi3, tgyzz = tgy * tgz * tgz * tgi3,
+                       tgzxx = tgz * tgx * tgx * tgi3, tgzyy = tgz * tgy * tgy * tgi3,
+                       tgxyz = tgx * tgy * tgz * tgi3;
+          const double y[7] = {factor * s52 * (3.0 * tgyxx - tgyyy),
+                               factor * s15 * 2.0 * tgxyz,
+                               factor * s32 * (4.0 * tgyzz - tgyxx - tgyyy),
+                               factor * (2.0 * tgzzz - 3.0 * (tgzxx + tgzyy)),
+                               factor * s32 * (4.0 * tgxzz - tgxxx - tgxyy),
+                               factor * s15 * (tgzxx - tgzyy),
+                               factor * s52 * (tgxxx - 3.0 * tgxyy)};
+          // gradients (d/dGx, d/dGy, d/dGz) of the cubic monomials over |G|^3
+          const double dxxx[3] = {3.0 * tgxx * (1.0 - tgxx) * tgi, -3.0 * tgxx * tgxy * tgi,
+                                  -3.0 * tgxx * tgxz * tgi};
+          const double dyyy[3] = {-3.0 * tgyy * tgxy * tgi, 3.0 * tgyy * (1.0 - tgyy) * tgi,
+                                  -3.0 * tgyy * tgyz * tgi};
+          const double dzzz[3] = {-3.0 * tgzz * tgxz * tgi, -3.0 * tgzz * tgyz * tgi,
+                                  3.0 * tgzz * (1.0 - tgzz) * tgi};
+          const double dxyy[3] = {tgyy * (1.0 - 3.0 * tgxx) * tgi, tgxy * (2.0 - 3.0 * tgyy) * tgi,
+                                  -3.0 * tgyy * tgxz * tgi};
+          const double dxzz[3] = {tgzz * (1.0 - 3.0 * tgxx) * tgi, -3.0 * tgzz * tgxy * tgi,
+                                  tgxz * (2.0 - 3.0 * tgzz) * tgi};
+          const double dyxx[3] = {tgxy * (2.0 - 3.0 * tgxx) * tgi, tgxx * (1.0 - 3.0 * tgyy) * tgi,
+                                  -3.0 * tgxx * tgyz * tgi};
+          const double dyzz[3] = {-3.0 * tgzz * tgxy * tgi, tgzz * (1.0 - 3.0 * tgyy) * tgi,
+                                  tgyz * (2.0 - 3.0 * tgzz) * tgi};
+          const double dzxx[3] = {tgxz * (2.0 - 3.0 * tgxx) * tgi, -3.0 * tgxx * tgyz * tgi,
+                                  tgxx * (1.0 - 3.0 * tgzz) * tgi};
+          const double dzyy[3] = {-3.0 * tgyy * tgxz * tgi, tgyz * (2.0 - 3.0 * tgyy) * tgi,
+                                  tgyy * (1.0 - 3.0 * tgzz) * tgi};
+          const double dxyz[3] = {tgyz * (1 - 3.0 * tgxx) * tgi, tgxz * (1 - 3.0 * tgyy) * tgi,
+                                  tgxy * (1 - 3.0 * tgzz) * tgi};
+          double dyv[7][3];
+          for (int c = 0; c < 3; c++) {
+            dyv[0][c] = factor * s52 * (3.0 * dyxx[c] - dyyy[c]);
+            dyv[1][c] = factor * s15 * 2.0 * dxyz[c];
+            dyv[2][c] = factor * s32 * (4.0 * dyzz[c] - dyxx[c] - dyyy[c]);
+            dyv[3][c] = factor * (2.0 * dzzz[c] - 3.0 * (dzxx[c] + dzyy[c]));
+            dyv[4][c] = factor * s32 * (4.0 * dxzz[c] - dxxx[c] - dxyy[c]);
+            dyv[5][c] = factor * s15 * (dzxx[c] - dzyy[c]);
+            dyv[6][c] = factor * s52 * (dxxx[c] - 3.0 * dxyy[c]);
+          }
+          const double g3[3] = {tgx, tgy, tgz};
+          const double uu[6] = {tgxx, tgyy, tgzz, tgxy, tgyz, tgxz};
+          static const int IJ[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
+          for (int m = 0; m < 7; m++) {
+            t[m][ig] = y[m] * v;
+            for (int ij = 0; ij < 6; ij++) {
+              const int a = IJ[ij][0], b = IJ[ij][1];
+              d[m][ij * ngw + ig] = y[m] * dv * tg * uu[ij] +
+                                    0.5 * v * (dyv[m][a] * g3[b] + dyv[m][b] * g3[a]);
+            }
+          }
+        }
       } else {   // l == 2 (:627-760): y4..y8 and their strain derivatives
         const double s54pi = std::sqrt(5.0 / fpi), s3 = std::sqrt(3.0);
         const double s20pi = std::sqrt(20.0 * M_PI), s20pi3 = std::sqrt(20.0 * M_PI / 3.0);
@@ -549,7 +631,7 @@ static void gemm_nn_acc(long K, int N, int M, const double* A, long lda,
 
 // -------------------------------------------------- NonLocalPotential
 // src/NonLocalPotential.cpp:1258-2069 restated for the real basis, KB
-// projectors with l <= 1.  The reference processes atoms in blocks of 32
+// projectors with l <= 3.  The reference processes atoms in blocks of 32
 // (:1274); the block size only reorders independent columns and is not
 // restated.
 // t (-i)^l (c + i s) as (re, im)

@@ -139,11 +139,11 @@ void splintd(int n, const double* xa, const double* ya, const double* y2a,
         h * (((1.0 / 6.0) - 0.5 * a * a) * y2a[kl] + (0.5 * b * b - (1.0 / 6.0)) * y2a[kh]);
 }
 
-// src/Species.cpp:510-1053 (SLPP; one projector per l, l <= 2)
+// src/Species.cpp:510-1053 (SLPP; one projector per l, l <= 3)
 void Species::initialize() {
   const double fpi = 4.0 * M_PI;
   const int np = (int)vlocal.size();
-  if (lmax > 2) throw std::runtime_error("oracle species: l > 2 not restated");
+  if (lmax > 3) throw std::runtime_error("oracle species: l > 3");
   nop = lmax + 1;
   nlm = 0;
   for (int l = 0; l <= lmax; l++) nlm += 2 * l + 1;
@@ -204,6 +204,35 @@ void Species::initialize() {
       v[ndft] = 0.0;
       cosft1(ndft, v.data());
       for (int i = 1; i < ndft; i++) fint[i] += -3.0 * v[i] / (gspl[i] * gspl[i]);
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = fint[i];
+      spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 0, 1, vnlg2[iop].data());
+    } else if (iop == 3) {  // :948-1046, j_3 in four transforms; 0 at G = 0
+      // 15/G^4 sum sin(Gr) w/r^2 - 6/G^2 sum sin(Gr) w + 1/G sum cos(Gr) r w
+      // - 15/G^3 sum cos(Gr) w/r
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = vnlr[i] / (rps[i] * rps[i]);
+      sinft(ndft, v.data());
+      fint[0] = 0.0;
+      for (int i = 1; i < ndft; i++) {
+        const double g = gspl[i];
+        fint[i] = 15.0 * v[i] / (g * g * g * g);
+      }
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i];
+      sinft(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] += -6.0 * v[i] / (gspl[i] * gspl[i]);
+      for (int i = 0; i < ndft; i++) v[i] = vnlr[i] * rps[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] += v[i] / gspl[i];
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = vnlr[i] / rps[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) {
+        const double g = gspl[i];
+        fint[i] += -15.0 * v[i] / (g * g * g);
+      }
       v[0] = 0.0;
       for (int i = 1; i < ndft; i++) v[i] = fint[i];
       spline(ndft, gspl.data(), v.data(), 0.0, 0.0, 0, 1, vnlg2[iop].data());

@@ -175,32 +175,71 @@ static void ef_local_tables(const qbg_ef* e, std::vector<double>& vps, std::vect
 }
 
 // real spherical harmonic of projector ipr of a species (one projector per
-// l: ipr 0 s; 1-3 p as x, y, z; 4-8 d as 3z^2-1, x^2-y^2, xy, yz, xz, the
-// order of src/NonLocalPotential.cpp:250-760) at the unit vector u: value
-// and gradient with respect to u
-static int proj_l(int ipr) { return ipr == 0 ? 0 : (ipr < 4 ? 1 : 2); }
+// l: ipr 0 s; 1-3 p as x, y, z; 4-8 d as 3z^2-1, x^2-y^2, xy, yz, xz; 9-15 f
+// as y(3x^2-y^2), xyz, y(4z^2-x^2-y^2), z(2z^2-3x^2-3y^2), x(4z^2-x^2-y^2),
+// z(x^2-y^2), x(x^2-3y^2), the order and normalization of
+// src/NonLocalPotential.cpp:250-1245) at the unit vector u: value and
+// gradient with respect to u (as a polynomial in u)
+static int proj_l(int ipr) { return ipr == 0 ? 0 : (ipr < 4 ? 1 : (ipr < 9 ? 2 : 3)); }
 static void harmonic(int ipr, const double* u, double& y, double* dy) {
   const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
   const double d5 = std::sqrt(5.0 / (4.0 * M_PI)), d53 = d5 * std::sqrt(3.0);
+  const double f = 0.5 * std::sqrt(7.0 / (4.0 * M_PI));
+  const double f52 = f * std::sqrt(2.5), f15 = f * std::sqrt(15.0), f32 = f * std::sqrt(1.5);
+  const double x = u[0], yy = u[1], z = u[2];
   dy[0] = dy[1] = dy[2] = 0.0;
   switch (ipr) {
     case 0: y = s14pi; break;
     case 1: case 2: case 3: y = s34pi * u[ipr - 1]; dy[ipr - 1] = s34pi; break;
-    case 4: y = 0.5 * d5 * (3.0 * u[2] * u[2] - 1.0); dy[2] = 3.0 * d5 * u[2]; break;
-    case 5: y = 0.5 * d53 * (u[0] * u[0] - u[1] * u[1]); dy[0] = d53 * u[0]; dy[1] = -d53 * u[1]; break;
-    case 6: y = d53 * u[0] * u[1]; dy[0] = d53 * u[1]; dy[1] = d53 * u[0]; break;
-    case 7: y = d53 * u[1] * u[2]; dy[1] = d53 * u[2]; dy[2] = d53 * u[1]; break;
-    default: y = d53 * u[0] * u[2]; dy[0] = d53 * u[2]; dy[2] = d53 * u[0]; break;
+    case 4: y = 0.5 * d5 * (3.0 * z * z - 1.0); dy[2] = 3.0 * d5 * z; break;
+    case 5: y = 0.5 * d53 * (x * x - yy * yy); dy[0] = d53 * x; dy[1] = -d53 * yy; break;
+    case 6: y = d53 * x * yy; dy[0] = d53 * yy; dy[1] = d53 * x; break;
+    case 7: y = d53 * yy * z; dy[1] = d53 * z; dy[2] = d53 * yy; break;
+    case 8: y = d53 * x * z; dy[0] = d53 * z; dy[2] = d53 * x; break;
+    case 9:
+      y = f52 * (3.0 * yy * x * x - yy * yy * yy);
+      dy[0] = 6.0 * f52 * x * yy; dy[1] = 3.0 * f52 * (x * x - yy * yy);
+      break;
+    case 10:
+      y = 2.0 * f15 * x * yy * z;
+      dy[0] = 2.0 * f15 * yy * z; dy[1] = 2.0 * f15 * x * z; dy[2] = 2.0 * f15 * x * yy;
+      break;
+    case 11:
+      y = f32 * (4.0 * yy * z * z - yy * x * x - yy * yy * yy);
+      dy[0] = -2.0 * f32 * x * yy; dy[1] = f32 * (4.0 * z * z - x * x - 3.0 * yy * yy);
+      dy[2] = 8.0 * f32 * yy * z;
+      break;
+    case 12:
+      y = f * (2.0 * z * z * z - 3.0 * (z * x * x + z * yy * yy));
+      dy[0] = -6.0 * f * z * x; dy[1] = -6.0 * f * z * yy;
+      dy[2] = f * (6.0 * z * z - 3.0 * (x * x + yy * yy));
+      break;
+    case 13:
+      y = f32 * (4.0 * x * z * z - x * x * x - x * yy * yy);
+      dy[0] = f32 * (4.0 * z * z - 3.0 * x * x - yy * yy); dy[1] = -2.0 * f32 * x * yy;
+      dy[2] = 8.0 * f32 * x * z;
+      break;
+    case 14:
+      y = f15 * (z * x * x - z * yy * yy);
+      dy[0] = 2.0 * f15 * z * x; dy[1] = -2.0 * f15 * z * yy; dy[2] = f15 * (x * x - yy * yy);
+      break;
+    default:
+      y = f52 * (x * x * x - 3.0 * x * yy * yy);
+      dy[0] = 3.0 * f52 * (x * x - yy * yy); dy[1] = -6.0 * f52 * x * yy;
+      break;
   }
 }
 
 // NonLocalPotential::update_twnl on the wavefunction basis of the current
-// cell (src/NonLocalPotential.cpp:235-760, KB projectors l <= 2): twnl[is] =
+// cell (src/NonLocalPotential.cpp:235-1245, KB projectors l <= 3): twnl[is] =
 // [ipr][ig] at offset sum_{js<is} npr_js ngw, dtwnl[is] = [ipr][ij][ig] at
 // six times that offset.  twnl = y(u) v_l(|G|), u = G/|G|; the strain
 // derivative is dtwnl_ij = y dv |G| u_i u_j - v sum_a dy/du_a (u_a u_i u_j -
 // delta_aj u_i), ij = xx yy zz xy yz xz, which is the reference's
 // expressions for l = 0, 1, 2 (:283-306, :426-512, :627-760) written once.
+// For l = 3 the reference (:1166-1236) writes the angular term with the
+// opposite sign, which the strain finite differences of the energy do not
+// confirm; this form (the sign of its l <= 2 branches) is used for every l.
 static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<double>& dtwnl) {
   static const int IJ[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
   const int ngw = e->wb.size;
@@ -218,7 +257,7 @@ static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<doub
       const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
       const double u[3] = {e->wb.gx[ig] * tgi, e->wb.gx[ngw + ig] * tgi,
                            e->wb.gx[2 * ngw + ig] * tgi};
-      double v[3], dv[3];
+      double v[4], dv[4];
       for (int l = 0; l <= q.lmax; l++) q.dvnlg(l, tg, v[l], dv[l]);
       for (int ipr = 0; ipr < npr; ipr++) {
         const int l = proj_l(ipr);
@@ -286,7 +325,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
                               q.dl, q.lmax);
       } else {
         ok = d.type == QBG_SPECIES_SLPP && d.vloc && d.proj && d.dl && d.lmax >= 0 &&
-             d.lmax <= 2 && d.nmesh > 0;
+             d.lmax <= 3 && d.nmesh > 0;
         if (ok) {
           q.lmax = d.lmax;
           q.vloc.assign(d.vloc, d.vloc + q.nmesh);
@@ -296,7 +335,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       }
       if (!ok || !q.init()) {
         delete e;
-        return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal l > 2?)");
+        return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal f channel of "
+                                    "a norm-conserving potential, or l > 3?)");
       }
       e->sp.push_back(q);
       e->na.push_back(na[is]);

@@ -536,12 +536,13 @@ __global__ __launch_bounds__(256) void k_anl(int ngw, int nat, long lda, const d
     for (int ipr = 0; ipr < npr; ipr++) {
       const int m = m0 + ipr * st;
       const double t = twnl[(long)m_twnl[m] + ig];
-      // (-i)^l (c + i s): l = 0 (c, s), 1 (s, -c), 2 (-c, -s)
+      // (-i)^l (c + i s): l = 0 (c, s), 1 (s, -c), 2 (-c, -s), 3 (-s, c)
       cpx2<T> v;
       const int l = m_l[m];
       if (l == 0) { v.x = (T)(t * c); v.y = (T)(t * s); }
       else if (l == 1) { v.x = (T)(t * s); v.y = (T)(-t * c); }
-      else { v.x = (T)(-t * c); v.y = (T)(-t * s); }
+      else if (l == 2) { v.x = (T)(-t * c); v.y = (T)(-t * s); }
+      else { v.x = (T)(-t * s); v.y = (T)(t * c); }
       reinterpret_cast<cpx2<T>*>(A + (long)m * lda)[ig] = v;
     }
   }
@@ -763,7 +764,9 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
     const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
-    double Z[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double Z[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) Z[i] = 0.0;
     for (int a = a0; a < a1; a++) {
       double sn, cs;
       sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
@@ -772,9 +775,11 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
         const T* yp = Y + (mb + ipr * st) * lda + 2 * ig;
         const double yr = (double)yp[0], yi = (double)yp[1];
         // Re(conj(phase) Y) for the phase (-i)^l (c + i s) of projector ipr:
-        // l = 0 (c, s), l = 1 (ipr 1-3) (s, -c), l = 2 (ipr 4-8) (-c, -s)
+        // l = 0 (c, s), l = 1 (ipr 1-3) (s, -c), l = 2 (ipr 4-8) (-c, -s),
+        // l = 3 (ipr 9-15) (-s, c)
         Z[ipr] += ipr == 0 ? cs * yr + sn * yi
-                           : (ipr < 4 ? sn * yr - cs * yi : -(cs * yr + sn * yi));
+                           : (ipr < 4 ? sn * yr - cs * yi
+                                      : (ipr < 9 ? -(cs * yr + sn * yi) : cs * yi - sn * yr));
       }
     }
     for (int ipr = 0; ipr < npr; ipr++)

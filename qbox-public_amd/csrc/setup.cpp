@@ -297,7 +297,7 @@ bool ncpp_to_kb(int lmax, int llocal, int nmesh, double deltar, const double* vp
 }
 
 bool Species::init() {
-  if (lmax < 0 || lmax > 2 || nmesh < 8 || !(deltar > 0.0)) return false;
+  if (lmax < 0 || lmax > 3 || nmesh < 8 || !(deltar > 0.0)) return false;
   const double fpi = 4.0 * M_PI;
   nlm = (lmax + 1) * (lmax + 1);
   ndft = 1;
@@ -346,6 +346,35 @@ bool Species::init() {
       v[0] = 0.0;
       for (int i = 1; i < ndft; i++) v[i] = fint[i] - v[i] / gspl[i];
       spline(ndft, gspl.data(), v.data(), true, vnlg2[l].data());
+    } else if (l == 3) {
+      // j_3 (src/Species.cpp:948-1046): 15/G^4 sum sin(Gr) w/r^2 - 6/G^2 sum sin(Gr) w
+      // + 1/G sum cos(Gr) r w - 15/G^3 sum cos(Gr) w/r; zero at G = 0
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = w[i] / (r[i] * r[i]);
+      sinft(ndft, v.data());
+      fint[0] = 0.0;
+      for (int i = 1; i < ndft; i++) {
+        const double g = gspl[i];
+        fint[i] = 15.0 * v[i] / (g * g * g * g);
+      }
+      for (int i = 0; i < ndft; i++) v[i] = w[i];
+      sinft(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] -= 6.0 * v[i] / (gspl[i] * gspl[i]);
+      for (int i = 0; i < ndft; i++) v[i] = w[i] * r[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) fint[i] += v[i] / gspl[i];
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = w[i] / r[i];
+      v[ndft] = 0.0;
+      cosft1(ndft, v.data());
+      for (int i = 1; i < ndft; i++) {
+        const double g = gspl[i];
+        fint[i] -= 15.0 * v[i] / (g * g * g);
+      }
+      v[0] = 0.0;
+      for (int i = 1; i < ndft; i++) v[i] = fint[i];
+      spline(ndft, gspl.data(), v.data(), false, vnlg2[l].data());
     } else {   // j_2: (3/G^3) sum sin(Gr) w/r - (1/G) sum sin(Gr) r w - (3/G^2) sum cos(Gr) w
                // (src/Species.cpp:872-940); zero at G = 0
       v[0] = 0.0;

@@ -58,6 +58,8 @@ SPECIES_A = SpeciesDef(zval=4, rc=1.0, sigma=0.8)
 SPECIES_B = SpeciesDef(zval=6, rc=0.7, sigma=0.6)
 # with a d projector r^2 exp(-r^2 / (2 sigma^2)), D_2 = 0.7 Ha
 SPECIES_D = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7))
+# with d and f projectors, r^3 exp(-r^2 / (2 sigma^2)) with D_3 = 0.5 Ha
+SPECIES_F = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7, 0.5))
 
 
 @dataclass
@@ -160,6 +162,11 @@ def config(index: int) -> SystemDef:
         s = SystemDef("t3: 2+1 atoms with d projectors 8x8.5x9 ecut=18Ry 6 bands",
                       np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_D, SPECIES_B], [2, 1], 6,
                       BASE_SEED + 13, 2.0)
+    elif index == 14:
+        # f projectors (l = 3) beside an s/p species, orthorhombic cell
+        s = SystemDef("t4: 2+1 atoms with f projectors 8x8.5x9 ecut=18Ry 6 bands",
+                      np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_F, SPECIES_B], [2, 1], 6,
+                      BASE_SEED + 14, 2.0)
     elif index == 12:
         # non-orthogonal (monoclinic) cell
         a = np.array([[9.0, 0.0, 0.0], [1.5, 8.5, 0.0], [0.0, 0.7, 8.0]])

@@ -122,8 +122,40 @@ def test_d_channel_evaluate(ctx, oracle_mod, prec):
     ef.release()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
+def test_f_projectors_evaluate(ctx, oracle_mod, prec):
+    """A semilocal species with s, p, d and f projectors (Gaussians r^l
+    exp(-r^2/2s^2), src/Species.cpp:948-1046) beside an s/p species in an
+    orthorhombic cell against the oracle: the j_3 transform, the seven f
+    harmonics of twnl, the (-i)^3 phase, and the f strain derivatives (in the
+    sign the strain finite differences confirm, tests/test_oracle_stress.py)."""
+    import qbg
+    s = cf.config(14)
+    print(f"[case] {s.name} {prec}", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    assert ef.nproj == 2 * 16 + 1 * 4
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(14, o.basis(0)["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        if prec == "fp64":
+            assert abs(got[t] - ref[t]) <= 1e-10 * max(abs(ref[t]), 1e-12), (t, got[t], ref[t])
+        else:
+            assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    ftol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.abs(got["fion"] - ref["fion"]).max() <= ftol * max(1.0, np.abs(ref["fion"]).max())
+    assert relerr(hp.download(), ref["hpsi"]) < ftol
+    assert np.abs(got["sigma"] - ref["sigma"]).max() < ftol * np.abs(ref["sigma"][0]).max()
+    ef.release()
+
+
 def test_f_channel_refused(ctx):
-    """A nonlocal f channel (l = 3) is outside what the library builds."""
+    """A nonlocal f channel (l = 3) of a norm-conserving potential is
+    refused: the reference's Kleinman-Bylander construction
+    (src/Species.cpp:330-505) has no l = 3 transform."""
     import qbg
     si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
     vps = np.vstack([si.vps, si.vps[2:3]])

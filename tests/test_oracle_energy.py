@@ -173,7 +173,7 @@ def _coeffs(oracle_mod, s, seed):
     return o, c
 
 
-CASES = [(11, None), (12, None), (11, (9, 1, 1)), (10, (5, 0, 1)), (13, None)]
+CASES = [(11, None), (12, None), (11, (9, 1, 1)), (10, (5, 0, 1)), (13, None), (14, None)]
 
 
 @pytest.mark.parametrize("idx,spin", CASES)

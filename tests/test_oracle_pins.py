@@ -8,10 +8,10 @@ independent of how the oracle computes them.
   accuracy.
 - Projectors: 4 pi int r^2 j_l(Gr) p_l(r) dr (src/Species.cpp:782-940) of the
   Gaussian projectors p_l = r^l exp(-r^2/(2 s^2)) are
-  (2 pi)^{3/2} s^{3+2l} G^l exp(-G^2 s^2/2) (l = 0, 1, 2).
-- d harmonics: the five real l = 2 harmonics of twnl satisfy the addition
-  theorem sum_m y_m(u) y_m(u') = (5 / 4 pi) P_2(u.u')
-  (src/NonLocalPotential.cpp:627-760).
+  (2 pi)^{3/2} s^{3+2l} G^l exp(-G^2 s^2/2) (l = 0, 1, 2, 3).
+- d and f harmonics: the five real l = 2 and seven l = 3 harmonics of twnl
+  satisfy the addition theorem sum_m y_m(u) y_m(u') = ((2l+1) / 4 pi)
+  P_l(u.u') (src/NonLocalPotential.cpp:627-760, 963-1245).
 - Nonlocal energy: one atom and one Gaussian orbital psi (s and p type) in a
   large cell; enl = occ D_l <beta_lm|psi>^2 with beta_lm = p_l(r) Y_lm and the
   overlap integral in closed form (src/NonLocalPotential.cpp:1469-1571).
@@ -90,6 +90,39 @@ def test_d_projector_transform_closed_form(oracle_mod):
     assert o.species_eval(0, np.zeros(1), nop=3)[2][2][0] == 0.0
 
 
+def test_f_projector_transform_closed_form(oracle_mod):
+    sp = cf.SPECIES_F
+    o = _species_oracle(oracle_mod, sp)
+    dg = math.pi / (o.ndft * sp.deltar)
+    s = sp.sigma
+    for g, tol in ((np.arange(1, int(12.0 / dg)) * dg, 1e-10),
+                   (np.arange(1, int(12.0 / dg)) * dg + 0.5 * dg, 1e-6)):
+        vn = o.species_eval(0, g, nop=4)[2]
+        e = np.exp(-g * g * s * s / 2)
+        assert _rel(vn[3], (2 * math.pi) ** 1.5 * s ** 9 * g ** 3 * e) < tol
+        assert _rel(vn[2], (2 * math.pi) ** 1.5 * s ** 7 * g * g * e) < tol
+    assert o.species_eval(0, np.zeros(1), nop=4)[2][3][0] == 0.0
+
+
+def test_f_harmonics_addition_theorem(oracle_mod):
+    s = dataclasses.replace(cf.config(14), na=[1, 0], species=[cf.SPECIES_F, cf.SPECIES_B],
+                            tau=np.zeros((1, 3)))
+    o = oracle_mod.Oracle(s)
+    b = o.basis(0)
+    ngw = o.ngw
+    tw = o.get(9, 0).reshape(16, ngw)
+    g = np.sqrt(b["g2"])
+    v = o.species_eval(0, g, nop=4)[2][3]
+    rng = np.random.default_rng(6)
+    i = rng.integers(1, ngw, 400)
+    j = rng.integers(1, ngw, 400)
+    u = (b["gx"] / np.where(g > 0, g, 1.0)).T
+    x = np.einsum("ij,ij->i", u[i], u[j])
+    lhs = np.einsum("mi,mi->i", tw[9:16][:, i], tw[9:16][:, j])
+    rhs = v[i] * v[j] * 7 / (4 * math.pi) * 0.5 * (5 * x ** 3 - 3 * x)
+    assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(v).max() ** 2
+
+
 def test_d_harmonics_addition_theorem(oracle_mod):
     s = dataclasses.replace(cf.config(13), na=[1, 0], species=[cf.SPECIES_D, cf.SPECIES_B],
                             tau=np.zeros((1, 3)))
@@ -126,16 +159,21 @@ def _gaussian_orbital(b, omega, a, kind):
     elif kind == "p":
         n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
         c = n * (-1j * a * a * gx) * fg
-    else:                                   # d: x y exp(-r^2/2a^2)
+    elif kind == "d":                       # d: x y exp(-r^2/2a^2)
         gy = b["gx"][1]
         n = ((a * a / 2) ** 2 * (math.pi * a * a) ** 1.5) ** -0.5
         c = n * (-(a ** 4) * gx * gy) * fg + 0j
+    else:                                   # f: x y z exp(-r^2/2a^2)
+        gy, gz = b["gx"][1], b["gx"][2]
+        n = ((a * a / 2) ** 3 * (math.pi * a * a) ** 1.5) ** -0.5
+        c = n * (1j * a ** 6 * gx * gy * gz) * fg
     return (c / math.sqrt(omega))[None, :]
 
 
 @pytest.mark.parametrize("sp,kind", [(cf.SPECIES_A, "s"), (cf.SPECIES_A, "p"),
                                      (cf.SPECIES_B, "s"), (cf.SPECIES_B, "p"),
-                                     (cf.SPECIES_D, "s"), (cf.SPECIES_D, "d")])
+                                     (cf.SPECIES_D, "s"), (cf.SPECIES_D, "d"),
+                                     (cf.SPECIES_F, "f")])
 def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
     L, a = 22.0, 1.1
     s = cf.SystemDef("one atom, one orbital", cf.cubic(L), 50.0, [sp], [1], 1, 1, 0.0,
@@ -157,10 +195,14 @@ def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
         n = ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
         ov = math.sqrt(3 / (4 * math.pi)) * n * s2 * (2 * math.pi * s2) ** 1.5
         ref = 2.0 * sp.d[1] * ov ** 2
-    else:   # beta = r^2 exp(-r^2/2s^2) sqrt(15/4pi) x y / r^2
+    elif kind == "d":   # beta = r^2 exp(-r^2/2s^2) sqrt(15/4pi) x y / r^2
         n = ((a * a / 2) ** 2 * (math.pi * a * a) ** 1.5) ** -0.5
         ov = math.sqrt(15 / (4 * math.pi)) * n * s2 ** 2 * (2 * math.pi * s2) ** 1.5
         ref = 2.0 * sp.d[2] * ov ** 2
+    else:   # beta = r^3 exp(-r^2/2s^2) sqrt(105/4pi) x y z / r^3
+        n = ((a * a / 2) ** 3 * (math.pi * a * a) ** 1.5) ** -0.5
+        ov = math.sqrt(105 / (4 * math.pi)) * n * s2 ** 3 * (2 * math.pi * s2) ** 1.5
+        ref = 2.0 * sp.d[3] * ov ** 2
     assert abs(enl - ref) < 1e-6 * abs(ref), (enl, ref)   # spline accuracy of twnl
 
 

@@ -33,7 +33,7 @@ def sym(v):
     return u
 
 
-@pytest.mark.parametrize("idx", [11, 12, 13])
+@pytest.mark.parametrize("idx", [11, 12, 13, 14])
 def test_stress_vs_strain_fd(oracle_mod, idx):
     s = cf.config(idx)
     ref = s.cell * 1.04                   # holds every strained sphere
