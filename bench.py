@@ -39,9 +39,9 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 # precision; "gemm" is the mean of the three projector products of a step.
 MEASURED_TRAFFIC = {
     ("c3: 256 atoms L=36 ecut=70Ry 512 bands", "bf16x2"): {
-        "zbwd": 1527674880, "xbwd": 3053592576, "yvmul": 5073543168, "ydens": 2539485184,
-        "xfwd": 3134738432, "zfwd": 2055003136,
-        "gemm": (5058459648 + 6154072064 + 3480879104) // 3,
+        "zbwd": 1527693688, "xbwd": 3053452345, "yvmul": 5114447872, "ydens": 2539485312,
+        "xfwd": 3134730534, "zfwd": 2054967092,
+        "gemm": (5437312691 + 6090662016 + 3451558048) // 3,
     },
 }
 
