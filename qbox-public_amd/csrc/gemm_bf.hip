@@ -68,6 +68,19 @@ __device__ __forceinline__ void split8(const float* v, u16x8 (&out)[NS]) {
   }
 }
 
+// split 4 values into 2 parts of 4 bf16 each (as split8)
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void split4(const float* v, u16x4& hi, u16x4& lo) {
+  bf16x4 h, l;
+#pragma unroll
+  for (int e = 0; e < 4; e++) h[e] = (__bf16)v[e];
+#pragma unroll
+  for (int e = 0; e < 4; e++) l[e] = (__bf16)(v[e] - (float)h[e]);
+  hi = __builtin_bit_cast(u16x4, h);
+  lo = __builtin_bit_cast(u16x4, l);
+}
+
 struct BfArgs {
   const float* P;
   long ldp;
@@ -438,9 +451,12 @@ __global__ __launch_bounds__(512) void k_gemm_bfb(BfArgs g) {
 // 256 x 256 tiles with the 2-part split and two LDS stages (160 KB): the
 // loop computes step i from one stage while the split parts of step i+1,
 // loaded into registers during step i-1, go into the other stage and the
-// loads of step i+2 are issued; one barrier per step.  Q along q (NN) is
-// loaded as pairs of adjacent q (one 8-byte load per lane and k row).
-// Otherwise as k_gemm_bfb (same fragments, MFMA order, epilogues).
+// loads of step i+2 are issued; one barrier per step.  An r-contiguous
+// operand is loaded with eight consecutive lanes on the 128 contiguous bytes
+// of one row (4 k per lane, rows 64 apart per load), so each load touches 8
+// whole cache lines; Q along q (NN) as pairs of adjacent q (one 8-byte load
+// per lane and k row).  Otherwise as k_gemm_bfb (same fragments, MFMA
+// order, epilogues).
 template <bool QR, typename TO, bool FE = false>
 __global__ __launch_bounds__(512) void k_gemm_bfd(BfArgs g) {
   __shared__ __align__(16) unsigned short sm[2][2][2][BB][LR];   // [stage][P/Q][part][row][k]
@@ -455,24 +471,25 @@ __global__ __launch_bounds__(512) void k_gemm_bfd(BfArgs g) {
   const long p0 = (long)ip * BB, q0 = (long)iq * BB;
   const long rb = (long)split * g.rper, re = min(g.nr, rb + g.rper);
   const long nsteps = (re - rb) / BK;
-  // P (and Q when r-contiguous): row srow, 16 k at sk
-  const int srow = (tid & 15) | ((tid >> 5) << 4), sk = ((tid >> 4) & 1) * 16;
+  // P (and Q when r-contiguous): rows lrow + 64 j (j < 4), 4 k at lk
+  const int lrow = tid >> 3, lk = (tid & 7) * 4;
   // Q along q: rows qq, qq + 1 (adjacent q), 8 k at qk
   const int qq = 2 * (tid & 127), qk = (tid >> 7) * 8;
-  const float* Pp = g.P + (p0 + srow) * g.ldp + sk;
-  const float* Qp = QR ? g.Q + (q0 + srow) * g.ldq + sk : g.Q + q0 + qq + (long)qk * g.ldq;
+  const float* Pp = g.P + (p0 + lrow) * g.ldp + lk;
+  const float* Qp = QR ? g.Q + (q0 + lrow) * g.ldq + lk : g.Q + q0 + qq + (long)qk * g.ldq;
+  const long p64 = 64 * g.ldp, q64 = 64 * g.ldq;
   float pv[16], qv[16];
   auto load = [&](long r0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const float4 t = *reinterpret_cast<const float4*>(Pp + r0 + 4 * i);
-      pv[4 * i] = t.x; pv[4 * i + 1] = t.y; pv[4 * i + 2] = t.z; pv[4 * i + 3] = t.w;
+    for (int j = 0; j < 4; j++) {
+      const float4 t = *reinterpret_cast<const float4*>(Pp + j * p64 + r0);
+      pv[4 * j] = t.x; pv[4 * j + 1] = t.y; pv[4 * j + 2] = t.z; pv[4 * j + 3] = t.w;
     }
     if (QR) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const float4 t = *reinterpret_cast<const float4*>(Qp + r0 + 4 * i);
-        qv[4 * i] = t.x; qv[4 * i + 1] = t.y; qv[4 * i + 2] = t.z; qv[4 * i + 3] = t.w;
+      for (int j = 0; j < 4; j++) {
+        const float4 t = *reinterpret_cast<const float4*>(Qp + j * q64 + r0);
+        qv[4 * j] = t.x; qv[4 * j + 1] = t.y; qv[4 * j + 2] = t.z; qv[4 * j + 3] = t.w;
       }
     } else {
       // qv[i] = Q(r0 + qk + i, qq), qv[8 + i] = Q(r0 + qk + i, qq + 1)
@@ -484,21 +501,23 @@ __global__ __launch_bounds__(512) void k_gemm_bfd(BfArgs g) {
     }
   };
   auto store = [&](int st) {
-    u16x8 h[2];
-    split8<2>(pv, h);
-    *reinterpret_cast<u16x8*>(&sm[st][0][0][srow][sk]) = h[0];
-    *reinterpret_cast<u16x8*>(&sm[st][0][1][srow][sk]) = h[1];
-    split8<2>(pv + 8, h);
-    *reinterpret_cast<u16x8*>(&sm[st][0][0][srow][sk + 8]) = h[0];
-    *reinterpret_cast<u16x8*>(&sm[st][0][1][srow][sk + 8]) = h[1];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      u16x4 hi, lo;
+      split4(pv + 4 * j, hi, lo);
+      *reinterpret_cast<u16x4*>(&sm[st][0][0][lrow + 64 * j][lk]) = hi;
+      *reinterpret_cast<u16x4*>(&sm[st][0][1][lrow + 64 * j][lk]) = lo;
+    }
     if (QR) {
-      split8<2>(qv, h);
-      *reinterpret_cast<u16x8*>(&sm[st][1][0][srow][sk]) = h[0];
-      *reinterpret_cast<u16x8*>(&sm[st][1][1][srow][sk]) = h[1];
-      split8<2>(qv + 8, h);
-      *reinterpret_cast<u16x8*>(&sm[st][1][0][srow][sk + 8]) = h[0];
-      *reinterpret_cast<u16x8*>(&sm[st][1][1][srow][sk + 8]) = h[1];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        u16x4 hi, lo;
+        split4(qv + 4 * j, hi, lo);
+        *reinterpret_cast<u16x4*>(&sm[st][1][0][lrow + 64 * j][lk]) = hi;
+        *reinterpret_cast<u16x4*>(&sm[st][1][1][lrow + 64 * j][lk]) = lo;
+      }
     } else {
+      u16x8 h[2];
       split8<2>(qv, h);
       *reinterpret_cast<u16x8*>(&sm[st][1][0][qq][qk]) = h[0];
       *reinterpret_cast<u16x8*>(&sm[st][1][1][qq][qk]) = h[1];
