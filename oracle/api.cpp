@@ -22,6 +22,7 @@ typedef struct {
   int llocal;
   const double* vps;    // (lmax+1) * nmesh
   const double* phi;    // (lmax+1) * nmesh
+  const double* nlcc;   // core density (nmesh) or null
 } qbo_species_desc;
 
 static std::string g_err;
@@ -51,6 +52,7 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
                            sp[is].proj + (size_t)(l + 1) * sp[is].nmesh);
         q.dl.assign(sp[is].dl, sp[is].dl + q.lmax + 1);
       }
+      if (sp[is].nlcc) q.nlcc.assign(sp[is].nlcc, sp[is].nlcc + sp[is].nmesh);
       s->sp.push_back(q);
       s->tau.emplace_back(tau + 3 * off, tau + 3 * (off + na[is]));
       off += na[is];
@@ -170,7 +172,9 @@ void qbo_update_vhxc(void* h, int stress) { ((System*)h)->update_vhxc(stress != 
 // 5 rhopst (ngv complex); 6 vps, 7 rhops, 8 dvps of species `is` (ngv);
 // 9 twnl of species is (npr*ngw); 10 sfac of species is (ngv complex);
 // 11 fion_esr (3 nat); 12 dtwnl (npr*6*ngw); 13 dvion_local_g;
-// 14 rhor, 15 v_r, 16 vxc_r of spin down (nspin 2; arrays 0-2 are spin up)
+// 14 rhor, 15 v_r, 16 vxc_r of spin down (nspin 2; arrays 0-2 are spin up);
+// 17 rhocore_r (N), 18 vxc_g (ngv complex), 19 rhocore_sp of species is (ngv)
+// (core density systems only, after update_vhxc)
 void qbo_get(void* h, int what, int is, double* out) {
   System* s = (System*)h;
   auto cpy = [&](const void* p, size_t bytes) { std::memcpy(out, p, bytes); };
@@ -192,6 +196,9 @@ void qbo_get(void* h, int what, int is, double* out) {
     case 14: cpy(s->rhor_dn.data(), 8 * s->rhor_dn.size()); break;
     case 15: cpy(s->v_r_dn.data(), 8 * s->v_r_dn.size()); break;
     case 16: cpy(s->vxc_r_dn.data(), 8 * s->vxc_r_dn.size()); break;
+    case 17: cpy(s->rhocore_r.data(), 8 * s->rhocore_r.size()); break;
+    case 18: cpy(s->vxc_g.data(), 16 * s->vxc_g.size()); break;
+    case 19: if (s->core) cpy(s->rhocore_sp[is].data(), 8 * s->rhocore_sp[is].size()); break;
   }
 }
 
@@ -255,6 +262,14 @@ double qbo_nl_energy(void* h, const double* c, int compute_hpsi, double* hpsi, i
   if (fion) std::memcpy(fion, f.data(), sizeof(double) * f.size());
   if (sigma6) std::memcpy(sigma6, sig, sizeof(sig));
   return enl;
+}
+
+// core density transform of species is at n points g: rho, drho/dg, and
+// its G = 0 value (the integral of the core density) as the return value
+double qbo_species_core(void* h, int is, int n, const double* g, double* rho, double* drho) {
+  const Species& q = ((System*)h)->sp[is];
+  for (int i = 0; i < n; i++) q.drhocoreg(g[i], rho[i], drho[i]);
+  return q.zcore;
 }
 
 void qbo_species_eval(void* h, int is, int n, const double* g, double* vloc,

@@ -42,6 +42,11 @@ void System::setup() {
   rhog.assign(vb.size, 0.0);
   rhogt.assign(vb.size, 0.0);
   rhoelg.assign(vb.size, 0.0);
+  core = false;                              // src/EnergyFunctional.cpp:135-163
+  for (auto& q : sp) core |= q.has_nlcc();
+  rhocore_r.assign(core ? N : 0, 0.0);
+  rhocore_g.assign(core ? vb.size : 0, 0.0);
+  vxc_g.assign(core ? vb.size : 0, 0.0);
   cell_moved();
   atoms_moved();
 }
@@ -54,6 +59,7 @@ void System::cell_moved() {
   vps.assign(nsp, std::vector<double>(ngv));
   dvps.assign(nsp, std::vector<double>(ngv));
   rhops.assign(nsp, std::vector<double>(ngv));
+  rhocore_sp.assign(core ? nsp : 0, std::vector<double>(ngv, 0.0));
   for (int is = 0; is < nsp; is++) {
 #pragma omp parallel for schedule(static)
     for (int ig = 0; ig < ngv; ig++) {
@@ -62,6 +68,11 @@ void System::cell_moved() {
       sp[is].dvlocg(vb.g[ig], v, dv);
       vps[is][ig] = v * omega_inv;
       dvps[is][ig] = dv * omega_inv;
+      if (core) {                               // :1224-1228
+        double rc, drc;
+        sp[is].drhocoreg(vb.g[ig], rc, drc);
+        rhocore_sp[is][ig] = rc * omega_inv;
+      }
     }
   }
   update_twnl();
@@ -524,34 +535,73 @@ void System::update_density(const cplx* c) {
 }
 
 // src/EnergyFunctional.cpp:231-394 and XCPotential.cpp:124-230, 467-516
+// the core density of the current positions and spin count: rhocore_g =
+// sum_is (omega / nspin) sfac rhocore_sp, rhocore_r = Re(backward) / omega
+// (src/EnergyFunctional.cpp:1064-1083; computed there in atoms_moved)
+void System::core_density() {
+  const int ngv = vb.size;
+  const double omega = cell.volume;
+  const long N = vft.np012();
+  std::fill(rhocore_g.begin(), rhocore_g.end(), cplx(0.0));
+  const double spin_fac = omega / nspin;
+  for (size_t is = 0; is < sp.size(); is++)
+    for (int ig = 0; ig < ngv; ig++) rhocore_g[ig] += spin_fac * sfac[is][ig] * rhocore_sp[is][ig];
+  std::vector<cplx> tmp(N);
+  vft.backward(rhocore_g.data(), tmp.data());
+  for (long i = 0; i < N; i++) rhocore_r[i] = tmp[i].real() / omega;
+}
+
 void System::update_vhxc(bool compute_stress) {
   const int ngv = vb.size;
   const double omega = cell.volume, omega_inv = 1.0 / omega;
   const long N = vft.np012();
   for (int ig = 0; ig < ngv; ig++) rhoelg[ig] = omega_inv * rhog[ig];
-  // LDA, or LSDA for nspin 2 (XCPotential.cpp:181-226)
+  if (core) core_density();
+  // LDA, or LSDA for nspin 2 (XCPotential.cpp:181-226), on the valence
+  // density plus each spin's share of the core density (:154-179)
+  auto rc = [&](long i) { return core ? rhocore_r[i] : 0.0; };
   double esum = 0.0, dsum = 0.0;
   if (nspin == 2) {
     for (long i = 0; i < N; i++) {
       double e, vu, vd;
-      lsda(rhor[i], rhor_dn[i], e, vu, vd);
-      const double r = rhor[i] + rhor_dn[i];
+      const double ru = rhor[i] + rc(i), rd = rhor_dn[i] + rc(i);
+      lsda(ru, rd, e, vu, vd);
+      const double r = ru + rd;
       esum += r * e;
-      dsum += r * e - rhor[i] * vu - rhor_dn[i] * vd;
+      dsum += r * e - ru * vu - rd * vd;
       vxc_r[i] = vu;
       vxc_r_dn[i] = vd;
     }
   } else {
     for (long i = 0; i < N; i++) {
       double e, v;
-      lda_unpolarized(rhor[i], e, v);
-      esum += rhor[i] * e;
-      dsum += rhor[i] * (e - v);
+      const double r = rhor[i] + rc(i);
+      lda_unpolarized(r, e, v);
+      esum += r * e;
+      dsum += r * (e - v);
       vxc_r[i] = v;
     }
   }
   exc = esum * omega / (double)N;
   dxc = dsum * omega / (double)N;
+  if (core) {
+    // vxc_g (spin average for nspin 2) and the core term of dxc:
+    // nspin (2 sum_G Re(vxc_g* rhocore_g) - the G = 0 term)
+    // (src/EnergyFunctional.cpp:275-306)
+    std::vector<cplx> tmp(N), vg(ngv);
+    for (long i = 0; i < N; i++) tmp[i] = vxc_r[i];
+    vft.forward(tmp.data(), vxc_g.data());
+    if (nspin == 2) {
+      for (long i = 0; i < N; i++) tmp[i] = vxc_r_dn[i];
+      vft.forward(tmp.data(), vg.data());
+      for (int ig = 0; ig < ngv; ig++) vxc_g[ig] = 0.5 * (vxc_g[ig] + vg[ig]);
+    }
+    double sum = 0.0;
+    for (int ig = 0; ig < ngv; ig++)
+      sum += vxc_g[ig].real() * rhocore_g[ig].real() + vxc_g[ig].imag() * rhocore_g[ig].imag();
+    sum = 2.0 * sum - std::real(std::conj(vxc_g[0]) * rhocore_g[0]);
+    dxc += nspin * sum;
+  }
   for (int i = 0; i < 6; i++) sigma_exc[i] = 0.0;
   if (compute_stress) {
     const double t = -dsum / (double)N;
@@ -830,8 +880,14 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
       const double fac2 = 2.0 * 0.25 * sp[is].rcps * sp[is].rcps;
       for (int ig = 0; ig < ngv; ig++) {
         const cplx sg = sfac[is][ig], rg = rhogt[ig];
-        const double temp = fac2 * (rg.real() * sg.real() + rg.imag() * sg.imag()) *
-                            rhops[is][ig] * vb.g2i[ig];
+        double temp = fac2 * (rg.real() * sg.real() + rg.imag() * sg.imag()) *
+                      rhops[is][ig] * vb.g2i[ig];
+        if (core) {                                 // :650-659
+          double rhoc, drhoc;
+          sp[is].drhocoreg(vb.g[ig], rhoc, drhoc);
+          temp -= (vxc_g[ig].real() * sg.real() + vxc_g[ig].imag() * sg.imag()) * drhoc *
+                  vb.gi[ig] * omega_inv / fpi;
+        }
         const double gx = vb.gx[ig], gy = vb.gx[ngv + ig], gz = vb.gx[2 * ngv + ig];
         sh[0] += temp * gx * gx; sh[1] += temp * gy * gy; sh[2] += temp * gz * gz;
         sh[3] += temp * gx * gy; sh[4] += temp * gy * gz; sh[5] += temp * gx * gz;
@@ -874,6 +930,7 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
       for (int ig = 0; ig < ngv; ig++) {
         const double tmp = fpi * rhops[is][ig] * vb.g2i[ig];
         vtemp[ig] = tmp * std::conj(rhogt[ig]) + vps[is][ig] * std::conj(rhoelg[ig]);
+        if (core) vtemp[ig] += std::conj(vxc_g[ig]) * rhocore_sp[is][ig];   // :855-856
       }
       const int na = (int)tau[is].size() / 3;
 #pragma omp parallel for schedule(dynamic)

@@ -23,22 +23,27 @@ class SpeciesDesc(C.Structure):
     _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
                 ("deltar", C.c_double), ("rcps", C.c_double),
                 ("vloc", _dp), ("proj", _dp), ("dl", _dp),
-                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp)]
+                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp),
+                ("nlcc", _dp)]
 
 
 def species_desc(sp, keep):
     """SpeciesDesc of a configs.SpeciesDef (SLPP form) or an NCPP species
     (semilocal potentials vps, pseudo-wavefunctions phi, llocal)."""
+    core = sp.core_density()
+    if core is not None:
+        core = np.ascontiguousarray(core, dtype=float)
+        keep.append(core)
     if getattr(sp, "kind", "") == "ncpp":
         vps = np.ascontiguousarray(sp.vps, dtype=float)
         phi = np.ascontiguousarray(sp.phi, dtype=float)
         keep += [vps, phi]
         return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
-                           1, sp.llocal, _ptr(vps), _ptr(phi))
+                           1, sp.llocal, _ptr(vps), _ptr(phi), _ptr(core))
     vloc, proj, dl = sp.radial()
     keep += [vloc, proj, dl]
     return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
-                       _ptr(vloc), _ptr(proj), _ptr(dl), 0, 0, None, None)
+                       _ptr(vloc), _ptr(proj), _ptr(dl), 0, 0, None, None, _ptr(core))
 
 
 def _ptr(a, t=_dp):
@@ -65,6 +70,8 @@ def lib():
                      "qbo_update_vhxc", "qbo_get", "qbo_set_vr", "qbo_species_eval"]:
             getattr(_lib, name).argtypes = None
         _lib.qbo_gram.argtypes = [_dp, C.c_int, C.c_int]
+        _lib.qbo_species_core.restype = C.c_double
+        _lib.qbo_species_core.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         _lib.qbo_sd_dot.restype = C.c_double
         _lib.qbo_sd_dot.argtypes = [_dp, _dp, C.c_int, C.c_int]
         _lib.qbo_prec_ekin.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
@@ -234,7 +241,8 @@ class Oracle:
                  9: (npr * self.ngw, float), 10: (self.ngv, complex),
                  11: (3 * self.sysdef.nat, float), 12: (npr * 6 * self.ngw, float),
                  13: (self.ngv, complex), 14: (self.N, float), 15: (self.N, float),
-                 16: (self.N, float)}
+                 16: (self.N, float), 17: (self.N, float), 18: (self.ngv, complex),
+                 19: (self.ngv, float)}
         n, t = sizes[what]
         out = np.zeros(n, dtype=np.complex128 if t is complex else np.float64)
         lib().qbo_get(self.h, what, isp, out.ctypes.data_as(_dp))
@@ -283,6 +291,14 @@ class Oracle:
 
     def mixer(self, alpha=0.5, rc_kerker=10.0, ndim=3, rc1=3.0):
         return Mixer(self, alpha, rc_kerker, ndim, rc1)
+
+    def species_core(self, isp, g):
+        """(rhocoreg, drhocoreg, zcore) of species isp at |G| = g: the core
+        density's transform (Species::drhocoreg) and its integral."""
+        g = np.ascontiguousarray(g, dtype=float)
+        rho, drho = np.zeros(g.size), np.zeros(g.size)
+        z = lib().qbo_species_core(self.h, isp, g.size, _ptr(g), _ptr(rho), _ptr(drho))
+        return rho, drho, z
 
     def species_eval(self, isp, g, nop=None):
         """(vlocg, dvlocg, vnlg[l], dvnlg[l]) of species isp at |G| = g; one

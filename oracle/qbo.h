@@ -82,14 +82,19 @@ struct Species {
   std::vector<double> vlocal;                  // input radial local potential
   std::vector<std::vector<double>> proj;       // proj[l][r]
   std::vector<double> dl;                      // D_l (one projector per l)
+  std::vector<double> nlcc;                    // core density on the mesh (NLCC), or empty
   // derived
   int ndft = 0, nop = 0, nlm = 0;
   std::vector<double> rps, gspl, vlocg, vlocg2;
   std::vector<std::vector<double>> vnlg, vnlg2;
+  std::vector<double> nlccg, nlccg2;           // core density transform and its spline
+  double zcore = 0.0;                          // integral of the core density
+  bool has_nlcc() const { return !nlcc.empty(); }
   void initialize();
   void dvlocg(double g, double& v, double& dv) const;     // :1145-1156
   void dvnlg(int iop, double g, double& v, double& dv) const; // :1172-1185
   double rhopsg(double g) const;                            // :1187-1191
+  void drhocoreg(double g, double& rho, double& drho) const; // :1194-1220
   double eself() const;                                     // Species.h:133
   double dmatrix(int ipr, int jpr) const;                   // :1450-1464
 };
@@ -149,6 +154,15 @@ struct System {
   std::vector<std::vector<double>> twnl, dtwnl;          // [is][ipr*ngw+ig]
   std::vector<std::vector<int>> lproj;
   std::vector<double> fion_esr; double esr = 0.0; double sigma_esr[6];
+  // non-linear core correction (any species with a core density):
+  // rhocore_sp[is][ig] = rhocoreg(g)/omega, rhocore_g = sum_is sfac
+  // rhocoreg / nspin, rhocore_r its real-space grid (each spin's share),
+  // vxc_g the transform of vxc_r (spin average) (src/EnergyFunctional.cpp:
+  // 155-163, 275-306, 1064-1083, 1224-1228)
+  bool core = false;
+  std::vector<std::vector<double>> rhocore_sp;           // [is][ngv]
+  std::vector<cplx> rhocore_g, vxc_g;
+  std::vector<double> rhocore_r;
   double eself_total = 0.0;
   // state
   std::vector<double> rhor;           // density, real space (spin up if nspin 2)
@@ -169,6 +183,7 @@ struct System {
   // back: rhor = Re(backward(rhog)) / omega (src/ChargeDensity.cpp:201-221)
   void rhog_of(const double* r, cplx* g);
   void rhor_of(const cplx* g, double* r);
+  void core_density();                  // src/EnergyFunctional.cpp:1064-1083
   void update_vhxc(bool compute_stress);// src/EnergyFunctional.cpp:231-394
   double energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                 bool compute_forces, bool compute_stress, Results& r);  // :397

@@ -176,6 +176,24 @@ void Species::initialize() {
   }
   vlocg2.resize(ndft);
   spline(ndft, gspl.data(), vlocg.data(), 0.0, 0.0, 0, 1, vlocg2.data());
+  // core density (Species::initialize_nlcc, :1056-1100, and :698-717):
+  // nlcc(G) = 4 pi / G sum_r sin(Gr) r dr nlcc(r), the G = 0 value the
+  // Simpson integral of 4 pi r^2 nlcc(r)
+  nlccg.clear();
+  nlccg2.clear();
+  zcore = 0.0;
+  if (has_nlcc()) {
+    nlccg.assign(ndft, 0.0);
+    for (int i = 0; i < (int)nlcc.size() && i < ndft; i++)
+      nlccg[i] = fpi * rps[i] * deltar * nlcc[i];
+    for (int i = 0; i < ndft; i++) fint[i] = nlccg[i] * rps[i];
+    zcore = simpsn(ndft, fint.data());
+    sinft(ndft, nlccg.data());
+    nlccg[0] = zcore;
+    for (int i = 1; i < ndft; i++) nlccg[i] /= gspl[i];
+    nlccg2.resize(ndft);
+    spline(ndft, gspl.data(), nlccg.data(), 0.0, 0.0, 0, 1, nlccg2.data());
+  }
 
   vnlg.assign(nop, std::vector<double>(ndft + 1, 0.0));
   vnlg2.assign(nop, std::vector<double>(ndft + 1, 0.0));
@@ -260,6 +278,11 @@ void Species::dvlocg(double g, double& v, double& dv) const {
 void Species::dvnlg(int iop, double g, double& v, double& dv) const {
   if (g > gspl[ndft - 1]) { v = 0.0; dv = 0.0; return; }
   splintd(ndft, gspl.data(), vnlg[iop].data(), vnlg2[iop].data(), g, &v, &dv);
+}
+
+void Species::drhocoreg(double g, double& rho, double& drho) const {
+  if (!has_nlcc() || g > gspl[ndft - 1]) { rho = 0.0; drho = 0.0; return; }
+  splintd(ndft, gspl.data(), nlccg.data(), nlccg2.data(), g, &rho, &drho);
 }
 
 double Species::rhopsg(double g) const {

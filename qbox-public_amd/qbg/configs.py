@@ -35,10 +35,21 @@ class SpeciesDef:
     deltar: float = DELTAR
     nmesh: int = NMESH
     rcps: float = RCPS
+    # non-linear core correction: a Gaussian core density A exp(-(r/rcore)^2),
+    # given as (A, rcore); () for none
+    core: tuple = ()
 
     @property
     def lmax(self) -> int:
         return len(self.d) - 1
+
+    def core_density(self):
+        """The core density on the radial mesh (Species nlcc_), or None."""
+        if not self.core:
+            return None
+        a, rcore = self.core
+        r = np.arange(self.nmesh) * self.deltar
+        return _round12(a * np.exp(-(r / rcore) ** 2))
 
     def radial(self):
         """Return (vloc[nmesh], proj[(lmax+1)*nmesh], dl[lmax+1])."""
@@ -60,6 +71,8 @@ SPECIES_B = SpeciesDef(zval=6, rc=0.7, sigma=0.6)
 SPECIES_D = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7))
 # with d and f projectors, r^3 exp(-r^2 / (2 sigma^2)) with D_3 = 0.5 Ha
 SPECIES_F = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7, 0.5))
+# with a core density 0.4 exp(-(r/0.9)^2) (about 0.75 electrons)
+SPECIES_C = SpeciesDef(zval=4, rc=1.0, sigma=0.8, d=(2.0, -1.0, 0.7), core=(0.4, 0.9))
 
 
 @dataclass
@@ -167,6 +180,12 @@ def config(index: int) -> SystemDef:
         s = SystemDef("t4: 2+1 atoms with f projectors 8x8.5x9 ecut=18Ry 6 bands",
                       np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_F, SPECIES_B], [2, 1], 6,
                       BASE_SEED + 14, 2.0)
+    elif index == 15:
+        # core densities (non-linear core correction) on a d species beside a
+        # species without one, orthorhombic cell
+        s = SystemDef("t5: 2+1 atoms with core densities 8x8.5x9 ecut=18Ry 6 bands",
+                      np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_C, SPECIES_B], [2, 1], 6,
+                      BASE_SEED + 15, 2.0)
     elif index == 12:
         # non-orthogonal (monoclinic) cell
         a = np.array([[9.0, 0.0, 0.0], [1.5, 8.5, 0.0], [0.0, 0.7, 8.0]])

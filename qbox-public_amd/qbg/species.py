@@ -33,12 +33,22 @@ class NcppSpecies:
     atomic_number: int = 0
     rcps: float = RCPS
     description: str = field(default="", repr=False)
+    nlcc: np.ndarray = field(default=None, repr=False)   # <core_density>, or None
 
     kind = "ncpp"
 
     @property
     def nmesh(self) -> int:
         return self.vps.shape[1]
+
+    def core_density(self):
+        """The core density on the radial mesh (Species nlcc_), or None."""
+        if self.nlcc is None or not self.nlcc.size:
+            return None
+        out = np.zeros(self.nmesh)
+        n = min(self.nmesh, self.nlcc.size)
+        out[:n] = self.nlcc[:n]
+        return out
 
 
 def _local(tag):
@@ -71,9 +81,12 @@ def parse_species_element(root) -> NcppSpecies:
         raise ValueError("no norm_conserving_pseudopotential element")
     p = {}
     proj = {}
+    nlcc = None
     for el in ncpp:
         name = _local(el.tag)
-        if name == "projector":
+        if name == "core_density":          # src/SpeciesHandler.cpp:214-220
+            nlcc = _numbers(el.text or "")[:int(el.attrib.get("size", "0"))]
+        elif name == "projector":
             l = int(el.attrib["l"])
             size = int(el.attrib["size"])
             d = {"size": size}
@@ -104,7 +117,7 @@ def parse_species_element(root) -> NcppSpecies:
                        phi=phi, nquad=nquad, rquad=float(p.get("rquad", "0")),
                        mass=float(vals.get("mass", "0") or 0),
                        atomic_number=int(vals.get("atomic_number", "0") or 0),
-                       description=vals.get("description", ""))
+                       description=vals.get("description", ""), nlcc=nlcc)
 
 
 def read_species(path) -> NcppSpecies:
@@ -127,6 +140,10 @@ def species_xml(sp: NcppSpecies, name: str) -> str:
            f"<nquad>{sp.nquad}</nquad>",
            f"<rquad>{sp.rquad:.12e}</rquad>",
            f"<mesh_spacing>{sp.deltar:.12e}</mesh_spacing>"]
+    if sp.core_density() is not None:               # src/Species.cpp:1310-1316
+        out.append(f'<core_density size="{sp.nlcc.size}">')
+        out += [repr(float(x)) for x in sp.nlcc]
+        out.append("</core_density>")
     for l in range(sp.lmax + 1):
         out.append(f'<projector l="{l}" size="{sp.nmesh}">')
         out.append("<radial_potential>")

@@ -173,7 +173,8 @@ def _coeffs(oracle_mod, s, seed):
     return o, c
 
 
-CASES = [(11, None), (12, None), (11, (9, 1, 1)), (10, (5, 0, 1)), (13, None), (14, None)]
+CASES = [(11, None), (12, None), (11, (9, 1, 1)), (10, (5, 0, 1)), (13, None), (14, None),
+         (15, None), (15, (12, 1, 1))]
 
 
 @pytest.mark.parametrize("idx,spin", CASES)
@@ -214,8 +215,28 @@ def test_forces_are_energy_gradient(oracle_mod, idx, spin):
         assert abs(fd - r["fion"][a, j]) <= 1e-6 * max(1.0, abs(fd))
 
 
-def test_spin_identical_sets_equal_unpolarized(oracle_mod):
-    s1 = cf.config(11)
+@pytest.mark.parametrize("idx,spin", [(11, None), (15, None), (15, (12, 1, 1))])
+def test_dxc_is_exc_minus_valence_potential_energy(oracle_mod, idx, spin):
+    """dxc = exc - omega/N sum_s sum_r rho_s(r) vxc_s(r) over the valence
+    densities: with a core density the reference adds the core part of
+    sum rho vxc back in G space (src/EnergyFunctional.cpp:275-306), so the
+    identity checks that term, its factor 2, its G = 0 term and its spin
+    factor against a real-space sum."""
+    s = _system(idx, spin)
+    o, c = _coeffs(oracle_mod, s, 600 + idx)
+    r = o.evaluate(c, hpsi=False, forces=False)
+    omega = abs(np.linalg.det(s.cell))
+    pv = np.sum(o.get(0) * o.get(2))
+    if getattr(s, "nspin", 1) == 2:
+        pv += np.sum(o.get(14) * o.get(16))
+    ref = r["exc"] - omega / o.N * pv
+    print(f"  dxc {r['dxc']:.14e} exc - <rho vxc> {ref:.14e}")
+    assert abs(r["dxc"] - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+@pytest.mark.parametrize("idx", [11, 15])
+def test_spin_identical_sets_equal_unpolarized(oracle_mod, idx):
+    s1 = cf.config(idx)
     o1, c = _coeffs(oracle_mod, s1, 500)
     r1 = o1.evaluate(c, hpsi=True, forces=True, stress=True)
     s2 = dataclasses.replace(s1, nst=2 * s1.nst, nspin=2, nst_up=s1.nst,

@@ -22,6 +22,11 @@ independent of how the oracle computes them.
   splitting parameter (src/EnergyFunctional.cpp:1097-1192, :327-351).
 - Smearing width: for a neutral system etotal does not depend on rcps
   (it moves energy between eps, ehart, esr and eself, src/AtomSet.cpp:43-44).
+- Core density (non-linear core correction): the transform of a Gaussian
+  core density A exp(-(r/rc)^2) is A pi^{3/2} rc^3 exp(-G^2 rc^2/4)
+  (src/Species.cpp:1056-1100, 698-717), and the core density on the grid is
+  the periodic sum of the atoms' Gaussians, half of it per spin with two
+  spins (src/EnergyFunctional.cpp:1064-1083).
 """
 import dataclasses
 import math
@@ -102,6 +107,59 @@ def test_f_projector_transform_closed_form(oracle_mod):
         assert _rel(vn[3], (2 * math.pi) ** 1.5 * s ** 9 * g ** 3 * e) < tol
         assert _rel(vn[2], (2 * math.pi) ** 1.5 * s ** 7 * g * g * e) < tol
     assert o.species_eval(0, np.zeros(1), nop=4)[2][3][0] == 0.0
+
+
+def test_core_density_transform_closed_form(oracle_mod):
+    sp = cf.SPECIES_C
+    o = _species_oracle(oracle_mod, sp)
+    dg = math.pi / (o.ndft * sp.deltar)
+    A, rc = sp.core
+    zref = A * math.pi ** 1.5 * rc ** 3
+    nodes = np.arange(1, int(12.0 / dg)) * dg
+    for g, tol in ((nodes, 1e-10), (nodes + 0.5 * dg, 1e-6)):
+        rho, drho, z = o.species_core(0, g)
+        ref = zref * np.exp(-g * g * rc * rc / 4)
+        assert _rel(rho, ref) < tol
+        if tol > 1e-10:
+            assert _rel(drho, -0.5 * g * rc * rc * ref) < 1e-5
+    rho0, drho0, z = o.species_core(0, np.zeros(1))
+    assert abs(z - zref) < 1e-9 * zref and rho0[0] == z and abs(drho0[0]) < 1e-12
+    # a species without a core density has none
+    assert np.all(_species_oracle(oracle_mod, cf.SPECIES_A).species_core(0, nodes)[0] == 0.0)
+
+
+@pytest.mark.parametrize("spin", [False, True])
+def test_core_density_real_space(oracle_mod, spin):
+    s = dataclasses.replace(cf.config(15), ecut_ry=40.0)
+    if spin:
+        s = cf.spin_polarized(s, 12, 0, 0)
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(7, o.basis(0)["g2"], s.nst))
+    if spin:
+        u = s.nst_up
+        c = np.concatenate([cf.gram_numpy(c[:u]), cf.gram_numpy(c[u:])])
+    o.evaluate(c, hpsi=False, forces=False)
+    got = o.get(17)
+    n0, n1, n2 = o.np0, o.np1, o.np2
+    L = np.diag(s.cell)
+    x = (np.arange(n0) / n0)[:, None, None] * L[0]
+    y = (np.arange(n1) / n1)[None, :, None] * L[1]
+    z = (np.arange(n2) / n2)[None, None, :] * L[2]
+    A, rc = cf.SPECIES_C.core
+    ref = np.zeros((n0, n1, n2))
+    for t in s.tau[:2]:                       # the two atoms of the core species
+        for i in (-1, 0, 1):
+            for j in (-1, 0, 1):
+                for k in (-1, 0, 1):
+                    d2 = ((x - t[0] - i * L[0]) ** 2 + (y - t[1] - j * L[1]) ** 2 +
+                          (z - t[2] - k * L[2]) ** 2)
+                    ref += A * np.exp(-d2 / rc ** 2)
+    ref = ref.transpose(2, 1, 0).ravel()      # i + np0 (j + np1 k)
+    if spin:
+        ref = 0.5 * ref
+    err = np.abs(got - ref).max() / ref.max()
+    print(f"  core density on the grid: max rel err {err:.2e}")
+    assert err < 1e-7
 
 
 def test_f_harmonics_addition_theorem(oracle_mod):
