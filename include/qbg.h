@@ -190,15 +190,22 @@ size_t qbg_gram_noalloc_bytes(void);
  * rcps is the pseudocharge radius (1.5 in Qbox, src/AtomSet.cpp:43-44).
  *   type QBG_SPECIES_SLPP: norm_conserving_semilocal_pseudopotential
  *     (src/SpeciesHandler.cpp:200-241, Species::initialize_slpp) with one
- *     projector per l = 0..lmax (lmax <= 1): local potential vloc[nmesh],
+ *     projector per l = 0..lmax (lmax <= 3): local potential vloc[nmesh],
  *     projectors proj[(lmax+1)*nmesh], D_l in dl[lmax+1];
  *   type QBG_SPECIES_NCPP: norm_conserving_pseudopotential in the
  *     Kleinman-Bylander form (nquad = 0; Species::initialize_ncpp,
  *     src/Species.cpp:71-507): semilocal potentials vps[(lmax+1)*nmesh]
  *     and pseudo-wavefunctions phi[(lmax+1)*nmesh] (the row of the local
  *     channel llocal may hold anything); lmax <= 3, every nonlocal channel
- *     l != llocal must have l <= 1.
- * Fields after dl were added for NCPP; a zeroed tail selects SLPP. */
+ *     l != llocal must have l <= 2.
+ *   nlcc (either type, may be NULL): the core density <core_density> on the
+ *     same mesh (nmesh values) for the non-linear core correction
+ *     (Species::initialize_nlcc, src/Species.cpp:1056-1100): the
+ *     exchange-correlation functional then sees the valence plus the core
+ *     density, with the matching dxc, force and stress terms
+ *     (src/EnergyFunctional.cpp:275-306, 650-659, 855-856, 1064-1083).
+ * Fields after dl were added later; a zeroed tail selects SLPP without a
+ * core density. */
 #define QBG_SPECIES_SLPP 0
 #define QBG_SPECIES_NCPP 1
 typedef struct {
@@ -214,6 +221,7 @@ typedef struct {
   int llocal;
   const double* vps;
   const double* phi;
+  const double* nlcc;
 } qbg_species_desc;
 
 /* EnergyFunctional(Sample&, ChargeDensity&) for one Gamma-point system
@@ -438,7 +446,9 @@ int qbg_polar(qbg_ctx* ctx, double* x, int n, double tol, int maxiter, int* iter
 
 /* host copy of a device array: 0 rhor, 1 v_r, 2 vxc_r, 3 rhog,
  * 4 vion_local_g, 5 rhopst, 6 sfac, 7 fion_esr; with nspin 2, 0-2 are the
- * spin-up arrays and 8 rhor, 9 v_r, 10 vxc_r the spin-down ones */
+ * spin-up arrays and 8 rhor, 9 v_r, 10 vxc_r the spin-down ones; with core
+ * densities 11 the core density on the grid (each spin's share, ChargeDensity
+ * rhocore_r) and 12 vxc_g (after update_vhxc) */
 int qbg_ef_get(qbg_ef* ef, int what, double* host_out);
 /* replace v_r (host array of np012 doubles) */
 int qbg_ef_set_vr(qbg_ef* ef, const double* host_v);

@@ -27,6 +27,7 @@ enum {
   S_NLS = 33,     // 6: NL stress sums tsum_ij
   S_HPS = 39,     // 6: species part of the Hartree stress
   S_FLAGS = 45,   // 2: forces, stress computed by the staged evaluation
+  S_CORE = 47,    // 1: sum_G w Re(conj(vxc_g) rhocore_g) (core density systems)
   S_COUNT = 48
 };
 
@@ -91,7 +92,8 @@ static void ef_free(qbg_ef* e) {
                e->d_dtwnl, e->d_dt_off, e->d_sp_npr, e->d_fac2, e->b_anl, e->b_c32,
                e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
                e->d_seg_ptr, e->d_seg_n, e->d_rhor2, e->d_vxc2, e->d_vr2, e->d_vloc,
-               e->d_rhot, e->b_fpart, e->b_fred};
+               e->d_rhot, e->b_fpart, e->b_fred, e->d_rhoc, e->d_drhoc, e->d_rhocg,
+               e->d_rhocr, e->d_vxcg};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
@@ -157,12 +159,18 @@ static void ef_esr(qbg_ef* e) {
 // EnergyFunctional::cell_moved's tables on the density basis of the current
 // cell (src/EnergyFunctional.cpp:1200-1245): rhops = rhopsg/omega,
 // vps = vlocg/omega, dvps = dvlocg/omega, species-major [is][ig]
+// and, with core densities, rhoc = rhocoreg/omega (:1224-1228) and
+// drhoc = drhocoreg gi / (4 pi omega) (the factor of the core term of the
+// Hartree stress, :650-659)
 static void ef_local_tables(const qbg_ef* e, std::vector<double>& vps, std::vector<double>& dvps,
-                            std::vector<double>& rhops) {
+                            std::vector<double>& rhops, std::vector<double>& rhoc,
+                            std::vector<double>& drhoc) {
   const int nsp = e->nsp, ngv = e->vb.size;
   vps.assign((size_t)nsp * ngv, 0.0);
   dvps.assign((size_t)nsp * ngv, 0.0);
   rhops.assign((size_t)nsp * ngv, 0.0);
+  rhoc.assign(e->core ? (size_t)nsp * ngv : 0, 0.0);
+  drhoc.assign(e->core ? (size_t)nsp * ngv : 0, 0.0);
   for (int is = 0; is < nsp; is++)
     for (int ig = 0; ig < ngv; ig++) {
       double v, dv;
@@ -171,7 +179,33 @@ static void ef_local_tables(const qbg_ef* e, std::vector<double>& vps, std::vect
       e->sp[is].dvlocg(g, v, dv);
       vps[(size_t)is * ngv + ig] = v * oi;
       dvps[(size_t)is * ngv + ig] = dv * oi;
+      if (e->core) {
+        double rc, drc;
+        e->sp[is].drhocoreg(g, rc, drc);
+        rhoc[(size_t)is * ngv + ig] = rc * oi;
+        drhoc[(size_t)is * ngv + ig] = drc * e->vb.gi[ig] * oi / (4.0 * M_PI);
+      }
     }
+}
+
+// the core density of the current positions and spin count: rhocore_g =
+// (omega / nspin) sum_s sfac_s rhoc_s, rhocore_r = Re(backward(rhocore_g)) /
+// omega on the density grid (src/EnergyFunctional.cpp:1064-1083)
+static int ef_core_density(qbg_ef* e) {
+  if (!e->core) return QBG_OK;
+  qbg_ctx* c = e->ctx;
+  hipStream_t s = c->stream;
+  launch_core_g(e->vb.size, e->nsp, e->d_sfac, e->d_rhoc, e->omega / e->nspin, e->d_rhocg, s);
+  qbg_grid* g = e->gv;
+  int rc = grid_ensure_ws(g, 1);
+  if (rc) return rc;
+  rc = zbwd(g, QBG_PREC_FP64, Z_SINGLE, e->d_rhocg, 0, 0, 0, 1, s);
+  if (!rc) rc = xbwd(g, QBG_PREC_FP64, 1, s);
+  if (!rc)
+    rc = ypass(g, QBG_PREC_FP64, Y_BWD_REAL_ADD, 1, nullptr, nullptr, nullptr, e->d_rhocr,
+               1.0 / e->omega, s);
+  if (rc) return c->fail(rc, "core density: density-grid transform refused");
+  return QBG_OK;
 }
 
 // real spherical harmonic of projector ipr of a species (one projector per
@@ -286,6 +320,8 @@ static int ef_atoms_moved(qbg_ef* e) {
   launch_sfac(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->nsp, e->d_sp_first,
               e->d_vps, e->d_dvps, e->d_rhops, e->d_sfac, e->d_vion, e->d_dvion, e->d_rhopst, s);
   c->prof_end();
+  int rc = ef_core_density(e);
+  if (rc) return rc;
   ef_esr(e);
   QBG_HIP(c, hipMemcpyAsync(e->d_fesr, e->fesr.data(), sizeof(double) * e->fesr.size(),
                             hipMemcpyHostToDevice, s));
@@ -333,6 +369,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
           q.dl.assign(d.dl, d.dl + q.lmax + 1);
         }
       }
+      if (ok && d.nlcc) q.nlcc.assign(d.nlcc, d.nlcc + q.nmesh);
       if (!ok || !q.init()) {
         delete e;
         return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal f channel of "
@@ -344,6 +381,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     }
     e->nat = e->sp_first.back();
     e->tau.assign(tau, tau + 3 * e->nat);
+    for (int is = 0; is < nsp; is++) e->core |= e->sp[is].has_nlcc();   // :135-153
     for (int is = 0; is < nsp; is++) e->eself += e->na[is] * e->sp[is].eself();
     // bases and grid (src/ChargeDensity.cpp:34-98)
     host::Cell def = e->cell;
@@ -374,8 +412,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     }
     const int ngw = e->wb.size, ngv = e->vb.size;
     // cell_moved tables (src/EnergyFunctional.cpp:1200-1245)
-    std::vector<double> vps, dvps, rhops, twnl, dtwnl;
-    ef_local_tables(e, vps, dvps, rhops);
+    std::vector<double> vps, dvps, rhops, twnl, dtwnl, rhoc, drhoc;
+    ef_local_tables(e, vps, dvps, rhops, rhoc, drhoc);
     ef_twnl(e, twnl, dtwnl);
     // projector layout: species-major, ipr-major within a species, atom-minor
     // (src/NonLocalPotential.cpp:1341-1395)
@@ -483,7 +521,11 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
         (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngw)) ||
         (rc = dalloc(c, &e->d_fnl, (size_t)2 * e->M * nst)) ||
         (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
-        (rc = dalloc(c, &e->d_work, e->work_doubles))) {
+        (rc = dalloc(c, &e->d_work, e->work_doubles)) ||
+        (e->core && ((rc = dput(c, &e->d_rhoc, rhoc)) || (rc = dput(c, &e->d_drhoc, drhoc)) ||
+                     (rc = dalloc(c, &e->d_rhocg, (size_t)2 * ngv)) ||
+                     (rc = dalloc(c, &e->d_rhocr, N)) ||
+                     (rc = dalloc(c, &e->d_vxcg, (size_t)2 * ngv))))) {
       ef_free(e);
       delete e;
       return rc;
@@ -601,8 +643,8 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
     e->omega = cell.volume;
     e->wb = std::move(wb);
     e->vb = std::move(vb);
-    std::vector<double> vps, dvps, rhops, twnl, dtwnl;
-    ef_local_tables(e, vps, dvps, rhops);
+    std::vector<double> vps, dvps, rhops, twnl, dtwnl, rhoc, drhoc;
+    ef_local_tables(e, vps, dvps, rhops, rhoc, drhoc);
     ef_twnl(e, twnl, dtwnl);
     std::vector<double> bm(9);
     for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
@@ -612,7 +654,8 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
         (rc = dset(c, e->d_gxv, e->vb.gx)) || (rc = dset(c, e->d_vps, vps)) ||
         (rc = dset(c, e->d_dvps, dvps)) || (rc = dset(c, e->d_rhops, rhops)) ||
         (rc = dset(c, e->d_twnl, twnl)) || (rc = dset(c, e->d_dtwnl, dtwnl)) ||
-        (rc = dset(c, e->d_b, bm)) || (rc = dset(c, e->d_dfac, density_factors(e))))
+        (rc = dset(c, e->d_b, bm)) || (rc = dset(c, e->d_dfac, density_factors(e))) ||
+        (e->core && ((rc = dset(c, e->d_rhoc, rhoc)) || (rc = dset(c, e->d_drhoc, drhoc)))))
       return rc;
     e->have_rho = false;
     e->xres_c = nullptr;
@@ -698,11 +741,12 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
     QBG_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     ctx->prof_begin(QBG_K_OTHER);
+    const double* rcore = e->core ? e->d_rhocr : nullptr;
     if (e->nspin == 2)
-      launch_lsda(e->d_rhor, e->d_rhor2, e->N, e->d_vxc, e->d_vxc2, e->d_partial,
+      launch_lsda(e->d_rhor, e->d_rhor2, rcore, e->N, e->d_vxc, e->d_vxc2, e->d_partial,
                   e->d_scal + S_LDA, s);
     else
-      launch_lda(e->d_rhor, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
+      launch_lda(e->d_rhor, rcore, e->N, e->d_vxc, e->d_partial, e->d_scal + S_LDA, s);
     launch_vhxc_g(e->d_rhog, e->vb.size, 1.0 / e->omega, e->d_vion, e->d_dvion, e->d_rhopst,
                   e->d_g2iv, e->d_giv, e->d_gxv, e->d_rhoelg, e->d_rhogt, e->d_vlocal,
                   compute_stress, e->d_partial, e->d_scal + S_VHXC, s);
@@ -732,6 +776,27 @@ int qbg_ef_update_vhxc(qbg_ef* e, int compute_stress) {
     }
     ctx->prof_end();
     if (r3) return ctx->fail(r3, "update_vhxc: density-grid transform refused");
+    if (e->core) {
+      // vxc_g = forward(vxc_r) (the spin average for nspin 2, by linearity
+      // the transform of the averaged grid) and the core term of dxc
+      // (src/EnergyFunctional.cpp:275-306)
+      const double* vx = e->d_vxc;
+      double scale = 1.0;
+      if (e->nspin == 2) {
+        add2(e->d_vxc, e->d_vxc2, e->d_vloc, e->N, s);   // d_vloc is free again here
+        vx = e->d_vloc;
+        scale = 0.5;
+      }
+      ctx->prof_begin(QBG_K_VGRID);
+      int r = ypass(g, QBG_PREC_FP64, Y_FWD_REAL, 1, vx, nullptr, nullptr, nullptr, scale, s);
+      if (!r) r = xfwd(g, QBG_PREC_FP64, 1, s);
+      if (!r) r = zfwd(g, QBG_PREC_FP64, Z_SINGLE, OUT_STORE, e->d_vxcg, 0, 0, 0, 1, s);
+      ctx->prof_end();
+      if (r) return ctx->fail(r, "update_vhxc: density-grid transform refused");
+      ctx->prof_begin(QBG_K_OTHER);
+      launch_core_dxc(e->vb.size, e->d_vxcg, e->d_rhocg, e->d_partial, e->d_scal + S_CORE, s);
+      ctx->prof_end();
+    }
     e->have_vhxc = true;
     e->vhxc_stress = compute_stress != 0;
     hipError_t er = hipGetLastError();
@@ -1052,7 +1117,8 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   if (compute_stress) {
     ctx->prof_begin(QBG_K_OTHER);
     launch_hart_ps_stress(e->d_rhogt, e->d_sfac, e->d_rhops, e->d_g2iv, e->d_gxv, e->vb.size,
-                          e->nsp, e->d_fac2, e->d_partial, e->d_scal + S_HPS, s);
+                          e->nsp, e->d_fac2, e->core ? e->d_vxcg : nullptr, e->d_drhoc,
+                          e->d_partial, e->d_scal + S_HPS, s);
     ctx->prof_end();
   }
   // local potential: hpsi += FFT(v_r * FFT^-1 c)
@@ -1076,8 +1142,8 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     const int nchunk = 64;
     launch_local_forces(e->d_idxv, e->vb.size, e->d_sftab, e->krange, e->kmin, e->d_lfgrp,
                         e->nlfgrp, e->nsp, e->d_vps, e->d_rhops, e->d_g2iv, e->d_gxv, e->d_rhogt,
-                        e->d_rhoelg, e->omega, e->d_fesr, e->nat, e->d_vtemp, e->d_partial,
-                        nchunk, e->d_fion, s);
+                        e->d_rhoelg, e->core ? e->d_vxcg : nullptr, e->d_rhoc, e->omega,
+                        e->d_fesr, e->nat, e->d_vtemp, e->d_partial, nchunk, e->d_fion, s);
     ctx->prof_end();
   }
   // what this evaluation computed travels with its results, so a replayed
@@ -1105,7 +1171,10 @@ static int ef_collect(qbg_ef* e, double* energies, double* fion, double* sigma) 
   const double* g0 = e->h_res + S_COUNT;   // rhoelg[0] (2), vion[0] (2)
   if (compute_forces && fion) std::memcpy(fion, e->h_res + S_COUNT + 4, sizeof(double) * 3 * e->nat);
   const double omega = e->omega, fpi = 4.0 * M_PI, Nd = (double)e->N;
-  const double exc = sc[S_LDA + 0] * omega / Nd, dxc = sc[S_LDA + 1] * omega / Nd;
+  const double exc = sc[S_LDA + 0] * omega / Nd;
+  // the core term of dxc, nspin sum_G w Re(conj(vxc_g) rhocore_g)
+  // (src/EnergyFunctional.cpp:290-305)
+  const double dxc = sc[S_LDA + 1] * omega / Nd + (e->core ? e->nspin * sc[S_CORE] : 0.0);
   const double charge = sc[S_LDA + 2] * omega / Nd;
   const double eps = (2.0 * sc[S_VHXC + 0] - (g0[0] * g0[2] + g0[1] * g0[3])) * omega;
   const double ehart = omega * fpi * sc[S_VHXC + 1];
@@ -1298,8 +1367,12 @@ int qbg_ef_get(qbg_ef* e, int what, double* host) {
       case 8: src = e->d_rhor2; n = e->N; break;
       case 9: src = e->d_vr2; n = e->N; break;
       case 10: src = e->d_vxc2; n = e->N; break;
+      case 11: src = e->d_rhocr; n = e->N; break;
+      case 12: src = e->d_vxcg; n = 2 * (size_t)ngv; break;
       default: return QBG_E_INVAL;
     }
+    if (!src && what >= 11)
+      return e->ctx->fail(QBG_E_STATE, "core density arrays exist with core densities only");
     if (!src) return e->ctx->fail(QBG_E_STATE, "spin-down arrays exist with nspin 2 only");
     QBG_HIP(e->ctx, hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost,
                                    e->ctx->stream));
@@ -1382,6 +1455,7 @@ int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
     e->d_dfac = nullptr;
     int rc = dput(c, &e->d_dfac, dfac);
     if (rc) return rc;
+    if ((rc = ef_core_density(e))) return rc;      // each spin's share of the core
     e->have_rho = e->have_vhxc = false;
     e->xres_c = nullptr;
     return QBG_OK;

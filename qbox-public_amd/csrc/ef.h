@@ -93,6 +93,13 @@ struct qbg_ef {
   int nspin = 1, nst_up = 0;
   double *d_rhor2 = nullptr, *d_vxc2 = nullptr, *d_vr2 = nullptr, *d_vloc = nullptr,
          *d_rhot = nullptr;
+  // non-linear core correction (any species with a core density): per
+  // species rhocoreg/omega and drhocoreg gi/(4 pi omega) [is][ig], the core
+  // density in G (each spin's share) and on the grid, vxc in G (spin
+  // average), written by update_vhxc for the forces and stress
+  bool core = false;
+  double *d_rhoc = nullptr, *d_drhoc = nullptr, *d_rhocg = nullptr, *d_rhocr = nullptr,
+         *d_vxcg = nullptr;
   // the bands (pointer, leading dimension, band-FFT precision) whose x-pass
   // output the last update_density left in the band grid's resident array;
   // xres_c = nullptr when there is none (consumed by an H*psi or not kept)

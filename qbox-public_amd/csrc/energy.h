@@ -13,11 +13,16 @@ int launch_kinetic(const double* c, long ldc, int nst, const double* occ, int ng
                    const double* g2, const double* gx, double* hpsi, long ldh, int store,
                    double* partial, double* out8, hipStream_t s,
                    float* c32 = nullptr, long ldk = 0);
-int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
-               hipStream_t s);
+int launch_lda(const double* rho, const double* rc, long n, double* vxc, double* partial,
+               double* out4, hipStream_t s);
 // spin-polarized LDA: vup, vdn and the same four partial sums as launch_lda
-int launch_lsda(const double* rup, const double* rdn, long n, double* vup, double* vdn,
-                double* partial, double* out4, hipStream_t s);
+int launch_lsda(const double* rup, const double* rdn, const double* rc, long n, double* vup,
+                double* vdn, double* partial, double* out4, hipStream_t s);
+// core density in G of the current structure factor, and the core term of dxc
+void launch_core_g(int ngv, int nsp, const double* sfac, const double* rhoc, double spin_fac,
+                   double* rhocg, hipStream_t s);
+void launch_core_dxc(int ngv, const double* vxcg, const double* rhocg, double* partial,
+                     double* out1, hipStream_t s);
 void launch_lda_points(const double* rho, int n, double* e, double* v, hipStream_t s);
 int launch_vhxc_g(const double* rhog, int ngv, double omega_inv, const double* vion,
                   const double* dvion, const double* rhopst, const double* g2i, const double* gi,
@@ -36,7 +41,7 @@ void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int kra
                          const int* kmin, const int32_t* grp, int ngrp, int nsp,
                          const double* vps, const double* rhops, const double* g2i,
                          const double* gx, const double* rhogt, const double* rhoelg,
-                         double omega, const double* fesr, int nat, double* vt, double* partial,
+                         const double* vxcg, const double* rhoc, double omega, const double* fesr, int nat, double* vt, double* partial,
                          int nchunk, double* fion, hipStream_t s);
 // anl columns of every atom's projectors (FP64 if prec == 0, else FP32)
 void launch_anl(int ngw, int nat, long lda, const double* gx, const double* tau,
@@ -66,7 +71,8 @@ void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const d
                         hipStream_t s);
 void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double* rhops,
                            const double* g2i, const double* gx, int ngv, int nsp,
-                           const double* fac2, double* partial, double* out6, hipStream_t s);
+                           const double* fac2, const double* vxcg, const double* drhoc,
+                           double* partial, double* out6, hipStream_t s);
 
 // gemm.hip.  prec (QBG_PREC_FP64 / QBG_PREC_FP32) selects the MFMA compute
 // type; operands of either type are converted as they are staged.

@@ -151,28 +151,31 @@ __device__ __forceinline__ void lda_pz(double rh, double& ee, double& vv) {
 }
 
 // vxc_r and partial sums [sum rho e, sum rho (e - v), sum rho, 0]
-// (src/XCPotential.cpp:193-229, src/ChargeDensity.cpp:178-187)
-__global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho, long n,
+// (src/XCPotential.cpp:193-229, src/ChargeDensity.cpp:178-187); with a core
+// density rc (or null) the functional sees rho + rc (src/XCPotential.cpp:
+// 154-179) while the third sum stays the valence charge
+__global__ __launch_bounds__(256) void k_lda(const double* __restrict__ rho,
+                                             const double* __restrict__ rc, long n,
                                              double* __restrict__ vxc,
                                              double* __restrict__ partial) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   double v[4] = {0, 0, 0, 0};
   if (i < n) {
     double e, vv;
-    const double r = rho[i];
+    const double r = rc ? rho[i] + rc[i] : rho[i];
     lda_pz(r, e, vv);
     vxc[i] = vv;
     v[0] = r * e;
     v[1] = r * (e - vv);
-    v[2] = r;
+    v[2] = rho[i];
   }
   block_reduce_store<4>(v, partial + (long)blockIdx.x * 4);
 }
 
-int launch_lda(const double* rho, long n, double* vxc, double* partial, double* out4,
-               hipStream_t s) {
+int launch_lda(const double* rho, const double* rc, long n, double* vxc, double* partial,
+               double* out4, hipStream_t s) {
   const int nb = (int)((n + 255) / 256);
-  hipLaunchKernelGGL(k_lda, dim3(nb), dim3(256), 0, s, rho, n, vxc, partial);
+  hipLaunchKernelGGL(k_lda, dim3(nb), dim3(256), 0, s, rho, rc, n, vxc, partial);
   reduce_partials(partial, nb, 4, out4, s);
   return nb;
 }
@@ -212,14 +215,17 @@ __device__ __forceinline__ void lda_pz_ferro(double rh, double& ee, double& vv) 
 // [sum rho e, sum (rho e - rho_up v_up - rho_dn v_dn), sum rho, 0]
 // (src/XCPotential.cpp:206-218)
 __global__ __launch_bounds__(256) void k_lsda(const double* __restrict__ rup,
-                                              const double* __restrict__ rdn, long n,
+                                              const double* __restrict__ rdn,
+                                              const double* __restrict__ rc, long n,
                                               double* __restrict__ vup,
                                               double* __restrict__ vdn,
                                               double* __restrict__ partial) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   double v[4] = {0, 0, 0, 0};
   if (i < n) {
-    const double ru = rup[i], rd = rdn[i], r = ru + rd;
+    // each spin with its share of the core density rc (or null)
+    const double ci = rc ? rc[i] : 0.0;
+    const double ru = rup[i] + ci, rd = rdn[i] + ci, r = ru + rd;
     double e = 0.0, vu = 0.0, vd = 0.0;
     if (r > 0.0) {
       const double kf = 1.0 / (2.0 * cbrt(2.0) - 2.0);
@@ -241,15 +247,15 @@ __global__ __launch_bounds__(256) void k_lsda(const double* __restrict__ rup,
     vdn[i] = vd;
     v[0] = r * e;
     v[1] = r * e - ru * vu - rd * vd;
-    v[2] = r;
+    v[2] = rup[i] + rdn[i];
   }
   block_reduce_store<4>(v, partial + (long)blockIdx.x * 4);
 }
 
-int launch_lsda(const double* rup, const double* rdn, long n, double* vup, double* vdn,
-                double* partial, double* out4, hipStream_t s) {
+int launch_lsda(const double* rup, const double* rdn, const double* rc, long n, double* vup,
+                double* vdn, double* partial, double* out4, hipStream_t s) {
   const int nb = (int)((n + 255) / 256);
-  hipLaunchKernelGGL(k_lsda, dim3(nb), dim3(256), 0, s, rup, rdn, n, vup, vdn, partial);
+  hipLaunchKernelGGL(k_lsda, dim3(nb), dim3(256), 0, s, rup, rdn, rc, n, vup, vdn, partial);
   reduce_partials(partial, nb, 4, out4, s);
   return nb;
 }
@@ -402,6 +408,51 @@ void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, con
                      dvion, rhopst);
 }
 
+// ---------------------------------------------------- core density (NLCC)
+// rhocore_g = spin_fac sum_s sfac_s rhoc_s with rhoc_s = rhocoreg/omega and
+// spin_fac = omega / nspin (src/EnergyFunctional.cpp:1064-1077)
+__global__ __launch_bounds__(256) void k_core_g(int ngv, int nsp, const double* __restrict__ sfac,
+                                                const double* __restrict__ rhoc, double spin_fac,
+                                                double* __restrict__ rhocg) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngv) return;
+  double re = 0.0, im = 0.0;
+  for (int is = 0; is < nsp; is++) {
+    const long o = (long)is * ngv + ig;
+    re += sfac[2 * o] * rhoc[o];
+    im += sfac[2 * o + 1] * rhoc[o];
+  }
+  rhocg[2 * ig] = spin_fac * re;
+  rhocg[2 * ig + 1] = spin_fac * im;
+}
+
+void launch_core_g(int ngv, int nsp, const double* sfac, const double* rhoc, double spin_fac,
+                   double* rhocg, hipStream_t s) {
+  hipLaunchKernelGGL(k_core_g, dim3((ngv + 255) / 256), dim3(256), 0, s, ngv, nsp, sfac, rhoc,
+                     spin_fac, rhocg);
+}
+
+// sum_G w_G Re(conj(vxc_g) rhocore_g), w = 1 at G = 0 (index 0) and 2
+// otherwise: the half-sphere form of the core term of dxc
+// (src/EnergyFunctional.cpp:290-302)
+__global__ __launch_bounds__(256) void k_core_dxc(int ngv, const double* __restrict__ vxcg,
+                                                  const double* __restrict__ rhocg,
+                                                  double* __restrict__ partial) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  if (ig < ngv)
+    v[0] = (ig == 0 ? 1.0 : 2.0) *
+           (vxcg[2 * ig] * rhocg[2 * ig] + vxcg[2 * ig + 1] * rhocg[2 * ig + 1]);
+  block_reduce_store<1>(v, partial + blockIdx.x);
+}
+
+void launch_core_dxc(int ngv, const double* vxcg, const double* rhocg, double* partial,
+                     double* out1, hipStream_t s) {
+  const int nb = (ngv + 255) / 256;
+  hipLaunchKernelGGL(k_core_dxc, dim3(nb), dim3(256), 0, s, ngv, vxcg, rhocg, partial);
+  reduce_partials(partial, nb, 1, out1, s);
+}
+
 // ---------------------------------------------------------- local forces
 // src/EnergyFunctional.cpp:841-918: per atom a of species s,
 //   vtemp_s = 4 pi rhops_s g2i conj(rhogt) + vps_s conj(rhoelg)
@@ -411,23 +462,33 @@ void launch_sfac(const int32_t* idx, int ngv, const double* tab, int krange, con
 // tables in LDS, so the G data is read once per group instead of per atom.
 constexpr int LF_AG = 8;
 
+//   (+ conj(vxc_g) rhoc_s with a core density, :855-856)
 __global__ __launch_bounds__(256) void k_vtemp(int ngv, int nsp, const double* __restrict__ vps,
                                                const double* __restrict__ rhops,
                                                const double* __restrict__ g2i,
                                                const double* __restrict__ rhogt,
                                                const double* __restrict__ rhoelg,
+                                               const double* __restrict__ vxcg,
+                                               const double* __restrict__ rhoc,
                                                double* __restrict__ vt) {
   const int ig = blockIdx.x * 256 + threadIdx.x;
   if (ig >= ngv) return;
   const double fpi = 4.0 * M_PI;
   const double gtr = rhogt[2 * ig], gti = rhogt[2 * ig + 1];
   const double elr = rhoelg[2 * ig], eli = rhoelg[2 * ig + 1];
+  const double vxr = vxcg ? vxcg[2 * ig] : 0.0, vxi = vxcg ? vxcg[2 * ig + 1] : 0.0;
   for (int is = 0; is < nsp; is++) {
     const long o = (long)is * ngv + ig;
     const double tmp = fpi * rhops[o] * g2i[ig];
     // conj: (x + iy)* = x - iy
-    vt[2 * o] = tmp * gtr + vps[o] * elr;
-    vt[2 * o + 1] = -tmp * gti - vps[o] * eli;
+    double re = tmp * gtr + vps[o] * elr;
+    double im = -tmp * gti - vps[o] * eli;
+    if (vxcg) {
+      re += vxr * rhoc[o];
+      im -= vxi * rhoc[o];
+    }
+    vt[2 * o] = re;
+    vt[2 * o + 1] = im;
   }
 }
 
@@ -490,10 +551,11 @@ void launch_local_forces(const int32_t* idx, int ngv, const double* tab, int kra
                          const int* kmin, const int32_t* grp, int ngrp, int nsp,
                          const double* vps, const double* rhops, const double* g2i,
                          const double* gx, const double* rhogt, const double* rhoelg,
+                         const double* vxcg, const double* rhoc,
                          double omega, const double* fesr, int nat, double* vt, double* partial,
                          int nchunk, double* fion, hipStream_t s) {
   hipLaunchKernelGGL(k_vtemp, dim3((ngv + 255) / 256), dim3(256), 0, s, ngv, nsp, vps, rhops, g2i,
-                     rhogt, rhoelg, vt);
+                     rhogt, rhoelg, vxcg, rhoc, vt);
   const int chunk = (ngv + nchunk - 1) / nchunk;
   const size_t lds = sizeof(double) * (size_t)local_forces_group_size(krange) * 3 * krange * 2;
   hipLaunchKernelGGL(k_local_forces, dim3(nchunk, ngrp), dim3(256), lds, s, idx, ngv, tab,
@@ -821,11 +883,14 @@ template void launch_nl_stress_y<float>(const float*, long, int, const double*, 
 
 // ------------------------------------------- Hartree pseudocharge stress
 // The species part of sigma_ehart (src/EnergyFunctional.cpp:649-676):
-// sum_s 2 (rcps_s^2 / 4) Re(conj(rhogt) sfac_s) rhops_s g2i G_i G_j.
+// sum_s 2 (rcps_s^2 / 4) Re(conj(rhogt) sfac_s) rhops_s g2i G_i G_j, and with
+// a core density - sum_s Re(conj(vxc_g) sfac_s) drhoc_s G_i G_j, where
+// drhoc_s = drhocoreg gi / (4 pi omega) (:650-659).
 __global__ __launch_bounds__(256) void k_hart_ps_stress(
     const double* __restrict__ rhogt, const double* __restrict__ sfac,
     const double* __restrict__ rhops, const double* __restrict__ g2i,
     const double* __restrict__ gx, int ngv, int nsp, const double* __restrict__ fac2,
+    const double* __restrict__ vxcg, const double* __restrict__ drhoc,
     double* __restrict__ partial) {
   const int ig = blockIdx.x * 256 + threadIdx.x;
   double v[6] = {0, 0, 0, 0, 0, 0};
@@ -837,6 +902,13 @@ __global__ __launch_bounds__(256) void k_hart_ps_stress(
       t += fac2[is] * (tr * sf[2 * ig] + ti * sf[2 * ig + 1]) * rhops[(long)is * ngv + ig];
     }
     t *= g2i[ig];
+    if (vxcg) {
+      const double vr = vxcg[2 * ig], vi = vxcg[2 * ig + 1];
+      for (int is = 0; is < nsp; is++) {
+        const double* sf = sfac + 2 * (long)is * ngv;
+        t -= (vr * sf[2 * ig] + vi * sf[2 * ig + 1]) * drhoc[(long)is * ngv + ig];
+      }
+    }
     const double x = gx[ig], y = gx[ngv + ig], z = gx[2 * ngv + ig];
     v[0] = t * x * x; v[1] = t * y * y; v[2] = t * z * z;
     v[3] = t * x * y; v[4] = t * y * z; v[5] = t * x * z;
@@ -846,10 +918,11 @@ __global__ __launch_bounds__(256) void k_hart_ps_stress(
 
 void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double* rhops,
                            const double* g2i, const double* gx, int ngv, int nsp,
-                           const double* fac2, double* partial, double* out6, hipStream_t s) {
+                           const double* fac2, const double* vxcg, const double* drhoc,
+                           double* partial, double* out6, hipStream_t s) {
   const int nb = (ngv + 255) / 256;
   hipLaunchKernelGGL(k_hart_ps_stress, dim3(nb), dim3(256), 0, s, rhogt, sfac, rhops, g2i, gx,
-                     ngv, nsp, fac2, partial);
+                     ngv, nsp, fac2, vxcg, drhoc, partial);
   reduce_partials(partial, nb, 6, out6, s);
 }
 

@@ -321,6 +321,23 @@ bool Species::init() {
   for (int i = 1; i < ndft; i++) { gspl[i] = i * dg; vlocg[i] /= gspl[i]; }
   vlocg2.resize(ndft);
   spline(ndft, gspl.data(), vlocg.data(), false, vlocg2.data());
+  // core density (Species::initialize_nlcc, src/Species.cpp:1056-1100, and
+  // :698-717): (4 pi / G) sum_r sin(Gr) r dr nlcc(r); G = 0: the Simpson
+  // integral of 4 pi r^2 nlcc(r)
+  nlccg.clear();
+  nlccg2.clear();
+  zcore = 0.0;
+  if (has_nlcc()) {
+    nlccg.assign(ndft, 0.0);
+    for (int i = 0; i < ndft && i < (int)nlcc.size(); i++) nlccg[i] = fpi * r[i] * deltar * nlcc[i];
+    for (int i = 0; i < ndft; i++) fint[i] = nlccg[i] * r[i];
+    zcore = simpsn(ndft, fint.data());
+    sinft(ndft, nlccg.data());
+    nlccg[0] = zcore;
+    for (int i = 1; i < ndft; i++) nlccg[i] /= gspl[i];
+    nlccg2.resize(ndft);
+    spline(ndft, gspl.data(), nlccg.data(), false, nlccg2.data());
+  }
   vnlg.assign(lmax + 1, std::vector<double>(ndft + 1, 0.0));
   vnlg2.assign(lmax + 1, std::vector<double>(ndft + 1, 0.0));
   for (int l = 0; l <= lmax; l++) {
@@ -406,6 +423,11 @@ void Species::dvlocg(double g, double& v, double& dv) const {
 void Species::dvnlg(int l, double g, double& v, double& dv) const {
   if (g > gspl[ndft - 1]) { v = dv = 0.0; return; }
   splintd(ndft, gspl.data(), vnlg[l].data(), vnlg2[l].data(), g, v, dv);
+}
+
+void Species::drhocoreg(double g, double& rho, double& drho) const {
+  if (!has_nlcc() || g > gspl[ndft - 1]) { rho = drho = 0.0; return; }
+  splintd(ndft, gspl.data(), nlccg.data(), nlccg2.data(), g, rho, drho);
 }
 
 double Species::rhopsg(double g) const {

@@ -40,12 +40,18 @@ struct Species {
   int zval = 0, lmax = 0, nmesh = 0;
   double deltar = 0, rcps = 0;
   std::vector<double> vloc, proj, dl;           // inputs
+  std::vector<double> nlcc;                     // core density on the mesh, or empty
   int ndft = 0, nlm = 0;
   std::vector<double> gspl, vlocg, vlocg2;
   std::vector<std::vector<double>> vnlg, vnlg2;
+  std::vector<double> nlccg, nlccg2;            // core density transform, spline
+  double zcore = 0;
+  bool has_nlcc() const { return !nlcc.empty(); }
   bool init();
   void dvlocg(double g, double& v, double& dv) const;
   void dvnlg(int l, double g, double& v, double& dv) const;
+  // core density transform and its derivative (src/Species.cpp:1194-1220)
+  void drhocoreg(double g, double& rho, double& drho) const;
   double rhopsg(double g) const;
   double eself() const;
 };

@@ -116,7 +116,8 @@ class SpeciesDesc(C.Structure):
     _fields_ = [("zval", C.c_int), ("lmax", C.c_int), ("nmesh", C.c_int),
                 ("deltar", C.c_double), ("rcps", C.c_double),
                 ("vloc", _dp), ("proj", _dp), ("dl", _dp),
-                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp)]
+                ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp),
+                ("nlcc", _dp)]
 
 
 def species_desc(sp, keep):
@@ -124,6 +125,10 @@ def species_desc(sp, keep):
     projectors) or of a species.NcppSpecies (semilocal potentials and
     pseudo-wavefunctions, turned into Kleinman-Bylander projectors by the
     library)."""
+    core = sp.core_density()
+    if core is not None:
+        core = np.ascontiguousarray(core, dtype=np.float64)
+        keep.append(core)
     if getattr(sp, "kind", "") == "ncpp":
         if sp.nquad != 0:
             raise ValueError("semilocal quadrature species (nquad > 0) are not supported")
@@ -131,11 +136,11 @@ def species_desc(sp, keep):
         phi = np.ascontiguousarray(sp.phi, dtype=np.float64)
         keep += [vps, phi]
         return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
-                           1, sp.llocal, _pd(vps), _pd(phi))
+                           1, sp.llocal, _pd(vps), _pd(phi), _pd(core))
     vloc, proj, dl = sp.radial()
     keep += [vloc, proj, dl]
     return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, _pd(vloc), _pd(proj),
-                       _pd(dl), 0, 0, None, None)
+                       _pd(dl), 0, 0, None, None, _pd(core))
 
 _lib = None
 
@@ -400,7 +405,7 @@ class _BorrowedGrid(Grid):
 
 
 def _pd(a):
-    return a.ctypes.data_as(_dp)
+    return None if a is None else a.ctypes.data_as(_dp)
 
 
 class Profiler:
@@ -587,10 +592,11 @@ class EnergyFunctional:
     def get(self, what):
         n = {0: self.N, 1: self.N, 2: self.N, 3: 2 * self.ngv, 4: 2 * self.ngv,
              5: 2 * self.ngv, 6: 2 * len(self.sysdef.species) * self.ngv,
-             7: 3 * self.nat, 8: self.N, 9: self.N, 10: self.N}[what]
+             7: 3 * self.nat, 8: self.N, 9: self.N, 10: self.N, 11: self.N,
+             12: 2 * self.ngv}[what]
         out = np.zeros(n)
         self.ctx.check(self.ctx.lib.qbg_ef_get(self.h, what, _pd(out)))
-        if what in (3, 4, 5, 6):
+        if what in (3, 4, 5, 6, 12):
             return out.view(np.complex128)
         return out
 

@@ -27,7 +27,7 @@ def relerr(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.fixture(scope="module", params=[10, 11, 12, 13, 0, 1])
+@pytest.fixture(scope="module", params=[10, 11, 12, 13, 15, 0, 1])
 def system(request, ctx, oracle_mod):
     import qbg
     idx = request.param
@@ -73,6 +73,9 @@ def test_evaluate_fp64(system, ctx):
     assert relerr(ef.get(1), o.get(1)) < 1e-10          # v_r
     assert relerr(hp.download(), ref["hpsi"]) < 1e-10
     assert relerr(got["fion"], ref["fion"]) < 1e-10
+    if any(sp.core_density() is not None for sp in s.species):
+        assert relerr(ef.get(11), o.get(17)) < 1e-10     # core density on the grid
+        assert relerr(ef.get(12), o.get(18)) < 1e-10     # vxc_g
 
 
 # mixed precision (FP32 band FFTs; projector GEMMs on FP32 MFMA or as

@@ -152,6 +152,39 @@ def test_f_projectors_evaluate(ctx, oracle_mod, prec):
     ef.release()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
+def test_core_density_ncpp(ctx, oracle_mod, prec):
+    """Si (norm-conserving, Kleinman-Bylander) with a core density
+    0.3 exp(-(r/1.1)^2) on its mesh in the si4gs cell against the oracle:
+    the non-linear core correction of a norm-conserving species
+    (src/Species.cpp:194, 1056-1100; src/EnergyFunctional.cpp:275-306,
+    650-659, 855-856, 1064-1083), energies, forces, H*psi and stress."""
+    import qbg
+    si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
+    r = np.arange(si.nmesh) * si.deltar
+    s = cf.si4gs(dataclasses.replace(si, nlcc=0.3 * np.exp(-(r / 1.1) ** 2)))
+    print(f"[case] si4gs with a core density {prec}", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(15, o.basis(0)["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        if prec == "fp64":
+            assert abs(got[t] - ref[t]) <= 1e-10 * max(abs(ref[t]), 1e-12), (t, got[t], ref[t])
+        else:
+            assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    ftol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.abs(got["fion"] - ref["fion"]).max() <= ftol * max(1.0, np.abs(ref["fion"]).max())
+    assert relerr(hp.download(), ref["hpsi"]) < ftol
+    assert np.abs(got["sigma"] - ref["sigma"]).max() < ftol * np.abs(ref["sigma"][0]).max()
+    if prec == "fp64":
+        assert relerr(ef.get(11), o.get(17)) < 1e-10
+    ef.release()
+
+
 def test_f_channel_refused(ctx):
     """A nonlocal f channel (l = 3) of a norm-conserving potential is
     refused: the reference's Kleinman-Bylander construction

@@ -35,11 +35,13 @@ def spin_coeffs(s, g2, seed):
     return np.concatenate([cf.gram_numpy(c[:u]), cf.gram_numpy(c[u:])])
 
 
-SYSTEMS = {"t1": (11, 9, 1, 1), "t2": (12, 11, 1, 0), "c2": (1, 255, 1, 2)}
+# t5: core densities (each spin carries half the core density)
+SYSTEMS = {"t1": (11, 9, 1, 1), "t2": (12, 11, 1, 0), "c2": (1, 255, 1, 2),
+           "t5": (15, 12, 1, 1)}
 
 
 @pytest.mark.parametrize("name,prec", [("t1", "fp64"), ("t2", "fp64"), ("t1", "bf16x2"),
-                                       ("c2", "bf16x2")])
+                                       ("c2", "bf16x2"), ("t5", "fp64"), ("t5", "bf16x2")])
 def test_spin_parity(ctx, oracle_mod, name, prec):
     import qbg
     idx, nel, ds, ne = SYSTEMS[name]
@@ -62,6 +64,9 @@ def test_spin_parity(ctx, oracle_mod, name, prec):
         assert np.abs(got["sigma"] - ref["sigma"]).max() / scale < 1e-10
         for dev, orc in ((0, 0), (8, 14), (1, 1), (9, 15), (2, 2), (10, 16)):
             assert relerr(ef.get(dev), o.get(orc)) < 1e-10, (dev, orc)
+        if name == "t5":
+            assert relerr(ef.get(11), o.get(17)) < 1e-10
+            assert relerr(ef.get(12), o.get(18)) < 1e-10
     else:
         for t in TERMS:
             assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])

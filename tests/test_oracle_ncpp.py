@@ -54,6 +54,22 @@ def test_read_species(si, oxy, hyd):
     assert abs(si.vps[2][-1] * r[-1] + si.zval) < 1e-2
 
 
+def test_core_density_xml_round_trip(si):
+    """A <core_density> element (src/SpeciesHandler.cpp:214-220) is read,
+    written where Species::print puts it (src/Species.cpp:1256, 1310-1316)
+    and read back unchanged; species without one have none."""
+    import dataclasses
+    from qbg.species import parse_species, species_xml
+    assert si.core_density() is None
+    r = np.arange(si.nmesh) * si.deltar
+    core = 0.3 * np.exp(-(r / 1.1) ** 2)
+    text = species_xml(dataclasses.replace(si, nlcc=core), "si_core")
+    assert text.index("<core_density") < text.index("<projector")
+    back = parse_species(text)
+    assert np.array_equal(back.core_density(), core)
+    assert np.array_equal(back.vps, si.vps) and np.array_equal(back.phi, si.phi)
+
+
 @pytest.mark.parametrize("name", ["si", "oxy"])
 def test_kb_weights_and_identity(oracle_mod, name, request):
     sp = request.getfixturevalue(name)
