@@ -3,6 +3,7 @@
 #include "qbo.h"
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 
 using namespace qbo;
@@ -23,6 +24,8 @@ typedef struct {
   const double* vps;    // (lmax+1) * nmesh
   const double* phi;    // (lmax+1) * nmesh
   const double* nlcc;   // core density (nmesh) or null
+  int nquad;            // NCPP: semilocal quadrature form when > 0
+  double rquad;
 } qbo_species_desc;
 
 static std::string g_err;
@@ -42,7 +45,17 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
       Species q;
       q.zval = sp[is].zval; q.lmax = sp[is].lmax; q.deltar = sp[is].deltar;
       q.rcps = sp[is].rcps;
-      if (sp[is].type == 1) {
+      if (sp[is].type == 1 && sp[is].nquad > 0) {
+        // semilocal quadrature form: the semilocal potentials as given
+        const int nm = sp[is].nmesh;
+        q.nquad = sp[is].nquad; q.rquad = sp[is].rquad; q.llocal = sp[is].llocal;
+        if (q.llocal < 0 || q.llocal > q.lmax || !(q.rquad > 0.0))
+          throw std::runtime_error("semilocal quadrature species: llocal or rquad");
+        q.vsl.resize(q.lmax + 1);
+        for (int l = 0; l <= q.lmax; l++)
+          q.vsl[l].assign(sp[is].vps + (size_t)l * nm, sp[is].vps + (size_t)(l + 1) * nm);
+        q.vlocal = q.vsl[q.llocal];
+      } else if (sp[is].type == 1) {
         ncpp_species(q, sp[is].lmax, sp[is].llocal, sp[is].nmesh, sp[is].vps, sp[is].phi);
       } else {
         q.vlocal.assign(sp[is].vloc, sp[is].vloc + sp[is].nmesh);

@@ -101,14 +101,25 @@ void System::update_twnl() {
     int ilm = 0;
     for (int iop = 0; iop < s.nop; iop++) {
       const int l = iop;
+      if (s.quad() && l == s.llocal) continue;   // no projectors for the local channel
+      // one radial function per channel, or one per quadrature point; the
+      // projector of (m, src) is ilm + m, or src + nquad (ilm + m) in the
+      // quadrature form (src/NonLocalPotential.cpp:196-229)
+      const int nsrc = s.quad() ? s.nquad : 1;
+      for (int src = 0; src < nsrc; src++) {
+      auto col = [&](int m) { return s.quad() ? src + s.nquad * (ilm + m) : ilm + m; };
+      auto radial = [&](double g, double& v, double& dv) {
+        if (s.quad()) s.quad_radial(l, src, g, v, dv);
+        else s.dvnlg(iop, g, v, dv);
+      };
       if (l == 0) {
-        lproj[is][ilm] = 0;
-        double* t0 = &twnl[is][(size_t)ngw * ilm];
-        double* dt = &dtwnl[is][(size_t)ngw * 6 * ilm];
+        lproj[is][col(0)] = 0;
+        double* t0 = &twnl[is][(size_t)ngw * col(0)];
+        double* dt = &dtwnl[is][(size_t)ngw * 6 * col(0)];
 #pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ngw; ig++) {
           double v, dv;
-          s.dvnlg(iop, kpg[ig], v, dv);
+          radial(kpg[ig], v, dv);
           t0[ig] = s14pi * v;
           const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
           const double tmp = kpgi[ig] * s14pi * dv;
@@ -120,18 +131,18 @@ void System::update_twnl() {
           dt[5 * ngw + ig] = tmp * tgx * tgz;
         }
       } else if (l == 1) {
-        for (int m = 0; m < 3; m++) lproj[is][ilm + m] = 1;
-        double* t1 = &twnl[is][(size_t)ngw * ilm];
-        double* t2 = &twnl[is][(size_t)ngw * (ilm + 1)];
-        double* t3 = &twnl[is][(size_t)ngw * (ilm + 2)];
-        double* d1 = &dtwnl[is][(size_t)ngw * 6 * ilm];
-        double* d2 = &dtwnl[is][(size_t)ngw * 6 * (ilm + 1)];
-        double* d3 = &dtwnl[is][(size_t)ngw * 6 * (ilm + 2)];
+        for (int m = 0; m < 3; m++) lproj[is][col(m)] = 1;
+        double* t1 = &twnl[is][(size_t)ngw * col(0)];
+        double* t2 = &twnl[is][(size_t)ngw * col(1)];
+        double* t3 = &twnl[is][(size_t)ngw * col(2)];
+        double* d1 = &dtwnl[is][(size_t)ngw * 6 * col(0)];
+        double* d2 = &dtwnl[is][(size_t)ngw * 6 * col(1)];
+        double* d3 = &dtwnl[is][(size_t)ngw * 6 * col(2)];
 #pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ngw; ig++) {
           double v, dv;
           const double tg = kpg[ig];
-          s.dvnlg(iop, tg, v, dv);
+          radial(tg, v, dv);
           const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
           const double tgx2 = tgx * tgx, tgy2 = tgy * tgy, tgz2 = tgz * tgz;
           const double tgi = kpgi[ig], tgi2 = tgi * tgi;
@@ -173,18 +184,18 @@ void System::update_twnl() {
         const double s74pi = std::sqrt(7.0 / fpi), s32 = std::sqrt(1.5), s52 = std::sqrt(2.5),
                      s15 = std::sqrt(15.0);
         const double factor = 0.5 * s74pi;
-        for (int m = 0; m < 7; m++) lproj[is][ilm + m] = 3;
+        for (int m = 0; m < 7; m++) lproj[is][col(m)] = 3;
         double* t[7];
         double* d[7];
         for (int m = 0; m < 7; m++) {
-          t[m] = &twnl[is][(size_t)ngw * (ilm + m)];
-          d[m] = &dtwnl[is][(size_t)ngw * 6 * (ilm + m)];
+          t[m] = &twnl[is][(size_t)ngw * col(m)];
+          d[m] = &dtwnl[is][(size_t)ngw * 6 * col(m)];
         }
 #pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ngw; ig++) {
           double v, dv;
           const double tg = kpg[ig];
-          s.dvnlg(iop, tg, v, dv);
+          radial(tg, v, dv);
           const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
           const double tgi = kpgi[ig], tgi2 = tgi * tgi, tgi3 = tgi * tgi2;
           const double tgxx = tgx * tgx * tgi2, tgyy = tgy * tgy * tgi2, tgzz = tgz * tgz * tgi2;
@@ -248,18 +259,18 @@ void System::update_twnl() {
       } else {   // l == 2 (:627-760): y4..y8 and their strain derivatives
         const double s54pi = std::sqrt(5.0 / fpi), s3 = std::sqrt(3.0);
         const double s20pi = std::sqrt(20.0 * M_PI), s20pi3 = std::sqrt(20.0 * M_PI / 3.0);
-        for (int m = 0; m < 5; m++) lproj[is][ilm + m] = 2;
+        for (int m = 0; m < 5; m++) lproj[is][col(m)] = 2;
         double* t[5];
         double* d[5];
         for (int m = 0; m < 5; m++) {
-          t[m] = &twnl[is][(size_t)ngw * (ilm + m)];
-          d[m] = &dtwnl[is][(size_t)ngw * 6 * (ilm + m)];
+          t[m] = &twnl[is][(size_t)ngw * col(m)];
+          d[m] = &dtwnl[is][(size_t)ngw * 6 * col(m)];
         }
 #pragma omp parallel for schedule(static)
         for (int ig = 0; ig < ngw; ig++) {
           double v, dv;
           const double tg = kpg[ig];
-          s.dvnlg(iop, tg, v, dv);
+          radial(tg, v, dv);
           const double tgx = kx[ig], tgy = ky[ig], tgz = kz[ig];
           const double tgi = kpgi[ig], tgi2 = tgi * tgi;
           // unit-vector products, ij order xx yy zz xy yz xz
@@ -289,6 +300,7 @@ void System::update_twnl() {
           }
         }
       }
+      }   // src
       ilm += 2 * l + 1;
     }
   }

@@ -24,7 +24,7 @@ class SpeciesDesc(C.Structure):
                 ("deltar", C.c_double), ("rcps", C.c_double),
                 ("vloc", _dp), ("proj", _dp), ("dl", _dp),
                 ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp),
-                ("nlcc", _dp)]
+                ("nlcc", _dp), ("nquad", C.c_int), ("rquad", C.c_double)]
 
 
 def species_desc(sp, keep):
@@ -39,11 +39,12 @@ def species_desc(sp, keep):
         phi = np.ascontiguousarray(sp.phi, dtype=float)
         keep += [vps, phi]
         return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
-                           1, sp.llocal, _ptr(vps), _ptr(phi), _ptr(core))
+                           1, sp.llocal, _ptr(vps), _ptr(phi), _ptr(core), sp.nquad,
+                           sp.rquad)
     vloc, proj, dl = sp.radial()
     keep += [vloc, proj, dl]
     return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps,
-                       _ptr(vloc), _ptr(proj), _ptr(dl), 0, 0, None, None, _ptr(core))
+                       _ptr(vloc), _ptr(proj), _ptr(dl), 0, 0, None, None, _ptr(core), 0, 0.0)
 
 
 def _ptr(a, t=_dp):

@@ -83,6 +83,17 @@ struct Species {
   std::vector<std::vector<double>> proj;       // proj[l][r]
   std::vector<double> dl;                      // D_l (one projector per l)
   std::vector<double> nlcc;                    // core density on the mesh (NLCC), or empty
+  // semilocal quadrature form (norm-conserving, nquad > 0): the semilocal
+  // potentials vsl[l] on the mesh and the local channel; projectors
+  // 4 pi j_l(G r_q) r_q at r_q = (q+1) h, h = rquad/(nquad+1), weights
+  // (v_l(r_q) - v_loc(r_q)) h (src/NonLocalPotential.cpp:104-229, 317-960)
+  int nquad = 0, llocal = -1;
+  double rquad = 0.0;
+  std::vector<std::vector<double>> vsl;
+  std::vector<double> rq;                      // quadrature points
+  std::vector<std::vector<double>> wql;        // [l][q] projector weights
+  bool quad() const { return nquad > 0; }
+  void quad_radial(int l, int q, double g, double& v, double& dv) const;
   // derived
   int ndft = 0, nop = 0, nlm = 0;
   std::vector<double> rps, gspl, vlocg, vlocg2;

@@ -147,13 +147,24 @@ void Species::initialize() {
   nop = lmax + 1;
   nlm = 0;
   for (int l = 0; l <= lmax; l++) nlm += 2 * l + 1;
+  if (quad()) {
+    // projectors per nonlocal channel and quadrature point (the projector
+    // count npr of src/NonLocalPotential.cpp:93-100)
+    nlm = 0;
+    for (int l = 0; l <= lmax; l++)
+      if (l != llocal) {
+        if (l > 2) throw std::runtime_error("semilocal quadrature: nonlocal l > 2");
+        nlm += (2 * l + 1) * nquad;
+      }
+  }
   ndft = 1;
   while (ndft * deltar < 40.0) ndft *= 2;
   rps.resize(ndft);
   for (int i = 0; i < ndft; i++) rps[i] = i * deltar;
   std::vector<std::vector<double>> phi(nop, std::vector<double>(ndft, 0.0));
-  for (int iop = 0; iop < nop; iop++)
-    for (int i = 0; i < np; i++) phi[iop][i] = proj[iop][i];
+  if (!quad())
+    for (int iop = 0; iop < nop; iop++)
+      for (int i = 0; i < np; i++) phi[iop][i] = proj[iop][i];
   std::vector<double> vlocr(ndft), fint(ndft);
   for (int i = 0; i < np; i++) vlocr[i] = vlocal[i];
   for (int i = np; i < ndft; i++) vlocr[i] = -zval / rps[i];
@@ -197,6 +208,34 @@ void Species::initialize() {
 
   vnlg.assign(nop, std::vector<double>(ndft + 1, 0.0));
   vnlg2.assign(nop, std::vector<double>(ndft + 1, 0.0));
+  if (quad()) {
+    // semilocal potentials extended by -Z/r and splined with v'(0) = 0 and
+    // v'(r_max) = Z/r_max^2 (src/Species.cpp:162-184, dvpsr :1120-1131);
+    // trapezoidal points and weights (src/NonLocalPotential.cpp:119-131)
+    const double h = rquad / (nquad + 1);
+    rq.resize(nquad);
+    for (int q = 0; q < nquad; q++) rq[q] = (q + 1) * h;
+    wql.assign(lmax + 1, std::vector<double>(nquad, 0.0));
+    std::vector<std::vector<double>> vs(lmax + 1, std::vector<double>(ndft)),
+        vs2(lmax + 1, std::vector<double>(ndft));
+    const double dvdr = zval / (rps[ndft - 1] * rps[ndft - 1]);
+    for (int l = 0; l <= lmax; l++) {
+      for (int i = 0; i < ndft; i++)
+        vs[l][i] = i < (int)vsl[l].size() ? vsl[l][i] : -zval / rps[i];
+      spline(ndft, rps.data(), vs[l].data(), 0.0, dvdr, 0, 0, vs2[l].data());
+    }
+    auto vpsr = [&](int l, double r) {
+      if (r > rps[ndft - 1]) return 0.0;
+      double v, dv;
+      splintd(ndft, rps.data(), vs[l].data(), vs2[l].data(), r, &v, &dv);
+      return v;
+    };
+    for (int l = 0; l <= lmax; l++) {
+      if (l == llocal) continue;
+      for (int q = 0; q < nquad; q++) wql[l][q] = (vpsr(l, rq[q]) - vpsr(llocal, rq[q])) * h;
+    }
+    return;
+  }
   for (int iop = 0; iop < nop; iop++) {
     std::vector<double> vnlr(ndft);
     for (int i = 0; i < ndft; i++) vnlr[i] = fpi * deltar * phi[iop][i];
@@ -280,6 +319,36 @@ void Species::dvnlg(int iop, double g, double& v, double& dv) const {
   splintd(ndft, gspl.data(), vnlg[iop].data(), vnlg2[iop].data(), g, &v, &dv);
 }
 
+// 4 pi j_l(G r_q) r_q and its G derivative, the expressions of
+// src/NonLocalPotential.cpp:317-420 (l = 0, with the G = 0 limit 4 pi r),
+// :560-580 (l = 1) and :838-855 (l = 2)
+void Species::quad_radial(int l, int q, double g, double& v, double& dv) const {
+  const double fpi = 4.0 * M_PI, r = rq[q], z = g * r;
+  v = 0.0;
+  dv = 0.0;
+  if (l == 0) {
+    if (g == 0.0) { v = fpi * r; return; }
+    const double gi = 1.0 / g, s = std::sin(z), c = std::cos(z);
+    v = fpi * s * gi;
+    dv = fpi * gi * (r * c - s * gi);
+    return;
+  }
+  if (z == 0.0) return;
+  const double zi = 1.0 / z, c = std::cos(z), s = std::sin(z);
+  if (l == 1) {
+    const double j1 = (s * zi - c) * zi;
+    const double dj1 = (2.0 * z * c + (z * z - 2.0) * s) * zi * zi * zi;
+    v = fpi * j1 * r;
+    dv = fpi * dj1 * r * r;
+  } else {
+    const double j2 = ((3.0 * zi * zi - 1.0) * s - 3.0 * zi * c) * zi;
+    const double z2 = z * z;
+    const double dj2 = ((4.0 * z2 - 9.0) * s + z * (9.0 - z2) * c) / (z2 * z2);
+    v = fpi * j2 * r;
+    dv = fpi * dj2 * r * r;
+  }
+}
+
 void Species::drhocoreg(double g, double& rho, double& drho) const {
   if (!has_nlcc() || g > gspl[ndft - 1]) { rho = 0.0; drho = 0.0; return; }
   splintd(ndft, gspl.data(), nlccg.data(), nlccg2.data(), g, &rho, &drho);
@@ -297,6 +366,17 @@ double Species::eself() const {
 
 // src/Species.cpp:1417-1464: nonzero only on the (l,m) diagonal
 double Species::dmatrix(int ipr, int jpr) const {
+  if (quad()) {
+    if (ipr != jpr) return 0.0;
+    const int q = ipr % nquad, ilm = ipr / nquad;
+    int j = 0;
+    for (int l = 0; l <= lmax; l++) {
+      if (l == llocal) continue;
+      if (ilm < j + 2 * l + 1) return wql[l][q];
+      j += 2 * l + 1;
+    }
+    return 0.0;
+  }
   auto lm = [&](int ipr_, int& l, int& m) {
     int j = 0;
     for (int ll = 0; ll <= lmax; ll++)

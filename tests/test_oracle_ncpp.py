@@ -189,3 +189,69 @@ def test_si4_scf_is_variational(oracle_mod, si):
     e0 = o.energy(c, hpsi=False, forces=False)["etotal"]
     for t in range(2):
         assert perturbed_energy(oracle_mod, o, c, 1e-2, t) > e0
+
+
+# ----------------------------------------- semilocal quadrature (nquad > 0)
+def _quad_si4gs(si, llocal):
+    import dataclasses
+    return cf.si4gs(dataclasses.replace(si, nquad=16, rquad=2.6, llocal=llocal))
+
+
+@pytest.mark.parametrize("llocal", [2, 0])
+def test_quadrature_forces_and_hpsi_are_gradients(oracle_mod, si, llocal):
+    """The quadrature projectors (src/NonLocalPotential.cpp:317-960) in a
+    whole evaluation: forces are -dE/dtau and H*psi is dE/dc* by central
+    differences (p and s, or p and d channels nonlocal)."""
+    import dataclasses
+    s = _quad_si4gs(si, llocal)
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(17, o.basis(0)["g2"], s.nst))
+    r = o.evaluate(c, hpsi=True, forces=True)
+    h = 1e-4
+    for a, j in ((0, 0), (1, 1), (3, 2)):
+        e = []
+        for sgn in (1.0, -1.0):
+            t = np.array(s.tau, dtype=float)
+            t[a, j] += sgn * h
+            e.append(oracle_mod.Oracle(dataclasses.replace(s, tau=t)).evaluate(
+                c, hpsi=False, forces=False)["etotal"])
+        fd = -(e[0] - e[1]) / (2 * h)
+        print(f"  f[{a},{j}] fd {fd:.10e} oracle {r['fion'][a, j]:.10e}")
+        assert abs(fd - r["fion"][a, j]) <= 1e-6 * max(1.0, abs(fd))
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    d[:, 0] = d[:, 0].real
+    d *= 1e-4 / np.abs(d).max()
+    full = lambda x, y: 2.0 * np.real(np.vdot(x, y)) - np.real(np.conj(x[0]) * y[0])
+    pred = sum(s.occ[n] * 2.0 * full(d[n], r["hpsi"][n]) for n in range(s.nst))
+    ep = o.evaluate(c + d, hpsi=False, forces=False)["etotal"]
+    em = o.evaluate(c - d, hpsi=False, forces=False)["etotal"]
+    assert abs((ep - em) / 2 - pred) <= 1e-6 * abs(pred)
+
+
+def test_quadrature_stress_vs_strain_fd(oracle_mod, si):
+    """sigma_enl and the total stress of a quadrature species against
+    Richardson-extrapolated central strain differences (the method of
+    test/si2stress/check_stress.sh:20-90)."""
+    from test_oracle_stress import strained, sym
+    s = _quad_si4gs(si, 0)
+    ref = s.cell * 1.04
+    o0 = oracle_mod.Oracle(s, refcell=ref)
+    c = cf.gram_numpy(cf.random_coefficients(19, o0.basis(0)["g2"], s.nst))
+    sig = o0.evaluate(c, hpsi=False, forces=False, stress=True)["sigma"]
+    omega = abs(np.linalg.det(s.cell))
+    v = 1e-4 * np.random.default_rng(5).uniform(-1.0, 1.0, 6)
+
+    def diff(u):
+        ep = oracle_mod.Oracle(strained(s, u), refcell=ref).evaluate(c, hpsi=False, forces=False)
+        em = oracle_mod.Oracle(strained(s, -u), refcell=ref).evaluate(c, hpsi=False, forces=False)
+        return {t: ep[t] - em[t] for t in ("enl", "etotal")}
+
+    d1, d2 = diff(sym(v)), diff(sym(0.5 * v))
+    for term, row in (("enl", 3), ("etotal", 0)):
+        de = (8.0 * d2[term] - d1[term]) / 3.0
+        st = sig[row]
+        pred = -2.0 * omega * (sum(v[k] * st[k] for k in range(3)) +
+                               2.0 * sum(v[k] * st[k] for k in range(3, 6)))
+        print(f"  {term}: fd {de:.10e} sigma {pred:.10e}")
+        assert abs(de - pred) <= 2e-7 * abs(de) + 1e-12

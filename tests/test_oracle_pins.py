@@ -264,6 +264,57 @@ def test_nonlocal_energy_single_atom(oracle_mod, sp, kind):
     assert abs(enl - ref) < 1e-6 * abs(ref), (enl, ref)   # spline accuracy of twnl
 
 
+@pytest.mark.parametrize("kind,llocal", [("s", 2), ("p", 2), ("p", 0), ("d", 0)])
+def test_semilocal_quadrature_single_atom(oracle_mod, kind, llocal):
+    """Semilocal quadrature form (nquad > 0): projectors 4 pi j_l(G r_q) r_q
+    Y_lm at the trapezoid points r_q = (q+1) rquad/(nquad+1) with weights
+    (v_l(r_q) - v_loc(r_q)) h (src/NonLocalPotential.cpp:104-229, 317-960).
+    For one atom and a Gaussian orbital psi = R(r) Y_lm the projection onto
+    the shell r_q is r_q R(r_q), so enl = occ sum_q h dv_l(r_q) r_q^2 R(r_q)^2,
+    evaluated here with scipy's clamped cubic spline of the semilocal
+    potentials (v'(0) = 0, v'(r_max) = Z/r_max^2, src/Species.cpp:162-184)."""
+    import dataclasses
+    from pathlib import Path
+    from scipy.interpolate import CubicSpline
+    from qbg.species import read_species
+    si = read_species(Path(__file__).resolve().parent / "golden" / "Si_VBC_LDA-1.0.xml")
+    nquad, rquad = 24, 2.6
+    sp = dataclasses.replace(si, nquad=nquad, rquad=rquad, llocal=llocal)
+    L, a = 22.0, 1.1
+    s = cf.SystemDef("one atom, one orbital", cf.cubic(L), 50.0, [sp], [1], 1, 1, 0.0,
+                     tau=np.zeros((1, 3)), occ_bands=np.array([2.0]))
+    o = oracle_mod.Oracle(s)
+    c = _gaussian_orbital(o.basis(0), L ** 3, a, kind)
+    enl = o.nl_energy(c, None, forces=False, stress=False)[0]
+    l = {"s": 0, "p": 1, "d": 2}[kind]
+    ndft = 1
+    while ndft * sp.deltar < 40.0:
+        ndft *= 2
+    r = np.arange(ndft) * sp.deltar
+
+    def vspl(ll):
+        v = np.empty(ndft)
+        v[:sp.nmesh] = sp.vps[ll]
+        v[sp.nmesh:] = -sp.zval / r[sp.nmesh:]
+        return CubicSpline(r, v, bc_type=((1, 0.0), (1, sp.zval / r[-1] ** 2)))
+
+    h = rquad / (nquad + 1)
+    rq = h * np.arange(1, nquad + 1)
+    dv = vspl(l)(rq) - vspl(llocal)(rq)
+    g = np.exp(-rq ** 2 / (2 * a * a))
+    if kind == "s":
+        R = math.sqrt(4 * math.pi) * (math.pi * a * a) ** -0.75 * g
+    elif kind == "p":
+        R = (math.sqrt(4 * math.pi / 3) * ((a * a / 2) * (math.pi * a * a) ** 1.5) ** -0.5
+             * rq * g)
+    else:
+        R = (math.sqrt(4 * math.pi / 15) * ((a * a / 2) ** 2 * (math.pi * a * a) ** 1.5) ** -0.5
+             * rq * rq * g)
+    ref = 2.0 * np.sum(h * dv * rq * rq * R * R)
+    print(f"  {kind} (llocal {llocal}): enl {enl:.14e} quadrature sum {ref:.14e}")
+    assert abs(enl - ref) < 1e-9 * abs(ref)
+
+
 def _ewald(cell, tau, z, eta):
     """Ewald energy of point charges z at tau in a neutralizing background."""
     A = np.asarray(cell, float)
