@@ -197,7 +197,11 @@ size_t qbg_gram_noalloc_bytes(void);
  *     src/Species.cpp:71-507): semilocal potentials vps[(lmax+1)*nmesh]
  *     and pseudo-wavefunctions phi[(lmax+1)*nmesh] (the row of the local
  *     channel llocal may hold anything); lmax <= 3, every nonlocal channel
- *     l != llocal must have l <= 2.
+ *     l != llocal must have l <= 2.  With nquad > 0 the semilocal
+ *     quadrature form instead (src/NonLocalPotential.cpp:104-229, 317-960):
+ *     phi is not used; each nonlocal channel l <= 2 gets (2l+1) nquad
+ *     projectors 4 pi j_l(G r_q) r_q Y_lm at r_q = (q+1) rquad/(nquad+1),
+ *     weighted by (v_l(r_q) - v_llocal(r_q)) rquad/(nquad+1).
  *   nlcc (either type, may be NULL): the core density <core_density> on the
  *     same mesh (nmesh values) for the non-linear core correction
  *     (Species::initialize_nlcc, src/Species.cpp:1056-1100): the
@@ -222,6 +226,8 @@ typedef struct {
   const double* vps;
   const double* phi;
   const double* nlcc;
+  int nquad;
+  double rquad;
 } qbg_species_desc;
 
 /* EnergyFunctional(Sample&, ChargeDensity&) for one Gamma-point system

@@ -208,13 +208,12 @@ static int ef_core_density(qbg_ef* e) {
   return QBG_OK;
 }
 
-// real spherical harmonic of projector ipr of a species (one projector per
-// l: ipr 0 s; 1-3 p as x, y, z; 4-8 d as 3z^2-1, x^2-y^2, xy, yz, xz; 9-15 f
-// as y(3x^2-y^2), xyz, y(4z^2-x^2-y^2), z(2z^2-3x^2-3y^2), x(4z^2-x^2-y^2),
-// z(x^2-y^2), x(x^2-3y^2), the order and normalization of
+// real spherical harmonic of index ipr = l^2 + m (0 s; 1-3 p as x, y, z;
+// 4-8 d as 3z^2-1, x^2-y^2, xy, yz, xz; 9-15 f as y(3x^2-y^2), xyz,
+// y(4z^2-x^2-y^2), z(2z^2-3x^2-3y^2), x(4z^2-x^2-y^2), z(x^2-y^2),
+// x(x^2-3y^2), the order and normalization of
 // src/NonLocalPotential.cpp:250-1245) at the unit vector u: value and
 // gradient with respect to u (as a polynomial in u)
-static int proj_l(int ipr) { return ipr == 0 ? 0 : (ipr < 4 ? 1 : (ipr < 9 ? 2 : 3)); }
 static void harmonic(int ipr, const double* u, double& y, double* dy) {
   const double s14pi = std::sqrt(1.0 / (4.0 * M_PI)), s34pi = std::sqrt(3.0 / (4.0 * M_PI));
   const double d5 = std::sqrt(5.0 / (4.0 * M_PI)), d53 = d5 * std::sqrt(3.0);
@@ -282,28 +281,43 @@ static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<doub
   for (int is = 0; is < e->nsp; is++) {
     const host::Species& q = e->sp[is];
     const int npr = q.nlm;
+    std::vector<int> pl, ph, pq;
+    std::vector<double> pd;
+    q.projectors(pl, ph, pq, pd);
     const size_t toff = twnl.size(), doff = dtwnl.size();
     twnl.resize(toff + (size_t)npr * ngw);
     dtwnl.resize(doff + (size_t)npr * 6 * ngw);
     double* t = &twnl[toff];
     double* d = &dtwnl[doff];
+    // radial functions per G: one per channel, or one per (channel,
+    // quadrature point) in the quadrature form
+    const int nq = q.quad() ? q.nquad : 1;
+    std::vector<double> rv((size_t)(q.lmax + 1) * nq), rdv((size_t)(q.lmax + 1) * nq);
     for (int ig = 0; ig < ngw; ig++) {
       const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
       const double u[3] = {e->wb.gx[ig] * tgi, e->wb.gx[ngw + ig] * tgi,
                            e->wb.gx[2 * ngw + ig] * tgi};
-      double v[4], dv[4];
-      for (int l = 0; l <= q.lmax; l++) q.dvnlg(l, tg, v[l], dv[l]);
+      for (int l = 0; l <= q.lmax; l++) {
+        if (!q.quad()) {
+          q.dvnlg(l, tg, rv[l], rdv[l]);
+        } else if (l != q.llocal) {
+          for (int k = 0; k < nq; k++)
+            q.quad_radial(l, k, tg, rv[(size_t)l * nq + k], rdv[(size_t)l * nq + k]);
+        }
+      }
       for (int ipr = 0; ipr < npr; ipr++) {
-        const int l = proj_l(ipr);
+        const int l = pl[ipr];
+        const size_t ri = (size_t)l * nq + (pq[ipr] < 0 ? 0 : pq[ipr]);
+        const double v = rv[ri], dv = rdv[ri];
         double y, dy[3];
-        harmonic(ipr, u, y, dy);
-        t[(size_t)ipr * ngw + ig] = y * v[l];
+        harmonic(ph[ipr], u, y, dy);
+        t[(size_t)ipr * ngw + ig] = y * v;
         for (int ij = 0; ij < 6; ij++) {
           const int a = IJ[ij][0], b = IJ[ij][1];
           double sum = 0.0;
           for (int c = 0; c < 3; c++)
             if (dy[c] != 0.0) sum += dy[c] * (u[c] * u[a] * u[b] - (c == b ? u[a] : 0.0));
-          d[((size_t)ipr * 6 + ij) * ngw + ig] = y * dv[l] * tg * u[a] * u[b] - v[l] * sum;
+          d[((size_t)ipr * 6 + ij) * ngw + ig] = y * dv * tg * u[a] * u[b] - v * sum;
         }
       }
     }
@@ -355,7 +369,17 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       const qbg_species_desc& d = spd[is];
       q.zval = d.zval; q.nmesh = d.nmesh; q.deltar = d.deltar; q.rcps = d.rcps;
       bool ok;
-      if (d.type == QBG_SPECIES_NCPP) {
+      if (d.type == QBG_SPECIES_NCPP && d.nquad > 0) {
+        // semilocal quadrature form: the semilocal potentials as given
+        ok = d.vps && d.nmesh > 0 && d.lmax >= 0 && d.lmax <= 3;
+        if (ok) {
+          q.lmax = d.lmax;
+          q.llocal = d.llocal;
+          q.nquad = d.nquad;
+          q.rquad = d.rquad;
+          q.vsl.assign(d.vps, d.vps + (size_t)(d.lmax + 1) * d.nmesh);
+        }
+      } else if (d.type == QBG_SPECIES_NCPP) {
         ok = d.vps && d.phi && d.nmesh > 0 &&
              host::ncpp_to_kb(d.lmax, d.llocal, d.nmesh, d.deltar, d.vps, d.phi, q.vloc, q.proj,
                               q.dl, q.lmax);
@@ -373,7 +397,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       if (!ok || !q.init()) {
         delete e;
         return c->fail(QBG_E_INVAL, "species: unsupported definition (nonlocal f channel of "
-                                    "a norm-conserving potential, or l > 3?)");
+                                    "a norm-conserving potential, l > 3, or a quadrature "
+                                    "species without rquad or with llocal out of range?)");
       }
       e->sp.push_back(q);
       e->na.push_back(na[is]);
@@ -429,14 +454,16 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       twnl_off[is] = tw;
       dt_off[is] = (int64_t)6 * tw;
       tw += npr * ngw;
-      std::vector<int> lp(npr);
+      std::vector<int> lp, hp, qp;
+      std::vector<double> dp;
+      q.projectors(lp, hp, qp, dp);
       sp_npr[is] = npr;
       fac2[is] = 2.0 * 0.25 * q.rcps * q.rcps;
-      for (int j = 0; j < npr; j++) lp[j] = proj_l(j);
-      // D matrix: diagonal in (l, m) (src/Species.cpp:1450-1464)
+      // D matrix: diagonal, D_l in (l, m) (src/Species.cpp:1450-1464) or the
+      // quadrature weights (src/NonLocalPotential.cpp:196-229)
       const int dbase = (int)dmat.size();
       for (int i = 0; i < npr; i++)
-        for (int j = 0; j < npr; j++) dmat.push_back(i == j ? q.dl[lp[i]] : 0.0);
+        for (int j = 0; j < npr; j++) dmat.push_back(i == j ? dp[i] : 0.0);
       const int Ms = (int)m_atom.size();
       for (int ipr = 0; ipr < npr; ipr++)
         for (int ia = 0; ia < e->na[is]; ia++) {
@@ -868,11 +895,11 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
       if (f32)
         launch_nl_stress_y((const float*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_partial, 64, e->d_scal + S_NLS, s);
+                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
       else
         launch_nl_stress_y((const double*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_partial, 64, e->d_scal + S_NLS, s);
+                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
     }
     ctx->prof_end();
   }
@@ -1066,7 +1093,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (compute_stress)
       launch_nl_stress_y((const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_tau, e->nsp,
                          e->d_sp_first, e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0,
-                         e->d_a_stride, e->d_partial, 64, e->d_scal + S_NLS, s);
+                         e->d_a_stride, e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
     ctx->prof_end();
   }
   return QBG_OK;

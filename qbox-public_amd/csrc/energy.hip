@@ -811,14 +811,18 @@ namespace qbg {
 // danl_ij = dtwnl_ij (-i)^l e^{-iG.tau}.  Summing the atoms of a species
 // first, Z_ipr(G) = sum_a Re-part of (phase_a(G) * Y_{m(a,ipr)}(G)), gives
 // tsum_ij = 4 sum_s sum_ipr sum_G dtwnl_s,ipr,ij(G) Z_ipr(G).
-// Block (chunk of G, species); partial[(s*nchunk + chunk)*6 + ij].
+// Block (chunk of G, species); partial[(s*nchunk + chunk)*6 + ij].  The
+// projectors of a species are taken 16 at a time (Z in registers), each with
+// the l of its column (m_l).
 template <typename T>
 __global__ __launch_bounds__(256) void k_nl_stress_y(
     const T* __restrict__ Y, long lda, int ngw, const double* __restrict__ gx,
     const double* __restrict__ tau, const int32_t* __restrict__ sp_first,
     const int32_t* __restrict__ sp_npr, const int64_t* __restrict__ dt_off,
     const double* __restrict__ dtwnl, const int32_t* __restrict__ a_m0,
-    const int32_t* __restrict__ a_stride, int chunk, double* __restrict__ partial) {
+    const int32_t* __restrict__ a_stride, const int32_t* __restrict__ m_l, int chunk,
+    double* __restrict__ partial) {
+  constexpr int ZB = 16;
   const int s = blockIdx.y;
   const int a0 = sp_first[s], a1 = sp_first[s + 1], npr = sp_npr[s];
   const double* dt = dtwnl + dt_off[s];
@@ -826,27 +830,37 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int ig = g0 + threadIdx.x; ig < g1; ig += 256) {
     const double x = gx[ig], y = gx[ngw + ig], z = gx[2 * ngw + ig];
-    double Z[16];
+    for (int p0 = 0; p0 < npr; p0 += ZB) {
+      double Z[ZB];
 #pragma unroll
-    for (int i = 0; i < 16; i++) Z[i] = 0.0;
-    for (int a = a0; a < a1; a++) {
-      double sn, cs;
-      sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
-      const long mb = a_m0[a], st = a_stride[a];
-      for (int ipr = 0; ipr < npr; ipr++) {
-        const T* yp = Y + (mb + ipr * st) * lda + 2 * ig;
-        const double yr = (double)yp[0], yi = (double)yp[1];
-        // Re(conj(phase) Y) for the phase (-i)^l (c + i s) of projector ipr:
-        // l = 0 (c, s), l = 1 (ipr 1-3) (s, -c), l = 2 (ipr 4-8) (-c, -s),
-        // l = 3 (ipr 9-15) (-s, c)
-        Z[ipr] += ipr == 0 ? cs * yr + sn * yi
-                           : (ipr < 4 ? sn * yr - cs * yi
-                                      : (ipr < 9 ? -(cs * yr + sn * yi) : cs * yi - sn * yr));
+      for (int k = 0; k < ZB; k++) Z[k] = 0.0;
+      for (int a = a0; a < a1; a++) {
+        double sn, cs;
+        sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
+        const long mb = a_m0[a], st = a_stride[a];
+#pragma unroll
+        for (int k = 0; k < ZB; k++) {
+          const int ipr = p0 + k;
+          if (ipr < npr) {
+            const long m = mb + ipr * st;
+            const T* yp = Y + m * lda + 2 * ig;
+            const double yr = (double)yp[0], yi = (double)yp[1];
+            // Re(conj(phase) Y) for the phase (-i)^l (c + i s):
+            // l = 0 (c, s), 1 (s, -c), 2 (-c, -s), 3 (-s, c)
+            const int l = m_l[m];
+            Z[k] += l == 0 ? cs * yr + sn * yi
+                           : (l == 1 ? sn * yr - cs * yi
+                                     : (l == 2 ? -(cs * yr + sn * yi) : cs * yi - sn * yr));
+          }
+        }
       }
-    }
-    for (int ipr = 0; ipr < npr; ipr++)
 #pragma unroll
-      for (int ij = 0; ij < 6; ij++) v[ij] += dt[(long)(ij + 6 * ipr) * ngw + ig] * Z[ipr];
+      for (int k = 0; k < ZB; k++)
+        if (p0 + k < npr)
+#pragma unroll
+          for (int ij = 0; ij < 6; ij++)
+            v[ij] += dt[(long)(ij + 6 * (p0 + k)) * ngw + ig] * Z[k];
+    }
   }
   block_reduce_store<6>(v, partial + ((long)s * gridDim.x + blockIdx.x) * 6);
 }
@@ -865,21 +879,21 @@ template <typename T>
 void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const double* tau,
                         int nsp, const int32_t* sp_first, const int32_t* sp_npr,
                         const int64_t* dt_off, const double* dtwnl, const int32_t* a_m0,
-                        const int32_t* a_stride, double* partial, int nchunk, double* out6,
-                        hipStream_t s) {
+                        const int32_t* a_stride, const int32_t* m_l, double* partial,
+                        int nchunk, double* out6, hipStream_t s) {
   const int chunk = (ngw + nchunk - 1) / nchunk;
   hipLaunchKernelGGL(k_nl_stress_y<T>, dim3(nchunk, nsp), dim3(256), 0, s, Y, lda, ngw, gx, tau,
-                     sp_first, sp_npr, dt_off, dtwnl, a_m0, a_stride, chunk, partial);
+                     sp_first, sp_npr, dt_off, dtwnl, a_m0, a_stride, m_l, chunk, partial);
   hipLaunchKernelGGL(k_nl_stress_final, dim3(1), dim3(64), 0, s, partial, nchunk * nsp, out6);
 }
 template void launch_nl_stress_y<double>(const double*, long, int, const double*, const double*,
                                          int, const int32_t*, const int32_t*, const int64_t*,
-                                         const double*, const int32_t*, const int32_t*, double*,
-                                         int, double*, hipStream_t);
+                                         const double*, const int32_t*, const int32_t*,
+                                         const int32_t*, double*, int, double*, hipStream_t);
 template void launch_nl_stress_y<float>(const float*, long, int, const double*, const double*,
                                         int, const int32_t*, const int32_t*, const int64_t*,
-                                        const double*, const int32_t*, const int32_t*, double*,
-                                        int, double*, hipStream_t);
+                                        const double*, const int32_t*, const int32_t*,
+                                        const int32_t*, double*, int, double*, hipStream_t);
 
 // ------------------------------------------- Hartree pseudocharge stress
 // The species part of sigma_ehart (src/EnergyFunctional.cpp:649-676):

@@ -246,6 +246,28 @@ static void spline(int n, const double* x, const double* y, bool nat_left, doubl
   for (int i = n - 2; i >= 0; i--) y2[i] = (y2[i] - e[i] * y2[i + 1]) / d[i];
 }
 
+// cubic spline with given first derivatives at both ends (src/spline.cpp:44-108
+// with bcnat_left = bcnat_right = 0)
+static void spline_clamped(int n, const double* x, const double* y, double yp1, double ypn,
+                           double* y2) {
+  std::vector<double> d(n), e(n), f(n);
+  const double h0 = x[1] - x[0], hn = x[n - 1] - x[n - 2];
+  d[0] = h0 / 3.0; e[0] = h0 / 6.0; f[0] = 0.0; y2[0] = (y[1] - y[0]) / h0 - yp1;
+  d[n - 1] = hn / 3.0; e[n - 1] = 0.0; f[n - 1] = hn / 6.0;
+  y2[n - 1] = ypn - (y[n - 1] - y[n - 2]) / hn;
+  for (int i = 1; i < n - 1; i++) {
+    const double hp = x[i + 1] - x[i], hm = x[i] - x[i - 1];
+    d[i] = (1.0 / 3.0) * (hp + hm);
+    e[i] = (1.0 / 6.0) * hp;
+    f[i] = (1.0 / 6.0) * hm;
+    y2[i] = (y[i + 1] - y[i]) / hp - (y[i] - y[i - 1]) / hm;
+  }
+  for (int i = 1; i < n; i++) { f[i] /= d[i - 1]; d[i] -= f[i] * e[i - 1]; }
+  for (int i = 1; i < n; i++) y2[i] -= f[i] * y2[i - 1];
+  y2[n - 1] /= d[n - 1];
+  for (int i = n - 2; i >= 0; i--) y2[i] = (y2[i] - e[i] * y2[i + 1]) / d[i];
+}
+
 static void splintd(int n, const double* xa, const double* ya, const double* y2a, double x,
                     double& y, double& dy) {
   int kl = 0, kh = n - 1;
@@ -300,6 +322,18 @@ bool Species::init() {
   if (lmax < 0 || lmax > 3 || nmesh < 8 || !(deltar > 0.0)) return false;
   const double fpi = 4.0 * M_PI;
   nlm = (lmax + 1) * (lmax + 1);
+  if (quad()) {
+    if (llocal < 0 || llocal > lmax || !(rquad > 0.0) ||
+        (int)vsl.size() != (lmax + 1) * nmesh)
+      return false;
+    nlm = 0;
+    for (int l = 0; l <= lmax; l++)
+      if (l != llocal) {
+        if (l > 2) return false;   // the reference has no l = 3 quadrature projectors
+        nlm += (2 * l + 1) * nquad;
+      }
+    vloc.assign(vsl.begin() + (size_t)llocal * nmesh, vsl.begin() + (size_t)(llocal + 1) * nmesh);
+  }
   ndft = 1;
   while (ndft * deltar < 40.0) ndft *= 2;
   std::vector<double> r(ndft), vr(ndft), fint(ndft);
@@ -340,6 +374,33 @@ bool Species::init() {
   }
   vnlg.assign(lmax + 1, std::vector<double>(ndft + 1, 0.0));
   vnlg2.assign(lmax + 1, std::vector<double>(ndft + 1, 0.0));
+  if (quad()) {
+    // semilocal potentials extended by -Z/r, splined with v'(0) = 0 and
+    // v'(r_max) = Z/r_max^2 (src/Species.cpp:162-184, Species::dvpsr
+    // :1120-1131); trapezoid rule with interior points
+    // (src/NonLocalPotential.cpp:119-131)
+    const double h = rquad / (nquad + 1);
+    rq.resize(nquad);
+    for (int q = 0; q < nquad; q++) rq[q] = (q + 1) * h;
+    std::vector<std::vector<double>> vs(lmax + 1, std::vector<double>(ndft)),
+        vs2(lmax + 1, std::vector<double>(ndft));
+    const double dvdr = zval / (r[ndft - 1] * r[ndft - 1]);
+    for (int l = 0; l <= lmax; l++) {
+      for (int i = 0; i < ndft; i++) vs[l][i] = i < nmesh ? vsl[(size_t)l * nmesh + i] : -zval / r[i];
+      spline_clamped(ndft, r.data(), vs[l].data(), 0.0, dvdr, vs2[l].data());
+    }
+    auto vpsr = [&](int l, double x) {
+      if (x > r[ndft - 1]) return 0.0;
+      double v, dv;
+      splintd(ndft, r.data(), vs[l].data(), vs2[l].data(), x, v, dv);
+      return v;
+    };
+    wql.assign(lmax + 1, std::vector<double>(nquad, 0.0));
+    for (int l = 0; l <= lmax; l++)
+      if (l != llocal)
+        for (int q = 0; q < nquad; q++) wql[l][q] = (vpsr(l, rq[q]) - vpsr(llocal, rq[q])) * h;
+    return true;
+  }
   for (int l = 0; l <= lmax; l++) {
     std::vector<double> w(ndft);
     for (int i = 0; i < ndft; i++) w[i] = i < nmesh ? fpi * deltar * proj[(size_t)l * nmesh + i] : 0.0;
@@ -423,6 +484,62 @@ void Species::dvlocg(double g, double& v, double& dv) const {
 void Species::dvnlg(int l, double g, double& v, double& dv) const {
   if (g > gspl[ndft - 1]) { v = dv = 0.0; return; }
   splintd(ndft, gspl.data(), vnlg[l].data(), vnlg2[l].data(), g, v, dv);
+}
+
+void Species::quad_radial(int l, int q, double g, double& v, double& dv) const {
+  const double fpi = 4.0 * M_PI, r = rq[q], z = g * r;
+  v = dv = 0.0;
+  if (l == 0) {   // 4 pi sin(Gr)/G, the G = 0 limit 4 pi r
+    if (g == 0.0) { v = fpi * r; return; }
+    const double gi = 1.0 / g, s = std::sin(z), c = std::cos(z);
+    v = fpi * s * gi;
+    dv = fpi * gi * (r * c - s * gi);
+    return;
+  }
+  if (z == 0.0) return;
+  const double zi = 1.0 / z, c = std::cos(z), s = std::sin(z);
+  if (l == 1) {
+    const double j1 = (s * zi - c) * zi;
+    const double dj1 = (2.0 * z * c + (z * z - 2.0) * s) * zi * zi * zi;
+    v = fpi * j1 * r;
+    dv = fpi * dj1 * r * r;
+  } else {
+    const double j2 = ((3.0 * zi * zi - 1.0) * s - 3.0 * zi * c) * zi;
+    const double z2 = z * z;
+    const double dj2 = ((4.0 * z2 - 9.0) * s + z * (9.0 - z2) * c) / (z2 * z2);
+    v = fpi * j2 * r;
+    dv = fpi * dj2 * r * r;
+  }
+}
+
+void Species::projectors(std::vector<int>& pl, std::vector<int>& ph, std::vector<int>& pq,
+                         std::vector<double>& pd) const {
+  pl.assign(nlm, 0);
+  ph.assign(nlm, 0);
+  pq.assign(nlm, -1);
+  pd.assign(nlm, 0.0);
+  if (!quad()) {
+    for (int ipr = 0; ipr < nlm; ipr++) {
+      const int l = ipr == 0 ? 0 : (ipr < 4 ? 1 : (ipr < 9 ? 2 : 3));
+      pl[ipr] = l;
+      ph[ipr] = ipr;
+      pd[ipr] = dl[l];   // D diagonal in (l, m) (src/Species.cpp:1450-1464)
+    }
+    return;
+  }
+  int ilm = 0;
+  for (int l = 0; l <= lmax; l++) {
+    if (l == llocal) continue;
+    for (int m = 0; m < 2 * l + 1; m++)
+      for (int q = 0; q < nquad; q++) {
+        const int ipr = q + nquad * (ilm + m);
+        pl[ipr] = l;
+        ph[ipr] = l * l + m;
+        pq[ipr] = q;
+        pd[ipr] = wql[l][q];
+      }
+    ilm += 2 * l + 1;
+  }
 }
 
 void Species::drhocoreg(double g, double& rho, double& drho) const {

@@ -41,6 +41,16 @@ struct Species {
   double deltar = 0, rcps = 0;
   std::vector<double> vloc, proj, dl;           // inputs
   std::vector<double> nlcc;                     // core density on the mesh, or empty
+  // semilocal quadrature form (norm-conserving, nquad > 0): semilocal
+  // potentials vsl[(lmax+1)*nmesh], local channel llocal; projectors
+  // 4 pi j_l(G r_q) r_q Y_lm at r_q = (q+1) rquad/(nquad+1) with weights
+  // (v_l(r_q) - v_loc(r_q)) rquad/(nquad+1) (src/NonLocalPotential.cpp:104-229)
+  int nquad = 0, llocal = -1;
+  double rquad = 0;
+  std::vector<double> vsl, rq;
+  std::vector<std::vector<double>> wql;         // [l][q]
+  bool quad() const { return nquad > 0; }
+  // nlm: the projector count (one per (l, m), or per (l, m, q))
   int ndft = 0, nlm = 0;
   std::vector<double> gspl, vlocg, vlocg2;
   std::vector<std::vector<double>> vnlg, vnlg2;
@@ -52,6 +62,15 @@ struct Species {
   void dvnlg(int l, double g, double& v, double& dv) const;
   // core density transform and its derivative (src/Species.cpp:1194-1220)
   void drhocoreg(double g, double& rho, double& drho) const;
+  // radial function 4 pi j_l(G r_q) r_q of a quadrature projector and its G
+  // derivative (src/NonLocalPotential.cpp:317-420, 560-580, 838-855)
+  void quad_radial(int l, int q, double g, double& v, double& dv) const;
+  // projector table: for projector ipr its l, real-harmonic index l^2 + m,
+  // quadrature point (-1 for a channel projector) and D-matrix diagonal
+  // (one projector per (l, m) in harmonic order; in the quadrature form
+  // ipr = q + nquad (ilm + m) over the nonlocal channels, :196-229)
+  void projectors(std::vector<int>& l, std::vector<int>& h, std::vector<int>& q,
+                  std::vector<double>& d) const;
   double rhopsg(double g) const;
   double eself() const;
 };

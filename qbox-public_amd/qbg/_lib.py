@@ -117,7 +117,7 @@ class SpeciesDesc(C.Structure):
                 ("deltar", C.c_double), ("rcps", C.c_double),
                 ("vloc", _dp), ("proj", _dp), ("dl", _dp),
                 ("type", C.c_int), ("llocal", C.c_int), ("vps", _dp), ("phi", _dp),
-                ("nlcc", _dp)]
+                ("nlcc", _dp), ("nquad", C.c_int), ("rquad", C.c_double)]
 
 
 def species_desc(sp, keep):
@@ -130,17 +130,15 @@ def species_desc(sp, keep):
         core = np.ascontiguousarray(core, dtype=np.float64)
         keep.append(core)
     if getattr(sp, "kind", "") == "ncpp":
-        if sp.nquad != 0:
-            raise ValueError("semilocal quadrature species (nquad > 0) are not supported")
         vps = np.ascontiguousarray(sp.vps, dtype=np.float64)
         phi = np.ascontiguousarray(sp.phi, dtype=np.float64)
         keep += [vps, phi]
         return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, None, None, None,
-                           1, sp.llocal, _pd(vps), _pd(phi), _pd(core))
+                           1, sp.llocal, _pd(vps), _pd(phi), _pd(core), sp.nquad, sp.rquad)
     vloc, proj, dl = sp.radial()
     keep += [vloc, proj, dl]
     return SpeciesDesc(sp.zval, sp.lmax, sp.nmesh, sp.deltar, sp.rcps, _pd(vloc), _pd(proj),
-                       _pd(dl), 0, 0, None, None, _pd(core))
+                       _pd(dl), 0, 0, None, None, _pd(core), 0, 0.0)
 
 _lib = None
 

@@ -185,6 +185,37 @@ def test_core_density_ncpp(ctx, oracle_mod, prec):
     ef.release()
 
 
+@pytest.mark.parametrize("llocal,prec", [(2, "fp64"), (0, "fp64"), (0, "bf16x2")])
+def test_quadrature_evaluate(ctx, oracle_mod, llocal, prec):
+    """Si in the semilocal quadrature form (nquad = 16, rquad = 2.6;
+    src/NonLocalPotential.cpp:104-229, 317-960) in the si4gs cell against
+    the oracle: 64 (s, p) or 128 (p, d) projectors per atom, their weights
+    from the splined semilocal potentials, energies, forces, H*psi and all
+    seven stress rows."""
+    import qbg
+    si = read_species(GOLD / "Si_VBC_LDA-1.0.xml")
+    s = cf.si4gs(dataclasses.replace(si, nquad=16, rquad=2.6, llocal=llocal))
+    print(f"[case] si4gs quadrature form llocal {llocal} {prec}", flush=True)
+    ef = qbg.EnergyFunctional(ctx, s)
+    assert ef.nproj == 4 * 16 * (4 if llocal == 2 else 8)
+    o = oracle_mod.Oracle(s)
+    c = cf.gram_numpy(cf.random_coefficients(16, o.basis(0)["g2"], s.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    got = ef.evaluate(dc, hp, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        if prec == "fp64":
+            assert abs(got[t] - ref[t]) <= 1e-10 * max(abs(ref[t]), 1e-12), (t, got[t], ref[t])
+        else:
+            assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    ftol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.abs(got["fion"] - ref["fion"]).max() <= ftol * max(1.0, np.abs(ref["fion"]).max())
+    assert relerr(hp.download(), ref["hpsi"]) < ftol
+    assert np.abs(got["sigma"] - ref["sigma"]).max() < ftol * np.abs(ref["sigma"][0]).max()
+    ef.release()
+
+
 def test_f_channel_refused(ctx):
     """A nonlocal f channel (l = 3) of a norm-conserving potential is
     refused: the reference's Kleinman-Bylander construction
