@@ -93,7 +93,7 @@ static void ef_free(qbg_ef* e) {
                e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
                e->d_seg_ptr, e->d_seg_n, e->d_rhor2, e->d_vxc2, e->d_vr2, e->d_vloc,
                e->d_rhot, e->b_fpart, e->b_fred, e->d_rhoc, e->d_drhoc, e->d_rhocg,
-               e->d_rhocr, e->d_vxcg};
+               e->d_rhocr, e->d_vxcg, e->b_yh};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
@@ -926,9 +926,9 @@ static int bf16_ensure(qbg_ef* e) {
   e->b_part_floats = (size_t)(1024 + ntile) * 128 * 128;
   const size_t n_anl = (size_t)e->Mp * e->Kp, n_c = (size_t)e->Np * e->Kp;
   const size_t n_f = (size_t)e->Mp * e->Np;
-  float** bufs[5] = {&e->b_anl, &e->b_c32, &e->b_y32, &e->b_fout, &e->b_w32};
-  const size_t sz[5] = {n_anl, n_c, n_anl, n_f, n_f};
-  for (int i = 0; i < 5; i++) {
+  float** bufs[6] = {&e->b_anl, &e->b_c32, &e->b_y32, &e->b_fout, &e->b_w32, &e->b_yh};
+  const size_t sz[6] = {n_anl, n_c, n_anl, n_f, n_f, n_c};
+  for (int i = 0; i < 6; i++) {
     if (hipMalloc((void**)bufs[i], sizeof(float) * sz[i]) != hipSuccess) {
       *bufs[i] = nullptr;
       return c->fail(QBG_E_NOMEM, "bf16 workspace");
@@ -1027,9 +1027,11 @@ static int bf16_tune(qbg_ef* e, const double* c, int ldc, int nst) {
 // (gemm_bf.hip) on FP32 copies of anl, c, fout and W
 // (bands [b0, b0 + nst) as nl_mfma; band columns [nst, e->nst) of the FP32
 // copies are zeroed, so the padded band sums see zeros there)
+// yh (or null): write the nonlocal part of H*psi there (FP32, ld Kp)
+// instead of adding it to hpsi
 static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi, int ldh,
                    int compute_forces, int compute_stress, int parts, bool c32_ready, int b0,
-                   int nst) {
+                   int nst, float* yh = nullptr) {
   qbg_ctx* ctx = e->ctx;
   hipStream_t s = ctx->stream;
   int rc = bf16_ensure(e);
@@ -1059,7 +1061,9 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_end();
   if (compute_hpsi) {
     ctx->prof_begin(QBG_K_GEMM);
-    rc = gemm_bf_nn(e->b_anl, Kp, e->b_fout, Mp, Kp, Np, Mp, K, nst, hpsi, 2L * ldh, 1, parts, s);
+    rc = yh ? gemm_bf_nn(e->b_anl, Kp, e->b_fout, Mp, Kp, Np, Mp, K, nst, yh, Kp, 0, parts, s)
+            : gemm_bf_nn(e->b_anl, Kp, e->b_fout, Mp, Kp, Np, Mp, K, nst, hpsi, 2L * ldh, 1,
+                         parts, s);
     ctx->prof_end();
     if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
   }
@@ -1126,10 +1130,14 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     const int rc = bf16_ensure(e);
     if (rc) return rc;
   }
+  // in the split modes the forward z pass completes H*psi: it stores
+  // 0.5 kpg2 c + the nonlocal part (FP32, from the NN product) + the local
+  // part, so neither the kinetic pass nor the NN product touches H*psi
+  const bool fin = split && compute_hpsi && hpsi != c && zfwd_fin_ok(e->gw);
   ctx->prof_begin(QBG_K_OTHER);
-  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh, compute_hpsi,
-                 e->d_partial, e->d_scal + S_KIN, s, split ? e->b_c32 : nullptr,
-                 split ? e->Kp : 0);
+  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh,
+                 compute_hpsi && !fin, e->d_partial, e->d_scal + S_KIN, s,
+                 split ? e->b_c32 : nullptr, split ? e->Kp : 0);
   ctx->prof_end();
   // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
   // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
@@ -1137,7 +1145,8 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   const long K = 2L * ngw;
   int nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
                 ? nl_bf16(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec == QBG_PREC_BF16X3 ? 3 : 2, true, 0, nst)
+                          prec == QBG_PREC_BF16X3 ? 3 : 2, true, 0, nst,
+                          fin ? e->b_yh : nullptr)
                 : nl_mfma(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
                           prec, 0, nst);
   if (nrc) return nrc;
@@ -1157,9 +1166,20 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
       spin_bands(e, sp, b0, nb, p0);
       if (nb == 0) continue;
       const double* v = sp == 0 ? e->d_vr : e->d_vr2;
-      const int rc = kept ? hpsi_resident(e->gw, v, nb, hpsi + 2L * ldh * b0, ldh, fprec, p0)
-                          : qbg_rs_mul_add(e->gw, v, c + 2L * ldc * b0, ldc, nb,
-                                           hpsi + 2L * ldh * b0, ldh, fprec);
+      ZFin zf;
+      if (fin) {
+        zf.c = c + 2L * ldc * b0;
+        zf.cpst = 2L * ldc;
+        zf.coff2 = ldc;
+        zf.g2 = e->d_g2w;
+        zf.y = e->b_yh + e->Kp * b0;
+        zf.ldy = e->Kp;
+      }
+      const int rc =
+          kept ? hpsi_resident(e->gw, v, nb, hpsi + 2L * ldh * b0, ldh, fprec, p0,
+                               fin ? &zf : nullptr)
+               : rs_mul_add_bands(e->gw, v, c + 2L * ldc * b0, ldc, nb, hpsi + 2L * ldh * b0,
+                                  ldh, fprec, fin ? &zf : nullptr);
       if (rc) return rc;
     }
     if (kept) e->xres_c = nullptr;   // transformed in place: consumed

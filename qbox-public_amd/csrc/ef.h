@@ -67,6 +67,9 @@ struct qbg_ef {
   long Kp = 0;
   float *b_anl = nullptr, *b_c32 = nullptr, *b_y32 = nullptr, *b_fout = nullptr,
         *b_w32 = nullptr, *b_part = nullptr;
+  // the nonlocal part of H*psi (Kp x Np, FP32) when the forward z pass
+  // completes H*psi (qbg::ZFin)
+  float* b_yh = nullptr;
   size_t b_part_floats = 0;
   // launch plan of the TN projections, timed at the first 2-part BF16 use
   qbg::BfTnPlan bf_tn;

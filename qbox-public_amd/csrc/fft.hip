@@ -561,12 +561,16 @@ int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const do
   return prec == QBG_PREC_FP64 ? ypass_t<double>(g, mode, nbatch, v, fac, rho, fgrid, scale, s)
                                : ypass_t<float>(g, mode, nbatch, v, fac, rho, fgrid, scale, s);
 }
+bool zfwd_fin_ok(const qbg_grid* g) { return g && g->f4[2]; }
+
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
-         int last_has2, int nbatch, hipStream_t s) {
+         int last_has2, int nbatch, hipStream_t s, const ZFin* fin) {
   if (!ws_ok(g, nbatch) || !out) return QBG_E_STATE;
+  if (fin && (!g->f4[2] || mode != Z_PAIR || !fin->c || !fin->y || !fin->g2))
+    return QBG_E_STATE;
   if (g->f4[2]) {
     g->f4[2]->zfwd(g, prec, mode == Z_SINGLE, outmode == OUT_ACCUM, out, pstride, off2,
-                   last_has2, nbatch, s);
+                   last_has2, nbatch, s, fin ? *fin : ZFin{});
     return launched();
   }
   return prec == QBG_PREC_FP64

@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
                                               const int32_t* __restrict__ im,
                                               const int32_t* __restrict__ glist,
                                               const qbg_tile* __restrict__ tiles, int ncols, int TC,
-                                              const cpx<T>* __restrict__ tw_g, double scale) {
+                                              const cpx<T>* __restrict__ tw_g, double scale,
+                                              ZFin fin) {
   constexpr int N = N1 * N2;
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem);
@@ -195,11 +196,34 @@ __global__ __launch_bounds__(256) void kz_fwd(const cpx<T>* __restrict__ zbuf,
   double* o1 = out + 2 * (long)pair * pstride;
   double* o2 = o1 + 2 * off2;
   const bool has2 = !single && (pair < nbatch - 1 || last_has2);
+  // completion of H*psi: 0.5 g2 c + y of the pair's bands
+  const double* c1 = fin.c ? fin.c + 2 * (long)pair * fin.cpst : nullptr;
+  const double* c2 = fin.c ? c1 + 2 * fin.coff2 : nullptr;
+  const float* y1 = fin.c ? fin.y + 2 * (long)pair * fin.ldy : nullptr;
+  const float* y2 = fin.c ? y1 + fin.ldy : nullptr;
   for (int j = t.g0 + tid; j < t.g1; j += 256) {
     const int ig = glist[j];
     const int sp = ip[ig] - slot0;
     const cpx<T> zpv = A[(sp % N) * TC + sp / N];
     const double a = (double)zpv.x * scale, b = (double)zpv.y * scale;
+    if (fin.c) {
+      const double kg = 0.5 * fin.g2[ig];
+      const double r1 = kg * c1[2 * ig] + (double)y1[2 * ig];
+      const double i1 = kg * c1[2 * ig + 1] + (double)y1[2 * ig + 1];
+      if (has2) {
+        const int sm = im[ig] - slot0;
+        const cpx<T> zmv = A[(sm % N) * TC + sm / N];
+        const double cc = (double)zmv.x * scale, d = (double)zmv.y * scale;
+        o1[2 * ig] = r1 + 0.5 * (a + cc);
+        o1[2 * ig + 1] = i1 + 0.5 * (b - d);
+        o2[2 * ig] = (kg * c2[2 * ig] + (double)y2[2 * ig]) + 0.5 * (b + d);
+        o2[2 * ig + 1] = (kg * c2[2 * ig + 1] + (double)y2[2 * ig + 1]) + 0.5 * (cc - a);
+      } else {
+        o1[2 * ig] = r1 + a;
+        o1[2 * ig + 1] = i1 + b;
+      }
+      continue;
+    }
     if (has2) {
       const int sm = im[ig] - slot0;
       const cpx<T> zmv = A[(sm % N) * TC + sm / N];
@@ -586,18 +610,20 @@ struct Launch {
 
   template <typename T>
   static void zfwd_t(qbg_grid* g, int single, int accum, double* out, long pstride, long off2,
-                     int last_has2, int nbatch, hipStream_t s) {
+                     int last_has2, int nbatch, hipStream_t s, ZFin fin) {
     const size_t lds = sizeof(cpx<T>) * ((size_t)g->zcols * N + N);
     const double scale = 1.0 / ((double)g->np0 * g->np1 * g->np2);
     hipLaunchKernelGGL((kz_fwd<T, N1, N2>), dim3(g->ntiles, nbatch), dim3(256), lds, s,
                        (const cpx<T>*)g->d_zbuf, out, pstride, off2, last_has2, nbatch, single,
                        accum, g->d_ip, g->d_im, g->d_glist, g->d_tiles, g->ncols, g->zcols, twp<T>(g, 2),
-                       scale);
+                       scale, fin);
   }
   static void zfwd(qbg_grid* g, int prec, int single, int accum, double* out, long pstride,
-                   long off2, int last_has2, int nbatch, hipStream_t s) {
-    if (prec == QBG_PREC_FP64) zfwd_t<double>(g, single, accum, out, pstride, off2, last_has2, nbatch, s);
-    else zfwd_t<float>(g, single, accum, out, pstride, off2, last_has2, nbatch, s);
+                   long off2, int last_has2, int nbatch, hipStream_t s, ZFin fin) {
+    if (prec == QBG_PREC_FP64)
+      zfwd_t<double>(g, single, accum, out, pstride, off2, last_has2, nbatch, s, fin);
+    else
+      zfwd_t<float>(g, single, accum, out, pstride, off2, last_has2, nbatch, s, fin);
   }
 
   static void xpass(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s) {

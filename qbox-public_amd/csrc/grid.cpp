@@ -691,8 +691,16 @@ int qbg::density_batches(qbg_grid* g, const double* c, int ldc, int nst, const d
   return last_launch(g);
 }
 
+// the completion arguments of the batch starting at band0
+static qbg::ZFin fin_batch(const qbg::ZFin* fin, int band0) {
+  qbg::ZFin f = *fin;
+  f.c = fin->c + fin->cpst * band0;
+  f.y = fin->y + fin->ldy * band0;
+  return f;
+}
+
 int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec,
-                       int pair_base) {
+                       int pair_base, const ZFin* fin) {
   const int npairs = (nst + 1) / 2;
   const int B = std::min(npairs, g->batch);
   TRY(ensure_ws(g, B));
@@ -708,8 +716,31 @@ int qbg::hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int 
     XwsScope xs(g, (char*)g->d_xres + (size_t)(pair_base + p0) * pb);
     PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
     PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
+    const ZFin fb = fin ? fin_batch(fin, band0) : ZFin{};
     PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh,
-                          last_has2, nb, s));
+                          last_has2, nb, s, fin ? &fb : nullptr));
+  }
+  return last_launch(g);
+}
+
+int qbg::rs_mul_add_bands(qbg_grid* g, const double* v, const double* c, int ldc, int nst,
+                          double* hpsi, int ldh, int prec, const ZFin* fin) {
+  const int npairs = (nst + 1) / 2;
+  const int B = std::min(npairs, g->batch);
+  TRY(ensure_ws(g, B));
+  TRY(qbg::grid_tune(g, prec));
+  hipStream_t s = g->ctx->stream;
+  for (int p0 = 0; p0 < npairs; p0 += B) {
+    const int nb = std::min(B, npairs - p0);
+    const int band0 = 2 * p0;
+    const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
+    PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
+    PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
+    PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
+    PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
+    const ZFin fb = fin ? fin_batch(fin, band0) : ZFin{};
+    PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh,
+                          last_has2, nb, s, fin ? &fb : nullptr));
   }
   return last_launch(g);
 }
@@ -797,22 +828,7 @@ int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc, int n
     TRY(checkprec(g, prec));
     if (nst <= 0 || ldc < g->ng || ldh < g->ng)
       return g->ctx->fail(QBG_E_INVAL, "rs_mul_add: bad arguments");
-    const int npairs = (nst + 1) / 2;
-    const int B = std::min(npairs, g->batch);
-    TRY(ensure_ws(g, B));
-    TRY(qbg::grid_tune(g, prec));
-    hipStream_t s = g->ctx->stream;
-    for (int p0 = 0; p0 < npairs; p0 += B) {
-      const int nb = std::min(B, npairs - p0);
-      const int band0 = 2 * p0;
-      const int last_has2 = (band0 + 2 * nb) <= nst ? 1 : 0;
-      PROF(QBG_K_ZBWD, zbwd(g, prec, Z_PAIR, c + 2L * ldc * band0, 2L * ldc, ldc, last_has2, nb, s));
-      PROF(QBG_K_XBWD, xbwd(g, prec, nb, s));
-      PROF(QBG_K_YVMUL, ypass(g, prec, Y_VMUL, nb, v, nullptr, nullptr, nullptr, 1.0, s));
-      PROF(QBG_K_XFWD, xfwd(g, prec, nb, s));
-      PROF(QBG_K_ZFWD, zfwd(g, prec, Z_PAIR, OUT_ACCUM, hpsi + 2L * ldh * band0, 2L * ldh, ldh, last_has2, nb, s));
-    }
-    return last_launch(g);
+    return qbg::rs_mul_add_bands(g, v, c, ldc, nst, hpsi, ldh, prec);
   });
 }
 

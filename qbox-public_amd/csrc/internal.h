@@ -67,12 +67,27 @@ struct qbg_tile {
 namespace qbg {
 // compiled four-step kernels of one line length n = n1*n2 (fft4_*.hip);
 // L = lines per block of the x and y passes
+// Completion of H*psi in the forward z pass (four-step kernels only): with
+// c set, the pass stores out = 0.5 g2 c + y + (the transform) instead of
+// adding the transform to out, so the kinetic and nonlocal parts need no
+// earlier pass over H*psi.  c and y follow out's pair layout: the pair's
+// first band at c + 2 pair cpst (its second at + 2 coff2) and at
+// y + 2 pair ldy (second at + ldy), y the FP32 nonlocal product (real view,
+// re/im interleaved), g2 the wavefunction basis |G|^2.
+struct ZFin {
+  const double* c = nullptr;
+  long cpst = 0, coff2 = 0;
+  const double* g2 = nullptr;
+  const float* y = nullptr;
+  long ldy = 0;
+};
+
 struct F4Ops {
   int n, n1, n2, L;
   void (*zbwd)(qbg_grid* g, int prec, int single, const double* c, long pstride, long off2,
                int last_has2, int nbatch, hipStream_t s);
   void (*zfwd)(qbg_grid* g, int prec, int single, int accum, double* out, long pstride,
-               long off2, int last_has2, int nbatch, hipStream_t s);
+               long off2, int last_has2, int nbatch, hipStream_t s, ZFin fin);
   void (*xpass)(qbg_grid* g, int prec, bool fwd, int nbatch, hipStream_t s);
   void (*ypass)(qbg_grid* g, int prec, int mode, int nbatch, int ppb, const double* v,
                 const double* fac, double* rho, double* fgrid, double scale, hipStream_t s);
@@ -167,7 +182,9 @@ int xfwd(qbg_grid* g, int prec, int nbatch, hipStream_t s);
 int ypass(qbg_grid* g, int prec, int mode, int nbatch, const double* v, const double* fac,
           double* rho, double* fgrid, double scale, hipStream_t s);
 int zfwd(qbg_grid* g, int prec, int mode, int outmode, double* out, long pstride, long off2,
-         int last_has2, int nbatch, hipStream_t s);
+         int last_has2, int nbatch, hipStream_t s, const ZFin* fin = nullptr);
+// whether zfwd can complete H*psi (ZFin) on this grid: four-step z kernels
+bool zfwd_fin_ok(const qbg_grid* g);
 // band-pair density passes with per-band factors already on the device
 // (fac[2*npairs], zero for a missing last band); no host synchronization.
 // resident: the x-pass output of every pair stays in g->d_xres, from pair
@@ -179,7 +196,11 @@ int density_batches(qbg_grid* g, const double* c, int ldc, int nst, const double
 // output density_batches(..., resident = true, pair_base) left in g->d_xres;
 // consumes it (the y pass transforms it in place)
 int hpsi_resident(qbg_grid* g, const double* v, int nst, double* hpsi, int ldh, int prec,
-                  int pair_base = 0);
+                  int pair_base = 0, const ZFin* fin = nullptr);
+// qbg_rs_mul_add, optionally completing H*psi (fin: c of the same bands,
+// ldc, y, ldy, g2; then hpsi is stored, not added to)
+int rs_mul_add_bands(qbg_grid* g, const double* v, const double* c, int ldc, int nst,
+                     double* hpsi, int ldh, int prec, const ZFin* fin = nullptr);
 // util.hip
 void fill_zero(double* p, size_t n, hipStream_t s);
 void fill_value(double* p, size_t n, double v, hipStream_t s);
