@@ -31,12 +31,17 @@ typedef struct {
 static std::string g_err;
 const char* qbo_last_error(void) { return g_err.c_str(); }
 
-void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
-                 int nsp, const qbo_species_desc* sp, const int* na,
-                 const double* tau, int nst, const double* occ, const int* grid3) {
+// kpoint3 (null: k = 0) in units of the reciprocal vectors; force_complex
+// builds the complex basis at k = 0 too
+void* qbo_create_k(const double* cell9, const double* refcell9, double ecut_ha,
+                   int nsp, const qbo_species_desc* sp, const int* na,
+                   const double* tau, int nst, const double* occ, const int* grid3,
+                   const double* kpoint3, int force_complex) {
   try {
     System* s = new System;
     s->cell.set(cell9);
+    if (kpoint3) for (int i = 0; i < 3; i++) s->kpoint[i] = kpoint3[i];
+    s->force_complex = force_complex != 0;
     if (grid3) for (int i = 0; i < 3; i++) s->grid[i] = grid3[i];
     if (refcell9) { s->has_ref = true; s->refcell.set(refcell9); }
     s->ecut = ecut_ha;
@@ -80,7 +85,23 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
   }
 }
 
+void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
+                 int nsp, const qbo_species_desc* sp, const int* na,
+                 const double* tau, int nst, const double* occ, const int* grid3) {
+  return qbo_create_k(cell9, refcell9, ecut_ha, nsp, sp, na, tau, nst, occ, grid3, nullptr, 0);
+}
+
 void qbo_destroy(void* h) { delete (System*)h; }
+
+// the wavefunction basis's k + G: |k+G|^2 (ngw) and its components (3 ngw);
+// returns 1 for a real basis, 0 for a complex one
+int qbo_kpg(void* h, double* kpg2, double* kpgx) {
+  System* s = (System*)h;
+  const Basis& b = s->wb;
+  if (kpg2) std::memcpy(kpg2, b.kpg2.data(), sizeof(double) * b.size);
+  if (kpgx) std::memcpy(kpgx, b.kpgx.data(), sizeof(double) * 3 * b.size);
+  return b.real ? 1 : 0;
+}
 
 // the Kleinman-Bylander form of an NCPP species: proj[(lkb+1)*nmesh], dl[lkb+1]
 // (room for lmax+1 rows); returns lkb or -1

@@ -123,10 +123,23 @@ static inline double norm2_hkl(const double* b0, const double* b1,
   return v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
 }
 
-// src/Basis.cpp:255-643 (one task, kpoint 0)
-void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha) {
+// norm2 of (kx+h)*b0 + (ky+k)*b1 + (kz+l)*b2 in the reference's order
+// (src/Basis.cpp:460)
+static inline double norm2_kpg(const double* b0, const double* b1, const double* b2,
+                               const double* kp, int h, int k, int l) {
+  double v[3];
+  for (int j = 0; j < 3; j++)
+    v[j] = ((kp[0] + h) * b0[j] + (kp[1] + k) * b1[j]) + (kp[2] + l) * b2[j];
+  return v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+}
+
+// src/Basis.cpp:255-643 (one task)
+void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha, const double* kp,
+                   bool force_complex) {
   cell = c;
   ecut = ecut_ha;
+  for (int i = 0; i < 3; i++) kpoint[i] = kp ? kp[i] : 0.0;
+  real = !force_complex && kpoint[0] == 0.0 && kpoint[1] == 0.0 && kpoint[2] == 0.0;
   const Cell& defcell = refcell ? *refcell : c;
   const double two_ecut = 2.0 * ecut;
   const double twopi = 2.0 * M_PI;
@@ -147,10 +160,38 @@ void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha) {
   struct R { int h, k, lmin, size; };
   std::vector<R> rods;
   int hmax_used = 0, hmin_used = 0, kmax_used = 0, kmin_used = 0;
+  int lmax_used = 0, lmin_used = 0;
+  size = 0;
+  if (!real) {
+    // k != 0: every rod (h, k) of the sphere |k + G| < sqrt(2 ecut) (:446-481)
+    const int hmin = -hmax;
+    hmax_used = hmin; hmin_used = hmax; kmax_used = kmin; kmin_used = kmax;
+    lmax_used = lmin; lmin_used = lmax;
+    for (int h = hmin; h <= hmax; h++)
+      for (int k = kmin; k <= kmax; k++) {
+        int lstart = lmax, lend = lmin;
+        bool found = false;
+        for (int l = lmin; l <= lmax; l++)
+          if (norm2_kpg(b0, b1, b2, kpoint, h, k, l) < two_ecut) {
+            lstart = std::min(l, lstart);
+            lend = std::max(l, lend);
+            found = true;
+          }
+        if (!found) continue;
+        const int rs = lend - lstart + 1;
+        size += rs;
+        rods.push_back({h, k, lstart, rs});
+        hmax_used = std::max(h, hmax_used); hmin_used = std::min(h, hmin_used);
+        kmax_used = std::max(k, kmax_used); kmin_used = std::min(k, kmin_used);
+        lmax_used = std::max(lend, lmax_used); lmin_used = std::min(lstart, lmin_used);
+      }
+  }
   int lend0 = (int)(std::sqrt(two_ecut * b2inv2));
-  rods.push_back({0, 0, 0, lend0 + 1});
-  int lmax_used = lend0, lmin_used = 0;
-  size = lend0 + 1;
+  if (real) {
+    rods.push_back({0, 0, 0, lend0 + 1});
+    lmax_used = lend0;
+    size = lend0 + 1;
+  }
   auto scan = [&](int h, int k, bool use_b0) {
     int lstart = lmax, lend = lmin;
     bool found = false;
@@ -176,9 +217,11 @@ void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha) {
       lmin_used = std::min(lstart, lmin_used);
     }
   };
-  for (int k = 1; k <= kmax; k++) scan(0, k, false);     // :366-390
-  for (int h = 1; h <= hmax; h++)                         // :391-420
-    for (int k = kmin; k <= kmax; k++) scan(h, k, true);
+  if (real) {
+    for (int k = 1; k <= kmax; k++) scan(0, k, false);     // :366-390
+    for (int h = 1; h <= hmax; h++)                         // :391-420
+      for (int k = kmin; k <= kmax; k++) scan(h, k, true);
+  }
   nrods = (int)rods.size();
   idxmax[0] = hmax_used; idxmin[0] = hmin_used;
   idxmax[1] = kmax_used; idxmin[1] = kmin_used;
@@ -195,6 +238,7 @@ void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha) {
   int r0 = -1;
   for (int i = 0; i < nrods; i++)
     if (rods[i].h == 0 && rods[i].k == 0) { r0 = i; break; }
+  if (r0 < 0) throw std::runtime_error("basis without the rod (0,0)");
   std::swap(rods[r0], rods[0]);
 
   rod_h.resize(nrods); rod_k.resize(nrods); rod_lmin.resize(nrods);
@@ -220,18 +264,27 @@ void Basis::resize(const Cell& c, const Cell* refcell, double ecut_ha) {
 void Basis::update_g() {
   g.resize(size); g2.resize(size); gi.resize(size); g2i.resize(size);
   gx.resize(3 * size);
+  kpg.resize(size); kpg2.resize(size); kpgi.resize(size); kpgx.resize(3 * size);
   for (int i = 0; i < size; i++) {
-    double gt[3];
-    for (int j = 0; j < 3; j++)
+    double gt[3], kt[3];
+    for (int j = 0; j < 3; j++) {
       gt[j] = (idx[3 * i] * cell.b[0][j] + idx[3 * i + 1] * cell.b[1][j]) +
               idx[3 * i + 2] * cell.b[2][j];
-    gx[i] = gt[0];
-    gx[size + i] = gt[1];
-    gx[2 * size + i] = gt[2];
+      kt[j] = ((kpoint[0] + idx[3 * i]) * cell.b[0][j] +
+               (kpoint[1] + idx[3 * i + 1]) * cell.b[1][j]) +
+              (kpoint[2] + idx[3 * i + 2]) * cell.b[2][j];
+    }
+    for (int j = 0; j < 3; j++) {
+      gx[j * size + i] = gt[j];
+      kpgx[j * size + i] = kt[j];
+    }
     g2[i] = dot3(gt, gt);
     g[i] = std::sqrt(g2[i]);
     gi[i] = g[i] > 0.0 ? 1.0 / g[i] : 0.0;
     g2i[i] = gi[i] * gi[i];
+    kpg2[i] = dot3(kt, kt);
+    kpg[i] = std::sqrt(kpg2[i]);
+    kpgi[i] = kpg[i] > 0.0 ? 1.0 / kpg[i] : 0.0;
   }
 }
 
@@ -243,8 +296,29 @@ void Mapping::build(const Basis& b, int n0, int n1, int n2) {
         b.idxmax[1] < np1 / 2 && b.idxmin[1] >= -np1 / 2 &&
         b.idxmax[2] < np2 / 2 && b.idxmin[2] >= -np2 / 2))
     throw std::runtime_error("basis does not fit in grid");
-  nvec = 2 * b.nrods - 1;
   ntrans0 = std::max(std::abs(b.idxmax[1]), std::abs(b.idxmin[1])) + 1;
+  if (!b.real) {
+    // complex basis: one column per rod, ip only (:90-93, :166-188, :208-220)
+    nvec = b.nrods;
+    ip.assign(b.size, 0);
+    im.assign(b.size, -1);
+    int ig = 0;
+    for (int irod = 0; irod < b.nrods; irod++)
+      for (int i = 0; i < b.rod_size[irod]; i++, ig++) {
+        int iz = i + b.rod_lmin[irod];
+        if (iz < 0) iz += np2;
+        ip[ig] = irod * np2 + iz;
+      }
+    zvec_to_val.clear();
+    for (int irod = 0; irod < b.nrods; irod++) {
+      int h = b.rod_h[irod], k = b.rod_k[irod];
+      if (h < 0) h += np0;
+      if (k < 0) k += np1;
+      zvec_to_val.push_back(h + np0 * k);
+    }
+    return;
+  }
+  nvec = 2 * b.nrods - 1;
   ip.assign(b.size, 0);
   im.assign(b.size, 0);
   int ig = 0;

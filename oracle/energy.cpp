@@ -14,7 +14,7 @@ static const double fpi = 4.0 * M_PI;
 // ---------------------------------------------------------------- setup
 void System::setup() {
   const Cell* ref = has_ref ? &refcell : nullptr;
-  wb.resize(cell, ref, ecut);
+  wb.resize(cell, ref, ecut, kpoint, force_complex);
   vb.resize(cell, ref, 4.0 * ecut);          // src/ChargeDensity.cpp:37
   int n[3];
   for (int i = 0; i < 3; i++) {              // src/ChargeDensity.cpp:44-49
@@ -87,11 +87,11 @@ void System::update_twnl() {
   twnl.assign(nsp, {});
   dtwnl.assign(nsp, {});
   lproj.assign(nsp, {});
-  const double* kpg = wb.g.data();
-  const double* kpgi = wb.gi.data();
-  const double* kx = &wb.gx[0];
-  const double* ky = &wb.gx[ngw];
-  const double* kz = &wb.gx[2 * ngw];
+  const double* kpg = wb.kpg.data();
+  const double* kpgi = wb.kpgi.data();
+  const double* kx = &wb.kpgx[0];
+  const double* ky = &wb.kpgx[ngw];
+  const double* kz = &wb.kpgx[2 * ngw];
   for (int is = 0; is < nsp; is++) {
     const Species& s = sp[is];
     const int npr = s.nlm;
@@ -417,12 +417,24 @@ void System::atoms_moved() {
 }
 
 // ------------------------------------------------------------ SlaterDet
-// src/SlaterDet.cpp:225-298 (real basis)
+// src/SlaterDet.cpp:225-298 (real basis: two bands per transform; complex
+// basis: one band per transform, rho += w occ |psi|^2 / omega)
 void compute_density(const FFT3& ft, const cplx* c, int ngw, int nst,
                      const double* occ, double weight, double omega, double* rho) {
   const long N = ft.np012();
   std::vector<cplx> tmp(N);
   const double omega_inv = 1.0 / omega;
+  if (!ft.basis->real) {
+    for (int n = 0; n < nst; n++) {
+      const double fac = weight * omega_inv * occ[n];
+      if (fac > 0.0) {
+        ft.backward(c + (long)n * ngw, tmp.data());
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < N; i++) rho[i] += fac * std::norm(tmp[i]);
+      }
+    }
+    return;
+  }
   for (int n = 0; n < nst - 1; n += 2) {
     const double fac1 = weight * omega_inv * occ[n];
     const double fac2 = weight * omega_inv * occ[n + 1];
@@ -449,11 +461,23 @@ void compute_density(const FFT3& ft, const cplx* c, int ngw, int nst,
   }
 }
 
-// src/SlaterDet.cpp:410-477 (real basis)
+// src/SlaterDet.cpp:410-477 (real basis: band pairs; complex basis: one
+// band per transform)
 void rs_mul_add(const FFT3& ft, const double* v, const cplx* c, int ngw,
                 int nst, cplx* hpsi) {
   const long N = ft.np012();
   std::vector<cplx> tmp(N), c1(ngw), c2(ngw);
+  if (!ft.basis->real) {
+    for (int n = 0; n < nst; n++) {
+      ft.backward(c + (long)n * ngw, tmp.data());
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; i++) tmp[i] *= v[i];
+      ft.forward(tmp.data(), c1.data());
+      cplx* h1 = hpsi + (long)n * ngw;
+      for (int ig = 0; ig < ngw; ig++) h1[ig] += c1[ig];
+    }
+    return;
+  }
   for (int n = 0; n < nst - 1; n += 2) {
     ft.backward(c + (long)n * ngw, c + (long)(n + 1) * ngw, tmp.data());
 #pragma omp parallel for schedule(static)
@@ -706,9 +730,119 @@ static inline void phase_l(int l, double t, double c, double s, double& re, doub
   }
 }
 
+// The complex basis (k != 0): anl and fnl complex, fnl = anl^H c (zgemm,
+// :1429-1445), enl = sum occ wt/omega Re(conj(fnl) D fnl) (:1572-1595),
+// H psi += anl fnl' (:1616-1630), forces from dfnl = (-i (k+G)_j anl)^H c
+// (:1722-1775); no stress.
+double System::nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                                 bool compute_forces, std::vector<double>& fion_enl) {
+  const int ngw = wb.size;
+  const double omega_inv = 1.0 / cell.volume;
+  const int nsp = (int)sp.size();
+  double enl = 0.0;
+  int nat = 0;
+  for (int is = 0; is < nsp; is++) nat += (int)tau[is].size() / 3;
+  fion_enl.assign(3 * nat, 0.0);
+  const double* kx[3] = {&wb.kpgx[0], &wb.kpgx[ngw], &wb.kpgx[2 * ngw]};
+  int atom_off = 0;
+  for (int is = 0; is < nsp; is++) {
+    const Species& s = sp[is];
+    const int na = (int)tau[is].size() / 3;
+    const int npr = s.nlm;
+    const int M = npr * na;
+    // anl[ig + ngw*(ia + ipr*na)] = t (-i)^l e^{-i(k+G).tau}, or -i (k+G)_dir anl
+    std::vector<cplx> anl((size_t)ngw * M);
+    auto build_anl = [&](int dir) {
+#pragma omp parallel for schedule(static) collapse(2)
+      for (int ipr = 0; ipr < npr; ipr++)
+        for (int ia = 0; ia < na; ia++) {
+          const double* t = &twnl[is][(size_t)ngw * ipr];
+          const double* tu = &tau[is][3 * ia];
+          const int l = lproj[is][ipr];
+          cplx* a = &anl[(size_t)ngw * (ia + ipr * na)];
+          for (int ig = 0; ig < ngw; ig++) {
+            const double arg = -(kx[0][ig] * tu[0] + kx[1][ig] * tu[1] + kx[2][ig] * tu[2]);
+            const int ll = dir < 0 ? l : l + 1;
+            const double tt = dir < 0 ? t[ig] : kx[dir][ig] * t[ig];
+            double re, im;
+            phase_l(ll, tt, std::cos(arg), std::sin(arg), re, im);
+            a[ig] = cplx(re, im);
+          }
+        }
+    };
+    auto project = [&](std::vector<cplx>& f) {   // f = anl^H c
+      f.assign((size_t)M * nst, cplx(0.0));
+#pragma omp parallel for schedule(static) collapse(2)
+      for (int n = 0; n < nst; n++)
+        for (int i = 0; i < M; i++) {
+          const cplx* a = &anl[(size_t)ngw * i];
+          const cplx* cn = c + (size_t)ngw * n;
+          cplx sacc(0.0);
+          for (int ig = 0; ig < ngw; ig++) sacc += std::conj(a[ig]) * cn[ig];
+          f[i + (size_t)n * M] = sacc;
+        }
+    };
+    build_anl(-1);
+    std::vector<cplx> fnl, fbuf((size_t)M * nst);
+    project(fnl);
+    for (int n = 0; n < nst; n++)
+      for (int ipr = 0; ipr < npr; ipr++)
+        for (int ia = 0; ia < na; ia++) {
+          cplx sacc(0.0);
+          for (int jpr = 0; jpr < npr; jpr++)
+            sacc += s.dmatrix(ipr, jpr) * fnl[(ia + jpr * na) + (size_t)n * M];
+          fbuf[(ia + ipr * na) + (size_t)n * M] = sacc;
+        }
+    for (int ipr = 0; ipr < npr; ipr++) {
+      const double fac = 1.0 * omega_inv;
+      for (int n = 0; n < nst; n++) {
+        const double facn = fac * occ[n];
+        for (int ia = 0; ia < na; ia++) {
+          const size_t i = ia + ipr * na + (size_t)n * M;
+          enl += facn * (fnl[i].real() * fbuf[i].real() + fnl[i].imag() * fbuf[i].imag());
+          fnl[i] = fac * fbuf[i];
+        }
+      }
+    }
+    if (compute_hpsi) {
+#pragma omp parallel for schedule(static)
+      for (int n = 0; n < nst; n++)
+        for (int i = 0; i < M; i++) {
+          const cplx f = fnl[i + (size_t)n * M];
+          const cplx* a = &anl[(size_t)ngw * i];
+          cplx* h = hpsi + (size_t)ngw * n;
+          for (int ig = 0; ig < ngw; ig++) h[ig] += a[ig] * f;
+        }
+    }
+    if (compute_forces) {
+      std::vector<cplx> dfnl;
+      for (int j = 0; j < 3; j++) {
+        build_anl(j);
+        project(dfnl);
+        for (int ipr = 0; ipr < npr; ipr++)
+          for (int n = 0; n < nst; n++) {
+            const double facn = 2.0 * occ[n];
+            for (int ia = 0; ia < na; ia++) {
+              const size_t i = ia + ipr * na + (size_t)n * M;
+              fion_enl[3 * (atom_off + ia) + j] -=
+                  facn * (fnl[i].real() * dfnl[i].real() + fnl[i].imag() * dfnl[i].imag());
+            }
+          }
+      }
+    }
+    atom_off += na;
+  }
+  return enl;
+}
+
 double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                          bool compute_forces, std::vector<double>& fion_enl,
                          bool compute_stress, double* sigma_enl) {
+  if (!wb.real) {
+    if (compute_stress) throw std::invalid_argument("stress at k != 0 is not restated");
+    for (int i = 0; i < 6; i++) sigma_enl[i] = 0.0;
+    return nl_energy_complex(c, compute_hpsi, hpsi, compute_forces, fion_enl);
+  }
   const int ngw = wb.size;
   const long K = 2L * ngw;
   const double omega_inv = 1.0 / cell.volume;
@@ -719,7 +853,7 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
   fion_enl.assign(3 * nat, 0.0);
   for (int i = 0; i < 6; i++) sigma_enl[i] = 0.0;
   double tsum[6] = {0, 0, 0, 0, 0, 0};
-  const double* kx[3] = {&wb.gx[0], &wb.gx[ngw], &wb.gx[2 * ngw]};
+  const double* kx[3] = {&wb.kpgx[0], &wb.kpgx[ngw], &wb.kpgx[2 * ngw]};
   const double* cp = reinterpret_cast<const double*>(c);
   int atom_off = 0;
   for (int is = 0; is < nsp; is++) {
@@ -848,19 +982,22 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
   for (auto& t : tau) nat += (int)t.size() / 3;
   std::vector<double> fion(3 * nat, 0.0);
   if (compute_hpsi) std::fill(hpsi, hpsi + (size_t)ngw * nst, cplx(0.0));
-  // ekin (:441-571), fac = 1 for the real basis
+  // ekin (:441-571), fac = 1 for the real basis (G and -G times 1/2), 1/2
+  // for the complex basis
+  if (compute_stress && !wb.real) throw std::invalid_argument("stress at k != 0 is not restated");
   double ts[7] = {0, 0, 0, 0, 0, 0, 0};
   {
+    const double kfac = wb.real ? 1.0 : 0.5;
     std::vector<double> psi2sum(ngw);
 #pragma omp parallel for schedule(static)
     for (int ig = 0; ig < ngw; ig++) {
       double t = 0.0;
-      for (int n = 0; n < nst; n++) t += 1.0 * occ[n] * std::norm(c[ig + (size_t)n * ngw]);
+      for (int n = 0; n < nst; n++) t += kfac * occ[n] * std::norm(c[ig + (size_t)n * ngw]);
       psi2sum[ig] = t;
     }
     for (int ig = 0; ig < ngw; ig++) {
       const double p = psi2sum[ig];
-      ts[0] += p * wb.g2[ig];
+      ts[0] += p * wb.kpg2[ig];
       if (compute_stress) {
         const double gx = wb.gx[ig], gy = wb.gx[ngw + ig], gz = wb.gx[2 * ngw + ig];
         const double f = 2.0 * p;
@@ -922,7 +1059,7 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
 #pragma omp parallel for schedule(static)
     for (int n = 0; n < nst; n++)
       for (int ig = 0; ig < ngw; ig++)
-        hpsi[ig + (size_t)n * ngw] += 0.5 * wb.g2[ig] * c[ig + (size_t)n * ngw];
+        hpsi[ig + (size_t)n * ngw] += 0.5 * wb.kpg2[ig] * c[ig + (size_t)n * ngw];
     // rs_mul_add with the potential of each band's spin (:831)
     const int nup = nspin == 2 ? nst_up : nst;
     rs_mul_add(ft, v_r.data(), c, ngw, nup, hpsi);

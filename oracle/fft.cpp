@@ -179,13 +179,18 @@ void FFT3::fwd_grid(cplx* f, std::vector<cplx>& zvec) const {
   }
 }
 
-// vector_to_zvec, src/BasisMapping.cpp:628-664 (real basis)
+// vector_to_zvec, src/BasisMapping.cpp:628-664 (real basis: G and -G;
+// complex basis: G only)
 void FFT3::backward(const cplx* c, cplx* f) const {
   std::vector<cplx> zvec((size_t)map.nvec * map.np2, cplx(0.0));
   const int ng = basis->size;
-  for (int ig = 0; ig < ng; ig++) {
-    zvec[map.ip[ig]] = c[ig];
-    zvec[map.im[ig]] = std::conj(c[ig]);
+  if (basis->real) {
+    for (int ig = 0; ig < ng; ig++) {
+      zvec[map.ip[ig]] = c[ig];
+      zvec[map.im[ig]] = std::conj(c[ig]);
+    }
+  } else {
+    for (int ig = 0; ig < ng; ig++) zvec[map.ip[ig]] = c[ig];
   }
   bwd_grid(zvec, f);
 }

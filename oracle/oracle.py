@@ -60,6 +60,11 @@ def lib():
         if not LIB.exists():
             raise RuntimeError(f"oracle library not built: {LIB}")
         _lib = C.CDLL(str(LIB))
+        _lib.qbo_create_k.restype = C.c_void_p
+        _lib.qbo_create_k.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
+                                      _ip, _dp, C.c_int, _dp, _ip, _dp, C.c_int]
+        _lib.qbo_kpg.restype = C.c_int
+        _lib.qbo_kpg.argtypes = [C.c_void_p, _dp, _dp]
         _lib.qbo_create.restype = C.c_void_p
         _lib.qbo_create.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
                                     _ip, _dp, C.c_int, _dp, _ip]
@@ -111,13 +116,16 @@ ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p",
 class Oracle:
     """One Qbox-like system on the CPU (EnergyFunctional + ChargeDensity)."""
 
-    def __init__(self, sysdef, refcell=None, grid=None):
+    def __init__(self, sysdef, refcell=None, grid=None, kpoint=None, force_complex=False):
         """refcell (3x3, rows a0 a1 a2): the reference cell that fixes the
         basis membership and grid sizes (src/Basis.cpp:314-325), so that
         strained copies of a system share one basis.  grid (np0, np1, np2):
         the density grid, as FourierTransform(basis, np0, np1, np2) takes it
         (src/FourierTransform.h:105), instead of ChargeDensity's rule; it must
-        hold the density basis."""
+        hold the density basis.  kpoint (3, in units of the reciprocal
+        vectors; default the system's own `kpoint` attribute or 0): k != 0
+        gives the complex (full-sphere) wavefunction basis, with one weight-1
+        k-point; force_complex builds that basis at k = 0 as well."""
         L = lib()
         self.sysdef = sysdef
         if refcell is None:
@@ -139,10 +147,17 @@ class Oracle:
         if grid is not None:
             g3 = np.array(grid, dtype=np.int32)
             self._keep.append(g3)
-        h = L.qbo_create(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
-                         len(sysdef.species), descs,
-                         _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ),
-                         _ptr(g3, _ip) if g3 is not None else None)
+        if kpoint is None:
+            kpoint = getattr(sysdef, "kpoint", None)
+        kp = None
+        if kpoint is not None:
+            kp = np.ascontiguousarray(kpoint, dtype=float).ravel()
+            self._keep.append(kp)
+        h = L.qbo_create_k(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
+                           len(sysdef.species), descs,
+                           _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ),
+                           _ptr(g3, _ip) if g3 is not None else None,
+                           _ptr(kp) if kp is not None else None, 1 if force_complex else 0)
         if not h:
             raise RuntimeError(L.qbo_last_error().decode())
         self.h = C.c_void_p(h)
@@ -178,6 +193,14 @@ class Oracle:
         return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip, im=im,
                     zvec_to_val=z2v, np=npb[:3].copy(), idxmin=npb[3:6].copy(),
                     idxmax=npb[6:9].copy())
+
+    def kpg(self):
+        """|k+G|^2 (ngw) and k+G (3 x ngw) of the wavefunction basis, and
+        whether the basis is real."""
+        kpg2 = np.zeros(self.ngw)
+        kpgx = np.zeros(3 * self.ngw)
+        real = lib().qbo_kpg(self.h, _ptr(kpg2), _ptr(kpgx))
+        return kpg2, kpgx.reshape(3, self.ngw), bool(real)
 
     def rods(self, which=0):
         n = self.nrods if which == 0 else self.nrods_v

@@ -30,23 +30,33 @@ struct Cell {
 };
 
 // ------------------------------------------------------------------- Basis
-// src/Basis.cpp:255-690 restated for one task and k = 0 (real basis).
+// src/Basis.cpp:255-690 restated for one task: the real basis at k = 0 and
+// the complex (full-sphere) basis at k != 0 (real_ = (k == 0), :237).
 struct Basis {
   Cell cell;
   double ecut = 0.0;            // Hartree
+  double kpoint[3] = {0.0, 0.0, 0.0};   // in units of the reciprocal vectors b
+  bool real = true;
   int idxmin[3], idxmax[3], np[3];
   int size = 0, nrods = 0;
   std::vector<int> rod_h, rod_k, rod_lmin, rod_size, rod_first;
   std::vector<int> idx;         // idx[3*i+j]
   std::vector<double> g, g2, gi, g2i, gx;   // gx[j*size+i]
-  // defcell: cell that selects the vectors (refcell if given, else cell)
-  void resize(const Cell& c, const Cell* refcell, double ecut_ha);
+  std::vector<double> kpg, kpg2, kpgi, kpgx;   // k + G (equal to g at k = 0)
+  // defcell: cell that selects the vectors (refcell if given, else cell).
+  // kpoint (null: k = 0); force_complex builds the complex branch at k = 0
+  // as well (the same formulas with k = 0; used to pin the complex path
+  // against the real one)
+  void resize(const Cell& c, const Cell* refcell, double ecut_ha,
+              const double* kpoint = nullptr, bool force_complex = false);
   void update_g();
 };
 bool factorizable(int n);       // src/Basis.cpp:105-133
 
 // ------------------------------------------------------------ BasisMapping
-// src/BasisMapping.cpp:38-228 (single task, real basis)
+// src/BasisMapping.cpp:38-228 (single task; real basis: ip and im, two
+// columns per rod but one for rod (0,0); complex basis: ip only, one column
+// per rod)
 struct Mapping {
   int np0, np1, np2, nvec;
   int ntrans0;                  // src/FourierTransform.cpp:104
@@ -61,7 +71,8 @@ struct FFT3 {
   Mapping map;
   int np012() const { return map.np0 * map.np1 * map.np2; }
   void init(const Basis& b, int n0, int n1, int n2);
-  // backward: f(r) = sum_G c(G) e^{+iGr}, unscaled
+  // backward: f(r) = sum_G c(G) e^{+iGr}, unscaled (real basis: G and -G
+  // from c(G) and conj(c(G)); complex basis: the G of the basis only)
   void backward(const cplx* c, cplx* f) const;                       // :175
   void backward(const cplx* c1, const cplx* c2, cplx* f) const;      // :187
   // forward: c(G) = 1/N sum_r f(r) e^{-iGr}; overwrites f
@@ -151,6 +162,10 @@ struct System {
   std::vector<std::vector<double>> tau;   // tau[is][3*ia]
   int nst = 0;
   std::vector<double> occ;
+  // one k-point (in units of b, weight 1): k != 0 (or force_complex) makes
+  // the wavefunction basis complex; stress is not restated there
+  double kpoint[3] = {0.0, 0.0, 0.0};
+  bool force_complex = false;
   // nspin 2: bands [0, nst_up) spin up, [nst_up, nst) spin down
   // (Wavefunction nst_[ispin], src/Wavefunction.cpp:326-336)
   int nspin = 1, nst_up = 0;
@@ -201,6 +216,8 @@ struct System {
   double nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                    bool compute_forces, std::vector<double>& fion_enl,
                    bool compute_stress, double* sigma_enl);   // NLP:1258-2069
+  double nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
+                           bool compute_forces, std::vector<double>& fion_enl);
 };
 
 // SlaterDet pieces (src/SlaterDet.cpp)

@@ -95,6 +95,10 @@ class SystemDef:
     # Wavefunction's reference cell (rows a0 a1 a2), which fixes the basis
     # membership and the grids (src/Basis.cpp:314-325); None: the cell itself
     refcell: np.ndarray = field(default=None)
+    # one k-point in units of the reciprocal vectors b0 b1 b2 (weight 1);
+    # None or 0: Gamma (the real basis); k != 0 gives the complex full-sphere
+    # basis (src/Basis.cpp:237, 446-481)
+    kpoint: np.ndarray = field(default=None)
 
     @property
     def ecut(self) -> float:         # Hartree (src/Ecut.h:38-57)
@@ -186,6 +190,12 @@ def config(index: int) -> SystemDef:
         s = SystemDef("t5: 2+1 atoms with core densities 8x8.5x9 ecut=18Ry 6 bands",
                       np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_C, SPECIES_B], [2, 1], 6,
                       BASE_SEED + 15, 2.0)
+    elif index == 16:
+        # one k-point off Gamma (complex full-sphere basis), d projectors
+        # beside an s/p species, orthorhombic cell
+        s = SystemDef("t6: 2+1 atoms at k = (0.25, 0.1, -0.2) 8x8.5x9 ecut=18Ry 6 bands",
+                      np.diag([8.0, 8.5, 9.0]), 18.0, [SPECIES_D, SPECIES_B], [2, 1], 6,
+                      BASE_SEED + 16, 2.0, kpoint=np.array([0.25, 0.1, -0.2]))
     elif index == 12:
         # non-orthogonal (monoclinic) cell
         a = np.array([[9.0, 0.0, 0.0], [1.5, 8.5, 0.0], [0.0, 0.7, 8.0]])
@@ -244,6 +254,40 @@ def random_coefficients(seed: int, g2: np.ndarray, nst: int) -> np.ndarray:
     c = (z[..., 0] + 1j * z[..., 1]) / (1.0 + g2)[None, :]
     c[:, 0] = c[:, 0].real
     return np.ascontiguousarray(c)
+
+
+def random_coefficients_k(seed: int, kpg2: np.ndarray, nst: int) -> np.ndarray:
+    """(N(0,1) + i N(0,1)) / (1 + |k+G|^2) on the full sphere of a complex
+    (k != 0) basis, shape (nst, ngw)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = rng.standard_normal((nst, kpg2.shape[0], 2))
+    return np.ascontiguousarray((z[..., 0] + 1j * z[..., 1]) / (1.0 + kpg2)[None, :])
+
+
+def orthonormal_k(c: np.ndarray) -> np.ndarray:
+    """Cholesky orthonormalization of complex-basis bands (rows of c):
+    S = c c^H = L L^H, c <- L^-1 c."""
+    S = c @ c.conj().T
+    L = np.linalg.cholesky(S)
+    return np.ascontiguousarray(np.linalg.solve(L, c))
+
+
+def supercell_x2(s: SystemDef, nst_factor: int = 1) -> SystemDef:
+    """The cell doubled along a0, every atom repeated at +a0 (species-major
+    order kept: each species' atoms, then their copies)."""
+    a = np.array(s.cell, dtype=float)
+    a2 = a.copy()
+    a2[0] *= 2.0
+    tau, na, off = [], [], 0
+    t = np.asarray(s.tau, dtype=float)
+    for n in s.na:
+        blk = t[off:off + n]
+        tau += [blk, blk + a[0]]
+        na.append(2 * n)
+        off += n
+    return SystemDef(s.name + " x2", a2, s.ecut_ry, list(s.species), na, s.nst * nst_factor,
+                     s.seed, s.min_sep, tau=np.concatenate(tau),
+                     occ_bands=None if s.occ_bands is None else np.asarray(s.occ_bands) * 2.0)
 
 
 def gram_numpy(c: np.ndarray) -> np.ndarray:
