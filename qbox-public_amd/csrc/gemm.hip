@@ -82,13 +82,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm(GemmArgs g) {
   // grid is nblk rounded up to a multiple of 8, the blocks past nblk return
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   if (b >= g.nblk) return;
-  const int ntile = g.tp * g.tq;
+  // tri == 1 (symmetric, tp == tq): only the tiles q >= p, row by row
+  const int ntile = g.tri == 1 ? g.tp * (g.tp + 1) / 2 : g.tp * g.tq;
   const int split = b / ntile, tt = b - split * ntile;
   int ip, iq;
-  if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
+  if (g.tri == 1) {
+    int r = tt;
+    ip = 0;
+    while (r >= g.tp - ip) { r -= g.tp - ip; ip++; }
+    iq = ip + r;
+  } else if (g.pfast) { ip = tt % g.tp; iq = tt / g.tp; }
   else { iq = tt % g.tq; ip = tt / g.tq; }
   const int p0 = ip * TP, q0 = iq * TQ;
-  if (g.tri == 1 && iq < ip) return;
   const long rb = (long)split * g.rper;
   long re = min(g.nr, rb + g.rper);
   if (g.tri == 2) re = min(re, (long)p0 + TP);
@@ -194,9 +199,39 @@ __global__ __launch_bounds__(256) void k_split_reduce(const double* __restrict__
 
 namespace {
 
+// compute units of the current device, cached per device
+int cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cache[dev]) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+    cache[dev] = std::max(1, ncu);
+  }
+  return cache[dev];
+}
+
+// the split count in [ceil(smax/2), smax] whose ntile*S blocks spread most
+// evenly over the CUs (blocks / (cus * rounds)); the smaller on ties
+long even_split(long smax, int ntile) {
+  if (smax <= 1) return smax;
+  const long cus = cu_count();
+  long best = smax;
+  double eff = 0.0;
+  for (long S = (smax + 1) / 2; S <= smax; S++) {
+    const long b = S * ntile;
+    const double e = (double)b / (double)(cus * ((b + cus - 1) / cus));
+    if (e > eff + 1e-9) { eff = e; best = S; }
+  }
+  return best;
+}
+
 template <typename TPm, typename TQm, typename TC, bool QR, typename TO>
 void launch_gemm(GemmArgs g, hipStream_t s) {
-  g.nblk = g.tp * g.tq * (int)((g.nr + g.rper - 1) / g.rper);
+  const int ntile = g.tri == 1 ? g.tp * (g.tp + 1) / 2 : g.tp * g.tq;
+  g.nblk = ntile * (int)((g.nr + g.rper - 1) / g.rper);
   const unsigned grid = (unsigned)((g.nblk + 7) & ~7);
   hipLaunchKernelGGL((k_gemm<TPm, TQm, TC, QR, TO>), dim3(grid), dim3(256), 0, s, g);
 }
@@ -216,15 +251,21 @@ int tn_splitk(const TQm* A, long lda, const double* B, long ldb, int M, int N, l
   g.pfast = 0;
   g.tri = sym ? 1 : 0;
   const long ksteps = (K + KC - 1) / KC;
-  const int ntile = g.tp * g.tq;
+  // tiles that do work: the upper triangle (q >= p) of a symmetric product
+  const int ntile = sym && g.tp == g.tq ? g.tp * (g.tp + 1) / 2 : g.tp * g.tq;
   // about four blocks per CU; for large outputs the partial traffic is kept
-  // well below the operand traffic (small outputs split freely)
+  // well below the operand traffic (small outputs split freely); at most 32
+  // splits for a general product (64 measured no faster on 512 x 512
+  // outputs); a symmetric product takes the count in [smax/2, smax] whose
+  // blocks spread most evenly over the CUs (ten 128-tiles at 512 bands: 51)
   long S = std::max<long>(1, (1024 + ntile - 1) / ntile);
   S = std::min<long>(S, std::max<long>(1, ksteps / 4));
   if ((long)M * N > (1L << 20))
     S = std::min<long>(S, std::max<long>(1, (long)((double)K * (M + N) / (4.0 * M * N))));
+  if (!sym) S = std::min<long>(S, 32);
   while (S > 1 && (size_t)S * M * N > work_doubles) S--;
   if ((size_t)S * M * N > work_doubles) return -1;
+  if (sym) S = even_split(S, ntile);
   g.rper = ((ksteps + S - 1) / S) * KC;
   S = (K + g.rper - 1) / g.rper;
   g.out = work; g.ldo = M; g.sstride = (long)M * N; g.accumulate = 0;

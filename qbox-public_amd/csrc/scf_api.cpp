@@ -89,7 +89,13 @@ int polar_enqueue(const PolarWs& w, double* Y, int n, double tol, int maxiter, d
   return QBG_OK;
 }
 
-size_t gemm_work(int nst) { return std::max<size_t>((size_t)1 << 20, (size_t)32 * nst * nst); }
+// split-K partials of the n x n products: up to 64 splits for n <= 1024 (the
+// symmetric overlap at 512 bands has ten 128-tiles and fills 256 CUs twice
+// at 51 splits), 32 above
+size_t gemm_work(int nst) {
+  const size_t n2 = (size_t)nst * nst;
+  return std::max<size_t>((size_t)1 << 20, (nst <= 1024 ? 64 : 32) * n2);
+}
 
 }  // namespace
 
