@@ -242,12 +242,30 @@ typedef struct {
 int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut, int nsp,
                   const qbg_species_desc* species, const int* na,
                   const double* tau, int nst, const double* occ, qbg_ef** out);
+/* The same for one k-point off Gamma, weight 1 (the Wavefunction k-point
+ * of src/Wavefunction.cpp:400-456 with src/Basis.cpp:237, 446-481):
+ * kpoint3 in units of the reciprocal vectors b0,b1,b2 (0,0,0 gives the
+ * Gamma functional of qbg_ef_create).  The wavefunction basis is then the
+ * full sphere |k+G| < sqrt(2 ecut) of the reference's complex Basis, in its
+ * order (qbg_ef_basis which 0 and 2), and the coefficients are complex on
+ * it; update_density, update_vhxc and the evaluation (energies, H*psi,
+ * forces) follow src/EnergyFunctional.cpp:397-1044 and
+ * src/NonLocalPotential.cpp:1429-1775 for the complex basis, in FP64 or
+ * FP32.  Not built at k != 0 (QBG_E_INVAL): the BF16 precisions, the
+ * stress, nspin 2, cell_moved, the NonLocalPotential and wavefunction-grid
+ * entry points and the SCF steppers. */
+int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
+                    const double* kpoint3, int nsp, const qbg_species_desc* species,
+                    const int* na, const double* tau, int nst, const double* occ, qbg_ef** out);
 int qbg_ef_destroy(qbg_ef* ef);
 /* out[10]: ngw, ngv, np0, np1, np2, nvec_w, nvec_v, nat, nproj, nst */
 int qbg_ef_sizes(const qbg_ef* ef, int* out);
 /* host copies of the wavefunction (which=0) or density (which=1) basis:
  * idx[3*ng], g2[ng], gx[3*ng] (x block, y block, z block), maps ip[ng],
- * im[ng], zvec_to_val[nvec]; any pointer may be null */
+ * im[ng], zvec_to_val[nvec]; any pointer may be null.  which=2: the
+ * wavefunction basis with g2 = |k+G|^2 and gx = k+G (Basis::kpg2, kpgx;
+ * equal to which=0 at Gamma).  A k != 0 basis has no maps (they must be
+ * null). */
 int qbg_ef_basis(const qbg_ef* ef, int which, int32_t* idx, double* g2,
                  double* gx, int32_t* ip, int32_t* im, int32_t* zvec_to_val);
 /* EnergyFunctional::atoms_moved() with new positions (host tau[3*nat]) */

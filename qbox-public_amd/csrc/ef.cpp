@@ -8,6 +8,7 @@
 #include <mutex>
 #include <new>
 #include <tuple>
+#include <unordered_map>
 #include "ef.h"
 #include "energy.h"
 #include "internal.h"
@@ -71,6 +72,14 @@ static void spin_bands(const qbg_ef* e, int s, int& b0, int& nb, int& pair0) {
 // second band of a spin's last pair contributes 0)
 static std::vector<double> density_factors(const qbg_ef* e) {
   std::vector<double> f;
+  if (e->cplx) {
+    // one complex band per pair: both real parts carry its occupation
+    for (int n = 0; n < e->nst; n++) {
+      f.push_back(e->occ[n] / e->omega);
+      f.push_back(e->occ[n] / e->omega);
+    }
+    return f;
+  }
   for (int s = 0; s < e->nspin; s++) {
     int b0, nb, p0;
     spin_bands(e, s, b0, nb, p0);
@@ -79,6 +88,12 @@ static std::vector<double> density_factors(const qbg_ef* e) {
     for (int n = 0; n < nb; n++) f[base + n] = e->occ[b0 + n] / e->omega;
   }
   return f;
+}
+
+static std::vector<double> occ_half(const std::vector<double>& occ) {
+  std::vector<double> h(occ);
+  for (double& x : h) x *= 0.5;
+  return h;
 }
 
 static void ef_free(qbg_ef* e) {
@@ -93,7 +108,8 @@ static void ef_free(qbg_ef* e) {
                e->b_y32, e->b_fout, e->b_w32, e->b_part, e->d_lfgrp, e->d_vtemp, e->d_dfac,
                e->d_seg_ptr, e->d_seg_n, e->d_rhor2, e->d_vxc2, e->d_vr2, e->d_vloc,
                e->d_rhot, e->b_fpart, e->b_fred, e->d_rhoc, e->d_drhoc, e->d_rhocg,
-               e->d_rhocr, e->d_vxcg, e->b_yh};
+               e->d_rhocr, e->d_vxcg, e->b_yh, e->d_fold, e->d_unfold, e->d_a_m0h,
+               e->d_cpair, e->d_hpair, e->d_occh};
   for (void* q : p)
     if (q) (void)hipFree(q);
   if (e->h_res) (void)hipHostFree(e->h_res);
@@ -266,7 +282,7 @@ static void harmonic(int ipr, const double* u, double& y, double* dy) {
 // NonLocalPotential::update_twnl on the wavefunction basis of the current
 // cell (src/NonLocalPotential.cpp:235-1245, KB projectors l <= 3): twnl[is] =
 // [ipr][ig] at offset sum_{js<is} npr_js ngw, dtwnl[is] = [ipr][ij][ig] at
-// six times that offset.  twnl = y(u) v_l(|G|), u = G/|G|; the strain
+// six times that offset.  twnl = y(u) v_l(|q|), u = q/|q|, q = k + G; the strain
 // derivative is dtwnl_ij = y dv |G| u_i u_j - v sum_a dy/du_a (u_a u_i u_j -
 // delta_aj u_i), ij = xx yy zz xy yz xz, which is the reference's
 // expressions for l = 0, 1, 2 (:283-306, :426-512, :627-760) written once.
@@ -294,9 +310,10 @@ static void ef_twnl(const qbg_ef* e, std::vector<double>& twnl, std::vector<doub
     const int nq = q.quad() ? q.nquad : 1;
     std::vector<double> rv((size_t)(q.lmax + 1) * nq), rdv((size_t)(q.lmax + 1) * nq);
     for (int ig = 0; ig < ngw; ig++) {
-      const double tg = e->wb.g[ig], tgi = e->wb.gi[ig];
-      const double u[3] = {e->wb.gx[ig] * tgi, e->wb.gx[ngw + ig] * tgi,
-                           e->wb.gx[2 * ngw + ig] * tgi};
+      // k + G (G itself at k = 0)
+      const double tg = e->wb.kpg[ig], tgi = e->wb.kpgi[ig];
+      const double u[3] = {e->wb.kpgx[ig] * tgi, e->wb.kpgx[ngw + ig] * tgi,
+                           e->wb.kpgx[2 * ngw + ig] * tgi};
       for (int l = 0; l <= q.lmax; l++) {
         if (!q.quad()) {
           q.dvnlg(l, tg, rv[l], rdv[l]);
@@ -344,12 +361,44 @@ static int ef_atoms_moved(qbg_ef* e) {
   return QBG_OK;
 }
 
-extern "C" {
+// the index tables between the complex basis wb and the real band basis ub
+// (see qbg_ef::cplx): fold[2p + {0,1}] over ub, unfold[j] over wb
+static bool kpoint_maps(const host::Basis& wb, const host::Basis& ub, std::vector<int32_t>& fold,
+                        std::vector<int32_t>& unfold) {
+  auto key = [](int h, int k, int l) -> int64_t {
+    return ((int64_t)(h + 4096) << 26) | ((int64_t)(k + 4096) << 13) | (int64_t)(l + 4096);
+  };
+  std::unordered_map<int64_t, int> in_wb;
+  in_wb.reserve((size_t)wb.size * 2);
+  for (int j = 0; j < wb.size; j++) in_wb[key(wb.idx[3 * j], wb.idx[3 * j + 1], wb.idx[3 * j + 2])] = j;
+  std::unordered_map<int64_t, int> in_ub;
+  in_ub.reserve((size_t)ub.size * 2);
+  fold.assign(2 * (size_t)ub.size, -1);
+  for (int p = 0; p < ub.size; p++) {
+    const int h = ub.idx[3 * p], k = ub.idx[3 * p + 1], l = ub.idx[3 * p + 2];
+    in_ub[key(h, k, l)] = p;
+    auto a = in_wb.find(key(h, k, l)), b = in_wb.find(key(-h, -k, -l));
+    if (a != in_wb.end()) fold[2 * p] = a->second;
+    if (b != in_wb.end()) fold[2 * p + 1] = b->second;
+  }
+  unfold.assign(wb.size, 0);
+  for (int j = 0; j < wb.size; j++) {
+    const int h = wb.idx[3 * j], k = wb.idx[3 * j + 1], l = wb.idx[3 * j + 2];
+    auto a = in_ub.find(key(h, k, l));
+    if (a != in_ub.end()) { unfold[j] = a->second; continue; }
+    auto b = in_ub.find(key(-h, -k, -l));
+    if (b == in_ub.end()) return false;
+    unfold[j] = -1 - b->second;
+  }
+  return true;
+}
 
-int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut, int nsp,
-                  const qbg_species_desc* spd, const int* na, const double* tau, int nst,
-                  const double* occ, qbg_ef** out) {
-  return qbg::guarded(c, [&]() -> int {
+// EnergyFunctional's constructor (src/EnergyFunctional.cpp:67-226) for one
+// k-point: kp null or 0 gives the real (Gamma) basis
+static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut,
+                     const double* kp, int nsp, const qbg_species_desc* spd, const int* na,
+                     const double* tau, int nst, const double* occ, qbg_ef** out) {
+  {
     if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
       return QBG_E_INVAL;
     *out = nullptr;
@@ -413,7 +462,8 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     if (refcell9) def.set(refcell9);
     e->defcell = def;
     e->has_ref = refcell9 != nullptr;
-    e->wb.build(def, e->cell, ecut);
+    e->cplx = kp && (kp[0] != 0.0 || kp[1] != 0.0 || kp[2] != 0.0);
+    e->wb.build(def, e->cell, ecut, e->cplx ? kp : nullptr);
     e->vb.build(def, e->cell, 4.0 * ecut);
     for (int i = 0; i < 3; i++) {
       int n = e->vb.np[i] + 2;
@@ -421,13 +471,35 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
       e->np[i] = n;
     }
     e->N = (long)e->np[0] * e->np[1] * e->np[2];
-    if (!e->wm.build(e->wb, e->np[0], e->np[1], e->np[2]) ||
+    // the basis the band transforms run on: wb, or at k != 0 the real basis
+    // of every |G| <= max over wb of |G| (norms in the defining cell)
+    const host::Basis* band = &e->wb;
+    std::vector<int32_t> fold, unfold;
+    if (e->cplx) {
+      double g2max = 0.0;
+      for (int j = 0; j < e->wb.size; j++) {
+        double t2 = 0.0;
+        for (int a = 0; a < 3; a++) {
+          const double t = (e->wb.idx[3 * j] * def.b[0][a] + e->wb.idx[3 * j + 1] * def.b[1][a]) +
+                           e->wb.idx[3 * j + 2] * def.b[2][a];
+          t2 += t * t;
+        }
+        g2max = std::max(g2max, t2);
+      }
+      e->ub.build(def, e->cell, 0.5 * (g2max * (1.0 + 1e-9) + 1e-12));
+      band = &e->ub;
+      if (!kpoint_maps(e->wb, e->ub, fold, unfold)) {
+        delete e;
+        return c->fail(QBG_E_INVAL, "k-point basis: a vector outside the band basis");
+      }
+    }
+    if (!e->wm.build(*band, e->np[0], e->np[1], e->np[2]) ||
         !e->vm.build(e->vb, e->np[0], e->np[1], e->np[2])) {
       delete e;
       return c->fail(QBG_E_INVAL, "basis does not fit in the grid");
     }
     int rc;
-    if ((rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->wb.size, e->wm.nvec,
+    if ((rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], band->size, e->wm.nvec,
                               e->wm.ip.data(), e->wm.im.data(), e->wm.zvec_to_val.data(), &e->gw)) ||
         (rc = qbg_grid_create(c, e->np[0], e->np[1], e->np[2], e->vb.size, e->vm.nvec,
                               e->vm.ip.data(), e->vm.im.data(), e->vm.zvec_to_val.data(), &e->gv))) {
@@ -484,7 +556,25 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
         a_stride[a] = e->na[is];
       }
     }
-    e->M = (int)m_atom.size();
+    e->M = e->Mh = (int)m_atom.size();
+    std::vector<int32_t> a_m0h;
+    if (e->cplx) {
+      // complex projections as two real ones: columns Mh + m are i times
+      // columns m ((-i)^(l+3) = i (-i)^l), same twnl, atom and D block
+      const int Mh = e->Mh;
+      for (int m = 0; m < Mh; m++) {
+        m_atom.push_back(m_atom[m]);
+        m_twnl.push_back(m_twnl[m]);
+        m_l.push_back((m_l[m] + 3) & 3);
+        m_ipr.push_back(m_ipr[m]);
+        m_npr.push_back(m_npr[m]);
+        m_stride.push_back(m_stride[m]);
+        m_dbase.push_back(m_dbase[m]);
+        m_wt.push_back(m_wt[m]);
+      }
+      e->M = 2 * Mh;
+      for (int a = 0; a < e->nat; a++) a_m0h.push_back(a_m0[a] + Mh);
+    }
     // structure factor table extents (src/StructureFactor.cpp:27-39)
     int krange = 0;
     for (int i = 0; i < 3; i++) {
@@ -519,7 +609,7 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
     // split-K partials of the fnl product: up to 40 splits of M x nst
     e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
-    if ((rc = dput(c, &e->d_g2w, e->wb.g2)) || (rc = dput(c, &e->d_gxw, e->wb.gx)) ||
+    if ((rc = dput(c, &e->d_g2w, e->wb.kpg2)) || (rc = dput(c, &e->d_gxw, e->wb.kpgx)) ||
         (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
         (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
         (rc = dput(c, &e->d_vps, vps)) || (rc = dput(c, &e->d_dvps, dvps)) ||
@@ -552,7 +642,12 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
         (e->core && ((rc = dput(c, &e->d_rhoc, rhoc)) || (rc = dput(c, &e->d_drhoc, drhoc)) ||
                      (rc = dalloc(c, &e->d_rhocg, (size_t)2 * ngv)) ||
                      (rc = dalloc(c, &e->d_rhocr, N)) ||
-                     (rc = dalloc(c, &e->d_vxcg, (size_t)2 * ngv))))) {
+                     (rc = dalloc(c, &e->d_vxcg, (size_t)2 * ngv)))) ||
+        (e->cplx && ((rc = dput(c, &e->d_fold, fold)) || (rc = dput(c, &e->d_unfold, unfold)) ||
+                     (rc = dput(c, &e->d_a_m0h, a_m0h)) ||
+                     (rc = dput(c, &e->d_occh, occ_half(e->occ))) ||
+                     (rc = dalloc(c, &e->d_cpair, (size_t)4 * e->ub.size * nst)) ||
+                     (rc = dalloc(c, &e->d_hpair, (size_t)4 * e->ub.size * nst))))) {
       ef_free(e);
       delete e;
       return rc;
@@ -584,6 +679,25 @@ int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, doubl
     }
     *out = e;
     return QBG_OK;
+  }
+}
+
+extern "C" {
+
+int qbg_ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut, int nsp,
+                  const qbg_species_desc* spd, const int* na, const double* tau, int nst,
+                  const double* occ, qbg_ef** out) {
+  return qbg::guarded(c, [&]() -> int {
+    return ef_create(c, cell9, refcell9, ecut, nullptr, nsp, spd, na, tau, nst, occ, out);
+  });
+}
+
+int qbg_ef_create_k(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut,
+                    const double* kpoint3, int nsp, const qbg_species_desc* spd, const int* na,
+                    const double* tau, int nst, const double* occ, qbg_ef** out) {
+  return qbg::guarded(c, [&]() -> int {
+    if (!kpoint3) return QBG_E_INVAL;
+    return ef_create(c, cell9, refcell9, ecut, kpoint3, nsp, spd, na, tau, nst, occ, out);
   });
 }
 
@@ -609,17 +723,21 @@ int qbg_ef_sizes(const qbg_ef* e, int* out) {
   });
 }
 
-// host copies of a basis (which 0: wavefunctions, 1: density):
-// idx[3*ng], g2[ng], gx[3*ng] (each may be null), and the maps
+// host copies of a basis (which 0: wavefunctions, 1: density, 2: the
+// wavefunctions' k + G): idx[3*ng], g2[ng], gx[3*ng] (each may be null), and
+// the maps (real bases only)
 int qbg_ef_basis(const qbg_ef* e, int which, int32_t* idx, double* g2, double* gx, int32_t* ip,
                  int32_t* im, int32_t* zvec_to_val) {
   return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
-    if (!e) return QBG_E_INVAL;
-    const host::Basis& b = which ? e->vb : e->wb;
-    const host::Maps& m = which ? e->vm : e->wm;
+    if (!e || which < 0 || which > 2) return QBG_E_INVAL;
+    const host::Basis& b = which == 1 ? e->vb : e->wb;
+    const host::Maps& m = which == 1 ? e->vm : e->wm;
+    if (which != 1 && e->cplx && (ip || im || zvec_to_val))
+      return e->ctx->fail(QBG_E_INVAL, "basis: the complex (k != 0) basis has no band-grid maps");
+    const bool kq = which == 2;
     if (idx) std::memcpy(idx, b.idx.data(), sizeof(int32_t) * b.idx.size());
-    if (g2) std::memcpy(g2, b.g2.data(), sizeof(double) * b.size);
-    if (gx) std::memcpy(gx, b.gx.data(), sizeof(double) * b.gx.size());
+    if (g2) std::memcpy(g2, (kq ? b.kpg2 : b.g2).data(), sizeof(double) * b.size);
+    if (gx) std::memcpy(gx, (kq ? b.kpgx : b.gx).data(), sizeof(double) * b.gx.size());
     if (ip) std::memcpy(ip, m.ip.data(), sizeof(int32_t) * m.ip.size());
     if (im) std::memcpy(im, m.im.data(), sizeof(int32_t) * m.im.size());
     if (zvec_to_val) std::memcpy(zvec_to_val, m.zvec_to_val.data(), sizeof(int32_t) * m.nvec);
@@ -645,6 +763,7 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
     if (!e || !cell9) return QBG_E_INVAL;
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "cell_moved during graph capture");
+    if (e->cplx) return c->fail(QBG_E_INVAL, "cell_moved: real (Gamma) basis only");
     host::Cell cell;
     cell.set(cell9);
     if (!(cell.volume > 0.0)) return c->fail(QBG_E_INVAL, "cell_moved: singular cell");
@@ -711,10 +830,19 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
     spin_bands(e, sp, b0, nb, p0);
     pairs = p0 + (nb + 1) / 2;
   }
-  const bool res = keep && grid_ensure_xres(e->gw, pairs);
+  const bool res = !e->cplx && keep && grid_ensure_xres(e->gw, pairs);
   int rc = QBG_OK;
+  if (e->cplx) {
+    // each complex band as the pair of its real and imaginary parts on the
+    // real band basis, both with the band's occupation
+    ctx->prof_begin(QBG_K_OTHER);
+    launch_fold(c, ldc, e->nst, e->ub.size, e->d_fold, e->d_cpair, e->ub.size, s);
+    ctx->prof_end();
+    rc = density_batches(e->gw, e->d_cpair, e->ub.size, 2 * e->nst, e->d_dfac, e->d_rhor, fprec,
+                         false, 0);
+  }
   // one density per spin (src/ChargeDensity.cpp:136-160)
-  for (int sp = 0; sp < e->nspin && !rc; sp++) {
+  for (int sp = 0; sp < e->nspin && !rc && !e->cplx; sp++) {
     int b0, nb, p0;
     spin_bands(e, sp, b0, nb, p0);
     if (nb > 0)
@@ -848,12 +976,20 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   float* anl32 = reinterpret_cast<float*>(e->d_anl);
   launch_anl(ngw, e->nat, K, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
              e->d_m_twnl, e->d_m_l, e->d_twnl, e->d_anl, f32 ? 1 : 0, s);
+  // k != 0: the second half of the columns, i anl (the imaginary part of
+  // the complex projection anl^H c comes out of the same real TN product)
+  if (e->cplx)
+    launch_anl(ngw, e->nat, K, e->d_gxw, e->d_tau, e->d_a_m0h, e->d_a_npr, e->d_a_stride,
+               e->d_m_twnl, e->d_m_l, e->d_twnl, e->d_anl, f32 ? 1 : 0, s);
   ctx->prof_end();
+  // Gamma: fnl = 2 anl^T c - anl0 c0 (half sphere); k != 0: the full sphere
+  // with neither the factor nor the G = 0 term (src/NonLocalPotential.cpp:1399-1445)
+  const double alpha = e->cplx ? 1.0 : 2.0, gamma = e->cplx ? 0.0 : -1.0;
   ctx->prof_begin(QBG_K_GEMM);
-  const int grc = f32 ? gemm_tn_splitk(anl32, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl, M,
-                                       e->d_work, e->work_doubles, prec, s)
-                      : gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, 2.0, -1.0, e->d_fnl,
-                                       M, e->d_work, e->work_doubles, prec, s);
+  const int grc = f32 ? gemm_tn_splitk(anl32, K, c, 2L * ldc, M, nst, K, alpha, gamma, e->d_fnl,
+                                       M, e->d_work, e->work_doubles, prec, s)
+                      : gemm_tn_splitk(e->d_anl, K, c, 2L * ldc, M, nst, K, alpha, gamma,
+                                       e->d_fnl, M, e->d_work, e->work_doubles, prec, s);
   ctx->prof_end();
   if (grc) return ctx->fail(QBG_E_INVAL, "gemm workspace too small");
   ctx->prof_begin(QBG_K_NL);
@@ -876,7 +1012,9 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces)
       fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
-    launch_force_w(e->d_fout, occ, M, nst, W, s);
+    // the force kernel's factor 4 is 2 (|F|^2) x 2 (G and -G); the full
+    // sphere of k != 0 takes half the occupation
+    launch_force_w(e->d_fout, e->cplx ? e->d_occh + b0 : occ, M, nst, W, s);
     ctx->prof_end();
     ctx->prof_begin(QBG_K_GEMM);
     if (f32) gemm_nn(c, 2L * ldc, W, nst, K, M, nst, (float*)Y, K, 0, prec, s);
@@ -884,12 +1022,16 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     ctx->prof_end();
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces) {
-      if (f32)
-        launch_nl_forces_y(anl32, (const float*)Y, K, ngw, e->d_gxw, e->d_a_m0, e->d_a_npr,
-                           e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
-      else
-        launch_nl_forces_y(e->d_anl, (const double*)Y, K, ngw, e->d_gxw, e->d_a_m0,
-                           e->d_a_npr, e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+      // k != 0: the i anl half again, at its own column offset
+      for (int half = 0; half < (e->cplx ? 2 : 1); half++) {
+        const int32_t* am0 = half ? e->d_a_m0h : e->d_a_m0;
+        if (f32)
+          launch_nl_forces_y(anl32, (const float*)Y, K, ngw, e->d_gxw, am0, e->d_a_npr,
+                             e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+        else
+          launch_nl_forces_y(e->d_anl, (const double*)Y, K, ngw, e->d_gxw, am0, e->d_a_npr,
+                             e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+      }
     }
     if (compute_stress) {
       if (f32)
@@ -1119,6 +1261,11 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   if (compute_stress && !e->vhxc_stress)
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
+  if (e->cplx && (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2))
+    return ctx->fail(QBG_E_INVAL, "energy at k != 0: FP64 or FP32 (the BF16 products are not built "
+                                  "for the complex basis)");
+  if (e->cplx && compute_stress)
+    return ctx->fail(QBG_E_INVAL, "energy at k != 0: the stress is not built for the complex basis");
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;   // band FFTs
   QBG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
@@ -1135,7 +1282,9 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   // part, so neither the kinetic pass nor the NN product touches H*psi
   const bool fin = split && compute_hpsi && hpsi != c && zfwd_fin_ok(e->gw);
   ctx->prof_begin(QBG_K_OTHER);
-  launch_kinetic(c, ldc, nst, e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh,
+  // (the half sphere's G and -G times 1/2 is occ |c|^2 |G|^2; the full sphere
+  // of k != 0 takes occ/2 |c|^2 |k+G|^2)
+  launch_kinetic(c, ldc, nst, e->cplx ? e->d_occh : e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh,
                  compute_hpsi && !fin, e->d_partial, e->d_scal + S_KIN, s,
                  split ? e->b_c32 : nullptr, split ? e->Kp : 0);
   ctx->prof_end();
@@ -1157,8 +1306,21 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
                           e->d_partial, e->d_scal + S_HPS, s);
     ctx->prof_end();
   }
-  // local potential: hpsi += FFT(v_r * FFT^-1 c)
-  if (compute_hpsi) {
+  // local potential: hpsi += FFT(v_r * FFT^-1 c); at k != 0 on the folded
+  // pairs, unfolded into hpsi
+  if (compute_hpsi && e->cplx) {
+    const int nu = e->ub.size;
+    ctx->prof_begin(QBG_K_OTHER);
+    launch_fold(c, ldc, nst, nu, e->d_fold, e->d_cpair, nu, s);
+    fill_zero(e->d_hpair, (size_t)4 * nu * nst, s);
+    ctx->prof_end();
+    const int rc = rs_mul_add_bands(e->gw, e->d_vr, e->d_cpair, nu, 2 * nst, e->d_hpair, nu,
+                                    fprec, nullptr);
+    if (rc) return rc;
+    ctx->prof_begin(QBG_K_OTHER);
+    launch_unfold_add(e->d_hpair, nu, nst, ngw, e->d_unfold, hpsi, ldh, s);
+    ctx->prof_end();
+  } else if (compute_hpsi) {
     // with the potential of each band's spin (src/EnergyFunctional.cpp:831)
     const bool kept = reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec;
     for (int sp = 0; sp < e->nspin; sp++) {
@@ -1306,6 +1468,7 @@ int qbg_nl_create(qbg_ef* e, int ispin, qbg_nl** out) {
     if (!e || !out) return QBG_E_INVAL;
     *out = nullptr;
     if (ispin < 0 || ispin >= e->nspin) return e->ctx->fail(QBG_E_INVAL, "nl: ispin");
+    if (e->cplx) return e->ctx->fail(QBG_E_INVAL, "nl: the complex (k != 0) basis runs inside qbg_ef_* only");
     qbg_nl* nl = new qbg_nl;
     nl->ef = e;
     nl->ctx = e->ctx;
@@ -1436,6 +1599,7 @@ int qbg_ef_wf_grid(qbg_ef* e, int* dims, qbg_grid** out) {
     if (!e || !out) return QBG_E_INVAL;
     *out = nullptr;
     qbg_ctx* c = e->ctx;
+    if (e->cplx) return c->fail(QBG_E_INVAL, "wf_grid: real (Gamma) basis only");
     if (!e->gwf) {
       if (c->capturing) return c->fail(QBG_E_STATE, "wf_grid during graph capture");
       qbg::host::Maps m;
@@ -1470,6 +1634,7 @@ int qbg_ef_gemm_plan(const qbg_ef* e, int* out) {
 int qbg_ef_band_grid(qbg_ef* e, qbg_grid** out) {
   return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
     if (!e || !out) return QBG_E_INVAL;
+    if (e->cplx) return e->ctx->fail(QBG_E_INVAL, "band_grid: real (Gamma) basis only");
     *out = e->gw;
     return e->gw ? QBG_OK : e->ctx->fail(QBG_E_STATE, "band grid not built");
   });
@@ -1484,6 +1649,7 @@ int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
     if (nspin == 1) nst_up = 0;
+    if (nspin == 2 && e->cplx) return c->fail(QBG_E_INVAL, "set_nspin: nspin 2 at k != 0 is not built");
     if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
       return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
     QBG_HIP(c, hipSetDevice(c->device));

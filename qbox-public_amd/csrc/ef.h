@@ -108,5 +108,20 @@ struct qbg_ef {
   // xres_c = nullptr when there is none (consumed by an H*psi or not kept)
   const double* xres_c = nullptr;
   int xres_ldc = 0, xres_prec = -1;
+  // one k-point off Gamma (qbg_ef_create_k): wb is then the complex basis
+  // |k+G| < sqrt(2 ecut) and d_g2w, d_gxw hold |k+G|^2 and k+G; the band
+  // transforms run on ub, a real basis holding every G of wb and its -G, on
+  // which each complex band travels as two real bands (the real and the
+  // imaginary part of psi(r)): fold[2p], fold[2p+1] = the wb index of G_p
+  // and of -G_p (or -1), unfold[j] = p (G_j is ub vector p) or -1-p (-G_j
+  // is); cpair/hpair: the folded bands and their H*psi (ub.size x 2 nst);
+  // occh = occ/2 (the full sphere's kinetic and force sums); the nonlocal
+  // projectors are doubled, M = 2 Mh: columns Mh.. are i times columns 0..
+  // (phase (-i)^(l+3)), a_m0h = a_m0 + Mh
+  bool cplx = false;
+  qbg::host::Basis ub;
+  int Mh = 0;
+  int32_t *d_fold = nullptr, *d_unfold = nullptr, *d_a_m0h = nullptr;
+  double *d_cpair = nullptr, *d_hpair = nullptr, *d_occh = nullptr;
   std::vector<void*> allocs;
 };

@@ -58,6 +58,13 @@ void launch_nl_forces(const double* fout, const double* dF, int M, int nst, cons
 
 void launch_force_w(const double* fout, const double* occ, int M, int nst, double* W,
                     hipStream_t s);
+// complex (k != 0) bands on a real band grid (kpoint.hip): each band n of c
+// (on S) to the pair (c1, c2) at columns 2n, 2n+1 of out (on U's half
+// sphere), and the pair transform's output back, added into hpsi
+void launch_fold(const double* c, long ldc, int nst, int ngwu, const int32_t* fold, double* out,
+                 long ldu, hipStream_t s);
+void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_t* unfold,
+                       double* hpsi, long ldh, hipStream_t s);
 template <typename T>
 void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double* gx,
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,

@@ -186,6 +186,7 @@ static int wfstep_create(qbg_ef* ef, int kind, double param, int nst, qbg_wfstep
   if (!ef) return QBG_E_INVAL;
   qbg_ctx* c = ef->ctx;
   if (kind < QBG_WF_SD || kind > QBG_WF_PSDA) return c->fail(QBG_E_INVAL, "wfstep: unknown kind");
+  if (ef->cplx) return c->fail(QBG_E_INVAL, "wfstep: real (Gamma) basis only");
   if (nst == 0 && ef->nspin != 1)
     return c->fail(QBG_E_INVAL, "wfstep: with nspin 2 create one stepper per spin "
                                 "(qbg_wfstep_create_bands)");

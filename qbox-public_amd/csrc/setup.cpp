@@ -81,7 +81,9 @@ bool factorizable(int n) {
 
 // Rods of the Gamma half sphere: (0,0) first, then (0,k>0), then (h>0,k);
 // stable by size with rod (0,0) exchanged into slot 0 afterwards.
-void Basis::build(const Cell& def, const Cell& cell, double ecut) {
+void Basis::build(const Cell& def, const Cell& cell, double ecut, const double* kp) {
+  for (int i = 0; i < 3; i++) kpoint[i] = kp ? kp[i] : 0.0;
+  real = kpoint[0] == 0.0 && kpoint[1] == 0.0 && kpoint[2] == 0.0;
   const double two_ecut = 2.0 * ecut;
   const double fac = std::sqrt(two_ecut) / (2.0 * M_PI);
   int nmax[3];
@@ -93,8 +95,27 @@ void Basis::build(const Cell& def, const Cell& cell, double ecut) {
   std::vector<Rod> rods;
   const double b2inv2 = 1.0 / dot3(b2, b2);
   const int lend0 = (int)std::sqrt(two_ecut * b2inv2);
-  rods.push_back({0, 0, 0, lend0 + 1});
   int used[3][2] = {{0, 0}, {0, 0}, {0, lend0}};   // [axis][min,max]
+  if (real) rods.push_back({0, 0, 0, lend0 + 1});
+  else
+    for (int i = 0; i < 3; i++) { used[i][0] = nmax[i]; used[i][1] = -nmax[i]; }
+  // a rod (h, k) of the full sphere around -k (:446-481)
+  auto add_rod_k = [&](int h, int k) {
+    int lo = nmax[2], hi = -nmax[2];
+    for (int l = -nmax[2]; l <= nmax[2]; l++) {
+      double g2 = 0.0;
+      for (int j = 0; j < 3; j++) {
+        const double t = ((kpoint[0] + h) * b0[j] + (kpoint[1] + k) * b1[j]) + (kpoint[2] + l) * b2[j];
+        g2 += t * t;
+      }
+      if (g2 < two_ecut) { lo = std::min(lo, l); hi = std::max(hi, l); }
+    }
+    if (lo > hi) return;
+    rods.push_back({h, k, lo, hi - lo + 1});
+    used[0][0] = std::min(used[0][0], h); used[0][1] = std::max(used[0][1], h);
+    used[1][0] = std::min(used[1][0], k); used[1][1] = std::max(used[1][1], k);
+    used[2][0] = std::min(used[2][0], lo); used[2][1] = std::max(used[2][1], hi);
+  };
   auto add_rod = [&](int h, int k) {
     int lo = nmax[2], hi = -nmax[2];
     for (int l = -nmax[2]; l <= nmax[2]; l++) {
@@ -111,9 +132,14 @@ void Basis::build(const Cell& def, const Cell& cell, double ecut) {
     used[1][0] = std::min(used[1][0], k); used[1][1] = std::max(used[1][1], k);
     used[2][0] = std::min(used[2][0], lo); used[2][1] = std::max(used[2][1], hi);
   };
-  for (int k = 1; k <= nmax[1]; k++) add_rod(0, k);
-  for (int h = 1; h <= nmax[0]; h++)
-    for (int k = -nmax[1]; k <= nmax[1]; k++) add_rod(h, k);
+  if (real) {
+    for (int k = 1; k <= nmax[1]; k++) add_rod(0, k);
+    for (int h = 1; h <= nmax[0]; h++)
+      for (int k = -nmax[1]; k <= nmax[1]; k++) add_rod(h, k);
+  } else {
+    for (int h = -nmax[0]; h <= nmax[0]; h++)
+      for (int k = -nmax[1]; k <= nmax[1]; k++) add_rod_k(h, k);
+  }
   for (int i = 0; i < 3; i++) { idxmin[i] = used[i][0]; idxmax[i] = used[i][1]; }
   for (int i = 0; i < 3; i++) {
     int n = 2 * nmax[i];
@@ -121,8 +147,10 @@ void Basis::build(const Cell& def, const Cell& cell, double ecut) {
     np[i] = n;
   }
   std::stable_sort(rods.begin(), rods.end(), [](const Rod& x, const Rod& y) { return x.size < y.size; });
+  bool rod0 = false;
   for (size_t i = 0; i < rods.size(); i++)
-    if (rods[i].h == 0 && rods[i].k == 0) { std::swap(rods[i], rods[0]); break; }
+    if (rods[i].h == 0 && rods[i].k == 0) { std::swap(rods[i], rods[0]); rod0 = true; break; }
+  if (!rod0) throw std::invalid_argument("basis without the rod (0,0): k outside the zone?");
   nrods = (int)rods.size();
   size = 0;
   rod_h.clear(); rod_k.clear(); rod_lmin.clear(); rod_size.clear(); idx.clear();
@@ -135,19 +163,27 @@ void Basis::build(const Cell& def, const Cell& cell, double ecut) {
   }
   // update_g with the current cell (src/Basis.cpp:646-690)
   g.resize(size); g2.resize(size); gi.resize(size); g2i.resize(size); gx.resize(3 * size);
+  kpg.resize(size); kpg2.resize(size); kpgi.resize(size); kpgx.resize(3 * size);
   for (int i = 0; i < size; i++) {
-    double t[3];
-    for (int j = 0; j < 3; j++)
+    double t[3], u[3];
+    for (int j = 0; j < 3; j++) {
       t[j] = (idx[3 * i] * cell.b[0][j] + idx[3 * i + 1] * cell.b[1][j]) + idx[3 * i + 2] * cell.b[2][j];
-    for (int j = 0; j < 3; j++) gx[j * size + i] = t[j];
+      u[j] = ((kpoint[0] + idx[3 * i]) * cell.b[0][j] + (kpoint[1] + idx[3 * i + 1]) * cell.b[1][j]) +
+             (kpoint[2] + idx[3 * i + 2]) * cell.b[2][j];
+    }
+    for (int j = 0; j < 3; j++) { gx[j * size + i] = t[j]; kpgx[j * size + i] = u[j]; }
     g2[i] = dot3(t, t);
     g[i] = std::sqrt(g2[i]);
     gi[i] = g[i] > 0.0 ? 1.0 / g[i] : 0.0;
     g2i[i] = gi[i] * gi[i];
+    kpg2[i] = dot3(u, u);
+    kpg[i] = std::sqrt(kpg2[i]);
+    kpgi[i] = kpg[i] > 0.0 ? 1.0 / kpg[i] : 0.0;
   }
 }
 
 bool Maps::build(const Basis& b, int np0, int np1, int np2) {
+  if (!b.real) return false;   // the band grids run on real bases only
   if (!(b.idxmax[0] < np0 / 2 && b.idxmin[0] >= -np0 / 2 && b.idxmax[1] < np1 / 2 &&
         b.idxmin[1] >= -np1 / 2 && b.idxmax[2] < np2 / 2 && b.idxmin[2] >= -np2 / 2))
     return false;

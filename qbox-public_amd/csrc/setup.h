@@ -18,14 +18,19 @@ struct Cell {
 
 bool factorizable(int n);                       // src/Basis.cpp:105-133
 
-// Basis at k = 0 on one task (src/Basis.cpp:255-690)
+// Basis on one task (src/Basis.cpp:255-690): the real (half-sphere) basis
+// at k = 0, or the complex (full-sphere) basis |k + G| < sqrt(2 ecut) of a
+// k-point k != 0 (kpoint in units of the reciprocal vectors, :237, 446-481)
 struct Basis {
   int size = 0, nrods = 0;
   int idxmin[3], idxmax[3], np[3];
+  bool real = true;
+  double kpoint[3] = {0.0, 0.0, 0.0};
   std::vector<int> rod_h, rod_k, rod_lmin, rod_size;
   std::vector<int32_t> idx;                     // 3*size
   std::vector<double> g, g2, gi, g2i, gx;       // gx[j*size + i]
-  void build(const Cell& defcell, const Cell& cell, double ecut);
+  std::vector<double> kpg, kpg2, kpgi, kpgx;    // k + G (equal to g at k = 0)
+  void build(const Cell& defcell, const Cell& cell, double ecut, const double* kpoint = nullptr);
 };
 
 // BasisMapping maps for one task, real basis (src/BasisMapping.cpp:38-228)

@@ -64,6 +64,8 @@ SIGNATURES = {
     "qbg_gram_noalloc_bytes": [],
     "qbg_ef_create": [_vp, _dp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
                       C.POINTER(_vp)],
+    "qbg_ef_create_k": [_vp, _dp, _dp, C.c_double, _dp, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
+                        C.POINTER(_vp)],
     "qbg_ef_destroy": [_vp],
     "qbg_ef_sizes": [_vp, _ip],
     "qbg_ef_basis": [_vp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip],
@@ -456,12 +458,15 @@ def gram_noalloc(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
 
 
 class EnergyFunctional:
-    """qbg_ef: one Gamma-point system resident on the device."""
+    """qbg_ef: one system (Gamma, or one k-point off Gamma) resident on the
+    device."""
 
-    def __init__(self, ctx: Context, sysdef, refcell=None):
+    def __init__(self, ctx: Context, sysdef, refcell=None, kpoint=None):
         """refcell (3x3, rows a0 a1 a2): Wavefunction's reference cell, which
         fixes the basis membership and the grids (src/Basis.cpp:314-325);
-        sysdef.refcell when not given."""
+        sysdef.refcell when not given.  kpoint (3, units of the reciprocal
+        vectors; sysdef.kpoint when not given): k != 0 makes the wavefunction
+        basis the complex full sphere (qbg_ef_create_k)."""
         self.ctx = ctx
         self.sysdef = sysdef
         if refcell is None:
@@ -480,10 +485,25 @@ class EnergyFunctional:
             ref = np.ascontiguousarray(refcell, dtype=np.float64).ravel()
         self._keep += [cell, na, tau, occ, ref]
         h = _vp()
-        ctx.check(ctx.lib.qbg_ef_create(ctx.h, _pd(cell), _pd(ref) if ref is not None else None,
-                                        sysdef.ecut, len(sysdef.species),
-                                        C.cast(descs, _vp), na.ctypes.data_as(_ip), _pd(tau),
-                                        sysdef.nst, _pd(occ), C.byref(h)))
+        if kpoint is None:
+            kpoint = getattr(sysdef, "kpoint", None)
+        self.kpoint = None
+        if kpoint is not None:
+            self.kpoint = np.ascontiguousarray(kpoint, dtype=np.float64).ravel()
+        # the complex (full-sphere) basis of a k-point off Gamma
+        self.complex = self.kpoint is not None and bool(np.any(self.kpoint != 0.0))
+        if self.kpoint is not None:
+            self._keep.append(self.kpoint)
+            ctx.check(ctx.lib.qbg_ef_create_k(ctx.h, _pd(cell),
+                                              _pd(ref) if ref is not None else None,
+                                              sysdef.ecut, _pd(self.kpoint), len(sysdef.species),
+                                              C.cast(descs, _vp), na.ctypes.data_as(_ip),
+                                              _pd(tau), sysdef.nst, _pd(occ), C.byref(h)))
+        else:
+            ctx.check(ctx.lib.qbg_ef_create(ctx.h, _pd(cell), _pd(ref) if ref is not None else None,
+                                            sysdef.ecut, len(sysdef.species),
+                                            C.cast(descs, _vp), na.ctypes.data_as(_ip), _pd(tau),
+                                            sysdef.nst, _pd(occ), C.byref(h)))
         self.h = h
         s = np.zeros(10, dtype=np.int32)
         ctx.check(ctx.lib.qbg_ef_sizes(h, s.ctypes.data_as(_ip)))
@@ -506,19 +526,24 @@ class EnergyFunctional:
             pass
 
     def basis(self, which=0):
-        n = self.ngw if which == 0 else self.ngv
-        nvec = self.nvec if which == 0 else self.nvec_v
+        """which 0: wavefunction basis, 1: density basis, 2: the wavefunction
+        basis's k + G (g2 = |k+G|^2, gx = k+G).  The band-grid maps (ip, im,
+        zvec_to_val) of a k != 0 functional are None."""
+        n = self.ngv if which == 1 else self.ngw
+        nvec = self.nvec_v if which == 1 else self.nvec
+        maps = which == 1 or not self.complex
         idx = np.zeros(3 * n, dtype=np.int32)
         g2 = np.zeros(n)
         gx = np.zeros(3 * n)
         ip = np.zeros(n, dtype=np.int32)
         im = np.zeros(n, dtype=np.int32)
         z2v = np.zeros(nvec, dtype=np.int32)
-        self.ctx.check(self.ctx.lib.qbg_ef_basis(self.h, which, idx.ctypes.data_as(_ip), _pd(g2),
-                                                 _pd(gx), ip.ctypes.data_as(_ip),
-                                                 im.ctypes.data_as(_ip), z2v.ctypes.data_as(_ip)))
-        return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip, im=im,
-                    zvec_to_val=z2v)
+        self.ctx.check(self.ctx.lib.qbg_ef_basis(
+            self.h, which, idx.ctypes.data_as(_ip), _pd(g2), _pd(gx),
+            ip.ctypes.data_as(_ip) if maps else None, im.ctypes.data_as(_ip) if maps else None,
+            z2v.ctypes.data_as(_ip) if maps else None))
+        return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip if maps else None,
+                    im=im if maps else None, zvec_to_val=z2v if maps else None)
 
     def set_positions(self, tau):
         tau = np.ascontiguousarray(tau, dtype=np.float64).ravel()
