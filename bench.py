@@ -327,6 +327,53 @@ def bench_scf(args):
         "eager_kernel_ms_per_iteration": {k: round(v[0] / 2, 4) for k, v in kt.items()}}))
 
 
+def bench_kpoint(args):
+    """One evaluation (density, update_vhxc, energy with H*psi and forces) of
+    the workload at one k-point off Gamma (qbg_ef_create_k, the complex
+    full-sphere basis), FP64 and BF16X2 on the same orthonormal random
+    complex bands; the BF16X2 result is checked against the FP64 one."""
+    import dataclasses
+    import qbg
+    from qbg import configs as cf
+    kp = np.array(args.kpoint, dtype=float)
+    sysdef = dataclasses.replace(cf.config(args.config), kpoint=kp)
+    ctx = qbg.Context(0)
+    ef = qbg.EnergyFunctional(ctx, sysdef)
+    kpg2 = ef.basis(2)["g2"]
+    c = cf.orthonormal_k(cf.random_coefficients_k(cf.BASE_SEED + 70 + args.config, kpg2,
+                                                  sysdef.nst))
+    dc = ctx.to_device(c)
+    hp = ctx.array(c.shape, np.complex128)
+    del c
+    nst, nat = sysdef.nst, sysdef.nat
+    print(f"[bench kpoint] {sysdef.name} k {tuple(kp)}: ngw {ef.ngw} nproj {ef.nproj}",
+          file=sys.stderr, flush=True)
+    r64, wall64, _, kt64 = run(ef, dc, hp, "fp64", args.steps, args.warmup, ctx)
+    h64 = hp.download()
+    rbf, wallbf, _, ktbf = run(ef, dc, hp, "bf16x2", args.steps, args.warmup, ctx)
+    hbf = hp.download()
+    err = max(abs(rbf[t] - r64[t]) / nat for t in TERMS)
+    ferr = float(np.abs(rbf["fion"] - r64["fion"]).max())
+    herr = float(np.abs(hbf - h64).max() / np.abs(h64).max())
+    t64, tbf = wall64 / args.steps, wallbf / args.steps
+    print(json.dumps({
+        "metric": "H*psi band-applies/s at one k-point off Gamma on 1 GPU (complex basis; "
+                  "density + update_vhxc + energy with H*psi and forces)",
+        "value": round(nst / tbf, 2), "unit": "band-applies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * tbf, 3),
+        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": sysdef.name, "kpoint": [float(x) for x in kp], "ngw": ef.ngw,
+                   "nproj": ef.nproj, "precision": "bf16x2"},
+        "fp64_value": round(nst / t64, 2), "fp64_ms_per_step": round(1e3 * t64, 3),
+        "bf16x2_vs_fp64": {"max_dE_per_atom": float("%.3e" % err),
+                           "force_max_abs_err": float("%.3e" % ferr),
+                           "hpsi_rel_err": float("%.3e" % herr),
+                           "meets_accuracy": bool(err <= DE_TOL and ferr <= F_TOL)},
+        "kernel_ms_per_step_fp64": {k: round(v[0] / args.steps, 3) for k, v in kt64.items()},
+        "kernel_ms_per_step_bf16x2": {k: round(v[0] / args.steps, 3) for k, v in ktbf.items()},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +387,9 @@ def main():
                     help="only the uneven batch: twelve systems, per-system loop vs one graph")
     ap.add_argument("--scf", action="store_true",
                     help="only time whole SCF iterations (PSDA stepper, Anderson mixing)")
+    ap.add_argument("--kpoint", type=float, nargs=3, default=None, metavar=("K0", "K1", "K2"),
+                    help="only time the evaluation at this k-point (units of b) on the "
+                         "complex basis, FP64 and BF16X2")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -359,6 +409,8 @@ def main():
         return bench_c4(args)
     if args.scf:
         return bench_scf(args)
+    if args.kpoint is not None:
+        return bench_kpoint(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (band sharding + RCCL reduction of rho), so "

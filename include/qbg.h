@@ -250,10 +250,10 @@ int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, dou
  * order (qbg_ef_basis which 0 and 2), and the coefficients are complex on
  * it; update_density, update_vhxc and the evaluation (energies, H*psi,
  * forces) follow src/EnergyFunctional.cpp:397-1044 and
- * src/NonLocalPotential.cpp:1429-1775 for the complex basis, in FP64 or
- * FP32.  Not built at k != 0 (QBG_E_INVAL): the BF16 precisions, the
- * stress, nspin 2, cell_moved, the NonLocalPotential and wavefunction-grid
- * entry points and the SCF steppers. */
+ * src/NonLocalPotential.cpp:1429-1775 for the complex basis, in every
+ * precision.  Not built at k != 0 (QBG_E_INVAL): the stress, nspin 2,
+ * cell_moved, the NonLocalPotential and wavefunction-grid entry points and
+ * the SCF steppers. */
 int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
                     const double* kpoint3, int nsp, const qbg_species_desc* species,
                     const int* na, const double* tau, int nst, const double* occ, qbg_ef** out);

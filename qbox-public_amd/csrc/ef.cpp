@@ -1184,13 +1184,16 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_begin(QBG_K_NL);
   launch_anl(ngw, e->nat, Kp, e->d_gxw, e->d_tau, e->d_a_m0, e->d_a_npr, e->d_a_stride,
              e->d_m_twnl, e->d_m_l, e->d_twnl, e->b_anl, 1, s);
+  if (e->cplx)   // k != 0: the i anl columns (see nl_mfma)
+    launch_anl(ngw, e->nat, Kp, e->d_gxw, e->d_tau, e->d_a_m0h, e->d_a_npr, e->d_a_stride,
+               e->d_m_twnl, e->d_m_l, e->d_twnl, e->b_anl, 1, s);
   if (!c32_ready) convert_f32(c, 2L * ldc, K, nst, e->b_c32, Kp, s, e->nst);
   ctx->prof_end();
   if (parts == 2 && (rc = bf16_tune(e, c, ldc, nst))) return rc;
   ctx->prof_begin(QBG_K_GEMM);
-  rc = gemm_bf_tn(e->b_anl, Kp, e->b_c32, Kp, Mp, Np, Kp, M, nst, c, 2L * ldc, 2.0, -1.0,
-                  e->d_fnl, M, e->b_part, e->b_part_floats, parts, s,
-                  parts == 2 ? &e->bf_tn : nullptr);
+  rc = gemm_bf_tn(e->b_anl, Kp, e->b_c32, Kp, Mp, Np, Kp, M, nst, c, 2L * ldc,
+                  e->cplx ? 1.0 : 2.0, e->cplx ? 0.0 : -1.0, e->d_fnl, M, e->b_part,
+                  e->b_part_floats, parts, s, parts == 2 ? &e->bf_tn : nullptr);
   ctx->prof_end();
   if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
   ctx->prof_begin(QBG_K_NL);
@@ -1198,8 +1201,10 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
                   e->d_scal + S_ENL, s);
   const bool needy = compute_forces || compute_stress;
-  fout_f32(e->d_fout, occ, M, nst, e->b_fout, Mp, needy ? e->b_w32 : nullptr, Np, s,
-           e->nst);
+  // W = occ fout for the force product (half the occupation on the full
+  // sphere of k != 0, see nl_mfma)
+  fout_f32(e->d_fout, e->cplx ? e->d_occh + b0 : occ, M, nst, e->b_fout, Mp,
+           needy ? e->b_w32 : nullptr, Np, s, e->nst);
   ctx->prof_end();
   if (compute_hpsi) {
     ctx->prof_begin(QBG_K_GEMM);
@@ -1219,8 +1224,10 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (rc == 0) {
       ctx->prof_begin(QBG_K_NL);
       fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
-      launch_nl_forces_tiles(e->b_fpart, M, (int)(Kp / 256), e->b_fred, e->d_a_m0, e->d_a_npr,
-                             e->d_a_stride, e->nat, e->d_fion, s);
+      for (int half = 0; half < (e->cplx ? 2 : 1); half++)
+        launch_nl_forces_tiles(e->b_fpart, M, (int)(Kp / 256), e->b_fred,
+                               half ? e->d_a_m0h : e->d_a_m0, e->d_a_npr, e->d_a_stride, e->nat,
+                               e->d_fion, s);
       ctx->prof_end();
       return QBG_OK;
     }
@@ -1233,8 +1240,10 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces) {
       fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
-      launch_nl_forces_y(e->b_anl, (const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_a_m0,
-                         e->d_a_npr, e->d_a_stride, e->nat, e->d_partial, 64, e->d_fion, s);
+      for (int half = 0; half < (e->cplx ? 2 : 1); half++)
+        launch_nl_forces_y(e->b_anl, (const float*)e->b_y32, Kp, ngw, e->d_gxw,
+                           half ? e->d_a_m0h : e->d_a_m0, e->d_a_npr, e->d_a_stride, e->nat,
+                           e->d_partial, 64, e->d_fion, s);
     }
     if (compute_stress)
       launch_nl_stress_y((const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_tau, e->nsp,
@@ -1261,9 +1270,6 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   if (compute_stress && !e->vhxc_stress)
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
-  if (e->cplx && (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2))
-    return ctx->fail(QBG_E_INVAL, "energy at k != 0: FP64 or FP32 (the BF16 products are not built "
-                                  "for the complex basis)");
   if (e->cplx && compute_stress)
     return ctx->fail(QBG_E_INVAL, "energy at k != 0: the stress is not built for the complex basis");
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;   // band FFTs
@@ -1280,7 +1286,7 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   // in the split modes the forward z pass completes H*psi: it stores
   // 0.5 kpg2 c + the nonlocal part (FP32, from the NN product) + the local
   // part, so neither the kinetic pass nor the NN product touches H*psi
-  const bool fin = split && compute_hpsi && hpsi != c && zfwd_fin_ok(e->gw);
+  const bool fin = split && compute_hpsi && hpsi != c && !e->cplx && zfwd_fin_ok(e->gw);
   ctx->prof_begin(QBG_K_OTHER);
   // (the half sphere's G and -G times 1/2 is occ |c|^2 |G|^2; the full sphere
   // of k != 0 takes occ/2 |c|^2 |k+G|^2)
