@@ -168,6 +168,12 @@ int qbg_rs_mul_add(qbg_grid* g, const double* v, const double* c, int ldc,
  * leading dimension ldc).  QBG_E_NOT_SPD if S is not positive definite, in
  * which case c is left as cleaned up but otherwise unchanged. */
 int qbg_gram(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst, int prec);
+/* SlaterDet::gram() for the complex basis of a k-point (src/SlaterDet.cpp:
+ * 523-532): S = C^H C (herk), S = L L^H (lower Cholesky), C <- C L^-H, in
+ * place on the device array c (ngw x nst complex, leading dimension ldc),
+ * FP64, nst <= 2048.  QBG_E_NOT_SPD if S is not positive definite, in which
+ * case c is unchanged.  Synchronizes the context stream. */
+int qbg_gram_complex(qbg_ctx* ctx, double* c, int ldc, int ngw, int nst);
 
 /* qbg_gram without any device allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph
@@ -250,10 +256,10 @@ int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, dou
  * order (qbg_ef_basis which 0 and 2), and the coefficients are complex on
  * it; update_density, update_vhxc and the evaluation (energies, H*psi,
  * forces) follow src/EnergyFunctional.cpp:397-1044 and
- * src/NonLocalPotential.cpp:1429-1775 for the complex basis, in every
- * precision.  Not built at k != 0 (QBG_E_INVAL): the stress, nspin 2,
- * cell_moved, the NonLocalPotential and wavefunction-grid entry points and
- * the SCF steppers. */
+ * src/NonLocalPotential.cpp:1429-2066 for the complex basis (the stress
+ * included), in every precision, with nspin 1 or 2.  Not built at k != 0
+ * (QBG_E_INVAL): cell_moved, the NonLocalPotential and wavefunction-grid
+ * entry points and the SCF steppers. */
 int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
                     const double* kpoint3, int nsp, const qbg_species_desc* species,
                     const int* na, const double* tau, int nst, const double* occ, qbg_ef** out);

@@ -196,6 +196,7 @@ void qbo_compute_density_occ(void* h, const double* c, int nst, const double* oc
   compute_density(s->ft, (const cplx*)c, s->wb.size, nst, occ, 1.0, s->cell.volume, rho);
 }
 int qbo_gram(double* c, int ngw, int nst) { return gram((cplx*)c, ngw, nst); }
+int qbo_gram_complex(double* c, int ngw, int nst) { return gram_complex((cplx*)c, ngw, nst); }
 
 void qbo_update_density(void* h, const double* c) {
   ((System*)h)->update_density((const cplx*)c);

@@ -545,6 +545,47 @@ int gram(cplx* c, int ngw, int nst) {
   return 0;
 }
 
+// src/SlaterDet.cpp:523-532 (complex basis): S = C^H C (herk), S = L L^H
+// (potrf, lower), C <- C L^-H (trsm 'r','l','c'); returns 0 or the failing
+// pivot + 1.  c is band-major (c[ig + ngw*n] = C[ig][n]).
+int gram_complex(cplx* c, int ngw, int nst) {
+  std::vector<cplx> S((size_t)nst * nst, cplx(0.0));   // S[i*nst + j], j <= i
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < nst; i++)
+    for (int j = 0; j <= i; j++) {
+      const cplx* a = c + (size_t)i * ngw;
+      const cplx* b = c + (size_t)j * ngw;
+      cplx sacc(0.0);
+      for (int g = 0; g < ngw; g++) sacc += std::conj(a[g]) * b[g];
+      S[(size_t)i * nst + j] = sacc;               // (C^H C)_ij
+    }
+  // Cholesky, lower: L_jj = sqrt(S_jj - sum_k |L_jk|^2),
+  // L_ij = (S_ij - sum_k L_ik conj(L_jk)) / L_jj
+  for (int j = 0; j < nst; j++) {
+    double d = S[(size_t)j * nst + j].real();
+    for (int k = 0; k < j; k++) d -= std::norm(S[(size_t)j * nst + k]);
+    if (!(d > 0.0)) return j + 1;
+    d = std::sqrt(d);
+    S[(size_t)j * nst + j] = d;
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i < nst; i++) {
+      cplx sacc = S[(size_t)i * nst + j];
+      for (int k = 0; k < j; k++) sacc -= S[(size_t)i * nst + k] * std::conj(S[(size_t)j * nst + k]);
+      S[(size_t)i * nst + j] = sacc / d;
+    }
+  }
+  // X L^H = C row by row: x_j = (c_j - sum_{k<j} x_k conj(L_jk)) / L_jj
+#pragma omp parallel for schedule(static)
+  for (int g = 0; g < ngw; g++) {
+    for (int j = 0; j < nst; j++) {
+      cplx sacc = c[g + (size_t)j * ngw];
+      for (int k = 0; k < j; k++) sacc -= c[g + (size_t)k * ngw] * std::conj(S[(size_t)j * nst + k]);
+      c[g + (size_t)j * ngw] = sacc / S[(size_t)j * nst + j].real();
+    }
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------- ChargeDensity
 // src/ChargeDensity.cpp:136-198
 // (nspin 2: one density per spin from its bands; rhog is the transform of the
@@ -733,9 +774,11 @@ static inline void phase_l(int l, double t, double c, double s, double& re, doub
 // The complex basis (k != 0): anl and fnl complex, fnl = anl^H c (zgemm,
 // :1429-1445), enl = sum occ wt/omega Re(conj(fnl) D fnl) (:1572-1595),
 // H psi += anl fnl' (:1616-1630), forces from dfnl = (-i (k+G)_j anl)^H c
-// (:1722-1775); no stress.
+// (:1722-1775), stress from dfnl = (dtwnl_ij phase)^H c (:1990-2037, with
+// dtwnl at k + G).
 double System::nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
-                                 bool compute_forces, std::vector<double>& fion_enl) {
+                                 bool compute_forces, std::vector<double>& fion_enl,
+                                 bool compute_stress, double* sigma_enl) {
   const int ngw = wb.size;
   const double omega_inv = 1.0 / cell.volume;
   const int nsp = (int)sp.size();
@@ -744,26 +787,30 @@ double System::nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
   for (int is = 0; is < nsp; is++) nat += (int)tau[is].size() / 3;
   fion_enl.assign(3 * nat, 0.0);
   const double* kx[3] = {&wb.kpgx[0], &wb.kpgx[ngw], &wb.kpgx[2 * ngw]};
+  double tsum[6] = {0, 0, 0, 0, 0, 0};
   int atom_off = 0;
   for (int is = 0; is < nsp; is++) {
     const Species& s = sp[is];
     const int na = (int)tau[is].size() / 3;
     const int npr = s.nlm;
     const int M = npr * na;
-    // anl[ig + ngw*(ia + ipr*na)] = t (-i)^l e^{-i(k+G).tau}, or -i (k+G)_dir anl
+    // anl[ig + ngw*(ia + ipr*na)] = t (-i)^l e^{-i(k+G).tau}; dir 0..2:
+    // -i (k+G)_dir anl; dir 3 + ij: dtwnl_ij (-i)^l e^{-i(k+G).tau}
     std::vector<cplx> anl((size_t)ngw * M);
     auto build_anl = [&](int dir) {
 #pragma omp parallel for schedule(static) collapse(2)
       for (int ipr = 0; ipr < npr; ipr++)
         for (int ia = 0; ia < na; ia++) {
-          const double* t = &twnl[is][(size_t)ngw * ipr];
+          const double* t = dir < 3 ? &twnl[is][(size_t)ngw * ipr]
+                                    : &dtwnl[is][(size_t)ngw * ((dir - 3) + 6 * ipr)];
           const double* tu = &tau[is][3 * ia];
           const int l = lproj[is][ipr];
           cplx* a = &anl[(size_t)ngw * (ia + ipr * na)];
           for (int ig = 0; ig < ngw; ig++) {
             const double arg = -(kx[0][ig] * tu[0] + kx[1][ig] * tu[1] + kx[2][ig] * tu[2]);
-            const int ll = dir < 0 ? l : l + 1;
-            const double tt = dir < 0 ? t[ig] : kx[dir][ig] * t[ig];
+            const bool force = dir >= 0 && dir < 3;
+            const int ll = force ? l + 1 : l;
+            const double tt = force ? kx[dir][ig] * t[ig] : t[ig];
             double re, im;
             phase_l(ll, tt, std::cos(arg), std::sin(arg), re, im);
             a[ig] = cplx(re, im);
@@ -830,7 +877,25 @@ double System::nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
           }
       }
     }
+    if (compute_stress) {
+      std::vector<cplx> dfnl;
+      for (int ij = 0; ij < 6; ij++) {
+        build_anl(3 + ij);
+        project(dfnl);
+        for (int n = 0; n < nst; n++) {
+          const double facn = 2.0 * occ[n];
+          for (int i = 0; i < M; i++) {
+            const size_t q = i + (size_t)n * M;
+            tsum[ij] += facn * (fnl[q].real() * dfnl[q].real() + fnl[q].imag() * dfnl[q].imag());
+          }
+        }
+      }
+    }
     atom_off += na;
+  }
+  if (compute_stress) {   // :2057-2066
+    for (int i = 0; i < 3; i++) sigma_enl[i] = (enl + tsum[i]) * omega_inv;
+    for (int i = 3; i < 6; i++) sigma_enl[i] = tsum[i] * omega_inv;
   }
   return enl;
 }
@@ -839,9 +904,9 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                          bool compute_forces, std::vector<double>& fion_enl,
                          bool compute_stress, double* sigma_enl) {
   if (!wb.real) {
-    if (compute_stress) throw std::invalid_argument("stress at k != 0 is not restated");
     for (int i = 0; i < 6; i++) sigma_enl[i] = 0.0;
-    return nl_energy_complex(c, compute_hpsi, hpsi, compute_forces, fion_enl);
+    return nl_energy_complex(c, compute_hpsi, hpsi, compute_forces, fion_enl, compute_stress,
+                             sigma_enl);
   }
   const int ngw = wb.size;
   const long K = 2L * ngw;
@@ -984,7 +1049,6 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
   if (compute_hpsi) std::fill(hpsi, hpsi + (size_t)ngw * nst, cplx(0.0));
   // ekin (:441-571), fac = 1 for the real basis (G and -G times 1/2), 1/2
   // for the complex basis
-  if (compute_stress && !wb.real) throw std::invalid_argument("stress at k != 0 is not restated");
   double ts[7] = {0, 0, 0, 0, 0, 0, 0};
   {
     const double kfac = wb.real ? 1.0 : 0.5;
@@ -999,7 +1063,7 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
       const double p = psi2sum[ig];
       ts[0] += p * wb.kpg2[ig];
       if (compute_stress) {
-        const double gx = wb.gx[ig], gy = wb.gx[ngw + ig], gz = wb.gx[2 * ngw + ig];
+        const double gx = wb.kpgx[ig], gy = wb.kpgx[ngw + ig], gz = wb.kpgx[2 * ngw + ig];
         const double f = 2.0 * p;
         ts[1] += f * gx * gx; ts[2] += f * gy * gy; ts[3] += f * gz * gz;
         ts[4] += f * gx * gy; ts[5] += f * gy * gz; ts[6] += f * gx * gz;

@@ -76,6 +76,7 @@ def lib():
                      "qbo_update_vhxc", "qbo_get", "qbo_set_vr", "qbo_species_eval"]:
             getattr(_lib, name).argtypes = None
         _lib.qbo_gram.argtypes = [_dp, C.c_int, C.c_int]
+        _lib.qbo_gram_complex.argtypes = [_dp, C.c_int, C.c_int]
         _lib.qbo_species_core.restype = C.c_double
         _lib.qbo_species_core.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         _lib.qbo_sd_dot.restype = C.c_double
@@ -483,6 +484,17 @@ def gram(c):
     c = np.array(c, dtype=np.complex128, order="C")
     nst, ngw = c.shape
     rc = lib().qbo_gram(_ptr(c), ngw, nst)
+    if rc != 0:
+        raise RuntimeError(f"overlap not positive definite at pivot {rc}")
+    return c
+
+
+def gram_complex(c):
+    """SlaterDet::gram of the complex (k != 0) basis on (nst, ngw)
+    coefficients: S = C^H C = L L^H, C <- C L^-H (a copy)."""
+    c = np.array(c, dtype=np.complex128, order="C")
+    nst, ngw = c.shape
+    rc = lib().qbo_gram_complex(_ptr(c), ngw, nst)
     if rc != 0:
         raise RuntimeError(f"overlap not positive definite at pivot {rc}")
     return c

@@ -163,7 +163,7 @@ struct System {
   int nst = 0;
   std::vector<double> occ;
   // one k-point (in units of b, weight 1): k != 0 (or force_complex) makes
-  // the wavefunction basis complex; stress is not restated there
+  // the wavefunction basis complex
   double kpoint[3] = {0.0, 0.0, 0.0};
   bool force_complex = false;
   // nspin 2: bands [0, nst_up) spin up, [nst_up, nst) spin down
@@ -217,7 +217,8 @@ struct System {
                    bool compute_forces, std::vector<double>& fion_enl,
                    bool compute_stress, double* sigma_enl);   // NLP:1258-2069
   double nl_energy_complex(const cplx* c, bool compute_hpsi, cplx* hpsi,
-                           bool compute_forces, std::vector<double>& fion_enl);
+                           bool compute_forces, std::vector<double>& fion_enl,
+                           bool compute_stress, double* sigma_enl);
 };
 
 // SlaterDet pieces (src/SlaterDet.cpp)
@@ -226,6 +227,7 @@ void compute_density(const FFT3& ft, const cplx* c, int ngw, int nst,
 void rs_mul_add(const FFT3& ft, const double* v, const cplx* c, int ngw,
                 int nst, cplx* hpsi);
 int gram(cplx* c, int ngw, int nst);   // returns 0 or the failing pivot+1
+int gram_complex(cplx* c, int ngw, int nst);   // complex basis (SlaterDet.cpp:523-532)
 void cleanup(cplx* c, int ngw, int nst);
 
 // SCF iteration pieces (stepper.cpp)

@@ -830,16 +830,23 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
     spin_bands(e, sp, b0, nb, p0);
     pairs = p0 + (nb + 1) / 2;
   }
-  const bool res = !e->cplx && keep && grid_ensure_xres(e->gw, pairs);
+  // (at k != 0 every band is one pair of the band grid)
+  const bool res = keep && grid_ensure_xres(e->gw, e->cplx ? e->nst : pairs);
   int rc = QBG_OK;
   if (e->cplx) {
     // each complex band as the pair of its real and imaginary parts on the
-    // real band basis, both with the band's occupation
+    // real band basis, both with the band's occupation; one density per spin
+    const int nu = e->ub.size;
     ctx->prof_begin(QBG_K_OTHER);
-    launch_fold(c, ldc, e->nst, e->ub.size, e->d_fold, e->d_cpair, e->ub.size, s);
+    launch_fold(c, ldc, e->nst, nu, e->d_fold, e->d_cpair, nu, s);
     ctx->prof_end();
-    rc = density_batches(e->gw, e->d_cpair, e->ub.size, 2 * e->nst, e->d_dfac, e->d_rhor, fprec,
-                         false, 0);
+    for (int sp = 0; sp < e->nspin && !rc; sp++) {
+      int b0, nb, p0;
+      spin_bands(e, sp, b0, nb, p0);
+      if (nb > 0)
+        rc = density_batches(e->gw, e->d_cpair + 4L * nu * b0, nu, 2 * nb, e->d_dfac + 2 * b0,
+                             sp == 0 ? e->d_rhor : e->d_rhor2, fprec, res, b0);
+    }
   }
   // one density per spin (src/ChargeDensity.cpp:136-160)
   for (int sp = 0; sp < e->nspin && !rc && !e->cplx; sp++) {
@@ -1037,11 +1044,13 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
       if (f32)
         launch_nl_stress_y((const float*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
+                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                           e->cplx ? 2 : 1, e->Mh);
       else
         launch_nl_stress_y((const double*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
+                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                           e->cplx ? 2 : 1, e->Mh);
     }
     ctx->prof_end();
   }
@@ -1248,7 +1257,8 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (compute_stress)
       launch_nl_stress_y((const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_tau, e->nsp,
                          e->d_sp_first, e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0,
-                         e->d_a_stride, e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s);
+                         e->d_a_stride, e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                         e->cplx ? 2 : 1, e->Mh);
     ctx->prof_end();
   }
   return QBG_OK;
@@ -1270,8 +1280,6 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   if (compute_stress && !e->vhxc_stress)
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
-  if (e->cplx && compute_stress)
-    return ctx->fail(QBG_E_INVAL, "energy at k != 0: the stress is not built for the complex basis");
   const int fprec = prec == QBG_PREC_FP64 ? QBG_PREC_FP64 : QBG_PREC_FP32;   // band FFTs
   QBG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
@@ -1316,13 +1324,26 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   // pairs, unfolded into hpsi
   if (compute_hpsi && e->cplx) {
     const int nu = e->ub.size;
+    // the x-pass output of the folded bands that update_density kept, or a
+    // fold and full transforms here
+    const bool kept = reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec;
     ctx->prof_begin(QBG_K_OTHER);
-    launch_fold(c, ldc, nst, nu, e->d_fold, e->d_cpair, nu, s);
+    if (!kept) launch_fold(c, ldc, nst, nu, e->d_fold, e->d_cpair, nu, s);
     fill_zero(e->d_hpair, (size_t)4 * nu * nst, s);
     ctx->prof_end();
-    const int rc = rs_mul_add_bands(e->gw, e->d_vr, e->d_cpair, nu, 2 * nst, e->d_hpair, nu,
-                                    fprec, nullptr);
-    if (rc) return rc;
+    // each band with its spin's potential (src/EnergyFunctional.cpp:831)
+    for (int sp = 0; sp < e->nspin; sp++) {
+      int b0, nb, p0;
+      spin_bands(e, sp, b0, nb, p0);
+      if (nb == 0) continue;
+      const double* v = sp == 0 ? e->d_vr : e->d_vr2;
+      double* hb = e->d_hpair + 4L * nu * b0;
+      const int rc = kept ? hpsi_resident(e->gw, v, 2 * nb, hb, nu, fprec, b0, nullptr)
+                          : rs_mul_add_bands(e->gw, v, e->d_cpair + 4L * nu * b0, nu, 2 * nb, hb,
+                                             nu, fprec, nullptr);
+      if (rc) return rc;
+    }
+    if (kept) e->xres_c = nullptr;
     ctx->prof_begin(QBG_K_OTHER);
     launch_unfold_add(e->d_hpair, nu, nst, ngw, e->d_unfold, hpsi, ldh, s);
     ctx->prof_end();
@@ -1655,7 +1676,6 @@ int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
     if (nspin == 1) nst_up = 0;
-    if (nspin == 2 && e->cplx) return c->fail(QBG_E_INVAL, "set_nspin: nspin 2 at k != 0 is not built");
     if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
       return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
     QBG_HIP(c, hipSetDevice(c->device));

@@ -75,7 +75,7 @@ void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const d
                         int nsp, const int32_t* sp_first, const int32_t* sp_npr,
                         const int64_t* dt_off, const double* dtwnl, const int32_t* a_m0,
                         const int32_t* a_stride, const int32_t* m_l, double* partial,
-                        int nchunk, double* out6, hipStream_t s);
+                        int nchunk, double* out6, hipStream_t s, int halves = 1, long mhalf = 0);
 void launch_hart_ps_stress(const double* rhogt, const double* sfac, const double* rhops,
                            const double* g2i, const double* gx, int ngv, int nsp,
                            const double* fac2, const double* vxcg, const double* drhoc,

@@ -821,7 +821,7 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
     const int32_t* __restrict__ sp_npr, const int64_t* __restrict__ dt_off,
     const double* __restrict__ dtwnl, const int32_t* __restrict__ a_m0,
     const int32_t* __restrict__ a_stride, const int32_t* __restrict__ m_l, int chunk,
-    double* __restrict__ partial) {
+    int halves, long mhalf, double* __restrict__ partial) {
   constexpr int ZB = 16;
   const int s = blockIdx.y;
   const int a0 = sp_first[s], a1 = sp_first[s + 1], npr = sp_npr[s];
@@ -837,7 +837,11 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
       for (int a = a0; a < a1; a++) {
         double sn, cs;
         sincos(-(x * tau[3 * a] + y * tau[3 * a + 1] + z * tau[3 * a + 2]), &sn, &cs);
-        const long mb = a_m0[a], st = a_stride[a];
+        const long st = a_stride[a];
+        // halves = 2 (k != 0): the i anl columns mhalf further, with their
+        // own phase index in m_l
+        for (int h = 0; h < halves; h++) {
+        const long mb = a_m0[a] + h * mhalf;
 #pragma unroll
         for (int k = 0; k < ZB; k++) {
           const int ipr = p0 + k;
@@ -852,6 +856,7 @@ __global__ __launch_bounds__(256) void k_nl_stress_y(
                            : (l == 1 ? sn * yr - cs * yi
                                      : (l == 2 ? -(cs * yr + sn * yi) : cs * yi - sn * yr));
           }
+        }
         }
       }
 #pragma unroll
@@ -880,20 +885,23 @@ void launch_nl_stress_y(const T* Y, long lda, int ngw, const double* gx, const d
                         int nsp, const int32_t* sp_first, const int32_t* sp_npr,
                         const int64_t* dt_off, const double* dtwnl, const int32_t* a_m0,
                         const int32_t* a_stride, const int32_t* m_l, double* partial,
-                        int nchunk, double* out6, hipStream_t s) {
+                        int nchunk, double* out6, hipStream_t s, int halves, long mhalf) {
   const int chunk = (ngw + nchunk - 1) / nchunk;
   hipLaunchKernelGGL(k_nl_stress_y<T>, dim3(nchunk, nsp), dim3(256), 0, s, Y, lda, ngw, gx, tau,
-                     sp_first, sp_npr, dt_off, dtwnl, a_m0, a_stride, m_l, chunk, partial);
+                     sp_first, sp_npr, dt_off, dtwnl, a_m0, a_stride, m_l, chunk, halves, mhalf,
+                     partial);
   hipLaunchKernelGGL(k_nl_stress_final, dim3(1), dim3(64), 0, s, partial, nchunk * nsp, out6);
 }
 template void launch_nl_stress_y<double>(const double*, long, int, const double*, const double*,
                                          int, const int32_t*, const int32_t*, const int64_t*,
                                          const double*, const int32_t*, const int32_t*,
-                                         const int32_t*, double*, int, double*, hipStream_t);
+                                         const int32_t*, double*, int, double*, hipStream_t,
+                                         int, long);
 template void launch_nl_stress_y<float>(const float*, long, int, const double*, const double*,
                                         int, const int32_t*, const int32_t*, const int64_t*,
                                         const double*, const int32_t*, const int32_t*,
-                                        const int32_t*, double*, int, double*, hipStream_t);
+                                        const int32_t*, double*, int, double*, hipStream_t,
+                                        int, long);
 
 // ------------------------------------------- Hartree pseudocharge stress
 // The species part of sigma_ehart (src/EnergyFunctional.cpp:649-676):

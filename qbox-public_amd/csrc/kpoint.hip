@@ -15,7 +15,10 @@
 // Both maps are gathers over precomputed index tables (fold: U -> S pairs,
 // unfold: S -> U), one coalesced pass over the coefficients each.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
 #include "energy.h"
+#include "internal.h"
 
 namespace qbg {
 
@@ -71,4 +74,187 @@ void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_
                      unfold, hpsi, ldh);
 }
 
+// ------------------------------------------------------------------------
+// SlaterDet::gram of the complex basis (src/SlaterDet.cpp:523-532):
+// S = C^H C (herk), S = L L^H (potrf, lower), C <- C L^-H (trsm).  With P
+// the real view of C (2 ngw x nst) and P' that of i C, Re S = P^T P and
+// Im S = -P^T P' (two real products on the FP64 MFMA kernels); the complex
+// Cholesky runs column by column (a column kernel and a trailing update per
+// column), L^-1 one block per column with the column in LDS, and
+// C L^-H = P Re(X) + P' Im(X), X = L^-H, is two real NN products.
+
+constexpr int ZG_NMAX = 2048;   // columns of L^-1 held in LDS (32 KiB)
+
+// out = the real view of i c: (re, im) -> (-im, re)
+__global__ __launch_bounds__(256) void k_times_i(const double* __restrict__ c, long ldc2, long ng,
+                                                 int n, double* __restrict__ out) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= ng * n) return;
+  const int col = (int)(t / ng);
+  const long g = t - (long)col * ng;
+  const double2 v = reinterpret_cast<const double2*>(c + col * ldc2)[g];
+  reinterpret_cast<double2*>(out + col * 2 * ng)[g] = double2{-v.y, v.x};
+}
+
+// S[i + j n] = (Sr, Si) (complex, column-major)
+__global__ __launch_bounds__(256) void k_zjoin(const double* __restrict__ Sr,
+                                               const double* __restrict__ Si, int n,
+                                               double* __restrict__ S) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)n * n) return;
+  reinterpret_cast<double2*>(S)[t] = double2{Sr[t], Si[t]};
+}
+
+// column j of the Cholesky factor: L_jj = sqrt(Re S_jj), L_ij = S_ij / L_jj;
+// a non-positive pivot sets info = j + 1 and every later launch does nothing
+__global__ __launch_bounds__(256) void k_zchol_col(double* __restrict__ S, int n, int j,
+                                                   int* __restrict__ info) {
+  if (*info) return;
+  __shared__ double d;
+  double2* A = reinterpret_cast<double2*>(S);
+  if (threadIdx.x == 0) {
+    const double v = A[j + (long)j * n].x;
+    d = v > 0.0 ? std::sqrt(v) : 0.0;
+    A[j + (long)j * n] = double2{d, 0.0};
+  }
+  __syncthreads();
+  if (!(d > 0.0)) {
+    if (threadIdx.x == 0) *info = j + 1;
+    return;
+  }
+  const double di = 1.0 / d;
+  for (int i = j + 1 + threadIdx.x; i < n; i += 256) {
+    double2 v = A[i + (long)j * n];
+    A[i + (long)j * n] = double2{v.x * di, v.y * di};
+  }
+}
+
+// trailing update after column j: S_ik -= L_ij conj(L_kj), n > i >= k > j
+__global__ __launch_bounds__(256) void k_zchol_upd(double* __restrict__ S, int n, int j,
+                                                   const int* __restrict__ info) {
+  if (*info) return;
+  const int k = j + 1 + blockIdx.y;
+  const int i = k + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double2* A = reinterpret_cast<double2*>(S);
+  const double2 li = A[i + (long)j * n], lk = A[k + (long)j * n];
+  double2 v = A[i + (long)k * n];
+  v.x -= li.x * lk.x + li.y * lk.y;
+  v.y -= li.y * lk.x - li.x * lk.y;
+  A[i + (long)k * n] = v;
+}
+
+// column j of L^-1 (lower): x_j = 1 / L_jj, x_i = -(sum_{j<=k<i} L_ik x_k) / L_ii,
+// written as X = (L^-1)^H split: Xr[m + p n] = Re conj(Linv_pm), Xi = Im conj(Linv_pm)
+__global__ __launch_bounds__(256) void k_ztrinv_adj(const double* __restrict__ S, int n,
+                                                    const int* __restrict__ info,
+                                                    double* __restrict__ Xr,
+                                                    double* __restrict__ Xi) {
+  if (*info) return;
+  __shared__ double2 x[ZG_NMAX];
+  __shared__ double2 red[256];
+  const double2* L = reinterpret_cast<const double2*>(S);
+  const int j = blockIdx.x, tid = threadIdx.x;
+  if (tid == 0) x[j] = double2{1.0 / L[j + (long)j * n].x, 0.0};
+  __syncthreads();
+  for (int i = j + 1; i < n; i++) {
+    double2 p = {0.0, 0.0};
+    for (int k = j + tid; k < i; k += 256) {
+      const double2 a = L[i + (long)k * n], b = x[k];
+      p.x += a.x * b.x - a.y * b.y;
+      p.y += a.x * b.y + a.y * b.x;
+    }
+    red[tid] = p;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (tid < st) { red[tid].x += red[tid + st].x; red[tid].y += red[tid + st].y; }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double dinv = 1.0 / L[i + (long)i * n].x;
+      x[i] = double2{-red[0].x * dinv, -red[0].y * dinv};
+    }
+    __syncthreads();
+  }
+  // row j of X = conj of column j of L^-1: X[j][i] = conj(Linv_ij) (zero for i < j)
+  for (int i = tid; i < n; i += 256) {
+    const double2 v = i >= j ? x[i] : double2{0.0, 0.0};
+    Xr[j + (long)i * n] = v.x;
+    Xi[j + (long)i * n] = -v.y;
+  }
+}
+
 }  // namespace qbg
+
+extern "C" int qbg_gram_complex(qbg_ctx* c, double* cc, int ldc, int ngw, int nst) {
+  return qbg::guarded(c, [&]() -> int {
+    if (!c || !cc || ngw <= 0 || nst <= 0 || ldc < ngw) return QBG_E_INVAL;
+    if (nst > qbg::ZG_NMAX) return c->fail(QBG_E_INVAL, "gram_complex: nst > 2048");
+    QBG_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const long K = 2L * ngw, n2 = (long)nst * nst;
+    const size_t work = (size_t)64 * n2 + ((size_t)1 << 20);
+    double *Sr = nullptr, *Si = nullptr, *S = nullptr, *Xr = nullptr, *Xi = nullptr,
+           *Pi = nullptr, *T = nullptr, *W = nullptr;
+    int* info = nullptr;
+    auto release = [&]() {
+      for (void* p : {(void*)Sr, (void*)Si, (void*)S, (void*)Xr, (void*)Xi, (void*)Pi, (void*)T,
+                      (void*)W, (void*)info})
+        if (p) (void)hipFree(p);
+    };
+    if (hipMalloc((void**)&Sr, 8 * n2) != hipSuccess || hipMalloc((void**)&Si, 8 * n2) != hipSuccess ||
+        hipMalloc((void**)&S, 16 * n2) != hipSuccess || hipMalloc((void**)&Xr, 8 * n2) != hipSuccess ||
+        hipMalloc((void**)&Xi, 8 * n2) != hipSuccess ||
+        hipMalloc((void**)&Pi, sizeof(double) * K * nst) != hipSuccess ||
+        hipMalloc((void**)&T, sizeof(double) * K * nst) != hipSuccess ||
+        hipMalloc((void**)&W, sizeof(double) * work) != hipSuccess ||
+        hipMalloc((void**)&info, sizeof(int)) != hipSuccess) {
+      release();
+      return c->fail(QBG_E_NOMEM, "hipMalloc failed (gram_complex workspace)");
+    }
+    c->prof_begin(QBG_K_GRAM);
+    (void)hipMemsetAsync(info, 0, sizeof(int), s);
+    const long tot = (long)ngw * nst;
+    hipLaunchKernelGGL(qbg::k_times_i, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, cc,
+                       2L * ldc, (long)ngw, nst, Pi);
+    // Re S = P^T P, Im S = -P^T P'
+    if (qbg::gemm_syrk(cc, 2L * ldc, nst, K, 1.0, 0.0, Sr, nst, W, work, s) ||
+        qbg::gemm_tn_splitk(cc, 2L * ldc, Pi, K, nst, nst, K, -1.0, 0.0, Si, nst, W, work,
+                            QBG_PREC_FP64, s)) {
+      (void)hipStreamSynchronize(s);
+      release();
+      return c->fail(QBG_E_INVAL, "gram_complex workspace");
+    }
+    hipLaunchKernelGGL(qbg::k_zjoin, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, Sr, Si,
+                       nst, S);
+    for (int j = 0; j < nst; j++) {
+      hipLaunchKernelGGL(qbg::k_zchol_col, dim3(1), dim3(256), 0, s, S, nst, j, info);
+      const int m = nst - j - 1;
+      if (m > 0)
+        hipLaunchKernelGGL(qbg::k_zchol_upd, dim3((unsigned)((m + 255) / 256), (unsigned)m),
+                           dim3(256), 0, s, S, nst, j, info);
+    }
+    c->prof_end();
+    // the coefficients are only touched once the factorization succeeded
+    int hinfo = 0;
+    (void)hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipError_t er = hipStreamSynchronize(s);
+    if (er != hipSuccess || hinfo) {
+      release();
+      if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+      return c->fail(QBG_E_NOT_SPD, "gram_complex: overlap matrix not positive definite");
+    }
+    c->prof_begin(QBG_K_GRAM);
+    hipLaunchKernelGGL(qbg::k_ztrinv_adj, dim3(nst), dim3(256), 0, s, S, nst, info, Xr, Xi);
+    // C L^-H = P Re X + P' Im X
+    qbg::gemm_nn(cc, 2L * ldc, Xr, nst, K, nst, nst, T, K, 0, QBG_PREC_FP64, s);
+    qbg::gemm_nn(Pi, K, Xi, nst, K, nst, nst, T, K, 1, QBG_PREC_FP64, s);
+    (void)hipMemcpy2DAsync(cc, sizeof(double) * 2 * ldc, T, sizeof(double) * K, sizeof(double) * K,
+                           nst, hipMemcpyDeviceToDevice, s);
+    c->prof_end();
+    er = hipStreamSynchronize(s);
+    release();
+    if (er != hipSuccess) return c->fail(QBG_E_HIP, hipGetErrorString(er));
+    return QBG_OK;
+  });
+}

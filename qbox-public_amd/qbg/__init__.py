@@ -6,4 +6,4 @@ the C ABI of include/qbg.h); this package is the Python harness over it.
 from . import configs  # noqa: F401
 from ._lib import (Context, DArray, EnergyFunctional, Graph, Grid, Mixer,  # noqa: F401
                    NonLocalPotential, Profiler, QbgError, WfStepper, ENERGY_NAMES, fft_splits,
-                   gram, gram_noalloc, load, polar, stream_join)
+                   gram, gram_complex, gram_noalloc, load, polar, stream_join)

@@ -60,6 +60,7 @@ SIGNATURES = {
     "qbg_prof_reset": [_vp],
     "qbg_prof_read": [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "qbg_gram": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "qbg_gram_complex": [_vp, _vp, C.c_int, C.c_int, C.c_int],
     "qbg_gram_noalloc": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "qbg_gram_noalloc_bytes": [],
     "qbg_ef_create": [_vp, _dp, _dp, C.c_double, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
@@ -449,6 +450,12 @@ def fft_splits(n) -> list:
 def gram(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):
     """SlaterDet::gram on device coefficients (in place)."""
     ctx.check(ctx.lib.qbg_gram(ctx.h, c.p, ldc, ngw, nst, PREC[prec]))
+
+
+def gram_complex(ctx: Context, c: DArray, ldc, ngw, nst):
+    """SlaterDet::gram of the complex (k != 0) basis on device coefficients
+    (in place, FP64)."""
+    ctx.check(ctx.lib.qbg_gram_complex(ctx.h, c.p, ldc, ngw, nst))
 
 
 def gram_noalloc(ctx: Context, c: DArray, ldc, ngw, nst, prec="fp64"):

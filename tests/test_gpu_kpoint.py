@@ -4,10 +4,10 @@ src/NonLocalPotential.cpp:1429-1775).
 
 The wavefunction basis (the full sphere |k+G| < sqrt(2 ecut), in the
 reference's rod order) equals the oracle's bit for bit, and so do |k+G|^2
-and k+G.  The evaluation (density, update_vhxc, energies, H*psi, forces)
-meets FP64 to 1e-10 (relative; arrays max abs difference over max abs
+and k+G.  The evaluation (density, update_vhxc, energies, H*psi, forces,
+and the seven-part stress; nspin 1 and 2) meets FP64 to 1e-10 (relative; arrays max abs difference over max abs
 value) and FP32, BF16X3 and BF16X2 at the metric's bar (1e-6 Ha/atom,
-1e-5 Ha/bohr, H*psi 1e-5).  kpoint 0 through qbg_ef_create_k is the Gamma functional, bit for
+1e-5 Ha/bohr, H*psi 1e-5, stress 1e-5 of its largest component).  kpoint 0 through qbg_ef_create_k is the Gamma functional, bit for
 bit.  What is not built at k != 0 is refused with QBG_E_INVAL.
 """
 import dataclasses
@@ -39,10 +39,11 @@ def _systems():
     t6 = cf.config(16)                                   # d + s/p species, k = (0.25, 0.1, -0.2)
     half = dataclasses.replace(cf.config(11), kpoint=np.array([0.5, 0.0, 0.0]))   # zone face
     mono = dataclasses.replace(cf.config(12), kpoint=np.array([-0.3, 0.2, 0.45]))  # monoclinic
-    return {"t6": t6, "t1_half": half, "t2_mono": mono}
+    spin = cf.spin_polarized(t6, 12, 1, 1)                 # nspin 2 at k != 0
+    return {"t6": t6, "t1_half": half, "t2_mono": mono, "t6_spin": spin}
 
 
-@pytest.fixture(scope="module", params=["t6", "t1_half", "t2_mono"])
+@pytest.fixture(scope="module", params=["t6", "t1_half", "t2_mono", "t6_spin"])
 def system(request, ctx, oracle_mod):
     import qbg
     s = _systems()[request.param]
@@ -84,6 +85,12 @@ def test_evaluate(system, ctx, prec):
             assert err < 1e-10, (t, got[t], ref[t], err)
         else:
             assert abs(got[t] - ref[t]) / s.nat < 1e-6, (t, got[t], ref[t])
+    # the evaluation's H*psi starts from the transforms update_density kept;
+    # energy() alone transforms again: the same numbers
+    hp2 = ctx.array(c.shape, np.complex128)
+    got2 = ef.energy(dc, hp2, forces=True, stress=False, prec=prec)
+    assert got2["etotal"] == got["etotal"] and np.array_equal(got2["fion"], got["fion"])
+    assert np.array_equal(hp2.download(), hp.download())
     if prec == "fp64":
         assert relerr(ef.get(0), o.get(0)) < 1e-10          # rhor
         assert relerr(ef.get(3), o.get(3)) < 1e-10          # rhog
@@ -94,16 +101,23 @@ def test_evaluate(system, ctx, prec):
         assert ferr < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "bf16x2"])
+def test_stress(system, ctx, prec):
+    name, s, o, ef, c = system
+    dc = ctx.to_device(c)
+    got = ef.evaluate(dc, None, forces=True, stress=True, prec=prec)
+    ref = o.evaluate(c, hpsi=False, forces=True, stress=True)
+    scale = np.abs(ref["sigma"][0]).max()
+    err = np.abs(got["sigma"] - ref["sigma"]).max(axis=1) / scale
+    print(f"  sigma rel err per row ({prec}):", " ".join(f"{x:.1e}" for x in err), flush=True)
+    assert err.max() < (1e-10 if prec == "fp64" else 1e-5)
+
+
 def test_not_built_is_refused(system, ctx):
     import qbg
     name, s, o, ef, c = system
     dc = ctx.to_device(c)
     hp = ctx.array(c.shape, np.complex128)
-    with pytest.raises(qbg.QbgError) as e:
-        ef.evaluate(dc, hp, forces=True, stress=True)
-    assert e.value.code == -1
-    with pytest.raises(qbg.QbgError):
-        ef.set_nspin(2, 1)
     with pytest.raises(qbg.QbgError):
         ef.cell_moved(np.asarray(s.cell) * 1.01)
     with pytest.raises(qbg.QbgError):
@@ -131,3 +145,34 @@ def test_zero_kpoint_is_gamma(ctx):
     rb = b.evaluate(dc, hb, forces=True)
     assert ra["etotal"] == rb["etotal"] and np.array_equal(ra["fion"], rb["fion"])
     assert np.array_equal(ha.download(), hb.download())
+
+
+@pytest.mark.parametrize("nst", [6, 40, 300])
+def test_gram_complex(ctx, oracle_mod, nst):
+    """qbg_gram_complex against the oracle's SlaterDet::gram of the complex
+    basis (pinned to numpy's Cholesky): 1e-12 relative, and orthonormal."""
+    import qbg
+    s = dataclasses.replace(cf.config(16), nst=nst, occ_bands=None)
+    print(f"[case] gram_complex nst {nst}", flush=True)
+    o = oracle_mod.Oracle(dataclasses.replace(s, nst=6))
+    kpg2 = o.kpg()[0]
+    c = cf.random_coefficients_k(2500 + nst, kpg2, nst)
+    ref = oracle_mod.gram_complex(c)
+    dc = ctx.to_device(c)
+    qbg.gram_complex(ctx, dc, o.ngw, o.ngw, nst)
+    got = dc.download()
+    assert relerr(got, ref) < 1e-12
+    assert np.abs(got @ got.conj().T - np.eye(nst)).max() < 1e-12
+
+
+def test_gram_complex_not_spd(ctx, oracle_mod):
+    import qbg
+    s = cf.config(16)
+    o = oracle_mod.Oracle(s)
+    c = cf.random_coefficients_k(2600, o.kpg()[0], 5)
+    c[3] = 0.0                                # a zero band: pivot 4 is exactly 0
+    dc = ctx.to_device(c)
+    with pytest.raises(qbg.QbgError) as e:
+        qbg.gram_complex(ctx, dc, o.ngw, o.ngw, 5)
+    assert e.value.code == -4
+    assert np.array_equal(dc.download(), c)

@@ -2,8 +2,8 @@
 wavefunction basis the full sphere |k+G| < sqrt(2 ecut) (src/Basis.cpp:237,
 446-481), one band per transform (src/SlaterDet.cpp:225-298, 410-477),
 kinetic energy with 1/2 |k+G|^2 (src/EnergyFunctional.cpp:441-571, 789-829),
-complex projections anl^H c, H psi and forces with k + G
-(src/NonLocalPotential.cpp:1429-1445, 1572-1630, 1722-1775).
+complex projections anl^H c, H psi, forces and stress with k + G
+(src/NonLocalPotential.cpp:1429-1445, 1572-1630, 1722-1775, 1990-2066).
 
 What pins it, beside the reference's formulas:
 - the complex branch at k = 0 on coefficients of real functions (c(-G) =
@@ -17,7 +17,9 @@ What pins it, beside the reference's formulas:
   G anywhere (kinetic, projectors, their phases) fails it;
 - time reversal: k -> -k with c(G) -> conj c(-G) leaves every energy and
   force unchanged and maps H psi the same way;
-- H psi and the forces are energy gradients (central differences).
+- H psi and the forces are energy gradients (central differences), and the
+  stress is the strain derivative of the energy at fixed k in units of b
+  (Richardson-extrapolated central differences, each term).
 """
 import dataclasses
 
@@ -52,8 +54,9 @@ def test_complex_branch_at_gamma_equals_real_basis(oracle_mod):
     minus = np.array([pos[tuple(-int(v) for v in g)] for g in b_r["idx"]])
     c_c[:, minus] = np.conj(c_r)
     c_c[:, plus] = c_r
-    r_r = o_r.evaluate(c_r, hpsi=True, forces=True)
-    r_c = o_c.evaluate(c_c, hpsi=True, forces=True)
+    r_r = o_r.evaluate(c_r, hpsi=True, forces=True, stress=True)
+    r_c = o_c.evaluate(c_c, hpsi=True, forces=True, stress=True)
+    assert np.abs(r_c["sigma"] - r_r["sigma"]).max() < 1e-11 * np.abs(r_r["sigma"]).max()
     for t in TERMS:
         print(f"  {t} real {r_r[t]:.12e} complex {r_c[t]:.12e}")
         assert abs(r_c[t] - r_r[t]) <= 1e-11 * max(1.0, abs(r_r[t])), t
@@ -85,8 +88,10 @@ def test_half_zone_band_is_doubled_cell_gamma_band(oracle_mod, idx):
     assert sorted(odd) == sorted(img)
     c_s = np.zeros((nst, o_s.ngw), dtype=complex)
     c_s[:, img] = c_p
-    r_p = o_p.evaluate(c_p, hpsi=True, forces=True)
-    r_s = o_s.evaluate(c_s, hpsi=True, forces=True)
+    r_p = o_p.evaluate(c_p, hpsi=True, forces=True, stress=True)
+    r_s = o_s.evaluate(c_s, hpsi=True, forces=True, stress=True)
+    # the stress is intensive: every part equal
+    assert np.abs(r_s["sigma"] - r_p["sigma"]).max() < 1e-10 * np.abs(r_p["sigma"]).max()
     for t in TERMS:
         print(f"  {t} 2 x cell {2 * r_p[t]:.12e} doubled cell {r_s[t]:.12e}")
         assert abs(r_s[t] - 2.0 * r_p[t]) <= 1e-10 * max(1.0, abs(r_p[t])), t
@@ -161,9 +166,75 @@ def test_forces_are_energy_gradient_at_k(oracle_mod):
         assert abs(fd - r["fion"][a, j]) <= 1e-6 * max(1.0, abs(fd))
 
 
-def test_stress_refused_at_k(oracle_mod):
+PAIRS = {"ekin": 1, "eps": 2, "enl": 3, "ehart": 4, "exc": 5, "esr": 6, "etotal": 0}
+VOIGT = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]
+
+
+@pytest.mark.parametrize("idx", [16, 12])
+def test_stress_vs_strain_fd_at_k(oracle_mod, idx):
+    s = cf.config(idx)
+    if s.kpoint is None:
+        s = dataclasses.replace(s, kpoint=np.array([-0.3, 0.2, 0.45]))
+    ref = s.cell * 1.04                   # one basis for every strained cell
+    o0 = oracle_mod.Oracle(s, refcell=ref)
+    c = _kcoeffs(o0, 2100 + idx, s.nst)
+    sigma = o0.evaluate(c, hpsi=False, forces=False, stress=True)["sigma"]
+    assert np.allclose(sigma[0], sigma[1:].sum(axis=0), rtol=0, atol=1e-14)
+    omega = abs(np.linalg.det(s.cell))
+    idx0 = o0.basis(0)["idx"]
+
+    def diff(u):
+        F = np.eye(3) + u
+        out = []
+        for sg in (1.0, -1.0):
+            Fs = np.eye(3) + sg * u
+            o = oracle_mod.Oracle(dataclasses.replace(s, cell=s.cell @ Fs.T, tau=s.tau @ Fs.T),
+                                  refcell=ref)
+            assert np.array_equal(o.basis(0)["idx"], idx0)
+            out.append(o.evaluate(c, hpsi=False, forces=False))
+        return {t: out[0][t] - out[1][t] for t in PAIRS}
+
+    rng = np.random.default_rng(idx)
+    v = 1e-4 * rng.uniform(-1.0, 1.0, 6)
+    u = np.zeros((3, 3))
+    for k, (i, j) in enumerate(VOIGT):
+        u[i, j] = u[j, i] = v[k]
+    d1, d2 = diff(u), diff(0.5 * u)
+    for term, row in PAIRS.items():
+        de = (8.0 * d2[term] - d1[term]) / 3.0
+        st = sigma[row]
+        pred = -2.0 * omega * (sum(v[k] * st[k] for k in range(3)) +
+                               2.0 * sum(v[k] * st[k] for k in range(3, 6)))
+        print(f"{term:7s} fd {de: .10e} sigma {pred: .10e} diff {de - pred: .2e}")
+        assert abs(de - pred) <= 2e-7 * abs(de) + 1e-12, (term, de, pred)
+
+
+def test_gram_complex(oracle_mod):
+    """SlaterDet::gram of the complex basis: the result is orthonormal, equals
+    numpy's Cholesky orthonormalization (S = c c^H = L L^H, c <- L^-1 c for
+    band rows), and at k = 0 on coefficients of real functions equals the
+    real basis's gram."""
     s = cf.config(16)
     o = oracle_mod.Oracle(s)
-    c = _kcoeffs(o, 2101, s.nst)
-    with pytest.raises(Exception):
-        o.evaluate(c, hpsi=False, forces=False, stress=True)
+    kpg2 = o.kpg()[0]
+    c = cf.random_coefficients_k(2400, kpg2, s.nst)
+    g = oracle_mod.gram_complex(c)
+    assert np.abs(g @ g.conj().T - np.eye(s.nst)).max() < 1e-13
+    assert np.abs(g - cf.orthonormal_k(c)).max() < 1e-12 * np.abs(g).max()
+    # the Gamma limit
+    s0 = cf.config(13)
+    o_r = oracle_mod.Oracle(s0)
+    o_c = oracle_mod.Oracle(s0, force_complex=True)
+    b_r = o_r.basis(0)
+    pos = _index(o_c.basis(0)["idx"])
+    plus = np.array([pos[tuple(int(v) for v in x)] for x in b_r["idx"]])
+    minus = np.array([pos[tuple(-int(v) for v in x)] for x in b_r["idx"]])
+    c_r = cf.random_coefficients(2401, b_r["g2"], s0.nst)
+    c_r[:, 0] = c_r[:, 0].real
+    c_c = np.zeros((s0.nst, o_c.ngw), dtype=complex)
+    c_c[:, minus] = np.conj(c_r)
+    c_c[:, plus] = c_r
+    g_r = oracle_mod.gram(c_r)
+    g_c = oracle_mod.gram_complex(c_c)
+    assert np.abs(g_c[:, plus] - g_r).max() < 1e-12
+    assert np.abs(g_c[:, minus] - np.conj(g_r)).max() < 1e-12
