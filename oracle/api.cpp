@@ -91,6 +91,44 @@ void* qbo_create(const double* cell9, const double* refcell9, double ecut_ha,
   return qbo_create_k(cell9, refcell9, ecut_ha, nsp, sp, na, tau, nst, occ, grid3, nullptr, 0);
 }
 
+// several k-points (kpoints3n: 3 per k-point, units of b; weights summing
+// to 1); every k-point on the complex basis
+void* qbo_create_kset(const double* cell9, const double* refcell9, double ecut_ha,
+                      int nsp, const qbo_species_desc* sp, const int* na,
+                      const double* tau, int nst, const double* occ, const int* grid3,
+                      int nkp, const double* kpoints3n, const double* weights) {
+  if (nkp < 1 || !kpoints3n || !weights) { g_err = "kset: nkp, kpoints, weights"; return nullptr; }
+  if (nkp == 1) return qbo_create_k(cell9, refcell9, ecut_ha, nsp, sp, na, tau, nst, occ, grid3,
+                                    kpoints3n, 1);
+  // the set is attached before setup: create with k-point 0, then rebuild
+  void* h = qbo_create_k(cell9, refcell9, ecut_ha, nsp, sp, na, tau, nst, occ, grid3,
+                         kpoints3n, 1);
+  if (!h) return nullptr;
+  try {
+    System* s = (System*)h;
+    s->kpts.assign(kpoints3n, kpoints3n + 3 * nkp);
+    s->kwts.assign(weights, weights + nkp);
+    s->setup();
+    return s;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    delete (System*)h;
+    return nullptr;
+  }
+}
+
+// k-point ik of a set (ik = 0 also for a single k-point): its basis size,
+// and idx (3 ngw), |k+G|^2, k+G (3 ngw) when the pointers are given
+int qbo_kset_basis(void* h, int ik, int* idx, double* kpg2, double* kpgx) {
+  System* s = (System*)h;
+  if (ik < 0 || (ik > 0 && ik >= (int)s->kset.size())) return -1;
+  const Basis& b = ik == 0 ? s->wb : s->kset[ik].wb;
+  if (idx) std::memcpy(idx, b.idx.data(), sizeof(int) * 3 * b.size);
+  if (kpg2) std::memcpy(kpg2, b.kpg2.data(), sizeof(double) * b.size);
+  if (kpgx) std::memcpy(kpgx, b.kpgx.data(), sizeof(double) * 3 * b.size);
+  return b.size;
+}
+
 void qbo_destroy(void* h) { delete (System*)h; }
 
 // the wavefunction basis's k + G: |k+G|^2 (ngw) and its components (3 ngw);

@@ -14,6 +14,11 @@ static const double fpi = 4.0 * M_PI;
 // ---------------------------------------------------------------- setup
 void System::setup() {
   const Cell* ref = has_ref ? &refcell : nullptr;
+  if (kpts.size() / 3 > 1) {           // a set: every k-point on the complex basis
+    for (int i = 0; i < 3; i++) kpoint[i] = kpts[i];
+    force_complex = true;
+    if (nspin != 1) throw std::invalid_argument("a k-point set with nspin 2 is not restated");
+  }
   wb.resize(cell, ref, ecut, kpoint, force_complex);
   vb.resize(cell, ref, 4.0 * ecut);          // src/ChargeDensity.cpp:37
   int n[3];
@@ -28,6 +33,17 @@ void System::setup() {
   }
   vft.init(vb, n[0], n[1], n[2]);
   ft.init(wb, n[0], n[1], n[2]);             // src/ChargeDensity.cpp:98
+  // further k-points of a set: their bases and transforms on the same grid
+  kset.clear();
+  const int nkp = (int)kpts.size() / 3;
+  if (nkp > 1) {
+    kset.resize(nkp);
+    for (int ik = 0; ik < nkp; ik++) kset[ik].weight = kwts[ik];
+    for (int ik = 1; ik < nkp; ik++) {
+      kset[ik].wb.resize(cell, ref, ecut, &kpts[3 * ik], true);
+      kset[ik].ft.init(kset[ik].wb, n[0], n[1], n[2]);
+    }
+  }
   for (auto& s : sp) s.initialize();
   eself_total = 0.0;
   for (size_t is = 0; is < sp.size(); is++)
@@ -76,6 +92,11 @@ void System::cell_moved() {
     }
   }
   update_twnl();
+  for (size_t ik = 1; ik < kset.size(); ik++) {
+    swap_k(kset[ik]);
+    update_twnl();
+    swap_k(kset[ik]);
+  }
 }
 
 // src/NonLocalPotential.cpp:54-232 (KB, wsg = 1) and :235-520 (l = 0, 1)
@@ -594,10 +615,20 @@ void System::update_density(const cplx* c) {
   std::fill(rhor.begin(), rhor.end(), 0.0);
   std::fill(rhor_dn.begin(), rhor_dn.end(), 0.0);
   const int nup = nspin == 2 ? nst_up : nst;
-  compute_density(ft, c, wb.size, nup, occ.data(), 1.0, cell.volume, rhor.data());
-  if (nspin == 2 && nst > nup)
-    compute_density(ft, c + (size_t)nup * wb.size, wb.size, nst - nup, occ.data() + nup, 1.0,
-                    cell.volume, rhor_dn.data());
+  // each k-point with its weight (src/ChargeDensity.cpp:136-160)
+  const int nk = kset.empty() ? 1 : (int)kset.size();
+  std::vector<size_t> koff(nk);            // (before any k-point is swapped in)
+  for (int ik = 0; ik < nk; ik++) koff[ik] = k_offset(ik);
+  for (int ik = 0; ik < nk; ik++) {
+    if (ik > 0) swap_k(kset[ik]);
+    const double w = kset.empty() ? 1.0 : kset[ik].weight;
+    const cplx* ck = c + koff[ik];
+    compute_density(ft, ck, wb.size, nup, occ.data(), w, cell.volume, rhor.data());
+    if (nspin == 2 && nst > nup)
+      compute_density(ft, ck + (size_t)nup * wb.size, wb.size, nst - nup, occ.data() + nup, w,
+                      cell.volume, rhor_dn.data());
+    if (ik > 0) swap_k(kset[ik]);
+  }
   const long N = ft.np012();
   const double omega = cell.volume;
   std::vector<cplx> rhotmp(N);
@@ -1039,36 +1070,92 @@ double System::nl_energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
 
 // src/EnergyFunctional.cpp:397-1044 (nspin = 1, one k-point, LDA, no
 // confinement, no external potential or fields)
+// the k-point set: exchange k-point k's basis, transform and tables with the
+// members (fixing the transforms' basis pointers)
+void System::swap_k(KState& k) {
+  std::swap(wb, k.wb);
+  std::swap(ft, k.ft);
+  std::swap(twnl, k.twnl);
+  std::swap(dtwnl, k.dtwnl);
+  std::swap(lproj, k.lproj);
+  ft.basis = &wb;
+  k.ft.basis = &k.wb;
+}
+
+// (with no k-point swapped in: the members hold k-point 0)
+size_t System::k_offset(int ik) const {
+  if (kset.empty()) return 0;
+  size_t off = 0;
+  for (int j = 0; j < ik; j++) off += (size_t)nst * (j == 0 ? wb.size : kset[j].wb.size);
+  return off;
+}
+
 double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
                       bool compute_forces, bool compute_stress, Results& r) {
-  const int ngw = wb.size, ngv = vb.size;
+  const int ngv = vb.size;
   const double omega = cell.volume, omega_inv = 1.0 / omega;
   int nat = 0;
   for (auto& t : tau) nat += (int)t.size() / 3;
   std::vector<double> fion(3 * nat, 0.0);
-  if (compute_hpsi) std::fill(hpsi, hpsi + (size_t)ngw * nst, cplx(0.0));
-  // ekin (:441-571), fac = 1 for the real basis (G and -G times 1/2), 1/2
-  // for the complex basis
+  const int nk = kset.empty() ? 1 : (int)kset.size();
+  if (compute_hpsi) std::fill(hpsi, hpsi + k_offset(nk - 1) + (size_t)nst *
+                                  (nk == 1 ? wb.size : kset[nk - 1].wb.size), cplx(0.0));
+  // the parts of each k-point, weighted by its weight (:446-543, 703-716):
+  // ekin (:441-571; fac = 1 for the real basis, G and -G times 1/2, and 1/2
+  // for the complex basis), the nonlocal energy, forces and stress, and
+  // H psi (unweighted, :789-838)
   double ts[7] = {0, 0, 0, 0, 0, 0, 0};
-  {
+  double enl = 0.0, sig_enl[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> fenl(3 * nat, 0.0);
+  std::vector<size_t> koff(nk);            // (before any k-point is swapped in)
+  for (int ik = 0; ik < nk; ik++) koff[ik] = k_offset(ik);
+  for (int ik = 0; ik < nk; ik++) {
+    if (ik > 0) swap_k(kset[ik]);
+    const double w = kset.empty() ? 1.0 : kset[ik].weight;
+    const int ngw = wb.size;
+    const cplx* ck = c + koff[ik];
+    cplx* hk = compute_hpsi ? hpsi + koff[ik] : nullptr;
     const double kfac = wb.real ? 1.0 : 0.5;
     std::vector<double> psi2sum(ngw);
 #pragma omp parallel for schedule(static)
     for (int ig = 0; ig < ngw; ig++) {
       double t = 0.0;
-      for (int n = 0; n < nst; n++) t += kfac * occ[n] * std::norm(c[ig + (size_t)n * ngw]);
+      for (int n = 0; n < nst; n++) t += kfac * occ[n] * std::norm(ck[ig + (size_t)n * ngw]);
       psi2sum[ig] = t;
     }
+    double tk[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int ig = 0; ig < ngw; ig++) {
       const double p = psi2sum[ig];
-      ts[0] += p * wb.kpg2[ig];
+      tk[0] += p * wb.kpg2[ig];
       if (compute_stress) {
         const double gx = wb.kpgx[ig], gy = wb.kpgx[ngw + ig], gz = wb.kpgx[2 * ngw + ig];
         const double f = 2.0 * p;
-        ts[1] += f * gx * gx; ts[2] += f * gy * gy; ts[3] += f * gz * gz;
-        ts[4] += f * gx * gy; ts[5] += f * gy * gz; ts[6] += f * gx * gz;
+        tk[1] += f * gx * gx; tk[2] += f * gy * gy; tk[3] += f * gz * gz;
+        tk[4] += f * gx * gy; tk[5] += f * gy * gz; tk[6] += f * gx * gz;
       }
     }
+    for (int i = 0; i < 7; i++) ts[i] += w * tk[i];
+    // NL (:686-742)
+    std::vector<double> fk;
+    double sk[6];
+    enl += w * nl_energy(ck, compute_hpsi, hk, compute_forces, fk, compute_stress, sk);
+    if (compute_forces)
+      for (int i = 0; i < 3 * nat; i++) fenl[i] += w * fk[i];
+    for (int i = 0; i < 6; i++) sig_enl[i] += w * sk[i];
+    // H psi (:789-838)
+    if (compute_hpsi) {
+#pragma omp parallel for schedule(static)
+      for (int n = 0; n < nst; n++)
+        for (int ig = 0; ig < ngw; ig++)
+          hk[ig + (size_t)n * ngw] += 0.5 * wb.kpg2[ig] * ck[ig + (size_t)n * ngw];
+      // rs_mul_add with the potential of each band's spin (:831)
+      const int nup = nspin == 2 ? nst_up : nst;
+      rs_mul_add(ft, v_r.data(), ck, ngw, nup, hk);
+      if (nspin == 2 && nst > nup)
+        rs_mul_add(ft, v_r_dn.data(), ck + (size_t)nup * ngw, ngw, nst - nup,
+                   hk + (size_t)nup * ngw);
+    }
+    if (ik > 0) swap_k(kset[ik]);
   }
   r.ekin = ts[0];
   for (int i = 0; i < 6; i++) r.sigma_ekin[i] = ts[i + 1] * omega_inv;
@@ -1109,28 +1196,14 @@ double System::energy(const cplx* c, bool compute_hpsi, cplx* hpsi,
     for (int i = 0; i < 6; i++)
       r.sigma_ehart[i] = (i < 3 ? ehart * omega_inv : 0.0) - 2.0 * fpi * sh[i];
   }
-  // NL (:686-742)
-  std::vector<double> fion_enl;
-  r.enl = nl_energy(c, compute_hpsi, hpsi, compute_forces, fion_enl, compute_stress, r.sigma_enl);
+  r.enl = enl;
+  for (int i = 0; i < 6; i++) r.sigma_enl[i] = sig_enl[i];
   if (compute_forces)
-    for (int i = 0; i < 3 * nat; i++) fion[i] += fion_enl[i];
+    for (int i = 0; i < 3 * nat; i++) fion[i] += fenl[i];
   r.eps = eps; r.ehart = ehart; r.ehart_e = ehart_e; r.ehart_ep = ehart_ep;
   r.ehart_p = ehart_p; r.exc = exc; r.dxc = dxc; r.esr = esr; r.eself = eself_total;
   r.ecoul = ehart + esr - eself_total;
   r.etotal = r.ekin + 0.0 + eps + r.enl + r.ecoul + exc + 0.0 + 0.0;
-  // H psi (:789-838)
-  if (compute_hpsi) {
-#pragma omp parallel for schedule(static)
-    for (int n = 0; n < nst; n++)
-      for (int ig = 0; ig < ngw; ig++)
-        hpsi[ig + (size_t)n * ngw] += 0.5 * wb.kpg2[ig] * c[ig + (size_t)n * ngw];
-    // rs_mul_add with the potential of each band's spin (:831)
-    const int nup = nspin == 2 ? nst_up : nst;
-    rs_mul_add(ft, v_r.data(), c, ngw, nup, hpsi);
-    if (nspin == 2 && nst > nup)
-      rs_mul_add(ft, v_r_dn.data(), c + (size_t)nup * ngw, ngw, nst - nup,
-                 hpsi + (size_t)nup * ngw);
-  }
   // local forces (:841-930), eigr as products of per-axis tables
   if (compute_forces) {
     const int* idx = vb.idx.data();

@@ -63,6 +63,11 @@ def lib():
         _lib.qbo_create_k.restype = C.c_void_p
         _lib.qbo_create_k.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
                                       _ip, _dp, C.c_int, _dp, _ip, _dp, C.c_int]
+        _lib.qbo_create_kset.restype = C.c_void_p
+        _lib.qbo_create_kset.argtypes = [_dp, _dp, C.c_double, C.c_int, C.POINTER(SpeciesDesc),
+                                         _ip, _dp, C.c_int, _dp, _ip, C.c_int, _dp, _dp]
+        _lib.qbo_kset_basis.restype = C.c_int
+        _lib.qbo_kset_basis.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp]
         _lib.qbo_kpg.restype = C.c_int
         _lib.qbo_kpg.argtypes = [C.c_void_p, _dp, _dp]
         _lib.qbo_create.restype = C.c_void_p
@@ -117,7 +122,8 @@ ENERGY_NAMES = ["ekin", "eps", "enl", "ehart", "ehart_e", "ehart_ep", "ehart_p",
 class Oracle:
     """One Qbox-like system on the CPU (EnergyFunctional + ChargeDensity)."""
 
-    def __init__(self, sysdef, refcell=None, grid=None, kpoint=None, force_complex=False):
+    def __init__(self, sysdef, refcell=None, grid=None, kpoint=None, force_complex=False,
+                 kpoints=None, weights=None):
         """refcell (3x3, rows a0 a1 a2): the reference cell that fixes the
         basis membership and grid sizes (src/Basis.cpp:314-325), so that
         strained copies of a system share one basis.  grid (np0, np1, np2):
@@ -126,7 +132,10 @@ class Oracle:
         hold the density basis.  kpoint (3, in units of the reciprocal
         vectors; default the system's own `kpoint` attribute or 0): k != 0
         gives the complex (full-sphere) wavefunction basis, with one weight-1
-        k-point; force_complex builds that basis at k = 0 as well."""
+        k-point; force_complex builds that basis at k = 0 as well.  kpoints
+        (nkp x 3) and weights (nkp, summing to 1): a k-point set, every
+        k-point on the complex basis; the coefficients are then a list of
+        (nst, ngw_k) arrays, one per k-point."""
         L = lib()
         self.sysdef = sysdef
         if refcell is None:
@@ -154,11 +163,23 @@ class Oracle:
         if kpoint is not None:
             kp = np.ascontiguousarray(kpoint, dtype=float).ravel()
             self._keep.append(kp)
-        h = L.qbo_create_k(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
-                           len(sysdef.species), descs,
-                           _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ),
-                           _ptr(g3, _ip) if g3 is not None else None,
-                           _ptr(kp) if kp is not None else None, 1 if force_complex else 0)
+        self.nkp = 0
+        if kpoints is not None:
+            kps = np.ascontiguousarray(kpoints, dtype=float).reshape(-1, 3)
+            wts = np.ascontiguousarray(weights, dtype=float).ravel()
+            self.nkp = kps.shape[0]
+            self._keep += [kps, wts]
+            h = L.qbo_create_kset(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
+                                  len(sysdef.species), descs, _ptr(na, _ip), _ptr(tau),
+                                  sysdef.nst, _ptr(occ),
+                                  _ptr(g3, _ip) if g3 is not None else None, self.nkp,
+                                  _ptr(kps), _ptr(wts))
+        else:
+            h = L.qbo_create_k(_ptr(cell), _ptr(ref) if ref is not None else None, sysdef.ecut,
+                               len(sysdef.species), descs,
+                               _ptr(na, _ip), _ptr(tau), sysdef.nst, _ptr(occ),
+                               _ptr(g3, _ip) if g3 is not None else None,
+                               _ptr(kp) if kp is not None else None, 1 if force_complex else 0)
         if not h:
             raise RuntimeError(L.qbo_last_error().decode())
         self.h = C.c_void_p(h)
@@ -251,8 +272,24 @@ class Oracle:
         lib().qbo_rs_mul_add(self.h, _ptr(v), _ptr(c), c.shape[0], _ptr(h))
         return h
 
+    def kset_basis(self, ik):
+        """k-point ik of a set: idx (ngw x 3), |k+G|^2, k+G (3 x ngw)."""
+        n = lib().qbo_kset_basis(self.h, ik, None, None, None)
+        idx = np.zeros(3 * n, dtype=np.int32)
+        kpg2 = np.zeros(n)
+        kpgx = np.zeros(3 * n)
+        lib().qbo_kset_basis(self.h, ik, _ptr(idx, _ip), _ptr(kpg2), _ptr(kpgx))
+        return dict(idx=idx.reshape(n, 3), kpg2=kpg2, kpgx=kpgx.reshape(3, n))
+
+    def _pack(self, c):
+        """a k-point set's per-k coefficient list as one buffer"""
+        if self.nkp:
+            return np.ascontiguousarray(np.concatenate([np.ravel(x) for x in c]),
+                                        dtype=np.complex128)
+        return np.ascontiguousarray(c, dtype=np.complex128)
+
     def update_density(self, c):
-        c = np.ascontiguousarray(c, dtype=np.complex128)
+        c = self._pack(c)
         lib().qbo_update_density(self.h, _ptr(c))
 
     def update_vhxc(self, stress=False):
@@ -278,7 +315,8 @@ class Oracle:
         lib().qbo_set_vr(self.h, _ptr(v))
 
     def energy(self, c, hpsi=True, forces=True, stress=False):
-        c = np.ascontiguousarray(c, dtype=np.complex128)
+        shapes = [np.shape(x) for x in c] if self.nkp else None
+        c = self._pack(c)
         h = np.zeros_like(c) if hpsi else None
         e = np.zeros(14)
         f = np.zeros(3 * self.sysdef.nat)
@@ -290,6 +328,13 @@ class Oracle:
         out = dict(zip(ENERGY_NAMES, e.tolist()))
         out["fion"] = f.reshape(-1, 3)
         out["sigma"] = sig.reshape(7, 6)
+        if h is not None and shapes is not None:
+            parts, off = [], 0
+            for sh in shapes:
+                n = sh[0] * sh[1]
+                parts.append(h[off:off + n].reshape(sh))
+                off += n
+            h = parts
         out["hpsi"] = h
         return out
 

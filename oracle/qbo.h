@@ -166,6 +166,23 @@ struct System {
   // the wavefunction basis complex
   double kpoint[3] = {0.0, 0.0, 0.0};
   bool force_complex = false;
+  // several k-points (Wavefunction::add_kpoint, src/Wavefunction.cpp:400-456):
+  // each holds its basis, transform and projector tables; the weights sum
+  // to 1; the coefficients of k-point ik follow those of ik - 1 (nst x ngw_k
+  // each).  Empty: the one k-point of wb and ft.  Between calls the members
+  // wb, ft, twnl, dtwnl, lproj hold k-point 0; a loop over the k-points
+  // swaps each one in (swap_k) and back.
+  struct KState {
+    Basis wb;
+    FFT3 ft;
+    std::vector<std::vector<double>> twnl, dtwnl;
+    std::vector<std::vector<int>> lproj;
+    double weight = 1.0;
+  };
+  std::vector<KState> kset;
+  std::vector<double> kpts, kwts;      // set before setup(): 3 per k-point, weights
+  void swap_k(KState& k);
+  size_t k_offset(int ik) const;       // complex elements before k-point ik's coefficients
   // nspin 2: bands [0, nst_up) spin up, [nst_up, nst) spin down
   // (Wavefunction nst_[ispin], src/Wavefunction.cpp:326-336)
   int nspin = 1, nst_up = 0;

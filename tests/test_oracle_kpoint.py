@@ -238,3 +238,73 @@ def test_gram_complex(oracle_mod):
     g_c = oracle_mod.gram_complex(c_c)
     assert np.abs(g_c[:, plus] - g_r).max() < 1e-12
     assert np.abs(g_c[:, minus] - np.conj(g_r)).max() < 1e-12
+
+
+# ------------------------------------------------------------ k-point sets
+def _kset_coeffs(o, nkp, nst, seed):
+    out = []
+    for ik in range(nkp):
+        kpg2 = o.kset_basis(ik)["kpg2"]
+        out.append(cf.orthonormal_k(cf.random_coefficients_k(seed + ik, kpg2, nst)))
+    return out
+
+
+def test_kset_repeated_point_is_single_point(oracle_mod):
+    """A set holding one k-point twice with weights 0.3 and 0.7 and the same
+    bands at both is that k-point alone (the weights sum, every part)."""
+    s = cf.config(16)
+    k = np.asarray(s.kpoint)
+    o1 = oracle_mod.Oracle(s)
+    o2 = oracle_mod.Oracle(s, kpoints=[k, k], weights=[0.3, 0.7])
+    c = _kcoeffs(o1, 2700, s.nst)
+    r1 = o1.evaluate(c, hpsi=True, forces=True, stress=True)
+    r2 = o2.evaluate([c, c], hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        assert abs(r2[t] - r1[t]) <= 1e-11 * max(1.0, abs(r1[t])), t
+    assert np.abs(r2["fion"] - r1["fion"]).max() < 1e-11
+    assert np.abs(r2["sigma"] - r1["sigma"]).max() < 1e-11 * np.abs(r1["sigma"]).max()
+    for h in r2["hpsi"]:
+        assert np.abs(h - r1["hpsi"]).max() < 1e-12 * np.abs(r1["hpsi"]).max()
+
+
+@pytest.mark.parametrize("idx", [11, 13])
+def test_kset_is_doubled_cell(oracle_mod, idx):
+    """The set {0, b0/2} with weights 1/2 samples the zone of the cell doubled
+    along a0: the doubled cell's Gamma bands (the images of both k-points'
+    bands) have twice every energy term, the same force on each atom and its
+    copy, the same stress, and the images of both k-points' H psi."""
+    base = cf.config(idx)
+    nst = 3
+    p = dataclasses.replace(base, nst=nst, occ_bands=np.ones(nst), kpoint=None)
+    kps = [np.zeros(3), np.array([0.5, 0.0, 0.0])]
+    o_p = oracle_mod.Oracle(p, kpoints=kps, weights=[0.5, 0.5])
+    sc = dataclasses.replace(cf.supercell_x2(p), nst=2 * nst, occ_bands=np.ones(2 * nst))
+    grid_s = (2 * o_p.np0, o_p.np1, o_p.np2)
+    o_s = oracle_mod.Oracle(sc, grid=grid_s, force_complex=True)
+    cp = _kset_coeffs(o_p, 2, nst, 2800 + idx)
+    pos = _index(o_s.basis(0)["idx"])
+    c_s = np.zeros((2 * nst, o_s.ngw), dtype=complex)
+    imgs = []
+    for ik in range(2):
+        idx_k = o_p.kset_basis(ik)["idx"]
+        img = np.array([pos[(2 * int(h) + ik, int(k), int(l))] for h, k, l in idx_k])
+        c_s[ik * nst:(ik + 1) * nst][:, img] = cp[ik]
+        imgs.append(img)
+    r_p = o_p.evaluate(cp, hpsi=True, forces=True, stress=True)
+    r_s = o_s.evaluate(c_s, hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        print(f"  {t} 2 x cell {2 * r_p[t]:.12e} doubled cell {r_s[t]:.12e}")
+        assert abs(r_s[t] - 2.0 * r_p[t]) <= 1e-10 * max(1.0, abs(r_p[t])), t
+    assert np.abs(r_s["sigma"] - r_p["sigma"]).max() < 1e-10 * np.abs(r_p["sigma"]).max()
+    fp, fs = r_p["fion"], r_s["fion"]
+    off_p = off_s = 0
+    for n in p.na:
+        for copy in range(2):
+            assert np.abs(fs[off_s + copy * n: off_s + (copy + 1) * n] -
+                          fp[off_p:off_p + n]).max() < 1e-9
+        off_p += n
+        off_s += 2 * n
+    hs = r_s["hpsi"]
+    for ik in range(2):
+        hp = r_p["hpsi"][ik]
+        assert np.abs(hs[ik * nst:(ik + 1) * nst][:, imgs[ik]] - hp).max() < 1e-10 * np.abs(hp).max()
