@@ -263,6 +263,25 @@ int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, dou
 int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
                     const double* kpoint3, int nsp, const qbg_species_desc* species,
                     const int* na, const double* tau, int nst, const double* occ, qbg_ef** out);
+/* A Wavefunction with nkp k-points (src/Wavefunction.cpp:400-456; nkp = 1
+ * is qbg_ef_create_k): kpoints3n[3*nkp] in units of b0,b1,b2, weights[nkp]
+ * (summing to 1, Wavefunction::weight).  Every k-point takes the complex
+ * basis (the Gamma point of a set too).  k-point ik's bands are the columns
+ * [ik*nst, (ik+1)*nst) of c and H*psi, with one leading dimension of at
+ * least the largest basis (qbg_ef_kset_basis); the density sums the
+ * k-points with their weights, and so do ekin, enl, the forces and the
+ * stress (src/EnergyFunctional.cpp:446-543, 703-716; ChargeDensity.cpp:
+ * 136-160).  Sets run in FP64 or FP32 with nspin 1 (otherwise
+ * QBG_E_INVAL).  Rows of a k-point's H*psi columns past its basis size are
+ * not written. */
+int qbg_ef_create_kset(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
+                       int nkp, const double* kpoints3n, const double* weights, int nsp,
+                       const qbg_species_desc* species, const int* na, const double* tau,
+                       int nst, const double* occ, qbg_ef** out);
+/* k-point ik of a functional's k-points (0 for one k-point): the basis size
+ * ngw and, for non-null pointers, idx[3*ngw], |k+G|^2[ngw], k+G[3*ngw]. */
+int qbg_ef_kset_basis(const qbg_ef* ef, int ik, int* ngw, int32_t* idx, double* kpg2,
+                      double* kpgx);
 int qbg_ef_destroy(qbg_ef* ef);
 /* out[10]: ngw, ngv, np0, np1, np2, nvec_w, nvec_v, nat, nproj, nst */
 int qbg_ef_sizes(const qbg_ef* ef, int* out);

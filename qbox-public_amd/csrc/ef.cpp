@@ -73,10 +73,14 @@ static void spin_bands(const qbg_ef* e, int s, int& b0, int& nb, int& pair0) {
 static std::vector<double> density_factors(const qbg_ef* e) {
   std::vector<double> f;
   if (e->cplx) {
-    // one complex band per pair: both real parts carry its occupation
-    for (int n = 0; n < e->nst; n++) {
-      f.push_back(e->occ[n] / e->omega);
-      f.push_back(e->occ[n] / e->omega);
+    // one complex band per pair: both real parts carry its occupation (times
+    // its k-point's weight in a set, k-point by k-point)
+    for (int ik = 0; ik < e->nkp; ik++) {
+      const double w = e->nkp > 1 ? e->kset[ik].weight : 1.0;
+      for (int n = 0; n < e->nst; n++) {
+        f.push_back(w * e->occ[n] / e->omega);
+        f.push_back(w * e->occ[n] / e->omega);
+      }
     }
     return f;
   }
@@ -96,6 +100,21 @@ static std::vector<double> occ_half(const std::vector<double>& occ) {
   return h;
 }
 
+// a k-point set: exchange k-point k's basis and tables with the members
+static void kswap(qbg_ef* e, qbg_ef::KPoint& k) {
+  std::swap(e->wb, k.wb);
+  std::swap(e->d_g2w, k.d_g2w);
+  std::swap(e->d_gxw, k.d_gxw);
+  std::swap(e->d_twnl, k.d_twnl);
+  std::swap(e->d_dtwnl, k.d_dtwnl);
+  std::swap(e->d_occ, k.d_occ);
+  std::swap(e->d_occh, k.d_occh);
+  std::swap(e->d_m_twnl, k.d_m_twnl);
+  std::swap(e->d_dt_off, k.d_dt_off);
+  std::swap(e->d_fold, k.d_fold);
+  std::swap(e->d_unfold, k.d_unfold);
+}
+
 static void ef_free(qbg_ef* e) {
   void* p[] = {e->d_g2w, e->d_gxw, e->d_idxv, e->d_g2iv, e->d_giv, e->d_gxv, e->d_vps, e->d_dvps,
                e->d_rhops, e->d_twnl, e->d_tau, e->d_b, e->d_sftab, e->d_sfac, e->d_vion,
@@ -112,6 +131,13 @@ static void ef_free(qbg_ef* e) {
                e->d_cpair, e->d_hpair, e->d_occh};
   for (void* q : p)
     if (q) (void)hipFree(q);
+  for (qbg_ef::KPoint& k : e->kset) {
+    void* kp[] = {k.d_g2w, k.d_gxw, k.d_twnl, k.d_dtwnl, k.d_occ, k.d_occh, k.d_m_twnl,
+                  k.d_fold, k.d_unfold, k.d_dt_off};
+    for (void* q : kp)
+      if (q) (void)hipFree(q);
+  }
+  if (e->d_kacc) (void)hipFree(e->d_kacc);
   if (e->h_res) (void)hipHostFree(e->h_res);
   if (e->gw) qbg_grid_destroy(e->gw);
   if (e->gwf) qbg_grid_destroy(e->gwf);
@@ -397,10 +423,12 @@ static bool kpoint_maps(const host::Basis& wb, const host::Basis& ub, std::vecto
 // k-point: kp null or 0 gives the real (Gamma) basis
 static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut,
                      const double* kp, int nsp, const qbg_species_desc* spd, const int* na,
-                     const double* tau, int nst, const double* occ, qbg_ef** out) {
+                     const double* tau, int nst, const double* occ, qbg_ef** out,
+                     int nkp = 1, const double* kwts = nullptr) {
   {
     if (!c || !cell9 || !spd || !na || !tau || !occ || !out || nsp <= 0 || nst <= 0 || !(ecut > 0))
       return QBG_E_INVAL;
+    if (nkp < 1 || (nkp > 1 && (!kp || !kwts))) return QBG_E_INVAL;
     *out = nullptr;
     QBG_HIP(c, hipSetDevice(c->device));
     qbg_ef* e = new (std::nothrow) qbg_ef;
@@ -462,8 +490,18 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
     if (refcell9) def.set(refcell9);
     e->defcell = def;
     e->has_ref = refcell9 != nullptr;
-    e->cplx = kp && (kp[0] != 0.0 || kp[1] != 0.0 || kp[2] != 0.0);
-    e->wb.build(def, e->cell, ecut, e->cplx ? kp : nullptr);
+    e->cplx = nkp > 1 || (kp && (kp[0] != 0.0 || kp[1] != 0.0 || kp[2] != 0.0));
+    e->nkp = nkp;
+    e->wb.build(def, e->cell, ecut, e->cplx ? kp : nullptr, nkp > 1);
+    if (nkp > 1) {
+      e->kset.resize(nkp);
+      for (int ik = 0; ik < nkp; ik++) {
+        e->kset[ik].weight = kwts[ik];
+        if (ik > 0) e->kset[ik].wb.build(def, e->cell, ecut, kp + 3 * ik, true);
+      }
+    }
+    e->ngw_max = e->wb.size;
+    for (int ik = 1; ik < nkp; ik++) e->ngw_max = std::max(e->ngw_max, e->kset[ik].wb.size);
     e->vb.build(def, e->cell, 4.0 * ecut);
     for (int i = 0; i < 3; i++) {
       int n = e->vb.np[i] + 2;
@@ -475,20 +513,27 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
     // of every |G| <= max over wb of |G| (norms in the defining cell)
     const host::Basis* band = &e->wb;
     std::vector<int32_t> fold, unfold;
+    std::vector<std::vector<int32_t>> kfold(nkp), kunfold(nkp);
     if (e->cplx) {
       double g2max = 0.0;
-      for (int j = 0; j < e->wb.size; j++) {
-        double t2 = 0.0;
-        for (int a = 0; a < 3; a++) {
-          const double t = (e->wb.idx[3 * j] * def.b[0][a] + e->wb.idx[3 * j + 1] * def.b[1][a]) +
-                           e->wb.idx[3 * j + 2] * def.b[2][a];
-          t2 += t * t;
+      for (int ik = 0; ik < nkp; ik++) {
+        const host::Basis& b = ik == 0 ? e->wb : e->kset[ik].wb;
+        for (int j = 0; j < b.size; j++) {
+          double t2 = 0.0;
+          for (int a = 0; a < 3; a++) {
+            const double t = (b.idx[3 * j] * def.b[0][a] + b.idx[3 * j + 1] * def.b[1][a]) +
+                             b.idx[3 * j + 2] * def.b[2][a];
+            t2 += t * t;
+          }
+          g2max = std::max(g2max, t2);
         }
-        g2max = std::max(g2max, t2);
       }
       e->ub.build(def, e->cell, 0.5 * (g2max * (1.0 + 1e-9) + 1e-12));
       band = &e->ub;
-      if (!kpoint_maps(e->wb, e->ub, fold, unfold)) {
+      bool ok = kpoint_maps(e->wb, e->ub, fold, unfold);
+      for (int ik = 1; ik < nkp && ok; ik++)
+        ok = kpoint_maps(e->kset[ik].wb, e->ub, kfold[ik], kunfold[ik]);
+      if (!ok) {
         delete e;
         return c->fail(QBG_E_INVAL, "k-point basis: a vector outside the band basis");
       }
@@ -507,7 +552,7 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
       delete e;
       return rc;
     }
-    const int ngw = e->wb.size, ngv = e->vb.size;
+    const int ngw = e->wb.size, ngv = e->vb.size, ngwm = e->ngw_max;
     // cell_moved tables (src/EnergyFunctional.cpp:1200-1245)
     std::vector<double> vps, dvps, rhops, twnl, dtwnl, rhoc, drhoc;
     ef_local_tables(e, vps, dvps, rhops, rhoc, drhoc);
@@ -605,17 +650,21 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
     e->partial_doubles = std::max<size_t>(nbmax * 18, (size_t)e->nat * 64 * 3 + 16);
     e->partial_doubles = std::max<size_t>(e->partial_doubles, ((size_t)e->M * nst + 255) / 256 + 16);
     e->partial_doubles = std::max<size_t>(e->partial_doubles,
-                                          (size_t)((ngw + 255) / 256) * ((nst + 31) / 32) * 8);
+                                          (size_t)((ngwm + 255) / 256) * ((nst + 31) / 32) * 8);
     e->partial_doubles = std::max<size_t>(e->partial_doubles, (size_t)nsp * 64 * 6 + 16);
     // split-K partials of the fnl product: up to 40 splits of M x nst
     e->work_doubles = std::max<size_t>((size_t)1 << 22, (size_t)40 * e->M * nst);
+    // occupations times the weight of k-point 0 (of a set; 1 otherwise)
+    std::vector<double> occ_w(e->occ);
+    if (nkp > 1)
+      for (double& x : occ_w) x *= e->kset[0].weight;
     if ((rc = dput(c, &e->d_g2w, e->wb.kpg2)) || (rc = dput(c, &e->d_gxw, e->wb.kpgx)) ||
         (rc = dput(c, &e->d_idxv, e->vb.idx)) || (rc = dput(c, &e->d_g2iv, g2iv)) ||
         (rc = dput(c, &e->d_giv, giv)) || (rc = dput(c, &e->d_gxv, e->vb.gx)) ||
         (rc = dput(c, &e->d_vps, vps)) || (rc = dput(c, &e->d_dvps, dvps)) ||
         (rc = dput(c, &e->d_rhops, rhops)) || (rc = dput(c, &e->d_twnl, twnl)) ||
         (rc = dput(c, &e->d_b, bm)) || (rc = dput(c, &e->d_sp_first, spf)) ||
-        (rc = dput(c, &e->d_atom_sp, atom_sp)) || (rc = dput(c, &e->d_occ, e->occ)) ||
+        (rc = dput(c, &e->d_atom_sp, atom_sp)) || (rc = dput(c, &e->d_occ, occ_w)) ||
         (rc = dput(c, &e->d_m_atom, m_atom)) || (rc = dput(c, &e->d_m_twnl, m_twnl)) ||
         (rc = dput(c, &e->d_m_l, m_l)) || (rc = dput(c, &e->d_m_ipr, m_ipr)) ||
         (rc = dput(c, &e->d_m_npr, m_npr)) || (rc = dput(c, &e->d_m_stride, m_stride)) ||
@@ -635,7 +684,7 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
         (rc = dalloc(c, &e->d_rhogt, (size_t)2 * ngv)) || (rc = dalloc(c, &e->d_vlocal, (size_t)2 * ngv)) ||
         (rc = dalloc(c, &e->d_fesr, 3 * e->nat)) || (rc = dalloc(c, &e->d_fion, 3 * e->nat)) ||
         (rc = dalloc(c, &e->d_scal, S_COUNT)) || (rc = dalloc(c, &e->d_partial, e->partial_doubles)) ||
-        (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngw)) ||
+        (rc = dalloc(c, &e->d_anl, (size_t)2 * e->M * 2 * ngwm)) ||
         (rc = dalloc(c, &e->d_fnl, (size_t)2 * e->M * nst)) ||
         (rc = dalloc(c, &e->d_fout, (size_t)e->M * nst)) ||
         (rc = dalloc(c, &e->d_work, e->work_doubles)) ||
@@ -645,12 +694,48 @@ static int ef_create(qbg_ctx* c, const double* cell9, const double* refcell9, do
                      (rc = dalloc(c, &e->d_vxcg, (size_t)2 * ngv)))) ||
         (e->cplx && ((rc = dput(c, &e->d_fold, fold)) || (rc = dput(c, &e->d_unfold, unfold)) ||
                      (rc = dput(c, &e->d_a_m0h, a_m0h)) ||
-                     (rc = dput(c, &e->d_occh, occ_half(e->occ))) ||
-                     (rc = dalloc(c, &e->d_cpair, (size_t)4 * e->ub.size * nst)) ||
-                     (rc = dalloc(c, &e->d_hpair, (size_t)4 * e->ub.size * nst))))) {
+                     (rc = dput(c, &e->d_occh, occ_half(occ_w))) ||
+                     (rc = dalloc(c, &e->d_cpair, (size_t)4 * e->ub.size * nst * nkp)) ||
+                     (rc = dalloc(c, &e->d_hpair, (size_t)4 * e->ub.size * nst * nkp)))) ||
+        (nkp > 1 && (rc = dalloc(c, &e->d_kacc, (size_t)16 * nkp)))) {
       ef_free(e);
       delete e;
       return rc;
+    }
+    e->o_kin = e->d_scal + S_KIN;
+    e->o_enl = e->d_scal + S_ENL;
+    e->o_nls = e->d_scal + S_NLS;
+    // the further k-points of a set: their tables (k + G, twnl, the twnl
+    // offsets of the projector columns, fold maps, weighted occupations)
+    for (int ik = 1; ik < nkp; ik++) {
+      qbg_ef::KPoint& k = e->kset[ik];
+      const int ngk = k.wb.size;
+      std::swap(e->wb, k.wb);
+      std::vector<double> tk, dtk;
+      ef_twnl(e, tk, dtk);
+      std::swap(e->wb, k.wb);
+      std::vector<int32_t> mt;
+      std::vector<int64_t> dto(nsp);
+      for (int is = 0, tw = 0; is < nsp; is++) {
+        const int npr = e->sp[is].nlm;
+        dto[is] = (int64_t)6 * tw;
+        for (int ipr = 0; ipr < npr; ipr++)
+          for (int ia = 0; ia < e->na[is]; ia++) mt.push_back(tw + ipr * ngk);
+        tw += npr * ngk;
+      }
+      const size_t mh = mt.size();
+      for (size_t m = 0; m < mh; m++) mt.push_back(mt[m]);   // the i anl half
+      std::vector<double> ow(e->occ);
+      for (double& x : ow) x *= k.weight;
+      if ((rc = dput(c, &k.d_g2w, k.wb.kpg2)) || (rc = dput(c, &k.d_gxw, k.wb.kpgx)) ||
+          (rc = dput(c, &k.d_twnl, tk)) || (rc = dput(c, &k.d_dtwnl, dtk)) ||
+          (rc = dput(c, &k.d_m_twnl, mt)) || (rc = dput(c, &k.d_dt_off, dto)) ||
+          (rc = dput(c, &k.d_fold, kfold[ik])) || (rc = dput(c, &k.d_unfold, kunfold[ik])) ||
+          (rc = dput(c, &k.d_occ, ow)) || (rc = dput(c, &k.d_occh, occ_half(ow)))) {
+        ef_free(e);
+        delete e;
+        return rc;
+      }
     }
     // density factors and result staging
     {
@@ -701,6 +786,19 @@ int qbg_ef_create_k(qbg_ctx* c, const double* cell9, const double* refcell9, dou
   });
 }
 
+int qbg_ef_create_kset(qbg_ctx* c, const double* cell9, const double* refcell9, double ecut,
+                       int nkp, const double* kpoints3n, const double* weights, int nsp,
+                       const qbg_species_desc* spd, const int* na, const double* tau, int nst,
+                       const double* occ, qbg_ef** out) {
+  return qbg::guarded(c, [&]() -> int {
+    if (!kpoints3n || !weights || nkp < 1) return QBG_E_INVAL;
+    if (nkp == 1)
+      return ef_create(c, cell9, refcell9, ecut, kpoints3n, nsp, spd, na, tau, nst, occ, out);
+    return ef_create(c, cell9, refcell9, ecut, kpoints3n, nsp, spd, na, tau, nst, occ, out, nkp,
+                     weights);
+  });
+}
+
 int qbg_ef_destroy(qbg_ef* e) {
   return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
     if (!e) return QBG_E_INVAL;
@@ -741,6 +839,21 @@ int qbg_ef_basis(const qbg_ef* e, int which, int32_t* idx, double* g2, double* g
     if (ip) std::memcpy(ip, m.ip.data(), sizeof(int32_t) * m.ip.size());
     if (im) std::memcpy(im, m.im.data(), sizeof(int32_t) * m.im.size());
     if (zvec_to_val) std::memcpy(zvec_to_val, m.zvec_to_val.data(), sizeof(int32_t) * m.nvec);
+    return QBG_OK;
+  });
+}
+
+// k-point ik of a k-point set (ik = 0 for a single k-point): its basis size
+// and, where the pointers are given, idx[3 ngw], |k+G|^2, k+G[3 ngw]
+int qbg_ef_kset_basis(const qbg_ef* e, int ik, int* ngw, int32_t* idx, double* kpg2,
+                      double* kpgx) {
+  return qbg::guarded((e ? e->ctx : nullptr), [&]() -> int {
+    if (!e || ik < 0 || ik >= e->nkp) return QBG_E_INVAL;
+    const host::Basis& b = ik == 0 ? e->wb : e->kset[ik].wb;
+    if (ngw) *ngw = b.size;
+    if (idx) std::memcpy(idx, b.idx.data(), sizeof(int32_t) * b.idx.size());
+    if (kpg2) std::memcpy(kpg2, b.kpg2.data(), sizeof(double) * b.size);
+    if (kpgx) std::memcpy(kpgx, b.kpgx.data(), sizeof(double) * b.kpgx.size());
     return QBG_OK;
   });
 }
@@ -815,7 +928,7 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
 // resident array when it fits (keep), for an H*psi of the same bands
 // (ef_launch with reuse); the density itself is the same either way.
 static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool keep) {
-  if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
+  if (!e || !c || ldc < e->ngw_max) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
   QBG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
@@ -831,16 +944,26 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
     pairs = p0 + (nb + 1) / 2;
   }
   // (at k != 0 every band is one pair of the band grid)
-  const bool res = keep && grid_ensure_xres(e->gw, e->cplx ? e->nst : pairs);
+  const bool res = keep && grid_ensure_xres(e->gw, e->cplx ? e->nst * e->nkp : pairs);
   int rc = QBG_OK;
   if (e->cplx) {
     // each complex band as the pair of its real and imaginary parts on the
     // real band basis, both with the band's occupation; one density per spin
-    const int nu = e->ub.size;
+    const int nu = e->ub.size, nst = e->nst;
     ctx->prof_begin(QBG_K_OTHER);
-    launch_fold(c, ldc, e->nst, nu, e->d_fold, e->d_cpair, nu, s);
+    for (int ik = 0; ik < e->nkp; ik++) {
+      if (ik > 0) kswap(e, e->kset[ik]);
+      launch_fold(c + 2L * ldc * nst * ik, ldc, nst, nu, e->d_fold, e->d_cpair + 4L * nu * nst * ik,
+                  nu, s);
+      if (ik > 0) kswap(e, e->kset[ik]);
+    }
     ctx->prof_end();
-    for (int sp = 0; sp < e->nspin && !rc; sp++) {
+    // a set: every k-point's pairs in one pass, each with its weighted
+    // occupations (src/ChargeDensity.cpp:136-160, nspin 1)
+    if (e->nkp > 1)
+      rc = density_batches(e->gw, e->d_cpair, nu, 2 * nst * e->nkp, e->d_dfac, e->d_rhor, fprec,
+                           res, 0);
+    for (int sp = 0; sp < e->nspin && !rc && e->nkp == 1; sp++) {
       int b0, nb, p0;
       spin_bands(e, sp, b0, nb, p0);
       if (nb > 0)
@@ -1002,7 +1125,7 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_begin(QBG_K_NL);
   launch_dmat_enl(e->d_fnl, M, nst, occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
-                  e->d_scal + S_ENL, s);
+                  e->o_enl, s);
   ctx->prof_end();
   if (compute_hpsi) {
     ctx->prof_begin(QBG_K_GEMM);
@@ -1017,7 +1140,7 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     double* W = e->d_fnl + (size_t)M * nst;
     void* Y = f32 ? (void*)(anl32 + (size_t)M * K) : (void*)(e->d_anl + (size_t)M * K);
     ctx->prof_begin(QBG_K_NL);
-    if (compute_forces)
+    if (compute_forces && !e->fion_keep)
       fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
     // the force kernel's factor 4 is 2 (|F|^2) x 2 (G and -G); the full
     // sphere of k != 0 takes half the occupation
@@ -1044,12 +1167,12 @@ static int nl_mfma(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
       if (f32)
         launch_nl_stress_y((const float*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                           e->d_m_l, e->d_partial, 64, e->o_nls, s,
                            e->cplx ? 2 : 1, e->Mh);
       else
         launch_nl_stress_y((const double*)Y, K, ngw, e->d_gxw, e->d_tau, e->nsp, e->d_sp_first,
                            e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0, e->d_a_stride,
-                           e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                           e->d_m_l, e->d_partial, 64, e->o_nls, s,
                            e->cplx ? 2 : 1, e->Mh);
     }
     ctx->prof_end();
@@ -1208,7 +1331,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
   ctx->prof_begin(QBG_K_NL);
   launch_dmat_enl(e->d_fnl, M, nst, occ, e->d_m_ipr, e->d_m_npr, e->d_m_stride,
                   e->d_m_dbase, e->d_dmat, e->d_m_wt, 1.0 / e->omega, e->d_fout, e->d_partial,
-                  e->d_scal + S_ENL, s);
+                  e->o_enl, s);
   const bool needy = compute_forces || compute_stress;
   // W = occ fout for the force product (half the occupation on the full
   // sphere of k != 0, see nl_mfma)
@@ -1232,7 +1355,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     ctx->prof_end();
     if (rc == 0) {
       ctx->prof_begin(QBG_K_NL);
-      fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
+      if (!e->fion_keep) fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
       for (int half = 0; half < (e->cplx ? 2 : 1); half++)
         launch_nl_forces_tiles(e->b_fpart, M, (int)(Kp / 256), e->b_fred,
                                half ? e->d_a_m0h : e->d_a_m0, e->d_a_npr, e->d_a_stride, e->nat,
@@ -1248,7 +1371,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (rc) return ctx->fail(QBG_E_INVAL, "bf16 gemm shape");
     ctx->prof_begin(QBG_K_NL);
     if (compute_forces) {
-      fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
+      if (!e->fion_keep) fill_zero(e->d_fion, 3 * (size_t)e->nat, s);
       for (int half = 0; half < (e->cplx ? 2 : 1); half++)
         launch_nl_forces_y(e->b_anl, (const float*)e->b_y32, Kp, ngw, e->d_gxw,
                            half ? e->d_a_m0h : e->d_a_m0, e->d_a_npr, e->d_a_stride, e->nat,
@@ -1257,7 +1380,7 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
     if (compute_stress)
       launch_nl_stress_y((const float*)e->b_y32, Kp, ngw, e->d_gxw, e->d_tau, e->nsp,
                          e->d_sp_first, e->d_sp_npr, e->d_dt_off, e->d_dtwnl, e->d_a_m0,
-                         e->d_a_stride, e->d_m_l, e->d_partial, 64, e->d_scal + S_NLS, s,
+                         e->d_a_stride, e->d_m_l, e->d_partial, 64, e->o_nls, s,
                          e->cplx ? 2 : 1, e->Mh);
     ctx->prof_end();
   }
@@ -1273,10 +1396,12 @@ static int nl_bf16(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double
 static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, double* hpsi,
                      int ldh, int compute_forces, int compute_stress, int prec,
                      bool reuse = false) {
-  if (!e || !c || ldc < e->wb.size) return QBG_E_INVAL;
+  if (!e || !c || ldc < e->ngw_max) return QBG_E_INVAL;
   qbg_ctx* ctx = e->ctx;
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
-  if (compute_hpsi && (!hpsi || ldh < e->wb.size)) return QBG_E_INVAL;
+  if (compute_hpsi && (!hpsi || ldh < e->ngw_max)) return QBG_E_INVAL;
+  if (e->nkp > 1 && (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2))
+    return ctx->fail(QBG_E_INVAL, "energy of a k-point set: FP64 or FP32");
   if (compute_stress && !e->vhxc_stress)
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
@@ -1295,23 +1420,47 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   // 0.5 kpg2 c + the nonlocal part (FP32, from the NN product) + the local
   // part, so neither the kinetic pass nor the NN product touches H*psi
   const bool fin = split && compute_hpsi && hpsi != c && !e->cplx && zfwd_fin_ok(e->gw);
-  ctx->prof_begin(QBG_K_OTHER);
-  // (the half sphere's G and -G times 1/2 is occ |c|^2 |G|^2; the full sphere
-  // of k != 0 takes occ/2 |c|^2 |k+G|^2)
-  launch_kinetic(c, ldc, nst, e->cplx ? e->d_occh : e->d_occ, ngw, e->d_g2w, e->d_gxw, hpsi, ldh,
-                 compute_hpsi && !fin, e->d_partial, e->d_scal + S_KIN, s,
-                 split ? e->b_c32 : nullptr, split ? e->Kp : 0);
-  ctx->prof_end();
-  // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
-  // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
-  // the GEMMs run on FP32 MFMA; in the BF16 modes on the BF16-split path
-  const long K = 2L * ngw;
-  int nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
-                ? nl_bf16(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec == QBG_PREC_BF16X3 ? 3 : 2, true, 0, nst,
-                          fin ? e->b_yh : nullptr)
-                : nl_mfma(e, c, ldc, compute_hpsi, hpsi, ldh, compute_forces, compute_stress,
-                          prec, 0, nst);
+  // the kinetic and nonlocal parts of each k-point (one pass for one
+  // k-point; a set sums each k-point's weighted parts, src/EnergyFunctional.cpp:446-543, 703-716)
+  const int nk = e->nkp;
+  if (nk > 1) fill_zero(e->d_kacc, (size_t)16 * nk, s);
+  int nrc = QBG_OK;
+  for (int ik = 0; ik < nk && !nrc; ik++) {
+    if (ik > 0) kswap(e, e->kset[ik]);
+    if (nk > 1) {
+      e->o_kin = e->d_kacc + 16 * ik;
+      e->o_enl = e->d_kacc + 16 * ik + 8;
+      e->o_nls = e->d_kacc + 16 * ik + 9;
+      e->fion_keep = ik > 0;
+    }
+    const int ngk = e->wb.size;
+    const double* ck = c + 2L * ldc * nst * ik;
+    double* hk = compute_hpsi ? hpsi + 2L * ldh * nst * ik : hpsi;
+    ctx->prof_begin(QBG_K_OTHER);
+    // (the half sphere's G and -G times 1/2 is occ |c|^2 |G|^2; the full
+    // sphere of k != 0 takes occ/2 |c|^2 |k+G|^2)
+    launch_kinetic(ck, ldc, nst, e->cplx ? e->d_occh : e->d_occ, ngk, e->d_g2w, e->d_gxw, hk,
+                   ldh, compute_hpsi && !fin, e->d_partial, e->o_kin, s,
+                   split ? e->b_c32 : nullptr, split ? e->Kp : 0);
+    ctx->prof_end();
+    // nonlocal: anl, fnl = 2 anl^T c - anl0 c0^T (FP64 partial sums), the D
+    // contraction and enl in FP64; in FP32 mode anl and Y are FP32 arrays and
+    // the GEMMs run on FP32 MFMA; in the BF16 modes on the BF16-split path
+    nrc = (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2)
+              ? nl_bf16(e, ck, ldc, compute_hpsi, hk, ldh, compute_forces, compute_stress,
+                        prec == QBG_PREC_BF16X3 ? 3 : 2, true, 0, nst,
+                        fin ? e->b_yh : nullptr)
+              : nl_mfma(e, ck, ldc, compute_hpsi, hk, ldh, compute_forces, compute_stress,
+                        prec, 0, nst);
+    if (ik > 0) kswap(e, e->kset[ik]);
+  }
+  if (nk > 1) {
+    e->o_kin = e->d_scal + S_KIN;
+    e->o_enl = e->d_scal + S_ENL;
+    e->o_nls = e->d_scal + S_NLS;
+    e->fion_keep = false;
+    launch_kacc_sum(e->d_kacc, nk, e->d_scal + S_KIN, e->d_scal + S_ENL, e->d_scal + S_NLS, s);
+  }
   if (nrc) return nrc;
   if (compute_stress) {
     ctx->prof_begin(QBG_K_OTHER);
@@ -1328,25 +1477,48 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     // fold and full transforms here
     const bool kept = reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec;
     ctx->prof_begin(QBG_K_OTHER);
-    if (!kept) launch_fold(c, ldc, nst, nu, e->d_fold, e->d_cpair, nu, s);
-    fill_zero(e->d_hpair, (size_t)4 * nu * nst, s);
-    ctx->prof_end();
-    // each band with its spin's potential (src/EnergyFunctional.cpp:831)
-    for (int sp = 0; sp < e->nspin; sp++) {
-      int b0, nb, p0;
-      spin_bands(e, sp, b0, nb, p0);
-      if (nb == 0) continue;
-      const double* v = sp == 0 ? e->d_vr : e->d_vr2;
-      double* hb = e->d_hpair + 4L * nu * b0;
-      const int rc = kept ? hpsi_resident(e->gw, v, 2 * nb, hb, nu, fprec, b0, nullptr)
-                          : rs_mul_add_bands(e->gw, v, e->d_cpair + 4L * nu * b0, nu, 2 * nb, hb,
-                                             nu, fprec, nullptr);
-      if (rc) return rc;
+    for (int ik = 0; ik < e->nkp && !kept; ik++) {
+      if (ik > 0) kswap(e, e->kset[ik]);
+      launch_fold(c + 2L * ldc * nst * ik, ldc, nst, nu, e->d_fold, e->d_cpair + 4L * nu * nst * ik,
+                  nu, s);
+      if (ik > 0) kswap(e, e->kset[ik]);
     }
-    if (kept) e->xres_c = nullptr;
-    ctx->prof_begin(QBG_K_OTHER);
-    launch_unfold_add(e->d_hpair, nu, nst, ngw, e->d_unfold, hpsi, ldh, s);
+    fill_zero(e->d_hpair, (size_t)4 * nu * nst * e->nkp, s);
     ctx->prof_end();
+    if (e->nkp > 1) {
+      // every k-point's pairs through one batched pass (nspin 1)
+      const int rc = kept ? hpsi_resident(e->gw, e->d_vr, 2 * nst * e->nkp, e->d_hpair, nu, fprec,
+                                          0, nullptr)
+                          : rs_mul_add_bands(e->gw, e->d_vr, e->d_cpair, nu, 2 * nst * e->nkp,
+                                             e->d_hpair, nu, fprec, nullptr);
+      if (rc) return rc;
+      ctx->prof_begin(QBG_K_OTHER);
+      for (int ik = 0; ik < e->nkp; ik++) {
+        if (ik > 0) kswap(e, e->kset[ik]);
+        launch_unfold_add(e->d_hpair + 4L * nu * nst * ik, nu, nst, e->wb.size, e->d_unfold,
+                          hpsi + 2L * ldh * nst * ik, ldh, s);
+        if (ik > 0) kswap(e, e->kset[ik]);
+      }
+      ctx->prof_end();
+      if (kept) e->xres_c = nullptr;
+    } else {
+      // each band with its spin's potential (src/EnergyFunctional.cpp:831)
+      for (int sp = 0; sp < e->nspin; sp++) {
+        int b0, nb, p0;
+        spin_bands(e, sp, b0, nb, p0);
+        if (nb == 0) continue;
+        const double* v = sp == 0 ? e->d_vr : e->d_vr2;
+        double* hb = e->d_hpair + 4L * nu * b0;
+        const int rc = kept ? hpsi_resident(e->gw, v, 2 * nb, hb, nu, fprec, b0, nullptr)
+                            : rs_mul_add_bands(e->gw, v, e->d_cpair + 4L * nu * b0, nu, 2 * nb, hb,
+                                               nu, fprec, nullptr);
+        if (rc) return rc;
+      }
+      if (kept) e->xres_c = nullptr;
+      ctx->prof_begin(QBG_K_OTHER);
+      launch_unfold_add(e->d_hpair, nu, nst, ngw, e->d_unfold, hpsi, ldh, s);
+      ctx->prof_end();
+    }
   } else if (compute_hpsi) {
     // with the potential of each band's spin (src/EnergyFunctional.cpp:831)
     const bool kept = reuse && e->xres_c == c && e->xres_ldc == ldc && e->xres_prec == fprec;
@@ -1676,6 +1848,7 @@ int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
     if (nspin == 1) nst_up = 0;
+    if (nspin == 2 && e->nkp > 1) return c->fail(QBG_E_INVAL, "set_nspin: nspin 2 of a k-point set");
     if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
       return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
     QBG_HIP(c, hipSetDevice(c->device));

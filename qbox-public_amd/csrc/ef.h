@@ -119,6 +119,28 @@ struct qbg_ef {
   // projectors are doubled, M = 2 Mh: columns Mh.. are i times columns 0..
   // (phase (-i)^(l+3)), a_m0h = a_m0 + Mh
   bool cplx = false;
+  // a k-point set (qbg_ef_create_kset, nkp > 1, every k-point complex):
+  // k-point ik's bands are the columns [ik nst, (ik+1) nst) of c and H*psi;
+  // wb and the per-k-point tables (d_g2w, d_gxw, d_twnl, d_dtwnl, d_occ,
+  // d_occh, d_m_twnl, d_dt_off, d_fold, d_unfold; occupations times the
+  // weight) hold k-point 0 between calls and kswap exchanges them with
+  // kset[ik]; kacc: each k-point's kinetic (8), enl (1) and nonlocal stress
+  // (6) sums, added into the scalars after the loop; o_kin, o_enl, o_nls:
+  // where the kinetic and nonlocal passes put those sums (the scalars for
+  // one k-point); fion_keep: the nonlocal force pass adds to fion
+  struct KPoint {
+    qbg::host::Basis wb;
+    double weight = 1.0;
+    double *d_g2w = nullptr, *d_gxw = nullptr, *d_twnl = nullptr, *d_dtwnl = nullptr,
+           *d_occ = nullptr, *d_occh = nullptr;
+    int32_t *d_m_twnl = nullptr, *d_fold = nullptr, *d_unfold = nullptr;
+    int64_t* d_dt_off = nullptr;
+  };
+  std::vector<KPoint> kset;
+  int nkp = 1, ngw_max = 0;
+  double* d_kacc = nullptr;
+  double *o_kin = nullptr, *o_enl = nullptr, *o_nls = nullptr;
+  bool fion_keep = false;
   qbg::host::Basis ub;
   int Mh = 0;
   int32_t *d_fold = nullptr, *d_unfold = nullptr, *d_a_m0h = nullptr;

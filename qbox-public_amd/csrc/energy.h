@@ -65,6 +65,10 @@ void launch_fold(const double* c, long ldc, int nst, int ngwu, const int32_t* fo
                  long ldu, hipStream_t s);
 void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_t* unfold,
                        double* hpsi, long ldh, hipStream_t s);
+// a k-point set's scalars: kin[0..7], enl[0], nls[0..5] = the sums over the
+// k-points of kacc[16 ik + {0..7, 8, 9..14}]
+void launch_kacc_sum(const double* kacc, int nkp, double* kin, double* enl, double* nls,
+                     hipStream_t s);
 template <typename T>
 void launch_nl_forces_y(const T* A, const T* Y, long lda, int ngw, const double* gx,
                         const int32_t* a_m0, const int32_t* a_npr, const int32_t* a_stride,

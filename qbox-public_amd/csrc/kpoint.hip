@@ -67,6 +67,24 @@ void launch_fold(const double* c, long ldc, int nst, int ngwu, const int32_t* fo
                      out, ldu);
 }
 
+// the scalars of a k-point set: S_KIN (8), S_ENL (1), S_NLS (6) = the sums
+// over the k-points of their kacc slots [16 ik + {0..7, 8, 9..14}]
+__global__ void k_kacc_sum(const double* __restrict__ kacc, int nkp, double* __restrict__ kin,
+                           double* __restrict__ enl, double* __restrict__ nls) {
+  const int t = threadIdx.x;
+  if (t >= 15) return;
+  double v = 0.0;
+  for (int k = 0; k < nkp; k++) v += kacc[16 * k + t];
+  if (t < 8) kin[t] = v;
+  else if (t == 8) enl[0] = v;
+  else nls[t - 9] = v;
+}
+
+void launch_kacc_sum(const double* kacc, int nkp, double* kin, double* enl, double* nls,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_kacc_sum, dim3(1), dim3(64), 0, s, kacc, nkp, kin, enl, nls);
+}
+
 void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_t* unfold,
                        double* hpsi, long ldh, hipStream_t s) {
   if (nst <= 0 || ngw <= 0) return;

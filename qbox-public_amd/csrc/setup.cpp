@@ -81,9 +81,10 @@ bool factorizable(int n) {
 
 // Rods of the Gamma half sphere: (0,0) first, then (0,k>0), then (h>0,k);
 // stable by size with rod (0,0) exchanged into slot 0 afterwards.
-void Basis::build(const Cell& def, const Cell& cell, double ecut, const double* kp) {
+void Basis::build(const Cell& def, const Cell& cell, double ecut, const double* kp,
+                  bool force_complex) {
   for (int i = 0; i < 3; i++) kpoint[i] = kp ? kp[i] : 0.0;
-  real = kpoint[0] == 0.0 && kpoint[1] == 0.0 && kpoint[2] == 0.0;
+  real = !force_complex && kpoint[0] == 0.0 && kpoint[1] == 0.0 && kpoint[2] == 0.0;
   const double two_ecut = 2.0 * ecut;
   const double fac = std::sqrt(two_ecut) / (2.0 * M_PI);
   int nmax[3];

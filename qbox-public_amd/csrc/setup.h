@@ -30,7 +30,9 @@ struct Basis {
   std::vector<int32_t> idx;                     // 3*size
   std::vector<double> g, g2, gi, g2i, gx;       // gx[j*size + i]
   std::vector<double> kpg, kpg2, kpgi, kpgx;    // k + G (equal to g at k = 0)
-  void build(const Cell& defcell, const Cell& cell, double ecut, const double* kpoint = nullptr);
+  // force_complex: the complex branch at k = 0 too (a k-point set's Gamma point)
+  void build(const Cell& defcell, const Cell& cell, double ecut, const double* kpoint = nullptr,
+             bool force_complex = false);
 };
 
 // BasisMapping maps for one task, real basis (src/BasisMapping.cpp:38-228)

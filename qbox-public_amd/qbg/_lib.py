@@ -67,6 +67,9 @@ SIGNATURES = {
                       C.POINTER(_vp)],
     "qbg_ef_create_k": [_vp, _dp, _dp, C.c_double, _dp, C.c_int, _vp, _ip, _dp, C.c_int, _dp,
                         C.POINTER(_vp)],
+    "qbg_ef_create_kset": [_vp, _dp, _dp, C.c_double, C.c_int, _dp, _dp, C.c_int, _vp, _ip, _dp,
+                           C.c_int, _dp, C.POINTER(_vp)],
+    "qbg_ef_kset_basis": [_vp, C.c_int, _ip, _ip, _dp, _dp],
     "qbg_ef_destroy": [_vp],
     "qbg_ef_sizes": [_vp, _ip],
     "qbg_ef_basis": [_vp, C.c_int, _ip, _dp, _dp, _ip, _ip, _ip],
@@ -468,12 +471,16 @@ class EnergyFunctional:
     """qbg_ef: one system (Gamma, or one k-point off Gamma) resident on the
     device."""
 
-    def __init__(self, ctx: Context, sysdef, refcell=None, kpoint=None):
+    def __init__(self, ctx: Context, sysdef, refcell=None, kpoint=None, kpoints=None,
+                 weights=None):
         """refcell (3x3, rows a0 a1 a2): Wavefunction's reference cell, which
         fixes the basis membership and the grids (src/Basis.cpp:314-325);
         sysdef.refcell when not given.  kpoint (3, units of the reciprocal
         vectors; sysdef.kpoint when not given): k != 0 makes the wavefunction
-        basis the complex full sphere (qbg_ef_create_k)."""
+        basis the complex full sphere (qbg_ef_create_k).  kpoints (nkp x 3)
+        and weights (nkp): a k-point set (qbg_ef_create_kset); k-point ik's
+        bands are then the rows [ik nst, (ik+1) nst) of the coefficient
+        array, whose width is ngw, the largest basis (kset_pack)."""
         self.ctx = ctx
         self.sysdef = sysdef
         if refcell is None:
@@ -499,7 +506,20 @@ class EnergyFunctional:
             self.kpoint = np.ascontiguousarray(kpoint, dtype=np.float64).ravel()
         # the complex (full-sphere) basis of a k-point off Gamma
         self.complex = self.kpoint is not None and bool(np.any(self.kpoint != 0.0))
-        if self.kpoint is not None:
+        self.nkp = 1
+        if kpoints is not None:
+            kps = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+            wts = np.ascontiguousarray(weights, dtype=np.float64).ravel()
+            self.nkp = kps.shape[0]
+            self.complex = True
+            self._keep += [kps, wts]
+            ctx.check(ctx.lib.qbg_ef_create_kset(ctx.h, _pd(cell),
+                                                 _pd(ref) if ref is not None else None,
+                                                 sysdef.ecut, self.nkp, _pd(kps), _pd(wts),
+                                                 len(sysdef.species), C.cast(descs, _vp),
+                                                 na.ctypes.data_as(_ip), _pd(tau), sysdef.nst,
+                                                 _pd(occ), C.byref(h)))
+        elif self.kpoint is not None:
             self._keep.append(self.kpoint)
             ctx.check(ctx.lib.qbg_ef_create_k(ctx.h, _pd(cell),
                                               _pd(ref) if ref is not None else None,
@@ -517,6 +537,12 @@ class EnergyFunctional:
         (self.ngw, self.ngv, self.np0, self.np1, self.np2, self.nvec, self.nvec_v, self.nat,
          self.nproj, self.nst) = [int(x) for x in s]
         self.N = self.np0 * self.np1 * self.np2
+        self.kngw = []
+        for ik in range(self.nkp):
+            n = C.c_int(0)
+            ctx.check(ctx.lib.qbg_ef_kset_basis(h, ik, C.byref(n), None, None, None))
+            self.kngw.append(int(n.value))
+        self.ngw = max(self.kngw)            # the coefficients' width
         ctx.adopt(self)
         if getattr(sysdef, "nspin", 1) == 2:
             self.set_nspin(2, sysdef.nst_up)
@@ -536,7 +562,7 @@ class EnergyFunctional:
         """which 0: wavefunction basis, 1: density basis, 2: the wavefunction
         basis's k + G (g2 = |k+G|^2, gx = k+G).  The band-grid maps (ip, im,
         zvec_to_val) of a k != 0 functional are None."""
-        n = self.ngv if which == 1 else self.ngw
+        n = self.ngv if which == 1 else self.kngw[0]
         nvec = self.nvec_v if which == 1 else self.nvec
         maps = which == 1 or not self.complex
         idx = np.zeros(3 * n, dtype=np.int32)
@@ -551,6 +577,27 @@ class EnergyFunctional:
             z2v.ctypes.data_as(_ip) if maps else None))
         return dict(idx=idx.reshape(n, 3), g2=g2, gx=gx.reshape(3, n), ip=ip if maps else None,
                     im=im if maps else None, zvec_to_val=z2v if maps else None)
+
+    def kset_basis(self, ik):
+        """k-point ik: idx (ngw x 3), |k+G|^2, k+G (3 x ngw)."""
+        n = self.kngw[ik]
+        idx = np.zeros(3 * n, dtype=np.int32)
+        kpg2 = np.zeros(n)
+        kpgx = np.zeros(3 * n)
+        self.ctx.check(self.ctx.lib.qbg_ef_kset_basis(self.h, ik, None, idx.ctypes.data_as(_ip),
+                                                      _pd(kpg2), _pd(kpgx)))
+        return dict(idx=idx.reshape(n, 3), kpg2=kpg2, kpgx=kpgx.reshape(3, n))
+
+    def kset_pack(self, cs):
+        """per-k-point (nst, ngw_k) arrays -> one (nkp nst, ngw) array, zero padded"""
+        out = np.zeros((self.nkp * self.nst, self.ngw), dtype=np.complex128)
+        for ik, c in enumerate(cs):
+            out[ik * self.nst:(ik + 1) * self.nst, :self.kngw[ik]] = c
+        return out
+
+    def kset_unpack(self, a):
+        return [np.ascontiguousarray(a[ik * self.nst:(ik + 1) * self.nst, :self.kngw[ik]])
+                for ik in range(self.nkp)]
 
     def set_positions(self, tau):
         tau = np.ascontiguousarray(tau, dtype=np.float64).ravel()
