@@ -271,7 +271,7 @@ int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, d
  * least the largest basis (qbg_ef_kset_basis); the density sums the
  * k-points with their weights, and so do ekin, enl, the forces and the
  * stress (src/EnergyFunctional.cpp:446-543, 703-716; ChargeDensity.cpp:
- * 136-160).  Sets run in FP64 or FP32 with nspin 1 (otherwise
+ * 136-160).  Sets run in every precision with nspin 1 (nspin 2:
  * QBG_E_INVAL).  Rows of a k-point's H*psi columns past its basis size are
  * not written. */
 int qbg_ef_create_kset(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,

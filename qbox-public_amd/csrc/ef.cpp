@@ -1193,7 +1193,7 @@ static int bf16_ensure(qbg_ef* e) {
   // padded to 128 are multiples of 256 already (no extra padded work);
   // otherwise 128
   const long edge = (up(e->M, 128) % 256 == 0 && up(e->nst, 128) % 256 == 0) ? 256 : 128;
-  e->Kp = up(2L * e->wb.size, edge);
+  e->Kp = up(2L * e->ngw_max, edge);   // (the largest basis of a k-point set)
   e->Mp = (int)up(e->M, edge);
   e->Np = (int)up(e->nst, edge);
   const int ntile = (e->Mp / 128) * (e->Np / 128);
@@ -1400,8 +1400,6 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
   qbg_ctx* ctx = e->ctx;
   if (!e->have_vhxc) return ctx->fail(QBG_E_STATE, "energy before update_vhxc");
   if (compute_hpsi && (!hpsi || ldh < e->ngw_max)) return QBG_E_INVAL;
-  if (e->nkp > 1 && (prec == QBG_PREC_BF16X3 || prec == QBG_PREC_BF16X2))
-    return ctx->fail(QBG_E_INVAL, "energy of a k-point set: FP64 or FP32");
   if (compute_stress && !e->vhxc_stress)
     return ctx->fail(QBG_E_STATE, "stress requested but update_vhxc ran without stress");
   if (prec < QBG_PREC_FP64 || prec > QBG_PREC_BF16X2) return ctx->fail(QBG_E_INVAL, "energy: prec");
@@ -1437,6 +1435,10 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     const double* ck = c + 2L * ldc * nst * ik;
     double* hk = compute_hpsi ? hpsi + 2L * ldh * nst * ik : hpsi;
     ctx->prof_begin(QBG_K_OTHER);
+    // a smaller basis than the set's largest: its FP32 copy's padding rows
+    // (written by a larger k-point) are cleared for the split products
+    if (split && nk > 1 && 2L * ngk < e->Kp)
+      launch_zero_rows_f32(e->b_c32, e->Kp, 2L * ngk, e->Kp, nst, s);
     // (the half sphere's G and -G times 1/2 is occ |c|^2 |G|^2; the full
     // sphere of k != 0 takes occ/2 |c|^2 |k+G|^2)
     launch_kinetic(ck, ldc, nst, e->cplx ? e->d_occh : e->d_occ, ngk, e->d_g2w, e->d_gxw, hk,

@@ -65,6 +65,8 @@ void launch_fold(const double* c, long ldc, int nst, int ngwu, const int32_t* fo
                  long ldu, hipStream_t s);
 void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_t* unfold,
                        double* hpsi, long ldh, hipStream_t s);
+// rows [r0, r1) of ncols columns (leading dimension ld) of a set to zero
+void launch_zero_rows_f32(float* a, long ld, long r0, long r1, int ncols, hipStream_t s);
 // a k-point set's scalars: kin[0..7], enl[0], nls[0..5] = the sums over the
 // k-points of kacc[16 ik + {0..7, 8, 9..14}]
 void launch_kacc_sum(const double* kacc, int nkp, double* kin, double* enl, double* nls,

@@ -85,6 +85,25 @@ void launch_kacc_sum(const double* kacc, int nkp, double* kin, double* enl, doub
   hipLaunchKernelGGL(k_kacc_sum, dim3(1), dim3(64), 0, s, kacc, nkp, kin, enl, nls);
 }
 
+// rows [r0, r1) of the ncols columns of a (ld) set to zero: a k-point set's
+// FP32 band copy below the largest basis (the split products reduce over
+// every padded row)
+__global__ __launch_bounds__(256) void k_zero_rows_f32(float* __restrict__ a, long ld, long r0,
+                                                       long r1, int ncols) {
+  const long nr = r1 - r0;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nr * ncols) return;
+  const long col = t / nr;
+  a[col * ld + r0 + (t - col * nr)] = 0.0f;
+}
+
+void launch_zero_rows_f32(float* a, long ld, long r0, long r1, int ncols, hipStream_t s) {
+  if (r1 <= r0 || ncols <= 0) return;
+  const long tot = (r1 - r0) * ncols;
+  hipLaunchKernelGGL(k_zero_rows_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a, ld,
+                     r0, r1, ncols);
+}
+
 void launch_unfold_add(const double* H, long ldu, int nst, int ngw, const int32_t* unfold,
                        double* hpsi, long ldh, hipStream_t s) {
   if (nst <= 0 || ngw <= 0) return;

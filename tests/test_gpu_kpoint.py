@@ -186,11 +186,12 @@ KSETS = {
 
 
 @pytest.mark.parametrize("name", sorted(KSETS))
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "bf16x2"])
 def test_kset_evaluate(ctx, oracle_mod, name, prec):
     """A k-point set (qbg_ef_create_kset) against the oracle's: every k-point's
     basis bit for bit, energies, density, forces, the stress and each
-    k-point's H*psi (FP64 1e-10, FP32 at the metric's bar)."""
+    k-point's H*psi (FP64 1e-10, FP32 and the BF16 splits at the metric's
+    bar)."""
     import qbg
     idx, kps, wts = KSETS[name]
     s = dataclasses.replace(cf.config(idx), kpoint=None)
@@ -230,5 +231,5 @@ def test_kset_evaluate(ctx, oracle_mod, name, prec):
     for a, b in zip(ef.kset_unpack(hp2.download()), hs):
         assert np.array_equal(a, b)
     with pytest.raises(qbg.QbgError):
-        ef.evaluate(dc, hp, forces=True, prec="bf16x2")
+        ef.set_nspin(2, 1)
     ef.release()
