@@ -1,7 +1,8 @@
 /* qbg.h -- C ABI of the MI355X H*psi / energy library (libqbg.so).
  *
  * The library replaces, for a single-rank (1x1 Context) Qbox run at the
- * Gamma point, the hot path inside EnergyFunctional::energy: the band FFTs
+ * Gamma point (and at k-points off Gamma, qbg_ef_create_k and
+ * qbg_ef_create_kset), the hot path inside EnergyFunctional::energy: the band FFTs
  * with their G-sphere scatter/gather, the local potential multiply, the
  * density, the Hartree/XC/local terms, the Kleinman-Bylander projections
  * and SlaterDet::gram.  All state stays on one GPU; arrays passed to the
@@ -271,8 +272,7 @@ int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, d
  * least the largest basis (qbg_ef_kset_basis); the density sums the
  * k-points with their weights, and so do ekin, enl, the forces and the
  * stress (src/EnergyFunctional.cpp:446-543, 703-716; ChargeDensity.cpp:
- * 136-160).  Sets run in every precision with nspin 1 (nspin 2:
- * QBG_E_INVAL).  Rows of a k-point's H*psi columns past its basis size are
+ * 136-160).  Sets run in every precision, with nspin 1 or 2.  Rows of a k-point's H*psi columns past its basis size are
  * not written. */
 int qbg_ef_create_kset(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
                        int nkp, const double* kpoints3n, const double* weights, int nsp,

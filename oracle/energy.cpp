@@ -17,7 +17,6 @@ void System::setup() {
   if (kpts.size() / 3 > 1) {           // a set: every k-point on the complex basis
     for (int i = 0; i < 3; i++) kpoint[i] = kpts[i];
     force_complex = true;
-    if (nspin != 1) throw std::invalid_argument("a k-point set with nspin 2 is not restated");
   }
   wb.resize(cell, ref, ecut, kpoint, force_complex);
   vb.resize(cell, ref, 4.0 * ecut);          // src/ChargeDensity.cpp:37

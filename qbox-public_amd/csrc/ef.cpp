@@ -1,4 +1,5 @@
-// ef.cpp -- EnergyFunctional drop-in: a whole Gamma-point system resident on
+// ef.cpp -- EnergyFunctional drop-in: a whole system (Gamma, or k-points off
+// Gamma on the complex basis) resident on
 // one GPU (src/EnergyFunctional.cpp, src/ChargeDensity.cpp,
 // src/NonLocalPotential.cpp, src/StructureFactor.cpp), plus SlaterDet::gram.
 #include <algorithm>
@@ -959,10 +960,20 @@ static int ef_update_density(qbg_ef* e, const double* c, int ldc, int prec, bool
     }
     ctx->prof_end();
     // a set: every k-point's pairs in one pass, each with its weighted
-    // occupations (src/ChargeDensity.cpp:136-160, nspin 1)
-    if (e->nkp > 1)
+    // occupations (src/ChargeDensity.cpp:136-160); with two spins each
+    // spin's bands of each k-point
+    if (e->nkp > 1 && e->nspin == 1)
       rc = density_batches(e->gw, e->d_cpair, nu, 2 * nst * e->nkp, e->d_dfac, e->d_rhor, fprec,
                            res, 0);
+    for (int ik = 0; ik < e->nkp && e->nkp > 1 && e->nspin == 2 && !rc; ik++)
+      for (int sp = 0; sp < 2 && !rc; sp++) {
+        int b0, nb, p0;
+        spin_bands(e, sp, b0, nb, p0);
+        const int q0 = ik * nst + b0;   // the first pair of these bands
+        if (nb > 0)
+          rc = density_batches(e->gw, e->d_cpair + 4L * nu * q0, nu, 2 * nb, e->d_dfac + 2 * q0,
+                               sp == 0 ? e->d_rhor : e->d_rhor2, fprec, res, q0);
+      }
     for (int sp = 0; sp < e->nspin && !rc && e->nkp == 1; sp++) {
       int b0, nb, p0;
       spin_bands(e, sp, b0, nb, p0);
@@ -1488,12 +1499,22 @@ static int ef_launch(qbg_ef* e, const double* c, int ldc, int compute_hpsi, doub
     fill_zero(e->d_hpair, (size_t)4 * nu * nst * e->nkp, s);
     ctx->prof_end();
     if (e->nkp > 1) {
-      // every k-point's pairs through one batched pass (nspin 1)
-      const int rc = kept ? hpsi_resident(e->gw, e->d_vr, 2 * nst * e->nkp, e->d_hpair, nu, fprec,
-                                          0, nullptr)
-                          : rs_mul_add_bands(e->gw, e->d_vr, e->d_cpair, nu, 2 * nst * e->nkp,
-                                             e->d_hpair, nu, fprec, nullptr);
-      if (rc) return rc;
+      // every k-point's pairs through one batched pass; with two spins each
+      // spin's bands of each k-point with that spin's potential
+      for (int ik = 0; ik < (e->nspin == 1 ? 1 : e->nkp); ik++)
+        for (int sp = 0; sp < e->nspin; sp++) {
+          int b0, nb, p0;
+          spin_bands(e, sp, b0, nb, p0);
+          const int q0 = e->nspin == 1 ? 0 : ik * nst + b0;
+          const int nbb = e->nspin == 1 ? nst * e->nkp : nb;
+          if (nbb == 0) continue;
+          const double* v = sp == 0 ? e->d_vr : e->d_vr2;
+          const int rc = kept ? hpsi_resident(e->gw, v, 2 * nbb, e->d_hpair + 4L * nu * q0, nu,
+                                              fprec, q0, nullptr)
+                              : rs_mul_add_bands(e->gw, v, e->d_cpair + 4L * nu * q0, nu, 2 * nbb,
+                                                 e->d_hpair + 4L * nu * q0, nu, fprec, nullptr);
+          if (rc) return rc;
+        }
       ctx->prof_begin(QBG_K_OTHER);
       for (int ik = 0; ik < e->nkp; ik++) {
         if (ik > 0) kswap(e, e->kset[ik]);
@@ -1850,7 +1871,6 @@ int qbg_ef_set_nspin(qbg_ef* e, int nspin, int nst_up) {
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "set_nspin during graph capture");
     if (nspin == 1) nst_up = 0;
-    if (nspin == 2 && e->nkp > 1) return c->fail(QBG_E_INVAL, "set_nspin: nspin 2 of a k-point set");
     if ((nspin != 1 && nspin != 2) || nst_up < 0 || nst_up > e->nst)
       return c->fail(QBG_E_INVAL, "set_nspin: nspin must be 1 or 2, 0 <= nst_up <= nst");
     QBG_HIP(c, hipSetDevice(c->device));

@@ -1,4 +1,5 @@
-// ef.h -- the device-resident state of one Gamma-point system (qbg_ef),
+// ef.h -- the device-resident state of one system (qbg_ef): Gamma, one
+// k-point off Gamma, or a k-point set,
 // shared by the energy functional (ef.cpp) and the SCF-iteration steppers
 // and charge mixer (scf_api.cpp).
 #pragma once

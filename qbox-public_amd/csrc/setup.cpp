@@ -1,4 +1,4 @@
-// setup.cpp -- host setup of a Gamma-point system (see setup.h).
+// setup.cpp -- host setup of a system (see setup.h).
 #include "setup.h"
 #include <algorithm>
 #include <cmath>

@@ -1,4 +1,4 @@
-// setup.h -- host-side setup of one Gamma-point system: the work Qbox does
+// setup.h -- host-side setup of one system (Gamma or k-points): the work Qbox does
 // on the host once per geometry (Basis, BasisMapping, Species SLPP tables,
 // NonLocalPotential::update_twnl, the esr lattice sums).  Everything here
 // runs at creation / atoms_moved / cell_moved time, never per evaluation.

@@ -182,6 +182,7 @@ def test_gram_complex_not_spd(ctx, oracle_mod):
 KSETS = {
     "t6_2k": (16, [[0.25, 0.1, -0.2], [-0.25, 0.35, 0.1]], [0.25, 0.75]),
     "t1_3k": (11, [[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.25, 0.25, 0.25]], [0.25, 0.25, 0.5]),
+    "t6_2k_spin": (16, [[0.25, 0.1, -0.2], [-0.25, 0.35, 0.1]], [0.25, 0.75]),
 }
 
 
@@ -195,6 +196,8 @@ def test_kset_evaluate(ctx, oracle_mod, name, prec):
     import qbg
     idx, kps, wts = KSETS[name]
     s = dataclasses.replace(cf.config(idx), kpoint=None)
+    if name.endswith("_spin"):
+        s = cf.spin_polarized(s, 12, 1, 1)
     print(f"[case] kset {name} {prec}", flush=True)
     o = oracle_mod.Oracle(s, kpoints=kps, weights=wts)
     ef = qbg.EnergyFunctional(ctx, s, kpoints=kps, weights=wts)
@@ -230,6 +233,4 @@ def test_kset_evaluate(ctx, oracle_mod, name, prec):
     assert got2["etotal"] == got["etotal"]
     for a, b in zip(ef.kset_unpack(hp2.download()), hs):
         assert np.array_equal(a, b)
-    with pytest.raises(qbg.QbgError):
-        ef.set_nspin(2, 1)
     ef.release()

@@ -308,3 +308,24 @@ def test_kset_is_doubled_cell(oracle_mod, idx):
     for ik in range(2):
         hp = r_p["hpsi"][ik]
         assert np.abs(hs[ik * nst:(ik + 1) * nst][:, imgs[ik]] - hp).max() < 1e-10 * np.abs(hp).max()
+
+
+def test_kset_spin_identical_sets_equal_unpolarized(oracle_mod):
+    """A k-point set with two spins whose up and down bands are the same, each
+    with occupation 1, equals the unpolarized set with occupation 2."""
+    s = dataclasses.replace(cf.config(16), kpoint=None)
+    kps, wts = [[0.25, 0.1, -0.2], [-0.25, 0.35, 0.1]], [0.25, 0.75]
+    o1 = oracle_mod.Oracle(s, kpoints=kps, weights=wts)
+    cs = _kset_coeffs(o1, 2, s.nst, 3000)
+    s2 = dataclasses.replace(s, nst=2 * s.nst, nspin=2, nst_up=s.nst,
+                             occ_bands=np.ones(2 * s.nst))
+    o2 = oracle_mod.Oracle(s2, kpoints=kps, weights=wts)
+    r1 = o1.evaluate(cs, hpsi=True, forces=True, stress=True)
+    r2 = o2.evaluate([np.concatenate([c, c]) for c in cs], hpsi=True, forces=True, stress=True)
+    for t in TERMS:
+        assert abs(r2[t] - r1[t]) <= 1e-11 * max(1.0, abs(r1[t])), t
+    assert np.abs(r2["fion"] - r1["fion"]).max() < 1e-11
+    assert np.abs(r2["sigma"] - r1["sigma"]).max() < 1e-11 * np.abs(r1["sigma"]).max()
+    for h2, h1 in zip(r2["hpsi"], r1["hpsi"]):
+        assert np.abs(h2[:s.nst] - h1).max() < 1e-11 * np.abs(h1).max()
+        assert np.abs(h2[s.nst:] - h1).max() < 1e-11 * np.abs(h1).max()
