@@ -877,7 +877,6 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
     if (!e || !cell9) return QBG_E_INVAL;
     qbg_ctx* c = e->ctx;
     if (c->capturing) return c->fail(QBG_E_STATE, "cell_moved during graph capture");
-    if (e->cplx) return c->fail(QBG_E_INVAL, "cell_moved: real (Gamma) basis only");
     host::Cell cell;
     cell.set(cell9);
     if (!(cell.volume > 0.0)) return c->fail(QBG_E_INVAL, "cell_moved: singular cell");
@@ -886,9 +885,15 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
     // functional
     const host::Cell& def = e->has_ref ? e->defcell : cell;
     host::Basis wb, vb;
-    wb.build(def, cell, e->ecut);
+    // (the complex basis of a k-point, the Gamma point of a set included)
+    wb.build(def, cell, e->ecut, e->cplx ? e->wb.kpoint : nullptr, e->nkp > 1);
     vb.build(def, cell, 4.0 * e->ecut);
     bool same = wb.idx == e->wb.idx && vb.idx == e->vb.idx;
+    std::vector<host::Basis> kwb(e->nkp);
+    for (int ik = 1; ik < e->nkp && same; ik++) {
+      kwb[ik].build(def, cell, e->ecut, e->kset[ik].wb.kpoint, true);
+      same = kwb[ik].idx == e->kset[ik].wb.idx;
+    }
     for (int i = 0; i < 3 && same; i++) {
       int n = vb.np[i] + 2;
       while (!host::factorizable(n)) n += 2;
@@ -909,7 +914,19 @@ int qbg_ef_cell_moved(qbg_ef* e, const double* cell9) {
     std::vector<double> bm(9);
     for (int i = 0; i < 9; i++) bm[i] = e->cell.b[i / 3][i % 3];
     int rc;
-    if ((rc = dset(c, e->d_g2w, e->wb.g2)) || (rc = dset(c, e->d_gxw, e->wb.gx)) ||
+    // the further k-points of a set: k + G and twnl of the new cell
+    for (int ik = 1; ik < e->nkp; ik++) {
+      qbg_ef::KPoint& k = e->kset[ik];
+      k.wb = std::move(kwb[ik]);
+      std::swap(e->wb, k.wb);
+      std::vector<double> tk, dtk;
+      ef_twnl(e, tk, dtk);
+      std::swap(e->wb, k.wb);
+      if ((rc = dset(c, k.d_g2w, k.wb.kpg2)) || (rc = dset(c, k.d_gxw, k.wb.kpgx)) ||
+          (rc = dset(c, k.d_twnl, tk)) || (rc = dset(c, k.d_dtwnl, dtk)))
+        return rc;
+    }
+    if ((rc = dset(c, e->d_g2w, e->wb.kpg2)) || (rc = dset(c, e->d_gxw, e->wb.kpgx)) ||
         (rc = dset(c, e->d_g2iv, e->vb.g2i)) || (rc = dset(c, e->d_giv, e->vb.gi)) ||
         (rc = dset(c, e->d_gxv, e->vb.gx)) || (rc = dset(c, e->d_vps, vps)) ||
         (rc = dset(c, e->d_dvps, dvps)) || (rc = dset(c, e->d_rhops, rhops)) ||
