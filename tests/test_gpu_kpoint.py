@@ -264,3 +264,37 @@ def test_cell_moved_at_k(ctx, kset):
     assert np.array_equal(ha.download(), hb.download())
     a.release()
     b.release()
+
+
+def test_autograd_at_k(ctx):
+    """Forces and stress at k != 0 through the autograd layer (cell_moved over
+    the reference cell for every strain) against central differences of the
+    device energy (test/si2stress/check_stress.sh's method)."""
+    import torch
+    import qbg
+    from qbg import autograd as qa
+    s = cf.config(16)
+    ref = s.cell * 1.04
+    ef0 = qbg.EnergyFunctional(ctx, s, refcell=ref)
+    c = cf.orthonormal_k(cf.random_coefficients_k(3200, ef0.basis(2)["g2"], s.nst))
+    ef0.release()
+    print("[case] autograd at k", flush=True)
+    model = qa.QboxModel(ctx, s, c, refcell=ref)
+    tau = torch.tensor(s.tau, dtype=torch.float64, requires_grad=True)
+    eps = torch.zeros((3, 3), dtype=torch.float64, requires_grad=True)
+    e = qa.energy(tau, eps, model)
+    e.backward()
+    t0, z, h = s.tau.copy(), np.zeros((3, 3)), 1e-4
+    for (a, j) in [(0, 0), (s.nat - 1, 2)]:
+        tp, tm = t0.copy(), t0.copy()
+        tp[a, j] += h
+        tm[a, j] -= h
+        fd = (model.evaluate(tp, z)[0] - model.evaluate(tm, z)[0]) / (2 * h)
+        assert abs(tau.grad[a, j].item() - fd) <= 1e-6 * max(abs(fd), 1e-3)
+    for (i, j) in [(1, 1), (0, 2)]:
+        u = np.zeros((3, 3))
+        u[i, j] = u[j, i] = h
+        fd = (model.evaluate(t0, u)[0] - model.evaluate(t0, -u)[0]) / (2 * h)
+        g = eps.grad[i, j].item() + (eps.grad[j, i].item() if i != j else 0.0)
+        print(f"  dE/deps[{i},{j}] adjoint {g: .10e} fd {fd: .10e}", flush=True)
+        assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-2)
