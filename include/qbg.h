@@ -259,8 +259,9 @@ int qbg_ef_create(qbg_ctx* ctx, const double* cell9, const double* refcell9, dou
  * forces) follow src/EnergyFunctional.cpp:397-1044 and
  * src/NonLocalPotential.cpp:1429-2066 for the complex basis (the stress
  * included), in every precision, with nspin 1 or 2.  Not built at k != 0
- * (QBG_E_INVAL): the NonLocalPotential and wavefunction-grid entry points
- * and the SCF steppers. */
+ * (QBG_E_INVAL): the wavefunction-grid entry points and the SCF steppers;
+ * the NonLocalPotential entry points (qbg_nl_*) take one k-point, not a
+ * set. */
 int qbg_ef_create_k(qbg_ctx* ctx, const double* cell9, const double* refcell9, double ecut,
                     const double* kpoint3, int nsp, const qbg_species_desc* species,
                     const int* na, const double* tau, int nst, const double* occ, qbg_ef** out);

@@ -1707,7 +1707,7 @@ int qbg_nl_create(qbg_ef* e, int ispin, qbg_nl** out) {
     if (!e || !out) return QBG_E_INVAL;
     *out = nullptr;
     if (ispin < 0 || ispin >= e->nspin) return e->ctx->fail(QBG_E_INVAL, "nl: ispin");
-    if (e->cplx) return e->ctx->fail(QBG_E_INVAL, "nl: the complex (k != 0) basis runs inside qbg_ef_* only");
+    if (e->nkp > 1) return e->ctx->fail(QBG_E_INVAL, "nl: one k-point per NonLocalPotential (a set runs inside qbg_ef_* only)");
     qbg_nl* nl = new qbg_nl;
     nl->ef = e;
     nl->ctx = e->ctx;

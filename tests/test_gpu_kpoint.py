@@ -298,3 +298,27 @@ def test_autograd_at_k(ctx):
         g = eps.grad[i, j].item() + (eps.grad[j, i].item() if i != j else 0.0)
         print(f"  dE/deps[{i},{j}] adjoint {g: .10e} fd {fd: .10e}", flush=True)
         assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x2"])
+def test_nl_energy_at_k(ctx, oracle_mod, prec):
+    """NonLocalPotential::energy (qbg_nl_*) at k != 0 against the oracle's
+    complex branch: enl, forces, stress and H*psi_NL added into dsd."""
+    import qbg
+    s = cf.config(16)
+    print(f"[case] nl at k {prec}", flush=True)
+    o = oracle_mod.Oracle(s)
+    ef = qbg.EnergyFunctional(ctx, s)
+    nl = qbg.NonLocalPotential(ef, 0)
+    c = cf.orthonormal_k(cf.random_coefficients_k(3300, o.kpg()[0], s.nst))
+    rng = np.random.default_rng(33)
+    h0 = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    dsd = ctx.to_device(h0)
+    enl, f, sig = nl.energy(ctx.to_device(c), dsd, forces=True, stress=True, prec=prec)
+    renl, rf, rsig, rh = o.nl_energy(c, np.zeros_like(c), forces=True, stress=True)
+    hd = dsd.download() - h0
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert abs(enl - renl) <= tol * max(abs(renl), 1.0)
+    assert np.abs(f - rf).max() <= tol * max(np.abs(rf).max(), 1.0)
+    assert np.abs(sig - rsig).max() <= tol * max(np.abs(rsig).max(), 1e-3)
+    assert relerr(hd, rh) < (1e-10 if prec == "fp64" else 1e-4)
