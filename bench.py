@@ -335,18 +335,26 @@ def bench_kpoint(args):
     import dataclasses
     import qbg
     from qbg import configs as cf
-    kp = np.array(args.kpoint, dtype=float)
-    sysdef = dataclasses.replace(cf.config(args.config), kpoint=kp)
+    if len(args.kpoint) % 3:
+        print(json.dumps({"error": "--kpoint takes three numbers per k-point"}))
+        sys.exit(2)
+    kps = np.array(args.kpoint, dtype=float).reshape(-1, 3)
+    nkp = kps.shape[0]
     ctx = qbg.Context(0)
-    ef = qbg.EnergyFunctional(ctx, sysdef)
-    kpg2 = ef.basis(2)["g2"]
-    c = cf.orthonormal_k(cf.random_coefficients_k(cf.BASE_SEED + 70 + args.config, kpg2,
-                                                  sysdef.nst))
+    if nkp == 1:
+        sysdef = dataclasses.replace(cf.config(args.config), kpoint=kps[0])
+        ef = qbg.EnergyFunctional(ctx, sysdef)
+    else:                           # a set with equal weights
+        sysdef = dataclasses.replace(cf.config(args.config), kpoint=None)
+        ef = qbg.EnergyFunctional(ctx, sysdef, kpoints=kps, weights=np.full(nkp, 1.0 / nkp))
+    c = ef.kset_pack([cf.orthonormal_k(cf.random_coefficients_k(
+        cf.BASE_SEED + 70 + args.config + ik, ef.kset_basis(ik)["kpg2"], sysdef.nst))
+        for ik in range(nkp)])
     dc = ctx.to_device(c)
     hp = ctx.array(c.shape, np.complex128)
     del c
-    nst, nat = sysdef.nst, sysdef.nat
-    print(f"[bench kpoint] {sysdef.name} k {tuple(kp)}: ngw {ef.ngw} nproj {ef.nproj}",
+    nst, nat = sysdef.nst * nkp, sysdef.nat     # band-applies count every k-point's bands
+    print(f"[bench kpoint] {sysdef.name} k {kps.tolist()}: ngw {ef.kngw} nproj {ef.nproj}",
           file=sys.stderr, flush=True)
     r64, wall64, _, kt64 = run(ef, dc, hp, "fp64", args.steps, args.warmup, ctx)
     h64 = hp.download()
@@ -357,13 +365,13 @@ def bench_kpoint(args):
     herr = float(np.abs(hbf - h64).max() / np.abs(h64).max())
     t64, tbf = wall64 / args.steps, wallbf / args.steps
     print(json.dumps({
-        "metric": "H*psi band-applies/s at one k-point off Gamma on 1 GPU (complex basis; "
+        "metric": "H*psi band-applies/s at k-points off Gamma on 1 GPU (complex basis; "
                   "density + update_vhxc + energy with H*psi and forces)",
         "value": round(nst / tbf, 2), "unit": "band-applies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * tbf, 3),
         "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": sysdef.name, "kpoint": [float(x) for x in kp], "ngw": ef.ngw,
-                   "nproj": ef.nproj, "precision": "bf16x2"},
+        "config": {"workload": sysdef.name, "kpoints": kps.tolist(), "ngw": ef.kngw,
+                   "nproj": ef.nproj, "precision": "bf16x2", "bands_per_kpoint": sysdef.nst},
         "fp64_value": round(nst / t64, 2), "fp64_ms_per_step": round(1e3 * t64, 3),
         "bf16x2_vs_fp64": {"max_dE_per_atom": float("%.3e" % err),
                            "force_max_abs_err": float("%.3e" % ferr),
@@ -387,9 +395,10 @@ def main():
                     help="only the uneven batch: twelve systems, per-system loop vs one graph")
     ap.add_argument("--scf", action="store_true",
                     help="only time whole SCF iterations (PSDA stepper, Anderson mixing)")
-    ap.add_argument("--kpoint", type=float, nargs=3, default=None, metavar=("K0", "K1", "K2"),
-                    help="only time the evaluation at this k-point (units of b) on the "
-                         "complex basis, FP64 and BF16X2")
+    ap.add_argument("--kpoint", type=float, nargs="+", default=None, metavar="K",
+                    help="only time the evaluation at these k-points (three numbers each, "
+                         "units of b; several: an equal-weight set) on the complex basis, "
+                         "FP64 and BF16X2")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
